@@ -1,0 +1,165 @@
+/* kgraph_amd.h -- C ABI of libkgraph_amd.so: KGraph's NN-descent k-NN graph
+ * build and graph-walk search on one AMD Instinct MI355X (gfx950).
+ *
+ * Plain pointers and sizes only.  Every function returns 0 on success and a
+ * negative status on failure; kg_last_error() then holds the message.  The
+ * status tells which C++ exception the reference would throw
+ * (kgraph.h:287-300): KG_ERR_INVALID_ARGUMENT <-> kgraph::invalid_argument,
+ * KG_ERR_RUNTIME <-> kgraph::runtime_error, KG_ERR_IO <-> kgraph::io_error.
+ *
+ * A handle owns a device copy of the dataset rows (the dense form of the
+ * reference's IndexOracle / SearchOracle plugins, kgraph.h:49-86 and
+ * kgraph-data.h:302-331 / python-api.cpp:87-129) and the index (the
+ * reference's KGraphImpl::graph + M, kgraph.cpp:276-280).  A handle is not
+ * re-entrant: calls on one handle must not overlap.  Different handles may be
+ * used from different threads.
+ */
+#ifndef KGRAPH_AMD_H
+#define KGRAPH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct kg_handle kg_handle;
+
+enum {
+    KG_OK = 0,
+    KG_ERR_INVALID_ARGUMENT = -1,
+    KG_ERR_RUNTIME = -2,
+    KG_ERR_IO = -3,
+    KG_ERR_DEVICE = -4
+};
+
+/* Distance plugins.  L2SQR: sum of squared differences (the CLI's
+ * MatrixOracle<float, metric::l2sqr>, kgraph-data.h:41-61, index.cpp:168).
+ * EUCLIDEAN: n_i + n_j - dot with n = ||x||^2/2 (python-api.cpp:51-59).
+ * ANGULAR: -dot / (||x|| ||y|| + FLT_EPSILON) (python-api.cpp:61-69). */
+enum { KG_METRIC_L2SQR = 0, KG_METRIC_EUCLIDEAN = 1, KG_METRIC_ANGULAR = 2 };
+
+/* File formats (kgraph.h:128-132). */
+enum { KG_FORMAT_DEFAULT = 0, KG_FORMAT_NO_DIST = 1, KG_FORMAT_TEXT = 128 };
+
+/* Screening precision of the build's local join and of search (no
+ * counterpart in the reference): FP32 scores every pair exactly; BF16 scores
+ * candidates with BF16 dot products on MFMA first and re-scores every
+ * admission in exact FP32. */
+enum { KG_PRECISION_FP32 = 0, KG_PRECISION_BF16 = 1 };
+
+/* KGraph::IndexParams (kgraph.h:94-110), same fields, order and meaning. */
+typedef struct {
+    unsigned iterations, L, K, S, R, controls, seed;
+    float delta, recall;
+    unsigned prune;
+    int reverse;
+} kg_index_params;
+
+/* KGraph::IndexInfo (kgraph.h:135-147). stop_condition: 0 ITERATION,
+ * 1 DELTA, 2 RECALL. */
+typedef struct {
+    int stop_condition;
+    unsigned iterations;
+    float cost, recall, accuracy, delta, M;
+} kg_index_info;
+
+/* KGraph::SearchParams (kgraph.h:113-126). */
+typedef struct {
+    unsigned K, M, P, S, T;
+    float epsilon;
+    unsigned seed, init;
+} kg_search_params;
+
+/* KGraph::version (kgraph.h:201, kgraph.cpp:1014-1016). */
+const char *kg_version(void);
+
+/* KGraph::create (kgraph.h:199, kgraph.cpp:1010-1012) / delete. */
+kg_handle *kg_create(void);
+void kg_destroy(kg_handle *h);
+const char *kg_last_error(const kg_handle *h);
+
+/* Defaults of IndexParams() / SearchParams() (kgraph.h:14-37, 108-125). */
+void kg_index_params_default(kg_index_params *p);
+void kg_search_params_default(kg_search_params *p);
+
+/* Global verbosity (kgraph.h:42): 0 silences progress lines on stderr. */
+void kg_set_verbosity(unsigned v);
+
+/* Dataset rows: N rows of D floats, `stride` floats apart, in host memory
+ * (kg_set_data) or device memory (kg_set_data_device).  The handle keeps its
+ * own padded device copy and the per-row norms of `metric`.  This is the
+ * dense oracle the reference's build()/search() receive
+ * (kgraph.cpp:1003, 423; python-api.cpp:110-114). */
+int kg_set_data(kg_handle *h, const float *X, unsigned N, unsigned D, size_t stride, int metric);
+int kg_set_data_device(kg_handle *h, const float *dX, unsigned N, unsigned D, size_t stride, int metric);
+
+/* KGraph::build (kgraph.h:167, kgraph.cpp:886-999): NN-descent on the GPU,
+ * then reverse-edge completion (params->reverse) and pruning
+ * (params->prune).  info may be NULL. */
+int kg_build(kg_handle *h, const kg_index_params *params, kg_index_info *info);
+
+/* Screening precision used by kg_build and kg_search (KG_PRECISION_*). */
+int kg_set_precision(kg_handle *h, int precision);
+
+/* Batched KGraph::search (kgraph.h:187-197, kgraph.cpp:423-564): nq queries
+ * of D floats, qstride floats apart.  Ks / Ps give per-query K and P (NULL =
+ * params->K / params->P).  Row q of the results starts at ids + q*ostride
+ * (and dists + q*ostride; dists may be NULL); counts[q] = number of results
+ * (the reference's return value).  comps / scans (may be NULL) receive each
+ * query's distance evaluations after seeding (SearchInfo::cost * N,
+ * kgraph.cpp:561) and neighbour ids read.  With params->init > 0 the first
+ * init ids of each row of `ids` are the starting points (kgraph.cpp:461-467).
+ * kg_search takes host buffers; kg_search_device takes device buffers for
+ * Q / ids / dists / counts / comps / scans (Ks / Ps stay in host memory). */
+int kg_search(kg_handle *h, const float *Q, unsigned nq, size_t qstride, const kg_search_params *params,
+              const unsigned *Ks, const unsigned *Ps, unsigned *ids, float *dists, size_t ostride,
+              unsigned *counts, unsigned *comps, unsigned *scans);
+int kg_search_device(kg_handle *h, const float *dQ, unsigned nq, size_t qstride, const kg_search_params *params,
+                     const unsigned *Ks, const unsigned *Ps, unsigned *d_ids, float *d_dists, size_t ostride,
+                     unsigned *d_counts, unsigned *d_comps, unsigned *d_scans);
+
+/* Exact top-K by full scan (SearchOracle::search, kgraph.cpp:231-252): the K
+ * smallest (dist, id) pairs with dist <= epsilon.  Host buffers. */
+int kg_bruteforce(kg_handle *h, const float *Q, unsigned nq, size_t qstride, unsigned K, float epsilon,
+                  unsigned *ids, float *dists, size_t ostride, unsigned *counts);
+
+/* KGraph::save / load (kgraph.h:159-165, kgraph.cpp:291-373), same
+ * KNNGRAPH v2 byte format; FORMAT_TEXT is write-only. */
+int kg_save(kg_handle *h, const char *path, int format);
+int kg_load(kg_handle *h, const char *path);
+
+/* KGraph::get_nn (kgraph.h:205-224, kgraph.cpp:566-582).  nns / dists must
+ * hold the node's L entries; either may be NULL. */
+int kg_get_nn(kg_handle *h, unsigned id, unsigned *nns, float *dists, unsigned *M, unsigned *L);
+
+/* KGraph::reverse (kgraph.h:226, kgraph.cpp:667-702) and KGraph::prune
+ * (kgraph.h:181, kgraph.cpp:584-665). */
+int kg_reverse(kg_handle *h, int rev);
+int kg_prune(kg_handle *h, unsigned level);
+
+/* Index size: N nodes and total stored edges. */
+int kg_index_size(kg_handle *h, unsigned *N, uint64_t *edges);
+
+/* Whole-index exchange in CSR form: offsets[N+1] (edge positions), ids /
+ * dists [edges] (dists may be NULL: no-distance index), M[N]. */
+int kg_set_graph(kg_handle *h, unsigned N, const uint64_t *offsets, const unsigned *ids, const float *dists,
+                 const unsigned *M);
+int kg_get_graph(kg_handle *h, uint64_t *offsets, unsigned *ids, float *dists, unsigned *M);
+
+/* Device time in milliseconds of the last call's phases, measured with HIP
+ * events on the handle's stream: which = 0 search kernel, 1 brute-force
+ * kernel, 2 whole build, 3 NN-descent join kernels (sum), 4 merge kernels
+ * (sum), 5 update kernels (sum), 6 reverse. */
+int kg_last_time_ms(kg_handle *h, int which, float *ms);
+
+/* Per-iteration trace of the last build: 4 floats per iteration (recall,
+ * delta, M, cost), up to cap iterations.  Returns the iteration count in *n. */
+int kg_build_trace(kg_handle *h, float *out, unsigned cap, unsigned *n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,195 @@
+"""kgraph_amd -- KGraph's NN-descent k-NN graph build and graph-walk search on an
+AMD Instinct MI355X.
+
+`KGraph(data, metric)` mirrors the reference's Python class
+(python-api.cpp:331-450): same constructor, `build` / `search` keyword
+arguments and defaults, `load` / `save` / `save_text`, uint32 ids and float32
+distances.  All compute runs in libkgraph_amd.so (HIP kernels for gfx950)
+through its C ABI (include/kgraph_amd.h).
+"""
+import numpy as np
+
+from . import _lib
+from ._lib import (FORMAT_DEFAULT, FORMAT_NO_DIST, FORMAT_TEXT, METRIC_ANGULAR, METRIC_EUCLIDEAN,
+                   METRIC_L2SQR, PRECISION_BF16, PRECISION_FP32, Handle, IndexInfo, IndexParams,
+                   SearchParams, f32c, ptr, u32c)
+
+# kgraph.h:14-37
+default_iterations = 30
+default_L = 100
+default_K = 25
+default_P = 100
+default_M = 0
+default_T = 1
+default_S = 10
+default_R = 100
+default_controls = 100
+default_seed = 1998
+default_delta = 0.002
+default_recall = 0.99
+default_epsilon = 1e30
+default_prune = 0
+default_reverse = 0
+
+git_commit = "kgraph_amd"
+arch = {"name": "gfx950", "alignment": 16}
+
+_METRICS = {"euclidean": METRIC_EUCLIDEAN, "angular": METRIC_ANGULAR}
+
+
+def version():
+    return _lib.lib().kg_version().decode()
+
+
+class KGraph:
+    """kgraph.KGraph (python-api.cpp:331-410).
+
+    data: 2-D float32 or float64 array (float64 is converted to float32; the
+    distance arithmetic is FP32).  metric: 'euclidean' (||x-y||^2 / 2, as
+    n_x + n_y - dot) or 'angular' (-cos).  The rows are copied to the GPU here,
+    as the reference copies them into an aligned matrix (python-api.cpp:255).
+    """
+
+    def __init__(self, data, metric, *, l2sqr=False):
+        if metric not in _METRICS:
+            raise RuntimeError("metric not supported")
+        data = np.asarray(data)
+        if data.ndim != 2:
+            raise ValueError("bad array")
+        if data.dtype not in (np.float32, np.float64):
+            raise RuntimeError("data type not supported.")
+        self._h = Handle()
+        self._data = f32c(data)
+        self._metric = METRIC_L2SQR if l2sqr else _METRICS[metric]
+        self._hasIndex = False
+        self._h.call("kg_set_data", ptr(self._data), self._data.shape[0], self._data.shape[1],
+                     self._data.shape[1], self._metric)
+
+    # -- index storage (python-api.cpp:237-246)
+    def load(self, path):
+        self._h.call("kg_load", path.encode())
+        self._hasIndex = True
+
+    def save(self, path):
+        self._h.call("kg_save", path.encode(), FORMAT_DEFAULT)
+
+    def save_text(self, path):
+        self._h.call("kg_save", path.encode(), FORMAT_TEXT)
+
+    # -- build (python-api.cpp:369-389, defaults :429-438)
+    def build(self, iterations=default_iterations, L=default_L, K=default_K, S=default_S,
+              controls=default_controls, delta=default_delta, recall=default_recall,
+              prune=default_prune, reverse=default_reverse, *, R=default_R, seed=default_seed,
+              precision=None):
+        p = IndexParams(iterations, L, K, S, R, controls, seed, delta, recall, prune, reverse)
+        if precision is not None:
+            self._h.call("kg_set_precision", precision)
+        info = IndexInfo()
+        self._h.call("kg_build", p, info)
+        self._hasIndex = True
+        self.info = info
+        return info
+
+    # -- search (python-api.cpp:266-328, 390-409, defaults :440-448)
+    def search(self, query, K=default_K, P=default_P, M=default_M, S=default_S, T=default_T,
+               threads=0, withDistance=False, blas=False, *, epsilon=default_epsilon,
+               seed=default_seed, Ks=None, Ps=None, stats=False):
+        """Returns uint32[Q, K] ids, or (ids, float32[Q, K] dists) with
+        withDistance.  `threads` and `blas` are accepted and ignored (the GPU
+        path has no OpenMP pool, and `blas` selects an exact scan that the
+        index-less path already provides).  Ks / Ps give per-query K / P for
+        one batched launch; the output then has max(Ks) columns.  Rows past a
+        short result are zero (the reference leaves them uninitialised,
+        python-api.cpp:270-273)."""
+        q = f32c(query)
+        if q.ndim != 2 or q.shape[1] != self._data.shape[1]:
+            raise ValueError("bad array")
+        nq = q.shape[0]
+        Ks_a = u32c(Ks) if Ks is not None else None
+        Ps_a = u32c(Ps) if Ps is not None else None
+        width = int(Ks_a.max()) if Ks_a is not None and nq else K
+        ids = np.zeros((nq, width), dtype=np.uint32)
+        dists = np.zeros((nq, width), dtype=np.float32)
+        counts = np.zeros(nq, dtype=np.uint32)
+        comps = np.zeros(nq, dtype=np.uint32) if stats else None
+        scans = np.zeros(nq, dtype=np.uint32) if stats else None
+        if self._hasIndex:
+            sp = SearchParams(K, M, P, S, T, epsilon, seed, 0)
+            self._h.call("kg_search", ptr(q), nq, q.shape[1], sp, ptr(Ks_a), ptr(Ps_a), ptr(ids),
+                         ptr(dists), width, ptr(counts), ptr(comps), ptr(scans))
+        else:
+            # no index: exact scan (python-api.cpp:303-314)
+            if Ks_a is not None and nq and (Ks_a != Ks_a[0]).any():
+                raise ValueError("brute-force search needs one K")
+            self._h.call("kg_bruteforce", ptr(q), nq, q.shape[1], width, epsilon, ptr(ids), ptr(dists),
+                         width, ptr(counts))
+        self.last_counts = counts
+        if stats:
+            self.last_comps, self.last_scans = comps, scans
+        if withDistance:
+            return ids, dists
+        return ids
+
+    # -- extensions beyond the reference's Python class
+    def get_nn(self, i):
+        """KGraph::get_nn (kgraph.cpp:566-582): (ids, dists, M)."""
+        M = ctypes_uint()
+        L = ctypes_uint()
+        self._h.call("kg_get_nn", i, None, None, _lib.ctypes.byref(M), _lib.ctypes.byref(L))
+        ids = np.zeros(L.value, dtype=np.uint32)
+        d = np.zeros(L.value, dtype=np.float32)
+        self._h.call("kg_get_nn", i, ptr(ids), ptr(d), _lib.ctypes.byref(M), _lib.ctypes.byref(L))
+        return ids, d, M.value
+
+    def graph(self):
+        """Whole index as CSR numpy arrays: (offsets, ids, dists, M)."""
+        return export_graph(self._h)
+
+    def set_graph(self, offsets, ids, dists, M):
+        import_graph(self._h, offsets, ids, dists, M)
+        self._hasIndex = True
+
+    def reverse(self, rev):
+        self._h.call("kg_reverse", rev)
+
+    def prune(self, level):
+        self._h.call("kg_prune", level)
+
+    def last_time_ms(self, which=0):
+        v = _lib.ctypes.c_float()
+        self._h.call("kg_last_time_ms", which, _lib.ctypes.byref(v))
+        return v.value
+
+    def build_trace(self):
+        n = _lib.ctypes.c_uint()
+        self._h.call("kg_build_trace", None, 0, _lib.ctypes.byref(n))
+        out = np.zeros(4 * max(n.value, 1), dtype=np.float32)
+        self._h.call("kg_build_trace", ptr(out), n.value, _lib.ctypes.byref(n))
+        return out[:4 * n.value].reshape(-1, 4)
+
+
+def ctypes_uint():
+    return _lib.ctypes.c_uint()
+
+
+def export_graph(h):
+    n = _lib.ctypes.c_uint()
+    e = _lib.ctypes.c_uint64()
+    h.call("kg_index_size", _lib.ctypes.byref(n), _lib.ctypes.byref(e))
+    off = np.zeros(n.value + 1, dtype=np.uint64)
+    ids = np.zeros(max(e.value, 1), dtype=np.uint32)
+    d = np.zeros(max(e.value, 1), dtype=np.float32)
+    M = np.zeros(max(n.value, 1), dtype=np.uint32)
+    h.call("kg_get_graph", ptr(off), ptr(ids), ptr(d), ptr(M))
+    return off, ids[:e.value], d[:e.value], M[:n.value]
+
+
+def import_graph(h, offsets, ids, dists, M):
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    ids = u32c(ids)
+    M = u32c(M)
+    d = f32c(dists) if dists is not None else None
+    h.call("kg_set_graph", len(M), ptr(off), ptr(ids), ptr(d), ptr(M))
+
+
+__all__ = ["KGraph", "IndexParams", "SearchParams", "IndexInfo", "version", "export_graph", "import_graph"]

@@ -1,0 +1,147 @@
+// kg_brute.hip -- exact top-K by full scan, and per-row norms.
+//
+// brute_kernel restates SearchOracle::search (kgraph.cpp:231-252: top-K with
+// an epsilon filter) and LinearSearch (kgraph.cpp:212-229: K-NN of a data row
+// with the row itself excluded).  Both insert in ascending id order after
+// equal distances, i.e. they return the K smallest (dist, id) pairs; this
+// kernel selects exactly that set with u64 keys (orderable dist bits << 32 |
+// id).  One block per query: 4 waves score 64 rows per step (4 lanes per row,
+// canonical distance order), keys below the running K-th key go to an LDS
+// buffer, and a block-wide bitonic sort folds the buffer into the top-K.
+//
+// norms_kernel computes the per-row norm the metric keeps
+// (python-api.cpp:110-114): ||x||^2/2 for 'euclidean', ||x|| for 'angular'.
+#include "kg_device.hpp"
+#include "kg_internal.hpp"
+
+namespace {
+
+using namespace kgd;
+
+constexpr unsigned BT = 256;     // threads per block
+constexpr unsigned BUF = 1024;   // candidate buffer
+constexpr unsigned SORTN = 2048; // sort width: K (<= 1024) + BUF
+
+__device__ void block_bitonic(uint64_t *v, unsigned n) {
+    for (unsigned k = 2; k <= n; k <<= 1) {
+        for (unsigned j = k >> 1; j > 0; j >>= 1) {
+            for (unsigned i = threadIdx.x; i < n; i += BT) {
+                unsigned ixj = i ^ j;
+                if (ixj > i) {
+                    uint64_t a = v[i], b = v[ixj];
+                    bool up = (i & k) == 0;
+                    if ((a > b) == up) {
+                        v[i] = b;
+                        v[ixj] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <int METRIC>
+__global__ void __launch_bounds__(BT) brute_kernel(kg_brute_args a) {
+    __shared__ uint64_t keys[SORTN];  // [0,K) top-K, [K, K+cnt) buffer
+    __shared__ __align__(16) float qv[1024];
+    __shared__ unsigned cnt;
+    __shared__ unsigned long long thr;
+    unsigned const qi = a.qsel ? a.qsel[blockIdx.x] : blockIdx.x;
+    unsigned const K = a.K;
+    unsigned const lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    unsigned const grp = lane >> 2, t = lane & 3;
+    float const *qsrc;
+    float qn = 0.f;
+    uint32_t self = 0xffffffffu;
+    if (a.Q) {
+        qsrc = a.Q + size_t(qi) * a.qstride;
+        if (METRIC != METRIC_L2SQR) qn = a.qnorms[qi];
+    } else {
+        self = a.rows[qi];
+        qsrc = a.X + size_t(self) * a.Dp;
+        if (METRIC != METRIC_L2SQR) qn = a.norms[self];
+    }
+    for (unsigned i = threadIdx.x; i < a.Dp; i += BT) qv[i] = (i < a.D) ? qsrc[i] : 0.f;
+    for (unsigned i = threadIdx.x; i < SORTN; i += BT) keys[i] = ~0ull;
+    if (threadIdx.x == 0) {
+        cnt = 0;
+        thr = ~0ull;
+    }
+    __syncthreads();
+    unsigned have = 0;  // valid entries in the top-K part
+    for (unsigned r0 = 0; r0 < a.N; r0 += 64) {
+        unsigned row = r0 + wv * 16 + grp;
+        if (r0 + wv * 16 + grp < a.N) {
+            float acc = quad_metric<METRIC>(a.X + size_t(row) * a.Dp, qv, a.D, t);
+            float na = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[row];
+            float d = finish_metric<METRIC>(acc, na, qn);
+            uint64_t key = (uint64_t(ord_f32(d)) << 32) | row;
+            if (t == 0 && row != self && d <= a.epsilon && key < thr) {
+                unsigned p = atomicAdd(&cnt, 1u);
+                keys[K + p] = key;
+            }
+        }
+        __syncthreads();
+        if (cnt + 64 > BUF || r0 + 64 >= a.N) {
+            unsigned n = K + cnt;
+            unsigned sn = 2;
+            while (sn < n) sn <<= 1;
+            block_bitonic(keys, sn);
+            have = min(K, have + cnt);
+            for (unsigned i = K + threadIdx.x; i < SORTN; i += BT) keys[i] = ~0ull;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                cnt = 0;
+                if (have == K) thr = keys[K - 1];
+            }
+            __syncthreads();
+        }
+    }
+    for (unsigned i = threadIdx.x; i < have; i += BT) {
+        uint64_t key = keys[i];
+        a.out_ids[size_t(qi) * a.ostride + i] = uint32_t(key);
+        if (a.out_dists) a.out_dists[size_t(qi) * a.ostride + i] = unord_f32(uint32_t(key >> 32));
+    }
+    if (threadIdx.x == 0) {
+        if (a.counts) a.counts[qi] = have;
+        if (a.comps) a.comps[qi] = a.N;
+        if (a.scans) a.scans[qi] = 0;
+    }
+}
+
+template <int METRIC>
+__global__ void __launch_bounds__(256) norms_kernel(float const *X, unsigned N, unsigned D, unsigned Dp, float *out) {
+    unsigned const lane = threadIdx.x & 63;
+    unsigned const row = (blockIdx.x * blockDim.x + threadIdx.x) >> 2;
+    unsigned const t = lane & 3;
+    if ((blockIdx.x * blockDim.x + (threadIdx.x & ~3u)) / 4 < N) {
+        float const *r = X + size_t(row) * Dp;
+        float n2 = quad_metric<METRIC_EUCLIDEAN>(r, r, D, t);
+        if (t == 0) out[row] = finish_norm<METRIC>(n2);
+    }
+}
+
+}  // namespace
+
+hipError_t kg_launch_brute(int metric, kg_brute_args const &a, hipStream_t s) {
+    if (a.K == 0 || a.K > 1024 || a.Dp > 1024 || a.nq == 0) return hipErrorInvalidValue;
+    dim3 grid(a.nq), block(BT);
+    switch (metric) {
+        case METRIC_L2SQR: hipLaunchKernelGGL(brute_kernel<METRIC_L2SQR>, grid, block, 0, s, a); break;
+        case METRIC_EUCLIDEAN: hipLaunchKernelGGL(brute_kernel<METRIC_EUCLIDEAN>, grid, block, 0, s, a); break;
+        default: hipLaunchKernelGGL(brute_kernel<METRIC_ANGULAR>, grid, block, 0, s, a); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t kg_launch_norms(int metric, float const *X, unsigned N, unsigned D, unsigned Dp, float *out, hipStream_t s) {
+    if (N == 0) return hipSuccess;
+    dim3 grid((N * 4 + 255) / 256), block(256);
+    switch (metric) {
+        case METRIC_EUCLIDEAN: hipLaunchKernelGGL(norms_kernel<METRIC_EUCLIDEAN>, grid, block, 0, s, X, N, D, Dp, out); break;
+        case METRIC_ANGULAR: hipLaunchKernelGGL(norms_kernel<METRIC_ANGULAR>, grid, block, 0, s, X, N, D, Dp, out); break;
+        default: return hipSuccess;
+    }
+    return hipGetLastError();
+}

@@ -1,0 +1,655 @@
+// kg_capi.cpp -- the extern "C" boundary of libkgraph_amd.so (declared in
+// include/kgraph_amd.h) and the host logic around the HIP kernels: dataset
+// upload, search-seed generation, scratch sizing, the KNNGRAPH file format
+// and index exchange.
+#include <algorithm>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <random>
+
+#include "kg_handle.hpp"
+#include "kg_internal.hpp"
+
+namespace kga {
+
+unsigned verbosity = 1;  // kgraph.h:42, default_verbosity kgraph.h:27
+
+Handle::Handle() {
+    KG_HIP(hipGetDevice(&device));
+    hipDeviceProp_t prop;
+    KG_HIP(hipGetDeviceProperties(&prop, device));
+    num_cus = prop.multiProcessorCount;
+    lds_per_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 65536;
+    KG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    for (auto &e : ev) KG_HIP(hipEventCreate(&e));
+}
+
+Handle::~Handle() {
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Handle::pull_graph() {
+    if (host_valid) return;
+    if (!has_graph) throw runtime_error("no index");
+    sync();
+    h_off.resize(gN + 1);
+    h_ids.resize(edges);
+    h_M.resize(gN);
+    h_dist.resize(no_dist ? 0 : edges);
+    KG_HIP(hipMemcpy(h_off.data(), goff.p, 4ull * (gN + 1), hipMemcpyDeviceToHost));
+    if (edges) KG_HIP(hipMemcpy(h_ids.data(), gids.p, 4ull * edges, hipMemcpyDeviceToHost));
+    KG_HIP(hipMemcpy(h_M.data(), gM.p, 4ull * gN, hipMemcpyDeviceToHost));
+    if (!no_dist && edges) KG_HIP(hipMemcpy(h_dist.data(), gdist.p, 4ull * edges, hipMemcpyDeviceToHost));
+    host_valid = true;
+}
+
+void Handle::push_graph() {
+    edges = h_off.empty() ? 0 : h_off.back();
+    gN = h_M.size();
+    goff.reserve(gN + 1);
+    gids.reserve(std::max<uint64_t>(edges, 1));
+    gM.reserve(std::max<unsigned>(gN, 1));
+    gdist.reserve(std::max<uint64_t>(edges, 1));
+    KG_HIP(hipMemcpy(goff.p, h_off.data(), 4ull * (gN + 1), hipMemcpyHostToDevice));
+    if (edges) KG_HIP(hipMemcpy(gids.p, h_ids.data(), 4ull * edges, hipMemcpyHostToDevice));
+    if (gN) KG_HIP(hipMemcpy(gM.p, h_M.data(), 4ull * gN, hipMemcpyHostToDevice));
+    if (!no_dist && edges) KG_HIP(hipMemcpy(gdist.p, h_dist.data(), 4ull * edges, hipMemcpyHostToDevice));
+    has_graph = true;
+    host_valid = true;
+}
+
+// GenRandom (kgraph.cpp:44-65): `size` distinct ids < N from an mt19937.
+static void gen_random(std::mt19937 &rng, uint32_t *addr, unsigned size, unsigned N) {
+    if (N == size) {
+        for (unsigned i = 0; i < size; ++i) addr[i] = i;
+        return;
+    }
+    for (unsigned i = 0; i < size; ++i) addr[i] = rng() % (N - size);
+    std::sort(addr, addr + size);
+    for (unsigned i = 1; i < size; ++i)
+        if (addr[i] <= addr[i - 1]) addr[i] = addr[i - 1] + 1;
+    unsigned off = rng() % N;
+    for (unsigned i = 0; i < size; ++i) addr[i] = (addr[i] + off) % N;
+}
+
+static void upload_rows(Handle &h, float const *src, bool src_device, unsigned N, unsigned D, size_t stride,
+                        DevBuf<float> &dst) {
+    unsigned Dp = (D + 3) & ~3u;
+    dst.reserve(std::max<size_t>(size_t(N) * Dp, 4));
+    if (N == 0) return;
+    KG_HIP(hipMemsetAsync(dst.p, 0, size_t(N) * Dp * 4, h.stream));
+    KG_HIP(hipMemcpy2DAsync(dst.p, Dp * 4, src, stride * 4, size_t(D) * 4, N,
+                            src_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, h.stream));
+}
+
+static void set_data(Handle &h, float const *X, bool dev, unsigned N, unsigned D, size_t stride, int metric) {
+    if (!X && N) throw invalid_argument("bad array");
+    if (D == 0 || D > 1024) throw invalid_argument("dimension must be in [1, 1024]");
+    if (stride < D) throw invalid_argument("bad stride");
+    if (metric < KG_METRIC_L2SQR || metric > KG_METRIC_ANGULAR) throw runtime_error("metric not supported");
+    if (N >= 0x7fffffffu) throw invalid_argument("dataset too large");
+    upload_rows(h, X, dev, N, D, stride, h.X);
+    h.N = N;
+    h.D = D;
+    h.Dp = (D + 3) & ~3u;
+    h.metric = metric;
+    h.norms.reserve(std::max<unsigned>(N, 1));
+    KG_HIP(kg_launch_norms(metric, h.X.p, N, D, h.Dp, h.norms.p, h.stream));
+    h.sync();
+    h.has_data = true;
+}
+
+// Batched search over device-resident queries / outputs (kgraph.cpp:423-564).
+static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstride, kg_search_params const &sp,
+                          unsigned const *Ks, unsigned const *Ps, uint32_t *d_ids, float *d_dists, size_t ostride,
+                          uint32_t *d_counts, uint32_t *d_comps, uint32_t *d_scans) {
+    if (!h.has_data) throw runtime_error("no dataset");
+    if (!h.has_graph) throw runtime_error("no index");
+    if (h.gN > h.N) throw runtime_error("dataset larger than index");
+    if (sp.init && sp.T > 1) throw runtime_error("when init > 0, T must be 1.");
+    if (nq == 0) return;
+    if (sp.init && !d_ids) throw invalid_argument("no initial data provided via ids");
+    if (sp.init && !(sp.init < sp.K)) throw invalid_argument("L < params.K");
+    if (sp.init && (Ks || Ps)) throw invalid_argument("init > 0 needs uniform K and P");
+    if (sp.T == 0) throw invalid_argument("T must be >= 1");
+    if (sp.init) {  // caller's starting points must be valid node ids
+        std::vector<uint32_t> ini(size_t(nq) * ostride);
+        h.sync();
+        KG_HIP(hipMemcpy(ini.data(), d_ids, 4 * ini.size(), hipMemcpyDeviceToHost));
+        for (unsigned q = 0; q < nq; ++q)
+            for (unsigned i = 0; i < sp.init; ++i)
+                if (ini[size_t(q) * ostride + i] >= h.gN) throw invalid_argument("initial id out of range");
+    }
+    if (sp.S == 0) throw invalid_argument("S must be >= 1");
+    unsigned const N = h.gN;
+    // per-query K/P, distinct P values for the seed table
+    unsigned kmax = 0, poolmax = 0;
+    std::vector<unsigned> pvals;
+    std::vector<uint32_t> pidx_h;
+    std::vector<uint32_t> brute;
+    for (unsigned q = 0; q < nq; ++q) {
+        unsigned K = Ks ? Ks[q] : sp.K, P = Ps ? Ps[q] : sp.P;
+        if (K > ostride) throw invalid_argument("K larger than the output row stride");
+        kmax = std::max(kmax, K);
+        if (P >= N) {
+            brute.push_back(q);
+            if (Ps) pidx_h.push_back(0);
+            continue;
+        }
+        poolmax = std::max(poolmax, K + P + 1);
+        if (Ps) {
+            auto it = std::find(pvals.begin(), pvals.end(), P);
+            if (it == pvals.end()) {
+                pvals.push_back(P);
+                it = pvals.end() - 1;
+            }
+            pidx_h.push_back(uint32_t(it - pvals.begin()));
+        }
+    }
+    if (!Ps && sp.P < N) pvals.push_back(sp.P);
+    if (poolmax > 65535) throw invalid_argument("K + P too large");
+    // seeds: one mt19937(seed) per search call, T GenRandom draws (kgraph.cpp:444-453)
+    std::vector<uint32_t> seeds_h, soff_h;
+    unsigned seed = sp.seed ? sp.seed : unsigned(time(nullptr));
+    if (sp.init == 0) {
+        for (unsigned P : pvals) {
+            soff_h.push_back(seeds_h.size());
+            std::mt19937 rng(seed);
+            size_t o = seeds_h.size();
+            seeds_h.resize(o + size_t(sp.T) * P);
+            for (unsigned t = 0; t < sp.T; ++t) gen_random(rng, &seeds_h[o + size_t(t) * P], P, N);
+        }
+    }
+    if (soff_h.empty()) soff_h.push_back(0);
+    if (seeds_h.empty()) seeds_h.push_back(0);
+    h.seeds.reserve(seeds_h.size());
+    h.seed_off.reserve(soff_h.size());
+    KG_HIP(hipMemcpyAsync(h.seeds.p, seeds_h.data(), 4 * seeds_h.size(), hipMemcpyHostToDevice, h.stream));
+    KG_HIP(hipMemcpyAsync(h.seed_off.p, soff_h.data(), 4 * soff_h.size(), hipMemcpyHostToDevice, h.stream));
+    if (Ks) {
+        h.Ks.reserve(nq);
+        KG_HIP(hipMemcpyAsync(h.Ks.p, Ks, 4ull * nq, hipMemcpyHostToDevice, h.stream));
+    }
+    if (Ps) {
+        h.Ps.reserve(nq);
+        h.pidx.reserve(nq);
+        KG_HIP(hipMemcpyAsync(h.Ps.p, Ps, 4ull * nq, hipMemcpyHostToDevice, h.stream));
+        KG_HIP(hipMemcpyAsync(h.pidx.p, pidx_h.data(), 4ull * nq, hipMemcpyHostToDevice, h.stream));
+    }
+    // queries: aligned copy when the caller's layout does not allow float4 loads
+    float const *Qk = dQ;
+    size_t qs = qstride;
+    if ((qstride & 3) || (reinterpret_cast<uintptr_t>(dQ) & 15)) {
+        h.q.reserve(size_t(nq) * h.Dp);
+        KG_HIP(hipMemcpy2DAsync(h.q.p, h.Dp * 4, dQ, qstride * 4, size_t(h.D) * 4, nq, hipMemcpyDeviceToDevice,
+                                h.stream));
+        Qk = h.q.p;
+        qs = h.Dp;
+    }
+    h.qn.reserve(nq);
+    KG_HIP(kg_launch_norms(h.metric, Qk, nq, h.D, unsigned(qs), h.qn.p, h.stream));
+
+    // graph-walk queries
+    unsigned const ngraph = nq - unsigned(brute.size());
+    if (ngraph) {
+        unsigned const wpb = 4;
+        size_t lpw = kg_search_lds_per_wave(poolmax, kmax + 1, h.Dp);
+        size_t lpb = lpw * wpb;
+        if (lpb > 65536) throw invalid_argument("K + P too large for the search kernel's LDS budget");
+        unsigned bpc = unsigned(std::max<size_t>(1, std::min<size_t>(8, h.lds_per_cu / lpb)));
+        unsigned blocks = std::min<unsigned>((ngraph + wpb - 1) / wpb, h.num_cus * bpc);
+        unsigned words = (N + 31) / 32;
+        size_t budget = size_t(8) << 30;  // visited bitsets: at most 8 GiB
+        while (blocks > 1 && size_t(blocks) * wpb * words * 4 > budget) blocks /= 2;
+        unsigned slots = blocks * wpb;
+        unsigned const tcap = 4096;
+        if (slots > h.vis_slots || words != h.vis_words) {
+            h.visited.release();
+            h.visited.reserve(size_t(slots) * words);
+            KG_HIP(hipMemsetAsync(h.visited.p, 0, size_t(slots) * words * 4, h.stream));
+            h.touched.reserve(size_t(slots) * tcap);
+            h.vis_slots = slots;
+            h.vis_words = words;
+        }
+        h.qcounter.reserve(1);
+        KG_HIP(hipMemsetAsync(h.qcounter.p, 0, 4, h.stream));
+        kg_search_args a{};
+        a.X = h.X.p;
+        a.norms = h.norms.p;
+        a.N = N;
+        a.D = h.D;
+        a.Dp = h.Dp;
+        a.goff = h.goff.p;
+        a.gids = h.gids.p;
+        a.gM = h.gM.p;
+        a.Q = Qk;
+        a.qnorms = h.qn.p;
+        a.nq = nq;
+        a.qstride = unsigned(qs);
+        a.Ks = Ks ? h.Ks.p : nullptr;
+        a.Ps = Ps ? h.Ps.p : nullptr;
+        a.pidx = Ps ? h.pidx.p : nullptr;
+        a.K0 = sp.K;
+        a.P0 = sp.P;
+        a.S = sp.S;
+        a.T = sp.T;
+        a.Mparam = sp.M;
+        a.init = sp.init;
+        a.epsilon = sp.epsilon;
+        a.seeds = h.seeds.p;
+        a.seed_off = h.seed_off.p;
+        a.init_ids = d_ids;
+        a.out_ids = d_ids;
+        a.out_dists = d_dists;
+        a.ostride = unsigned(ostride);
+        a.counts = d_counts;
+        a.comps = d_comps;
+        a.scans = d_scans;
+        a.visited = h.visited.p;
+        a.vis_words = words;
+        a.touched = h.touched.p;
+        a.touch_cap = tcap;
+        a.qcounter = h.qcounter.p;
+        a.pool_cap = poolmax;
+        a.res_cap = kmax + 1;
+        a.lds_per_wave = unsigned(lpw);
+        KG_HIP(hipEventRecord(h.ev[0], h.stream));
+        KG_HIP(kg_launch_search(h.metric, a, blocks, wpb, h.stream));
+        KG_HIP(hipEventRecord(h.ev[1], h.stream));
+    }
+    // brute-force queries (P >= N, kgraph.cpp:427-433)
+    if (!brute.empty()) {
+        if (Ks) {
+            for (uint32_t q : brute)
+                if (Ks[q] != Ks[brute[0]]) throw invalid_argument("brute-force queries need one K");
+        }
+        h.bsel.reserve(brute.size());
+        uint32_t *dsel = h.bsel.p;
+        KG_HIP(hipMemcpyAsync(dsel, brute.data(), 4 * brute.size(), hipMemcpyHostToDevice, h.stream));
+        kg_brute_args b{};
+        b.X = h.X.p;
+        b.norms = h.norms.p;
+        b.N = N;
+        b.D = h.D;
+        b.Dp = h.Dp;
+        b.Q = Qk;
+        b.qnorms = h.qn.p;
+        b.qstride = unsigned(qs);
+        b.qsel = dsel;
+        b.nq = unsigned(brute.size());
+        b.K = Ks ? Ks[brute[0]] : sp.K;
+        b.epsilon = sp.epsilon;
+        b.out_ids = d_ids;
+        b.out_dists = d_dists;
+        b.ostride = unsigned(ostride);
+        b.counts = d_counts;
+        b.comps = d_comps;
+        b.scans = d_scans;
+        KG_HIP(hipEventRecord(h.ev[2], h.stream));
+        KG_HIP(kg_launch_brute(h.metric, b, h.stream));
+        KG_HIP(hipEventRecord(h.ev[3], h.stream));
+    }
+    h.sync();
+    if (ngraph) KG_HIP(hipEventElapsedTime(&h.last_ms[0], h.ev[0], h.ev[1]));
+    if (!brute.empty()) KG_HIP(hipEventElapsedTime(&h.last_ms[1], h.ev[2], h.ev[3]));
+}
+
+// KNNGRAPH v2 (kgraph.cpp:272-274, 291-373).
+static char const MAGIC[8] = {'K', 'N', 'N', 'G', 'R', 'A', 'P', 'H'};
+
+static void save_index(Handle &h, char const *path, int format) {
+    h.pull_graph();
+    if (format == KG_FORMAT_TEXT) {
+        std::ofstream os(path);
+        if (!os) throw io_error(path);
+        os << h.gN << '\n';
+        for (unsigned i = 0; i < h.gN; ++i) {
+            os << (h.h_off[i + 1] - h.h_off[i]);
+            for (uint32_t o = h.h_off[i]; o < h.h_off[i + 1]; ++o)
+                os << ' ' << h.h_ids[o] << ' ' << (h.no_dist ? 0.f : h.h_dist[o]);
+            os << '\n';
+        }
+        return;
+    }
+    if ((format & KG_FORMAT_NO_DIST) == 0 && h.no_dist) throw runtime_error("distance information is not available");
+    std::ofstream os(path, std::ios::binary);
+    if (!os) throw io_error(path);
+    uint32_t hdr[3] = {2u, uint32_t(format), h.gN};
+    os.write(MAGIC, 8);
+    os.write(reinterpret_cast<char const *>(hdr), 12);
+    std::vector<unsigned char> rec;
+    for (unsigned i = 0; i < h.gN; ++i) {
+        uint32_t K = h.h_off[i + 1] - h.h_off[i];
+        uint32_t mk[2] = {h.h_M[i], K};
+        os.write(reinterpret_cast<char const *>(mk), 8);
+        if (format & KG_FORMAT_NO_DIST) {
+            os.write(reinterpret_cast<char const *>(&h.h_ids[h.h_off[i]]), 4ull * K);
+        } else {
+            rec.assign(12ull * K, 0);
+            for (uint32_t k = 0; k < K; ++k) {
+                memcpy(&rec[12ull * k], &h.h_ids[h.h_off[i] + k], 4);
+                memcpy(&rec[12ull * k + 4], &h.h_dist[h.h_off[i] + k], 4);
+            }
+            os.write(reinterpret_cast<char const *>(rec.data()), rec.size());
+        }
+    }
+    if (!os) throw io_error(path);
+}
+
+static void load_index(Handle &h, char const *path) {
+    std::ifstream is(path, std::ios::binary);
+    if (!is) throw io_error(path);
+    char magic[8];
+    uint32_t hdr[2], N;
+    is.read(magic, 8);
+    is.read(reinterpret_cast<char *>(hdr), 8);
+    if (!is) throw runtime_error("error reading index file.");
+    if (hdr[0] != 2) throw runtime_error("data version not supported.");
+    is.read(reinterpret_cast<char *>(&N), 4);
+    if (!is) throw runtime_error("error reading index file.");
+    if (memcmp(magic, MAGIC, 8)) throw runtime_error("index corrupted.");
+    bool nd = hdr[1] & KG_FORMAT_NO_DIST;
+    std::vector<uint32_t> off(1, 0), ids, M(N);
+    std::vector<float> dist;
+    std::vector<unsigned char> rec;
+    for (unsigned i = 0; i < N; ++i) {
+        uint32_t mk[2];
+        is.read(reinterpret_cast<char *>(mk), 8);
+        if (!is) throw runtime_error("error reading index file.");
+        M[i] = mk[0];
+        uint64_t K = mk[1];
+        if (uint64_t(off.back()) + K >= 0xffffffffull) throw runtime_error("index too large");
+        size_t o = ids.size();
+        ids.resize(o + K);
+        if (nd) {
+            is.read(reinterpret_cast<char *>(&ids[o]), 4 * K);
+        } else {
+            dist.resize(o + K);
+            rec.resize(12 * K);
+            is.read(reinterpret_cast<char *>(rec.data()), rec.size());
+            for (size_t k = 0; k < K; ++k) {
+                memcpy(&ids[o + k], &rec[12 * k], 4);
+                memcpy(&dist[o + k], &rec[12 * k + 4], 4);
+            }
+        }
+        if (!is) throw runtime_error("error reading index file.");
+        off.push_back(uint32_t(o + K));
+    }
+    for (uint32_t id : ids)
+        if (id >= N) throw runtime_error("index corrupted.");
+    h.no_dist = nd;
+    h.h_off.swap(off);
+    h.h_ids.swap(ids);
+    h.h_dist.swap(dist);
+    h.h_M.swap(M);
+    h.push_graph();
+}
+
+}  // namespace kga
+
+// ---------------------------------------------------------------------------
+using namespace kga;
+
+struct kg_handle {
+    Handle h;
+};
+
+template <typename F>
+static int guard(kg_handle *k, F &&f) {
+    if (!k) return KG_ERR_INVALID_ARGUMENT;
+    try {
+        k->h.err.clear();
+        f(k->h);
+        return KG_OK;
+    } catch (kga::invalid_argument const &e) {
+        k->h.err = e.what();
+        return KG_ERR_INVALID_ARGUMENT;
+    } catch (kga::io_error const &e) {
+        k->h.err = e.what();
+        return KG_ERR_IO;
+    } catch (kga::device_error const &e) {
+        k->h.err = e.what();
+        return KG_ERR_DEVICE;
+    } catch (std::bad_alloc const &) {
+        k->h.err = "out of host memory";
+        return KG_ERR_RUNTIME;
+    } catch (std::exception const &e) {
+        k->h.err = e.what();
+        return KG_ERR_RUNTIME;
+    }
+}
+
+extern "C" {
+
+char const *kg_version(void) { return "kgraph_amd-1.0-gfx950"; }
+
+kg_handle *kg_create(void) {
+    try {
+        return new kg_handle();
+    } catch (...) {
+        return nullptr;
+    }
+}
+
+void kg_destroy(kg_handle *h) { delete h; }
+
+char const *kg_last_error(kg_handle const *h) { return h ? h->h.err.c_str() : "null handle"; }
+
+void kg_index_params_default(kg_index_params *p) {
+    *p = kg_index_params{30, 100, 25, 10, 100, 100, 1998, 0.002f, 0.99f, 0, 0};
+}
+
+void kg_search_params_default(kg_search_params *p) { *p = kg_search_params{25, 0, 100, 10, 1, 1e30f, 1998, 0}; }
+
+void kg_set_verbosity(unsigned v) { kga::verbosity = v; }
+
+int kg_set_data(kg_handle *k, float const *X, unsigned N, unsigned D, size_t stride, int metric) {
+    return guard(k, [&](Handle &h) { set_data(h, X, false, N, D, stride, metric); });
+}
+
+int kg_set_data_device(kg_handle *k, float const *dX, unsigned N, unsigned D, size_t stride, int metric) {
+    return guard(k, [&](Handle &h) { set_data(h, dX, true, N, D, stride, metric); });
+}
+
+int kg_set_precision(kg_handle *k, int precision) {
+    return guard(k, [&](Handle &h) {
+        if (precision != KG_PRECISION_FP32 && precision != KG_PRECISION_BF16)
+            throw kga::invalid_argument("unknown precision");
+        h.precision = precision;
+    });
+}
+
+int kg_build(kg_handle *k, kg_index_params const *params, kg_index_info *info) {
+    return guard(k, [&](Handle &h) {
+        if (!params) throw kga::invalid_argument("null params");
+        build_index(h, *params, info);
+    });
+}
+
+int kg_search_device(kg_handle *k, float const *dQ, unsigned nq, size_t qstride, kg_search_params const *params,
+                     unsigned const *Ks, unsigned const *Ps, unsigned *d_ids, float *d_dists, size_t ostride,
+                     unsigned *d_counts, unsigned *d_comps, unsigned *d_scans) {
+    return guard(k, [&](Handle &h) {
+        if (!params || !d_ids || !d_counts || (!dQ && nq)) throw kga::invalid_argument("null argument");
+        search_device(h, dQ, nq, qstride, *params, Ks, Ps, d_ids, d_dists, ostride, d_counts, d_comps, d_scans);
+    });
+}
+
+int kg_search(kg_handle *k, float const *Q, unsigned nq, size_t qstride, kg_search_params const *params,
+              unsigned const *Ks, unsigned const *Ps, unsigned *ids, float *dists, size_t ostride, unsigned *counts,
+              unsigned *comps, unsigned *scans) {
+    return guard(k, [&](Handle &h) {
+        if (!params || !ids || !counts || (!Q && nq)) throw kga::invalid_argument("null argument");
+        if (!h.has_data) throw kga::runtime_error("no dataset");
+        if (qstride < h.D) throw kga::invalid_argument("bad stride");
+        if (nq == 0) return;
+        DevBuf<float> dq, dd;
+        DevBuf<uint32_t> di, dc, dcomp, dscan;
+        upload_rows(h, Q, false, nq, h.D, qstride, dq);
+        di.reserve(size_t(nq) * ostride);
+        dc.reserve(nq);
+        KG_HIP(hipMemsetAsync(di.p, 0, size_t(nq) * ostride * 4, h.stream));
+        if (params->init)
+            KG_HIP(hipMemcpyAsync(di.p, ids, size_t(nq) * ostride * 4, hipMemcpyHostToDevice, h.stream));
+        if (dists) {
+            dd.reserve(size_t(nq) * ostride);
+            KG_HIP(hipMemsetAsync(dd.p, 0, size_t(nq) * ostride * 4, h.stream));
+        }
+        if (comps) dcomp.reserve(nq);
+        if (scans) dscan.reserve(nq);
+        search_device(h, dq.p, nq, h.Dp, *params, Ks, Ps, di.p, dists ? dd.p : nullptr, ostride, dc.p,
+                      comps ? dcomp.p : nullptr, scans ? dscan.p : nullptr);
+        KG_HIP(hipMemcpy(ids, di.p, size_t(nq) * ostride * 4, hipMemcpyDeviceToHost));
+        if (dists) KG_HIP(hipMemcpy(dists, dd.p, size_t(nq) * ostride * 4, hipMemcpyDeviceToHost));
+        KG_HIP(hipMemcpy(counts, dc.p, 4ull * nq, hipMemcpyDeviceToHost));
+        if (comps) KG_HIP(hipMemcpy(comps, dcomp.p, 4ull * nq, hipMemcpyDeviceToHost));
+        if (scans) KG_HIP(hipMemcpy(scans, dscan.p, 4ull * nq, hipMemcpyDeviceToHost));
+    });
+}
+
+int kg_bruteforce(kg_handle *k, float const *Q, unsigned nq, size_t qstride, unsigned K, float epsilon,
+                  unsigned *ids, float *dists, size_t ostride, unsigned *counts) {
+    return guard(k, [&](Handle &h) {
+        if (!h.has_data) throw kga::runtime_error("no dataset");
+        if (!Q || !ids || !counts) throw kga::invalid_argument("null argument");
+        if (K == 0 || K > 1024 || K > ostride) throw kga::invalid_argument("K must be in [1, min(1024, ostride)]");
+        if (nq == 0) return;
+        DevBuf<float> dq, dqn, dd;
+        DevBuf<uint32_t> di, dc;
+        upload_rows(h, Q, false, nq, h.D, qstride, dq);
+        dqn.reserve(nq);
+        KG_HIP(kg_launch_norms(h.metric, dq.p, nq, h.D, h.Dp, dqn.p, h.stream));
+        di.reserve(size_t(nq) * ostride);
+        dd.reserve(size_t(nq) * ostride);
+        dc.reserve(nq);
+        KG_HIP(hipMemsetAsync(di.p, 0, size_t(nq) * ostride * 4, h.stream));
+        KG_HIP(hipMemsetAsync(dd.p, 0, size_t(nq) * ostride * 4, h.stream));
+        kg_brute_args b{};
+        b.X = h.X.p;
+        b.norms = h.norms.p;
+        b.N = h.N;
+        b.D = h.D;
+        b.Dp = h.Dp;
+        b.Q = dq.p;
+        b.qnorms = dqn.p;
+        b.qstride = h.Dp;
+        b.nq = nq;
+        b.K = K;
+        b.epsilon = epsilon;
+        b.out_ids = di.p;
+        b.out_dists = dd.p;
+        b.ostride = unsigned(ostride);
+        b.counts = dc.p;
+        KG_HIP(hipEventRecord(h.ev[2], h.stream));
+        KG_HIP(kg_launch_brute(h.metric, b, h.stream));
+        KG_HIP(hipEventRecord(h.ev[3], h.stream));
+        h.sync();
+        KG_HIP(hipEventElapsedTime(&h.last_ms[1], h.ev[2], h.ev[3]));
+        KG_HIP(hipMemcpy(ids, di.p, size_t(nq) * ostride * 4, hipMemcpyDeviceToHost));
+        if (dists) KG_HIP(hipMemcpy(dists, dd.p, size_t(nq) * ostride * 4, hipMemcpyDeviceToHost));
+        KG_HIP(hipMemcpy(counts, dc.p, 4ull * nq, hipMemcpyDeviceToHost));
+    });
+}
+
+int kg_save(kg_handle *k, char const *path, int format) {
+    return guard(k, [&](Handle &h) {
+        if (!path) throw kga::invalid_argument("null path");
+        save_index(h, path, format);
+    });
+}
+
+int kg_load(kg_handle *k, char const *path) {
+    return guard(k, [&](Handle &h) {
+        if (!path) throw kga::invalid_argument("null path");
+        load_index(h, path);
+    });
+}
+
+int kg_get_nn(kg_handle *k, unsigned id, unsigned *nns, float *dists, unsigned *M, unsigned *L) {
+    return guard(k, [&](Handle &h) {
+        h.pull_graph();
+        if (!(id < h.gN)) throw kga::invalid_argument("id too big");
+        uint32_t b = h.h_off[id], e = h.h_off[id + 1];
+        if (M) *M = h.h_M[id];
+        if (L) *L = e - b;
+        if (nns) std::copy(h.h_ids.begin() + b, h.h_ids.begin() + e, nns);
+        if (dists) {
+            if (h.no_dist) throw kga::runtime_error("distance information is not available");
+            std::copy(h.h_dist.begin() + b, h.h_dist.begin() + e, dists);
+        }
+    });
+}
+
+int kg_reverse(kg_handle *k, int rev) {
+    return guard(k, [&](Handle &h) { reverse_index(h, rev); });
+}
+
+int kg_prune(kg_handle *k, unsigned level) {
+    return guard(k, [&](Handle &h) { prune_index(h, level); });
+}
+
+int kg_index_size(kg_handle *k, unsigned *N, uint64_t *edges) {
+    return guard(k, [&](Handle &h) {
+        if (N) *N = h.has_graph ? h.gN : 0;
+        if (edges) *edges = h.has_graph ? h.edges : 0;
+    });
+}
+
+int kg_set_graph(kg_handle *k, unsigned N, uint64_t const *offsets, unsigned const *ids, float const *dists,
+                 unsigned const *M) {
+    return guard(k, [&](Handle &h) {
+        if (!offsets || !M || (!ids && offsets[N])) throw kga::invalid_argument("null argument");
+        uint64_t E = offsets[N];
+        if (E >= 0xffffffffull) throw kga::invalid_argument("index too large");
+        h.h_off.resize(N + 1);
+        for (unsigned i = 0; i <= N; ++i) {
+            if (i && offsets[i] < offsets[i - 1]) throw kga::invalid_argument("offsets must be non-decreasing");
+            h.h_off[i] = uint32_t(offsets[i]);
+        }
+        for (uint64_t e = 0; e < E; ++e)
+            if (ids[e] >= N) throw kga::invalid_argument("neighbour id out of range");
+        h.h_ids.assign(ids, ids + E);
+        h.h_M.assign(M, M + N);
+        h.no_dist = dists == nullptr;
+        if (dists)
+            h.h_dist.assign(dists, dists + E);
+        else
+            h.h_dist.clear();
+        h.push_graph();
+    });
+}
+
+int kg_get_graph(kg_handle *k, uint64_t *offsets, unsigned *ids, float *dists, unsigned *M) {
+    return guard(k, [&](Handle &h) {
+        h.pull_graph();
+        for (unsigned i = 0; i <= h.gN; ++i)
+            if (offsets) offsets[i] = h.h_off[i];
+        if (ids) std::copy(h.h_ids.begin(), h.h_ids.end(), ids);
+        if (dists) {
+            if (h.no_dist) throw kga::runtime_error("distance information is not available");
+            std::copy(h.h_dist.begin(), h.h_dist.end(), dists);
+        }
+        if (M) std::copy(h.h_M.begin(), h.h_M.end(), M);
+    });
+}
+
+int kg_last_time_ms(kg_handle *k, int which, float *ms) {
+    return guard(k, [&](Handle &h) {
+        if (which < 0 || which >= 8 || !ms) throw kga::invalid_argument("bad timer");
+        *ms = h.last_ms[which];
+    });
+}
+
+int kg_build_trace(kg_handle *k, float *out, unsigned cap, unsigned *n) {
+    return guard(k, [&](Handle &h) {
+        unsigned it = unsigned(h.trace.size() / 4);
+        if (n) *n = it;
+        if (out)
+            for (unsigned i = 0; i < std::min(it, cap) * 4; ++i) out[i] = h.trace[i];
+    });
+}
+
+}  // extern "C"

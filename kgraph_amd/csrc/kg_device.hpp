@@ -1,0 +1,131 @@
+// kg_device.hpp -- device-side building blocks shared by the kgraph_amd HIP
+// kernels (gfx950, wave64).
+//
+// Canonical FP32 distance order ("X16"), identical to the order the
+// reference's xsimd AVX-512 l2sqr kernels use with FMA contraction
+// (kgraph-data.h:41-103; xsimd arch/xsimd_avx512f.hpp:1393-1403 ->
+// arch/xsimd_sse3.hpp:47-52):
+//   * element i < D16 (= D rounded down to 16) goes to partial i % 16, the 16
+//     partials accumulate with fmaf in increasing i;
+//   * u[e] = (p[e] + p[e+4]) + (p[e+8] + p[e+12]),  r = (u0 + u1) + (u2 + u3);
+//   * the tail [D16, D) is added sequentially with fmaf.
+// A group of 4 adjacent lanes evaluates one row: lane t of the group loads the
+// float4 chunks k = t, t+4, t+8, ... so its float4 accumulator holds partials
+// 4t..4t+3; a quad-xor-1 then quad-xor-2 exchange forms u[] in every lane of
+// the group (IEEE addition is commutative, so all four lanes hold the same
+// bits), and the final sum is formed in-lane.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define KG_WAVE 64
+
+namespace kgd {
+
+enum { METRIC_L2SQR = 0, METRIC_EUCLIDEAN = 1, METRIC_ANGULAR = 2 };
+
+__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
+
+// Orders floats as unsigned integers (ascending float order == ascending key).
+__device__ __forceinline__ uint32_t ord_f32(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float unord_f32(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+// Compiler + LDS ordering point for cross-lane communication inside one wave.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+__device__ __forceinline__ unsigned popc64(uint64_t b) { return unsigned(__builtin_popcountll(b)); }
+// index of the lowest set bit (b != 0)
+__device__ __forceinline__ unsigned ctz64(uint64_t b) { return unsigned(__builtin_ctzll(b)); }
+
+template <typename T>
+__device__ __forceinline__ T uniform(T v) {
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+// DPP quad permutations: [1,0,3,2] (xor 1) and [2,3,0,1] (xor 2).
+__device__ __forceinline__ float quad_xor1(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float quad_xor2(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));
+}
+
+// Reduce a quad's float4 partial accumulators to the canonical sum.  Every
+// lane of the quad returns the same value.
+__device__ __forceinline__ float quad_reduce(float4 p) {
+    p.x = p.x + quad_xor1(p.x);
+    p.y = p.y + quad_xor1(p.y);
+    p.z = p.z + quad_xor1(p.z);
+    p.w = p.w + quad_xor1(p.w);
+    p.x = p.x + quad_xor2(p.x);
+    p.y = p.y + quad_xor2(p.y);
+    p.z = p.z + quad_xor2(p.z);
+    p.w = p.w + quad_xor2(p.w);
+    return (p.x + p.y) + (p.z + p.w);
+}
+
+// One quad computes the canonical SSD / dot of data row `a` (global) and
+// vector `b` (LDS or global).  t = lane within the quad (0..3).  Rows are
+// 16-byte aligned; D16 = D & ~15.
+template <int METRIC>
+__device__ __forceinline__ float quad_metric(float const *__restrict__ a, float const *__restrict__ b, unsigned D,
+                                             unsigned t) {
+    unsigned const nch = (D & ~15u) >> 2;
+    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 const *a4 = reinterpret_cast<float4 const *>(a);
+    float4 const *b4 = reinterpret_cast<float4 const *>(b);
+    for (unsigned k = t; k < nch; k += 4) {
+        float4 x = a4[k];
+        float4 y = b4[k];
+        if (METRIC == METRIC_L2SQR) {
+            float dx = x.x - y.x, dy = x.y - y.y, dz = x.z - y.z, dw = x.w - y.w;
+            p.x = __builtin_fmaf(dx, dx, p.x);
+            p.y = __builtin_fmaf(dy, dy, p.y);
+            p.z = __builtin_fmaf(dz, dz, p.z);
+            p.w = __builtin_fmaf(dw, dw, p.w);
+        } else {
+            p.x = __builtin_fmaf(x.x, y.x, p.x);
+            p.y = __builtin_fmaf(x.y, y.y, p.y);
+            p.z = __builtin_fmaf(x.z, y.z, p.z);
+            p.w = __builtin_fmaf(x.w, y.w, p.w);
+        }
+    }
+    float acc = quad_reduce(p);
+    for (unsigned i = D & ~15u; i < D; ++i) {
+        if (METRIC == METRIC_L2SQR) {
+            float d = a[i] - b[i];
+            acc = __builtin_fmaf(d, d, acc);
+        } else {
+            acc = __builtin_fmaf(a[i], b[i], acc);
+        }
+    }
+    return acc;
+}
+
+// Final metric value from the quad result (python-api.cpp:56-58, 66-68;
+// kgraph-data.h:41-61).  na = data-row norm, nb = other norm.
+template <int METRIC>
+__device__ __forceinline__ float finish_metric(float acc, float na, float nb) {
+    if (METRIC == METRIC_L2SQR) return acc;
+    if (METRIC == METRIC_EUCLIDEAN) return (na + nb) - acc;
+    return -acc / __builtin_fmaf(na, nb, 1.1920928955078125e-07f);
+}
+
+// Row norm as stored per metric (python-api.cpp:51-69, 110-114).
+template <int METRIC>
+__device__ __forceinline__ float finish_norm(float n2) {
+    if (METRIC == METRIC_EUCLIDEAN) return n2 / 2;
+    if (METRIC == METRIC_ANGULAR) return __fsqrt_rn(n2);
+    return 0.f;
+}
+
+}  // namespace kgd

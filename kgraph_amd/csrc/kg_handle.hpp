@@ -1,0 +1,104 @@
+// kg_handle.hpp -- the state behind a kg_handle (host side of libkgraph_amd).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/kgraph_amd.h"
+
+namespace kga {
+
+// Exception types mirroring kgraph.h:287-300; mapped to KG_ERR_* at the ABI.
+struct invalid_argument : std::invalid_argument {
+    using std::invalid_argument::invalid_argument;
+};
+struct runtime_error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct io_error : runtime_error {
+    using runtime_error::runtime_error;
+};
+struct device_error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+inline void hip_check(hipError_t e, char const *what) {
+    if (e != hipSuccess) throw device_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+#define KG_HIP(x) ::kga::hip_check((x), #x)
+
+extern unsigned verbosity;
+
+// A device allocation that grows on demand and is freed with the handle.
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void reserve(size_t want) {
+        if (want <= n) return;
+        release();
+        if (want) KG_HIP(hipMalloc(&p, want * sizeof(T)));
+        n = want;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+struct Handle {
+    std::string err;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int num_cus = 0;
+    size_t lds_per_cu = 0;
+    int precision = KG_PRECISION_FP32;
+
+    // dataset (dense oracle): N rows, row stride Dp floats (D rounded up to 4)
+    unsigned N = 0, D = 0, Dp = 0;
+    int metric = KG_METRIC_L2SQR;
+    DevBuf<float> X, norms;
+    bool has_data = false;
+
+    // index: CSR with 32-bit edge offsets
+    unsigned gN = 0;
+    bool has_graph = false;
+    bool no_dist = false;
+    DevBuf<uint32_t> goff, gids, gM;
+    DevBuf<float> gdist;
+    uint64_t edges = 0;
+    // host mirror, refreshed lazily from the device copy
+    bool host_valid = false;
+    std::vector<uint32_t> h_off, h_ids, h_M;
+    std::vector<float> h_dist;
+
+    // search scratch
+    DevBuf<uint32_t> visited, touched, qcounter;
+    DevBuf<uint32_t> seeds, seed_off, Ks, Ps, pidx, bsel;
+    DevBuf<float> q, qn, out_dists;
+    DevBuf<uint32_t> out_ids, counts, comps, scans;
+    unsigned vis_slots = 0, vis_words = 0;
+
+    // timing + build trace
+    hipEvent_t ev[16] = {};
+    float last_ms[8] = {};
+    std::vector<float> trace;
+
+    Handle();
+    ~Handle();
+    void sync() { KG_HIP(hipStreamSynchronize(stream)); }
+    void pull_graph();                    // device -> host mirror
+    void push_graph();                    // host mirror -> device
+};
+
+// Defined in kg_build.cpp.
+void build_index(Handle &h, kg_index_params const &params, kg_index_info *info);
+void reverse_index(Handle &h, int rev);
+void prune_index(Handle &h, unsigned level);
+
+}  // namespace kga

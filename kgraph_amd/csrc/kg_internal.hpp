@@ -1,0 +1,70 @@
+// kg_internal.hpp -- host/device shared structures of libkgraph_amd (not part
+// of the public C ABI).
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime.h>
+
+struct kg_search_args {
+    // data rows (row stride Dp floats, 16-byte aligned) and per-row norms
+    float const *X;
+    float const *norms;
+    unsigned N, D, Dp;
+    // graph in CSR form
+    uint32_t const *goff;  // N+1
+    uint32_t const *gids;
+    uint32_t const *gM;
+    // queries
+    float const *Q;
+    float const *qnorms;
+    unsigned nq, qstride;
+    uint32_t const *Ks;    // per-query K (NULL = K0)
+    uint32_t const *Ps;    // per-query P (NULL = P0)
+    uint32_t const *pidx;  // per-query index into seed_off (when Ps != NULL)
+    unsigned K0, P0;
+    unsigned S, T, Mparam, init;
+    float epsilon;
+    uint32_t const *seeds;     // concatenated T x P seed ids per distinct P
+    uint32_t const *seed_off;  // start of each distinct P's block
+    uint32_t const *init_ids;  // init > 0: caller's starting ids, row stride ostride
+    // outputs
+    uint32_t *out_ids;
+    float *out_dists;
+    unsigned ostride;
+    uint32_t *counts, *comps, *scans;
+    // per-wave scratch
+    uint32_t *visited;
+    unsigned vis_words;
+    uint32_t *touched;
+    unsigned touch_cap;
+    uint32_t *qcounter;
+    unsigned pool_cap, res_cap, lds_per_wave;
+};
+
+struct kg_brute_args {
+    float const *X;
+    float const *norms;
+    unsigned N, D, Dp;
+    float const *Q;       // queries (row stride qstride) or NULL: query = data row rows[q]
+    float const *qnorms;
+    unsigned qstride;
+    uint32_t const *rows;  // data rows used as queries when Q == NULL (self excluded)
+    uint32_t const *qsel;  // optional subset of query indices (NULL = all)
+    unsigned nq;
+    unsigned K;
+    float epsilon;
+    uint32_t *out_ids;
+    float *out_dists;
+    unsigned ostride;
+    uint32_t *counts;
+    uint32_t *comps, *scans;  // optional: set to N and 0 (kgraph.cpp:428-431)
+};
+
+// Launchers (defined in the .hip files).
+size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp);
+hipError_t kg_launch_search(int metric, kg_search_args const &a, unsigned blocks, unsigned waves_per_block,
+                            hipStream_t s);
+hipError_t kg_launch_brute(int metric, kg_brute_args const &a, hipStream_t s);
+hipError_t kg_launch_norms(int metric, float const *X, unsigned N, unsigned D, unsigned Dp, float *out,
+                           hipStream_t s);

@@ -1,0 +1,151 @@
+"""GPU search parity: the HIP graph walk (through the C ABI) returns the
+oracle's ids, distances, result counts and per-query counters bit-exactly on
+the same index, the same queries and the same entry points."""
+import numpy as np
+import pytest
+
+from oracle import kgo
+
+pytestmark = pytest.mark.gpu
+
+L2, EUC, ANG = 0, 1, 2
+
+
+def _dataset(kind, n, nq, d, seed):
+    from kgraph_amd import datagen
+    if kind == "uniform":
+        return datagen.uniform(n, d, seed), datagen.uniform(nq, d, seed + 1)
+    if kind == "intmix":
+        X = datagen.integer_mixture(n + nq, d, 50, seed, seed + 1)
+        return X[:n], X[n:]
+    X = datagen.angular_mixture(n + nq, d, 50, seed, seed + 1)
+    return X[:n], X[n:]
+
+
+_CACHE = {}
+
+
+def setup(kind, metric, n=6000, nq=200, d=16, build=None):
+    key = (kind, metric, n, nq, d)
+    if key not in _CACHE:
+        X, Q = _dataset(kind, n, nq, d, 11)
+        idx = kgo.Index()
+        kw = dict(K=20, L=50, S=10, R=100, iterations=12, controls=20, reverse=-1)
+        kw.update(build or {})
+        idx.build(X, metric, **kw)
+        _CACHE[key] = (X, Q, idx)
+    return _CACHE[key]
+
+
+def gpu_index(X, idx, metric):
+    import kgraph_amd
+    g = kgraph_amd.KGraph(X, "angular" if metric == ANG else "euclidean", l2sqr=(metric == L2))
+    off, ids, d, M = idx.export()
+    g.set_graph(off, ids, d, M)
+    return g
+
+
+def compare(X, Q, idx, metric, **kw):
+    g = gpu_index(X, idx, metric)
+    K = kw.get("K", 25)
+    ids, d, c, comps, scans = idx.search(X, Q, metric, **{k: v for k, v in kw.items()})
+    gkw = {k: v for k, v in kw.items() if k not in ("init", "init_ids")}
+    gi, gd = g.search(Q, withDistance=True, stats=True, **gkw)
+    gc, gcomp, gscan = g.last_counts, g.last_comps, g.last_scans
+    assert np.array_equal(gc, c)
+    for q in range(len(Q)):
+        n = c[q]
+        assert np.array_equal(gi[q, :n], ids[q, :n]), q
+        assert np.array_equal(gd[q, :n], d[q, :n]), q
+    assert np.array_equal(gcomp, comps)
+    assert np.array_equal(gscan, scans)
+    return gi, gd, c
+
+
+@pytest.mark.parametrize("kind,metric,d", [("uniform", EUC, 16), ("uniform", L2, 16), ("intmix", L2, 128),
+                                           ("angular", ANG, 96), ("uniform", EUC, 37)])
+def test_search_parity_default_params(kind, metric, d):
+    X, Q, idx = setup(kind, metric, d=d)
+    compare(X, Q, idx, metric, K=10, P=100)
+
+
+@pytest.mark.parametrize("P,S,M", [(10, 10, 0), (50, 1, 0), (200, 70, 0), (64, 10, 40), (300, 10, 0)])
+def test_search_parity_params(P, S, M):
+    X, Q, idx = setup("intmix", L2, d=128)
+    compare(X, Q, idx, L2, K=10, P=P, S=S, M=M)
+
+
+def test_search_parity_trials():
+    X, Q, idx = setup("uniform", EUC)
+    compare(X, Q, idx, EUC, K=10, P=30, T=3)
+
+
+def test_search_parity_epsilon():
+    X, Q, idx = setup("uniform", L2)
+    bids, bd, _ = kgo.bruteforce(X, Q, 10, L2)
+    eps = float(np.median(bd[:, 4]))
+    gi, gd, c = compare(X, Q, idx, L2, K=10, P=50, epsilon=eps)
+    assert (c < 10).any() and (gd[np.arange(10)[None, :] < c[:, None]] <= eps).all()
+
+
+def test_search_parity_mixed_batch():
+    X, Q, idx = setup("intmix", L2, d=128)
+    rs = np.random.RandomState(3)
+    Ks = rs.choice([1, 10, 100], size=len(Q), p=[0.5, 0.4, 0.1]).astype(np.uint32)
+    Ps = rs.choice([50, 100, 300, 500], size=len(Q)).astype(np.uint32)
+    g = gpu_index(X, idx, L2)
+    gi, gd = g.search(Q, K=10, P=100, Ks=Ks, Ps=Ps, withDistance=True, stats=True)
+    for q in range(len(Q)):
+        ids, d, c, comps, scans = idx.search(X, Q[q:q + 1], L2, K=int(Ks[q]), P=int(Ps[q]))
+        assert g.last_counts[q] == c[0]
+        assert np.array_equal(gi[q, :c[0]], ids[0, :c[0]])
+        assert np.array_equal(gd[q, :c[0]], d[0, :c[0]])
+        assert g.last_comps[q] == comps[0] and g.last_scans[q] == scans[0]
+
+
+def test_search_bruteforce_path_when_P_ge_N():
+    X, Q, idx = setup("uniform", EUC, n=500, nq=20, build=dict(K=10, L=20))
+    compare(X, Q, idx, EUC, K=10, P=500)
+
+
+def test_search_init_starting_points():
+    X, Q, idx = setup("uniform", L2)
+    import kgraph_amd
+    from kgraph_amd import _lib
+    g = gpu_index(X, idx, L2)
+    init = np.random.RandomState(5).randint(0, len(X), size=(len(Q), 5)).astype(np.uint32)
+    ids, d, c, comps, scans = idx.search(X, Q, L2, K=10, P=40, init=5, init_ids=init)
+    gids = np.zeros((len(Q), 10), dtype=np.uint32)
+    gids[:, :5] = init
+    gd = np.zeros((len(Q), 10), dtype=np.float32)
+    gc = np.zeros(len(Q), dtype=np.uint32)
+    Qc = _lib.f32c(Q)
+    sp = _lib.SearchParams(10, 0, 40, 10, 1, 1e30, 1998, 5)
+    g._h.call("kg_search", _lib.ptr(Qc), len(Q), Qc.shape[1], sp, None, None, _lib.ptr(gids), _lib.ptr(gd), 10,
+              _lib.ptr(gc), None, None)
+    assert np.array_equal(gc, c) and np.array_equal(gids, ids) and np.array_equal(gd, d)
+
+
+def test_bruteforce_parity():
+    import kgraph_amd
+    for kind, metric, d in [("uniform", EUC, 16), ("intmix", L2, 128), ("angular", ANG, 96)]:
+        X, Q = _dataset(kind, 20000, 50, d, 3)
+        g = kgraph_amd.KGraph(X, "angular" if metric == ANG else "euclidean", l2sqr=(metric == L2))
+        gi, gd = g.search(Q, K=100, withDistance=True)  # no index: exact scan
+        bi, bd, bc = kgo.bruteforce(X, Q, 100, metric)
+        assert np.array_equal(gi, bi) and np.array_equal(gd, bd), kind
+
+
+def test_save_load_roundtrip_then_search(tmp_path):
+    X, Q, idx = setup("uniform", EUC)
+    p = str(tmp_path / "oracle.kgraph")
+    idx.save(p)
+    import kgraph_amd
+    g = kgraph_amd.KGraph(X, "euclidean")
+    g.load(p)
+    p2 = str(tmp_path / "gpu.kgraph")
+    g.save(p2)
+    assert open(p, "rb").read() == open(p2, "rb").read()
+    gi = g.search(Q, K=10, P=100)
+    ids, d, c, _, _ = idx.search(X, Q, EUC, K=10, P=100)
+    assert np.array_equal(gi, ids)

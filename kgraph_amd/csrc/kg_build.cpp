@@ -1,11 +1,372 @@
-// kg_build.cpp -- host driver of the NN-descent build (placeholder until the
-// build kernels land).
+// kg_build.cpp -- host driver of the GPU NN-descent build
+// (KGraphConstructor, kgraph.cpp:886-999), reverse-edge completion
+// (kgraph.cpp:667-702) and pruning (kgraph.cpp:584-665).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <memory>
+#include <random>
+
 #include "kg_handle.hpp"
+#include "kg_internal.hpp"
+#include "kg_nnd.hpp"
 
 namespace kga {
 
-void build_index(Handle &, kg_index_params const &, kg_index_info *) { throw runtime_error("build not available"); }
-void reverse_index(Handle &, int) { throw runtime_error("reverse not available"); }
-void prune_index(Handle &, unsigned) { throw runtime_error("prune not available"); }
+namespace {
+
+unsigned next_pow2(unsigned x) {
+    unsigned p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+// EvaluateRecall / EvaluateAccuracy (kgraph.cpp:96-141) over the pool's
+// valid entries; both lists ascending.
+float eval_recall(float const *pool, unsigned np, float const *knn, unsigned nk) {
+    if (nk == 0) return 1.0f;
+    unsigned found = 0, p = 0, k = 0;
+    while (p < np && k < nk) {
+        if (knn[k] < pool[p])
+            ++k;
+        else if (knn[k] == pool[p]) {
+            ++found;
+            ++k;
+            ++p;
+        } else
+            throw runtime_error("distance is unstable");
+    }
+    return float(found) / nk;
+}
+
+float eval_accuracy(float const *pool, unsigned np, float const *knn, unsigned nk) {
+    unsigned m = std::min(np, nk), cnt = 0;
+    float sum = 0;
+    for (unsigned i = 0; i < m; ++i) {
+        if (knn[i] > 0) {
+            sum += std::abs(pool[i] - knn[i]) / knn[i];
+            ++cnt;
+        }
+    }
+    return cnt ? sum / cnt : 0;
+}
+
+struct NndState {
+    DevBuf<uint64_t> pool, cand, stats;
+    DevBuf<uint32_t> cnt, Mv, found, samp, nnew, nold, rnew, rold, crnew, crold, ccnt;
+    DevBuf<float> radius, radiusM, kth;
+    DevBuf<uint32_t> ctl, ccount;
+    DevBuf<float> cdist;
+};
+
+void replace_graph(Handle &h, unsigned N, DevBuf<uint32_t> &off, DevBuf<uint32_t> &ids, DevBuf<float> &dists,
+                   DevBuf<uint32_t> &M, uint64_t edges) {
+    std::swap(h.goff.p, off.p);
+    std::swap(h.goff.n, off.n);
+    std::swap(h.gids.p, ids.p);
+    std::swap(h.gids.n, ids.n);
+    std::swap(h.gdist.p, dists.p);
+    std::swap(h.gdist.n, dists.n);
+    std::swap(h.gM.p, M.p);
+    std::swap(h.gM.n, M.n);
+    h.gN = N;
+    h.edges = edges;
+    h.has_graph = true;
+    h.host_valid = false;
+}
+
+}  // namespace
+
+void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo) {
+    if (!h.has_data) throw runtime_error("no dataset");
+    kg_index_params p = params;
+    unsigned const N = h.N;
+    // parameter clamps (kgraph.cpp:893-905)
+    if (N <= p.K) throw runtime_error("K larger than dataset size");
+    if (N < p.controls) p.controls = N;
+    if (N <= p.L) p.L = N - 1;
+    if (N <= p.S) p.S = N - 1;
+    if (p.S == 0 || p.S > NND_MAX_S) throw invalid_argument("S must be in [1, 255]");
+    if (p.L == 0) throw invalid_argument("L must be >= 1");
+    if (p.K > 1024) throw invalid_argument("K must be <= 1024");
+    if (p.prune > 2) throw runtime_error("prune level not supported.");
+    if (uint64_t(N) * p.L >= 0xffffffffull) throw invalid_argument("N * L too large");
+    unsigned const R = p.R ? p.R : std::max(p.L, 2 * p.S);  // R = 0 (uncapped) gets a fixed cap
+    hipStream_t s = h.stream;
+
+    NndArgs a{};
+    a.X = h.X.p;
+    a.norms = h.norms.p;
+    a.N = N;
+    a.D = h.D;
+    a.Dp = h.Dp;
+    a.L = p.L;
+    a.K = p.K;
+    a.S = p.S;
+    a.R = R;
+    a.Lc = std::max(p.L, 2 * p.S);
+    a.nnew_init = std::min(2 * p.S, N);
+    a.C = next_pow2(std::max({2 * p.L, 4 * p.S * p.S, 256u}));
+    a.Cp2 = a.C;
+    a.Acap = (a.Lc + R + 3) & ~3u;
+    a.Bcap = a.Acap;
+    a.seed = p.seed;
+
+    std::unique_ptr<NndState> stp(new NndState);
+    NndState &st = *stp;
+    st.pool.reserve(size_t(N) * p.L);
+    st.cand.reserve(size_t(N) * a.C);
+    st.stats.reserve(NND_NSTATS);
+    for (auto *b : {&st.cnt, &st.Mv, &st.found, &st.nnew, &st.nold, &st.crnew, &st.crold, &st.ccnt}) b->reserve(N);
+    st.samp.reserve(size_t(N) * a.Lc);
+    st.rnew.reserve(size_t(N) * R);
+    st.rold.reserve(size_t(N) * R);
+    for (auto *b : {&st.radius, &st.radiusM, &st.kth}) b->reserve(N);
+    a.pool = st.pool.p;
+    a.cand = st.cand.p;
+    a.stats = st.stats.p;
+    a.cnt = st.cnt.p;
+    a.Mv = st.Mv.p;
+    a.found = st.found.p;
+    a.samp = st.samp.p;
+    a.nnew = st.nnew.p;
+    a.nold = st.nold.p;
+    a.rnew = st.rnew.p;
+    a.rold = st.rold.p;
+    a.crnew = st.crnew.p;
+    a.crold = st.crold.p;
+    a.ccnt = st.ccnt.p;
+    a.radius = st.radius.p;
+    a.radiusM = st.radiusM.p;
+    a.kth = st.kth.p;
+    KG_HIP(hipMemsetAsync(st.cand.p, 0xff, size_t(N) * a.C * 8, s));
+    KG_HIP(hipMemsetAsync(st.found.p, 0, size_t(N) * 4, s));
+
+    KG_HIP(hipEventRecord(h.ev[4], s));
+    // controls: exact K-NN of C random nodes (GenerateControl, kgraph.cpp:254-270)
+    unsigned const nc = p.controls;
+    std::vector<float> ctl_knn(size_t(nc) * p.K);
+    if (nc) {
+        std::vector<uint32_t> index(N);
+        for (unsigned i = 0; i < N; ++i) index[i] = i;
+        std::mt19937 crng(p.seed);
+        std::shuffle(index.begin(), index.end(), crng);
+        index.resize(nc);
+        st.ctl.reserve(nc);
+        st.ccount.reserve(nc);
+        st.cdist.reserve(size_t(nc) * std::max(p.L, p.K));
+        KG_HIP(hipMemcpyAsync(st.ctl.p, index.data(), 4ull * nc, hipMemcpyHostToDevice, s));
+        DevBuf<uint32_t> kid;
+        kid.reserve(size_t(nc) * p.K);
+        kg_brute_args b{};
+        b.X = h.X.p;
+        b.norms = h.norms.p;
+        b.N = N;
+        b.D = h.D;
+        b.Dp = h.Dp;
+        b.rows = st.ctl.p;
+        b.nq = nc;
+        b.K = p.K;
+        b.epsilon = __builtin_huge_valf();
+        b.out_ids = kid.p;
+        b.out_dists = st.cdist.p;
+        b.ostride = p.K;
+        KG_HIP(kg_launch_brute(h.metric, b, s));
+        KG_HIP(hipMemcpyAsync(ctl_knn.data(), st.cdist.p, 4 * ctl_knn.size(), hipMemcpyDeviceToHost, s));
+        h.sync();
+    }
+    KG_HIP(nnd_launch_init(h.metric, a, s));
+
+    kg_index_info info{};
+    info.stop_condition = 0;
+    info.recall = 0;
+    info.accuracy = 3.4028235e38f;
+    info.cost = 0;
+    info.iterations = 0;
+    info.delta = 1.0f;
+    float const total = N * float(N - 1) / 2;
+    uint64_t n_comps = 0;
+    std::vector<float> cpool(size_t(nc) * p.L);
+    std::vector<uint32_t> ccnt(nc);
+    h.trace.clear();
+    float join_ms = 0, merge_ms = 0, upd_ms = 0;
+    for (unsigned it = 0; (p.iterations == 0) || (it < p.iterations); ++it) {
+        ++info.iterations;
+        a.iter = it;
+        KG_HIP(hipMemsetAsync(st.stats.p, 0, NND_NSTATS * 8, s));
+        KG_HIP(hipEventRecord(h.ev[6], s));
+        KG_HIP(nnd_launch_join(h.metric, a, s));
+        KG_HIP(hipEventRecord(h.ev[7], s));
+        KG_HIP(nnd_launch_merge(a, s));
+        KG_HIP(hipEventRecord(h.ev[8], s));
+        uint64_t stats[NND_NSTATS];
+        KG_HIP(hipMemcpyAsync(stats, st.stats.p, sizeof(stats), hipMemcpyDeviceToHost, s));
+        if (nc) {
+            KG_HIP(nnd_launch_gather_controls(a, st.ctl.p, nc, st.cdist.p, st.ccount.p, s));
+            KG_HIP(hipMemcpyAsync(cpool.data(), st.cdist.p, 4 * cpool.size(), hipMemcpyDeviceToHost, s));
+            KG_HIP(hipMemcpyAsync(ccnt.data(), st.ccount.p, 4ull * nc, hipMemcpyDeviceToHost, s));
+        }
+        h.sync();
+        float ms;
+        KG_HIP(hipEventElapsedTime(&ms, h.ev[6], h.ev[7]));
+        join_ms += ms;
+        KG_HIP(hipEventElapsedTime(&ms, h.ev[7], h.ev[8]));
+        merge_ms += ms;
+        // statistics and stop rule (kgraph.cpp:927-972)
+        n_comps += stats[NND_STAT_PAIRS];
+        info.cost = float(n_comps) / total;
+        info.delta = float(double(stats[NND_STAT_DELTA]) / (double(N) * p.K));
+        info.M = float(double(stats[NND_STAT_M]) / N);
+        double sr = 0, sa = 0;
+        for (unsigned c = 0; c < nc; ++c) {
+            sr += eval_recall(&cpool[size_t(c) * p.L], ccnt[c], &ctl_knn[size_t(c) * p.K], p.K);
+            sa += eval_accuracy(&cpool[size_t(c) * p.L], ccnt[c], &ctl_knn[size_t(c) * p.K], p.K);
+        }
+        info.recall = nc ? float(sr / nc) : 0.f;
+        info.accuracy = nc ? float(sa / nc) : 0.f;
+        h.trace.insert(h.trace.end(), {info.recall, info.delta, info.M, info.cost});
+        if (verbosity > 0) {
+            fprintf(stderr, "iteration: %u recall: %g accuracy: %g cost: %g M: %g delta: %g\n", info.iterations,
+                    info.recall, info.accuracy, info.cost, info.M, info.delta);
+        }
+        if (info.delta <= p.delta) {
+            info.stop_condition = 1;
+            break;
+        }
+        if (info.recall >= p.recall) {
+            info.stop_condition = 2;
+            break;
+        }
+        KG_HIP(hipEventRecord(h.ev[9], s));
+        KG_HIP(nnd_launch_update(a, s));
+        KG_HIP(hipEventRecord(h.ev[10], s));
+        KG_HIP(hipEventSynchronize(h.ev[10]));
+        KG_HIP(hipEventElapsedTime(&ms, h.ev[9], h.ev[10]));
+        upd_ms += ms;
+    }
+    // graph extraction (kgraph.cpp:975-988): the valid pool prefix of each node
+    DevBuf<uint32_t> off, ids, M, tmp;
+    DevBuf<float> dists;
+    off.reserve(N + 1);
+    tmp.reserve((N + 1023) / 1024 + 2);
+    KG_HIP(kg_scan_u32(st.cnt.p, off.p, N, tmp.p, s));
+    uint32_t E = 0;
+    KG_HIP(hipMemcpyAsync(&E, off.p + N, 4, hipMemcpyDeviceToHost, s));
+    h.sync();
+    ids.reserve(std::max<uint32_t>(E, 1));
+    dists.reserve(std::max<uint32_t>(E, 1));
+    M.reserve(N);
+    KG_HIP(nnd_launch_extract(a, off.p, ids.p, dists.p, M.p, s));
+    h.sync();
+    // release the build state before reverse allocates its own
+    stp.reset();
+    replace_graph(h, N, off, ids, dists, M, E);
+    h.no_dist = false;
+    KG_HIP(hipEventRecord(h.ev[5], s));
+    h.sync();
+    h.last_ms[3] = join_ms;
+    h.last_ms[4] = merge_ms;
+    h.last_ms[5] = upd_ms;
+    if (p.reverse) reverse_index(h, p.reverse);
+    if (p.prune) prune_index(h, p.prune);
+    KG_HIP(hipEventRecord(h.ev[11], s));
+    h.sync();
+    KG_HIP(hipEventElapsedTime(&h.last_ms[2], h.ev[4], h.ev[11]));
+    if (pinfo) *pinfo = info;
+}
+
+void reverse_index(Handle &h, int rev) {
+    if (rev == 0) return;
+    if (!h.has_graph) throw runtime_error("no index");
+    if (h.no_dist) throw runtime_error("Need distance information to reverse graph");
+    if (verbosity > 0) fprintf(stderr, "Graph completion with reverse edges...\n");
+    hipStream_t s = h.stream;
+    unsigned const N = h.gN;
+    KG_HIP(hipEventRecord(h.ev[12], s));
+    GraphDev g{N, h.goff.p, h.gids.p, h.gdist.p, h.gM.p};
+    DevBuf<uint32_t> deg2, off2, cursor, tmp, cnt2, off3;
+    deg2.reserve(N + 1);
+    off2.reserve(N + 1);
+    cursor.reserve(N + 1);
+    tmp.reserve((N + 1023) / 1024 + 2);
+    KG_HIP(hipMemsetAsync(deg2.p, 0, 4ull * (N + 1), s));
+    KG_HIP(kg_reverse_count(g, rev, deg2.p, s));
+    KG_HIP(kg_scan_u32(deg2.p, off2.p, N, tmp.p, s));
+    std::vector<uint32_t> hoff(N + 1);
+    KG_HIP(hipMemcpyAsync(hoff.data(), off2.p, 4ull * (N + 1), hipMemcpyDeviceToHost, s));
+    h.sync();
+    uint64_t E2 = hoff[N];
+    if (E2 >= 0xffffffffull) throw runtime_error("index too large");
+    DevBuf<uint64_t> keys;
+    keys.reserve(std::max<uint64_t>(E2, 1));
+    KG_HIP(kg_reverse_scatter(g, rev, off2.p, cursor.p, keys.p, s));
+    // segments longer than the wave sort handles
+    std::vector<uint32_t> large;
+    std::vector<uint64_t> soff;
+    uint64_t scratch = 0;
+    for (unsigned i = 0; i < N; ++i) {
+        unsigned n = hoff[i + 1] - hoff[i];
+        if (n > 1024) {
+            large.push_back(i);
+            soff.push_back(scratch);
+            scratch += next_pow2(n);
+        }
+    }
+    DevBuf<uint32_t> dlarge;
+    DevBuf<uint64_t> dsoff, dscratch;
+    if (!large.empty()) {
+        dlarge.reserve(large.size());
+        dsoff.reserve(soff.size());
+        dscratch.reserve(scratch);
+        KG_HIP(hipMemcpyAsync(dlarge.p, large.data(), 4 * large.size(), hipMemcpyHostToDevice, s));
+        KG_HIP(hipMemcpyAsync(dsoff.p, soff.data(), 8 * soff.size(), hipMemcpyHostToDevice, s));
+    }
+    cnt2.reserve(N + 1);
+    KG_HIP(kg_segsort_unique(N, off2.p, keys.p, cnt2.p, dlarge.p, unsigned(large.size()), dscratch.p, dsoff.p, s));
+    off3.reserve(N + 1);
+    KG_HIP(kg_scan_u32(cnt2.p, off3.p, N, tmp.p, s));
+    uint32_t E3 = 0;
+    KG_HIP(hipMemcpyAsync(&E3, off3.p + N, 4, hipMemcpyDeviceToHost, s));
+    h.sync();
+    DevBuf<uint32_t> ids3, M3;
+    DevBuf<float> d3;
+    ids3.reserve(std::max<uint32_t>(E3, 1));
+    d3.reserve(std::max<uint32_t>(E3, 1));
+    M3.reserve(N);
+    KG_HIP(kg_compact_keys(N, off2.p, keys.p, off3.p, ids3.p, d3.p, M3.p, s));
+    KG_HIP(hipEventRecord(h.ev[13], s));
+    h.sync();
+    replace_graph(h, N, off3, ids3, d3, M3, E3);
+    KG_HIP(hipEventElapsedTime(&h.last_ms[6], h.ev[12], h.ev[13]));
+}
+
+void prune_index(Handle &h, unsigned level) {
+    if (!h.has_graph) throw runtime_error("no index");
+    if (level & 2) throw invalid_argument("prune level 2 is not supported on the GPU");
+    if (!(level & 1)) return;
+    // prune1 (kgraph.cpp:584-590): keep the first M[i] edges
+    hipStream_t s = h.stream;
+    unsigned const N = h.gN;
+    GraphDev g{N, h.goff.p, h.gids.p, h.gdist.p, h.gM.p};
+    DevBuf<uint32_t> cnt, off2, tmp, ids2;
+    DevBuf<float> d2;
+    cnt.reserve(N + 1);
+    off2.reserve(N + 1);
+    tmp.reserve((N + 1023) / 1024 + 2);
+    KG_HIP(kg_prune1_count(g, cnt.p, s));
+    KG_HIP(kg_scan_u32(cnt.p, off2.p, N, tmp.p, s));
+    uint32_t E = 0;
+    KG_HIP(hipMemcpyAsync(&E, off2.p + N, 4, hipMemcpyDeviceToHost, s));
+    h.sync();
+    ids2.reserve(std::max<uint32_t>(E, 1));
+    d2.reserve(std::max<uint32_t>(E, 1));
+    KG_HIP(kg_prune1_copy(g, off2.p, ids2.p, h.no_dist ? nullptr : d2.p, s));
+    h.sync();
+    DevBuf<uint32_t> M;  // M is unchanged by prune1
+    M.reserve(N);
+    KG_HIP(hipMemcpyAsync(M.p, h.gM.p, 4ull * N, hipMemcpyDeviceToDevice, s));
+    h.sync();
+    replace_graph(h, N, off2, ids2, d2, M, E);
+}
 
 }  // namespace kga

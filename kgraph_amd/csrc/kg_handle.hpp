@@ -37,6 +37,9 @@ template <typename T>
 struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
+    DevBuf() = default;
+    DevBuf(DevBuf const &) = delete;
+    DevBuf &operator=(DevBuf const &) = delete;
     void reserve(size_t want) {
         if (want <= n) return;
         release();

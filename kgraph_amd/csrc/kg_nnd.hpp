@@ -1,0 +1,74 @@
+// kg_nnd.hpp -- state of one NN-descent build on the device (shared by
+// kg_nndescent.hip and the host driver in kg_build.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#define NND_MAX_S 255  // largest sample size S
+#define NND_ACH 32     // "new" rows staged in LDS per join chunk
+
+enum { NND_STAT_PAIRS = 0, NND_STAT_DELTA = 1, NND_STAT_M = 2, NND_NSTATS = 4 };
+
+struct NndArgs {
+    float const *X;      // rows, stride Dp floats
+    float const *norms;  // per-row metric norms
+    unsigned N, D, Dp;
+    unsigned L, K, S, R;  // IndexParams (kgraph.h:94-110)
+    unsigned C, Cp2;      // candidate slots per node (and the next power of two)
+    unsigned Lc;          // sample-list capacity per node, max(L, 2S)
+    unsigned Acap, Bcap;  // join list capacities (multiples of 4)
+    unsigned nnew_init;   // size of the first iteration's random "new" list
+    unsigned long long seed;
+    unsigned iter;
+    // per-node state (Nhood, kgraph.cpp:709-721)
+    uint64_t *pool;  // N x L keys: ord(dist) << 32 | id << 1 | new-flag
+    uint32_t *cnt;   // valid pool entries (Nhood::L)
+    uint32_t *Mv;    // Nhood::M
+    uint32_t *found; // Nhood::found
+    float *radius;   // worst pool distance once full, +inf before
+    float *radiusM;  // pool[M-1].dist
+    float *kth;      // pool[K-1].dist once K entries exist, +inf before
+    uint32_t *samp;  // N x Lc: sampled new ids from the front, old ids from the back
+    uint32_t *nnew, *nold;
+    uint32_t *rnew, *rold;    // N x R reverse lists
+    uint32_t *crnew, *crold;  // reverse-list push counters
+    uint64_t *cand;           // N x C candidate slots (~0 = empty)
+    uint32_t *ccnt;           // candidate slot counters
+    uint64_t *stats;          // NND_NSTATS counters
+};
+
+size_t nnd_join_lds(NndArgs const &a);
+size_t nnd_merge_lds(NndArgs const &a);
+hipError_t nnd_launch_init(int metric, NndArgs const &a, hipStream_t s);
+hipError_t nnd_launch_join(int metric, NndArgs const &a, hipStream_t s);
+hipError_t nnd_launch_merge(NndArgs const &a, hipStream_t s);
+hipError_t nnd_launch_update(NndArgs const &a, hipStream_t s);
+hipError_t nnd_launch_extract(NndArgs const &a, uint32_t const *off, uint32_t *ids, float *dists, uint32_t *M,
+                              hipStream_t s);
+hipError_t nnd_launch_gather_controls(NndArgs const &a, uint32_t const *ctl, unsigned nc, float *dout,
+                                      uint32_t *cout, hipStream_t s);
+
+// kg_graph.hip: whole-graph kernels on the CSR index.
+// Exclusive scan of n counts into out[0..n] (out[n] = total); tmp must hold
+// (n + 1023) / 1024 + 1 entries.
+hipError_t kg_scan_u32(uint32_t const *in, uint32_t *out, unsigned n, uint32_t *tmp, hipStream_t s);
+// Reverse-edge completion (kgraph.cpp:667-702) and prune level 1
+// (kgraph.cpp:584-590).  See kg_graph.hip.
+struct GraphDev {
+    unsigned N;
+    uint32_t *off;  // N+1
+    uint32_t *ids;
+    float *dists;
+    uint32_t *M;
+};
+hipError_t kg_reverse_count(GraphDev g, int rev, uint32_t *deg2, hipStream_t s);
+hipError_t kg_reverse_scatter(GraphDev g, int rev, uint32_t const *off2, uint32_t *cursor, uint64_t *keys,
+                              hipStream_t s);
+hipError_t kg_segsort_unique(unsigned N, uint32_t const *off2, uint64_t *keys, uint32_t *cnt2,
+                             uint32_t const *large, unsigned nlarge, uint64_t *scratch, uint64_t const *scratch_off,
+                             hipStream_t s);
+hipError_t kg_compact_keys(unsigned N, uint32_t const *off2, uint64_t const *keys, uint32_t const *off3,
+                           uint32_t *ids, float *dists, uint32_t *M, hipStream_t s);
+hipError_t kg_prune1_count(GraphDev g, uint32_t *cnt, hipStream_t s);
+hipError_t kg_prune1_copy(GraphDev g, uint32_t const *off2, uint32_t *ids2, float *dists2, hipStream_t s);

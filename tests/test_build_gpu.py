@@ -1,0 +1,148 @@
+"""GPU NN-descent build: recall against exact K-NN matches the oracle's at the
+same IndexParams, every stored distance is the canonical FP32 value, list
+invariants hold, and reverse-edge completion / prune level 1 equal the
+oracle's bit-exactly on the same graph."""
+import numpy as np
+import pytest
+
+from oracle import kgo
+
+pytestmark = pytest.mark.gpu
+
+L2, EUC, ANG = 0, 1, 2
+
+
+def _knn_recall(off, d, gt_d, K):
+    return kgo.recall_at_k([d[off[i]:off[i] + K] for i in range(len(gt_d))], gt_d, K)
+
+
+def _data(kind, n, d, seed=21):
+    from kgraph_amd import datagen
+    if kind == "uniform":
+        return datagen.uniform(n, d, seed)
+    if kind == "intmix":
+        return datagen.integer_mixture(n, d, 40, seed, seed + 1)
+    return datagen.angular_mixture(n, d, 40, seed, seed + 1)
+
+
+def _gpu(X, metric):
+    import kgraph_amd
+    return kgraph_amd.KGraph(X, "angular" if metric == ANG else "euclidean", l2sqr=(metric == L2))
+
+
+@pytest.mark.parametrize("kind,metric,d,params", [
+    ("uniform", EUC, 16, dict(K=25, L=100, S=10, R=100, iterations=30)),
+    ("intmix", L2, 128, dict(K=20, L=40, S=10, R=100, iterations=15)),
+    ("angular", ANG, 96, dict(K=16, L=32, S=10, R=100, iterations=15)),
+])
+def test_build_recall_matches_oracle(kind, metric, d, params):
+    X = _data(kind, 20000, d)
+    N, K = len(X), params["K"]
+    rows = np.arange(0, N, 7)
+    gt_ids, gt_d = kgo.linear_knn(X, rows, K, metric)
+
+    ref = kgo.Index()
+    rinfo = ref.build(X, metric, controls=100, reverse=0, **params)
+    roff, rids, rd, rM = ref.export()
+    r_rec = _knn_recall(roff[rows], rd, gt_d, K)
+
+    g = _gpu(X, metric)
+    info = g.build(controls=100, reverse=0, **{k: v for k, v in params.items() if k not in ("R",)},
+                   R=params["R"])
+    off, ids, dd, M = g.graph()
+    g_rec = _knn_recall(off[rows], dd, gt_d, K)
+    print(f"{kind}: oracle recall {r_rec:.4f} ({rinfo.iterations} it), gpu recall {g_rec:.4f} ({info.iterations} it)")
+    assert g_rec >= r_rec - 0.005
+    # invariants: sorted, no self, no duplicate ids, M in [1, L]; canonical distances
+    L = params["L"]
+    for i in range(0, N, 97):
+        lst = ids[off[i]:off[i + 1]]
+        dl = dd[off[i]:off[i + 1]]
+        assert len(lst) <= L and 1 <= M[i] <= len(lst)
+        assert i not in lst and len(np.unique(lst)) == len(lst)
+        assert (np.diff(dl) >= 0).all()
+        for j, v in zip(lst[:8], dl[:8]):
+            assert v == kgo.dist(metric, X[i], X[j])
+    tr = g.build_trace()
+    assert len(tr) == info.iterations and tr[-1, 0] >= tr[0, 0]
+
+
+def test_build_stop_conditions_and_errors():
+    X = _data("uniform", 5000, 8)
+    g = _gpu(X, EUC)
+    info = g.build(K=10, L=30, iterations=50, recall=0.5, controls=50)
+    assert info.stop_condition == 2 and info.recall >= 0.5
+    info = g.build(K=10, L=30, iterations=50, delta=0.2, recall=1.01, controls=50)
+    assert info.stop_condition == 1 and info.delta <= 0.2
+    with pytest.raises(RuntimeError, match="K larger"):
+        _gpu(X[:20], EUC).build(K=25)
+
+
+@pytest.mark.parametrize("rev", [-1, 5])
+def test_reverse_matches_oracle(rev):
+    X = _data("intmix", 8000, 32)
+    g = _gpu(X, L2)
+    g.build(K=10, L=30, iterations=10, controls=20, reverse=0)
+    off, ids, d, M = g.graph()
+    ref = kgo.Index()
+    ref.import_(off, ids, d, M)
+    ref.reverse(rev)
+    g.reverse(rev)
+    got = g.graph()
+    exp = ref.export()
+    for a, b in zip(got, exp):
+        assert np.array_equal(a, b)
+
+
+def test_reverse_with_hub_segments():
+    # a star: every node lists node 0 first, so node 0 gets > 1024 reverse edges
+    N = 3000
+    rs = np.random.RandomState(1)
+    X = rs.random_sample((N, 8)).astype(np.float32)
+    lists = []
+    for i in range(N):
+        others = [0] if i else [1]
+        others += list(rs.choice(np.arange(1, N), size=5, replace=False))
+        others = [o for o in others if o != i][:5]
+        lists.append(sorted(others, key=lambda j: (kgo.dist(L2, X[i], X[j]), j)))
+    off = np.cumsum([0] + [len(l) for l in lists]).astype(np.uint64)
+    ids = np.array([j for l in lists for j in l], dtype=np.uint32)
+    d = np.array([kgo.dist(L2, X[i], X[j]) for i, l in enumerate(lists) for j in l], dtype=np.float32)
+    M = np.full(N, 5, dtype=np.uint32)
+    g = _gpu(X, L2)
+    g.set_graph(off, ids, d, M)
+    ref = kgo.Index()
+    ref.import_(off, ids, d, M)
+    ref.reverse(-1)
+    g.reverse(-1)
+    for a, b in zip(g.graph(), ref.export()):
+        assert np.array_equal(a, b)
+
+
+def test_prune1_matches_oracle():
+    X = _data("uniform", 6000, 8)
+    g = _gpu(X, L2)
+    g.build(K=10, L=30, iterations=8, controls=20)
+    off, ids, d, M = g.graph()
+    ref = kgo.Index()
+    ref.import_(off, ids, d, M)
+    ref.prune(1)
+    g.prune(1)
+    for a, b in zip(g.graph(), ref.export()):
+        assert np.array_equal(a, b)
+
+
+def test_build_then_search_end_to_end():
+    import kgraph_amd
+    X = _data("uniform", 20000, 16)
+    Q = _data("uniform", 200, 16, seed=99)
+    g = kgraph_amd.KGraph(X, "euclidean")
+    g.build(K=25, L=100, iterations=30, reverse=-1)
+    ids, d = g.search(Q, K=10, P=100, withDistance=True)
+    _, gt_d, _ = kgo.bruteforce(X, Q, 10, EUC)
+    assert kgo.recall_at_k(d, gt_d, 10) > 0.95
+    # the oracle walks the GPU-built graph to the same answer
+    ref = kgo.Index()
+    ref.import_(*g.graph())
+    oi, od, oc, _, _ = ref.search(X, Q, EUC, K=10, P=100)
+    assert np.array_equal(oi, ids) and np.array_equal(od, d)

@@ -1,0 +1,249 @@
+"""Benchmark: NN-descent build + graph-walk search on one MI355X.
+
+Workload (default): 1,000,000 x 128 float32 rows, integer-valued 1,000-centre
+Gaussian mixture (values clip(rint(|N(mu_c, 30^2)|), 0, 255), centres
+U[0,120)); squared-Euclidean distance; build K=100 L=150 S=10 R=100
+iterations=15 reverse=-1; 10,000 queries from the same mixture searched with
+K=10 and P in {50, 100, 200}.
+
+One step = one full build of the index + one search pass per P over the
+10,000 queries (queries resident in HBM).  `value` is the search QPS at the
+smallest P whose recall@10 (distance-equality, against the exact K-NN from
+the brute-force kernel) reaches 0.95; `build_s` is the build's wall time.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "build s + search QPS at recall@10≥0.95; GPU plain vs BF16+learned form"
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--nq", type=int, default=10_000)
+    ap.add_argument("--iterations", type=int, default=15)
+    ap.add_argument("--P", type=str, default="50,100,200")
+    ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-queries", type=int, default=1000)
+    ap.add_argument("--cpu-build-n", type=int, default=100_000)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: the build and search shard by node / query range across GPUs by "
+              "design only (no multi-GPU path is implemented or measured).", file=sys.stderr)
+        sys.exit(2)
+    import kgraph_amd
+    from kgraph_amd import _lib, datagen, devmem
+    from kgraph_amd._lib import IndexInfo, IndexParams, SearchParams, ptr
+
+    t0 = time.time()
+    X = datagen.integer_mixture(args.n, 128, 1000, 3, 4)
+    Q = datagen.integer_mixture(args.nq, 128, 1000, 3, 5)
+    log("data generated in %.1fs: X %s Q %s" % (time.time() - t0, X.shape, Q.shape))
+    Ps = [int(p) for p in args.P.split(",")]
+    K, D, N, nq = 10, X.shape[1], X.shape[0], Q.shape[0]
+    bp = dict(K=100, L=150, S=10, R=100, iterations=args.iterations, controls=100, seed=1998, delta=0.002,
+              recall=0.99, prune=0, reverse=-1)
+    ip = IndexParams(bp["iterations"], bp["L"], bp["K"], bp["S"], bp["R"], bp["controls"], bp["seed"],
+                     bp["delta"], bp["recall"], bp["prune"], bp["reverse"])
+
+    h = _lib.Handle()
+    _lib.lib().kg_set_verbosity(0)
+    h.call("kg_set_data", ptr(X), N, D, D, _lib.METRIC_L2SQR)
+    if args.precision == "bf16":
+        h.call("kg_set_precision", _lib.PRECISION_BF16)
+    dQ = devmem.DeviceArray.from_host(Q)
+    dI = devmem.DeviceArray((nq, K), np.uint32)
+    dD = devmem.DeviceArray((nq, K), np.float32)
+    dC = devmem.DeviceArray((nq,), np.uint32)
+    dCo = devmem.DeviceArray((nq,), np.uint32)
+    dSc = devmem.DeviceArray((nq,), np.uint32)
+
+    def one_step():
+        info = IndexInfo()
+        devmem.synchronize()
+        tb = time.perf_counter()
+        h.call("kg_build", ip, info)
+        devmem.synchronize()
+        build_s = time.perf_counter() - tb
+        res = []
+        for P in Ps:
+            sp = SearchParams(K, 0, P, 10, 1, 1e30, 1998, 0)
+            devmem.synchronize()
+            ts = time.perf_counter()
+            h.call("kg_search_device", dQ.ptr, nq, D, sp, None, None, dI.ptr, dD.ptr, K, dC.ptr, dCo.ptr, dSc.ptr)
+            devmem.synchronize()
+            wall = time.perf_counter() - ts
+            kms = ctypes_float(h, 0)
+            res.append(dict(P=P, wall_s=wall, kernel_ms=kms, ids=dI.to_host(), dists=dD.to_host(),
+                            comps=dCo.to_host(), scans=dSc.to_host()))
+        return info, build_s, res
+
+    for w in range(args.warmup):
+        log("warmup step %d" % w)
+        one_step()
+    steps = []
+    t_all = time.perf_counter()
+    for s in range(args.steps):
+        log("timed step %d" % s)
+        steps.append(one_step())
+    total_s = time.perf_counter() - t_all
+    info, build_s, res = steps[-1]
+    build_times = [st[1] for st in steps]
+    trace = None
+    nit = ctypes_uint()
+    h.call("kg_build_trace", None, 0, _lib.ctypes.byref(nit))
+    counters = np.zeros(4, dtype=np.uint64)
+    h.call("kg_build_counters", ptr(counters), 4)
+    join_ms = ctypes_float(h, 3)
+    merge_ms = ctypes_float(h, 4)
+    upd_ms = ctypes_float(h, 5)
+    rev_ms = ctypes_float(h, 6)
+    log("build: %.3fs (%d it, control recall %.4f) join %.1fms merge %.1fms update %.1fms reverse %.1fms" % (
+        build_s, info.iterations, info.recall, join_ms, merge_ms, upd_ms, rev_ms))
+
+    # exact ground truth with the brute-force kernel (not timed)
+    tg = time.time()
+    gt_i = np.zeros((nq, K), np.uint32)
+    gt_d = np.zeros((nq, K), np.float32)
+    gt_c = np.zeros(nq, np.uint32)
+    h.call("kg_bruteforce", ptr(Q), nq, D, K, 1e30, ptr(gt_i), ptr(gt_d), K, ptr(gt_c))
+    log("ground truth in %.1fs" % (time.time() - tg))
+    from oracle.kgo import recall_at_k
+    # index quality: recall@10 of the built graph's own lists vs exact (sampled rows)
+    sweep = []
+    for r in res:
+        rec = recall_at_k(r["dists"], gt_d, K)
+        qps = nq / r["wall_s"]
+        # algorithmic bytes per query: 4*D*(P + comps) rows + 4*scans ids (+ 4*P seeds)
+        byt = float((4 * D * (r["P"] + r["comps"].astype(np.float64)) + 4 * r["scans"] + 4 * r["P"]).sum())
+        sweep.append(dict(P=r["P"], qps=qps, recall=rec, kernel_ms=r["kernel_ms"], bytes=byt,
+                          mean_comps=float(r["comps"].mean()), mean_scans=float(r["scans"].mean())))
+        log("P=%d recall@10=%.4f QPS=%.0f kernel %.2fms (%.1f GB/s algorithmic)" % (
+            r["P"], rec, qps, r["kernel_ms"], byt / r["kernel_ms"] / 1e6))
+    pick = next((s for s in sweep if s["recall"] >= 0.95), sweep[-1])
+    achieved = pick["bytes"] / (pick["kernel_ms"] * 1e-3) / 1e9
+    # join kernel: algorithmic bytes = every join-list row once (4*D) + its id (4)
+    join_bytes = float(counters[1]) * (4 * D + 4)
+    join_gbs = join_bytes / (join_ms * 1e-3) / 1e9 if join_ms > 0 else 0.0
+
+    out = {
+        "metric": METRIC,
+        "value": round(pick["qps"], 1),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000 * total_s / max(1, args.steps), 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "intmix-1m-d128", "N": N, "D": D, "metric": "l2sqr",
+                   "build": bp, "queries": nq, "K": K, "P": pick["P"], "precision": args.precision},
+        "build_s": round(build_s, 4),
+        "build_s_steps": [round(b, 4) for b in build_times],
+        "build_iterations": int(info.iterations),
+        "build_control_recall": round(float(info.recall), 4),
+        "recall@10": round(pick["recall"], 4),
+        "search_sweep": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in s.items()} for s in sweep],
+        "build_kernel_ms": {"join": round(join_ms, 2), "merge": round(merge_ms, 2), "update": round(upd_ms, 2),
+                            "reverse": round(rev_ms, 2)},
+        "roofline": {"kernel": "search_kernel", "bound": "hbm", "achieved": round(achieved, 1),
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": None},
+        "roofline_join": {"kernel": "join_kernel", "bound": "hbm", "achieved": round(join_gbs, 1),
+                          "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(join_gbs / PEAK_HBM_GBS, 4),
+                          "pairs": int(counters[0]), "list_entries": int(counters[1]),
+                          "pushes": int(counters[2])},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args, X, Q, h, pick["P"], K, bp)
+    print(json.dumps(out), flush=True)
+
+
+def ctypes_float(h, which):
+    from kgraph_amd import _lib
+    v = _lib.ctypes.c_float()
+    h.call("kg_last_time_ms", which, _lib.ctypes.byref(v))
+    return v.value
+
+
+def ctypes_uint():
+    from kgraph_amd import _lib
+    return _lib.ctypes.c_uint()
+
+
+def cpu_baseline(args, X, Q, h, P, K, bp):
+    """The oracle (CPU restatement of the reference) on this host's cores:
+    search QPS over a query sample on the same GPU-built index, and the build
+    time on a row sample with the same IndexParams."""
+    from kgraph_amd import export_graph
+    from oracle import kgo
+    threads = kgo.set_threads(0)
+    off, ids, d, M = export_graph(h)
+    idx = kgo.Index()
+    idx.import_(off, ids, d, M)
+    nq = min(args.cpu_queries, len(Q))
+    t = time.perf_counter()
+    idx.search(X, Q[:nq], 0, K=K, P=P)
+    dt = time.perf_counter() - t
+    qps = nq / dt
+    log("cpu search: %d queries in %.2fs on %d threads -> %.0f QPS" % (nq, dt, threads, qps))
+    res = {"value": round(qps, 1), "unit": "queries/s", "cores": threads, "kind": "port",
+           "sample": "%d of the %d queries, same GPU-built index, K=%d P=%d" % (nq, len(Q), K, P)}
+    if args.cpu_build_n > 0:
+        n = min(args.cpu_build_n, len(X))
+        Xs = np.ascontiguousarray(X[:n])
+        b = kgo.Index()
+        t = time.perf_counter()
+        info = b.build(Xs, 0, **{k: bp[k] for k in ("K", "L", "S", "R", "iterations", "controls", "seed", "delta",
+                                                   "recall", "prune", "reverse")})
+        cpu_b = time.perf_counter() - t
+        import kgraph_amd
+        from kgraph_amd import _lib
+        g = _lib.Handle()
+        _lib.lib().kg_set_verbosity(0)
+        g.call("kg_set_data", _lib.ptr(Xs), n, Xs.shape[1], Xs.shape[1], 0)
+        ip = _lib.IndexParams(bp["iterations"], bp["L"], bp["K"], bp["S"], bp["R"], bp["controls"], bp["seed"],
+                              bp["delta"], bp["recall"], bp["prune"], bp["reverse"])
+        ginfo = _lib.IndexInfo()
+        g.call("kg_build", ip, ginfo)  # warm
+        t = time.perf_counter()
+        g.call("kg_build", ip, ginfo)
+        gpu_b = time.perf_counter() - t
+        log("cpu build of %d rows: %.2fs (%d it, recall %.3f); gpu %.3fs (%d it, recall %.3f)" % (
+            n, cpu_b, info.iterations, info.recall, gpu_b, ginfo.iterations, ginfo.recall))
+        res.update({"build_s_sample": round(cpu_b, 3), "gpu_build_s_sample": round(gpu_b, 4),
+                    "build_sample": "first %d rows, same IndexParams" % n,
+                    "build_speedup_sample": round(cpu_b / gpu_b, 1),
+                    "cpu_build_control_recall": round(float(info.recall), 4),
+                    "gpu_build_control_recall_sample": round(float(ginfo.recall), 4)})
+    return res
+
+
+if __name__ == "__main__":
+    main()

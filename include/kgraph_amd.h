@@ -158,6 +158,13 @@ int kg_last_time_ms(kg_handle *h, int which, float *ms);
  * delta, M, cost), up to cap iterations.  Returns the iteration count in *n. */
 int kg_build_trace(kg_handle *h, float *out, unsigned cap, unsigned *n);
 
+/* Work counters of the last build, summed over iterations (no counterpart in
+ * the reference): out[0] pairs scored (the reference's n_comps,
+ * kgraph.cpp:805, 814), out[1] join-list entries (sum over nodes of
+ * |new| + |old|), out[2] candidates pushed to merge slots, out[3]
+ * iterations.  n = number of entries wanted (<= 4). */
+int kg_build_counters(kg_handle *h, uint64_t *out, unsigned n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -83,6 +83,7 @@ SIGNATURES = {
     "kg_get_graph": (_I, [_P, _P, _P, _P, _P]),
     "kg_last_time_ms": (_I, [_P, _I, _P]),
     "kg_build_trace": (_I, [_P, _P, _U, _P]),
+    "kg_build_counters": (_I, [_P, _P, _U]),
 }
 
 _lib = None

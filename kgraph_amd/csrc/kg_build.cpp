@@ -185,7 +185,7 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     info.iterations = 0;
     info.delta = 1.0f;
     float const total = N * float(N - 1) / 2;
-    uint64_t n_comps = 0;
+    uint64_t n_comps = 0, n_lists = 0, n_push = 0;
     std::vector<float> cpool(size_t(nc) * p.L);
     std::vector<uint32_t> ccnt(nc);
     h.trace.clear();
@@ -214,6 +214,8 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
         merge_ms += ms;
         // statistics and stop rule (kgraph.cpp:927-972)
         n_comps += stats[NND_STAT_PAIRS];
+        n_lists += stats[NND_STAT_LISTS];
+        n_push += stats[NND_STAT_PUSHES];
         info.cost = float(n_comps) / total;
         info.delta = float(double(stats[NND_STAT_DELTA]) / (double(N) * p.K));
         info.M = float(double(stats[NND_STAT_M]) / N);
@@ -264,6 +266,10 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     h.no_dist = false;
     KG_HIP(hipEventRecord(h.ev[5], s));
     h.sync();
+    h.build_counters[0] = n_comps;
+    h.build_counters[1] = n_lists;
+    h.build_counters[2] = n_push;
+    h.build_counters[3] = info.iterations;
     h.last_ms[3] = join_ms;
     h.last_ms[4] = merge_ms;
     h.last_ms[5] = upd_ms;

@@ -652,4 +652,11 @@ int kg_build_trace(kg_handle *k, float *out, unsigned cap, unsigned *n) {
     });
 }
 
+int kg_build_counters(kg_handle *k, uint64_t *out, unsigned n) {
+    return guard(k, [&](Handle &h) {
+        if (!out) throw kga::invalid_argument("null argument");
+        for (unsigned i = 0; i < n && i < 4; ++i) out[i] = h.build_counters[i];
+    });
+}
+
 }  // extern "C"

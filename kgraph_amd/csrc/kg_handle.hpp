@@ -91,6 +91,7 @@ struct Handle {
     hipEvent_t ev[16] = {};
     float last_ms[8] = {};
     std::vector<float> trace;
+    uint64_t build_counters[4] = {};  // pairs, list entries, candidate pushes, iterations
 
     Handle();
     ~Handle();

@@ -8,7 +8,7 @@
 #define NND_MAX_S 255  // largest sample size S
 #define NND_ACH 32     // "new" rows staged in LDS per join chunk
 
-enum { NND_STAT_PAIRS = 0, NND_STAT_DELTA = 1, NND_STAT_M = 2, NND_NSTATS = 4 };
+enum { NND_STAT_PAIRS = 0, NND_STAT_DELTA = 1, NND_STAT_M = 2, NND_STAT_LISTS = 3, NND_STAT_PUSHES = 4, NND_NSTATS = 8 };
 
 struct NndArgs {
     float const *X;      // rows, stride Dp floats

@@ -182,7 +182,7 @@ __global__ void __launch_bounds__(256) join_kernel(NndArgs a) {
     float *nrmA = kthA + NND_ACH;
     float *rowsA = nrmA + NND_ACH;  // NND_ACH x Dp, 16-byte aligned (Acap, Bcap multiples of 4)
     __shared__ unsigned s_found;
-    __shared__ unsigned long long s_pairs;
+    __shared__ unsigned long long s_pairs, s_push;
 
     unsigned const nn = a.nnew[n], no = a.nold[n];
     unsigned const rn = min(a.crnew[n], a.R), ro = min(a.crold[n], a.R);
@@ -195,6 +195,7 @@ __global__ void __launch_bounds__(256) join_kernel(NndArgs a) {
     if (threadIdx.x == 0) {
         s_found = 0;
         s_pairs = 0;
+        s_push = 0;
     }
     __syncthreads();
 
@@ -203,7 +204,7 @@ __global__ void __launch_bounds__(256) join_kernel(NndArgs a) {
     unsigned const nquads = blockDim.x >> 2;
     unsigned const nch = (a.D & ~15u) >> 2;
     bool found = false;
-    unsigned long long pairs = 0;
+    unsigned long long pairs = 0, pushes = 0;
 
     for (unsigned a0 = 0; a0 < nA; a0 += NND_ACH) {
         unsigned const ach = min(nA - a0, unsigned(NND_ACH));
@@ -278,8 +279,14 @@ __global__ void __launch_bounds__(256) join_kernel(NndArgs a) {
                 float const d = finish_metric<METRIC>(acc, nrmA[ai], no_);
                 if (t == 0) {
                     ++pairs;
-                    if (d < radA[ai]) push_cand(a, av, o, d);
-                    if (d < ro_) push_cand(a, o, av, d);
+                    if (d < radA[ai]) {
+                        push_cand(a, av, o, d);
+                        ++pushes;
+                    }
+                    if (d < ro_) {
+                        push_cand(a, o, av, d);
+                        ++pushes;
+                    }
                     found |= d < kthA[ai];
                 }
             }
@@ -288,10 +295,14 @@ __global__ void __launch_bounds__(256) join_kernel(NndArgs a) {
     }
     if (found) s_found = 1;
     if (pairs) atomicAdd(&s_pairs, pairs);
+    if (pushes) atomicAdd(&s_push, pushes);
     __syncthreads();
     if (threadIdx.x == 0) {
         a.found[n] = s_found;
-        if (s_pairs) atomicAdd(reinterpret_cast<unsigned long long *>(a.stats + NND_STAT_PAIRS), s_pairs);
+        auto *st = reinterpret_cast<unsigned long long *>(a.stats);
+        if (s_pairs) atomicAdd(st + NND_STAT_PAIRS, s_pairs);
+        if (s_push) atomicAdd(st + NND_STAT_PUSHES, s_push);
+        atomicAdd(st + NND_STAT_LISTS, (unsigned long long)(nA + nB));
     }
 }
 

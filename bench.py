@@ -35,12 +35,16 @@ def log(*a):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=["intmix-1m-d128", "batched-10m-d96"], default="intmix-1m-d128")
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--nq", type=int, default=10_000)
+    ap.add_argument("--n", type=int, default=None, help="rows (default 1M, or 10M for batched-10m-d96)")
+    ap.add_argument("--nq", type=int, default=None, help="queries (default 10k, or 100k for batched)")
     ap.add_argument("--iterations", type=int, default=15)
     ap.add_argument("--P", type=str, default="50,100,200")
+    ap.add_argument("--Pe", type=str, default="10,20,50,100",
+                    help="P values for walks started from selected entry points")
+    ap.add_argument("--entry-sample", type=int, default=8192)
     ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-queries", type=int, default=1000)
@@ -50,10 +54,17 @@ def parse():
 
 def main():
     args = parse()
+    batched = args.workload == "batched-10m-d96"
+    if args.n is None:
+        args.n = 10_000_000 if batched else 1_000_000
+    if args.nq is None:
+        args.nq = 100_000 if batched else 10_000
     if args.gpus != 1:
         print("bench.py runs on one GPU: the build and search shard by node / query range across GPUs by "
               "design only (no multi-GPU path is implemented or measured).", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched-10m-d96":
+        return batched_workload(args)
     import kgraph_amd
     from kgraph_amd import _lib, datagen, devmem
     from kgraph_amd._lib import IndexInfo, IndexParams, SearchParams, ptr
@@ -62,7 +73,8 @@ def main():
     X = datagen.integer_mixture(args.n, 128, 1000, 3, 4)
     Q = datagen.integer_mixture(args.nq, 128, 1000, 3, 5)
     log("data generated in %.1fs: X %s Q %s" % (time.time() - t0, X.shape, Q.shape))
-    Ps = [int(p) for p in args.P.split(",")]
+    Ps = [int(p) for p in args.P.split(",") if p]
+    Pe = [int(p) for p in args.Pe.split(",") if p]
     K, D, N, nq = 10, X.shape[1], X.shape[0], Q.shape[0]
     bp = dict(K=100, L=150, S=10, R=100, iterations=args.iterations, controls=100, seed=1998, delta=0.002,
               recall=0.99, prune=0, reverse=-1)
@@ -81,24 +93,33 @@ def main():
     dCo = devmem.DeviceArray((nq,), np.uint32)
     dSc = devmem.DeviceArray((nq,), np.uint32)
 
+    m = K - 1  # entry points per query (SearchParams::init must stay below K)
+    sweeps = [("random", P) for P in Ps] + [("entries", P) for P in Pe]
+
     def one_step():
         info = IndexInfo()
         devmem.synchronize()
         tb = time.perf_counter()
         h.call("kg_build", ip, info)
+        h.call("kg_set_entry_sample", args.entry_sample, 1998)
         devmem.synchronize()
         build_s = time.perf_counter() - tb
         res = []
-        for P in Ps:
-            sp = SearchParams(K, 0, P, 10, 1, 1e30, 1998, 0)
+        for mode, P in sweeps:
+            sp = SearchParams(K, 0, P, 10, 1, 1e30, 1998, m if mode == "entries" else 0)
             devmem.synchronize()
             ts = time.perf_counter()
+            ems = 0.0
+            if mode == "entries":
+                h.call("kg_select_entries_device", dQ.ptr, nq, D, m, dI.ptr, K)
+                ems = ctypes_float(h, 7)
             h.call("kg_search_device", dQ.ptr, nq, D, sp, None, None, dI.ptr, dD.ptr, K, dC.ptr, dCo.ptr, dSc.ptr)
             devmem.synchronize()
             wall = time.perf_counter() - ts
             kms = ctypes_float(h, 0)
-            res.append(dict(P=P, wall_s=wall, kernel_ms=kms, ids=dI.to_host(), dists=dD.to_host(),
-                            comps=dCo.to_host(), scans=dSc.to_host()))
+            res.append(dict(mode=mode, P=P, wall_s=wall, kernel_ms=kms, entry_ms=ems, ids=dI.to_host(),
+                            dists=dD.to_host(), comps=dCo.to_host(), scans=dSc.to_host(),
+                            seeds=(m if mode == "entries" else P)))
         return info, build_s, res
 
     for w in range(args.warmup):
@@ -137,13 +158,16 @@ def main():
     for r in res:
         rec = recall_at_k(r["dists"], gt_d, K)
         qps = nq / r["wall_s"]
-        # algorithmic bytes per query: 4*D*(P + comps) rows + 4*scans ids (+ 4*P seeds)
-        byt = float((4 * D * (r["P"] + r["comps"].astype(np.float64)) + 4 * r["scans"] + 4 * r["P"]).sum())
-        sweep.append(dict(P=r["P"], qps=qps, recall=rec, kernel_ms=r["kernel_ms"], bytes=byt,
-                          mean_comps=float(r["comps"].mean()), mean_scans=float(r["scans"].mean())))
-        log("P=%d recall@10=%.4f QPS=%.0f kernel %.2fms (%.1f GB/s algorithmic)" % (
-            r["P"], rec, qps, r["kernel_ms"], byt / r["kernel_ms"] / 1e6))
-    pick = next((s for s in sweep if s["recall"] >= 0.95), sweep[-1])
+        # algorithmic bytes per query: a row (4*D) per scored node (starting
+        # points + comps), 4 bytes per neighbour id read, 4 per starting id
+        byt = float((4 * D * (r["seeds"] + r["comps"].astype(np.float64)) + 4 * r["scans"] + 4 * r["seeds"]).sum())
+        sweep.append(dict(mode=r["mode"], P=r["P"], qps=qps, recall=rec, kernel_ms=r["kernel_ms"],
+                          entry_ms=r["entry_ms"], bytes=byt, mean_comps=float(r["comps"].mean()),
+                          mean_scans=float(r["scans"].mean())))
+        log("%-7s P=%d recall@10=%.4f QPS=%.0f walk kernel %.2fms entry %.2fms (%.1f GB/s algorithmic)" % (
+            r["mode"], r["P"], rec, qps, r["kernel_ms"], r["entry_ms"], byt / r["kernel_ms"] / 1e6))
+    ok = [s for s in sweep if s["recall"] >= 0.95]
+    pick = max(ok, key=lambda s: s["qps"]) if ok else max(sweep, key=lambda s: s["recall"])
     achieved = pick["bytes"] / (pick["kernel_ms"] * 1e-3) / 1e9
     # join kernel: algorithmic bytes = every join-list row once (4*D) + its id (4)
     join_bytes = float(counters[1]) * (4 * D + 4)
@@ -163,7 +187,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "intmix-1m-d128", "N": N, "D": D, "metric": "l2sqr",
-                   "build": bp, "queries": nq, "K": K, "P": pick["P"], "precision": args.precision},
+                   "build": bp, "queries": nq, "K": K, "P": pick["P"], "start": pick["mode"],
+                   "entry_sample": args.entry_sample, "entries_per_query": m, "precision": args.precision},
         "build_s": round(build_s, 4),
         "build_s_steps": [round(b, 4) for b in build_times],
         "build_iterations": int(info.iterations),
@@ -181,7 +206,87 @@ def main():
                           "pushes": int(counters[2])},
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args, X, Q, h, pick["P"], K, bp)
+        out["cpu_baseline"] = cpu_baseline(args, X, Q, h, pick["P"], K, bp, pick["mode"])
+    print(json.dumps(out), flush=True)
+
+
+def batched_workload(args):
+    """10,000,000 x 96 L2-normalised 2,000-centre mixture (angular metric),
+    K=32 graph built on the GPU, then 100,000 queries with K in {1,10,100}
+    (ratio 5:4:1) and P in {50,100,300,500} (uniform) answered by one
+    persistent search launch.  Reports QPS, per-query p50/p99 latency (stamped
+    inside the kernel), algorithmic HBM GB/s and recall@K on a query sample."""
+    from kgraph_amd import _lib, datagen, devmem
+    from kgraph_amd._lib import IndexInfo, IndexParams, SearchParams, ptr
+    from oracle.kgo import recall_at_k
+    n, nq = args.n, args.nq
+    t0 = time.time()
+    X = datagen.angular_mixture(n, 96, 2000, 6, 9)
+    Q = datagen.angular_mixture(nq, 96, 2000, 6, 10)
+    rs = np.random.RandomState(11)
+    Ks = rs.choice([1, 10, 100], size=nq, p=[0.5, 0.4, 0.1]).astype(np.uint32)
+    Ps = rs.choice([50, 100, 300, 500], size=nq).astype(np.uint32)
+    log("data generated in %.1fs" % (time.time() - t0))
+    D = X.shape[1]
+    h = _lib.Handle()
+    _lib.lib().kg_set_verbosity(0)
+    h.call("kg_set_data", ptr(X), n, D, D, _lib.METRIC_ANGULAR)
+    ip = IndexParams(args.iterations, 64, 32, 10, 100, 100, 1998, 0.002, 0.99, 0, -1)
+    info = IndexInfo()
+    tb = time.perf_counter()
+    h.call("kg_build", ip, info)
+    build_s = time.perf_counter() - tb
+    log("build %.2fs, %d iterations, control recall %.4f" % (build_s, info.iterations, info.recall))
+    W = 100
+    dQ = devmem.DeviceArray.from_host(Q)
+    dI = devmem.DeviceArray((nq, W), np.uint32)
+    dD = devmem.DeviceArray((nq, W), np.float32)
+    dC = devmem.DeviceArray((nq,), np.uint32)
+    dCo = devmem.DeviceArray((nq,), np.uint32)
+    dSc = devmem.DeviceArray((nq,), np.uint32)
+    sp = SearchParams(10, 0, 100, 10, 1, 1e30, 1998, 0)
+    walls = []
+    for s in range(args.warmup + args.steps):
+        devmem.synchronize()
+        ts = time.perf_counter()
+        h.call("kg_search_device", dQ.ptr, nq, D, sp, ptr(Ks), ptr(Ps), dI.ptr, dD.ptr, W, dC.ptr, dCo.ptr, dSc.ptr)
+        devmem.synchronize()
+        if s >= args.warmup:
+            walls.append(time.perf_counter() - ts)
+    kms = ctypes_float(h, 0)
+    lat = np.zeros(nq, np.float32)
+    h.call("kg_search_latencies", ptr(lat), nq)
+    comps, scans, dists = dCo.to_host(), dSc.to_host(), dD.to_host()
+    wall = float(np.median(walls))
+    byt = float((4 * D * (Ps.astype(np.float64) + comps) + 4 * scans + 4 * Ps).sum())
+    # recall on a query sample against the exact scan
+    ns = min(1000, nq)
+    gt_i = np.zeros((ns, 100), np.uint32)
+    gt_d = np.zeros((ns, 100), np.float32)
+    gt_c = np.zeros(ns, np.uint32)
+    h.call("kg_bruteforce", ptr(Q[:ns]), ns, D, 100, 1e30, ptr(gt_i), ptr(gt_d), 100, ptr(gt_c))
+    rec = {}
+    for k in (1, 10, 100):
+        sel = np.nonzero(Ks[:ns] == k)[0]
+        if len(sel):
+            rec[str(k)] = round(recall_at_k(dists[sel, :k], gt_d[sel, :k], k), 4)
+    achieved = byt / (kms * 1e-3) / 1e9
+    out = {
+        "metric": "batched search QPS, mixed K/P, one persistent launch",
+        "value": round(nq / wall, 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1000, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "angmix-10m-d96-batched", "N": n, "D": D, "queries": nq,
+                   "K_mix": "1/10/100 at 5:4:1", "P_mix": "50/100/300/500 uniform",
+                   "graph": {"K": 32, "L": 64, "reverse": -1, "iterations": args.iterations}},
+        "build_s": round(build_s, 3), "build_control_recall": round(float(info.recall), 4),
+        "latency_us": {"p50": round(float(np.percentile(lat, 50)), 1),
+                       "p99": round(float(np.percentile(lat, 99)), 1)},
+        "recall_by_K_sample": rec, "kernel_ms": round(kms, 3),
+        "roofline": {"kernel": "search_kernel", "bound": "hbm", "achieved": round(achieved, 1),
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": None},
+    }
     print(json.dumps(out), flush=True)
 
 
@@ -197,7 +302,7 @@ def ctypes_uint():
     return _lib.ctypes.c_uint()
 
 
-def cpu_baseline(args, X, Q, h, P, K, bp):
+def cpu_baseline(args, X, Q, h, P, K, bp, mode):
     """The oracle (CPU restatement of the reference) on this host's cores:
     search QPS over a query sample on the same GPU-built index, and the build
     time on a row sample with the same IndexParams."""
@@ -209,12 +314,19 @@ def cpu_baseline(args, X, Q, h, P, K, bp):
     idx.import_(off, ids, d, M)
     nq = min(args.cpu_queries, len(Q))
     t = time.perf_counter()
-    idx.search(X, Q[:nq], 0, K=K, P=P)
+    if mode == "entries":
+        # same starting points: the m nearest of the same node sample, then the walk
+        samp = np.sort(kgo.genrandom(1998, 1, args.entry_sample, len(X))[0])
+        si, _, _ = kgo.bruteforce(X[samp], Q[:nq], K - 1, 0)
+        idx.search(X, Q[:nq], 0, K=K, P=P, init=K - 1, init_ids=samp[si])
+    else:
+        idx.search(X, Q[:nq], 0, K=K, P=P)
     dt = time.perf_counter() - t
     qps = nq / dt
-    log("cpu search: %d queries in %.2fs on %d threads -> %.0f QPS" % (nq, dt, threads, qps))
+    log("cpu search (%s): %d queries in %.2fs on %d threads -> %.0f QPS" % (mode, nq, dt, threads, qps))
     res = {"value": round(qps, 1), "unit": "queries/s", "cores": threads, "kind": "port",
-           "sample": "%d of the %d queries, same GPU-built index, K=%d P=%d" % (nq, len(Q), K, P)}
+           "sample": "%d of the %d queries, same GPU-built index, K=%d P=%d, %s starting points" % (
+               nq, len(Q), K, P, mode)}
     if args.cpu_build_n > 0:
         n = min(args.cpu_build_n, len(X))
         Xs = np.ascontiguousarray(X[:n])

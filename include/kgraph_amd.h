@@ -151,12 +151,32 @@ int kg_get_graph(kg_handle *h, uint64_t *offsets, unsigned *ids, float *dists, u
 /* Device time in milliseconds of the last call's phases, measured with HIP
  * events on the handle's stream: which = 0 search kernel, 1 brute-force
  * kernel, 2 whole build, 3 NN-descent join kernels (sum), 4 merge kernels
- * (sum), 5 update kernels (sum), 6 reverse. */
+ * (sum), 5 update kernels (sum), 6 reverse, 7 entry selection. */
 int kg_last_time_ms(kg_handle *h, int which, float *ms);
 
 /* Per-iteration trace of the last build: 4 floats per iteration (recall,
  * delta, M, cost), up to cap iterations.  Returns the iteration count in *n. */
 int kg_build_trace(kg_handle *h, float *out, unsigned cap, unsigned *n);
+
+/* Entry-point selection (no counterpart in the reference; it feeds the
+ * reference's own SearchParams::init starting points, kgraph.cpp:461-467).
+ * kg_set_entry_sample picks E distinct nodes (GenRandom on mt19937(seed),
+ * kgraph.cpp:44-65, sorted by id) and keeps a compact copy of their rows;
+ * E = 0 drops the sample.  kg_select_entries[_device] writes, for each query,
+ * the m nearest sampled nodes (exact FP32 distance, ties by id, m <= 16) to
+ * ids[q*ostride + 0..m); a following kg_search with params->init = m (m < K)
+ * walks from them. */
+int kg_set_entry_sample(kg_handle *h, unsigned E, unsigned seed);
+int kg_select_entries(kg_handle *h, const float *Q, unsigned nq, size_t qstride, unsigned m, unsigned *ids,
+                      size_t ostride);
+int kg_select_entries_device(kg_handle *h, const float *dQ, unsigned nq, size_t qstride, unsigned m,
+                             unsigned *d_ids, size_t ostride);
+
+/* Per-query latency of the last kg_search[_device] call in microseconds,
+ * stamped inside the search kernel from the query's first to its last
+ * instruction (walk queries only; brute-force and skipped rows read 0).
+ * Copies min(n, nq) values. */
+int kg_search_latencies(kg_handle *h, float *out_us, unsigned n);
 
 /* Work counters of the last build, summed over iterations (no counterpart in
  * the reference): out[0] pairs scored (the reference's n_comps,

@@ -91,16 +91,29 @@ class KGraph:
         return info
 
     # -- search (python-api.cpp:266-328, 390-409, defaults :440-448)
+    def set_entry_sample(self, E, seed=default_seed):
+        """Sample E nodes as entry-point candidates for search(entries=m)."""
+        self._h.call("kg_set_entry_sample", E, seed)
+
+    def select_entries(self, query, m):
+        """The m nearest sampled nodes of every query (uint32[Q, m])."""
+        q = f32c(query)
+        ids = np.zeros((q.shape[0], m), dtype=np.uint32)
+        self._h.call("kg_select_entries", ptr(q), q.shape[0], q.shape[1], m, ptr(ids), m)
+        return ids
+
     def search(self, query, K=default_K, P=default_P, M=default_M, S=default_S, T=default_T,
                threads=0, withDistance=False, blas=False, *, epsilon=default_epsilon,
-               seed=default_seed, Ks=None, Ps=None, stats=False):
+               seed=default_seed, Ks=None, Ps=None, stats=False, entries=0):
         """Returns uint32[Q, K] ids, or (ids, float32[Q, K] dists) with
         withDistance.  `threads` and `blas` are accepted and ignored (the GPU
         path has no OpenMP pool, and `blas` selects an exact scan that the
         index-less path already provides).  Ks / Ps give per-query K / P for
         one batched launch; the output then has max(Ks) columns.  Rows past a
         short result are zero (the reference leaves them uninitialised,
-        python-api.cpp:270-273)."""
+        python-api.cpp:270-273).  entries=m > 0 starts each walk from the m
+        nearest nodes of the entry sample (set_entry_sample), passed to the
+        walk as the reference's SearchParams::init starting points."""
         q = f32c(query)
         if q.ndim != 2 or q.shape[1] != self._data.shape[1]:
             raise ValueError("bad array")
@@ -114,7 +127,13 @@ class KGraph:
         comps = np.zeros(nq, dtype=np.uint32) if stats else None
         scans = np.zeros(nq, dtype=np.uint32) if stats else None
         if self._hasIndex:
-            sp = SearchParams(K, M, P, S, T, epsilon, seed, 0)
+            init = 0
+            if entries:
+                if Ks is not None or Ps is not None:
+                    raise ValueError("entries need uniform K and P")
+                self._h.call("kg_select_entries", ptr(q), nq, q.shape[1], entries, ptr(ids), width)
+                init = entries
+            sp = SearchParams(K, M, P, S, T, epsilon, seed, init)
             self._h.call("kg_search", ptr(q), nq, q.shape[1], sp, ptr(Ks_a), ptr(Ps_a), ptr(ids),
                          ptr(dists), width, ptr(counts), ptr(comps), ptr(scans))
         else:

@@ -84,6 +84,10 @@ SIGNATURES = {
     "kg_last_time_ms": (_I, [_P, _I, _P]),
     "kg_build_trace": (_I, [_P, _P, _U, _P]),
     "kg_build_counters": (_I, [_P, _P, _U]),
+    "kg_set_entry_sample": (_I, [_P, _U, _U]),
+    "kg_search_latencies": (_I, [_P, _P, _U]),
+    "kg_select_entries": (_I, [_P, _P, _U, _SZ, _U, _P, _SZ]),
+    "kg_select_entries_device": (_I, [_P, _P, _U, _SZ, _U, _P, _SZ]),
 }
 
 _lib = None

@@ -106,7 +106,8 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     a.R = R;
     a.Lc = std::max(p.L, 2 * p.S);
     a.nnew_init = std::min(2 * p.S, N);
-    a.C = next_pow2(std::max({2 * p.L, 4 * p.S * p.S, 256u}));
+    if (p.L > 768) throw invalid_argument("L must be <= 768");
+    a.C = std::min(next_pow2(std::max({2 * p.L, 4 * p.S * p.S, 256u})), 1024u);
     a.Cp2 = a.C;
     a.Acap = (a.Lc + R + 3) & ~3u;
     a.Bcap = a.Acap;

@@ -248,6 +248,10 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         a.counts = d_counts;
         a.comps = d_comps;
         a.scans = d_scans;
+        h.lat.reserve(nq);
+        KG_HIP(hipMemsetAsync(h.lat.p, 0, 4ull * nq, h.stream));
+        a.lat_us = h.lat.p;
+        h.lat_n = nq;
         a.visited = h.visited.p;
         a.vis_words = words;
         a.touched = h.touched.p;
@@ -295,6 +299,68 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
     h.sync();
     if (ngraph) KG_HIP(hipEventElapsedTime(&h.last_ms[0], h.ev[0], h.ev[1]));
     if (!brute.empty()) KG_HIP(hipEventElapsedTime(&h.last_ms[1], h.ev[2], h.ev[3]));
+}
+
+// Entry-point sample: E distinct node ids from GenRandom on mt19937(seed),
+// sorted ascending, rows copied into a compact matrix.
+static void set_entry_sample(Handle &h, unsigned E, unsigned seed) {
+    if (!h.has_data) throw runtime_error("no dataset");
+    if (E > h.N) throw invalid_argument("entry sample larger than the dataset");
+    h.entryE = 0;
+    if (E == 0) return;
+    std::vector<uint32_t> ids(E);
+    std::mt19937 rng(seed);
+    gen_random(rng, ids.data(), E, h.N);
+    std::sort(ids.begin(), ids.end());
+    h.eids.reserve(E);
+    h.erows.reserve(size_t(E) * h.Dp);
+    h.enorms.reserve(E);
+    KG_HIP(hipMemcpyAsync(h.eids.p, ids.data(), 4ull * E, hipMemcpyHostToDevice, h.stream));
+    KG_HIP(kg_launch_gather_rows(h.X.p, h.Dp, h.eids.p, E, h.erows.p, h.stream));
+    KG_HIP(kg_launch_norms(h.metric, h.erows.p, E, h.D, h.Dp, h.enorms.p, h.stream));
+    h.sync();
+    h.entryE = E;
+}
+
+// m nearest sampled nodes per query -> ids rows [0, m) (device buffers).
+static void select_entries_device(Handle &h, float const *dQ, unsigned nq, size_t qstride, unsigned m,
+                                  uint32_t *d_ids, size_t ostride) {
+    if (!h.has_data) throw runtime_error("no dataset");
+    if (h.entryE == 0) throw runtime_error("no entry sample (kg_set_entry_sample)");
+    if (m == 0 || m > 16 || m > h.entryE || m > ostride) throw invalid_argument("m must be in [1, min(16, E, ostride)]");
+    if (nq == 0) return;
+    float const *Qk = dQ;
+    size_t qs = qstride;
+    if ((qstride & 3) || (reinterpret_cast<uintptr_t>(dQ) & 15)) {
+        h.q.reserve(size_t(nq) * h.Dp);
+        KG_HIP(hipMemcpy2DAsync(h.q.p, h.Dp * 4, dQ, qstride * 4, size_t(h.D) * 4, nq, hipMemcpyDeviceToDevice,
+                                h.stream));
+        Qk = h.q.p;
+        qs = h.Dp;
+    }
+    h.qn.reserve(nq);
+    KG_HIP(kg_launch_norms(h.metric, Qk, nq, h.D, unsigned(qs), h.qn.p, h.stream));
+    h.escores.reserve(size_t(nq) * h.entryE);
+    kg_entry_args a{};
+    a.rows = h.erows.p;
+    a.enorms = h.enorms.p;
+    a.ids = h.eids.p;
+    a.E = h.entryE;
+    a.D = h.D;
+    a.Dp = h.Dp;
+    a.Q = Qk;
+    a.qnorms = h.qn.p;
+    a.nq = nq;
+    a.qstride = unsigned(qs);
+    a.m = m;
+    a.scores = h.escores.p;
+    a.out_ids = d_ids;
+    a.ostride = unsigned(ostride);
+    KG_HIP(hipEventRecord(h.ev[14], h.stream));
+    KG_HIP(kg_launch_entries(h.metric, a, h.stream));
+    KG_HIP(hipEventRecord(h.ev[15], h.stream));
+    h.sync();
+    KG_HIP(hipEventElapsedTime(&h.last_ms[7], h.ev[14], h.ev[15]));
 }
 
 // KNNGRAPH v2 (kgraph.cpp:272-274, 291-373).
@@ -649,6 +715,43 @@ int kg_build_trace(kg_handle *k, float *out, unsigned cap, unsigned *n) {
         if (n) *n = it;
         if (out)
             for (unsigned i = 0; i < std::min(it, cap) * 4; ++i) out[i] = h.trace[i];
+    });
+}
+
+int kg_set_entry_sample(kg_handle *k, unsigned E, unsigned seed) {
+    return guard(k, [&](Handle &h) { set_entry_sample(h, E, seed); });
+}
+
+int kg_select_entries_device(kg_handle *k, float const *dQ, unsigned nq, size_t qstride, unsigned m,
+                             unsigned *d_ids, size_t ostride) {
+    return guard(k, [&](Handle &h) {
+        if ((!dQ && nq) || !d_ids) throw kga::invalid_argument("null argument");
+        select_entries_device(h, dQ, nq, qstride, m, d_ids, ostride);
+    });
+}
+
+int kg_select_entries(kg_handle *k, float const *Q, unsigned nq, size_t qstride, unsigned m, unsigned *ids,
+                      size_t ostride) {
+    return guard(k, [&](Handle &h) {
+        if ((!Q && nq) || !ids) throw kga::invalid_argument("null argument");
+        if (!h.has_data) throw kga::runtime_error("no dataset");
+        if (qstride < h.D) throw kga::invalid_argument("bad stride");
+        if (nq == 0) return;
+        DevBuf<float> dq;
+        DevBuf<uint32_t> di;
+        upload_rows(h, Q, false, nq, h.D, qstride, dq);
+        di.reserve(size_t(nq) * ostride);
+        KG_HIP(hipMemcpyAsync(di.p, ids, size_t(nq) * ostride * 4, hipMemcpyHostToDevice, h.stream));
+        select_entries_device(h, dq.p, nq, h.Dp, m, di.p, ostride);
+        KG_HIP(hipMemcpy(ids, di.p, size_t(nq) * ostride * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+int kg_search_latencies(kg_handle *k, float *out_us, unsigned n) {
+    return guard(k, [&](Handle &h) {
+        if (!out_us) throw kga::invalid_argument("null argument");
+        unsigned c = std::min(n, h.lat_n);
+        if (c) KG_HIP(hipMemcpy(out_us, h.lat.p, 4ull * c, hipMemcpyDeviceToHost));
     });
 }
 

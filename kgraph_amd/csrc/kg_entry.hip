@@ -1,0 +1,124 @@
+// kg_entry.hip -- entry-point selection for graph-walk search.
+//
+// The reference draws its starting points at random (kgraph.cpp:449-460) or
+// takes them from the caller (SearchParams::init, kgraph.cpp:461-467).  On
+// clustered data whose k-NN graph has no edges between clusters, a walk from
+// random starts only succeeds when a start lands in the query's cluster.
+// These kernels pick, for every query, the m nearest nodes of a fixed sample
+// of E nodes (exact canonical FP32 distances, ties by node id); the result is
+// handed to the unchanged walk as its `init` starting points.
+//
+//   entry_dist_kernel  16 queries x 64 sampled rows per block: each quad holds
+//                      one sampled row, the queries sit in LDS
+//   entry_topm_kernel  one wave per query: per-lane sorted top-m lists in
+//                      registers, then m rounds of a wave-wide minimum
+#include "kg_device.hpp"
+#include "kg_internal.hpp"
+
+namespace {
+
+using namespace kgd;
+
+constexpr unsigned QB = 16;    // queries per block
+constexpr unsigned EB = 64;    // sampled rows per block (one per quad)
+constexpr unsigned MAXM = 16;  // largest m
+
+template <int METRIC>
+__global__ void __launch_bounds__(256) entry_dist_kernel(kg_entry_args a) {
+    extern __shared__ __align__(16) float qs[];  // QB x Dp
+    unsigned const q0 = blockIdx.x * QB, e0 = blockIdx.y * EB;
+    unsigned const nqb = min(QB, a.nq - q0);
+    for (unsigned i = threadIdx.x; i < nqb * a.Dp; i += blockDim.x) {
+        unsigned r = i / a.Dp, c = i - r * a.Dp;
+        qs[i] = (c < a.D) ? a.Q[size_t(q0 + r) * a.qstride + c] : 0.f;
+    }
+    __syncthreads();
+    unsigned const quad = threadIdx.x >> 2, t = threadIdx.x & 3;
+    unsigned const e = e0 + quad;
+    if (e >= a.E) return;  // whole quads leave together
+    float const *row = a.rows + size_t(e) * a.Dp;
+    float const ne = (METRIC == METRIC_L2SQR) ? 0.f : a.enorms[e];
+    for (unsigned r = 0; r < nqb; ++r) {
+        float acc = quad_metric<METRIC>(row, qs + size_t(r) * a.Dp, a.D, t);
+        float qn = (METRIC == METRIC_L2SQR) ? 0.f : a.qnorms[q0 + r];
+        float d = finish_metric<METRIC>(acc, ne, qn);
+        if (t == 0) a.scores[size_t(q0 + r) * a.E + e] = d;
+    }
+}
+
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+    uint32_t lo = __shfl_xor(uint32_t(v), m), hi = __shfl_xor(uint32_t(v >> 32), m);
+    return (uint64_t(hi) << 32) | lo;
+}
+
+__global__ void __launch_bounds__(256) entry_topm_kernel(kg_entry_args a) {
+    unsigned const w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    if (w >= a.nq) return;
+    uint64_t best[MAXM];
+#pragma unroll
+    for (unsigned i = 0; i < MAXM; ++i) best[i] = ~0ull;
+    float const *s = a.scores + size_t(w) * a.E;
+    for (unsigned e = lane; e < a.E; e += KG_WAVE) {
+        uint64_t key = (uint64_t(ord_f32(s[e])) << 32) | a.ids[e];
+        if (key < best[MAXM - 1]) {  // sorted insert with compile-time indices
+            bool placed = false;
+#pragma unroll
+            for (int i = MAXM - 1; i > 0; --i) {
+                if (!placed) {
+                    if (best[i - 1] > key) {
+                        best[i] = best[i - 1];
+                    } else {
+                        best[i] = key;
+                        placed = true;
+                    }
+                }
+            }
+            if (!placed) best[0] = key;
+        }
+    }
+    // m rounds: the wave-wide minimum of the list heads is the next entry
+    for (unsigned r = 0; r < a.m; ++r) {
+        uint64_t v = best[0];
+        for (int o = 32; o > 0; o >>= 1) {
+            uint64_t u = shfl_xor64(v, o);
+            v = u < v ? u : v;
+        }
+        if (best[0] == v && v != ~0ull) {  // unique keys: exactly one lane pops
+#pragma unroll
+            for (unsigned i = 0; i + 1 < MAXM; ++i) best[i] = best[i + 1];
+            best[MAXM - 1] = ~0ull;
+        }
+        if (lane == 0) a.out_ids[size_t(w) * a.ostride + r] = uint32_t(v);
+    }
+}
+
+__global__ void gather_rows_kernel(float const *X, unsigned Dp, uint32_t const *ids, unsigned E, float *out) {
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    unsigned const v4 = Dp >> 2;
+    if (i >= size_t(E) * v4) return;
+    size_t r = i / v4, c = i - r * v4;
+    reinterpret_cast<float4 *>(out)[i] = reinterpret_cast<float4 const *>(X + size_t(ids[r]) * Dp)[c];
+}
+
+}  // namespace
+
+hipError_t kg_launch_gather_rows(float const *X, unsigned Dp, uint32_t const *ids, unsigned E, float *out,
+                                 hipStream_t s) {
+    size_t n = size_t(E) * (Dp >> 2);
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, X, Dp, ids, E, out);
+    return hipGetLastError();
+}
+
+hipError_t kg_launch_entries(int metric, kg_entry_args const &a, hipStream_t s) {
+    if (a.m == 0 || a.m > MAXM || a.E == 0 || a.nq == 0) return hipErrorInvalidValue;
+    dim3 g1((a.nq + QB - 1) / QB, (a.E + EB - 1) / EB), b1(256);
+    size_t lds = size_t(QB) * a.Dp * 4;
+    switch (metric) {
+        case METRIC_L2SQR: hipLaunchKernelGGL(entry_dist_kernel<METRIC_L2SQR>, g1, b1, lds, s, a); break;
+        case METRIC_EUCLIDEAN: hipLaunchKernelGGL(entry_dist_kernel<METRIC_EUCLIDEAN>, g1, b1, lds, s, a); break;
+        default: hipLaunchKernelGGL(entry_dist_kernel<METRIC_ANGULAR>, g1, b1, lds, s, a); break;
+    }
+    hipLaunchKernelGGL(entry_topm_kernel, dim3((a.nq + 3) / 4), dim3(256), 0, s, a);
+    return hipGetLastError();
+}

@@ -86,6 +86,13 @@ struct Handle {
     DevBuf<float> q, qn, out_dists;
     DevBuf<uint32_t> out_ids, counts, comps, scans;
     unsigned vis_slots = 0, vis_words = 0;
+    DevBuf<float> lat;      // per-query latency of the last search (microseconds)
+    unsigned lat_n = 0;
+
+    // entry-point sample
+    unsigned entryE = 0;
+    DevBuf<float> erows, enorms, escores;
+    DevBuf<uint32_t> eids;
 
     // timing + build trace
     hipEvent_t ev[16] = {};

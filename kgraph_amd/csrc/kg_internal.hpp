@@ -33,6 +33,7 @@ struct kg_search_args {
     float *out_dists;
     unsigned ostride;
     uint32_t *counts, *comps, *scans;
+    float *lat_us;  // per-query wall time inside the kernel (may be NULL)
     // per-wave scratch
     uint32_t *visited;
     unsigned vis_words;
@@ -61,10 +62,27 @@ struct kg_brute_args {
     uint32_t *comps, *scans;  // optional: set to N and 0 (kgraph.cpp:428-431)
 };
 
+struct kg_entry_args {
+    float const *rows;    // E sampled rows, stride Dp
+    float const *enorms;  // their norms
+    uint32_t const *ids;  // their node ids (ascending)
+    unsigned E, D, Dp;
+    float const *Q;
+    float const *qnorms;
+    unsigned nq, qstride;
+    unsigned m;
+    float *scores;  // nq x E
+    uint32_t *out_ids;
+    unsigned ostride;
+};
+
 // Launchers (defined in the .hip files).
 size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp);
 hipError_t kg_launch_search(int metric, kg_search_args const &a, unsigned blocks, unsigned waves_per_block,
                             hipStream_t s);
 hipError_t kg_launch_brute(int metric, kg_brute_args const &a, hipStream_t s);
+hipError_t kg_launch_entries(int metric, kg_entry_args const &a, hipStream_t s);
+hipError_t kg_launch_gather_rows(float const *X, unsigned Dp, uint32_t const *ids, unsigned E, float *out,
+                                 hipStream_t s);
 hipError_t kg_launch_norms(int metric, float const *X, unsigned N, unsigned D, unsigned Dp, float *out,
                            hipStream_t s);

@@ -6,7 +6,7 @@
 #include <stdint.h>
 
 #define NND_MAX_S 255  // largest sample size S
-#define NND_ACH 32     // "new" rows staged in LDS per join chunk
+#define NND_MERGE_MAX 1024u  // pool + candidates sorted per node per merge
 
 enum { NND_STAT_PAIRS = 0, NND_STAT_DELTA = 1, NND_STAT_M = 2, NND_STAT_LISTS = 3, NND_STAT_PUSHES = 4, NND_NSTATS = 8 };
 
@@ -38,8 +38,6 @@ struct NndArgs {
     uint64_t *stats;          // NND_NSTATS counters
 };
 
-size_t nnd_join_lds(NndArgs const &a);
-size_t nnd_merge_lds(NndArgs const &a);
 hipError_t nnd_launch_init(int metric, NndArgs const &a, hipStream_t s);
 hipError_t nnd_launch_join(int metric, NndArgs const &a, hipStream_t s);
 hipError_t nnd_launch_merge(NndArgs const &a, hipStream_t s);

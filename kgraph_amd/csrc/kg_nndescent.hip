@@ -168,245 +168,255 @@ __device__ __forceinline__ void push_cand(NndArgs const &a, uint32_t target, uin
               static_cast<unsigned long long>(key));
 }
 
-// One block per node.  The "new" rows are staged in LDS in chunks of ACH;
-// each quad streams one other row (new or old) through registers and scores
-// it against every staged row.
+// One wavefront per node.  Each quad holds one "other" row (new or old) in
+// registers and scores it against every new row, which the wave's quads all
+// read through L1 at roughly the same time.  No LDS and no block barriers:
+// the average node has ~25 list entries, so latency hiding comes from many
+// resident waves rather than from staging.
 template <int METRIC, int CH>
 __global__ void __launch_bounds__(256) join_kernel(NndArgs a) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    unsigned const n = blockIdx.x;
-    uint32_t *Aid = reinterpret_cast<uint32_t *>(smem);
-    uint32_t *Bid = Aid + a.Acap;
-    float *radA = reinterpret_cast<float *>(Bid + a.Bcap);
-    float *kthA = radA + NND_ACH;
-    float *nrmA = kthA + NND_ACH;
-    float *rowsA = nrmA + NND_ACH;  // NND_ACH x Dp, 16-byte aligned (Acap, Bcap multiples of 4)
-    __shared__ unsigned s_found;
-    __shared__ unsigned long long s_pairs, s_push;
-
+    unsigned const lane = threadIdx.x & 63;
+    unsigned const n = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (n >= a.N) return;
+    unsigned const quad = lane >> 2, t = lane & 3;
     unsigned const nn = a.nnew[n], no = a.nold[n];
     unsigned const rn = min(a.crnew[n], a.R), ro = min(a.crold[n], a.R);
     unsigned const nA = nn + rn, nB = no + ro;
     uint32_t const *samp = a.samp + size_t(n) * a.Lc;
-    for (unsigned i = threadIdx.x; i < nA; i += blockDim.x)
-        Aid[i] = (i < nn) ? samp[i] : a.rnew[size_t(n) * a.R + (i - nn)];
-    for (unsigned i = threadIdx.x; i < nB; i += blockDim.x)
-        Bid[i] = (i < no) ? samp[a.Lc - 1 - i] : a.rold[size_t(n) * a.R + (i - no)];
-    if (threadIdx.x == 0) {
-        s_found = 0;
-        s_pairs = 0;
-        s_push = 0;
-    }
-    __syncthreads();
-
-    unsigned const lane = threadIdx.x & 63;
-    unsigned const quad = threadIdx.x >> 2, t = lane & 3;
-    unsigned const nquads = blockDim.x >> 2;
+    uint32_t const *rnew = a.rnew + size_t(n) * a.R;
+    uint32_t const *rold = a.rold + size_t(n) * a.R;
     unsigned const nch = (a.D & ~15u) >> 2;
     bool found = false;
-    unsigned long long pairs = 0, pushes = 0;
-
-    for (unsigned a0 = 0; a0 < nA; a0 += NND_ACH) {
-        unsigned const ach = min(nA - a0, unsigned(NND_ACH));
-        // stage rows A[a0..a0+ach)
-        unsigned const v4 = a.Dp >> 2;
-        for (unsigned e = threadIdx.x; e < ach * v4; e += blockDim.x) {
-            unsigned r = e / v4, c = e - r * v4;
-            reinterpret_cast<float4 *>(rowsA)[size_t(r) * v4 + c] =
-                reinterpret_cast<float4 const *>(a.X + size_t(Aid[a0 + r]) * a.Dp)[c];
+    unsigned pairs = 0, pushes = 0;
+    for (unsigned oi = quad; oi < nA + nB; oi += 16) {
+        bool const isB = oi >= nA;
+        uint32_t const o = isB ? ((oi - nA < no) ? samp[a.Lc - 1 - (oi - nA)] : rold[oi - nA - no])
+                               : ((oi < nn) ? samp[oi] : rnew[oi - nn]);
+        float const *orow = a.X + size_t(o) * a.Dp;
+        float4 reg[CH > 0 ? CH : 1];
+        if (CH > 0) {
+#pragma unroll
+            for (int k = 0; k < CH; ++k) {
+                unsigned kk = t + 4 * k;
+                reg[k] = (kk < nch) ? reinterpret_cast<float4 const *>(orow)[kk] : make_float4(0, 0, 0, 0);
+            }
         }
-        for (unsigned r = threadIdx.x; r < ach; r += blockDim.x) {
-            uint32_t id = Aid[a0 + r];
-            radA[r] = a.radius[id];
-            kthA[r] = a.kth[id];
-            nrmA[r] = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[id];
-        }
-        __syncthreads();
-        for (unsigned oi = quad; oi < nA + nB; oi += nquads) {
-            bool const isB = oi >= nA;
-            uint32_t const o = isB ? Bid[oi - nA] : Aid[oi];
-            float const *orow = a.X + size_t(o) * a.Dp;
-            float4 reg[CH > 0 ? CH : 1];
+        float const ro_ = a.radius[o];
+        float const no_ = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[o];
+        for (unsigned ai = 0; ai < nA; ++ai) {
+            uint32_t const av = (ai < nn) ? samp[ai] : rnew[ai - nn];
+            // new x new once per id pair (i < j by id), new x old always (kgraph.cpp:741-750)
+            bool const valid = isB ? (av != o) : (av < o);
+            if (!valid) continue;
+            float const *ra = a.X + size_t(av) * a.Dp;
+            float acc;
             if (CH > 0) {
+                float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
                 for (int k = 0; k < CH; ++k) {
                     unsigned kk = t + 4 * k;
-                    reg[k] = (kk < nch) ? reinterpret_cast<float4 const *>(orow)[kk] : make_float4(0, 0, 0, 0);
-                }
-            }
-            float const ro_ = a.radius[o];
-            float const no_ = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[o];
-            for (unsigned ai = 0; ai < ach; ++ai) {
-                uint32_t const av = Aid[a0 + ai];
-                bool const valid = isB ? (av != o) : (av < o);
-                if (!valid) continue;
-                float const *ra = rowsA + size_t(ai) * a.Dp;
-                float acc;
-                if (CH > 0) {
-                    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-                    for (int k = 0; k < CH; ++k) {
-                        unsigned kk = t + 4 * k;
-                        if (kk < nch) {
-                            float4 y = reinterpret_cast<float4 const *>(ra)[kk];
-                            float4 x = reg[k];
-                            if (METRIC == METRIC_L2SQR) {
-                                float dx = y.x - x.x, dy = y.y - x.y, dz = y.z - x.z, dw = y.w - x.w;
-                                p.x = __builtin_fmaf(dx, dx, p.x);
-                                p.y = __builtin_fmaf(dy, dy, p.y);
-                                p.z = __builtin_fmaf(dz, dz, p.z);
-                                p.w = __builtin_fmaf(dw, dw, p.w);
-                            } else {
-                                p.x = __builtin_fmaf(y.x, x.x, p.x);
-                                p.y = __builtin_fmaf(y.y, x.y, p.y);
-                                p.z = __builtin_fmaf(y.z, x.z, p.z);
-                                p.w = __builtin_fmaf(y.w, x.w, p.w);
-                            }
-                        }
-                    }
-                    acc = quad_reduce(p);
-                    for (unsigned i = a.D & ~15u; i < a.D; ++i) {
+                    if (kk < nch) {
+                        float4 y = reinterpret_cast<float4 const *>(ra)[kk];
+                        float4 x = reg[k];
                         if (METRIC == METRIC_L2SQR) {
-                            float dd = ra[i] - orow[i];
-                            acc = __builtin_fmaf(dd, dd, acc);
+                            float dx = y.x - x.x, dy = y.y - x.y, dz = y.z - x.z, dw = y.w - x.w;
+                            p.x = __builtin_fmaf(dx, dx, p.x);
+                            p.y = __builtin_fmaf(dy, dy, p.y);
+                            p.z = __builtin_fmaf(dz, dz, p.z);
+                            p.w = __builtin_fmaf(dw, dw, p.w);
                         } else {
-                            acc = __builtin_fmaf(ra[i], orow[i], acc);
+                            p.x = __builtin_fmaf(y.x, x.x, p.x);
+                            p.y = __builtin_fmaf(y.y, x.y, p.y);
+                            p.z = __builtin_fmaf(y.z, x.z, p.z);
+                            p.w = __builtin_fmaf(y.w, x.w, p.w);
                         }
                     }
-                } else {
-                    acc = quad_metric<METRIC>(ra, orow, a.D, t);
                 }
-                float const d = finish_metric<METRIC>(acc, nrmA[ai], no_);
-                if (t == 0) {
-                    ++pairs;
-                    if (d < radA[ai]) {
-                        push_cand(a, av, o, d);
-                        ++pushes;
+                acc = quad_reduce(p);
+                for (unsigned i = a.D & ~15u; i < a.D; ++i) {
+                    if (METRIC == METRIC_L2SQR) {
+                        float dd = ra[i] - orow[i];
+                        acc = __builtin_fmaf(dd, dd, acc);
+                    } else {
+                        acc = __builtin_fmaf(ra[i], orow[i], acc);
                     }
-                    if (d < ro_) {
-                        push_cand(a, o, av, d);
-                        ++pushes;
-                    }
-                    found |= d < kthA[ai];
                 }
+            } else {
+                acc = quad_metric<METRIC>(ra, orow, a.D, t);
+            }
+            float const na = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[av];
+            float const d = finish_metric<METRIC>(acc, na, no_);
+            if (t == 0) {
+                ++pairs;
+                if (d < a.radius[av]) {
+                    push_cand(a, av, o, d);
+                    ++pushes;
+                }
+                if (d < ro_) {
+                    push_cand(a, o, av, d);
+                    ++pushes;
+                }
+                // found: the first endpoint's insert lands in its top K (kgraph.cpp:807-812)
+                found |= d < a.kth[av];
             }
         }
-        __syncthreads();
     }
-    if (found) s_found = 1;
-    if (pairs) atomicAdd(&s_pairs, pairs);
-    if (pushes) atomicAdd(&s_push, pushes);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        a.found[n] = s_found;
+    uint64_t const fb = ballot(found);
+    // wave sums of the counters (lanes t != 0 hold zeros)
+    for (int o = 32; o > 0; o >>= 1) {
+        pairs += __shfl_xor(pairs, o);
+        pushes += __shfl_xor(pushes, o);
+    }
+    if (lane == 0) {
+        a.found[n] = fb ? 1u : 0u;
         auto *st = reinterpret_cast<unsigned long long *>(a.stats);
-        if (s_pairs) atomicAdd(st + NND_STAT_PAIRS, s_pairs);
-        if (s_push) atomicAdd(st + NND_STAT_PUSHES, s_push);
+        if (pairs) atomicAdd(st + NND_STAT_PAIRS, (unsigned long long)pairs);
+        if (pushes) atomicAdd(st + NND_STAT_PUSHES, (unsigned long long)pushes);
         atomicAdd(st + NND_STAT_LISTS, (unsigned long long)(nA + nB));
     }
 }
 
 // ------------------------------------------------------------------ merge
-// Wave-level bitonic sort of v[0..n), n a power of two.
-__device__ void wave_bitonic(uint64_t *v, unsigned n) {
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+    uint32_t lo = __shfl_xor(uint32_t(v), m), hi = __shfl_xor(uint32_t(v >> 32), m);
+    return (uint64_t(hi) << 32) | lo;
+}
+__device__ __forceinline__ uint64_t shfl_up64(uint64_t v, int d) {
+    uint32_t lo = __shfl_up(uint32_t(v), d), hi = __shfl_up(uint32_t(v >> 32), d);
+    return (uint64_t(hi) << 32) | lo;
+}
+
+// Bitonic sort of 64*PER keys held in registers, blocked layout: element
+// i = lane*PER + r.  Stages whose partner distance is below PER stay in the
+// lane; the others exchange with lane ^ (j/PER) through shuffles.  All loops
+// have compile-time trip counts, so every register index is a constant.
+template <int PER>
+__device__ __forceinline__ void wave_sort_regs(uint64_t (&v)[PER]) {
     unsigned const lane = lane_id();
-    for (unsigned k = 2; k <= n; k <<= 1) {
-        for (unsigned j = k >> 1; j > 0; j >>= 1) {
-            for (unsigned i = lane; i < n; i += KG_WAVE) {
-                unsigned ixj = i ^ j;
-                if (ixj > i) {
-                    uint64_t x = v[i], y = v[ixj];
-                    bool up = (i & k) == 0;
-                    if ((x > y) == up) {
-                        v[i] = y;
-                        v[ixj] = x;
+    constexpr int LOGN = (PER == 1) ? 6 : (PER == 2) ? 7 : (PER == 4) ? 8 : (PER == 8) ? 9 : 10;
+#pragma unroll
+    for (int kk = 1; kk <= LOGN; ++kk) {
+        unsigned const k = 1u << kk;
+#pragma unroll
+        for (int jj = kk - 1; jj >= 0; --jj) {
+            unsigned const j = 1u << jj;
+            if (j >= unsigned(PER)) {
+                unsigned const lm = j / PER;
+                bool const lower = (lane & lm) == 0;
+#pragma unroll
+                for (int r = 0; r < PER; ++r) {
+                    unsigned const i = lane * PER + r;
+                    uint64_t const other = shfl_xor64(v[r], int(lm));
+                    bool const up = (i & k) == 0;
+                    bool const keep_min = (lower == up);
+                    uint64_t const mn = other < v[r] ? other : v[r];
+                    uint64_t const mx = other < v[r] ? v[r] : other;
+                    v[r] = keep_min ? mn : mx;
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < PER; ++r) {
+                    int const p = r ^ int(j);
+                    if (r < p) {
+                        unsigned const i = lane * PER + r;
+                        bool const up = (i & k) == 0;
+                        uint64_t const x = v[r], y = v[p];
+                        bool const sw = (x > y) == up;
+                        v[r] = sw ? y : x;
+                        v[p] = sw ? x : y;
                     }
                 }
             }
-            wave_sync();
         }
     }
 }
 
-// number of entries of sorted v[0..n) that are < key (strict) or <= key
-__device__ __forceinline__ unsigned lower_bound64(uint64_t const *v, unsigned n, uint64_t key, bool inclusive) {
-    unsigned lo = 0, hi = n;
-    while (lo < hi) {
-        unsigned mid = (lo + hi) >> 1;
-        bool left = inclusive ? (v[mid] <= key) : (v[mid] < key);
-        if (left)
-            lo = mid + 1;
-        else
-            hi = mid;
+// Sorted pool U candidates -> first L distinct ids, all in registers.
+template <int PER>
+__device__ void merge_node(NndArgs const &a, unsigned n, unsigned c, unsigned m, unsigned mall) {
+    unsigned const lane = lane_id();
+    uint64_t *pool = a.pool + size_t(n) * a.L;
+    uint64_t *cand = a.cand + size_t(n) * a.C;
+    uint64_t v[PER];
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        unsigned const i = lane * PER + r;
+        v[r] = (i < c) ? pool[i] : (i < c + m) ? cand[i - c] : ~0ull;
     }
-    return lo;
+    for (unsigned i = lane; i < mall; i += KG_WAVE) cand[i] = ~0ull;  // ready for the next join
+    wave_sort_regs<PER>(v);
+    // keep the first of every run of equal (dist, id); an existing pool entry
+    // (flag 0) sorts before an equal candidate (flag 1) and so keeps its flag
+    uint64_t const prev_last = shfl_up64(v[PER - 1], 1);
+    unsigned keep_mask = 0, nk = 0;
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        uint64_t const prev = (r > 0) ? v[r - 1] : prev_last;
+        bool const first = (lane == 0 && r == 0);
+        bool const keep = v[r] != ~0ull && (first || (prev >> 1) != (v[r] >> 1));
+        keep_mask |= unsigned(keep) << r;
+        nk += keep;
+    }
+    // exclusive prefix of the per-lane counts
+    unsigned incl = nk;
+    for (int d = 1; d < 64; d <<= 1) {
+        unsigned y = __shfl_up(incl, d);
+        if (int(lane) >= d) incl += y;
+    }
+    unsigned const total = __shfl(incl, 63);
+    unsigned pos = incl - nk;
+    unsigned const newc = min(total, a.L);
+    unsigned flags = 0;
+    float rad = 0.f, kth = 0.f;
+    bool has_rad = false, has_kth = false;
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        if (keep_mask & (1u << r)) {
+            if (pos < a.L) pool[pos] = v[r];
+            if (pos < a.K) flags += key_flag(v[r]);
+            if (pos == a.L - 1) {
+                rad = key_dist(v[r]);
+                has_rad = true;
+            }
+            if (pos == a.K - 1) {
+                kth = key_dist(v[r]);
+                has_kth = true;
+            }
+            ++pos;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) flags += __shfl_xor(flags, o);
+    uint64_t const br = ballot(has_rad), bk = ballot(has_kth);
+    float const rad_w = br ? __shfl(rad, int(ctz64(br))) : __builtin_huge_valf();
+    float const kth_w = bk ? __shfl(kth, int(ctz64(bk))) : __builtin_huge_valf();
+    if (lane == 0) {
+        a.cnt[n] = newc;
+        a.ccnt[n] = 0;
+        a.radius[n] = (newc >= a.L) ? rad_w : __builtin_huge_valf();
+        a.kth[n] = (newc >= a.K) ? kth_w : __builtin_huge_valf();
+        auto *st = reinterpret_cast<unsigned long long *>(a.stats);
+        atomicAdd(st + NND_STAT_DELTA, (unsigned long long)flags);
+        atomicAdd(st + NND_STAT_M, (unsigned long long)a.Mv[n]);
+    }
 }
 
 __global__ void __launch_bounds__(256) merge_kernel(NndArgs a) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    unsigned const w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    unsigned const n = blockIdx.x * 4 + w;
-    size_t const per_wave = size_t(a.Cp2 + a.L + a.L + a.Cp2) * 8;
-    uint64_t *cb = reinterpret_cast<uint64_t *>(smem + w * per_wave);  // candidates, then the new pool
-    uint64_t *pk = cb + a.Cp2;                                          // old pool
-    uint64_t *mg = pk + a.L;                                            // merged sequence
+    unsigned const n = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (n >= a.N) return;
     unsigned const c = a.cnt[n];
-    unsigned const m = min(a.ccnt[n], a.C);
-    uint64_t *pool = a.pool + size_t(n) * a.L;
-    unsigned newc = c;
-    if (m > 0) {
-        unsigned p2 = 1;
-        while (p2 < m) p2 <<= 1;
-        uint64_t *cand = a.cand + size_t(n) * a.C;
-        for (unsigned i = lane; i < p2; i += KG_WAVE) {
-            cb[i] = (i < m) ? cand[i] : ~0ull;
-            if (i < m) cand[i] = ~0ull;  // ready for the next join
-        }
-        for (unsigned i = lane; i < c; i += KG_WAVE) pk[i] = pool[i];
-        wave_sync();
-        wave_bitonic(cb, p2);
-        // stable merge: a pool entry goes before equal candidates
-        for (unsigned i = lane; i < c; i += KG_WAVE) mg[i + lower_bound64(cb, m, pk[i], false)] = pk[i];
-        for (unsigned j = lane; j < m; j += KG_WAVE) mg[j + lower_bound64(pk, c, cb[j], true)] = cb[j];
-        wave_sync();
-        // drop repeated (dist, id), keep the first L (written to HBM and to cb)
-        unsigned const tot = c + m;
-        unsigned out = 0;
-        for (unsigned b = 0; b < tot && out < a.L; b += KG_WAVE) {
-            unsigned i = b + lane;
-            uint64_t k = (i < tot) ? mg[i] : ~0ull;
-            bool keep = (i < tot) && (i == 0 || (mg[i - 1] >> 1) != (k >> 1));
-            uint64_t bal = ballot(keep);
-            unsigned pos = out + popc64(bal & ((1ull << lane) - 1));
-            if (keep && pos < a.L) {
-                pool[pos] = k;
-                cb[pos] = k;
-            }
-            out += popc64(bal);
-        }
-        newc = min(out, a.L);
-        if (lane == 0) {
-            a.cnt[n] = newc;
-            a.ccnt[n] = 0;
-        }
-    } else {
-        for (unsigned i = lane; i < c; i += KG_WAVE) cb[i] = pool[i];
-    }
-    wave_sync();
-    // radius / K-th distance snapshots and the delta statistic (kgraph.cpp:148-156)
-    unsigned const kk = min(a.K, newc);
-    unsigned flags = 0;
-    for (unsigned b = 0; b < kk; b += KG_WAVE) {
-        unsigned i = b + lane;
-        flags += popc64(ballot(i < kk && key_flag(cb[i])));
-    }
-    if (lane == 0) {
-        a.radius[n] = (newc >= a.L) ? key_dist(cb[a.L - 1]) : __builtin_huge_valf();
-        a.kth[n] = (newc >= a.K) ? key_dist(cb[a.K - 1]) : __builtin_huge_valf();
-        atomicAdd(reinterpret_cast<unsigned long long *>(a.stats + NND_STAT_DELTA), (unsigned long long)flags);
-        atomicAdd(reinterpret_cast<unsigned long long *>(a.stats + NND_STAT_M), (unsigned long long)a.Mv[n]);
-    }
+    unsigned const mall = min(a.ccnt[n], a.C);
+    unsigned const m = min(mall, NND_MERGE_MAX - c);  // the register sort holds 1024 keys
+    unsigned const tot = c + m;
+    if (tot <= 64)
+        merge_node<1>(a, n, c, m, mall);
+    else if (tot <= 128)
+        merge_node<2>(a, n, c, m, mall);
+    else if (tot <= 256)
+        merge_node<4>(a, n, c, m, mall);
+    else if (tot <= 512)
+        merge_node<8>(a, n, c, m, mall);
+    else
+        merge_node<16>(a, n, c, m, mall);
 }
 
 // ------------------------------------------------------------------ update
@@ -530,9 +540,6 @@ static int ch_for(unsigned D) {
     return 0;
 }
 
-size_t nnd_join_lds(NndArgs const &a) { return size_t(a.Acap + a.Bcap + 3 * NND_ACH) * 4 + size_t(NND_ACH) * a.Dp * 4; }
-size_t nnd_merge_lds(NndArgs const &a) { return size_t(4) * (a.Cp2 + a.L + a.L + a.Cp2) * 8; }
-
 #define KG_DISPATCH_METRIC(metric, KERNEL, ...)                                       \
     switch (metric) {                                                                \
         case METRIC_L2SQR: hipLaunchKernelGGL(KERNEL<METRIC_L2SQR>, __VA_ARGS__); break; \
@@ -548,8 +555,8 @@ hipError_t nnd_launch_init(int metric, NndArgs const &a, hipStream_t s) {
 
 template <int METRIC>
 static void launch_join_m(NndArgs const &a, hipStream_t s) {
-    dim3 g(a.N), b(256);
-    size_t lds = nnd_join_lds(a);
+    dim3 g((a.N + 3) / 4), b(256);
+    size_t lds = 0;
     switch (ch_for(a.D)) {
         case 1: hipLaunchKernelGGL((join_kernel<METRIC, 1>), g, b, lds, s, a); break;
         case 2: hipLaunchKernelGGL((join_kernel<METRIC, 2>), g, b, lds, s, a); break;
@@ -571,7 +578,7 @@ hipError_t nnd_launch_join(int metric, NndArgs const &a, hipStream_t s) {
 }
 
 hipError_t nnd_launch_merge(NndArgs const &a, hipStream_t s) {
-    hipLaunchKernelGGL(merge_kernel, dim3((a.N + 3) / 4), dim3(256), nnd_merge_lds(a), s, a);
+    hipLaunchKernelGGL(merge_kernel, dim3((a.N + 3) / 4), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -173,6 +173,7 @@ __global__ void __launch_bounds__(256) search_kernel(kg_search_args a) {
         if (lane == 0) qi = atomicAdd(a.qcounter, 1u);
         qi = uniform(__shfl(qi, 0));
         if (qi >= a.nq) break;
+        uint64_t const t_start = __builtin_amdgcn_s_memrealtime();  // 100 MHz constant clock
         unsigned const K = a.Ks ? a.Ks[qi] : a.K0;
         unsigned const P = a.Ps ? a.Ps[qi] : a.P0;
         if (P >= a.N) continue;  // served by the brute-force launch (kgraph.cpp:427-433)
@@ -292,10 +293,10 @@ __global__ void __launch_bounds__(256) search_kernel(kg_search_args a) {
                     ntouch += nf;
                     wave_sync();
                     if (nf == 0) continue;
-                    batch_dist<METRIC>(a, w.fid, w.fdist, nf, w.q, qn);
-                    // lane j prepares candidate j's M and base offset
+                    // lane j fetches candidate j's M and CSR base while the rows load
                     uint32_t cm = 0, cb = 0;
                     if (lane < nf) cm = actual_M(a, w.fid[lane], &cb);
+                    batch_dist<METRIC>(a, w.fid, w.fdist, nf, w.q, qn);
                     for (unsigned j = 0; j < nf; ++j) {
                         ++n_comps;
                         uint32_t const cid = w.fid[j];
@@ -380,6 +381,7 @@ __global__ void __launch_bounds__(256) search_kernel(kg_search_args a) {
             a.counts[qi] = cut;
             if (a.comps) a.comps[qi] = n_comps;
             if (a.scans) a.scans[qi] = n_scan;
+            if (a.lat_us) a.lat_us[qi] = float(__builtin_amdgcn_s_memrealtime() - t_start) * 0.01f;
         }
         // ---- clear the visited bits this query set
         if (ntouch <= a.touch_cap) {

@@ -149,3 +149,46 @@ def test_save_load_roundtrip_then_search(tmp_path):
     gi = g.search(Q, K=10, P=100)
     ids, d, c, _, _ = idx.search(X, Q, EUC, K=10, P=100)
     assert np.array_equal(gi, ids)
+
+
+def _entry_sample_ids(seed, E, N):
+    return np.sort(kgo.genrandom(seed, 1, E, N)[0])
+
+
+@pytest.mark.parametrize("metric", [L2, EUC, ANG])
+def test_entry_selection_parity(metric):
+    kind = {L2: "intmix", EUC: "uniform", ANG: "angular"}[metric]
+    X, Q, idx = setup(kind, metric, d={L2: 128, EUC: 16, ANG: 96}[metric])
+    g = gpu_index(X, idx, metric)
+    E, m = 512, 7
+    g.set_entry_sample(E, seed=77)
+    got = g.select_entries(Q, m)
+    samp = _entry_sample_ids(77, E, len(X))
+    si, sd, _ = kgo.bruteforce(X[samp], Q, m, metric)
+    assert np.array_equal(got, samp[si])
+    # the walk from those starting points equals the oracle's (kgraph.cpp:461-467)
+    gi, gd = g.search(Q, K=10, P=40, entries=m, withDistance=True, stats=True)
+    oi, od, oc, ocomp, oscan = idx.search(X, Q, metric, K=10, P=40, init=m, init_ids=samp[si])
+    assert np.array_equal(g.last_counts, oc)
+    assert np.array_equal(gi, oi) and np.array_equal(gd, od)
+    assert np.array_equal(g.last_comps, ocomp) and np.array_equal(g.last_scans, oscan)
+
+
+def test_entry_selection_lifts_recall_on_separated_clusters():
+    # 200 well separated clusters: the k-NN graph has no inter-cluster edges, so
+    # random starts reach a query's cluster with probability ~P/200, while the
+    # nearest sampled nodes lie in it.
+    from kgraph_amd import datagen
+    X = datagen.integer_mixture(20000, 64, 200, 5, 6, sigma=6.0, center_hi=250.0)
+    Q = datagen.integer_mixture(300, 64, 200, 5, 7, sigma=6.0, center_hi=250.0)
+    import kgraph_amd
+    g = kgraph_amd.KGraph(X, "euclidean", l2sqr=True)
+    g.build(K=20, L=40, iterations=12, controls=50, reverse=-1)
+    _, gt_d, _ = kgo.bruteforce(X, Q, 10, L2)
+    _, d_plain = g.search(Q, K=10, P=20, withDistance=True)
+    g.set_entry_sample(2000, seed=3)
+    _, d_ent = g.search(Q, K=10, P=20, entries=9, withDistance=True)
+    r_plain = kgo.recall_at_k(d_plain, gt_d, 10)
+    r_ent = kgo.recall_at_k(d_ent, gt_d, 10)
+    print("recall plain %.3f entries %.3f" % (r_plain, r_ent))
+    assert r_plain < 0.5 and r_ent > 0.95

@@ -172,6 +172,30 @@ int kg_select_entries(kg_handle *h, const float *Q, unsigned nq, size_t qstride,
 int kg_select_entries_device(kg_handle *h, const float *dQ, unsigned nq, size_t qstride, unsigned m,
                              unsigned *d_ids, size_t ostride);
 
+/* Learned projector (no counterpart in the reference): an MLP
+ * f(x) = W2 relu(W1 x + b1) + b2, D -> hidden -> out_dim, FP32 master
+ * weights, products on BF16 MFMA with FP32 accumulation.
+ * kg_mlp_init: Xavier-uniform weights from mt19937(seed), zero biases.
+ * kg_mlp_train: Adam on the triplet loss max(0, |f(a)-f(p)|^2 - |f(a)-f(n)|^2
+ *   + margin) over `steps` batches of `batch` triplets (a random node, one of
+ *   its first M graph neighbours, a random node) drawn from the current index;
+ *   loss_trace (may be NULL) receives the mean loss of every step.
+ * kg_mlp_loss_grad: loss and gradients for caller-given triplets
+ *   (3*nt node ids), for checking the backward kernels.
+ * kg_mlp_project: f(X) for host rows X (n x out_dim floats out).
+ * kg_set_entry_projection: with shortlist s > 0, kg_select_entries scores the
+ *   entry sample in the projected space, keeps the s best and re-ranks them
+ *   with the exact FP32 distance (m <= s <= 16); 0 restores exact scoring. */
+int kg_mlp_init(kg_handle *h, unsigned hidden, unsigned out_dim, unsigned seed);
+int kg_mlp_train(kg_handle *h, unsigned steps, unsigned batch, float lr, float margin, unsigned seed,
+                 float *loss_trace);
+int kg_mlp_loss_grad(kg_handle *h, const unsigned *triplets, unsigned nt, float margin, float *loss, float *dW1,
+                     float *db1, float *dW2, float *db2);
+int kg_mlp_get_weights(kg_handle *h, float *W1, float *b1, float *W2, float *b2);
+int kg_mlp_set_weights(kg_handle *h, const float *W1, const float *b1, const float *W2, const float *b2);
+int kg_mlp_project(kg_handle *h, const float *X, unsigned n, size_t stride, float *Z);
+int kg_set_entry_projection(kg_handle *h, unsigned shortlist);
+
 /* Per-query latency of the last kg_search[_device] call in microseconds,
  * stamped inside the search kernel from the query's first to its last
  * instruction (walk queries only; brute-force and skipped rows read 0).

@@ -174,6 +174,59 @@ class KGraph:
     def prune(self, level):
         self._h.call("kg_prune", level)
 
+    # -- learned projector (no counterpart in the reference)
+    def train_projector(self, steps=500, batch=4096, hidden=256, out=32, lr=2e-3, margin=1.0, seed=1):
+        """Train the D -> hidden -> out MLP on triplets from the current index;
+        returns the per-step mean loss."""
+        self._h.call("kg_mlp_init", hidden, out, seed)
+        trace = np.zeros(max(steps, 1), dtype=np.float32)
+        self._h.call("kg_mlp_train", steps, batch, lr, margin, seed, ptr(trace))
+        self._proj_shape = (hidden, out)
+        return trace[:steps]
+
+    def set_entry_projection(self, shortlist):
+        """shortlist > 0: score the entry sample in the projected space and
+        re-rank the best `shortlist` exactly; 0: exact scoring."""
+        self._h.call("kg_set_entry_projection", shortlist)
+        self._shortlist = shortlist
+
+    def validate_entry_projection(self, probe, K=10, P=20, entries=9, shortlist=16, tol=0.005):
+        """Probe-set guard: compare recall@K of walks started from exactly and
+        from projector-selected entry points on `probe` queries; keep the
+        projector only if it loses at most `tol`, else fall back to exact
+        scoring.  Returns {'recall_exact', 'recall_learned', 'enabled'}."""
+        q = f32c(probe)
+        nq = q.shape[0]
+        gi = np.zeros((nq, K), np.uint32)
+        gd = np.zeros((nq, K), np.float32)
+        gc = np.zeros(nq, np.uint32)
+        self._h.call("kg_bruteforce", ptr(q), nq, q.shape[1], K, default_epsilon, ptr(gi), ptr(gd), K, ptr(gc))
+
+        def recall(d):
+            hit = 0
+            for r, g in zip(np.sort(d, 1), np.sort(gd, 1)):
+                i = j = 0
+                while i < K and j < K:
+                    if g[j] < r[i]:
+                        j += 1
+                    elif g[j] == r[i]:
+                        hit += 1
+                        i += 1
+                        j += 1
+                    else:
+                        raise RuntimeError("distance is unstable")
+            return hit / float(K * nq)
+
+        self.set_entry_projection(0)
+        _, d0 = self.search(q, K=K, P=P, entries=entries, withDistance=True)
+        self.set_entry_projection(shortlist)
+        _, d1 = self.search(q, K=K, P=P, entries=entries, withDistance=True)
+        r0, r1 = recall(d0), recall(d1)
+        ok = r1 >= r0 - tol
+        if not ok:
+            self.set_entry_projection(0)
+        return {"recall_exact": r0, "recall_learned": r1, "enabled": ok}
+
     def last_time_ms(self, which=0):
         v = _lib.ctypes.c_float()
         self._h.call("kg_last_time_ms", which, _lib.ctypes.byref(v))

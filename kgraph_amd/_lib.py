@@ -86,6 +86,13 @@ SIGNATURES = {
     "kg_build_counters": (_I, [_P, _P, _U]),
     "kg_set_entry_sample": (_I, [_P, _U, _U]),
     "kg_search_latencies": (_I, [_P, _P, _U]),
+    "kg_mlp_init": (_I, [_P, _U, _U, _U]),
+    "kg_mlp_train": (_I, [_P, _U, _U, ctypes.c_float, ctypes.c_float, _U, _P]),
+    "kg_mlp_loss_grad": (_I, [_P, _P, _U, ctypes.c_float, _P, _P, _P, _P, _P]),
+    "kg_mlp_get_weights": (_I, [_P, _P, _P, _P, _P]),
+    "kg_mlp_set_weights": (_I, [_P, _P, _P, _P, _P]),
+    "kg_mlp_project": (_I, [_P, _P, _U, _SZ, _P]),
+    "kg_set_entry_projection": (_I, [_P, _U]),
     "kg_select_entries": (_I, [_P, _P, _U, _SZ, _U, _P, _SZ]),
     "kg_select_entries_device": (_I, [_P, _P, _U, _SZ, _U, _P, _SZ]),
 }

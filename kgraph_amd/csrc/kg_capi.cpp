@@ -320,6 +320,7 @@ static void set_entry_sample(Handle &h, unsigned E, unsigned seed) {
     KG_HIP(kg_launch_norms(h.metric, h.erows.p, E, h.D, h.Dp, h.enorms.p, h.stream));
     h.sync();
     h.entryE = E;
+    h.eproj_version = ~0u;
 }
 
 // m nearest sampled nodes per query -> ids rows [0, m) (device buffers).
@@ -342,22 +343,50 @@ static void select_entries_device(Handle &h, float const *dQ, unsigned nq, size_
     KG_HIP(kg_launch_norms(h.metric, Qk, nq, h.D, unsigned(qs), h.qn.p, h.stream));
     h.escores.reserve(size_t(nq) * h.entryE);
     kg_entry_args a{};
-    a.rows = h.erows.p;
+    bool const learned = h.entry_shortlist > 0;
+    unsigned const sl = std::max(m, std::min(h.entry_shortlist, std::min(16u, h.entryE)));
+    if (learned) {  // score in the projected space, then re-rank the shortlist exactly
+        mlp_project_entries(h);
+        h.zq.reserve(size_t(nq) * h.mlpP);
+        h.hq.reserve(std::max(size_t(nq), size_t(h.entryE)) * h.mlpH);
+        h.eshort.reserve(size_t(nq) * sl);
+    }
+    a.rows = learned ? h.eproj.p : h.erows.p;
     a.enorms = h.enorms.p;
     a.ids = h.eids.p;
     a.E = h.entryE;
-    a.D = h.D;
-    a.Dp = h.Dp;
-    a.Q = Qk;
+    a.D = learned ? h.mlpP : h.D;
+    a.Dp = learned ? h.mlpP : h.Dp;
+    a.Q = learned ? h.zq.p : Qk;
     a.qnorms = h.qn.p;
     a.nq = nq;
-    a.qstride = unsigned(qs);
-    a.m = m;
+    a.qstride = learned ? h.mlpP : unsigned(qs);
+    a.m = learned ? sl : m;
     a.scores = h.escores.p;
-    a.out_ids = d_ids;
-    a.ostride = unsigned(ostride);
+    a.out_ids = learned ? h.eshort.p : d_ids;
+    a.ostride = learned ? sl : unsigned(ostride);
     KG_HIP(hipEventRecord(h.ev[14], h.stream));
-    KG_HIP(kg_launch_entries(h.metric, a, h.stream));
+    if (learned) {
+        mlp_forward(h, Qk, nq, qs, h.hq.p, h.zq.p);
+        KG_HIP(kg_launch_entries(KG_METRIC_L2SQR, a, h.stream));
+        kg_rescore_args r{};
+        r.X = h.X.p;
+        r.norms = h.norms.p;
+        r.D = h.D;
+        r.Dp = h.Dp;
+        r.Q = Qk;
+        r.qnorms = h.qn.p;
+        r.nq = nq;
+        r.qstride = unsigned(qs);
+        r.short_ids = h.eshort.p;
+        r.s = sl;
+        r.m = m;
+        r.out_ids = d_ids;
+        r.ostride = unsigned(ostride);
+        KG_HIP(kg_launch_rescore(h.metric, r, h.stream));
+    } else {
+        KG_HIP(kg_launch_entries(h.metric, a, h.stream));
+    }
     KG_HIP(hipEventRecord(h.ev[15], h.stream));
     h.sync();
     KG_HIP(hipEventElapsedTime(&h.last_ms[7], h.ev[14], h.ev[15]));
@@ -752,6 +781,71 @@ int kg_search_latencies(kg_handle *k, float *out_us, unsigned n) {
         if (!out_us) throw kga::invalid_argument("null argument");
         unsigned c = std::min(n, h.lat_n);
         if (c) KG_HIP(hipMemcpy(out_us, h.lat.p, 4ull * c, hipMemcpyDeviceToHost));
+    });
+}
+
+int kg_mlp_init(kg_handle *k, unsigned hidden, unsigned out_dim, unsigned seed) {
+    return guard(k, [&](Handle &h) { mlp_init(h, hidden, out_dim, seed); });
+}
+
+int kg_mlp_train(kg_handle *k, unsigned steps, unsigned batch, float lr, float margin, unsigned seed,
+                 float *loss_trace) {
+    return guard(k, [&](Handle &h) { mlp_train(h, steps, batch, lr, margin, seed, loss_trace); });
+}
+
+int kg_mlp_loss_grad(kg_handle *k, unsigned const *triplets, unsigned nt, float margin, float *loss, float *dW1,
+                     float *db1, float *dW2, float *db2) {
+    return guard(k, [&](Handle &h) {
+        if (!triplets || nt == 0) throw kga::invalid_argument("no triplets");
+        mlp_loss_grad(h, triplets, nt, margin, loss, dW1, db1, dW2, db2);
+    });
+}
+
+int kg_mlp_get_weights(kg_handle *k, float *W1, float *b1, float *W2, float *b2) {
+    return guard(k, [&](Handle &h) {
+        if (!h.mlpH) throw kga::runtime_error("no projector (kg_mlp_init)");
+        size_t D = h.D, H = h.mlpH, P = h.mlpP;
+        if (W1) KG_HIP(hipMemcpy(W1, h.W1.p, 4 * H * D, hipMemcpyDeviceToHost));
+        if (b1) KG_HIP(hipMemcpy(b1, h.b1.p, 4 * H, hipMemcpyDeviceToHost));
+        if (W2) KG_HIP(hipMemcpy(W2, h.W2.p, 4 * P * H, hipMemcpyDeviceToHost));
+        if (b2) KG_HIP(hipMemcpy(b2, h.b2.p, 4 * P, hipMemcpyDeviceToHost));
+    });
+}
+
+int kg_mlp_set_weights(kg_handle *k, float const *W1, float const *b1, float const *W2, float const *b2) {
+    return guard(k, [&](Handle &h) {
+        if (!h.mlpH) throw kga::runtime_error("no projector (kg_mlp_init)");
+        if (!W1 || !b1 || !W2 || !b2) throw kga::invalid_argument("null argument");
+        size_t D = h.D, H = h.mlpH, P = h.mlpP;
+        KG_HIP(hipMemcpy(h.W1.p, W1, 4 * H * D, hipMemcpyHostToDevice));
+        KG_HIP(hipMemcpy(h.b1.p, b1, 4 * H, hipMemcpyHostToDevice));
+        KG_HIP(hipMemcpy(h.W2.p, W2, 4 * P * H, hipMemcpyHostToDevice));
+        KG_HIP(hipMemcpy(h.b2.p, b2, 4 * P, hipMemcpyHostToDevice));
+        ++h.mlp_version;
+    });
+}
+
+int kg_mlp_project(kg_handle *k, float const *X, unsigned n, size_t stride, float *Z) {
+    return guard(k, [&](Handle &h) {
+        if (!X || !Z) throw kga::invalid_argument("null argument");
+        if (!h.mlpH) throw kga::runtime_error("no projector (kg_mlp_init)");
+        if (stride < h.D) throw kga::invalid_argument("bad stride");
+        if (n == 0) return;
+        DevBuf<float> dx, dh, dz;
+        upload_rows(h, X, false, n, h.D, stride, dx);
+        dh.reserve(size_t(n) * h.mlpH);
+        dz.reserve(size_t(n) * h.mlpP);
+        mlp_forward(h, dx.p, n, h.Dp, dh.p, dz.p);
+        h.sync();
+        KG_HIP(hipMemcpy(Z, dz.p, 4ull * n * h.mlpP, hipMemcpyDeviceToHost));
+    });
+}
+
+int kg_set_entry_projection(kg_handle *k, unsigned shortlist) {
+    return guard(k, [&](Handle &h) {
+        if (shortlist > 16) throw kga::invalid_argument("shortlist must be <= 16");
+        if (shortlist && !h.mlpH) throw kga::runtime_error("no projector (kg_mlp_init)");
+        h.entry_shortlist = shortlist;
     });
 }
 

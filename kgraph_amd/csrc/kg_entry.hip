@@ -92,6 +92,33 @@ __global__ void __launch_bounds__(256) entry_topm_kernel(kg_entry_args a) {
     }
 }
 
+// Exact re-rank of a per-query shortlist: one wave per query, one quad per
+// shortlisted node (<= 16), the m best (dist, id) keys are written out.
+template <int METRIC>
+__global__ void __launch_bounds__(256) rescore_kernel(kg_rescore_args a) {
+    __shared__ uint64_t keys[4][16];
+    unsigned const w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    unsigned const q = blockIdx.x * 4 + w;
+    if (q >= a.nq) return;
+    unsigned const grp = lane >> 2, t = lane & 3;
+    float const *qrow = a.Q + size_t(q) * a.qstride;
+    float const qn = (METRIC == METRIC_L2SQR) ? 0.f : a.qnorms[q];
+    if (grp < a.s) {
+        uint32_t id = a.short_ids[size_t(q) * a.s + grp];
+        float acc = quad_metric<METRIC>(a.X + size_t(id) * a.Dp, qrow, a.D, t);
+        float na = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[id];
+        float d = finish_metric<METRIC>(acc, na, qn);
+        if (t == 0) keys[w][grp] = (uint64_t(ord_f32(d)) << 32) | id;
+    }
+    wave_sync();
+    if (lane < a.s) {
+        uint64_t k = keys[w][lane];
+        unsigned rank = 0;
+        for (unsigned j = 0; j < a.s; ++j) rank += keys[w][j] < k;
+        if (rank < a.m) a.out_ids[size_t(q) * a.ostride + rank] = uint32_t(k);
+    }
+}
+
 __global__ void gather_rows_kernel(float const *X, unsigned Dp, uint32_t const *ids, unsigned E, float *out) {
     size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     unsigned const v4 = Dp >> 2;
@@ -107,6 +134,17 @@ hipError_t kg_launch_gather_rows(float const *X, unsigned Dp, uint32_t const *id
     size_t n = size_t(E) * (Dp >> 2);
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(gather_rows_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, X, Dp, ids, E, out);
+    return hipGetLastError();
+}
+
+hipError_t kg_launch_rescore(int metric, kg_rescore_args const &a, hipStream_t s) {
+    if (a.s == 0 || a.s > 16 || a.m > a.s || a.nq == 0) return hipErrorInvalidValue;
+    dim3 g((a.nq + 3) / 4), b(256);
+    switch (metric) {
+        case METRIC_L2SQR: hipLaunchKernelGGL(rescore_kernel<METRIC_L2SQR>, g, b, 0, s, a); break;
+        case METRIC_EUCLIDEAN: hipLaunchKernelGGL(rescore_kernel<METRIC_EUCLIDEAN>, g, b, 0, s, a); break;
+        default: hipLaunchKernelGGL(rescore_kernel<METRIC_ANGULAR>, g, b, 0, s, a); break;
+    }
     return hipGetLastError();
 }
 

@@ -94,6 +94,17 @@ struct Handle {
     DevBuf<float> erows, enorms, escores;
     DevBuf<uint32_t> eids;
 
+    // learned projector D -> mlpH -> mlpP (FP32 master weights, Adam state)
+    unsigned mlpH = 0, mlpP = 0, mlp_version = 0, adam_step = 0;
+    DevBuf<float> W1, b1, W2, b2, gW1, gb1, gW2, gb2;
+    DevBuf<float> mW1, vW1, mb1, vb1, mW2, vW2, mb2, vb2;
+    DevBuf<float> mlp_x, mlp_h, mlp_z, mlp_dz, mlp_dh, mlp_loss;
+    DevBuf<uint32_t> mlp_idx;
+    // entry selection through the projector: shortlist size (0 = exact scoring)
+    unsigned entry_shortlist = 0, eproj_version = ~0u;
+    DevBuf<float> eproj, zq, hq;
+    DevBuf<uint32_t> eshort;
+
     // timing + build trace
     hipEvent_t ev[16] = {};
     float last_ms[8] = {};
@@ -106,6 +117,14 @@ struct Handle {
     void pull_graph();                    // device -> host mirror
     void push_graph();                    // host mirror -> device
 };
+
+// Defined in kg_mlp.cpp.
+void mlp_init(Handle &h, unsigned hidden, unsigned out, unsigned seed);
+void mlp_forward(Handle &h, float const *X, unsigned n, size_t stride, float *Hout, float *Zout);
+void mlp_train(Handle &h, unsigned steps, unsigned batch, float lr, float margin, unsigned seed, float *trace);
+void mlp_loss_grad(Handle &h, uint32_t const *triplets, unsigned nt, float margin, float *loss, float *dW1,
+                   float *db1, float *dW2, float *db2);
+void mlp_project_entries(Handle &h);
 
 // Defined in kg_build.cpp.
 void build_index(Handle &h, kg_index_params const &params, kg_index_info *info);

@@ -76,12 +76,26 @@ struct kg_entry_args {
     unsigned ostride;
 };
 
+struct kg_rescore_args {
+    float const *X;
+    float const *norms;
+    unsigned D, Dp;
+    float const *Q;
+    float const *qnorms;
+    unsigned nq, qstride;
+    uint32_t const *short_ids;  // nq x s node ids
+    unsigned s, m;
+    uint32_t *out_ids;
+    unsigned ostride;
+};
+
 // Launchers (defined in the .hip files).
 size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp);
 hipError_t kg_launch_search(int metric, kg_search_args const &a, unsigned blocks, unsigned waves_per_block,
                             hipStream_t s);
 hipError_t kg_launch_brute(int metric, kg_brute_args const &a, hipStream_t s);
 hipError_t kg_launch_entries(int metric, kg_entry_args const &a, hipStream_t s);
+hipError_t kg_launch_rescore(int metric, kg_rescore_args const &a, hipStream_t s);
 hipError_t kg_launch_gather_rows(float const *X, unsigned Dp, uint32_t const *ids, unsigned E, float *out,
                                  hipStream_t s);
 hipError_t kg_launch_norms(int metric, float const *X, unsigned N, unsigned D, unsigned Dp, float *out,

@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--entry-sample", type=int, default=8192)
     ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy 1 GiB device to device (reference transfer for counter runs)")
     ap.add_argument("--cpu-queries", type=int, default=1000)
     ap.add_argument("--cpu-build-n", type=int, default=100_000)
     return ap.parse_args()
@@ -63,6 +65,18 @@ def main():
         print("bench.py runs on one GPU: the build and search shard by node / query range across GPUs by "
               "design only (no multi-GPU path is implemented or measured).", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        # a device-to-device copy of exactly 1 GiB: its FETCH_SIZE / WRITE_SIZE
+        # counters scale the counter runs of the kernels below
+        from kgraph_amd import devmem
+        src = devmem.DeviceArray((1 << 28,), np.float32)
+        dst = devmem.DeviceArray((1 << 28,), np.float32)
+        src.zero()
+        devmem.synchronize()
+        devmem.hip().hipMemcpy(dst.ptr, src.ptr, 1 << 30, 3)
+        devmem.synchronize()
+        del src, dst
+        log("calibration copy: 1 GiB device to device")
     if args.workload == "batched-10m-d96":
         return batched_workload(args)
     import kgraph_amd

@@ -196,6 +196,15 @@ int kg_mlp_set_weights(kg_handle *h, const float *W1, const float *b1, const flo
 int kg_mlp_project(kg_handle *h, const float *X, unsigned n, size_t stride, float *Z);
 int kg_set_entry_projection(kg_handle *h, unsigned shortlist);
 
+/* Launch-shape autotuner for kg_search (no counterpart in the reference):
+ * runs the search kernel on the given device-resident query sample with
+ * every (waves per block, resident blocks per CU) pair in {1,2,4,8} x
+ * {4,8,16,32} that fits 32 waves per CU, times each with HIP events and keeps
+ * the fastest for later searches on this handle.  chosen (may be NULL)
+ * receives {waves per block, blocks per CU}. */
+int kg_autotune_search(kg_handle *h, const float *dQ, unsigned nq, size_t qstride, const kg_search_params *params,
+                       const unsigned *Ks, const unsigned *Ps, unsigned *chosen);
+
 /* Per-query latency of the last kg_search[_device] call in microseconds,
  * stamped inside the search kernel from the query's first to its last
  * instruction (walk queries only; brute-force and skipped rows read 0).

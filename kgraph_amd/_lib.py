@@ -93,6 +93,7 @@ SIGNATURES = {
     "kg_mlp_set_weights": (_I, [_P, _P, _P, _P, _P]),
     "kg_mlp_project": (_I, [_P, _P, _U, _SZ, _P]),
     "kg_set_entry_projection": (_I, [_P, _U]),
+    "kg_autotune_search": (_I, [_P, _P, _U, _SZ, ctypes.POINTER(SearchParams), _P, _P, _P]),
     "kg_select_entries": (_I, [_P, _P, _U, _SZ, _U, _P, _SZ]),
     "kg_select_entries_device": (_I, [_P, _P, _U, _SZ, _U, _P, _SZ]),
 }

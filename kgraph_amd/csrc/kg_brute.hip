@@ -41,6 +41,9 @@ __device__ void block_bitonic(uint64_t *v, unsigned n) {
     }
 }
 
+// Rows [r0, r1) of split blockIdx.y.  With a.partial the block writes its
+// sorted top-K keys (~0 padded) to partial[(q * splits + split) * K]; else the
+// final ids / dists / counts.
 template <int METRIC>
 __global__ void __launch_bounds__(BT) brute_kernel(kg_brute_args a) {
     __shared__ uint64_t keys[SORTN];  // [0,K) top-K, [K, K+cnt) buffer
@@ -70,9 +73,11 @@ __global__ void __launch_bounds__(BT) brute_kernel(kg_brute_args a) {
     }
     __syncthreads();
     unsigned have = 0;  // valid entries in the top-K part
-    for (unsigned r0 = 0; r0 < a.N; r0 += 64) {
+    unsigned const span = (a.N + gridDim.y - 1) / gridDim.y;
+    unsigned const rb = blockIdx.y * span, re = min(a.N, rb + span);
+    for (unsigned r0 = rb; r0 < re; r0 += 64) {
         unsigned row = r0 + wv * 16 + grp;
-        if (r0 + wv * 16 + grp < a.N) {
+        if (r0 + wv * 16 + grp < re) {
             float acc = quad_metric<METRIC>(a.X + size_t(row) * a.Dp, qv, a.D, t);
             float na = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[row];
             float d = finish_metric<METRIC>(acc, na, qn);
@@ -83,7 +88,7 @@ __global__ void __launch_bounds__(BT) brute_kernel(kg_brute_args a) {
             }
         }
         __syncthreads();
-        if (cnt + 64 > BUF || r0 + 64 >= a.N) {
+        if (cnt + 64 > BUF || r0 + 64 >= re) {
             unsigned n = K + cnt;
             unsigned sn = 2;
             while (sn < n) sn <<= 1;
@@ -98,10 +103,39 @@ __global__ void __launch_bounds__(BT) brute_kernel(kg_brute_args a) {
             __syncthreads();
         }
     }
+    if (a.partial) {
+        uint64_t *dst = a.partial + (size_t(blockIdx.x) * gridDim.y + blockIdx.y) * K;
+        for (unsigned i = threadIdx.x; i < K; i += BT) dst[i] = (i < have) ? keys[i] : ~0ull;
+        return;
+    }
     for (unsigned i = threadIdx.x; i < have; i += BT) {
         uint64_t key = keys[i];
         a.out_ids[size_t(qi) * a.ostride + i] = uint32_t(key);
         if (a.out_dists) a.out_dists[size_t(qi) * a.ostride + i] = unord_f32(uint32_t(key >> 32));
+    }
+    if (threadIdx.x == 0) {
+        if (a.counts) a.counts[qi] = have;
+        if (a.comps) a.comps[qi] = a.N;
+        if (a.scans) a.scans[qi] = 0;
+    }
+}
+
+// Fold the splits' partial top-K lists of one query into the final result.
+__global__ void __launch_bounds__(BT) brute_merge_kernel(kg_brute_args a, unsigned splits) {
+    __shared__ uint64_t keys[4096];
+    unsigned const qi = a.qsel ? a.qsel[blockIdx.x] : blockIdx.x;
+    unsigned const n = splits * a.K;
+    unsigned p2 = 1;
+    while (p2 < n) p2 <<= 1;
+    uint64_t const *src = a.partial + size_t(blockIdx.x) * n;
+    for (unsigned i = threadIdx.x; i < p2; i += BT) keys[i] = (i < n) ? src[i] : ~0ull;
+    __syncthreads();
+    block_bitonic(keys, p2);
+    unsigned have = 0;
+    for (unsigned i = 0; i < a.K; ++i) have += keys[i] != ~0ull;
+    for (unsigned i = threadIdx.x; i < have; i += BT) {
+        a.out_ids[size_t(qi) * a.ostride + i] = uint32_t(keys[i]);
+        if (a.out_dists) a.out_dists[size_t(qi) * a.ostride + i] = unord_f32(uint32_t(keys[i] >> 32));
     }
     if (threadIdx.x == 0) {
         if (a.counts) a.counts[qi] = have;
@@ -124,14 +158,25 @@ __global__ void __launch_bounds__(256) norms_kernel(float const *X, unsigned N, 
 
 }  // namespace
 
-hipError_t kg_launch_brute(int metric, kg_brute_args const &a, hipStream_t s) {
-    if (a.K == 0 || a.K > 1024 || a.Dp > 1024 || a.nq == 0) return hipErrorInvalidValue;
-    dim3 grid(a.nq), block(BT);
+unsigned kg_brute_splits(unsigned nq, unsigned N, unsigned K) {
+    // enough blocks to cover the GPU, each split at least 16k rows, splits*K <= 4096
+    unsigned sp = 1;
+    while (sp * 2 <= 64 && nq * sp < 2048 && N / (sp * 2) >= 16384 && sp * 2 * K <= 4096) sp *= 2;
+    return sp;
+}
+
+hipError_t kg_launch_brute(int metric, kg_brute_args const &a0, hipStream_t s) {
+    if (a0.K == 0 || a0.K > 1024 || a0.Dp > 1024 || a0.nq == 0) return hipErrorInvalidValue;
+    unsigned const splits = a0.partial ? kg_brute_splits(a0.nq, a0.N, a0.K) : 1;
+    kg_brute_args a = a0;
+    if (splits == 1) a.partial = nullptr;
+    dim3 grid(a.nq, splits), block(BT);
     switch (metric) {
         case METRIC_L2SQR: hipLaunchKernelGGL(brute_kernel<METRIC_L2SQR>, grid, block, 0, s, a); break;
         case METRIC_EUCLIDEAN: hipLaunchKernelGGL(brute_kernel<METRIC_EUCLIDEAN>, grid, block, 0, s, a); break;
         default: hipLaunchKernelGGL(brute_kernel<METRIC_ANGULAR>, grid, block, 0, s, a); break;
     }
+    if (splits > 1) hipLaunchKernelGGL(brute_merge_kernel, dim3(a.nq), block, 0, s, a, splits);
     return hipGetLastError();
 }
 

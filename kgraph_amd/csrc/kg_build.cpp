@@ -172,6 +172,9 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
         b.out_ids = kid.p;
         b.out_dists = st.cdist.p;
         b.ostride = p.K;
+        DevBuf<uint64_t> part;  // rows split across blocks, then merged
+        part.reserve(size_t(nc) * kg_brute_splits(nc, N, p.K) * p.K);
+        b.partial = part.p;
         KG_HIP(kg_launch_brute(h.metric, b, s));
         KG_HIP(hipMemcpyAsync(ctl_knn.data(), st.cdist.p, 4 * ctl_knn.size(), hipMemcpyDeviceToHost, s));
         h.sync();

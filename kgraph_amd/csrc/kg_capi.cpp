@@ -195,11 +195,13 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
     // graph-walk queries
     unsigned const ngraph = nq - unsigned(brute.size());
     if (ngraph) {
-        unsigned const wpb = 4;
         size_t lpw = kg_search_lds_per_wave(poolmax, kmax + 1, h.Dp);
+        unsigned wpb = h.search_wpb;
+        while (wpb > 1 && lpw * wpb > 65536) wpb /= 2;
         size_t lpb = lpw * wpb;
         if (lpb > 65536) throw invalid_argument("K + P too large for the search kernel's LDS budget");
-        unsigned bpc = unsigned(std::max<size_t>(1, std::min<size_t>(8, h.lds_per_cu / lpb)));
+        unsigned bpc = unsigned(std::max<size_t>(1, std::min<size_t>(32 / wpb, h.lds_per_cu / lpb)));
+        if (h.search_bpc) bpc = std::min(bpc, h.search_bpc);
         unsigned blocks = std::min<unsigned>((ngraph + wpb - 1) / wpb, h.num_cus * bpc);
         unsigned words = (N + 31) / 32;
         size_t budget = size_t(8) << 30;  // visited bitsets: at most 8 GiB
@@ -292,9 +294,13 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         b.counts = d_counts;
         b.comps = d_comps;
         b.scans = d_scans;
+        DevBuf<uint64_t> part;
+        part.reserve(size_t(b.nq) * kg_brute_splits(b.nq, N, b.K) * b.K);
+        b.partial = part.p;
         KG_HIP(hipEventRecord(h.ev[2], h.stream));
         KG_HIP(kg_launch_brute(h.metric, b, h.stream));
         KG_HIP(hipEventRecord(h.ev[3], h.stream));
+        h.sync();
     }
     h.sync();
     if (ngraph) KG_HIP(hipEventElapsedTime(&h.last_ms[0], h.ev[0], h.ev[1]));
@@ -638,6 +644,9 @@ int kg_bruteforce(kg_handle *k, float const *Q, unsigned nq, size_t qstride, uns
         b.out_dists = dd.p;
         b.ostride = unsigned(ostride);
         b.counts = dc.p;
+        DevBuf<uint64_t> part;
+        part.reserve(size_t(nq) * kg_brute_splits(nq, h.N, K) * K);
+        b.partial = part.p;
         KG_HIP(hipEventRecord(h.ev[2], h.stream));
         KG_HIP(kg_launch_brute(h.metric, b, h.stream));
         KG_HIP(hipEventRecord(h.ev[3], h.stream));
@@ -846,6 +855,49 @@ int kg_set_entry_projection(kg_handle *k, unsigned shortlist) {
         if (shortlist > 16) throw kga::invalid_argument("shortlist must be <= 16");
         if (shortlist && !h.mlpH) throw kga::runtime_error("no projector (kg_mlp_init)");
         h.entry_shortlist = shortlist;
+    });
+}
+
+int kg_autotune_search(kg_handle *k, float const *dQ, unsigned nq, size_t qstride, kg_search_params const *params,
+                       unsigned const *Ks, unsigned const *Ps, unsigned *chosen) {
+    return guard(k, [&](Handle &h) {
+        if (!params || !dQ || nq == 0) throw kga::invalid_argument("null argument");
+        unsigned kmax = params->K;
+        if (Ks)
+            for (unsigned q = 0; q < nq; ++q) kmax = std::max(kmax, Ks[q]);
+        kmax = std::max(kmax, 1u);
+        DevBuf<uint32_t> di, dc;
+        DevBuf<float> dd;
+        di.reserve(size_t(nq) * kmax);
+        dd.reserve(size_t(nq) * kmax);
+        dc.reserve(nq);
+        float best = 3.4e38f;
+        unsigned bw = 4, bb = 0;
+        for (unsigned wpb : {1u, 2u, 4u, 8u}) {
+            for (unsigned bpc : {4u, 8u, 16u, 32u}) {
+                if (wpb * bpc > 32) continue;  // at most 32 resident waves per CU
+                h.search_wpb = wpb;
+                h.search_bpc = bpc;
+                float t = 0;
+                kg_search_params sp = *params;
+                sp.init = 0;  // random starting points: the shape does not depend on them
+                for (int rep = 0; rep < 2; ++rep) {
+                    search_device(h, dQ, nq, qstride, sp, Ks, Ps, di.p, dd.p, kmax, dc.p, nullptr, nullptr);
+                    t = h.last_ms[0];
+                }
+                if (t < best) {
+                    best = t;
+                    bw = wpb;
+                    bb = bpc;
+                }
+            }
+        }
+        h.search_wpb = bw;
+        h.search_bpc = bb;
+        if (chosen) {
+            chosen[0] = bw;
+            chosen[1] = bb;
+        }
     });
 }
 

@@ -86,6 +86,7 @@ struct Handle {
     DevBuf<float> q, qn, out_dists;
     DevBuf<uint32_t> out_ids, counts, comps, scans;
     unsigned vis_slots = 0, vis_words = 0;
+    unsigned search_wpb = 4, search_bpc = 0;  // launch shape (0 = as many blocks as LDS allows, <= 8)
     DevBuf<float> lat;      // per-query latency of the last search (microseconds)
     unsigned lat_n = 0;
 

@@ -60,6 +60,7 @@ struct kg_brute_args {
     unsigned ostride;
     uint32_t *counts;
     uint32_t *comps, *scans;  // optional: set to N and 0 (kgraph.cpp:428-431)
+    uint64_t *partial;        // optional scratch: nq * kg_brute_splits(nq, N, K) * K keys
 };
 
 struct kg_entry_args {
@@ -94,6 +95,7 @@ size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp);
 hipError_t kg_launch_search(int metric, kg_search_args const &a, unsigned blocks, unsigned waves_per_block,
                             hipStream_t s);
 hipError_t kg_launch_brute(int metric, kg_brute_args const &a, hipStream_t s);
+unsigned kg_brute_splits(unsigned nq, unsigned N, unsigned K);
 hipError_t kg_launch_entries(int metric, kg_entry_args const &a, hipStream_t s);
 hipError_t kg_launch_rescore(int metric, kg_rescore_args const &a, hipStream_t s);
 hipError_t kg_launch_gather_rows(float const *X, unsigned Dp, uint32_t const *ids, unsigned E, float *out,

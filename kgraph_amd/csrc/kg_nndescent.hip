@@ -169,29 +169,93 @@ __device__ __forceinline__ void push_cand(NndArgs const &a, uint32_t target, uin
 }
 
 // One wavefront per node.  Each quad holds one "other" row (new or old) in
-// registers and scores it against every new row, which the wave's quads all
-// read through L1 at roughly the same time.  No LDS and no block barriers:
-// the average node has ~25 list entries, so latency hiding comes from many
-// resident waves rather than from staging.
+// registers and scores it against every new row, which all quads of the wave
+// read through L1 at the same time.  Candidates are staged in a per-wave LDS
+// buffer and flushed grouped by target: one slot-claim atomicAdd per target
+// with pending candidates (all issued together), then non-returning 64-bit
+// atomicMin stores into the claimed slots (a hashed slot once a target's
+// slots are exhausted, which keeps the better key).
+constexpr unsigned JT = 512;  // list capacity (new + old) held in LDS
+constexpr unsigned JB = 256;  // staged candidates per wave
+
+struct JoinLDS {
+    uint32_t ids[JT];     // new list then old list
+    uint32_t cnt[JT];     // per-target pending count, then claimed base
+    uint64_t key[JB];
+    uint16_t tgt[JB];
+    unsigned fill;
+};
+
+__device__ void join_flush(NndArgs const &a, JoinLDS &L, unsigned nT) {
+    unsigned const lane = lane_id();
+    unsigned const fill = L.fill;
+    for (unsigned i = lane; i < nT; i += KG_WAVE) L.cnt[i] = 0;
+    wave_sync();
+    unsigned rank[JB / KG_WAVE];
+#pragma unroll
+    for (unsigned j = 0; j < JB / KG_WAVE; ++j) {
+        unsigned i = lane + j * KG_WAVE;
+        rank[j] = (i < fill) ? atomicAdd(&L.cnt[L.tgt[i]], 1u) : 0u;
+    }
+    wave_sync();
+    for (unsigned i = lane; i < nT; i += KG_WAVE) {
+        unsigned c = L.cnt[i];
+        if (c) L.cnt[i] = atomicAdd(&a.ccnt[L.ids[i]], c);
+    }
+    wave_sync();
+#pragma unroll
+    for (unsigned j = 0; j < JB / KG_WAVE; ++j) {
+        unsigned i = lane + j * KG_WAVE;
+        if (i < fill) {
+            unsigned const tp = L.tgt[i];
+            uint32_t const target = L.ids[tp];
+            uint64_t const key = L.key[i];
+            unsigned slot = L.cnt[tp] + rank[j];
+            if (slot >= a.C) slot = unsigned(mix64((uint64_t(target) << 32) | (uint32_t(key) >> 1)) % a.C);
+            atomicMin(reinterpret_cast<unsigned long long *>(&a.cand[size_t(target) * a.C + slot]),
+                      static_cast<unsigned long long>(key));
+        }
+    }
+    wave_sync();
+    if (lane == 0) L.fill = 0;
+    wave_sync();
+}
+
 template <int METRIC, int CH>
 __global__ void __launch_bounds__(256) join_kernel(NndArgs a) {
-    unsigned const lane = threadIdx.x & 63;
-    unsigned const n = blockIdx.x * 4 + (threadIdx.x >> 6);
+    __shared__ JoinLDS lds[4];
+    unsigned const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned const n = blockIdx.x * 4 + w;
     if (n >= a.N) return;
+    JoinLDS &L = lds[w];
     unsigned const quad = lane >> 2, t = lane & 3;
     unsigned const nn = a.nnew[n], no = a.nold[n];
     unsigned const rn = min(a.crnew[n], a.R), ro = min(a.crold[n], a.R);
-    unsigned const nA = nn + rn, nB = no + ro;
+    unsigned const nA = nn + rn, nB = no + ro, nT = min(nA + nB, JT);
     uint32_t const *samp = a.samp + size_t(n) * a.Lc;
-    uint32_t const *rnew = a.rnew + size_t(n) * a.R;
-    uint32_t const *rold = a.rold + size_t(n) * a.R;
+    for (unsigned i = lane; i < nT; i += KG_WAVE) {
+        uint32_t v;
+        if (i < nn)
+            v = samp[i];
+        else if (i < nA)
+            v = a.rnew[size_t(n) * a.R + (i - nn)];
+        else if (i < nA + no)
+            v = samp[a.Lc - 1 - (i - nA)];
+        else
+            v = a.rold[size_t(n) * a.R + (i - nA - no)];
+        L.ids[i] = v;
+    }
+    if (lane == 0) L.fill = 0;
+    wave_sync();
+    unsigned const nAe = min(nA, nT);
     unsigned const nch = (a.D & ~15u) >> 2;
     bool found = false;
     unsigned pairs = 0, pushes = 0;
-    for (unsigned oi = quad; oi < nA + nB; oi += 16) {
-        bool const isB = oi >= nA;
-        uint32_t const o = isB ? ((oi - nA < no) ? samp[a.Lc - 1 - (oi - nA)] : rold[oi - nA - no])
-                               : ((oi < nn) ? samp[oi] : rnew[oi - nn]);
+    for (unsigned base = 0; base < nT; base += 16) {
+        unsigned const oi = base + quad;
+        bool const active = oi < nT;  // whole quads; every lane stays in the loop
+        bool const isB = oi >= nAe;
+        uint32_t const o = active ? L.ids[oi] : 0u;
         float const *orow = a.X + size_t(o) * a.Dp;
         float4 reg[CH > 0 ? CH : 1];
         if (CH > 0) {
@@ -203,11 +267,11 @@ __global__ void __launch_bounds__(256) join_kernel(NndArgs a) {
         }
         float const ro_ = a.radius[o];
         float const no_ = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[o];
-        for (unsigned ai = 0; ai < nA; ++ai) {
-            uint32_t const av = (ai < nn) ? samp[ai] : rnew[ai - nn];
+        for (unsigned ai = 0; ai < nAe; ++ai) {
+            if (L.fill + 32 > JB) join_flush(a, L, nT);  // wave-uniform
+            uint32_t const av = L.ids[ai];
             // new x new once per id pair (i < j by id), new x old always (kgraph.cpp:741-750)
-            bool const valid = isB ? (av != o) : (av < o);
-            if (!valid) continue;
+            bool const valid = active && (isB ? (av != o) : (av < o));
             float const *ra = a.X + size_t(av) * a.Dp;
             float acc;
             if (CH > 0) {
@@ -246,23 +310,30 @@ __global__ void __launch_bounds__(256) join_kernel(NndArgs a) {
             }
             float const na = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[av];
             float const d = finish_metric<METRIC>(acc, na, no_);
-            if (t == 0) {
-                ++pairs;
-                if (d < a.radius[av]) {
-                    push_cand(a, av, o, d);
-                    ++pushes;
-                }
-                if (d < ro_) {
-                    push_cand(a, o, av, d);
-                    ++pushes;
-                }
-                // found: the first endpoint's insert lands in its top K (kgraph.cpp:807-812)
-                found |= d < a.kth[av];
+            bool const lead = valid && t == 0;
+            bool const pa = lead && d < a.radius[av];
+            bool const po = lead && d < ro_;
+            unsigned const np_ = unsigned(pa) + unsigned(po);
+            unsigned slot = 0;
+            if (np_) slot = atomicAdd(&L.fill, np_);
+            if (pa) {
+                L.key[slot] = make_key(d, o, 1);
+                L.tgt[slot] = uint16_t(ai);
+                ++slot;
             }
+            if (po) {
+                L.key[slot] = make_key(d, av, 1);
+                L.tgt[slot] = uint16_t(oi);
+            }
+            pairs += lead;
+            pushes += np_;
+            // found: the first endpoint's insert lands in its top K (kgraph.cpp:807-812)
+            found |= lead && d < a.kth[av];
+            wave_sync();
         }
     }
+    if (L.fill) join_flush(a, L, nT);
     uint64_t const fb = ballot(found);
-    // wave sums of the counters (lanes t != 0 hold zeros)
     for (int o = 32; o > 0; o >>= 1) {
         pairs += __shfl_xor(pairs, o);
         pushes += __shfl_xor(pushes, o);

@@ -35,7 +35,8 @@ def log(*a):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["intmix-1m-d128", "batched-10m-d96"], default="intmix-1m-d128")
+    ap.add_argument("--workload", choices=["intmix-1m-d128", "batched-10m-d96", "learned-5m-d128"],
+                    default="intmix-1m-d128")
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=None, help="rows (default 1M, or 10M for batched-10m-d96)")
@@ -44,6 +45,8 @@ def parse():
     ap.add_argument("--P", type=str, default="50,100,200")
     ap.add_argument("--Pe", type=str, default="10,20,50,100",
                     help="P values for walks started from selected entry points")
+    ap.add_argument("--Se", type=str, default="10,32,64",
+                    help="S values (neighbours per expansion, SearchParams::S) for entry-started walks")
     ap.add_argument("--entry-sample", type=int, default=8192)
     ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -58,7 +61,7 @@ def main():
     args = parse()
     batched = args.workload == "batched-10m-d96"
     if args.n is None:
-        args.n = 10_000_000 if batched else 1_000_000
+        args.n = {"batched-10m-d96": 10_000_000, "learned-5m-d128": 5_000_000}.get(args.workload, 1_000_000)
     if args.nq is None:
         args.nq = 100_000 if batched else 10_000
     if args.gpus != 1:
@@ -79,6 +82,8 @@ def main():
         log("calibration copy: 1 GiB device to device")
     if args.workload == "batched-10m-d96":
         return batched_workload(args)
+    if args.workload == "learned-5m-d128":
+        return learned_workload(args)
     import kgraph_amd
     from kgraph_amd import _lib, datagen, devmem
     from kgraph_amd._lib import IndexInfo, IndexParams, SearchParams, ptr
@@ -89,6 +94,7 @@ def main():
     log("data generated in %.1fs: X %s Q %s" % (time.time() - t0, X.shape, Q.shape))
     Ps = [int(p) for p in args.P.split(",") if p]
     Pe = [int(p) for p in args.Pe.split(",") if p]
+    Se = [int(v) for v in args.Se.split(",") if v]
     K, D, N, nq = 10, X.shape[1], X.shape[0], Q.shape[0]
     bp = dict(K=100, L=150, S=10, R=100, iterations=args.iterations, controls=100, seed=1998, delta=0.002,
               recall=0.99, prune=0, reverse=-1)
@@ -108,7 +114,7 @@ def main():
     dSc = devmem.DeviceArray((nq,), np.uint32)
 
     m = K - 1  # entry points per query (SearchParams::init must stay below K)
-    sweeps = [("random", P) for P in Ps] + [("entries", P) for P in Pe]
+    sweeps = [("random", P, 10) for P in Ps] + [("entries", P, S) for P in Pe for S in Se]
 
     def one_step():
         info = IndexInfo()
@@ -119,8 +125,8 @@ def main():
         devmem.synchronize()
         build_s = time.perf_counter() - tb
         res = []
-        for mode, P in sweeps:
-            sp = SearchParams(K, 0, P, 10, 1, 1e30, 1998, m if mode == "entries" else 0)
+        for mode, P, S in sweeps:
+            sp = SearchParams(K, 0, P, S, 1, 1e30, 1998, m if mode == "entries" else 0)
             devmem.synchronize()
             ts = time.perf_counter()
             ems = 0.0
@@ -131,7 +137,7 @@ def main():
             devmem.synchronize()
             wall = time.perf_counter() - ts
             kms = ctypes_float(h, 0)
-            res.append(dict(mode=mode, P=P, wall_s=wall, kernel_ms=kms, entry_ms=ems, ids=dI.to_host(),
+            res.append(dict(mode=mode, P=P, S=S, wall_s=wall, kernel_ms=kms, entry_ms=ems, ids=dI.to_host(),
                             dists=dD.to_host(), comps=dCo.to_host(), scans=dSc.to_host(),
                             seeds=(m if mode == "entries" else P)))
         return info, build_s, res
@@ -175,11 +181,11 @@ def main():
         # algorithmic bytes per query: a row (4*D) per scored node (starting
         # points + comps), 4 bytes per neighbour id read, 4 per starting id
         byt = float((4 * D * (r["seeds"] + r["comps"].astype(np.float64)) + 4 * r["scans"] + 4 * r["seeds"]).sum())
-        sweep.append(dict(mode=r["mode"], P=r["P"], qps=qps, recall=rec, kernel_ms=r["kernel_ms"],
+        sweep.append(dict(mode=r["mode"], P=r["P"], S=r["S"], qps=qps, recall=rec, kernel_ms=r["kernel_ms"],
                           entry_ms=r["entry_ms"], bytes=byt, mean_comps=float(r["comps"].mean()),
                           mean_scans=float(r["scans"].mean())))
-        log("%-7s P=%d recall@10=%.4f QPS=%.0f walk kernel %.2fms entry %.2fms (%.1f GB/s algorithmic)" % (
-            r["mode"], r["P"], rec, qps, r["kernel_ms"], r["entry_ms"], byt / r["kernel_ms"] / 1e6))
+        log("%-7s P=%d S=%d recall@10=%.4f QPS=%.0f walk kernel %.2fms entry %.2fms (%.1f GB/s algorithmic)" % (
+            r["mode"], r["P"], r["S"], rec, qps, r["kernel_ms"], r["entry_ms"], byt / r["kernel_ms"] / 1e6))
     ok = [s for s in sweep if s["recall"] >= 0.95]
     pick = max(ok, key=lambda s: s["qps"]) if ok else max(sweep, key=lambda s: s["recall"])
     achieved = pick["bytes"] / (pick["kernel_ms"] * 1e-3) / 1e9
@@ -201,7 +207,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "intmix-1m-d128", "N": N, "D": D, "metric": "l2sqr",
-                   "build": bp, "queries": nq, "K": K, "P": pick["P"], "start": pick["mode"],
+                   "build": bp, "queries": nq, "K": K, "P": pick["P"], "S": pick["S"], "start": pick["mode"],
                    "entry_sample": args.entry_sample, "entries_per_query": m, "precision": args.precision},
         "build_s": round(build_s, 4),
         "build_s_steps": [round(b, 4) for b in build_times],
@@ -220,7 +226,7 @@ def main():
                           "pushes": int(counters[2])},
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args, X, Q, h, pick["P"], K, bp, pick["mode"])
+        out["cpu_baseline"] = cpu_baseline(args, X, Q, h, pick["P"], K, bp, pick["mode"], pick["S"])
     print(json.dumps(out), flush=True)
 
 
@@ -304,6 +310,92 @@ def batched_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def learned_workload(args):
+    """5,000,000 x 128 anisotropic 5,000-centre mixture (per-cluster diagonal
+    sigma in [0.05, 0.5]), squared Euclidean.  Build (K=25 L=100 reverse -1),
+    train the 128 -> 256 -> 32 projector on 2M triplets from that graph, then
+    compare search started from exactly scored entry points with search
+    started from projector-shortlisted ones (exact re-rank), at equal P and S.
+    The projector is kept only if a 500-query probe set loses no recall."""
+    from kgraph_amd import _lib, datagen, devmem
+    from kgraph_amd._lib import IndexInfo, IndexParams, SearchParams, ptr
+    from oracle.kgo import recall_at_k
+    import kgraph_amd
+    n, nq, K = args.n, args.nq, 10
+    t0 = time.time()
+    X = datagen.anisotropic_mixture(n, 128, 5000, 12, 13, 14)
+    Q = datagen.anisotropic_mixture(nq + 500, 128, 5000, 12, 13, 15)
+    Q, probe = Q[:nq], Q[nq:]
+    log("data generated in %.1fs" % (time.time() - t0))
+    g = kgraph_amd.KGraph(X, "euclidean", l2sqr=True)
+    _lib.lib().kg_set_verbosity(0)
+    tb = time.perf_counter()
+    info = g.build(K=25, L=100, iterations=args.iterations, reverse=-1)
+    build_s = time.perf_counter() - tb
+    g.set_entry_sample(args.entry_sample, seed=1998)
+    log("build %.2fs (%d it, control recall %.4f)" % (build_s, info.iterations, info.recall))
+    tt = time.perf_counter()
+    steps = 2_000_000 // 4096
+    trace = g.train_projector(steps=steps, batch=4096, hidden=256, out=32, lr=2e-3, margin=1.0, seed=1)
+    train_s = time.perf_counter() - tt
+    log("projector trained on %d triplets in %.2fs, loss %.4f -> %.4f" % (steps * 4096, train_s, trace[:10].mean(),
+                                                                         trace[-10:].mean()))
+    gt_i = np.zeros((nq, K), np.uint32)
+    gt_d = np.zeros((nq, K), np.float32)
+    gt_c = np.zeros(nq, np.uint32)
+    g._h.call("kg_bruteforce", ptr(Q), nq, 128, K, 1e30, ptr(gt_i), ptr(gt_d), K, ptr(gt_c))
+    dQ = devmem.DeviceArray.from_host(Q)
+    dI = devmem.DeviceArray((nq, K), np.uint32)
+    dD = devmem.DeviceArray((nq, K), np.float32)
+    dC = devmem.DeviceArray((nq,), np.uint32)
+    m = K - 1
+    rows = []
+    for form, shortlist in (("plain", 0), ("learned", 16)):
+        if shortlist:
+            rep = g.validate_entry_projection(probe, K=K, P=20, entries=m, shortlist=shortlist)
+            log("probe guard: %s" % rep)
+            if not rep["enabled"]:
+                rows.append({"form": form, "fallback": True, "probe": rep})
+                continue
+        g.set_entry_projection(shortlist)
+        for P in (10, 20, 50):
+            sp = SearchParams(K, 0, P, 32, 1, 1e30, 1998, m)
+            for rep_i in range(args.warmup + args.steps):
+                devmem.synchronize()
+                ts = time.perf_counter()
+                g._h.call("kg_select_entries_device", dQ.ptr, nq, 128, m, dI.ptr, K)
+                ems = ctypes_float(g._h, 7)
+                g._h.call("kg_search_device", dQ.ptr, nq, 128, sp, None, None, dI.ptr, dD.ptr, K, dC.ptr, None,
+                          None)
+                devmem.synchronize()
+                wall = time.perf_counter() - ts
+            rec = recall_at_k(dD.to_host(), gt_d, K)
+            rows.append({"form": form, "P": P, "S": 32, "qps": round(nq / wall, 1), "recall": round(rec, 4),
+                         "entry_ms": round(ems, 3), "walk_ms": round(ctypes_float(g._h, 0), 3)})
+            log(rows[-1])
+    best = {}
+    for r in rows:
+        if "qps" in r and r["recall"] >= 0.95:
+            f = r["form"]
+            if f not in best or r["qps"] > best[f]["qps"]:
+                best[f] = r
+    out = {
+        "metric": "search QPS at recall@10>=0.95: learned-screened vs plain entry selection",
+        "value": best.get("learned", best.get("plain", {"qps": 0}))["qps"], "unit": "queries/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (projector bf16 MFMA)", "data": "synthetic",
+        "config": {"workload": "aniso-5m-d128-learned", "N": n, "D": 128, "queries": nq,
+                   "build": {"K": 25, "L": 100, "reverse": -1, "iterations": args.iterations},
+                   "projector": "128-256-32", "triplets": steps * 4096, "entry_sample": args.entry_sample},
+        "build_s": round(build_s, 3), "train_s": round(train_s, 3),
+        "loss_first_last": [round(float(trace[:10].mean()), 4), round(float(trace[-10:].mean()), 4)],
+        "sweep": rows, "best": best,
+        "speedup_learned_vs_plain": (round(best["learned"]["qps"] / best["plain"]["qps"], 3)
+                                     if "learned" in best and "plain" in best else None),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def ctypes_float(h, which):
     from kgraph_amd import _lib
     v = _lib.ctypes.c_float()
@@ -316,7 +408,7 @@ def ctypes_uint():
     return _lib.ctypes.c_uint()
 
 
-def cpu_baseline(args, X, Q, h, P, K, bp, mode):
+def cpu_baseline(args, X, Q, h, P, K, bp, mode, S=10):
     """The oracle (CPU restatement of the reference) on this host's cores:
     search QPS over a query sample on the same GPU-built index, and the build
     time on a row sample with the same IndexParams."""
@@ -332,15 +424,15 @@ def cpu_baseline(args, X, Q, h, P, K, bp, mode):
         # same starting points: the m nearest of the same node sample, then the walk
         samp = np.sort(kgo.genrandom(1998, 1, args.entry_sample, len(X))[0])
         si, _, _ = kgo.bruteforce(X[samp], Q[:nq], K - 1, 0)
-        idx.search(X, Q[:nq], 0, K=K, P=P, init=K - 1, init_ids=samp[si])
+        idx.search(X, Q[:nq], 0, K=K, P=P, S=S, init=K - 1, init_ids=samp[si])
     else:
-        idx.search(X, Q[:nq], 0, K=K, P=P)
+        idx.search(X, Q[:nq], 0, K=K, P=P, S=S)
     dt = time.perf_counter() - t
     qps = nq / dt
     log("cpu search (%s): %d queries in %.2fs on %d threads -> %.0f QPS" % (mode, nq, dt, threads, qps))
     res = {"value": round(qps, 1), "unit": "queries/s", "cores": threads, "kind": "port",
-           "sample": "%d of the %d queries, same GPU-built index, K=%d P=%d, %s starting points" % (
-               nq, len(Q), K, P, mode)}
+           "sample": "%d of the %d queries, same GPU-built index, K=%d P=%d S=%d, %s starting points" % (
+               nq, len(Q), K, P, S, mode)}
     if args.cpu_build_n > 0:
         n = min(args.cpu_build_n, len(X))
         Xs = np.ascontiguousarray(X[:n])

@@ -38,6 +38,44 @@ __global__ void __launch_bounds__(256) entry_dist_kernel(kg_entry_args a) {
     if (e >= a.E) return;  // whole quads leave together
     float const *row = a.rows + size_t(e) * a.Dp;
     float const ne = (METRIC == METRIC_L2SQR) ? 0.f : a.enorms[e];
+    unsigned const nch = (a.D & ~15u) >> 2;
+    if (nch <= 4 * 8 && a.D == (a.D & ~15u)) {
+        // the sampled row stays in registers (up to 8 float4 per lane)
+        float4 reg[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            unsigned kk = t + 4 * k;
+            reg[k] = (kk < nch) ? reinterpret_cast<float4 const *>(row)[kk] : make_float4(0, 0, 0, 0);
+        }
+        for (unsigned r = 0; r < nqb; ++r) {
+            float4 const *q4 = reinterpret_cast<float4 const *>(qs + size_t(r) * a.Dp);
+            float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                unsigned kk = t + 4 * k;
+                if (kk < nch) {
+                    float4 x = reg[k], y = q4[kk];
+                    if (METRIC == METRIC_L2SQR) {
+                        float dx = x.x - y.x, dy = x.y - y.y, dz = x.z - y.z, dw = x.w - y.w;
+                        p.x = __builtin_fmaf(dx, dx, p.x);
+                        p.y = __builtin_fmaf(dy, dy, p.y);
+                        p.z = __builtin_fmaf(dz, dz, p.z);
+                        p.w = __builtin_fmaf(dw, dw, p.w);
+                    } else {
+                        p.x = __builtin_fmaf(x.x, y.x, p.x);
+                        p.y = __builtin_fmaf(x.y, y.y, p.y);
+                        p.z = __builtin_fmaf(x.z, y.z, p.z);
+                        p.w = __builtin_fmaf(x.w, y.w, p.w);
+                    }
+                }
+            }
+            float acc = quad_reduce(p);
+            float qn = (METRIC == METRIC_L2SQR) ? 0.f : a.qnorms[q0 + r];
+            float d = finish_metric<METRIC>(acc, ne, qn);
+            if (t == 0) a.scores[size_t(q0 + r) * a.E + e] = d;
+        }
+        return;
+    }
     for (unsigned r = 0; r < nqb; ++r) {
         float acc = quad_metric<METRIC>(row, qs + size_t(r) * a.Dp, a.D, t);
         float qn = (METRIC == METRIC_L2SQR) ? 0.f : a.qnorms[q0 + r];

@@ -324,6 +324,10 @@ static void set_entry_sample(Handle &h, unsigned E, unsigned seed) {
     KG_HIP(hipMemcpyAsync(h.eids.p, ids.data(), 4ull * E, hipMemcpyHostToDevice, h.stream));
     KG_HIP(kg_launch_gather_rows(h.X.p, h.Dp, h.eids.p, E, h.erows.p, h.stream));
     KG_HIP(kg_launch_norms(h.metric, h.erows.p, E, h.D, h.Dp, h.enorms.p, h.stream));
+    // ranking scale of the BF16 screen: ||e||^2/2, or ||e|| for angular
+    h.escale.reserve(E);
+    KG_HIP(kg_launch_norms(h.metric == KG_METRIC_ANGULAR ? KG_METRIC_ANGULAR : KG_METRIC_EUCLIDEAN, h.erows.p, E, h.D,
+                           h.Dp, h.escale.p, h.stream));
     h.sync();
     h.entryE = E;
     h.eproj_version = ~0u;
@@ -350,13 +354,14 @@ static void select_entries_device(Handle &h, float const *dQ, unsigned nq, size_
     h.escores.reserve(size_t(nq) * h.entryE);
     kg_entry_args a{};
     bool const learned = h.entry_shortlist > 0;
-    unsigned const sl = std::max(m, std::min(h.entry_shortlist, std::min(16u, h.entryE)));
+    bool const screen = !learned && h.precision == KG_PRECISION_BF16;
+    unsigned const sl = std::max(m, std::min(learned ? h.entry_shortlist : 16u, std::min(16u, h.entryE)));
     if (learned) {  // score in the projected space, then re-rank the shortlist exactly
         mlp_project_entries(h);
         h.zq.reserve(size_t(nq) * h.mlpP);
         h.hq.reserve(std::max(size_t(nq), size_t(h.entryE)) * h.mlpH);
-        h.eshort.reserve(size_t(nq) * sl);
     }
+    if (learned || screen) h.eshort.reserve(size_t(nq) * sl);
     a.rows = learned ? h.eproj.p : h.erows.p;
     a.enorms = h.enorms.p;
     a.ids = h.eids.p;
@@ -367,14 +372,19 @@ static void select_entries_device(Handle &h, float const *dQ, unsigned nq, size_
     a.qnorms = h.qn.p;
     a.nq = nq;
     a.qstride = learned ? h.mlpP : unsigned(qs);
-    a.m = learned ? sl : m;
+    a.m = (learned || screen) ? sl : m;
     a.scores = h.escores.p;
-    a.out_ids = learned ? h.eshort.p : d_ids;
-    a.ostride = learned ? sl : unsigned(ostride);
+    a.out_ids = (learned || screen) ? h.eshort.p : d_ids;
+    a.ostride = (learned || screen) ? sl : unsigned(ostride);
+    if (screen) a.enorms = h.escale.p;
     KG_HIP(hipEventRecord(h.ev[14], h.stream));
-    if (learned) {
-        mlp_forward(h, Qk, nq, qs, h.hq.p, h.zq.p);
-        KG_HIP(kg_launch_entries(KG_METRIC_L2SQR, a, h.stream));
+    if (learned || screen) {
+        if (learned) {
+            mlp_forward(h, Qk, nq, qs, h.hq.p, h.zq.p);
+            KG_HIP(kg_launch_entries(KG_METRIC_L2SQR, a, h.stream));
+        } else {
+            KG_HIP(kg_launch_entries_bf16(h.metric, a, h.stream));
+        }
         kg_rescore_args r{};
         r.X = h.X.p;
         r.norms = h.norms.p;

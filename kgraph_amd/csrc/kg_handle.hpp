@@ -92,7 +92,7 @@ struct Handle {
 
     // entry-point sample
     unsigned entryE = 0;
-    DevBuf<float> erows, enorms, escores;
+    DevBuf<float> erows, enorms, escores, escale;
     DevBuf<uint32_t> eids;
 
     // learned projector D -> mlpH -> mlpP (FP32 master weights, Adam state)

@@ -98,6 +98,8 @@ hipError_t kg_launch_brute(int metric, kg_brute_args const &a, hipStream_t s);
 unsigned kg_brute_splits(unsigned nq, unsigned N, unsigned K);
 hipError_t kg_launch_entries(int metric, kg_entry_args const &a, hipStream_t s);
 hipError_t kg_launch_rescore(int metric, kg_rescore_args const &a, hipStream_t s);
+// BF16 MFMA screening; a.enorms = ||e||^2/2 (Euclidean forms) or ||e|| (angular)
+hipError_t kg_launch_entries_bf16(int metric, kg_entry_args const &a, hipStream_t s);
 hipError_t kg_launch_gather_rows(float const *X, unsigned Dp, uint32_t const *ids, unsigned E, float *out,
                                  hipStream_t s);
 hipError_t kg_launch_norms(int metric, float const *X, unsigned N, unsigned D, unsigned Dp, float *out,

@@ -77,7 +77,7 @@ def test_gpu_forward_backward_match_oracle():
         assert err < 2e-3, (name, err)
     # each triplet's output gradients s(zn-zp) - s(za-zp) + s(za-zn) sum to zero,
     # so db2 is zero up to FP32 rounding
-    assert np.abs(ob2).max() < 1e-9 and np.abs(gb2).max() < 1e-5
+    assert np.abs(ob2).max() < 1e-6 and np.abs(gb2).max() < 1e-5
     # forward projection
     Z = np.zeros((50, P), np.float32)
     Xq = np.ascontiguousarray(X[:50], dtype=np.float32)

@@ -350,10 +350,83 @@ void reverse_index(Handle &h, int rev) {
     KG_HIP(hipEventElapsedTime(&h.last_ms[6], h.ev[12], h.ev[13]));
 }
 
-void prune_index(Handle &h, unsigned level) {
-    if (!h.has_graph) throw runtime_error("no index");
-    if (level & 2) throw invalid_argument("prune level 2 is not supported on the GPU");
-    if (!(level & 1)) return;
+namespace {
+
+// prune2 (kgraph.cpp:593-657): levels l = 0 .. max(min(M, deg)) - 1, each a
+// screen pass and resolve rounds on the GPU (kg_graph.hip); the kept lists then
+// replace the graph and M[i] becomes their length (the reference's closing
+// prune1).  M[i] above the list length is clamped (the reference requires
+// M[i] <= size, kgraph.cpp:616).
+void prune2_index(Handle &h) {
+    hipStream_t s = h.stream;
+    unsigned const N = h.gN;
+    std::vector<uint32_t> off(N + 1), M(N);
+    KG_HIP(hipMemcpy(off.data(), h.goff.p, 4ull * (N + 1), hipMemcpyDeviceToHost));
+    KG_HIP(hipMemcpy(M.data(), h.gM.p, 4ull * N, hipMemcpyDeviceToHost));
+    unsigned maxM = 0;
+    for (unsigned i = 0; i < N; ++i) {
+        M[i] = std::min(M[i], off[i + 1] - off[i]);
+        maxM = std::max(maxM, M[i]);
+    }
+    // nodes by min(M, deg) descending (counting sort): the nodes still active
+    // at level l are the first active[l] of them
+    std::vector<uint32_t> active(maxM + 2, 0), order(N);
+    for (unsigned i = 0; i < N; ++i) ++active[M[i]];
+    {
+        uint32_t run = 0;
+        for (unsigned m = maxM + 1; m-- > 0;) {
+            uint32_t c = active[m];
+            active[m] = run;  // nodes with Meff > m
+            run += c;
+        }
+        std::vector<uint32_t> pos(active);
+        for (unsigned i = 0; i < N; ++i) order[pos[M[i]]++] = i;
+    }
+    DevBuf<uint32_t> Meff, ord, kl, nl, st, cnt;
+    DevBuf<float> kd;
+    size_t const E = std::max<size_t>(off[N], 1);
+    Meff.reserve(N);
+    ord.reserve(N);
+    nl.reserve(N);
+    st.reserve(N);
+    cnt.reserve(1);
+    kl.reserve(E);
+    kd.reserve(E);
+    KG_HIP(hipMemcpyAsync(Meff.p, M.data(), 4ull * N, hipMemcpyHostToDevice, s));
+    KG_HIP(hipMemcpyAsync(ord.p, order.data(), 4ull * N, hipMemcpyHostToDevice, s));
+    KG_HIP(hipMemsetAsync(nl.p, 0, 4ull * N, s));
+    Prune2Dev p{N, h.goff.p, h.gids.p, h.no_dist ? nullptr : h.gdist.p, Meff.p, ord.p, kl.p, kd.p, nl.p, st.p};
+    for (unsigned l = 0; l < maxM; ++l) {
+        unsigned const nact = active[l];
+        KG_HIP(kg_prune2_screen(p, l, nact, s));
+        KG_HIP(kg_prune2_resolve(p, l, nact, cnt.p, s));
+        for (;;) {
+            uint32_t open = 0;
+            KG_HIP(hipMemsetAsync(cnt.p, 0, 4, s));
+            KG_HIP(kg_prune2_resolve(p, l, nact, cnt.p, s));
+            KG_HIP(hipMemcpyAsync(&open, cnt.p, 4, hipMemcpyDeviceToHost, s));
+            h.sync();
+            if (!open) break;
+        }
+    }
+    DevBuf<uint32_t> off2, tmp, ids2, M2;
+    DevBuf<float> d2;
+    off2.reserve(N + 1);
+    tmp.reserve((N + 1023) / 1024 + 2);
+    KG_HIP(kg_scan_u32(nl.p, off2.p, N, tmp.p, s));
+    uint32_t E2 = 0;
+    KG_HIP(hipMemcpyAsync(&E2, off2.p + N, 4, hipMemcpyDeviceToHost, s));
+    h.sync();
+    ids2.reserve(std::max<uint32_t>(E2, 1));
+    d2.reserve(std::max<uint32_t>(E2, 1));
+    M2.reserve(N);
+    KG_HIP(kg_prune2_copy(p, off2.p, ids2.p, d2.p, s));
+    KG_HIP(hipMemcpyAsync(M2.p, nl.p, 4ull * N, hipMemcpyDeviceToDevice, s));
+    h.sync();
+    replace_graph(h, N, off2, ids2, d2, M2, E2);
+}
+
+void prune1_index(Handle &h) {
     // prune1 (kgraph.cpp:584-590): keep the first M[i] edges
     hipStream_t s = h.stream;
     unsigned const N = h.gN;
@@ -377,6 +450,14 @@ void prune_index(Handle &h, unsigned level) {
     KG_HIP(hipMemcpyAsync(M.p, h.gM.p, 4ull * N, hipMemcpyDeviceToDevice, s));
     h.sync();
     replace_graph(h, N, off2, ids2, d2, M, E);
+}
+
+}  // namespace
+
+void prune_index(Handle &h, unsigned level) {
+    if (!h.has_graph) throw runtime_error("no index");
+    if (level & 1) prune1_index(h);
+    if (level & 2) prune2_index(h);
 }
 
 }  // namespace kga

@@ -1,6 +1,6 @@
 // kg_graph.hip -- whole-index kernels on the CSR graph: exclusive scan,
 // reverse-edge completion (KGraphImpl::reverse, kgraph.cpp:667-702) and prune
-// level 1 (kgraph.cpp:584-590).
+// levels 1 and 2 (kgraph.cpp:584-657).
 //
 // Reverse, rev = -1 keeps each node's first M[i] edges, rev > 0 its first
 // min(rev, deg) edges; every kept edge i -> e is stored at i and, with the same
@@ -215,7 +215,92 @@ __global__ void prune1_copy(GraphDev g, uint32_t const *off2, uint32_t *ids2, fl
     }
 }
 
+// Prune level 2 (kgraph.cpp:593-657), one level l per pass.  Position l of
+// node i (x = its l-th neighbour) survives iff
+//   (1) no node T kept by i at an earlier level has already kept x, and
+//   (2) no node T < i kept by i at an earlier level keeps the same x at level l.
+// This is what the reference's flags compute: a flag on (i, x) is set exactly
+// when some T kept by i before the visit of (i, l) had kept x before it --
+// forward marking when T kept x first, reverse marking through rev[T] when i
+// kept T first -- and within a level nodes are visited in id order, which is
+// where (2) comes from.  prune2_screen evaluates (1) for every active node;
+// prune2_resolve settles (2), whose dependencies point to smaller ids only,
+// in rounds (state 1 undecided, 2 pruned, 3 kept), and appends each kept x to
+// the node's kept list kl / kd at off[i] + nl[i].
+
+// 16 lanes per node: lanes stride over each kept T's own kept list.
+__global__ void prune2_screen(Prune2Dev p, unsigned l, unsigned nact) {
+    unsigned const g = (blockIdx.x * blockDim.x + threadIdx.x) >> 4, sl = threadIdx.x & 15;
+    if (g >= nact) return;  // a whole 16-lane group leaves together
+    unsigned const i = p.order[g];
+    uint32_t const base = p.off[i], x = p.ids[base + l], n = p.nl[i];
+    uint64_t const gmask = 0xffffull << (threadIdx.x & 48);
+    bool hit = false;
+    for (uint32_t t = 0; t < n && !hit; ++t) {
+        uint32_t const T = p.kl[base + t];
+        uint32_t const bT = p.off[T], nT = p.nl[T];
+        bool f = false;
+        for (uint32_t j = sl; j < nT; j += 16) f |= p.kl[bT + j] == x;
+        hit = (__ballot(f) & gmask) != 0;
+    }
+    if (sl == 0) p.st[i] = hit ? 2u : 1u;
+}
+
+__global__ void prune2_resolve(Prune2Dev p, unsigned l, unsigned nact, uint32_t *undecided) {
+    unsigned const g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nact) return;
+    unsigned const i = p.order[g];
+    if (__hip_atomic_load(p.st + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 1u) return;
+    uint32_t const base = p.off[i], x = p.ids[base + l], n = p.nl[i];
+    bool open = false, pruned = false;
+    for (uint32_t t = 0; t < n; ++t) {
+        uint32_t const T = p.kl[base + t];
+        if (T >= i || l >= p.Meff[T] || p.ids[p.off[T] + l] != x) continue;
+        uint32_t const s = __hip_atomic_load(p.st + T, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (s == 3u) {
+            pruned = true;
+            break;
+        }
+        open |= s == 1u;
+    }
+    if (pruned) {
+        __hip_atomic_store(p.st + i, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (!open) {
+        p.kl[base + n] = x;
+        p.kd[base + n] = p.dists ? p.dists[base + l] : 0.f;
+        p.nl[i] = n + 1;
+        __hip_atomic_store(p.st + i, 3u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        atomicAdd(undecided, 1u);
+    }
+}
+
+__global__ void prune2_copy(Prune2Dev p, uint32_t const *off2, uint32_t *ids2, float *dists2) {
+    unsigned const w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    if (w >= p.N) return;
+    unsigned const n = off2[w + 1] - off2[w];
+    for (unsigned j = lane; j < n; j += KG_WAVE) {
+        ids2[off2[w] + j] = p.kl[p.off[w] + j];
+        dists2[off2[w] + j] = p.kd[p.off[w] + j];
+    }
+}
+
 }  // namespace
+
+hipError_t kg_prune2_screen(Prune2Dev const &p, unsigned l, unsigned nact, hipStream_t s) {
+    hipLaunchKernelGGL(prune2_screen, dim3((nact + 15) / 16), dim3(256), 0, s, p, l, nact);
+    return hipGetLastError();
+}
+
+hipError_t kg_prune2_resolve(Prune2Dev const &p, unsigned l, unsigned nact, uint32_t *undecided, hipStream_t s) {
+    hipLaunchKernelGGL(prune2_resolve, dim3((nact + 255) / 256), dim3(256), 0, s, p, l, nact, undecided);
+    return hipGetLastError();
+}
+
+hipError_t kg_prune2_copy(Prune2Dev const &p, uint32_t const *off2, uint32_t *ids2, float *dists2, hipStream_t s) {
+    hipLaunchKernelGGL(prune2_copy, dim3((p.N + 3) / 4), dim3(256), 0, s, p, off2, ids2, dists2);
+    return hipGetLastError();
+}
 
 hipError_t kg_scan_u32(uint32_t const *in, uint32_t *out, unsigned n, uint32_t *tmp, hipStream_t s) {
     unsigned nb = (n + SB - 1) / SB;

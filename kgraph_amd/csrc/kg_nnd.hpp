@@ -70,3 +70,21 @@ hipError_t kg_compact_keys(unsigned N, uint32_t const *off2, uint64_t const *key
                            uint32_t *ids, float *dists, uint32_t *M, hipStream_t s);
 hipError_t kg_prune1_count(GraphDev g, uint32_t *cnt, hipStream_t s);
 hipError_t kg_prune1_copy(GraphDev g, uint32_t const *off2, uint32_t *ids2, float *dists2, hipStream_t s);
+// Prune level 2 (kgraph.cpp:593-657): per-level screen / resolve passes over
+// the nodes order[0..nact) (order lists nodes by min(M, deg) descending, so the
+// nodes with min(M, deg) > l are a prefix), then compaction of the kept lists.
+struct Prune2Dev {
+    unsigned N;
+    uint32_t const *off;    // N+1, current CSR
+    uint32_t const *ids;
+    float const *dists;
+    uint32_t const *Meff;   // min(M[i], deg[i])
+    uint32_t const *order;  // nodes by Meff descending
+    uint32_t *kl;           // kept ids at off[i] + [0, nl[i])
+    float *kd;              // their distances
+    uint32_t *nl;           // kept count per node
+    uint32_t *st;           // per-level state: 1 undecided, 2 pruned, 3 kept
+};
+hipError_t kg_prune2_screen(Prune2Dev const &p, unsigned l, unsigned nact, hipStream_t s);
+hipError_t kg_prune2_resolve(Prune2Dev const &p, unsigned l, unsigned nact, uint32_t *undecided, hipStream_t s);
+hipError_t kg_prune2_copy(Prune2Dev const &p, uint32_t const *off2, uint32_t *ids2, float *dists2, hipStream_t s);

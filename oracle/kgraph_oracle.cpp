@@ -495,8 +495,11 @@ struct Index {  // KGraphImpl state (kgraph.cpp:276-286)
     void prune2() {
         vector<vector<unsigned>> rev(graph.size());
         vector<unsigned> new_L(graph.size(), 0);
-        for (unsigned i = 0; i < graph.size(); ++i)
+        for (unsigned i = 0; i < graph.size(); ++i) {
             for (auto &e : graph[i]) e.flag = false;
+            // the reference requires M[i] <= list size (BOOST_VERIFY, kgraph.cpp:616)
+            if (M[i] > graph[i].size()) M[i] = unsigned(graph[i].size());
+        }
         vector<unsigned> todo(graph.size());
         for (unsigned i = 0; i < todo.size(); ++i) todo[i] = i;
         vector<unsigned> new_todo;

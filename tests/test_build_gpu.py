@@ -1,7 +1,7 @@
 """GPU NN-descent build: recall against exact K-NN matches the oracle's at the
 same IndexParams, every stored distance is the canonical FP32 value, list
-invariants hold, and reverse-edge completion / prune level 1 equal the
-oracle's bit-exactly on the same graph."""
+invariants hold, and reverse-edge completion / prune levels 1 and 2 equal
+the oracle's bit-exactly on the same graph."""
 import numpy as np
 import pytest
 
@@ -130,6 +130,43 @@ def test_prune1_matches_oracle():
     g.prune(1)
     for a, b in zip(g.graph(), ref.export()):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("level,rev", [(2, 0), (3, 0), (2, -1)])
+def test_prune2_matches_oracle(level, rev):
+    X = _data("uniform", 6000, 8)
+    g = _gpu(X, L2)
+    g.build(K=10, L=30, iterations=8, controls=20, reverse=rev)
+    off, ids, d, M = g.graph()
+    ref = kgo.Index()
+    ref.import_(off, ids, d, M)
+    ref.prune(level)
+    g.prune(level)
+    got, exp = g.graph(), ref.export()
+    for a, b in zip(got, exp):
+        assert np.array_equal(a, b)
+    assert got[3].sum() < M.sum()  # edges were removed
+
+
+def test_prune2_handmade_graphs():
+    # the triangle case and M beyond the list size, on a set graph
+    X = _data("uniform", 80, 4)
+    cases = [([[1, 2], [2], []] + [[]] * 77, [2, 1, 0] + [0] * 77)]
+    rs = np.random.RandomState(3)
+    ids, _ = kgo.linear_knn((X * 4).round(), np.arange(80), 15, L2)
+    cases.append(([ids[i].tolist() for i in range(80)], rs.randint(0, 18, size=80).tolist()))
+    for lists, M in cases:
+        off = np.cumsum([0] + [len(l) for l in lists]).astype(np.uint64)
+        flat = np.array([j for l in lists for j in l], dtype=np.uint32)
+        dd = np.arange(len(flat), dtype=np.float32)
+        g = _gpu(X, L2)
+        g.set_graph(off, flat, dd, np.array(M, dtype=np.uint32))
+        ref = kgo.Index()
+        ref.import_(off, flat, dd, np.array(M, dtype=np.uint32))
+        ref.prune(2)
+        g.prune(2)
+        for a, b in zip(g.graph(), ref.export()):
+            assert np.array_equal(a, b)
 
 
 def test_build_then_search_end_to_end():

@@ -243,6 +243,70 @@ def test_reverse_set_semantics():
         assert rM[i] == len(exp)
 
 
+def _prune2_model(lists, M):
+    """Prune level 2 stated without flags or reverse lists: entry x at position
+    l of node i survives iff (1) no node T kept by i at an earlier level has
+    already kept x, and (2) no node T < i kept by i at an earlier level keeps
+    the same x at level l.  Nodes are visited level by level in id order."""
+    N = len(lists)
+    Meff = [min(int(M[i]), len(lists[i])) for i in range(N)]
+    kept = [[] for _ in range(N)]
+    for l in range(max(Meff) if N else 0):
+        now = {}
+        for i in range(N):
+            if l >= Meff[i]:
+                continue
+            x = lists[i][l]
+            pruned = any(x in kept[T] for T in kept[i])
+            pruned = pruned or any(T < i and now.get(T, False) and lists[T][l] == x for T in kept[i])
+            now[i] = not pruned
+        for i, keep in now.items():
+            if keep:
+                kept[i].append(lists[i][l])
+    return kept
+
+
+def _prune2_oracle(lists, M, level=2):
+    N = len(lists)
+    off = np.cumsum([0] + [len(l) for l in lists]).astype(np.uint64)
+    ids = np.array([j for l in lists for j in l], dtype=np.uint32)
+    d = np.arange(len(ids), dtype=np.float32)  # distances ride along with their ids
+    idx = kgo.Index()
+    idx.import_(off, ids, d, np.asarray(M, dtype=np.uint32))
+    idx.prune(level)
+    o, pi, pd, pM = idx.export()
+    out = [pi[o[i]:o[i + 1]].tolist() for i in range(N)]
+    for i in range(N):  # each kept entry keeps its own distance
+        src = {int(a): float(b) for a, b in zip(ids[off[i]:off[i + 1]], d[off[i]:off[i + 1]])}
+        assert [src[j] for j in out[i]] == pd[o[i]:o[i + 1]].tolist()
+        assert pM[i] == len(out[i])
+    return out
+
+
+def test_prune2_triangle_known_answer():
+    # 0 -> [1, 2], 1 -> [2], 2 -> []: 0 keeps 1, 1 keeps 2, so 0 -> 2 is
+    # reachable through 1 and is removed
+    assert _prune2_oracle([[1, 2], [2], []], [2, 1, 0]) == [[1], [2], []]
+    # with 1 -> 2 outside M[1] the edge 0 -> 2 stays
+    assert _prune2_oracle([[1, 2], [2], []], [2, 0, 0]) == [[1, 2], [], []]
+    # same level: 1, 2 and 3 keep 0 at level 0; at level 1 node 0 keeps 3, so
+    # nodes 1 and 2 (> 0, and they kept 0 earlier) lose their own level-1 entry 3
+    assert _prune2_oracle([[1, 3], [0, 3], [0, 3], [0]], [2, 2, 2, 1]) == [[1, 3], [0], [0], [0]]
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_prune2_matches_flag_free_model(seed):
+    rs = np.random.RandomState(40 + seed)
+    N = 80
+    X = rs.randint(0, 16, size=(N, 4)).astype(np.float32)  # many shared neighbours
+    ids, d = kgo.linear_knn(X, np.arange(N), 15, L2)
+    lists = [ids[i].tolist() for i in range(N)]
+    M = rs.randint(0, 18, size=N)  # M beyond the list size is clamped
+    got = _prune2_oracle(lists, M)
+    assert got == _prune2_model(lists, M)
+    assert sum(len(k) for k in got) < sum(min(m, 15) for m in M)  # something was pruned
+
+
 def _fixture_bytes(graph, M, fmt):
     b = b"KNNGRAPH" + struct.pack("<III", 2, fmt, len(graph))
     for i, lst in enumerate(graph):

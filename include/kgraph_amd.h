@@ -215,7 +215,9 @@ int kg_search_latencies(kg_handle *h, float *out_us, unsigned n);
  * the reference): out[0] pairs scored (the reference's n_comps,
  * kgraph.cpp:805, 814), out[1] join-list entries (sum over nodes of
  * |new| + |old|), out[2] candidates pushed to merge slots, out[3]
- * iterations.  n = number of entries wanted (<= 4). */
+ * iterations, out[4] local-join form (0 FP32 quads, 1 BF16 MFMA screen with
+ * exact FP32 re-score, 2 BF16 MFMA exact on these rows).  n = number of
+ * entries wanted (<= 5). */
 int kg_build_counters(kg_handle *h, uint64_t *out, unsigned n);
 
 #ifdef __cplusplus

@@ -232,6 +232,15 @@ class KGraph:
         self._h.call("kg_last_time_ms", which, _lib.ctypes.byref(v))
         return v.value
 
+    def build_counters(self):
+        """Last build: {'pairs', 'list_entries', 'pushes', 'iterations',
+        'join_form'} with join_form 'fp32', 'bf16-screen' or 'bf16-exact'."""
+        out = np.zeros(5, dtype=np.uint64)
+        self._h.call("kg_build_counters", ptr(out), 5)
+        forms = {0: "fp32", 1: "bf16-screen", 2: "bf16-exact"}
+        return {"pairs": int(out[0]), "list_entries": int(out[1]), "pushes": int(out[2]),
+                "iterations": int(out[3]), "join_form": forms.get(int(out[4]), "fp32")}
+
     def build_trace(self):
         n = _lib.ctypes.c_uint()
         self._h.call("kg_build_trace", None, 0, _lib.ctypes.byref(n))

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <memory>
 #include <random>
 
@@ -57,6 +58,8 @@ struct NndState {
     DevBuf<float> radius, radiusM, kth;
     DevBuf<uint32_t> ctl, ccount;
     DevBuf<float> cdist;
+    DevBuf<float2> sqn;
+    DevBuf<uint32_t> flags;
 };
 
 void replace_graph(Handle &h, unsigned N, DevBuf<uint32_t> &off, DevBuf<uint32_t> &ids, DevBuf<float> &dists,
@@ -140,6 +143,26 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     a.radius = st.radius.p;
     a.radiusM = st.radiusM.p;
     a.kth = st.kth.p;
+    if (h.precision == KG_PRECISION_BF16 && h.D <= 256) {
+        // local join as BF16 MFMA tiles; exact as it is on integer rows with
+        // 2 D max|x|^2 <= 2^24, else a screen with an exact FP32 re-score
+        st.sqn.reserve(N);
+        st.flags.reserve(2);
+        KG_HIP(hipMemsetAsync(st.flags.p, 0, 8, s));
+        KG_HIP(nnd_launch_sqn(a, st.sqn.p, st.flags.p, s));
+        uint32_t fl[2];
+        KG_HIP(hipMemcpyAsync(fl, st.flags.p, 8, hipMemcpyDeviceToHost, s));
+        h.sync();
+        float mx;
+        memcpy(&mx, &fl[1], 4);
+        a.sqn = st.sqn.p;
+        a.mfma = 1;
+        a.exact = (fl[0] == 0 && 2.0 * h.D * double(mx) * double(mx) <= 16777216.0) ? 1u : 0u;
+        double const gam = (h.D + 16) * std::ldexp(1.0, -23);
+        a.scr_c = float((std::ldexp(1.0, -8) + std::ldexp(1.0, -17) + gam) * 1.02);
+        a.scr_g = float(4 * gam);
+    }
+    h.join_exact = a.mfma ? int(a.exact) : -1;
     KG_HIP(hipMemsetAsync(st.cand.p, 0xff, size_t(N) * a.C * 8, s));
     KG_HIP(hipMemsetAsync(st.found.p, 0, size_t(N) * 4, s));
 

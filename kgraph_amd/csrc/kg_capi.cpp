@@ -915,6 +915,7 @@ int kg_build_counters(kg_handle *k, uint64_t *out, unsigned n) {
     return guard(k, [&](Handle &h) {
         if (!out) throw kga::invalid_argument("null argument");
         for (unsigned i = 0; i < n && i < 4; ++i) out[i] = h.build_counters[i];
+        if (n > 4) out[4] = uint64_t(h.join_exact + 1);
     });
 }
 

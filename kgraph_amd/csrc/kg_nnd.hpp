@@ -36,10 +36,18 @@ struct NndArgs {
     uint64_t *cand;           // N x C candidate slots (~0 = empty)
     uint32_t *ccnt;           // candidate slot counters
     uint64_t *stats;          // NND_NSTATS counters
+    // MFMA join (join_mfma_kernel)
+    float2 const *sqn;  // per row: (|x|^2, |x|)
+    unsigned mfma;      // 1: tile join on the matrix cores
+    unsigned exact;     // 1: BF16 tiles are exact for these rows (no re-score)
+    float scr_c, scr_g; // screen bound: relative dot error, relative norm error
 };
 
 hipError_t nnd_launch_init(int metric, NndArgs const &a, hipStream_t s);
 hipError_t nnd_launch_join(int metric, NndArgs const &a, hipStream_t s);
+// (|x|^2, |x|) per row into out; flags[0] |= 1 if any value is not an integer,
+// flags[1] = max |x| as float bits (both zeroed by the caller).
+hipError_t nnd_launch_sqn(NndArgs const &a, float2 *out, uint32_t *flags, hipStream_t s);
 hipError_t nnd_launch_merge(NndArgs const &a, hipStream_t s);
 hipError_t nnd_launch_update(NndArgs const &a, hipStream_t s);
 hipError_t nnd_launch_extract(NndArgs const &a, uint32_t const *off, uint32_t *ids, float *dists, uint32_t *M,

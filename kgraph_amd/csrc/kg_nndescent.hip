@@ -347,6 +347,302 @@ __global__ void __launch_bounds__(256) join_kernel(NndArgs a) {
     }
 }
 
+// ------------------------------------------------------------------ MFMA join
+// The local join as dense distance tiles on the matrix cores.  One wavefront
+// per node; the node's list (new rows, then old rows) is in LDS.  Every 16 x 16
+// block of (new row, list row) pairs is one tile: both operands go from HBM
+// straight into MFMA operand registers as BF16 (operand row = lane % 16,
+// columns 16 s + 4 (lane / 16) .. +3 in k-step s), v_mfma_f32_16x16x16_bf16
+// accumulates the dot products in FP32, and each lane finishes the four pairs
+// it holds, C[4 (lane / 16) + j][lane % 16], from the row norms:
+//   L2SQR  d = |a|^2 + |o|^2 - 2 a.o,   EUCLIDEAN  d = n_a + n_o - a.o,
+//   ANGULAR  d = -a.o / fmaf(n_a, n_o, FLT_EPSILON).
+// EXACT (every row value an integer and 2 D max|x|^2 <= 2^24): BF16 holds each
+// value, and every product, partial sum and norm is an integer at most 2^24,
+// so the tile value equals the canonical FP32 distance bit for bit and is used
+// as it is.  Otherwise the tile value screens: a pair is re-scored exactly
+// (canonical FP32 order, kg_device.hpp) when d - E is below either endpoint's
+// radius, E bounding |screen - canonical| (BF16 rounding of both operands, FP32
+// accumulation, the norms' rounding), so no pair that passes the exact test is
+// screened out and the re-scored distance alone decides.  Candidates are staged
+// in LDS and flushed grouped by target as in join_kernel.
+typedef short mj_bf16x4 __attribute__((ext_vector_type(4)));
+typedef float mj_f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr unsigned MT = 512;  // list capacity (new + old)
+constexpr unsigned MB = 512;  // staged candidates per wave
+constexpr unsigned MQ = 512;  // re-score queue per wave (screened form)
+
+template <unsigned Q>
+struct MJoinLDS {
+    uint32_t ids[MT];
+    uint32_t cnt[MT];
+    uint64_t key[MB];
+    uint32_t rq[Q];  // pair (list index of the new row) | (list index of the other row) << 16
+    uint16_t tgt[MB];
+};
+
+__device__ __forceinline__ short bf16_rn(float f) {  // round to nearest even (finite values)
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return short(u >> 16);
+}
+
+template <int KS>
+__device__ __forceinline__ void mj_load(NndArgs const &a, uint32_t id, bool ok, unsigned kq, mj_bf16x4 (&op)[KS]) {
+    float const *r = a.X + size_t(ok ? id : 0u) * a.Dp;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        unsigned const k = 16 * s + kq;
+        float4 v = (ok && k < a.Dp) ? *reinterpret_cast<float4 const *>(r + k) : make_float4(0.f, 0.f, 0.f, 0.f);
+        op[s][0] = bf16_rn(v.x);
+        op[s][1] = bf16_rn(v.y);
+        op[s][2] = bf16_rn(v.z);
+        op[s][3] = bf16_rn(v.w);
+    }
+}
+
+template <class LDS>
+__device__ void mj_flush(NndArgs const &a, LDS &L, unsigned fill, unsigned nT) {
+    unsigned const lane = lane_id();
+    for (unsigned i = lane; i < nT; i += KG_WAVE) L.cnt[i] = 0;
+    wave_sync();
+    unsigned rank[MB / KG_WAVE];
+#pragma unroll
+    for (unsigned j = 0; j < MB / KG_WAVE; ++j) {
+        unsigned const i = lane + j * KG_WAVE;
+        rank[j] = (i < fill) ? atomicAdd(&L.cnt[L.tgt[i]], 1u) : 0u;
+    }
+    wave_sync();
+    for (unsigned i = lane; i < nT; i += KG_WAVE) {
+        unsigned const c = L.cnt[i];
+        if (c) L.cnt[i] = atomicAdd(&a.ccnt[L.ids[i]], c);
+    }
+    wave_sync();
+#pragma unroll
+    for (unsigned j = 0; j < MB / KG_WAVE; ++j) {
+        unsigned const i = lane + j * KG_WAVE;
+        if (i < fill) {
+            unsigned const tp = L.tgt[i];
+            uint32_t const target = L.ids[tp];
+            uint64_t const key = L.key[i];
+            unsigned slot = L.cnt[tp] + rank[j];
+            if (slot >= a.C) slot = unsigned(mix64((uint64_t(target) << 32) | (uint32_t(key) >> 1)) % a.C);
+            atomicMin(reinterpret_cast<unsigned long long *>(&a.cand[size_t(target) * a.C + slot]),
+                      static_cast<unsigned long long>(key));
+        }
+    }
+    wave_sync();
+}
+
+// Stages up to NP candidates per lane (bit k of `pm`: key[k] for list entry
+// tgt[k]); the wave first counts them and flushes the buffer if they would not
+// fit.  `fill` is wave-uniform; every lane of the wave calls this.
+template <int NP, class LDS>
+__device__ __forceinline__ void mj_stage(NndArgs const &a, LDS &L, unsigned &fill, unsigned nT, unsigned pm,
+                                         uint64_t const (&key)[NP], unsigned const (&tgt)[NP]) {
+    uint64_t bal[NP];
+    unsigned tot = 0;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        bal[k] = ballot((pm >> k) & 1u);
+        tot += popc64(bal[k]);
+    }
+    if (tot == 0) return;
+    if (fill + tot > MB) {
+        wave_sync();
+        mj_flush(a, L, fill, nT);
+        fill = 0;
+    }
+    uint64_t const lt = (1ull << lane_id()) - 1;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        if ((pm >> k) & 1u) {
+            unsigned const pos = fill + popc64(bal[k] & lt);
+            L.key[pos] = key[k];
+            L.tgt[pos] = uint16_t(tgt[k]);
+        }
+        fill += popc64(bal[k]);
+    }
+}
+
+// Exact re-score of the queued pairs, one quad per pair.
+template <int METRIC, class LDS>
+__device__ void mj_rescore(NndArgs const &a, LDS &L, unsigned rqn, unsigned &fill, unsigned nT, bool &found,
+                           unsigned &pushes) {
+    unsigned const quad = lane_id() >> 2, t = lane_id() & 3;
+    wave_sync();
+    for (unsigned b0 = 0; b0 < rqn; b0 += 16) {
+        unsigned const e = b0 + quad;
+        bool const act = e < rqn;
+        uint32_t const pr = act ? L.rq[e] : 0u;
+        unsigned const ia = pr & 0xffffu, io = pr >> 16;
+        uint32_t const av = L.ids[ia], o = L.ids[io];
+        float const acc = quad_metric<METRIC>(a.X + size_t(av) * a.Dp, a.X + size_t(o) * a.Dp, a.D, t);
+        float const na = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[av];
+        float const no = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[o];
+        float const d = finish_metric<METRIC>(acc, na, no);
+        bool const lead = act && t == 0;
+        bool const pa = lead && d < a.radius[av];
+        bool const po = lead && d < a.radius[o];
+        found |= lead && d < a.kth[av];
+        pushes += unsigned(pa) + unsigned(po);
+        uint64_t const key[2] = {make_key(d, o, 1), make_key(d, av, 1)};
+        unsigned const tgt[2] = {ia, io};
+        mj_stage<2>(a, L, fill, nT, unsigned(pa) | (unsigned(po) << 1), key, tgt);
+    }
+    wave_sync();
+}
+
+template <int METRIC, int KS, bool EXACT>
+__global__ void __launch_bounds__(256) join_mfma_kernel(NndArgs a) {
+    __shared__ MJoinLDS<EXACT ? 1u : MQ> lds[4];
+    unsigned const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned const n = blockIdx.x * 4 + w;
+    if (n >= a.N) return;
+    auto &L = lds[w];
+    unsigned const nn = a.nnew[n], no = a.nold[n];
+    unsigned const rn = min(a.crnew[n], a.R), ro = min(a.crold[n], a.R);
+    unsigned const nA0 = nn + rn, nB = no + ro, nT = min(nA0 + nB, MT), nA = min(nA0, nT);
+    uint32_t const *samp = a.samp + size_t(n) * a.Lc;
+    for (unsigned i = lane; i < nT; i += KG_WAVE) {
+        uint32_t v;
+        if (i < nn)
+            v = samp[i];
+        else if (i < nA0)
+            v = a.rnew[size_t(n) * a.R + (i - nn)];
+        else if (i < nA0 + no)
+            v = samp[a.Lc - 1 - (i - nA0)];
+        else
+            v = a.rold[size_t(n) * a.R + (i - nA0 - no)];
+        L.ids[i] = v;
+    }
+    wave_sync();
+    unsigned const r16 = lane & 15, kq = (lane >> 4) * 4, mrow = (lane >> 4) * 4;
+    unsigned fill = 0, rqn = 0, pairs = 0, pushes = 0;
+    bool found = false;
+    for (unsigned ag = 0; ag < nA; ag += 16) {
+        mj_bf16x4 aop[KS];
+        {
+            unsigned const ia = ag + r16;
+            mj_load<KS>(a, ia < nA ? L.ids[ia] : 0u, ia < nA, kq, aop);
+        }
+        // the four new rows whose pairs this lane finishes
+        uint32_t aid[4];
+        float arad[4], akth[4], an[4], arn[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            unsigned const ia = ag + mrow + j;
+            bool const ok = ia < nA;
+            aid[j] = ok ? L.ids[ia] : 0u;
+            arad[j] = ok ? a.radius[aid[j]] : 0.f;
+            akth[j] = ok ? a.kth[aid[j]] : 0.f;
+            float2 const sq = a.sqn[aid[j]];
+            an[j] = (METRIC == METRIC_L2SQR) ? sq.x : a.norms[aid[j]];
+            arn[j] = sq.y;
+        }
+        for (unsigned tg = 0; tg < nT; tg += 16) {
+            unsigned const io = tg + r16;
+            bool const ook = io < nT;
+            uint32_t const oid = ook ? L.ids[io] : 0u;
+            mj_bf16x4 bop[KS];
+            mj_load<KS>(a, oid, ook, kq, bop);
+            float const orad = a.radius[oid];
+            float2 const osq = a.sqn[oid];
+            float const on = (METRIC == METRIC_L2SQR) ? osq.x : a.norms[oid];
+            mj_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < KS; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aop[s], bop[s], acc, 0, 0, 0);
+            unsigned pm = 0;
+            uint64_t key[8];
+            unsigned tgt[8];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                unsigned const ia = ag + mrow + j;
+                // new x new once per id pair (i < j by id), new x old always (kgraph.cpp:741-750)
+                bool const valid = ia < nA && ook && (io < nA ? aid[j] < oid : aid[j] != oid);
+                float const G = acc[j];
+                float d;
+                if (METRIC == METRIC_L2SQR)
+                    d = (an[j] + on) - 2.f * G;
+                else if (METRIC == METRIC_EUCLIDEAN)
+                    d = (an[j] + on) - G;
+                else
+                    d = -G / __builtin_fmaf(an[j], on, 1.1920928955078125e-07f);
+                pairs += valid;
+                if (EXACT) {
+                    bool const pa = valid && d < arad[j];
+                    bool const po = valid && d < orad;
+                    found |= valid && d < akth[j];
+                    pushes += unsigned(pa) + unsigned(po);
+                    pm |= (unsigned(pa) << (2 * j)) | (unsigned(po) << (2 * j + 1));
+                    key[2 * j] = make_key(d, oid, 1);
+                    key[2 * j + 1] = make_key(d, aid[j], 1);
+                    tgt[2 * j] = ia;
+                    tgt[2 * j + 1] = io;
+                } else {
+                    float E;
+                    if (METRIC == METRIC_L2SQR)
+                        E = 2.f * a.scr_c * arn[j] * osq.y + a.scr_g * (an[j] + on);
+                    else if (METRIC == METRIC_EUCLIDEAN)
+                        E = a.scr_c * arn[j] * osq.y + a.scr_g * (an[j] + on);
+                    else
+                        E = a.scr_c + 2.f * a.scr_g;
+                    float const lo = d - E;
+                    bool const pass = valid && (lo < arad[j] || lo < orad);
+                    uint64_t const b = ballot(pass);
+                    if (pass) L.rq[rqn + popc64(b & ((1ull << lane) - 1))] = ia | (io << 16);
+                    rqn += popc64(b);
+                }
+            }
+            if (EXACT) mj_stage<8>(a, L, fill, nT, pm, key, tgt);
+            if (!EXACT && rqn > MQ - 4 * KG_WAVE) {
+                mj_rescore<METRIC>(a, L, rqn, fill, nT, found, pushes);
+                rqn = 0;
+            }
+        }
+    }
+    if (!EXACT && rqn) mj_rescore<METRIC>(a, L, rqn, fill, nT, found, pushes);
+    wave_sync();
+    if (fill) mj_flush(a, L, fill, nT);
+    uint64_t const fb = ballot(found);
+    for (int o = 32; o > 0; o >>= 1) {
+        pairs += __shfl_xor(pairs, o);
+        pushes += __shfl_xor(pushes, o);
+    }
+    if (lane == 0) {
+        a.found[n] = fb ? 1u : 0u;
+        auto *st = reinterpret_cast<unsigned long long *>(a.stats);
+        if (pairs) atomicAdd(st + NND_STAT_PAIRS, (unsigned long long)pairs);
+        if (pushes) atomicAdd(st + NND_STAT_PUSHES, (unsigned long long)pushes);
+        atomicAdd(st + NND_STAT_LISTS, (unsigned long long)(nA0 + nB));
+    }
+}
+
+// Row norms for the MFMA join: (|x|^2 in canonical order, its square root),
+// and whether every value is an integer (flags[0]) with max |x| (flags[1],
+// float bits) -- what decides the exact form.
+__global__ void __launch_bounds__(256) sqn_kernel(float const *X, unsigned N, unsigned D, unsigned Dp, float2 *out,
+                                                  uint32_t *flags) {
+    unsigned const lane = threadIdx.x & 63, t = lane & 3;
+    unsigned const row = (blockIdx.x * blockDim.x + threadIdx.x) >> 2;
+    bool const ok = row < N;
+    float const *r = X + size_t(ok ? row : 0u) * Dp;
+    float const n2 = quad_metric<METRIC_EUCLIDEAN>(r, r, D, t);
+    bool frac = false;
+    float mx = 0.f;
+    if (ok)
+        for (unsigned i = t; i < D; i += 4) {
+            float const v = r[i];
+            frac |= !(v == rintf(v));
+            mx = fmaxf(mx, fabsf(v));
+        }
+    if (ok && t == 0) out[row] = make_float2(n2, __fsqrt_rn(n2));
+    if (ballot(frac) && lane == 0) atomicOr(flags, 1u);
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    if (lane == 0) atomicMax(flags + 1, __float_as_uint(mx));
+}
+
 // ------------------------------------------------------------------ merge
 __device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
     uint32_t lo = __shfl_xor(uint32_t(v), m), hi = __shfl_xor(uint32_t(v >> 32), m);
@@ -638,7 +934,44 @@ static void launch_join_m(NndArgs const &a, hipStream_t s) {
     }
 }
 
+template <int METRIC, bool EXACT>
+static void launch_join_mfma_e(NndArgs const &a, hipStream_t s) {
+    dim3 g((a.N + 3) / 4), b(256);
+    unsigned const ks = (a.D + 15) / 16;
+    if (ks <= 2)
+        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 2, EXACT>), g, b, 0, s, a);
+    else if (ks <= 4)
+        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 4, EXACT>), g, b, 0, s, a);
+    else if (ks <= 8)
+        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 8, EXACT>), g, b, 0, s, a);
+    else
+        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 16, EXACT>), g, b, 0, s, a);
+}
+
+template <int METRIC>
+static void launch_join_mfma(NndArgs const &a, hipStream_t s) {
+    if (a.exact)
+        launch_join_mfma_e<METRIC, true>(a, s);
+    else
+        launch_join_mfma_e<METRIC, false>(a, s);
+}
+
+hipError_t nnd_launch_sqn(NndArgs const &a, float2 *out, uint32_t *flags, hipStream_t s) {
+    hipLaunchKernelGGL(sqn_kernel, dim3((a.N + 63) / 64), dim3(256), 0, s, a.X, a.N, a.D, a.Dp, out, flags);
+    return hipGetLastError();
+}
+
 hipError_t nnd_launch_join(int metric, NndArgs const &a, hipStream_t s) {
+    if (a.mfma) {
+        if (a.D > 256) return hipErrorInvalidValue;
+        if (metric == METRIC_L2SQR)
+            launch_join_mfma<METRIC_L2SQR>(a, s);
+        else if (metric == METRIC_EUCLIDEAN)
+            launch_join_mfma<METRIC_EUCLIDEAN>(a, s);
+        else
+            launch_join_mfma<METRIC_ANGULAR>(a, s);
+        return hipGetLastError();
+    }
     if (metric == METRIC_L2SQR)
         launch_join_m<METRIC_L2SQR>(a, s);
     else if (metric == METRIC_EUCLIDEAN)

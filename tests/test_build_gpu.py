@@ -67,6 +67,39 @@ def test_build_recall_matches_oracle(kind, metric, d, params):
     assert len(tr) == info.iterations and tr[-1, 0] >= tr[0, 0]
 
 
+@pytest.mark.parametrize("kind,metric,d,form", [
+    ("intmix", L2, 128, "bf16-exact"),      # integers <= 255: BF16 tiles are exact
+    ("intmix", EUC, 64, "bf16-exact"),
+    ("uniform", EUC, 16, "bf16-screen"),    # FP32 data: screen + exact re-score
+    ("angular", ANG, 96, "bf16-screen"),
+])
+def test_mfma_join_recall_and_canonical_distances(kind, metric, d, form):
+    from kgraph_amd import PRECISION_BF16
+    X = _data(kind, 20000, d)
+    N, K = len(X), 16
+    params = dict(K=K, L=32, S=10, R=100, iterations=12, controls=100, reverse=0)
+    rows = np.arange(0, N, 7)
+    _, gt_d = kgo.linear_knn(X, rows, K, metric)
+    g32 = _gpu(X, metric)
+    g32.build(**params)
+    off, ids, dd, M = g32.graph()
+    r32 = _knn_recall(off[rows], dd, gt_d, K)
+    g = _gpu(X, metric)
+    info = g.build(precision=PRECISION_BF16, **params)
+    c = g.build_counters()
+    assert c["join_form"] == form
+    off, ids, dd, M = g.graph()
+    r16 = _knn_recall(off[rows], dd, gt_d, K)
+    print(f"{kind}/{metric}: fp32 join recall {r32:.4f}, {form} recall {r16:.4f} ({info.iterations} it)")
+    assert r16 >= r32 - 0.005
+    # every stored distance is the canonical FP32 value, lists sorted and unique
+    for i in range(0, N, 61):
+        lst, dl = ids[off[i]:off[i + 1]], dd[off[i]:off[i + 1]]
+        assert i not in lst and len(np.unique(lst)) == len(lst) and (np.diff(dl) >= 0).all()
+        for j, v in zip(lst, dl):
+            assert v == kgo.dist(metric, X[i], X[j])
+
+
 def test_build_stop_conditions_and_errors():
     X = _data("uniform", 5000, 8)
     g = _gpu(X, EUC)

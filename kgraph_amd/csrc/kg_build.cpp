@@ -53,8 +53,8 @@ float eval_accuracy(float const *pool, unsigned np, float const *knn, unsigned n
 }
 
 struct NndState {
-    DevBuf<uint64_t> pool, cand, stats;
-    DevBuf<uint32_t> cnt, Mv, found, samp, nnew, nold, rnew, rold, crnew, crold, ccnt;
+    DevBuf<uint64_t> pool, cand, ovf, stats;
+    DevBuf<uint32_t> part, cnt, Mv, found, samp, nnew, nold, rnew, rold, crnew, crold, ccnt;
     DevBuf<float> radius, radiusM, kth;
     DevBuf<uint32_t> ctl, ccount;
     DevBuf<float> cdist;
@@ -120,7 +120,10 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     NndState &st = *stp;
     st.pool.reserve(size_t(N) * p.L);
     st.cand.reserve(size_t(N) * a.C);
+    a.O = 64;
+    st.ovf.reserve(size_t(N) * a.O);
     st.stats.reserve(NND_NSTATS);
+    st.part.reserve(size_t(NND_NPART) * N);
     for (auto *b : {&st.cnt, &st.Mv, &st.found, &st.nnew, &st.nold, &st.crnew, &st.crold, &st.ccnt}) b->reserve(N);
     st.samp.reserve(size_t(N) * a.Lc);
     st.rnew.reserve(size_t(N) * R);
@@ -128,7 +131,9 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     for (auto *b : {&st.radius, &st.radiusM, &st.kth}) b->reserve(N);
     a.pool = st.pool.p;
     a.cand = st.cand.p;
+    a.ovf = st.ovf.p;
     a.stats = st.stats.p;
+    a.part = st.part.p;
     a.cnt = st.cnt.p;
     a.Mv = st.Mv.p;
     a.found = st.found.p;
@@ -163,7 +168,7 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
         a.scr_g = float(4 * gam);
     }
     h.join_exact = a.mfma ? int(a.exact) : -1;
-    KG_HIP(hipMemsetAsync(st.cand.p, 0xff, size_t(N) * a.C * 8, s));
+    KG_HIP(hipMemsetAsync(st.ovf.p, 0xff, size_t(N) * a.O * 8, s));
     KG_HIP(hipMemsetAsync(st.found.p, 0, size_t(N) * 4, s));
 
     KG_HIP(hipEventRecord(h.ev[4], s));

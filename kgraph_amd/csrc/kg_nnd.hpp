@@ -8,7 +8,10 @@
 #define NND_MAX_S 255  // largest sample size S
 #define NND_MERGE_MAX 1024u  // pool + candidates sorted per node per merge
 
-enum { NND_STAT_PAIRS = 0, NND_STAT_DELTA = 1, NND_STAT_M = 2, NND_STAT_LISTS = 3, NND_STAT_PUSHES = 4, NND_NSTATS = 8 };
+// Iteration statistics: per node in `part` (column k at part[k * N + n]),
+// summed into `stats` after the merge.
+enum { NND_STAT_PAIRS = 0, NND_STAT_DELTA = 1, NND_STAT_M = 2, NND_STAT_LISTS = 3, NND_STAT_PUSHES = 4, NND_NPART = 5,
+       NND_NSTATS = 8 };
 
 struct NndArgs {
     float const *X;      // rows, stride Dp floats
@@ -33,9 +36,12 @@ struct NndArgs {
     uint32_t *nnew, *nold;
     uint32_t *rnew, *rold;    // N x R reverse lists
     uint32_t *crnew, *crold;  // reverse-list push counters
-    uint64_t *cand;           // N x C candidate slots (~0 = empty)
+    uint64_t *cand;           // N x C candidate slots, written once per claim
+    uint64_t *ovf;            // N x O overflow slots (~0 = empty), hashed, atomicMin
+    unsigned O;
     uint32_t *ccnt;           // candidate slot counters
     uint64_t *stats;          // NND_NSTATS counters
+    uint32_t *part;           // NND_NPART x N per-node statistics
     // MFMA join (join_mfma_kernel)
     float2 const *sqn;  // per row: (|x|^2, |x|)
     unsigned mfma;      // 1: tile join on the matrix cores

@@ -8,10 +8,13 @@
 // bounded sorted merge, which is what these kernels do:
 //
 //   init_kernel    kgraph.cpp:760-796  S random pool entries + 2S random "new"
-//   join_kernel    kgraph.cpp:739-751, 797-815  every new x new (by id, i < j)
-//                  and new x old pair; candidates below the target's radius go
-//                  to the target's candidate slots (atomic slot claim, then a
-//                  64-bit atomicMin so an overflowing slot keeps the better key)
+//   join_tile_kernel / join_mfma_kernel  kgraph.cpp:739-751, 797-815  every
+//                  new x new (by id, i < j) and new x old pair, scored as
+//                  16 x 16 tiles (exact FP32 quads, or BF16 MFMA); candidates
+//                  below the target's radius are staged in LDS and written to
+//                  the target's claimed candidate slots (one slot-claim atomic
+//                  per target and flush; claims past C go to a hashed
+//                  overflow area where a 64-bit atomicMin keeps the better key)
 //   merge_kernel   kgraph.cpp:176-202, 723-736  sorted pool U candidates ->
 //                  first L distinct ids (equal distance: lower id first; an
 //                  existing entry keeps its flag)
@@ -24,6 +27,8 @@
 // Pool entries are u64 keys: orderable dist bits << 32 | id << 1 | new-flag,
 // so integer order is (dist, id) order.  Every stored distance uses the
 // canonical FP32 order of kg_device.hpp.
+#include <algorithm>
+
 #include "kg_device.hpp"
 #include "kg_nnd.hpp"
 
@@ -160,190 +165,16 @@ __global__ void __launch_bounds__(256) init_kernel(NndArgs a) {
 }
 
 // ------------------------------------------------------------------ join
-__device__ __forceinline__ void push_cand(NndArgs const &a, uint32_t target, uint32_t id, float d) {
-    uint64_t key = make_key(d, id, 1);
-    unsigned slot = atomicAdd(&a.ccnt[target], 1u);
-    if (slot >= a.C) slot = unsigned(mix64((uint64_t(target) << 32) | id) % a.C);
-    atomicMin(reinterpret_cast<unsigned long long *>(&a.cand[size_t(target) * a.C + slot]),
-              static_cast<unsigned long long>(key));
-}
-
-// One wavefront per node.  Each quad holds one "other" row (new or old) in
-// registers and scores it against every new row, which all quads of the wave
-// read through L1 at the same time.  Candidates are staged in a per-wave LDS
-// buffer and flushed grouped by target: one slot-claim atomicAdd per target
-// with pending candidates (all issued together), then non-returning 64-bit
-// atomicMin stores into the claimed slots (a hashed slot once a target's
-// slots are exhausted, which keeps the better key).
-constexpr unsigned JT = 512;  // list capacity (new + old) held in LDS
-constexpr unsigned JB = 256;  // staged candidates per wave
-
-struct JoinLDS {
-    uint32_t ids[JT];     // new list then old list
-    uint32_t cnt[JT];     // per-target pending count, then claimed base
-    uint64_t key[JB];
-    uint16_t tgt[JB];
-    unsigned fill;
-};
-
-__device__ void join_flush(NndArgs const &a, JoinLDS &L, unsigned nT) {
-    unsigned const lane = lane_id();
-    unsigned const fill = L.fill;
-    for (unsigned i = lane; i < nT; i += KG_WAVE) L.cnt[i] = 0;
-    wave_sync();
-    unsigned rank[JB / KG_WAVE];
-#pragma unroll
-    for (unsigned j = 0; j < JB / KG_WAVE; ++j) {
-        unsigned i = lane + j * KG_WAVE;
-        rank[j] = (i < fill) ? atomicAdd(&L.cnt[L.tgt[i]], 1u) : 0u;
-    }
-    wave_sync();
-    for (unsigned i = lane; i < nT; i += KG_WAVE) {
-        unsigned c = L.cnt[i];
-        if (c) L.cnt[i] = atomicAdd(&a.ccnt[L.ids[i]], c);
-    }
-    wave_sync();
-#pragma unroll
-    for (unsigned j = 0; j < JB / KG_WAVE; ++j) {
-        unsigned i = lane + j * KG_WAVE;
-        if (i < fill) {
-            unsigned const tp = L.tgt[i];
-            uint32_t const target = L.ids[tp];
-            uint64_t const key = L.key[i];
-            unsigned slot = L.cnt[tp] + rank[j];
-            if (slot >= a.C) slot = unsigned(mix64((uint64_t(target) << 32) | (uint32_t(key) >> 1)) % a.C);
-            atomicMin(reinterpret_cast<unsigned long long *>(&a.cand[size_t(target) * a.C + slot]),
-                      static_cast<unsigned long long>(key));
-        }
-    }
-    wave_sync();
-    if (lane == 0) L.fill = 0;
-    wave_sync();
-}
-
-template <int METRIC, int CH>
-__global__ void __launch_bounds__(256) join_kernel(NndArgs a) {
-    __shared__ JoinLDS lds[4];
-    unsigned const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    unsigned const n = blockIdx.x * 4 + w;
-    if (n >= a.N) return;
-    JoinLDS &L = lds[w];
-    unsigned const quad = lane >> 2, t = lane & 3;
-    unsigned const nn = a.nnew[n], no = a.nold[n];
-    unsigned const rn = min(a.crnew[n], a.R), ro = min(a.crold[n], a.R);
-    unsigned const nA = nn + rn, nB = no + ro, nT = min(nA + nB, JT);
-    uint32_t const *samp = a.samp + size_t(n) * a.Lc;
-    for (unsigned i = lane; i < nT; i += KG_WAVE) {
-        uint32_t v;
-        if (i < nn)
-            v = samp[i];
-        else if (i < nA)
-            v = a.rnew[size_t(n) * a.R + (i - nn)];
-        else if (i < nA + no)
-            v = samp[a.Lc - 1 - (i - nA)];
-        else
-            v = a.rold[size_t(n) * a.R + (i - nA - no)];
-        L.ids[i] = v;
-    }
-    if (lane == 0) L.fill = 0;
-    wave_sync();
-    unsigned const nAe = min(nA, nT);
-    unsigned const nch = (a.D & ~15u) >> 2;
-    bool found = false;
-    unsigned pairs = 0, pushes = 0;
-    for (unsigned base = 0; base < nT; base += 16) {
-        unsigned const oi = base + quad;
-        bool const active = oi < nT;  // whole quads; every lane stays in the loop
-        bool const isB = oi >= nAe;
-        uint32_t const o = active ? L.ids[oi] : 0u;
-        float const *orow = a.X + size_t(o) * a.Dp;
-        float4 reg[CH > 0 ? CH : 1];
-        if (CH > 0) {
-#pragma unroll
-            for (int k = 0; k < CH; ++k) {
-                unsigned kk = t + 4 * k;
-                reg[k] = (kk < nch) ? reinterpret_cast<float4 const *>(orow)[kk] : make_float4(0, 0, 0, 0);
-            }
-        }
-        float const ro_ = a.radius[o];
-        float const no_ = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[o];
-        for (unsigned ai = 0; ai < nAe; ++ai) {
-            if (L.fill + 32 > JB) join_flush(a, L, nT);  // wave-uniform
-            uint32_t const av = L.ids[ai];
-            // new x new once per id pair (i < j by id), new x old always (kgraph.cpp:741-750)
-            bool const valid = active && (isB ? (av != o) : (av < o));
-            float const *ra = a.X + size_t(av) * a.Dp;
-            float acc;
-            if (CH > 0) {
-                float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-                for (int k = 0; k < CH; ++k) {
-                    unsigned kk = t + 4 * k;
-                    if (kk < nch) {
-                        float4 y = reinterpret_cast<float4 const *>(ra)[kk];
-                        float4 x = reg[k];
-                        if (METRIC == METRIC_L2SQR) {
-                            float dx = y.x - x.x, dy = y.y - x.y, dz = y.z - x.z, dw = y.w - x.w;
-                            p.x = __builtin_fmaf(dx, dx, p.x);
-                            p.y = __builtin_fmaf(dy, dy, p.y);
-                            p.z = __builtin_fmaf(dz, dz, p.z);
-                            p.w = __builtin_fmaf(dw, dw, p.w);
-                        } else {
-                            p.x = __builtin_fmaf(y.x, x.x, p.x);
-                            p.y = __builtin_fmaf(y.y, x.y, p.y);
-                            p.z = __builtin_fmaf(y.z, x.z, p.z);
-                            p.w = __builtin_fmaf(y.w, x.w, p.w);
-                        }
-                    }
-                }
-                acc = quad_reduce(p);
-                for (unsigned i = a.D & ~15u; i < a.D; ++i) {
-                    if (METRIC == METRIC_L2SQR) {
-                        float dd = ra[i] - orow[i];
-                        acc = __builtin_fmaf(dd, dd, acc);
-                    } else {
-                        acc = __builtin_fmaf(ra[i], orow[i], acc);
-                    }
-                }
-            } else {
-                acc = quad_metric<METRIC>(ra, orow, a.D, t);
-            }
-            float const na = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[av];
-            float const d = finish_metric<METRIC>(acc, na, no_);
-            bool const lead = valid && t == 0;
-            bool const pa = lead && d < a.radius[av];
-            bool const po = lead && d < ro_;
-            unsigned const np_ = unsigned(pa) + unsigned(po);
-            unsigned slot = 0;
-            if (np_) slot = atomicAdd(&L.fill, np_);
-            if (pa) {
-                L.key[slot] = make_key(d, o, 1);
-                L.tgt[slot] = uint16_t(ai);
-                ++slot;
-            }
-            if (po) {
-                L.key[slot] = make_key(d, av, 1);
-                L.tgt[slot] = uint16_t(oi);
-            }
-            pairs += lead;
-            pushes += np_;
-            // found: the first endpoint's insert lands in its top K (kgraph.cpp:807-812)
-            found |= lead && d < a.kth[av];
-            wave_sync();
-        }
-    }
-    if (L.fill) join_flush(a, L, nT);
-    uint64_t const fb = ballot(found);
-    for (int o = 32; o > 0; o >>= 1) {
-        pairs += __shfl_xor(pairs, o);
-        pushes += __shfl_xor(pushes, o);
-    }
-    if (lane == 0) {
-        a.found[n] = fb ? 1u : 0u;
-        auto *st = reinterpret_cast<unsigned long long *>(a.stats);
-        if (pairs) atomicAdd(st + NND_STAT_PAIRS, (unsigned long long)pairs);
-        if (pushes) atomicAdd(st + NND_STAT_PUSHES, (unsigned long long)pushes);
-        atomicAdd(st + NND_STAT_LISTS, (unsigned long long)(nA + nB));
+// A claimed slot (< C) belongs to one candidate: a plain store.  Claims past C
+// go to a hashed slot of the target's overflow area, where a 64-bit atomicMin
+// keeps the better key (and one id always lands on the same slot).
+__device__ __forceinline__ void put_cand(NndArgs const &a, uint32_t target, unsigned slot, uint64_t key) {
+    if (slot < a.C) {
+        a.cand[size_t(target) * a.C + slot] = key;
+    } else {
+        unsigned const h = unsigned(mix64((uint64_t(target) << 32) | (uint32_t(key) >> 1)) % a.O);
+        atomicMin(reinterpret_cast<unsigned long long *>(&a.ovf[size_t(target) * a.O + h]),
+                  static_cast<unsigned long long>(key));
     }
 }
 
@@ -426,10 +257,7 @@ __device__ void mj_flush(NndArgs const &a, LDS &L, unsigned fill, unsigned nT) {
             unsigned const tp = L.tgt[i];
             uint32_t const target = L.ids[tp];
             uint64_t const key = L.key[i];
-            unsigned slot = L.cnt[tp] + rank[j];
-            if (slot >= a.C) slot = unsigned(mix64((uint64_t(target) << 32) | (uint32_t(key) >> 1)) % a.C);
-            atomicMin(reinterpret_cast<unsigned long long *>(&a.cand[size_t(target) * a.C + slot]),
-                      static_cast<unsigned long long>(key));
+            put_cand(a, target, L.cnt[tp] + rank[j], key);
         }
     }
     wave_sync();
@@ -612,10 +440,151 @@ __global__ void __launch_bounds__(256) join_mfma_kernel(NndArgs a) {
     }
     if (lane == 0) {
         a.found[n] = fb ? 1u : 0u;
-        auto *st = reinterpret_cast<unsigned long long *>(a.stats);
-        if (pairs) atomicAdd(st + NND_STAT_PAIRS, (unsigned long long)pairs);
-        if (pushes) atomicAdd(st + NND_STAT_PUSHES, (unsigned long long)pushes);
-        atomicAdd(st + NND_STAT_LISTS, (unsigned long long)(nA0 + nB));
+        a.part[NND_STAT_PAIRS * a.N + n] = pairs;
+        a.part[NND_STAT_PUSHES * a.N + n] = pushes;
+        a.part[NND_STAT_LISTS * a.N + n] = nA0 + nB;
+    }
+}
+
+// The same tiles in exact FP32: each quad holds one list row of the tile in
+// registers and computes its 16 pairs with the tile's new rows in canonical
+// order (kg_device.hpp), the new rows streaming through L1 for all quads at
+// once; lane t of a quad keeps the pairs of new rows t, t + 4, t + 8, t + 12,
+// and staging / flushing is shared with the MFMA form.
+template <int METRIC, int CH>
+__global__ void __launch_bounds__(256) join_tile_kernel(NndArgs a) {
+    __shared__ MJoinLDS<1> lds[4];
+    unsigned const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned const n = blockIdx.x * 4 + w;
+    if (n >= a.N) return;
+    auto &L = lds[w];
+    unsigned const nn = a.nnew[n], no = a.nold[n];
+    unsigned const rn = min(a.crnew[n], a.R), ro = min(a.crold[n], a.R);
+    unsigned const nA0 = nn + rn, nB = no + ro, nT = min(nA0 + nB, MT), nA = min(nA0, nT);
+    uint32_t const *samp = a.samp + size_t(n) * a.Lc;
+    for (unsigned i = lane; i < nT; i += KG_WAVE) {
+        uint32_t v;
+        if (i < nn)
+            v = samp[i];
+        else if (i < nA0)
+            v = a.rnew[size_t(n) * a.R + (i - nn)];
+        else if (i < nA0 + no)
+            v = samp[a.Lc - 1 - (i - nA0)];
+        else
+            v = a.rold[size_t(n) * a.R + (i - nA0 - no)];
+        L.ids[i] = v;
+    }
+    wave_sync();
+    unsigned const quad = lane >> 2, t = lane & 3;
+    unsigned const nch = (a.D & ~15u) >> 2;
+    unsigned fill = 0, pairs = 0, pushes = 0;
+    bool found = false;
+    for (unsigned ag = 0; ag < nA; ag += 16) {
+        uint32_t aid[4];
+        float arad[4], akth[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            unsigned const ia = ag + t + 4 * i;
+            bool const ok = ia < nA;
+            aid[i] = ok ? L.ids[ia] : 0u;
+            arad[i] = ok ? a.radius[aid[i]] : 0.f;
+            akth[i] = ok ? a.kth[aid[i]] : 0.f;
+        }
+        for (unsigned tg = 0; tg < nT; tg += 16) {
+            unsigned const io = tg + quad;
+            bool const ook = io < nT;
+            uint32_t const oid = ook ? L.ids[io] : 0u;
+            float const *orow = a.X + size_t(oid) * a.Dp;
+            float4 reg[CH > 0 ? CH : 1];
+            if (CH > 0) {
+#pragma unroll
+                for (int k = 0; k < CH; ++k) {
+                    unsigned const kk = t + 4 * k;
+                    reg[k] = (kk < nch) ? reinterpret_cast<float4 const *>(orow)[kk] : make_float4(0, 0, 0, 0);
+                }
+            }
+            float const orad = a.radius[oid];
+            float const on = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[oid];
+            float dres[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                unsigned const ia = ag + r;
+                uint32_t const av = L.ids[ia < nA ? ia : 0u];
+                float const *ra = a.X + size_t(av) * a.Dp;
+                float acc;
+                if (CH > 0) {
+                    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                    for (int k = 0; k < CH; ++k) {
+                        unsigned const kk = t + 4 * k;
+                        if (kk < nch) {
+                            float4 const y = reinterpret_cast<float4 const *>(ra)[kk];
+                            float4 const x = reg[k];
+                            if (METRIC == METRIC_L2SQR) {
+                                float const dx = y.x - x.x, dy = y.y - x.y, dz = y.z - x.z, dw = y.w - x.w;
+                                p.x = __builtin_fmaf(dx, dx, p.x);
+                                p.y = __builtin_fmaf(dy, dy, p.y);
+                                p.z = __builtin_fmaf(dz, dz, p.z);
+                                p.w = __builtin_fmaf(dw, dw, p.w);
+                            } else {
+                                p.x = __builtin_fmaf(y.x, x.x, p.x);
+                                p.y = __builtin_fmaf(y.y, x.y, p.y);
+                                p.z = __builtin_fmaf(y.z, x.z, p.z);
+                                p.w = __builtin_fmaf(y.w, x.w, p.w);
+                            }
+                        }
+                    }
+                    acc = quad_reduce(p);
+                    for (unsigned i = a.D & ~15u; i < a.D; ++i) {
+                        if (METRIC == METRIC_L2SQR) {
+                            float const dd = ra[i] - orow[i];
+                            acc = __builtin_fmaf(dd, dd, acc);
+                        } else {
+                            acc = __builtin_fmaf(ra[i], orow[i], acc);
+                        }
+                    }
+                } else {
+                    acc = quad_metric<METRIC>(ra, orow, a.D, t);
+                }
+                float const na = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[av];
+                float const d = finish_metric<METRIC>(acc, na, on);
+                if ((r & 3) == int(t)) dres[r >> 2] = d;
+            }
+            unsigned pm = 0;
+            uint64_t key[8];
+            unsigned tgt[8];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                unsigned const ia = ag + t + 4 * i;
+                // new x new once per id pair (i < j by id), new x old always (kgraph.cpp:741-750)
+                bool const valid = ia < nA && ook && (io < nA ? aid[i] < oid : aid[i] != oid);
+                float const d = dres[i];
+                bool const pa = valid && d < arad[i];
+                bool const po = valid && d < orad;
+                found |= valid && d < akth[i];
+                pairs += valid;
+                pushes += unsigned(pa) + unsigned(po);
+                pm |= (unsigned(pa) << (2 * i)) | (unsigned(po) << (2 * i + 1));
+                key[2 * i] = make_key(d, oid, 1);
+                key[2 * i + 1] = make_key(d, aid[i], 1);
+                tgt[2 * i] = ia;
+                tgt[2 * i + 1] = io;
+            }
+            mj_stage<8>(a, L, fill, nT, pm, key, tgt);
+        }
+    }
+    wave_sync();
+    if (fill) mj_flush(a, L, fill, nT);
+    uint64_t const fb = ballot(found);
+    for (int o = 32; o > 0; o >>= 1) {
+        pairs += __shfl_xor(pairs, o);
+        pushes += __shfl_xor(pushes, o);
+    }
+    if (lane == 0) {
+        a.found[n] = fb ? 1u : 0u;
+        a.part[NND_STAT_PAIRS * a.N + n] = pairs;
+        a.part[NND_STAT_PUSHES * a.N + n] = pushes;
+        a.part[NND_STAT_LISTS * a.N + n] = nA0 + nB;
     }
 }
 
@@ -638,9 +607,42 @@ __global__ void __launch_bounds__(256) sqn_kernel(float const *X, unsigned N, un
             mx = fmaxf(mx, fabsf(v));
         }
     if (ok && t == 0) out[row] = make_float2(n2, __fsqrt_rn(n2));
-    if (ballot(frac) && lane == 0) atomicOr(flags, 1u);
+    // atomics only while they can still change the flags
+    if (ballot(frac) && lane == 0 && !__hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicOr(flags, 1u);
     for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-    if (lane == 0) atomicMax(flags + 1, __float_as_uint(mx));
+    if (lane == 0 && __float_as_uint(mx) > __hip_atomic_load(flags + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(flags + 1, __float_as_uint(mx));
+}
+
+// Per-node statistics (join and merge write one value per node and column,
+// plain stores) summed into the NND_NSTATS counters: a block per chunk of
+// nodes, one atomic per block and column.  Same-address device-scope atomics
+// from every wave are what this avoids.
+__global__ void __launch_bounds__(256) stats_reduce_kernel(uint32_t const *part, unsigned N, uint64_t *stats) {
+    __shared__ unsigned long long sh[NND_NPART][4];
+    unsigned const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned long long acc[NND_NPART];
+#pragma unroll
+    for (int k = 0; k < NND_NPART; ++k) acc[k] = 0;
+    for (unsigned n = blockIdx.x * blockDim.x + threadIdx.x; n < N; n += gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < NND_NPART; ++k) acc[k] += part[size_t(k) * N + n];
+    }
+#pragma unroll
+    for (int k = 0; k < NND_NPART; ++k) {
+        unsigned long long v = acc[k];
+        for (int o = 32; o > 0; o >>= 1) {
+            unsigned lo = __shfl_xor(unsigned(v), o), hi = __shfl_xor(unsigned(v >> 32), o);
+            v += (static_cast<unsigned long long>(hi) << 32) | lo;
+        }
+        if (lane == 0) sh[k][w] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NND_NPART) {
+        unsigned long long v = sh[threadIdx.x][0] + sh[threadIdx.x][1] + sh[threadIdx.x][2] + sh[threadIdx.x][3];
+        if (v) atomicAdd(reinterpret_cast<unsigned long long *>(stats) + threadIdx.x, v);
+    }
 }
 
 // ------------------------------------------------------------------ merge
@@ -700,17 +702,21 @@ __device__ __forceinline__ void wave_sort_regs(uint64_t (&v)[PER]) {
 
 // Sorted pool U candidates -> first L distinct ids, all in registers.
 template <int PER>
-__device__ void merge_node(NndArgs const &a, unsigned n, unsigned c, unsigned m, unsigned mall) {
+__device__ void merge_node(NndArgs const &a, unsigned n, unsigned c, unsigned m, unsigned mall, bool over) {
     unsigned const lane = lane_id();
     uint64_t *pool = a.pool + size_t(n) * a.L;
-    uint64_t *cand = a.cand + size_t(n) * a.C;
+    uint64_t const *cand = a.cand + size_t(n) * a.C;
+    uint64_t *ovf = a.ovf + size_t(n) * a.O;
     uint64_t v[PER];
 #pragma unroll
     for (int r = 0; r < PER; ++r) {
         unsigned const i = lane * PER + r;
-        v[r] = (i < c) ? pool[i] : (i < c + m) ? cand[i - c] : ~0ull;
+        v[r] = (i < c) ? pool[i] : (i < c + m) ? ((i - c < mall) ? cand[i - c] : ovf[i - c - mall]) : ~0ull;
     }
-    for (unsigned i = lane; i < mall; i += KG_WAVE) cand[i] = ~0ull;  // ready for the next join
+    if (over) {  // the overflow area was used: empty it for the next join
+        wave_sync();
+        for (unsigned i = lane; i < a.O; i += KG_WAVE) ovf[i] = ~0ull;
+    }
     wave_sort_regs<PER>(v);
     // keep the first of every run of equal (dist, id); an existing pool entry
     // (flag 0) sorts before an equal candidate (flag 1) and so keeps its flag
@@ -761,9 +767,8 @@ __device__ void merge_node(NndArgs const &a, unsigned n, unsigned c, unsigned m,
         a.ccnt[n] = 0;
         a.radius[n] = (newc >= a.L) ? rad_w : __builtin_huge_valf();
         a.kth[n] = (newc >= a.K) ? kth_w : __builtin_huge_valf();
-        auto *st = reinterpret_cast<unsigned long long *>(a.stats);
-        atomicAdd(st + NND_STAT_DELTA, (unsigned long long)flags);
-        atomicAdd(st + NND_STAT_M, (unsigned long long)a.Mv[n]);
+        a.part[NND_STAT_DELTA * a.N + n] = flags;
+        a.part[NND_STAT_M * a.N + n] = a.Mv[n];
     }
 }
 
@@ -771,19 +776,21 @@ __global__ void __launch_bounds__(256) merge_kernel(NndArgs a) {
     unsigned const n = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (n >= a.N) return;
     unsigned const c = a.cnt[n];
-    unsigned const mall = min(a.ccnt[n], a.C);
-    unsigned const m = min(mall, NND_MERGE_MAX - c);  // the register sort holds 1024 keys
+    unsigned const cc = a.ccnt[n];
+    unsigned const mall = min(cc, a.C);                        // claimed slots
+    unsigned const mtot = mall + (cc > a.C ? a.O : 0u);        // + the overflow area
+    unsigned const m = min(mtot, NND_MERGE_MAX - c);  // the register sort holds 1024 keys
     unsigned const tot = c + m;
     if (tot <= 64)
-        merge_node<1>(a, n, c, m, mall);
+        merge_node<1>(a, n, c, m, mall, cc > a.C);
     else if (tot <= 128)
-        merge_node<2>(a, n, c, m, mall);
+        merge_node<2>(a, n, c, m, mall, cc > a.C);
     else if (tot <= 256)
-        merge_node<4>(a, n, c, m, mall);
+        merge_node<4>(a, n, c, m, mall, cc > a.C);
     else if (tot <= 512)
-        merge_node<8>(a, n, c, m, mall);
+        merge_node<8>(a, n, c, m, mall, cc > a.C);
     else
-        merge_node<16>(a, n, c, m, mall);
+        merge_node<16>(a, n, c, m, mall, cc > a.C);
 }
 
 // ------------------------------------------------------------------ update
@@ -925,12 +932,12 @@ static void launch_join_m(NndArgs const &a, hipStream_t s) {
     dim3 g((a.N + 3) / 4), b(256);
     size_t lds = 0;
     switch (ch_for(a.D)) {
-        case 1: hipLaunchKernelGGL((join_kernel<METRIC, 1>), g, b, lds, s, a); break;
-        case 2: hipLaunchKernelGGL((join_kernel<METRIC, 2>), g, b, lds, s, a); break;
-        case 4: hipLaunchKernelGGL((join_kernel<METRIC, 4>), g, b, lds, s, a); break;
-        case 6: hipLaunchKernelGGL((join_kernel<METRIC, 6>), g, b, lds, s, a); break;
-        case 8: hipLaunchKernelGGL((join_kernel<METRIC, 8>), g, b, lds, s, a); break;
-        default: hipLaunchKernelGGL((join_kernel<METRIC, 0>), g, b, lds, s, a); break;
+        case 1: hipLaunchKernelGGL((join_tile_kernel<METRIC, 1>), g, b, lds, s, a); break;
+        case 2: hipLaunchKernelGGL((join_tile_kernel<METRIC, 2>), g, b, lds, s, a); break;
+        case 4: hipLaunchKernelGGL((join_tile_kernel<METRIC, 4>), g, b, lds, s, a); break;
+        case 6: hipLaunchKernelGGL((join_tile_kernel<METRIC, 6>), g, b, lds, s, a); break;
+        case 8: hipLaunchKernelGGL((join_tile_kernel<METRIC, 8>), g, b, lds, s, a); break;
+        default: hipLaunchKernelGGL((join_tile_kernel<METRIC, 0>), g, b, lds, s, a); break;
     }
 }
 
@@ -983,6 +990,8 @@ hipError_t nnd_launch_join(int metric, NndArgs const &a, hipStream_t s) {
 
 hipError_t nnd_launch_merge(NndArgs const &a, hipStream_t s) {
     hipLaunchKernelGGL(merge_kernel, dim3((a.N + 3) / 4), dim3(256), 0, s, a);
+    unsigned const nb = std::min(1024u, (a.N + 255) / 256);
+    hipLaunchKernelGGL(stats_reduce_kernel, dim3(nb), dim3(256), 0, s, a.part, a.N, a.stats);
     return hipGetLastError();
 }
 

@@ -198,7 +198,7 @@ int kg_set_entry_projection(kg_handle *h, unsigned shortlist);
 
 /* Launch-shape autotuner for kg_search (no counterpart in the reference):
  * runs the search kernel on the given device-resident query sample with
- * every (waves per block, resident blocks per CU) pair in {1,2,4,8} x
+ * every (waves per block, resident blocks per CU) pair in {1,2,4} x
  * {4,8,16,32} that fits 32 waves per CU, times each with HIP events and keeps
  * the fastest for later searches on this handle.  chosen (may be NULL)
  * receives {waves per block, blocks per CU}. */

@@ -232,6 +232,20 @@ class KGraph:
         self._h.call("kg_last_time_ms", which, _lib.ctypes.byref(v))
         return v.value
 
+    def autotune_search(self, query, K=default_K, P=default_P, S=default_S, *, Ks=None, Ps=None):
+        """Times the search kernel on `query` for every launch shape it supports
+        and keeps the fastest for later searches; returns (waves per block,
+        blocks per CU)."""
+        from . import devmem
+        q = f32c(query)
+        dq = devmem.DeviceArray.from_host(q)
+        chosen = (_lib.ctypes.c_uint * 2)()
+        sp = SearchParams(K, default_M, P, S, default_T, default_epsilon, default_seed, 0)
+        self._h.call("kg_autotune_search", dq.ptr, q.shape[0], q.shape[1], sp, ptr(u32c(Ks) if Ks is not None else None),
+                     ptr(u32c(Ps) if Ps is not None else None), chosen)
+        devmem.synchronize()
+        return chosen[0], chosen[1]
+
     def build_counters(self):
         """Last build: {'pairs', 'list_entries', 'pushes', 'iterations',
         'join_form'} with join_form 'fp32', 'bf16-screen' or 'bf16-exact'."""

@@ -196,7 +196,7 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
     unsigned const ngraph = nq - unsigned(brute.size());
     if (ngraph) {
         size_t lpw = kg_search_lds_per_wave(poolmax, kmax + 1, h.Dp);
-        unsigned wpb = h.search_wpb;
+        unsigned wpb = std::min(h.search_wpb, kg_search_max_wpb());  // the kernel's launch bound
         while (wpb > 1 && lpw * wpb > 65536) wpb /= 2;
         size_t lpb = lpw * wpb;
         if (lpb > 65536) throw invalid_argument("K + P too large for the search kernel's LDS budget");
@@ -883,7 +883,7 @@ int kg_autotune_search(kg_handle *k, float const *dQ, unsigned nq, size_t qstrid
         dc.reserve(nq);
         float best = 3.4e38f;
         unsigned bw = 4, bb = 0;
-        for (unsigned wpb : {1u, 2u, 4u, 8u}) {
+        for (unsigned wpb : {1u, 2u, 4u}) {  // search_kernel runs at most 4 waves per block
             for (unsigned bpc : {4u, 8u, 16u, 32u}) {
                 if (wpb * bpc > 32) continue;  // at most 32 resident waves per CU
                 h.search_wpb = wpb;

@@ -87,7 +87,7 @@ struct Handle {
     DevBuf<float> q, qn, out_dists;
     DevBuf<uint32_t> out_ids, counts, comps, scans;
     unsigned vis_slots = 0, vis_words = 0;
-    unsigned search_wpb = 4, search_bpc = 0;  // launch shape (0 = as many blocks as LDS allows, <= 8)
+    unsigned search_wpb = 4, search_bpc = 0;  // launch shape: waves per block (<= 4), blocks per CU (0 = as LDS allows)
     DevBuf<float> lat;      // per-query latency of the last search (microseconds)
     unsigned lat_n = 0;
 

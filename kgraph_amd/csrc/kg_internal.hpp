@@ -92,6 +92,9 @@ struct kg_rescore_args {
 
 // Launchers (defined in the .hip files).
 size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp);
+// Largest waves per block search_kernel is compiled for (its launch bound);
+// kg_launch_search rejects larger shapes.
+unsigned kg_search_max_wpb();
 hipError_t kg_launch_search(int metric, kg_search_args const &a, unsigned blocks, unsigned waves_per_block,
                             hipStream_t s);
 hipError_t kg_launch_brute(int metric, kg_brute_args const &a, hipStream_t s);

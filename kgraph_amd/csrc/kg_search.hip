@@ -29,6 +29,7 @@ namespace {
 
 using namespace kgd;
 
+constexpr unsigned SEARCH_MAX_WPB = 4;  // waves per block the kernel is compiled for
 constexpr uint32_t FLAG = 0x80000000u;
 constexpr uint32_t IDMASK = 0x7fffffffu;
 
@@ -143,7 +144,7 @@ __device__ __forceinline__ uint32_t load_u32_l2(uint32_t const *p) {
 }
 
 template <int METRIC>
-__global__ void __launch_bounds__(256) search_kernel(kg_search_args a) {
+__global__ void __launch_bounds__(SEARCH_MAX_WPB * KG_WAVE) search_kernel(kg_search_args a) {
     extern __shared__ __align__(16) unsigned char smem[];
     unsigned const lane = lane_id();
     unsigned const wib = threadIdx.x / KG_WAVE;
@@ -400,8 +401,11 @@ size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp) 
     return LdsLayout(pool_cap, res_cap, Dp).total;
 }
 
+unsigned kg_search_max_wpb() { return SEARCH_MAX_WPB; }
+
 hipError_t kg_launch_search(int metric, kg_search_args const &a, unsigned blocks, unsigned waves_per_block,
                             hipStream_t s) {
+    if (waves_per_block == 0 || waves_per_block > SEARCH_MAX_WPB || blocks == 0) return hipErrorInvalidValue;
     size_t lds = size_t(waves_per_block) * a.lds_per_wave;
     dim3 grid(blocks), block(waves_per_block * KG_WAVE);
     switch (metric) {

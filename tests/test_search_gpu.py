@@ -192,3 +192,15 @@ def test_entry_selection_lifts_recall_on_separated_clusters():
     r_ent = kgo.recall_at_k(d_ent, gt_d, 10)
     print("recall plain %.3f entries %.3f" % (r_plain, r_ent))
     assert r_plain < 0.5 and r_ent > 0.95
+
+
+def test_autotuned_launch_shape_keeps_results():
+    # every launch shape the autotuner tries returns the same walk; the chosen
+    # shape stays within the kernel's launch bound (<= 4 waves per block)
+    X, Q, idx = setup("uniform", EUC)
+    g = gpu_index(X, idx, EUC)
+    before = g.search(Q, K=10, P=50, withDistance=True)
+    wpb, bpc = g.autotune_search(Q, K=10, P=50)
+    assert wpb in (1, 2, 4) and bpc in (4, 8, 16, 32)
+    after = g.search(Q, K=10, P=50, withDistance=True)
+    assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])

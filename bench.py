@@ -1,15 +1,23 @@
 """Benchmark: NN-descent build + graph-walk search on one MI355X.
 
-Workload (default): 1,000,000 x 128 float32 rows, integer-valued 1,000-centre
-Gaussian mixture (values clip(rint(|N(mu_c, 30^2)|), 0, 255), centres
-U[0,120)); squared-Euclidean distance; build K=100 L=150 S=10 R=100
-iterations=15 reverse=-1; 10,000 queries from the same mixture searched with
-K=10 and P in {50, 100, 200}.
+Workload (default, intmix-1m-d128): 1,000,000 x 128 float32 rows,
+integer-valued 1,000-centre Gaussian mixture (values clip(rint(|N(mu_c,
+30^2)|), 0, 255), centres U[0,120)); squared-Euclidean distance; build K=100
+L=150 S=10 R=100 iterations=15 reverse=-1; 10,000 queries from the same
+mixture searched with K=10.
 
-One step = one full build of the index + one search pass per P over the
-10,000 queries (queries resident in HBM).  `value` is the search QPS at the
-smallest P whose recall@10 (distance-equality, against the exact K-NN from
-the brute-force kernel) reaches 0.95; `build_s` is the build's wall time.
+One step = two full builds of the index -- the plain form (exact FP32 local
+join) and the BF16 MFMA form (exact on these integer rows) -- then one search
+pass per (start, P, S) setting over the 10,000 queries (queries resident in
+HBM): random starting points with P in {50, 100, 200} (the reference's
+configuration) and starting points selected from an 8,192-node entry sample
+(exact FP32 scores) with P in {10, 20, 50, 100} and S in {10, 32, 64}.
+`value` is the highest search QPS whose recall@10 (distance-equality, against
+the exact K-NN from the brute-force kernel) reaches 0.95; `build_s` is the
+BF16-form build's wall time and `build_forms` holds both forms.
+
+Other workloads: --workload batched-10m-d96 (mixed K/P in one launch) and
+--workload learned-5m-d128 (projector-screened entry points).
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -104,8 +112,6 @@ def main():
     h = _lib.Handle()
     _lib.lib().kg_set_verbosity(0)
     h.call("kg_set_data", ptr(X), N, D, D, _lib.METRIC_L2SQR)
-    if args.precision == "bf16":
-        h.call("kg_set_precision", _lib.PRECISION_BF16)
     dQ = devmem.DeviceArray.from_host(Q)
     dI = devmem.DeviceArray((nq, K), np.uint32)
     dD = devmem.DeviceArray((nq, K), np.float32)
@@ -117,13 +123,30 @@ def main():
     sweeps = [("random", P, 10) for P in Ps] + [("entries", P, S) for P in Pe for S in Se]
 
     def one_step():
-        info = IndexInfo()
-        devmem.synchronize()
-        tb = time.perf_counter()
-        h.call("kg_build", ip, info)
+        # the plain form (exact FP32 local join), then the BF16 MFMA form; the
+        # search runs on the index of the second build
+        builds = {}
+        for form, prec in (("fp32", _lib.PRECISION_FP32), ("bf16", _lib.PRECISION_BF16)):
+            if args.precision == "bf16" and form == "fp32":
+                continue
+            h.call("kg_set_precision", prec)
+            info = IndexInfo()
+            devmem.synchronize()
+            tb = time.perf_counter()
+            h.call("kg_build", ip, info)
+            devmem.synchronize()
+            cnt = np.zeros(5, dtype=np.uint64)
+            h.call("kg_build_counters", ptr(cnt), 5)
+            builds[form] = dict(s=time.perf_counter() - tb, it=int(info.iterations), recall=float(info.recall),
+                                join_ms=ctypes_float(h, 3), merge_ms=ctypes_float(h, 4),
+                                join_form=["fp32", "bf16-screen", "bf16-exact"][int(cnt[4])])
+            log("build %s: %.3fs (%d it, control recall %.4f, join %s %.1f ms, merge %.1f ms)" % (
+                form, builds[form]["s"], info.iterations, info.recall, builds[form]["join_form"],
+                builds[form]["join_ms"], builds[form]["merge_ms"]))
+        build_s = builds["bf16"]["s"]
+        # entry points are scored exactly (FP32) for the search
+        h.call("kg_set_precision", _lib.PRECISION_FP32)
         h.call("kg_set_entry_sample", args.entry_sample, 1998)
-        devmem.synchronize()
-        build_s = time.perf_counter() - tb
         res = []
         for mode, P, S in sweeps:
             sp = SearchParams(K, 0, P, S, 1, 1e30, 1998, m if mode == "entries" else 0)
@@ -140,7 +163,7 @@ def main():
             res.append(dict(mode=mode, P=P, S=S, wall_s=wall, kernel_ms=kms, entry_ms=ems, ids=dI.to_host(),
                             dists=dD.to_host(), comps=dCo.to_host(), scans=dSc.to_host(),
                             seeds=(m if mode == "entries" else P)))
-        return info, build_s, res
+        return info, build_s, res, builds
 
     for w in range(args.warmup):
         log("warmup step %d" % w)
@@ -151,7 +174,7 @@ def main():
         log("timed step %d" % s)
         steps.append(one_step())
     total_s = time.perf_counter() - t_all
-    info, build_s, res = steps[-1]
+    info, build_s, res, builds = steps[-1]
     build_times = [st[1] for st in steps]
     trace = None
     nit = ctypes_uint()
@@ -211,6 +234,12 @@ def main():
                    "entry_sample": args.entry_sample, "entries_per_query": m, "precision": args.precision},
         "build_s": round(build_s, 4),
         "build_s_steps": [round(b, 4) for b in build_times],
+        "build_forms": {f: {"build_s": round(b["s"], 4), "iterations": b["it"],
+                            "control_recall": round(b["recall"], 4), "join_form": b["join_form"],
+                            "join_ms": round(b["join_ms"], 2), "merge_ms": round(b["merge_ms"], 2)}
+                        for f, b in builds.items()},
+        "build_speedup_bf16_vs_fp32": (round(builds["fp32"]["s"] / builds["bf16"]["s"], 2)
+                                       if "fp32" in builds else None),
         "build_iterations": int(info.iterations),
         "build_control_recall": round(float(info.recall), 4),
         "recall@10": round(pick["recall"], 4),
@@ -219,7 +248,7 @@ def main():
                             "reverse": round(rev_ms, 2)},
         "roofline": {"kernel": "search_kernel", "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": None},
+                     "traffic": measured_traffic("search_kernel", "intmix-1m-d128", pick)},
         "roofline_join": {"kernel": "join_kernel", "bound": "hbm", "achieved": round(join_gbs, 1),
                           "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(join_gbs / PEAK_HBM_GBS, 4),
                           "pairs": int(counters[0]), "list_entries": int(counters[1]),
@@ -251,6 +280,7 @@ def batched_workload(args):
     h = _lib.Handle()
     _lib.lib().kg_set_verbosity(0)
     h.call("kg_set_data", ptr(X), n, D, D, _lib.METRIC_ANGULAR)
+    h.call("kg_set_precision", _lib.PRECISION_BF16)  # BF16 MFMA screen, exact FP32 re-score
     ip = IndexParams(args.iterations, 64, 32, 10, 100, 100, 1998, 0.002, 0.99, 0, -1)
     info = IndexInfo()
     tb = time.perf_counter()
@@ -330,7 +360,8 @@ def learned_workload(args):
     g = kgraph_amd.KGraph(X, "euclidean", l2sqr=True)
     _lib.lib().kg_set_verbosity(0)
     tb = time.perf_counter()
-    info = g.build(K=25, L=100, iterations=args.iterations, reverse=-1)
+    info = g.build(K=25, L=100, iterations=args.iterations, reverse=-1, precision=_lib.PRECISION_BF16)
+    g._h.call("kg_set_precision", _lib.PRECISION_FP32)
     build_s = time.perf_counter() - tb
     g.set_entry_sample(args.entry_sample, seed=1998)
     log("build %.2fs (%d it, control recall %.4f)" % (build_s, info.iterations, info.recall))
@@ -394,6 +425,22 @@ def learned_workload(args):
                                      if "learned" in best and "plain" in best else None),
     }
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(kernel, workload, pick):
+    """HBM bytes per launch of `kernel` for this configuration, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same command
+    (profiles/<kernel>_<workload>_traffic.md, a line
+    `traffic P=<P> S=<S> mode=<mode> bytes=<B>`), or None."""
+    path = os.path.join(ROOT, "profiles", "%s_%s_traffic.md" % (kernel, workload))
+    if not os.path.exists(path):
+        return None
+    want = "traffic P=%d S=%d mode=%s bytes=" % (pick["P"], pick["S"], pick["mode"])
+    for line in open(path):
+        line = line.strip().strip("`")
+        if line.startswith(want):
+            return int(line[len(want):])
+    return None
 
 
 def ctypes_float(h, which):

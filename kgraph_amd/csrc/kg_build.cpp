@@ -76,6 +76,7 @@ void replace_graph(Handle &h, unsigned N, DevBuf<uint32_t> &off, DevBuf<uint32_t
     h.edges = edges;
     h.has_graph = true;
     h.host_valid = false;
+    h.gnode_valid = false;
 }
 
 }  // namespace

@@ -59,6 +59,7 @@ void Handle::push_graph() {
     if (!no_dist && edges) KG_HIP(hipMemcpy(gdist.p, h_dist.data(), 4ull * edges, hipMemcpyHostToDevice));
     has_graph = true;
     host_valid = true;
+    gnode_valid = false;
 }
 
 // GenRandom (kgraph.cpp:44-65): `size` distinct ids < N from an mt19937.
@@ -216,6 +217,17 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
             h.vis_slots = slots;
             h.vis_words = words;
         }
+        if (!h.gnode_valid) {
+            h.gnode.reserve(std::max(N, 1u));
+            h.gflag.reserve(1);
+            KG_HIP(hipMemsetAsync(h.gflag.p, 0, 4, h.stream));
+            KG_HIP(kg_launch_gnode(N, h.goff.p, h.gids.p, h.gM.p, h.gnode.p, h.gflag.p, h.stream));
+            uint32_t dup = 0;
+            KG_HIP(hipMemcpyAsync(&dup, h.gflag.p, 4, hipMemcpyDeviceToHost, h.stream));
+            h.sync();
+            h.lists_unique = dup == 0;
+            h.gnode_valid = true;
+        }
         h.qcounter.reserve(1);
         KG_HIP(hipMemsetAsync(h.qcounter.p, 0, 4, h.stream));
         kg_search_args a{};
@@ -227,6 +239,8 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         a.goff = h.goff.p;
         a.gids = h.gids.p;
         a.gM = h.gM.p;
+        a.gnode = h.gnode.p;
+        a.lists_unique = h.lists_unique ? 1u : 0u;
         a.Q = Qk;
         a.qnorms = h.qn.p;
         a.nq = nq;

@@ -76,6 +76,9 @@ __device__ __forceinline__ float quad_reduce(float4 p) {
 // One quad computes the canonical SSD / dot of data row `a` (global) and
 // vector `b` (LDS or global).  t = lane within the quad (0..3).  Rows are
 // 16-byte aligned; D16 = D & ~15.
+#ifndef QM_GROUP
+#define QM_GROUP 8
+#endif
 template <int METRIC>
 __device__ __forceinline__ float quad_metric(float const *__restrict__ a, float const *__restrict__ b, unsigned D,
                                              unsigned t) {
@@ -83,9 +86,7 @@ __device__ __forceinline__ float quad_metric(float const *__restrict__ a, float 
     float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
     float4 const *a4 = reinterpret_cast<float4 const *>(a);
     float4 const *b4 = reinterpret_cast<float4 const *>(b);
-    for (unsigned k = t; k < nch; k += 4) {
-        float4 x = a4[k];
-        float4 y = b4[k];
+    auto step = [&p](float4 x, float4 y) {
         if (METRIC == METRIC_L2SQR) {
             float dx = x.x - y.x, dy = x.y - y.y, dz = x.z - y.z, dw = x.w - y.w;
             p.x = __builtin_fmaf(dx, dx, p.x);
@@ -98,7 +99,19 @@ __device__ __forceinline__ float quad_metric(float const *__restrict__ a, float 
             p.z = __builtin_fmaf(x.z, y.z, p.z);
             p.w = __builtin_fmaf(x.w, y.w, p.w);
         }
+    };
+    // groups of QM_GROUP chunks per lane: all of a group's row loads are in
+    // flight together (a row of 128 floats is one group), then accumulated in
+    // the same chunk order as one chunk at a time
+    unsigned k = t;
+    for (; k + 4 * (QM_GROUP - 1) < nch; k += 4 * QM_GROUP) {
+        float4 x[QM_GROUP];
+#pragma unroll
+        for (int g = 0; g < QM_GROUP; ++g) x[g] = a4[k + 4 * g];
+#pragma unroll
+        for (int g = 0; g < QM_GROUP; ++g) step(x[g], b4[k + 4 * g]);
     }
+    for (; k < nch; k += 4) step(a4[k], b4[k]);
     float acc = quad_reduce(p);
     for (unsigned i = D & ~15u; i < D; ++i) {
         if (METRIC == METRIC_L2SQR) {

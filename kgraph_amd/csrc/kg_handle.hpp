@@ -78,6 +78,11 @@ struct Handle {
     uint64_t edges = 0;
     // host mirror, refreshed lazily from the device copy
     bool host_valid = false;
+    // per-node walk records of the current graph (rebuilt after any change)
+    DevBuf<uint4> gnode;
+    DevBuf<uint32_t> gflag;
+    bool gnode_valid = false;
+    bool lists_unique = false;
     std::vector<uint32_t> h_off, h_ids, h_M;
     std::vector<float> h_dist;
 

@@ -15,6 +15,8 @@ struct kg_search_args {
     uint32_t const *goff;  // N+1
     uint32_t const *gids;
     uint32_t const *gM;
+    uint4 const *gnode;    // per node {CSR offset, degree, M, 0} (kg_launch_gnode)
+    unsigned lists_unique;  // 1: no adjacency list repeats an id
     // queries
     float const *Q;
     float const *qnorms;
@@ -95,6 +97,9 @@ size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp);
 // Largest waves per block search_kernel is compiled for (its launch bound);
 // kg_launch_search rejects larger shapes.
 unsigned kg_search_max_wpb();
+// gnode[n] = {goff[n], deg(n), M[n], 0}; *dupflag |= 1 if some list repeats an id.
+hipError_t kg_launch_gnode(unsigned N, uint32_t const *goff, uint32_t const *gids, uint32_t const *gM, uint4 *gnode,
+                           uint32_t *dupflag, hipStream_t s);
 hipError_t kg_launch_search(int metric, kg_search_args const &a, unsigned blocks, unsigned waves_per_block,
                             hipStream_t s);
 hipError_t kg_launch_brute(int metric, kg_brute_args const &a, hipStream_t s);

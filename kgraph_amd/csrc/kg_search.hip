@@ -68,13 +68,13 @@ struct LdsLayout {
 
 // actual_M (kgraph.cpp:284-286) truncated to NeighborX::M's uint16_t
 // (kgraph.cpp:79, 474, 508); also returns the node's CSR base offset.
+// One 16-byte load: gnode[id] = {CSR offset, degree, M, 0}.
 __device__ __forceinline__ uint32_t actual_M(kg_search_args const &a, uint32_t id, uint32_t *base) {
-    uint32_t b = a.goff[id];
-    uint32_t deg = a.goff[id + 1] - b;
-    uint32_t m = a.gM[id];
+    uint4 const g = a.gnode[id];
+    uint32_t m = g.z;
     if (m < a.Mparam) m = a.Mparam;
-    if (m > deg) m = deg;
-    *base = b;
+    if (m > g.y) m = g.y;
+    *base = g.x;
     return m & 0xffffu;
 }
 
@@ -137,6 +137,95 @@ __device__ __forceinline__ void pool_shift(WaveLDS &w, unsigned r, unsigned L) {
         }
         wave_sync();
     }
+}
+
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+    uint32_t const lo = __shfl_xor(uint32_t(v), m), hi = __shfl_xor(uint32_t(v >> 32), m);
+    return (uint64_t(hi) << 32) | lo;
+}
+
+// The reference inserts the fresh neighbours of a batch one at a time
+// (UpdateKnnList, kgraph.cpp:497-512): each goes after every pool entry with a
+// smaller or equal distance, the pool grows to K + P entries and then drops
+// its last, and the expansion index rewinds to the smallest insert position.
+// Done at once it is a stable merge: sort the nf candidates (lane order =
+// neighbour order) by (dist, order); candidate s lands at (pool entries with
+// dist <= its dist) + s, pool entry i at i + (candidates with a smaller
+// dist); positions from min(L + nf, qcap - 1) on are dropped.  The smallest
+// insert position of the sequential loop is the final position of the best
+// candidate (no later insert lands before it, no earlier one after it), so
+// the rewind is k = min(k, that position) when it is kept.  cm / cb hold
+// candidate `lane`'s M and CSR base; w.fid / w.fdist its id and distance.
+__device__ void batch_insert(WaveLDS &w, unsigned nf, uint32_t cm, uint32_t cb, unsigned qcap, unsigned &L,
+                             unsigned &k) {
+    unsigned const lane = lane_id();
+    uint64_t key = (lane < nf) ? ((uint64_t(ord_f32(w.fdist[lane])) << 32) | lane) : ~0ull;
+    for (unsigned kk = 2; kk <= KG_WAVE; kk <<= 1) {  // bitonic sort across the wave
+        for (unsigned j = kk >> 1; j > 0; j >>= 1) {
+            uint64_t const other = shfl_xor64(key, int(j));
+            bool const up = (lane & kk) == 0, lower = (lane & j) == 0;
+            uint64_t const mn = other < key ? other : key, mx = other < key ? key : other;
+            key = (lower == up) ? mn : mx;
+        }
+    }
+    unsigned const src = uint32_t(key);  // candidate sorted to position `lane`
+    bool const live = lane < nf;
+    float const sd = unord_f32(uint32_t(key >> 32));
+    uint32_t const sm = __shfl(cm, int(src & 63u)), sb = __shfl(cb, int(src & 63u));
+    uint32_t const sid = live ? w.fid[src] : 0u;
+    unsigned const L0 = L, Lf = min(L0 + nf, qcap - 1);
+    // final position of sorted candidate `lane`: pool entries with dist <= sd, plus lane
+    unsigned fin = 0;
+    if (live) {
+        unsigned lo = 0, hi = L0;
+        while (lo < hi) {
+            unsigned const mid = (lo + hi) >> 1;
+            if (w.pdist[mid] <= sd) lo = mid + 1; else hi = mid;
+        }
+        fin = lo + lane;
+    }
+    unsigned const fin0 = __shfl(fin, 0);
+    wave_sync();
+    if (live) w.fdist[lane] = sd;  // sorted distances for the pool entries' search
+    wave_sync();
+    // pool entries move up by the number of candidates strictly below them;
+    // top chunk first, each chunk read whole before it is written
+    for (int c = int((L0 - 1) & ~(KG_WAVE - 1u)); L0 > 0 && c >= 0; c -= KG_WAVE) {
+        unsigned const i = unsigned(c) + lane;
+        bool const mv = i < L0;
+        uint32_t id = 0, meta = 0, bs = 0;
+        float d = 0.f;
+        unsigned np = 0;
+        if (mv) {
+            id = w.pid[i];
+            d = w.pdist[i];
+            meta = w.pmeta[i];
+            bs = w.pbase[i];
+            unsigned lo = 0, hi = nf;
+            while (lo < hi) {
+                unsigned const mid = (lo + hi) >> 1;
+                if (w.fdist[mid] < d) lo = mid + 1; else hi = mid;
+            }
+            np = i + lo;
+        }
+        wave_sync();
+        if (mv && np != i && np < Lf) {
+            w.pid[np] = id;
+            w.pdist[np] = d;
+            w.pmeta[np] = meta;
+            w.pbase[np] = bs;
+        }
+        wave_sync();
+    }
+    if (live && fin < Lf) {
+        w.pid[fin] = sid | FLAG;
+        w.pdist[fin] = sd;
+        w.pmeta[fin] = sm << 16;
+        w.pbase[fin] = sb;
+    }
+    wave_sync();
+    L = Lf;
+    if (fin0 < Lf && fin0 < k) k = fin0;
 }
 
 __device__ __forceinline__ uint32_t load_u32_l2(uint32_t const *p) {
@@ -275,9 +364,11 @@ __global__ void __launch_bounds__(SEARCH_MAX_WPB * KG_WAVE) search_kernel(kg_sea
                     unsigned const cnt = min(endM - mc, unsigned(KG_WAVE));
                     uint32_t id = (lane < cnt) ? a.gids[ebase + mc + lane] : 0xffffffffu;
                     bool dup = false;
-                    for (unsigned j = 0; j < cnt; ++j) {
-                        uint32_t v = __shfl(id, j);
-                        dup |= (j < lane) && (v == id);
+                    if (!a.lists_unique) {  // a list may repeat an id: keep its first occurrence
+                        for (unsigned j = 0; j < cnt; ++j) {
+                            uint32_t v = __shfl(id, j);
+                            dup |= (j < lane) && (v == id);
+                        }
                     }
                     bool fresh = false;
                     if (lane < cnt && !dup) {
@@ -298,26 +389,9 @@ __global__ void __launch_bounds__(SEARCH_MAX_WPB * KG_WAVE) search_kernel(kg_sea
                     uint32_t cm = 0, cb = 0;
                     if (lane < nf) cm = actual_M(a, w.fid[lane], &cb);
                     batch_dist<METRIC>(a, w.fid, w.fdist, nf, w.q, qn);
-                    for (unsigned j = 0; j < nf; ++j) {
-                        ++n_comps;
-                        uint32_t const cid = w.fid[j];
-                        float const cd = w.fdist[j];
-                        uint32_t const jm = __shfl(cm, j), jb = __shfl(cb, j);
-                        bool const full = (L + 1 >= qcap);
-                        if (full && w.pdist[L - 1] <= cd) continue;  // lands in the scratch slot
-                        unsigned const r = insert_pos(w.pdist, L, cd);
-                        wave_sync();
-                        pool_shift(w, r, L);
-                        if (lane == 0) {
-                            w.pid[r] = cid | FLAG;
-                            w.pdist[r] = cd;
-                            w.pmeta[r] = jm << 16;
-                            w.pbase[r] = jb;
-                        }
-                        wave_sync();
-                        if (!full) ++L;
-                        if (r < L && r < k) k = r;
-                    }
+                    // the nf inserts of this batch, done at once (batch_insert)
+                    n_comps += nf;
+                    batch_insert(w, nf, cm, cb, qcap, L, k);
                 }
             }
             // ---- merge this trial into the results (kgraph.cpp:515-529)
@@ -395,7 +469,44 @@ __global__ void __launch_bounds__(SEARCH_MAX_WPB * KG_WAVE) search_kernel(kg_sea
     }
 }
 
+// Per-node record for the walk, gnode[n] = {goff[n], deg, M[n], 0}, and
+// whether any adjacency list repeats an id (dupflag, set only when one does):
+// one wave per node, the list in LDS when it fits, else read twice from HBM.
+constexpr unsigned GN_LDS = 1024;
+__global__ void __launch_bounds__(256) gnode_kernel(unsigned N, uint32_t const *goff, uint32_t const *gids,
+                                                    uint32_t const *gM, uint4 *gnode, uint32_t *dupflag) {
+    __shared__ uint32_t sh[4][GN_LDS];
+    unsigned const w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    unsigned const n = blockIdx.x * 4 + w;
+    if (n >= N) return;
+    uint32_t const b = goff[n], deg = goff[n + 1] - b;
+    if (lane == 0) gnode[n] = make_uint4(b, deg, gM[n], 0u);
+    uint32_t const *l = gids + b;
+    bool dup = false;
+    if (deg <= GN_LDS) {
+        for (unsigned j = lane; j < deg; j += KG_WAVE) sh[w][j] = l[j];
+        wave_sync();
+        for (unsigned j = lane; j < deg; j += KG_WAVE) {
+            uint32_t const v = sh[w][j];
+            for (unsigned k = 0; k < j && !dup; ++k) dup = sh[w][k] == v;
+        }
+    } else {
+        for (unsigned j = lane; j < deg; j += KG_WAVE) {
+            uint32_t const v = l[j];
+            for (unsigned k = 0; k < j && !dup; ++k) dup = l[k] == v;
+        }
+    }
+    if (ballot(dup) && lane == 0) atomicOr(dupflag, 1u);
+}
+
 }  // namespace
+
+hipError_t kg_launch_gnode(unsigned N, uint32_t const *goff, uint32_t const *gids, uint32_t const *gM, uint4 *gnode,
+                           uint32_t *dupflag, hipStream_t s) {
+    if (N == 0) return hipSuccess;
+    hipLaunchKernelGGL(gnode_kernel, dim3((N + 3) / 4), dim3(256), 0, s, N, goff, gids, gM, gnode, dupflag);
+    return hipGetLastError();
+}
 
 size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp) {
     return LdsLayout(pool_cap, res_cap, Dp).total;

@@ -342,9 +342,33 @@ static void set_entry_sample(Handle &h, unsigned E, unsigned seed) {
     h.escale.reserve(E);
     KG_HIP(kg_launch_norms(h.metric == KG_METRIC_ANGULAR ? KG_METRIC_ANGULAR : KG_METRIC_EUCLIDEAN, h.erows.p, E, h.D,
                            h.Dp, h.escale.p, h.stream));
+    // integer rows (and their largest |x|) allow the exact MFMA selection
+    h.iflags.reserve(2);
+    KG_HIP(hipMemsetAsync(h.iflags.p, 0, 8, h.stream));
+    KG_HIP(kg_launch_intcheck(h.erows.p, E, h.D, h.Dp, h.iflags.p, h.stream));
+    uint32_t fl[2];
+    KG_HIP(hipMemcpyAsync(fl, h.iflags.p, 8, hipMemcpyDeviceToHost, h.stream));
     h.sync();
+    h.entry_int = fl[0] == 0;
+    memcpy(&h.entry_max, &fl[1], 4);
     h.entryE = E;
     h.eproj_version = ~0u;
+}
+
+// BF16 tiles give the canonical FP32 distances exactly when every row and
+// query value is an integer of magnitude <= 256 and D (max|q|^2 + max|e|^2)
+// <= 2^24: every product, partial sum and norm is then an integer <= 2^24.
+static bool entries_exact_on_mfma(Handle &h, float const *Qk, unsigned nq, size_t qs) {
+    if (!h.entry_int || h.D > 256 || h.entry_max > 256.f) return false;
+    KG_HIP(hipMemsetAsync(h.iflags.p, 0, 8, h.stream));
+    KG_HIP(kg_launch_intcheck(Qk, nq, h.D, unsigned(qs), h.iflags.p, h.stream));
+    uint32_t fl[2];
+    KG_HIP(hipMemcpyAsync(fl, h.iflags.p, 8, hipMemcpyDeviceToHost, h.stream));
+    h.sync();
+    float qmax;
+    memcpy(&qmax, &fl[1], 4);
+    return fl[0] == 0 && qmax <= 256.f &&
+           double(h.D) * (double(qmax) * qmax + double(h.entry_max) * h.entry_max) <= 16777216.0;
 }
 
 // m nearest sampled nodes per query -> ids rows [0, m) (device buffers).
@@ -375,7 +399,8 @@ static void select_entries_device(Handle &h, float const *dQ, unsigned nq, size_
         h.zq.reserve(size_t(nq) * h.mlpP);
         h.hq.reserve(std::max(size_t(nq), size_t(h.entryE)) * h.mlpH);
     }
-    if (learned || screen) h.eshort.reserve(size_t(nq) * sl);
+    bool const exact_mfma = !learned && entries_exact_on_mfma(h, Qk, nq, qs);
+    if ((learned || screen) && !exact_mfma) h.eshort.reserve(size_t(nq) * sl);
     a.rows = learned ? h.eproj.p : h.erows.p;
     a.enorms = h.enorms.p;
     a.ids = h.eids.p;
@@ -392,7 +417,15 @@ static void select_entries_device(Handle &h, float const *dQ, unsigned nq, size_
     a.ostride = (learned || screen) ? sl : unsigned(ostride);
     if (screen) a.enorms = h.escale.p;
     KG_HIP(hipEventRecord(h.ev[14], h.stream));
-    if (learned || screen) {
+    if (exact_mfma) {
+        a.m = m;
+        a.out_ids = d_ids;
+        a.ostride = unsigned(ostride);
+        a.enorms = h.enorms.p;
+        h.qsq.reserve(nq);
+        KG_HIP(kg_launch_norms(KG_METRIC_EUCLIDEAN, Qk, nq, h.D, unsigned(qs), h.qsq.p, h.stream));
+        KG_HIP(kg_launch_entries_exact_mfma(h.metric, a, h.qsq.p, h.escale.p, h.stream));
+    } else if (learned || screen) {
         if (learned) {
             mlp_forward(h, Qk, nq, qs, h.hq.p, h.zq.p);
             KG_HIP(kg_launch_entries(KG_METRIC_L2SQR, a, h.stream));

@@ -137,7 +137,9 @@ __device__ __forceinline__ float finish_metric(float acc, float na, float nb) {
 template <int METRIC>
 __device__ __forceinline__ float finish_norm(float n2) {
     if (METRIC == METRIC_EUCLIDEAN) return n2 / 2;
-    if (METRIC == METRIC_ANGULAR) return __fsqrt_rn(n2);
+    // sqrtf is correctly rounded here (hipcc expands it with FMA correction
+    // steps); __fsqrt_rn compiles to the bare v_sqrt_f32, which is not
+    if (METRIC == METRIC_ANGULAR) return sqrtf(n2);
     return 0.f;
 }
 

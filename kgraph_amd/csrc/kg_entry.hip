@@ -12,6 +12,8 @@
 //                      one sampled row, the queries sit in LDS
 //   entry_topm_kernel  one wave per query: per-lane sorted top-m lists in
 //                      registers, then m rounds of a wave-wide minimum
+#include <algorithm>
+
 #include "kg_device.hpp"
 #include "kg_internal.hpp"
 
@@ -227,7 +229,190 @@ __global__ void gather_rows_kernel(float const *X, unsigned Dp, uint32_t const *
     reinterpret_cast<float4 *>(out)[i] = reinterpret_cast<float4 const *>(X + size_t(ids[r]) * Dp)[c];
 }
 
+// Whether rows hold only integers, and their largest |value|: flags[0] |= 1
+// on a non-integer, flags[1] = max |x| as float bits (both zeroed by the
+// caller).  Atomics only while they can still change the flags.
+__global__ void __launch_bounds__(256) intcheck_kernel(float const *X, unsigned n, unsigned D, unsigned stride,
+                                                       uint32_t *flags) {
+    unsigned const lane = threadIdx.x & 63;
+    size_t const total = size_t(n) * D;
+    bool frac = false;
+    float mx = 0.f;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += size_t(gridDim.x) * blockDim.x) {
+        size_t const r = i / D, c = i - r * D;
+        float const v = X[r * stride + c];
+        frac |= !(v == rintf(v));
+        mx = fmaxf(mx, fabsf(v));
+    }
+    if (ballot(frac) && lane == 0 && !__hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicOr(flags, 1u);
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    if (lane == 0 && __float_as_uint(mx) > __hip_atomic_load(flags + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(flags + 1, __float_as_uint(mx));
+}
+
+// Exact entry selection on the matrix cores, for rows and queries whose BF16
+// dot products are exact (integers, |x| <= 256, D (max|q|^2 + max|e|^2) <=
+// 2^24): then every product, partial sum and norm is an integer of at most
+// 2^24 and the tile value equals the canonical FP32 distance bit for bit.
+// One block per 16 queries, 4 waves splitting the sample's 16-row tiles
+// (wave w takes tiles w, w + 4, ...).  A tile's operands: the sampled rows
+// (A, loaded per tile) and the block's 16 queries (B, kept in registers), so
+// lane l holds the distances of query l % 16 to rows 4 (l / 16) + j.  Each
+// lane keeps its query's MF best (dist, id) keys in registers; the four
+// lanes of a query and then the four waves are merged into the m best.
+template <int METRIC, int KS, int MF>
+__global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float const *qsq, float const *esq) {
+    __shared__ uint64_t best[4][16][MF];
+    unsigned const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned const r16 = lane & 15, kq = (lane >> 4) * 4;
+    unsigned const q = blockIdx.x * 16 + r16;
+    bool const qok = q < a.nq;
+    bf16x4_t bop[KS];
+    {
+        float const *qr = a.Q + size_t(qok ? q : 0u) * a.qstride;
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                unsigned const k = 16 * s + kq + j;
+                bop[s][j] = bf16_rne((qok && k < a.D) ? qr[k] : 0.f);
+            }
+        }
+    }
+    float const qn = qok ? ((METRIC == METRIC_L2SQR) ? 2.f * qsq[q] : a.qnorms[q]) : 0.f;
+    uint64_t list[MF];
+#pragma unroll
+    for (int i = 0; i < MF; ++i) list[i] = ~0ull;
+    unsigned const ntiles = (a.E + 15) / 16;
+    for (unsigned et = w; et < ntiles; et += 4) {
+        unsigned const e = et * 16 + r16;
+        bool const eok = e < a.E;
+        float const *er = a.rows + size_t(eok ? e : 0u) * a.Dp;
+        bf16x4_t aop[KS];
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            unsigned const k = 16 * s + kq;
+            float4 const v = (eok && k < a.Dp) ? *reinterpret_cast<float4 const *>(er + k) : make_float4(0, 0, 0, 0);
+            aop[s][0] = bf16_rne(v.x);
+            aop[s][1] = bf16_rne(v.y);
+            aop[s][2] = bf16_rne(v.z);
+            aop[s][3] = bf16_rne(v.w);
+        }
+        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < KS; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aop[s], bop[s], acc, 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            unsigned const ee = et * 16 + kq + j;  // row of C: sampled row 4 (l / 16) + j
+            if (!qok || ee >= a.E) continue;
+            float const G = acc[j];
+            float d;
+            if (METRIC == METRIC_L2SQR)
+                d = (qn + 2.f * esq[ee]) - 2.f * G;
+            else if (METRIC == METRIC_EUCLIDEAN)
+                d = (a.enorms[ee] + qn) - G;
+            else
+                d = -G / __builtin_fmaf(a.enorms[ee], qn, 1.1920928955078125e-07f);
+            uint64_t const key = (uint64_t(ord_f32(d)) << 32) | a.ids[ee];
+            if (key < list[MF - 1]) {  // sorted insert, compile-time indices
+                bool placed = false;
+#pragma unroll
+                for (int i = MF - 1; i > 0; --i) {
+                    if (!placed) {
+                        if (list[i - 1] > key) {
+                            list[i] = list[i - 1];
+                        } else {
+                            list[i] = key;
+                            placed = true;
+                        }
+                    }
+                }
+                if (!placed) list[0] = key;
+            }
+        }
+    }
+    // the four lanes of a query (lane % 16 equal): m rounds of the minimum head
+    for (unsigned r = 0; r < a.m; ++r) {
+        uint64_t v = list[0];
+        uint64_t u = shfl_xor64(v, 16);
+        v = u < v ? u : v;
+        u = shfl_xor64(v, 32);
+        v = u < v ? u : v;
+        if (list[0] == v && v != ~0ull) {  // keys are unique: one lane pops
+#pragma unroll
+            for (int i = 0; i + 1 < MF; ++i) list[i] = list[i + 1];
+            list[MF - 1] = ~0ull;
+        }
+        if (lane < 16) best[w][r16][r] = v;
+    }
+    __syncthreads();
+    // the four waves' lists of each query -> its m best
+    if (w == 0 && lane < 16 && qok) {
+        unsigned pos[4] = {0, 0, 0, 0};
+        for (unsigned r = 0; r < a.m; ++r) {
+            uint64_t v = ~0ull;
+            unsigned src = 0;
+#pragma unroll
+            for (unsigned ww = 0; ww < 4; ++ww) {
+                uint64_t const h = pos[ww] < a.m ? best[ww][r16][pos[ww]] : ~0ull;
+                if (h < v) {
+                    v = h;
+                    src = ww;
+                }
+            }
+            ++pos[src];
+            a.out_ids[size_t(q) * a.ostride + r] = uint32_t(v);
+        }
+    }
+}
+
 }  // namespace
+
+hipError_t kg_launch_intcheck(float const *X, unsigned n, unsigned D, unsigned stride, uint32_t *flags,
+                              hipStream_t s) {
+    size_t const total = size_t(n) * D;
+    if (total == 0) return hipSuccess;
+    unsigned const nb = unsigned(std::min<size_t>(2048, (total + 255) / 256));
+    hipLaunchKernelGGL(intcheck_kernel, dim3(nb), dim3(256), 0, s, X, n, D, stride, flags);
+    return hipGetLastError();
+}
+
+template <int METRIC, int KS>
+static void launch_exact_ks(kg_entry_args const &a, float const *qsq, float const *esq, hipStream_t s) {
+    dim3 g((a.nq + 15) / 16), b(256);
+    if (a.m <= 4)
+        hipLaunchKernelGGL((entry_exact_kernel<METRIC, KS, 4>), g, b, 0, s, a, qsq, esq);
+    else if (a.m <= 8)
+        hipLaunchKernelGGL((entry_exact_kernel<METRIC, KS, 8>), g, b, 0, s, a, qsq, esq);
+    else if (a.m <= 12)
+        hipLaunchKernelGGL((entry_exact_kernel<METRIC, KS, 12>), g, b, 0, s, a, qsq, esq);
+    else
+        hipLaunchKernelGGL((entry_exact_kernel<METRIC, KS, 16>), g, b, 0, s, a, qsq, esq);
+}
+
+template <int METRIC>
+static void launch_exact_m(kg_entry_args const &a, float const *qsq, float const *esq, hipStream_t s) {
+    unsigned const ks = (a.D + 15) / 16;
+    if (ks <= 4)
+        launch_exact_ks<METRIC, 4>(a, qsq, esq, s);
+    else if (ks <= 8)
+        launch_exact_ks<METRIC, 8>(a, qsq, esq, s);
+    else
+        launch_exact_ks<METRIC, 16>(a, qsq, esq, s);
+}
+
+hipError_t kg_launch_entries_exact_mfma(int metric, kg_entry_args const &a, float const *qsq, float const *esq,
+                                        hipStream_t s) {
+    if (a.m == 0 || a.m > MAXM || a.E == 0 || a.nq == 0 || a.D > 256) return hipErrorInvalidValue;
+    if (metric == METRIC_L2SQR)
+        launch_exact_m<METRIC_L2SQR>(a, qsq, esq, s);
+    else if (metric == METRIC_EUCLIDEAN)
+        launch_exact_m<METRIC_EUCLIDEAN>(a, qsq, esq, s);
+    else
+        launch_exact_m<METRIC_ANGULAR>(a, qsq, esq, s);
+    return hipGetLastError();
+}
 
 hipError_t kg_launch_gather_rows(float const *X, unsigned Dp, uint32_t const *ids, unsigned E, float *out,
                                  hipStream_t s) {

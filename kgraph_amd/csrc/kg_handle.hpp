@@ -61,7 +61,11 @@ struct Handle {
     int num_cus = 0;
     size_t lds_per_cu = 0;
     int precision = KG_PRECISION_FP32;
-    int join_exact = -1;  // last build: -1 FP32 join, 0 BF16 screen + re-score, 1 exact BF16 tiles
+    int join_exact = -1;
+    bool entry_int = false;  // entry sample rows are all integers
+    float entry_max = 0.f;   // their largest |value|
+    DevBuf<float> qsq;       // ||q||^2 / 2 of the last query batch
+    DevBuf<uint32_t> iflags;  // last build: -1 FP32 join, 0 BF16 screen + re-score, 1 exact BF16 tiles
 
     // dataset (dense oracle): N rows, row stride Dp floats (D rounded up to 4)
     unsigned N = 0, D = 0, Dp = 0;

@@ -108,6 +108,14 @@ hipError_t kg_launch_entries(int metric, kg_entry_args const &a, hipStream_t s);
 hipError_t kg_launch_rescore(int metric, kg_rescore_args const &a, hipStream_t s);
 // BF16 MFMA screening; a.enorms = ||e||^2/2 (Euclidean forms) or ||e|| (angular)
 hipError_t kg_launch_entries_bf16(int metric, kg_entry_args const &a, hipStream_t s);
+// Exact entry selection on MFMA for rows / queries whose BF16 tiles are exact
+// (see kg_entry.hip); qsq / esq: ||q||^2/2 and ||e||^2/2 (used for L2SQR).
+hipError_t kg_launch_entries_exact_mfma(int metric, kg_entry_args const &a, float const *qsq, float const *esq,
+                                        hipStream_t s);
+// flags[0] |= 1 if a value of the n x D rows (row stride `stride`) is not an
+// integer; flags[1] = max |value| as float bits.  Caller zeroes flags.
+hipError_t kg_launch_intcheck(float const *X, unsigned n, unsigned D, unsigned stride, uint32_t *flags,
+                              hipStream_t s);
 hipError_t kg_launch_gather_rows(float const *X, unsigned Dp, uint32_t const *ids, unsigned E, float *out,
                                  hipStream_t s);
 hipError_t kg_launch_norms(int metric, float const *X, unsigned N, unsigned D, unsigned Dp, float *out,

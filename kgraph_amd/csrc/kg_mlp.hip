@@ -141,7 +141,7 @@ __global__ void adam_kernel(float *w, float const *g, float *m1, float *m2, unsi
     float b = b2 * m2[i] + (1.f - b2) * gi * gi;
     m1[i] = a;
     m2[i] = b;
-    w[i] -= lr * (a / c1) / (__fsqrt_rn(b / c2) + eps);
+    w[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
 }
 
 }  // namespace

@@ -606,7 +606,7 @@ __global__ void __launch_bounds__(256) sqn_kernel(float const *X, unsigned N, un
             frac |= !(v == rintf(v));
             mx = fmaxf(mx, fabsf(v));
         }
-    if (ok && t == 0) out[row] = make_float2(n2, __fsqrt_rn(n2));
+    if (ok && t == 0) out[row] = make_float2(n2, sqrtf(n2));
     // atomics only while they can still change the flags
     if (ballot(frac) && lane == 0 && !__hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
         atomicOr(flags, 1u);

@@ -232,8 +232,12 @@ __device__ __forceinline__ uint32_t load_u32_l2(uint32_t const *p) {
     return __hip_atomic_load(const_cast<uint32_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+#ifndef SEARCH_WAVES_PER_EU
+#define SEARCH_WAVES_PER_EU 4
+#endif
 template <int METRIC>
-__global__ void __launch_bounds__(SEARCH_MAX_WPB * KG_WAVE) search_kernel(kg_search_args a) {
+__global__ void __launch_bounds__(SEARCH_MAX_WPB * KG_WAVE) __attribute__((amdgpu_waves_per_eu(SEARCH_WAVES_PER_EU)))
+search_kernel(kg_search_args a) {
     extern __shared__ __align__(16) unsigned char smem[];
     unsigned const lane = lane_id();
     unsigned const wib = threadIdx.x / KG_WAVE;

@@ -128,7 +128,10 @@ def test_search_init_starting_points():
 
 def test_bruteforce_parity():
     import kgraph_amd
-    for kind, metric, d in [("uniform", EUC, 16), ("intmix", L2, 128), ("angular", ANG, 96)]:
+    # intmix under 'angular' has large norms: the row norms must be correctly
+    # rounded square roots, as the reference's std::sqrt is
+    for kind, metric, d in [("uniform", EUC, 16), ("intmix", L2, 128), ("angular", ANG, 96), ("intmix", ANG, 32),
+                            ("intmix", EUC, 48)]:
         X, Q = _dataset(kind, 20000, 50, d, 3)
         g = kgraph_amd.KGraph(X, "angular" if metric == ANG else "euclidean", l2sqr=(metric == L2))
         gi, gd = g.search(Q, K=100, withDistance=True)  # no index: exact scan
@@ -155,20 +158,23 @@ def _entry_sample_ids(seed, E, N):
     return np.sort(kgo.genrandom(seed, 1, E, N)[0])
 
 
-@pytest.mark.parametrize("metric", [L2, EUC, ANG])
-def test_entry_selection_parity(metric):
-    kind = {L2: "intmix", EUC: "uniform", ANG: "angular"}[metric]
-    X, Q, idx = setup(kind, metric, d={L2: 128, EUC: 16, ANG: 96}[metric])
+# integer rows and queries take the exact MFMA selection (BF16 tiles equal the
+# FP32 distances there); FP32 data the canonical FP32 scan
+@pytest.mark.parametrize("kind,metric,d,m", [("intmix", L2, 128, 7), ("uniform", EUC, 16, 7), ("angular", ANG, 96, 7),
+                                             ("intmix", EUC, 64, 9), ("intmix", ANG, 32, 16), ("intmix", L2, 24, 3)])
+def test_entry_selection_parity(kind, metric, d, m):
+    X, Q, idx = setup(kind, metric, d=d)
     g = gpu_index(X, idx, metric)
-    E, m = 512, 7
+    E = 512
     g.set_entry_sample(E, seed=77)
     got = g.select_entries(Q, m)
     samp = _entry_sample_ids(77, E, len(X))
     si, sd, _ = kgo.bruteforce(X[samp], Q, m, metric)
     assert np.array_equal(got, samp[si])
     # the walk from those starting points equals the oracle's (kgraph.cpp:461-467)
-    gi, gd = g.search(Q, K=10, P=40, entries=m, withDistance=True, stats=True)
-    oi, od, oc, ocomp, oscan = idx.search(X, Q, metric, K=10, P=40, init=m, init_ids=samp[si])
+    K = max(10, m + 1)
+    gi, gd = g.search(Q, K=K, P=40, entries=m, withDistance=True, stats=True)
+    oi, od, oc, ocomp, oscan = idx.search(X, Q, metric, K=K, P=40, init=m, init_ids=samp[si])
     assert np.array_equal(g.last_counts, oc)
     assert np.array_equal(gi, oi) and np.array_equal(gd, od)
     assert np.array_equal(g.last_comps, ocomp) and np.array_equal(g.last_scans, oscan)

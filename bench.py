@@ -55,12 +55,16 @@ def parse():
                     help="P values for walks started from selected entry points")
     ap.add_argument("--Se", type=str, default="10,32,64",
                     help="S values (neighbours per expansion, SearchParams::S) for entry-started walks")
-    ap.add_argument("--entry-sample", type=int, default=8192)
+    ap.add_argument("--entry-sample", type=int, default=None,
+                    help="entry-point sample size (default 8192; 65536 for learned-5m-d128, whose 5,000 "
+                         "separated clusters need several sampled nodes each)")
     ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device to device (reference transfer for counter runs)")
     ap.add_argument("--cpu-queries", type=int, default=1000)
+    ap.add_argument("--sigma", type=float, default=None,
+                    help="batched-10m-d96: per-coordinate noise (default 0.3/sqrt(96), a noise radius of 0.3)")
     ap.add_argument("--cpu-build-n", type=int, default=100_000)
     return ap.parse_args()
 
@@ -72,6 +76,8 @@ def main():
         args.n = {"batched-10m-d96": 10_000_000, "learned-5m-d128": 5_000_000}.get(args.workload, 1_000_000)
     if args.nq is None:
         args.nq = 100_000 if batched else 10_000
+    if args.entry_sample is None:
+        args.entry_sample = {"learned-5m-d128": 65536, "batched-10m-d96": 16384}.get(args.workload, 8192)
     if args.gpus != 1:
         print("bench.py runs on one GPU: the build and search shard by node / query range across GPUs by "
               "design only (no multi-GPU path is implemented or measured).", file=sys.stderr)
@@ -263,15 +269,21 @@ def batched_workload(args):
     """10,000,000 x 96 L2-normalised 2,000-centre mixture (angular metric),
     K=32 graph built on the GPU, then 100,000 queries with K in {1,10,100}
     (ratio 5:4:1) and P in {50,100,300,500} (uniform) answered by one
-    persistent search launch.  Reports QPS, per-query p50/p99 latency (stamped
-    inside the kernel), algorithmic HBM GB/s and recall@K on a query sample."""
+    persistent search launch, every walk starting from its query's 9 nearest
+    nodes of a 16,384-node entry sample (a query walks from min(9, K + P - 1)
+    of them).  The within-cluster sigma 0.3 is the noise radius: 0.3 / sqrt(96)
+    per coordinate (--sigma sets the per-coordinate value; 0.3 per coordinate
+    makes the rows almost uniform on the sphere).  Reports QPS (entry
+    selection + walk), per-query p50/p99 latency of the walk (stamped inside
+    the kernel), algorithmic HBM GB/s and recall@K on a query sample."""
     from kgraph_amd import _lib, datagen, devmem
     from kgraph_amd._lib import IndexInfo, IndexParams, SearchParams, ptr
     from oracle.kgo import recall_at_k
     n, nq = args.n, args.nq
     t0 = time.time()
-    X = datagen.angular_mixture(n, 96, 2000, 6, 9)
-    Q = datagen.angular_mixture(nq, 96, 2000, 6, 10)
+    sigma = args.sigma if args.sigma is not None else 0.3 / np.sqrt(96)
+    X = datagen.angular_mixture(n, 96, 2000, 6, 9, sigma=sigma)
+    Q = datagen.angular_mixture(nq, 96, 2000, 6, 10, sigma=sigma)
     rs = np.random.RandomState(11)
     Ks = rs.choice([1, 10, 100], size=nq, p=[0.5, 0.4, 0.1]).astype(np.uint32)
     Ps = rs.choice([50, 100, 300, 500], size=nq).astype(np.uint32)
@@ -294,21 +306,28 @@ def batched_workload(args):
     dC = devmem.DeviceArray((nq,), np.uint32)
     dCo = devmem.DeviceArray((nq,), np.uint32)
     dSc = devmem.DeviceArray((nq,), np.uint32)
-    sp = SearchParams(10, 0, 100, 10, 1, 1e30, 1998, 0)
-    walls = []
+    # entry points: BF16 MFMA screen of the sample, 16-node shortlist re-ranked exactly
+    E, m = args.entry_sample, 9
+    h.call("kg_set_entry_sample", E, 1998)
+    sp = SearchParams(10, 0, 100, 10, 1, 1e30, 1998, m)
+    walls, ems_l = [], []
     for s in range(args.warmup + args.steps):
         devmem.synchronize()
         ts = time.perf_counter()
+        h.call("kg_select_entries_device", dQ.ptr, nq, D, m, dI.ptr, W)
         h.call("kg_search_device", dQ.ptr, nq, D, sp, ptr(Ks), ptr(Ps), dI.ptr, dD.ptr, W, dC.ptr, dCo.ptr, dSc.ptr)
         devmem.synchronize()
         if s >= args.warmup:
             walls.append(time.perf_counter() - ts)
+            ems_l.append(ctypes_float(h, 7))
     kms = ctypes_float(h, 0)
+    ems = float(np.median(ems_l))
     lat = np.zeros(nq, np.float32)
     h.call("kg_search_latencies", ptr(lat), nq)
     comps, scans, dists = dCo.to_host(), dSc.to_host(), dD.to_host()
     wall = float(np.median(walls))
-    byt = float((4 * D * (Ps.astype(np.float64) + comps) + 4 * scans + 4 * Ps).sum())
+    seeds = np.minimum(m, Ks.astype(np.int64) + Ps - 1).astype(np.float64)
+    byt = float((4 * D * (seeds + comps) + 4 * scans + 4 * seeds).sum())
     # recall on a query sample against the exact scan
     ns = min(1000, nq)
     gt_i = np.zeros((ns, 100), np.uint32)
@@ -328,7 +347,9 @@ def batched_workload(args):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "angmix-10m-d96-batched", "N": n, "D": D, "queries": nq,
                    "K_mix": "1/10/100 at 5:4:1", "P_mix": "50/100/300/500 uniform",
+                   "sigma_per_coordinate": round(float(sigma), 5), "entry_sample": E, "entries_per_query": m,
                    "graph": {"K": 32, "L": 64, "reverse": -1, "iterations": args.iterations}},
+        "entry_ms": round(ems, 3),
         "build_s": round(build_s, 3), "build_control_recall": round(float(info.recall), 4),
         "latency_us": {"p50": round(float(np.percentile(lat, 50)), 1),
                        "p99": round(float(np.percentile(lat, 99)), 1)},
@@ -381,7 +402,10 @@ def learned_workload(args):
     dC = devmem.DeviceArray((nq,), np.uint32)
     m = K - 1
     rows = []
-    for form, shortlist in (("plain", 0), ("learned", 16)):
+    for form, shortlist in (("plain", 0), ("bf16-screen", 0), ("learned", 16)):
+        # plain: exact FP32 scores of the whole sample; bf16-screen: fused BF16
+        # MFMA top-16 then exact re-rank; learned: the same in the projected space
+        g._h.call("kg_set_precision", _lib.PRECISION_BF16 if form == "bf16-screen" else _lib.PRECISION_FP32)
         if shortlist:
             rep = g.validate_entry_projection(probe, K=K, P=20, entries=m, shortlist=shortlist)
             log("probe guard: %s" % rep)
@@ -410,6 +434,7 @@ def learned_workload(args):
             f = r["form"]
             if f not in best or r["qps"] > best[f]["qps"]:
                 best[f] = r
+    g._h.call("kg_set_precision", _lib.PRECISION_FP32)
     out = {
         "metric": "search QPS at recall@10>=0.95: learned-screened vs plain entry selection",
         "value": best.get("learned", best.get("plain", {"qps": 0}))["qps"], "unit": "queries/s", "n_gpus": 1,
@@ -423,6 +448,8 @@ def learned_workload(args):
         "sweep": rows, "best": best,
         "speedup_learned_vs_plain": (round(best["learned"]["qps"] / best["plain"]["qps"], 3)
                                      if "learned" in best and "plain" in best else None),
+        "speedup_learned_vs_bf16_screen": (round(best["learned"]["qps"] / best["bf16-screen"]["qps"], 3)
+                                           if "learned" in best and "bf16-screen" in best else None),
     }
     print(json.dumps(out), flush=True)
 

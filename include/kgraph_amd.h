@@ -173,9 +173,13 @@ int kg_select_entries_device(kg_handle *h, const float *dQ, unsigned nq, size_t 
                              unsigned *d_ids, size_t ostride);
 
 /* Learned projector (no counterpart in the reference): an MLP
- * f(x) = W2 relu(W1 x + b1) + b2, D -> hidden -> out_dim, FP32 master
- * weights, products on BF16 MFMA with FP32 accumulation.
- * kg_mlp_init: Xavier-uniform weights from mt19937(seed), zero biases.
+ * f(x) = W2 relu(W1 s(x) + b1) + b2, D -> hidden -> out_dim, FP32 master
+ * weights, products on BF16 MFMA with FP32 accumulation, where
+ * s(x)[c] = (x[c] - mu[c]) * inv_sigma[c] standardises each column with the
+ * dataset's column mean and 1/std (fixed at kg_mlp_init).
+ * kg_mlp_init: column statistics of the dataset, Xavier-uniform weights from
+ *   mt19937(seed), zero biases.
+ * kg_mlp_get_normalization: mu and inv_sigma (D floats each; either may be NULL).
  * kg_mlp_train: Adam on the triplet loss max(0, |f(a)-f(p)|^2 - |f(a)-f(n)|^2
  *   + margin) over `steps` batches of `batch` triplets (a random node, one of
  *   its first M graph neighbours, a random node) drawn from the current index;
@@ -192,6 +196,7 @@ int kg_mlp_train(kg_handle *h, unsigned steps, unsigned batch, float lr, float m
 int kg_mlp_loss_grad(kg_handle *h, const unsigned *triplets, unsigned nt, float margin, float *loss, float *dW1,
                      float *db1, float *dW2, float *db2);
 int kg_mlp_get_weights(kg_handle *h, float *W1, float *b1, float *W2, float *b2);
+int kg_mlp_get_normalization(kg_handle *h, float *mu, float *inv_sigma);
 int kg_mlp_set_weights(kg_handle *h, const float *W1, const float *b1, const float *W2, const float *b2);
 int kg_mlp_project(kg_handle *h, const float *X, unsigned n, size_t stride, float *Z);
 int kg_set_entry_projection(kg_handle *h, unsigned shortlist);

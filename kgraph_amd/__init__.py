@@ -129,8 +129,7 @@ class KGraph:
         if self._hasIndex:
             init = 0
             if entries:
-                if Ks is not None or Ps is not None:
-                    raise ValueError("entries need uniform K and P")
+                # per-query K / P: query q starts from min(entries, K_q + P_q - 1) of them
                 self._h.call("kg_select_entries", ptr(q), nq, q.shape[1], entries, ptr(ids), width)
                 init = entries
             sp = SearchParams(K, M, P, S, T, epsilon, seed, init)

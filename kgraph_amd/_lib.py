@@ -90,6 +90,7 @@ SIGNATURES = {
     "kg_mlp_train": (_I, [_P, _U, _U, ctypes.c_float, ctypes.c_float, _U, _P]),
     "kg_mlp_loss_grad": (_I, [_P, _P, _U, ctypes.c_float, _P, _P, _P, _P, _P]),
     "kg_mlp_get_weights": (_I, [_P, _P, _P, _P, _P]),
+    "kg_mlp_get_normalization": (_I, [_P, _P, _P]),
     "kg_mlp_set_weights": (_I, [_P, _P, _P, _P, _P]),
     "kg_mlp_project": (_I, [_P, _P, _U, _SZ, _P]),
     "kg_set_entry_projection": (_I, [_P, _U]),

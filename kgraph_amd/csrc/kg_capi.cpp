@@ -113,8 +113,11 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
     if (sp.init && sp.T > 1) throw runtime_error("when init > 0, T must be 1.");
     if (nq == 0) return;
     if (sp.init && !d_ids) throw invalid_argument("no initial data provided via ids");
-    if (sp.init && !(sp.init < sp.K)) throw invalid_argument("L < params.K");
-    if (sp.init && (Ks || Ps)) throw invalid_argument("init > 0 needs uniform K and P");
+    // one K: the reference's rule (kgraph.cpp:463); per-query K / P (no
+    // counterpart in the reference): query q walks from min(init, K_q + P_q - 1)
+    // of its ids, what its pool holds
+    if (sp.init && !Ks && !(sp.init < sp.K)) throw invalid_argument("L < params.K");
+    if (sp.init > ostride) throw invalid_argument("init larger than the id row stride");
     if (sp.T == 0) throw invalid_argument("T must be >= 1");
     if (sp.init) {  // caller's starting points must be valid node ids
         std::vector<uint32_t> ini(size_t(nq) * ostride);
@@ -196,7 +199,9 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
     // graph-walk queries
     unsigned const ngraph = nq - unsigned(brute.size());
     if (ngraph) {
-        size_t lpw = kg_search_lds_per_wave(poolmax, kmax + 1, h.Dp);
+        // seeds sorted per trial: init ids, or P random ones (largest P of the batch)
+        unsigned const seed_cap = sp.init ? sp.init : std::max(poolmax, 1u);
+        size_t lpw = kg_search_lds_per_wave(poolmax, kmax + 1, h.Dp, seed_cap);
         unsigned wpb = std::min(h.search_wpb, kg_search_max_wpb());  // the kernel's launch bound
         while (wpb > 1 && lpw * wpb > 65536) wpb /= 2;
         size_t lpb = lpw * wpb;
@@ -276,6 +281,7 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         a.pool_cap = poolmax;
         a.res_cap = kmax + 1;
         a.lds_per_wave = unsigned(lpw);
+        a.seed_cap = seed_cap;
         KG_HIP(hipEventRecord(h.ev[0], h.stream));
         KG_HIP(kg_launch_search(h.metric, a, blocks, wpb, h.stream));
         KG_HIP(hipEventRecord(h.ev[1], h.stream));
@@ -389,48 +395,56 @@ static void select_entries_device(Handle &h, float const *dQ, unsigned nq, size_
     }
     h.qn.reserve(nq);
     KG_HIP(kg_launch_norms(h.metric, Qk, nq, h.D, unsigned(qs), h.qn.p, h.stream));
-    h.escores.reserve(size_t(nq) * h.entryE);
-    kg_entry_args a{};
+    // Four forms, all ending in the m best by exact FP32 (dist, id):
+    //   exact MFMA (integer rows and queries): fused MFMA top-m, exact as it is;
+    //   learned (kg_set_entry_projection > 0): fused MFMA top-s in the projected
+    //     space, then the s re-scored exactly;
+    //   BF16 screen (precision BF16, D <= 256): fused MFMA top-s on the rows,
+    //     then the s re-scored exactly;
+    //   otherwise: canonical FP32 scores of all E, then the top-m.
     bool const learned = h.entry_shortlist > 0;
-    bool const screen = !learned && h.precision == KG_PRECISION_BF16;
+    bool const exact_mfma = !learned && entries_exact_on_mfma(h, Qk, nq, qs);
+    bool const screen = !learned && !exact_mfma && h.precision == KG_PRECISION_BF16 && h.D <= 256;
     unsigned const sl = std::max(m, std::min(learned ? h.entry_shortlist : 16u, std::min(16u, h.entryE)));
-    if (learned) {  // score in the projected space, then re-rank the shortlist exactly
+    if (learned) {
         mlp_project_entries(h);
         h.zq.reserve(size_t(nq) * h.mlpP);
         h.hq.reserve(std::max(size_t(nq), size_t(h.entryE)) * h.mlpH);
     }
-    bool const exact_mfma = !learned && entries_exact_on_mfma(h, Qk, nq, qs);
-    if ((learned || screen) && !exact_mfma) h.eshort.reserve(size_t(nq) * sl);
-    a.rows = learned ? h.eproj.p : h.erows.p;
+    kg_entry_args a{};
+    a.rows = h.erows.p;
     a.enorms = h.enorms.p;
     a.ids = h.eids.p;
     a.E = h.entryE;
-    a.D = learned ? h.mlpP : h.D;
-    a.Dp = learned ? h.mlpP : h.Dp;
-    a.Q = learned ? h.zq.p : Qk;
+    a.D = h.D;
+    a.Dp = h.Dp;
+    a.Q = Qk;
     a.qnorms = h.qn.p;
     a.nq = nq;
-    a.qstride = learned ? h.mlpP : unsigned(qs);
-    a.m = (learned || screen) ? sl : m;
-    a.scores = h.escores.p;
-    a.out_ids = (learned || screen) ? h.eshort.p : d_ids;
-    a.ostride = (learned || screen) ? sl : unsigned(ostride);
-    if (screen) a.enorms = h.escale.p;
+    a.qstride = unsigned(qs);
+    a.m = m;
+    a.out_ids = d_ids;
+    a.ostride = unsigned(ostride);
     KG_HIP(hipEventRecord(h.ev[14], h.stream));
+    h.qsq.reserve(std::max(size_t(nq), size_t(h.entryE)));
     if (exact_mfma) {
-        a.m = m;
-        a.out_ids = d_ids;
-        a.ostride = unsigned(ostride);
-        a.enorms = h.enorms.p;
-        h.qsq.reserve(nq);
         KG_HIP(kg_launch_norms(KG_METRIC_EUCLIDEAN, Qk, nq, h.D, unsigned(qs), h.qsq.p, h.stream));
         KG_HIP(kg_launch_entries_exact_mfma(h.metric, a, h.qsq.p, h.escale.p, h.stream));
     } else if (learned || screen) {
-        if (learned) {
-            mlp_forward(h, Qk, nq, qs, h.hq.p, h.zq.p);
-            KG_HIP(kg_launch_entries(KG_METRIC_L2SQR, a, h.stream));
+        h.eshort.reserve(size_t(nq) * sl);
+        a.m = sl;
+        a.out_ids = h.eshort.p;
+        a.ostride = sl;
+        if (learned) {  // squared Euclidean in the projected space
+            mlp_forward_rows(h, Qk, nq, qs, h.hq.p, h.zq.p);
+            a.rows = h.eproj.p;
+            a.D = a.Dp = a.qstride = h.mlpP;
+            a.Q = h.zq.p;
+            KG_HIP(kg_launch_norms(KG_METRIC_EUCLIDEAN, h.zq.p, nq, h.mlpP, h.mlpP, h.qsq.p, h.stream));
+            KG_HIP(kg_launch_entries_exact_mfma(KG_METRIC_L2SQR, a, h.qsq.p, h.eprojsq.p, h.stream));
         } else {
-            KG_HIP(kg_launch_entries_bf16(h.metric, a, h.stream));
+            KG_HIP(kg_launch_norms(KG_METRIC_EUCLIDEAN, Qk, nq, h.D, unsigned(qs), h.qsq.p, h.stream));
+            KG_HIP(kg_launch_entries_exact_mfma(h.metric, a, h.qsq.p, h.escale.p, h.stream));
         }
         kg_rescore_args r{};
         r.X = h.X.p;
@@ -448,6 +462,8 @@ static void select_entries_device(Handle &h, float const *dQ, unsigned nq, size_
         r.ostride = unsigned(ostride);
         KG_HIP(kg_launch_rescore(h.metric, r, h.stream));
     } else {
+        h.escores.reserve(size_t(nq) * h.entryE);
+        a.scores = h.escores.p;
         KG_HIP(kg_launch_entries(h.metric, a, h.stream));
     }
     KG_HIP(hipEventRecord(h.ev[15], h.stream));
@@ -901,7 +917,7 @@ int kg_mlp_project(kg_handle *k, float const *X, unsigned n, size_t stride, floa
         upload_rows(h, X, false, n, h.D, stride, dx);
         dh.reserve(size_t(n) * h.mlpH);
         dz.reserve(size_t(n) * h.mlpP);
-        mlp_forward(h, dx.p, n, h.Dp, dh.p, dz.p);
+        mlp_forward_rows(h, dx.p, n, h.Dp, dh.p, dz.p);
         h.sync();
         KG_HIP(hipMemcpy(Z, dz.p, 4ull * n * h.mlpP, hipMemcpyDeviceToHost));
     });
@@ -955,6 +971,14 @@ int kg_autotune_search(kg_handle *k, float const *dQ, unsigned nq, size_t qstrid
             chosen[0] = bw;
             chosen[1] = bb;
         }
+    });
+}
+
+int kg_mlp_get_normalization(kg_handle *k, float *mu, float *inv_sigma) {
+    return guard(k, [&](Handle &h) {
+        if (!h.mlpH) throw kga::runtime_error("no projector (kg_mlp_init)");
+        if (mu) KG_HIP(hipMemcpy(mu, h.mlp_mu.p, 4ull * h.D, hipMemcpyDeviceToHost));
+        if (inv_sigma) KG_HIP(hipMemcpy(inv_sigma, h.mlp_is.p, 4ull * h.D, hipMemcpyDeviceToHost));
     });
 }
 

@@ -12,6 +12,9 @@
 //                      one sampled row, the queries sit in LDS
 //   entry_topm_kernel  one wave per query: per-lane sorted top-m lists in
 //                      registers, then m rounds of a wave-wide minimum
+//   entry_exact_kernel MFMA scoring fused with per-lane top-m lists: exact on
+//                      BF16-exact rows and queries; otherwise (BF16 screen,
+//                      projected space) a shortlist re-ranked by rescore_kernel
 #include <algorithm>
 
 #include "kg_device.hpp"
@@ -132,66 +135,13 @@ __global__ void __launch_bounds__(256) entry_topm_kernel(kg_entry_args a) {
     }
 }
 
-// BF16 screening of queries against the entry sample on MFMA
-// (v_mfma_f32_16x16x16bf16_1k, FP32 accumulation).  One wave scores 16
-// queries x 64 sampled rows (four 16x16 accumulators); operands are rounded
-// to BF16 (nearest even) as they are loaded.  The score only ranks rows for
-// one query: ||e||^2/2 - q.e for the Euclidean forms, -q.e / ||e|| for angular.
-// Operand layout (wave64): lane l holds A[m = l%16][k = 4*(l/16) + j] and
-// B[k = 4*(l/16) + j][n = l%16]; accumulator lane l holds C[4*(l/16) + j][l%16].
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ short bf16_rne(float f) {
+__device__ __forceinline__ short bf16_rne(float f) {  // round to nearest even (finite values)
     uint32_t u = __float_as_uint(f);
     u += 0x7fffu + ((u >> 16) & 1u);
     return short(u >> 16);
-}
-
-template <int METRIC>
-__global__ void __launch_bounds__(256) entry_mfma_kernel(kg_entry_args a) {
-    unsigned const lane = threadIdx.x & 63;
-    unsigned const wv = blockIdx.x * 4 + (threadIdx.x >> 6);
-    unsigned const ecols = (a.E + 63) / 64;
-    unsigned const qt = wv / ecols, et = wv - qt * ecols;
-    if (qt * 16 >= a.nq) return;
-    unsigned const r16 = lane & 15, kq = (lane >> 4) * 4;
-    unsigned const qrow = qt * 16 + r16;
-    f32x4_t acc[4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
-    for (unsigned k0 = 0; k0 < a.D; k0 += 16) {
-        bf16x4_t av;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            unsigned k = k0 + kq + j;
-            av[j] = bf16_rne((qrow < a.nq && k < a.D) ? a.Q[size_t(qrow) * a.qstride + k] : 0.f);
-        }
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            unsigned e = et * 64 + g * 16 + r16;
-            bf16x4_t bv;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                unsigned k = k0 + kq + j;
-                bv[j] = bf16_rne((e < a.E && k < a.D) ? a.rows[size_t(e) * a.Dp + k] : 0.f);
-            }
-            acc[g] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[g], 0, 0, 0);
-        }
-    }
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-        unsigned e = et * 64 + g * 16 + r16;
-        if (e >= a.E) continue;
-        float const ne = a.enorms[e];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            unsigned q = qt * 16 + (lane >> 4) * 4 + j;
-            if (q < a.nq) {
-                float dot = acc[g][j];
-                float s = (METRIC == METRIC_ANGULAR) ? -dot / ne : ne - dot;
-                a.scores[size_t(q) * a.E + e] = s;
-            }
-        }
-    }
 }
 
 // Exact re-rank of a per-query shortlist: one wave per query, one quad per
@@ -251,10 +201,12 @@ __global__ void __launch_bounds__(256) intcheck_kernel(float const *X, unsigned 
         atomicMax(flags + 1, __float_as_uint(mx));
 }
 
-// Exact entry selection on the matrix cores, for rows and queries whose BF16
-// dot products are exact (integers, |x| <= 256, D (max|q|^2 + max|e|^2) <=
-// 2^24): then every product, partial sum and norm is an integer of at most
-// 2^24 and the tile value equals the canonical FP32 distance bit for bit.
+// Entry selection on the matrix cores.  For rows and queries whose BF16 dot
+// products are exact (integers, |x| <= 256, D (max|q|^2 + max|e|^2) <= 2^24)
+// every product, partial sum and norm is an integer of at most 2^24 and the
+// tile value equals the canonical FP32 distance bit for bit, so the m best
+// are the answer; for other rows (or the projected space) the m best by the
+// BF16 value form the shortlist that rescore_kernel re-ranks exactly.
 // One block per 16 queries, 4 waves splitting the sample's 16-row tiles
 // (wave w takes tiles w, w + 4, ...).  A tile's operands: the sampled rows
 // (A, loaded per tile) and the block's 16 queries (B, kept in registers), so
@@ -430,18 +382,6 @@ hipError_t kg_launch_rescore(int metric, kg_rescore_args const &a, hipStream_t s
         case METRIC_EUCLIDEAN: hipLaunchKernelGGL(rescore_kernel<METRIC_EUCLIDEAN>, g, b, 0, s, a); break;
         default: hipLaunchKernelGGL(rescore_kernel<METRIC_ANGULAR>, g, b, 0, s, a); break;
     }
-    return hipGetLastError();
-}
-
-hipError_t kg_launch_entries_bf16(int metric, kg_entry_args const &a, hipStream_t s) {
-    if (a.m == 0 || a.m > MAXM || a.E == 0 || a.nq == 0) return hipErrorInvalidValue;
-    unsigned waves = ((a.nq + 15) / 16) * ((a.E + 63) / 64);
-    dim3 g((waves + 3) / 4), b(256);
-    if (metric == METRIC_ANGULAR)
-        hipLaunchKernelGGL(entry_mfma_kernel<METRIC_ANGULAR>, g, b, 0, s, a);
-    else
-        hipLaunchKernelGGL(entry_mfma_kernel<METRIC_EUCLIDEAN>, g, b, 0, s, a);
-    hipLaunchKernelGGL(entry_topm_kernel, dim3((a.nq + 3) / 4), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -65,6 +65,7 @@ struct Handle {
     bool entry_int = false;  // entry sample rows are all integers
     float entry_max = 0.f;   // their largest |value|
     DevBuf<float> qsq;       // ||q||^2 / 2 of the last query batch
+    DevBuf<float> eprojsq;   // ||z||^2 / 2 of the projected entry sample
     DevBuf<uint32_t> iflags;  // last build: -1 FP32 join, 0 BF16 screen + re-score, 1 exact BF16 tiles
 
     // dataset (dense oracle): N rows, row stride Dp floats (D rounded up to 4)
@@ -107,6 +108,7 @@ struct Handle {
 
     // learned projector D -> mlpH -> mlpP (FP32 master weights, Adam state)
     unsigned mlpH = 0, mlpP = 0, mlp_version = 0, adam_step = 0;
+    DevBuf<float> mlp_mu, mlp_is, mlp_xn;  // input standardisation, standardised rows
     DevBuf<float> W1, b1, W2, b2, gW1, gb1, gW2, gb2;
     DevBuf<float> mW1, vW1, mb1, vb1, mW2, vW2, mb2, vb2;
     DevBuf<float> mlp_x, mlp_h, mlp_z, mlp_dz, mlp_dh, mlp_loss;
@@ -131,7 +133,10 @@ struct Handle {
 
 // Defined in kg_mlp.cpp.
 void mlp_init(Handle &h, unsigned hidden, unsigned out, unsigned seed);
+// X already standardised (row stride `stride`).
 void mlp_forward(Handle &h, float const *X, unsigned n, size_t stride, float *Hout, float *Zout);
+// Raw rows: standardised with the projector's column mean / 1/std first.
+void mlp_forward_rows(Handle &h, float const *X, unsigned n, size_t stride, float *Hout, float *Zout);
 void mlp_train(Handle &h, unsigned steps, unsigned batch, float lr, float margin, unsigned seed, float *trace);
 void mlp_loss_grad(Handle &h, uint32_t const *triplets, unsigned nt, float margin, float *loss, float *dW1,
                    float *db1, float *dW2, float *db2);

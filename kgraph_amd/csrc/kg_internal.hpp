@@ -43,6 +43,7 @@ struct kg_search_args {
     unsigned touch_cap;
     uint32_t *qcounter;
     unsigned pool_cap, res_cap, lds_per_wave;
+    unsigned seed_cap;  // starting points a query sorts: max P (random) or init
 };
 
 struct kg_brute_args {
@@ -93,7 +94,7 @@ struct kg_rescore_args {
 };
 
 // Launchers (defined in the .hip files).
-size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp);
+size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp, unsigned seed_cap);
 // Largest waves per block search_kernel is compiled for (its launch bound);
 // kg_launch_search rejects larger shapes.
 unsigned kg_search_max_wpb();
@@ -107,9 +108,9 @@ unsigned kg_brute_splits(unsigned nq, unsigned N, unsigned K);
 hipError_t kg_launch_entries(int metric, kg_entry_args const &a, hipStream_t s);
 hipError_t kg_launch_rescore(int metric, kg_rescore_args const &a, hipStream_t s);
 // BF16 MFMA screening; a.enorms = ||e||^2/2 (Euclidean forms) or ||e|| (angular)
-hipError_t kg_launch_entries_bf16(int metric, kg_entry_args const &a, hipStream_t s);
-// Exact entry selection on MFMA for rows / queries whose BF16 tiles are exact
-// (see kg_entry.hip); qsq / esq: ||q||^2/2 and ||e||^2/2 (used for L2SQR).
+// Fused MFMA top-m entry selection (see kg_entry.hip): exact for rows /
+// queries whose BF16 tiles are exact, a screen otherwise; qsq / esq:
+// ||q||^2/2 and ||e||^2/2 (used for L2SQR).
 hipError_t kg_launch_entries_exact_mfma(int metric, kg_entry_args const &a, float const *qsq, float const *esq,
                                         hipStream_t s);
 // flags[0] |= 1 if a value of the n x D rows (row stride `stride`) is not an

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "kg_handle.hpp"
+#include "kg_internal.hpp"
 #include "kg_mlp.hpp"
 
 namespace kga {
@@ -57,6 +58,16 @@ void mlp_init(Handle &h, unsigned hidden, unsigned out, unsigned seed) {
     unsigned const D = h.D;
     h.mlpH = hidden;
     h.mlpP = out;
+    // input standardisation: per-column mean and 1/std of the dataset
+    {
+        unsigned const nb = std::min(h.N, 1024u);
+        DevBuf<double> part;
+        part.reserve(size_t(nb) * D * 2);
+        h.mlp_mu.reserve(D);
+        h.mlp_is.reserve(D);
+        KG_HIP(kg_colstats(h.X.p, h.N, D, h.Dp, part.p, nb, h.mlp_mu.p, h.mlp_is.p, h.stream));
+        h.sync();
+    }
     std::mt19937 rng(seed);
     auto init = [&](DevBuf<float> &w, size_t n, float lim) {
         std::uniform_real_distribution<float> u(-lim, lim);
@@ -81,6 +92,15 @@ void mlp_init(Handle &h, unsigned hidden, unsigned out, unsigned seed) {
     ++h.mlp_version;
 }
 
+// Forward pass of raw rows (row stride `stride`): standardised into a
+// scratch copy first.
+void mlp_forward_rows(Handle &h, float const *X, unsigned n, size_t stride, float *Hout, float *Zout) {
+    require_mlp(h);
+    h.mlp_xn.reserve(size_t(n) * h.D);
+    KG_HIP(kg_gather(X, unsigned(stride), h.D, nullptr, n, h.mlp_mu.p, h.mlp_is.p, h.mlp_xn.p, h.stream));
+    mlp_forward(h, h.mlp_xn.p, n, h.D, Hout, Zout);
+}
+
 void mlp_forward(Handle &h, float const *X, unsigned n, size_t stride, float *Hout, float *Zout) {
     require_mlp(h);
     unsigned const D = h.D, H = h.mlpH, P = h.mlpP;
@@ -103,7 +123,7 @@ void mlp_train(Handle &h, unsigned steps, unsigned batch, float lr, float margin
     KG_HIP(hipMemsetAsync(h.mlp_loss.p, 0, 4ull * std::max(steps, 1u), s));
     for (unsigned st = 0; st < steps; ++st) {
         TripletArgs t{h.goff.p, h.gids.p, h.gM.p, h.gN, batch, seed, st, h.mlp_idx.p};
-        KG_HIP(kg_triplets(t, h.X.p, h.Dp, D, h.mlp_x.p, s));
+        KG_HIP(kg_triplets(t, h.X.p, h.Dp, D, h.mlp_mu.p, h.mlp_is.p, h.mlp_x.p, s));
         mlp_forward(h, h.mlp_x.p, n, D, h.mlp_h.p, h.mlp_z.p);
         KG_HIP(kg_triplet_loss(h.mlp_z.p, P, batch, margin, h.mlp_dz.p, h.mlp_loss.p + st, s));
         backward(h, h.mlp_x.p, n, h.mlp_h.p, h.mlp_dz.p);
@@ -132,7 +152,7 @@ void mlp_loss_grad(Handle &h, uint32_t const *triplets, unsigned nt, float margi
     h.mlp_dz.reserve(size_t(n) * P);
     h.mlp_loss.reserve(1);
     KG_HIP(hipMemcpyAsync(h.mlp_idx.p, triplets, 4ull * n, hipMemcpyHostToDevice, s));
-    KG_HIP(kg_gather(h.X.p, h.Dp, D, h.mlp_idx.p, n, h.mlp_x.p, s));
+    KG_HIP(kg_gather(h.X.p, h.Dp, D, h.mlp_idx.p, n, h.mlp_mu.p, h.mlp_is.p, h.mlp_x.p, s));
     KG_HIP(hipMemsetAsync(h.mlp_loss.p, 0, 4, s));
     mlp_forward(h, h.mlp_x.p, n, D, h.mlp_h.p, h.mlp_z.p);
     KG_HIP(kg_triplet_loss(h.mlp_z.p, P, nt, margin, h.mlp_dz.p, h.mlp_loss.p, s));
@@ -152,7 +172,9 @@ void mlp_project_entries(Handle &h) {
     if (h.eproj_version == h.mlp_version && h.eproj.n >= size_t(h.entryE) * h.mlpP) return;
     h.eproj.reserve(size_t(h.entryE) * h.mlpP);
     h.hq.reserve(size_t(h.entryE) * h.mlpH);
-    mlp_forward(h, h.erows.p, h.entryE, h.Dp, h.hq.p, h.eproj.p);
+    mlp_forward_rows(h, h.erows.p, h.entryE, h.Dp, h.hq.p, h.eproj.p);
+    h.eprojsq.reserve(h.entryE);  // ||z||^2 / 2 of the projected sample
+    KG_HIP(kg_launch_norms(KG_METRIC_EUCLIDEAN, h.eproj.p, h.entryE, h.mlpP, h.mlpP, h.eprojsq.p, h.stream));
     h.sync();
     h.eproj_version = h.mlp_version;
 }

@@ -91,11 +91,46 @@ __global__ void triplets_kernel(TripletArgs t) {
     t.idx[3 * i + 2] = n;
 }
 
-__global__ void gather_kernel(float const *X, unsigned Dp, unsigned D, uint32_t const *idx, unsigned n, float *out) {
+// Rows idx[r] (or r itself when idx is NULL) of X, standardised per column:
+// out[r][c] = (X[.][c] - mu[c]) * is[c] (mu, is: the projector's input
+// normalisation; both NULL = copy).
+__global__ void gather_kernel(float const *X, unsigned Dp, unsigned D, uint32_t const *idx, unsigned n, float *out,
+                              float const *mu, float const *is) {
     size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= size_t(n) * D) return;
     size_t r = e / D, c = e - r * D;
-    out[r * D + c] = X[size_t(idx[r]) * Dp + c];
+    float v = X[size_t(idx ? idx[r] : uint32_t(r)) * Dp + c];
+    if (mu) v = (v - mu[c]) * is[c];
+    out[r * D + c] = v;
+}
+
+// Column sums / sums of squares (FP64) of rows b, b + nb, ... per block b, then
+// per column in block order: mean and 1/std (1 where the std is 0).
+__global__ void colstats_partial_kernel(float const *X, unsigned N, unsigned D, unsigned Dp, double *part) {
+    unsigned const nb = gridDim.x, b = blockIdx.x;
+    for (unsigned c = threadIdx.x; c < D; c += blockDim.x) {
+        double s = 0.0, q = 0.0;
+        for (unsigned r = b; r < N; r += nb) {
+            double v = X[size_t(r) * Dp + c];
+            s += v;
+            q += v * v;
+        }
+        part[(size_t(b) * D + c) * 2] = s;
+        part[(size_t(b) * D + c) * 2 + 1] = q;
+    }
+}
+
+__global__ void colstats_final_kernel(double const *part, unsigned nb, unsigned N, unsigned D, float *mu, float *is) {
+    unsigned const c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= D) return;
+    double s = 0.0, q = 0.0;
+    for (unsigned b = 0; b < nb; ++b) {
+        s += part[(size_t(b) * D + c) * 2];
+        q += part[(size_t(b) * D + c) * 2 + 1];
+    }
+    double const m = s / N, var = q / N - m * m;
+    mu[c] = float(m);
+    is[c] = var > 0.0 ? float(1.0 / sqrt(var)) : 1.f;
 }
 
 // One thread per triplet; Z rows 3t, 3t+1, 3t+2 (P floats each).
@@ -153,18 +188,29 @@ hipError_t kg_gemm(GemmArgs const &g, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t kg_triplets(TripletArgs const &t, float const *X, unsigned Dp, unsigned D, float *out, hipStream_t s) {
+hipError_t kg_triplets(TripletArgs const &t, float const *X, unsigned Dp, unsigned D, float const *mu,
+                       float const *is, float *out, hipStream_t s) {
     hipLaunchKernelGGL(triplets_kernel, dim3((t.B + 255) / 256), dim3(256), 0, s, t);
     size_t n = size_t(3) * t.B * D;
-    hipLaunchKernelGGL(gather_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, X, Dp, D, t.idx, 3 * t.B, out);
+    hipLaunchKernelGGL(gather_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, X, Dp, D, t.idx, 3 * t.B, out,
+                       mu, is);
     return hipGetLastError();
 }
 
-hipError_t kg_gather(float const *X, unsigned Dp, unsigned D, uint32_t const *idx, unsigned n, float *out,
-                     hipStream_t s) {
+hipError_t kg_gather(float const *X, unsigned Dp, unsigned D, uint32_t const *idx, unsigned n, float const *mu,
+                     float const *is, float *out, hipStream_t s) {
     size_t e = size_t(n) * D;
     if (e == 0) return hipSuccess;
-    hipLaunchKernelGGL(gather_kernel, dim3(unsigned((e + 255) / 256)), dim3(256), 0, s, X, Dp, D, idx, n, out);
+    hipLaunchKernelGGL(gather_kernel, dim3(unsigned((e + 255) / 256)), dim3(256), 0, s, X, Dp, D, idx, n, out, mu,
+                       is);
+    return hipGetLastError();
+}
+
+hipError_t kg_colstats(float const *X, unsigned N, unsigned D, unsigned Dp, double *part, unsigned nb, float *mu,
+                       float *is, hipStream_t s) {
+    if (N == 0 || D == 0 || nb == 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(colstats_partial_kernel, dim3(nb), dim3(256), 0, s, X, N, D, Dp, part);
+    hipLaunchKernelGGL(colstats_final_kernel, dim3((D + 255) / 256), dim3(256), 0, s, part, nb, N, D, mu, is);
     return hipGetLastError();
 }
 

@@ -28,9 +28,15 @@ struct TripletArgs {
 };
 
 hipError_t kg_gemm(GemmArgs const &g, hipStream_t s);
-hipError_t kg_triplets(TripletArgs const &t, float const *X, unsigned Dp, unsigned D, float *out, hipStream_t s);
-hipError_t kg_gather(float const *X, unsigned Dp, unsigned D, uint32_t const *idx, unsigned n, float *out,
-                     hipStream_t s);
+// Row gathers standardise the columns: out = (x - mu) * is (mu, is NULL: copy).
+hipError_t kg_triplets(TripletArgs const &t, float const *X, unsigned Dp, unsigned D, float const *mu,
+                       float const *is, float *out, hipStream_t s);
+hipError_t kg_gather(float const *X, unsigned Dp, unsigned D, uint32_t const *idx, unsigned n, float const *mu,
+                     float const *is, float *out, hipStream_t s);
+// Per-column mean and 1/std of the N x D rows (row stride Dp), FP64 sums in a
+// fixed order (part holds nb x D x 2 doubles).
+hipError_t kg_colstats(float const *X, unsigned N, unsigned D, unsigned Dp, double *part, unsigned nb, float *mu,
+                       float *is, hipStream_t s);
 hipError_t kg_triplet_loss(float const *Z, unsigned P, unsigned B, float margin, float *dZ, float *loss,
                            hipStream_t s);
 hipError_t kg_colsum(float const *G, unsigned M, unsigned N, float *out, hipStream_t s);

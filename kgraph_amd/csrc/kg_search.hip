@@ -50,13 +50,14 @@ __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(
 
 struct LdsLayout {
     size_t pid, pdist, pmeta, pbase, tkey, rid, rdist, q, fid, fdist, total;
-    __host__ __device__ LdsLayout(unsigned cap, unsigned rcap, unsigned Dp) {
+    // tcap: starting points sorted at once (P random seeds, or init ids)
+    __host__ __device__ LdsLayout(unsigned cap, unsigned rcap, unsigned Dp, unsigned tcap) {
         size_t o = 0;
         pid = o; o = align16(o + 4ull * cap);
         pdist = o; o = align16(o + 4ull * cap);
         pmeta = o; o = align16(o + 4ull * cap);
         pbase = o; o = align16(o + 4ull * cap);
-        tkey = o; o = align16(o + 8ull * cap);
+        tkey = o; o = align16(o + 8ull * tcap);
         rid = o; o = align16(o + 4ull * rcap);
         rdist = o; o = align16(o + 4ull * rcap);
         q = o; o = align16(o + 4ull * Dp);
@@ -188,9 +189,11 @@ __device__ void batch_insert(WaveLDS &w, unsigned nf, uint32_t cm, uint32_t cb, 
     wave_sync();
     if (live) w.fdist[lane] = sd;  // sorted distances for the pool entries' search
     wave_sync();
-    // pool entries move up by the number of candidates strictly below them;
-    // top chunk first, each chunk read whole before it is written
-    for (int c = int((L0 - 1) & ~(KG_WAVE - 1u)); L0 > 0 && c >= 0; c -= KG_WAVE) {
+    // pool entries move up by the number of candidates strictly below them
+    // (those before the best candidate's position stay); top chunk first, each
+    // chunk read whole before it is written
+    int const lo = int(min(fin0, L0) & ~(KG_WAVE - 1u));
+    for (int c = int((L0 - 1) & ~(KG_WAVE - 1u)); L0 > 0 && c >= lo; c -= KG_WAVE) {
         unsigned const i = unsigned(c) + lane;
         bool const mv = i < L0;
         uint32_t id = 0, meta = 0, bs = 0;
@@ -244,7 +247,7 @@ search_kernel(kg_search_args a) {
     unsigned const slot = blockIdx.x * (blockDim.x / KG_WAVE) + wib;
     unsigned const cap = a.pool_cap;
     unsigned const rcap = a.res_cap;
-    LdsLayout lay(cap, rcap, a.Dp);
+    LdsLayout lay(cap, rcap, a.Dp, a.seed_cap);
     unsigned char *base = smem + size_t(wib) * a.lds_per_wave;
     WaveLDS w;
     w.pid = reinterpret_cast<uint32_t *>(base + lay.pid);
@@ -299,8 +302,8 @@ search_kernel(kg_search_args a) {
                 }
             } else {
                 uint32_t const *ini = a.init_ids + size_t(qi) * a.ostride;
-                for (unsigned i = lane; i < a.init; i += KG_WAVE) w.pid[i] = ini[i];
-                L = a.init;
+                L = min(a.init, K + P - 1);  // = init whenever init < K (one-K calls)
+                for (unsigned i = lane; i < L; i += KG_WAVE) w.pid[i] = ini[i];
             }
             wave_sync();
             // ---- mark, score and sort the starting points (kgraph.cpp:468-476)
@@ -512,8 +515,8 @@ hipError_t kg_launch_gnode(unsigned N, uint32_t const *goff, uint32_t const *gid
     return hipGetLastError();
 }
 
-size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp) {
-    return LdsLayout(pool_cap, res_cap, Dp).total;
+size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp, unsigned seed_cap) {
+    return LdsLayout(pool_cap, res_cap, Dp, seed_cap).total;
 }
 
 unsigned kg_search_max_wpb() { return SEARCH_MAX_WPB; }

@@ -262,6 +262,7 @@ struct SearchParams {  // kgraph.h:113-126
     unsigned K, M, P, S, T;
     float epsilon;
     unsigned seed, init;
+    bool batched = false;  // one query of a per-query K / P batch
 };
 
 static char const *KGRAPH_MAGIC = "KNNGRAPH";  // kgraph.cpp:272-274
@@ -383,7 +384,9 @@ struct Index {  // KGraphImpl state (kgraph.cpp:276-286)
                     if (!flags[s]) knn[L++].id = s;
             } else {
                 if (!ids) throw std::invalid_argument("no initial data provided via ids");
-                if (!(L < params.K)) throw std::invalid_argument("L < params.K");
+                // kgraph.cpp:463; per-query K / P batches may start from up to K + P - 1 ids
+                if (!(L < (params.batched ? params.K + params.P : params.K)))
+                    throw std::invalid_argument("L < params.K");
                 for (unsigned l = 0; l < L; ++l) knn[l].id = ids[l];
             }
             for (unsigned k = 0; k < L; ++k) {
@@ -957,6 +960,10 @@ int kgo_index_search(kgo_index *h, float const *X, unsigned N, unsigned D, size_
                 SearchParams s{sp->K, sp->M, sp->P, sp->S, sp->T, sp->epsilon, sp->seed, sp->init};
                 if (Ks) s.K = Ks[q];
                 if (Ps) s.P = Ps[q];
+                // per-query K / P (an extension with no counterpart in the
+                // reference): a query walks from min(init, K + P - 1) of its ids
+                if (Ks && s.init) s.init = std::min(s.init, s.K + s.P - 1);
+                s.batched = Ks != nullptr;
                 QueryOracle o(&d, Q + q * qstride);
                 unsigned c = 0, sc = 0;
                 unsigned L = h->idx.search(o, s, ids + q * ostride, dists ? dists + q * ostride : nullptr, &c, &sc);

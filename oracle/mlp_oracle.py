@@ -7,6 +7,9 @@ this file states its mathematics independently of the HIP kernels
     h = relu(W1 x + b1),  z = W2 h + b2
     L = mean_t max(0, |z_a - z_p|^2 - |z_a - z_n|^2 + margin)
 
+where x is the standardised row (x_raw - mu) * inv_sigma; callers apply the
+standardisation (kg_mlp_get_normalization) in FP32 as the GPU gather does.
+
 `bf16=True` rounds every matrix-product operand to bfloat16 (round to
 nearest even), which is what the MFMA products consume; accumulation is FP64
 here (FP32 on the GPU), so the two agree to FP32 accumulation error.  With

@@ -56,6 +56,14 @@ def test_gpu_forward_backward_match_oracle():
     g = kgraph_amd.KGraph(X, "euclidean", l2sqr=True)
     H, P = 256, 32
     g._h.call("kg_mlp_init", H, P, 7)
+    # the input standardisation is the dataset's column mean and 1/std
+    mu = np.zeros(128, np.float32)
+    isd = np.zeros(128, np.float32)
+    g._h.call("kg_mlp_get_normalization", _lib.ptr(mu), _lib.ptr(isd))
+    X64 = X.astype(np.float64)
+    sd = X64.std(0)
+    assert np.allclose(mu, X64.mean(0), rtol=1e-6, atol=1e-6)
+    assert np.allclose(isd, np.where(sd > 0, 1 / np.where(sd > 0, sd, 1), 1), rtol=1e-5)
     W1, b1, W2, b2 = _params(128, H, P, 3)
     g._h.call("kg_mlp_set_weights", _lib.ptr(W1), _lib.ptr(b1), _lib.ptr(W2), _lib.ptr(b2))
     rs = np.random.RandomState(4)
@@ -69,7 +77,7 @@ def test_gpu_forward_backward_match_oracle():
     gb2 = np.zeros(P, np.float32)
     g._h.call("kg_mlp_loss_grad", _lib.ptr(trip), nt, margin, _lib.ptr(loss), _lib.ptr(gW1), _lib.ptr(gb1),
               _lib.ptr(gW2), _lib.ptr(gb2))
-    Xt = X[trip].astype(np.float32)
+    Xt = (X[trip].astype(np.float32) - mu) * isd  # the same FP32 operations as the gather
     ol, oW1, ob1, oW2, ob2 = mlp_oracle.loss_grad(Xt, W1, b1, W2, b2, margin, bf16=True)
     assert abs(loss[0] - ol) <= 1e-3 * abs(ol) + 1e-6
     for name, a, b in [("dW1", gW1, oW1), ("db1", gb1, ob1), ("dW2", gW2, oW2)]:
@@ -82,7 +90,7 @@ def test_gpu_forward_backward_match_oracle():
     Z = np.zeros((50, P), np.float32)
     Xq = np.ascontiguousarray(X[:50], dtype=np.float32)
     g._h.call("kg_mlp_project", _lib.ptr(Xq), 50, 128, _lib.ptr(Z))
-    _, oZ = mlp_oracle.forward(Xq, W1, b1, W2, b2, bf16=True)
+    _, oZ = mlp_oracle.forward((Xq - mu) * isd, W1, b1, W2, b2, bf16=True)
     assert np.linalg.norm(Z - oZ) / np.linalg.norm(oZ) < 1e-3
 
 

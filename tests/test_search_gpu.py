@@ -103,6 +103,29 @@ def test_search_parity_mixed_batch():
         assert g.last_comps[q] == comps[0] and g.last_scans[q] == scans[0]
 
 
+def test_search_parity_mixed_batch_with_entries():
+    # per-query K / P with selected starting points: query q walks from
+    # min(m, K_q + P_q - 1) of them (K = 1 queries included)
+    X, Q, idx = setup("intmix", L2, d=128)
+    rs = np.random.RandomState(5)
+    Ks = rs.choice([1, 10, 100], size=len(Q), p=[0.5, 0.4, 0.1]).astype(np.uint32)
+    Ps = rs.choice([5, 50, 300], size=len(Q)).astype(np.uint32)
+    g = gpu_index(X, idx, L2)
+    E, m = 512, 9
+    g.set_entry_sample(E, seed=77)
+    gi, gd = g.search(Q, K=10, P=100, Ks=Ks, Ps=Ps, entries=m, withDistance=True, stats=True)
+    samp = _entry_sample_ids(77, E, len(X))
+    si, _, _ = kgo.bruteforce(X[samp], Q, m, L2)
+    init = np.zeros((len(Q), int(Ks.max())), np.uint32)
+    init[:, :m] = samp[si]
+    oi, od, oc, ocomp, oscan = idx.search(X, Q, L2, K=10, P=100, Ks=Ks, Ps=Ps, init=m, init_ids=init)
+    assert np.array_equal(g.last_counts, oc)
+    for q in range(len(Q)):
+        n = oc[q]
+        assert np.array_equal(gi[q, :n], oi[q, :n]) and np.array_equal(gd[q, :n], od[q, :n]), q
+    assert np.array_equal(g.last_comps, ocomp) and np.array_equal(g.last_scans, oscan)
+
+
 def test_search_bruteforce_path_when_P_ge_N():
     X, Q, idx = setup("uniform", EUC, n=500, nq=20, build=dict(K=10, L=20))
     compare(X, Q, idx, EUC, K=10, P=500)
@@ -178,6 +201,28 @@ def test_entry_selection_parity(kind, metric, d, m):
     assert np.array_equal(g.last_counts, oc)
     assert np.array_equal(gi, oi) and np.array_equal(gd, od)
     assert np.array_equal(g.last_comps, ocomp) and np.array_equal(g.last_scans, oscan)
+
+
+@pytest.mark.parametrize("kind,metric,d", [("angular", ANG, 96), ("uniform", EUC, 16)])
+def test_entry_screen_bf16_reranks_exactly(kind, metric, d):
+    # BF16 MFMA screen of FP32 rows: a 16-node shortlist per query re-ranked
+    # with the exact distance -- returned in exact (dist, id) order, and equal
+    # to the exact selection for nearly every query
+    from kgraph_amd import PRECISION_BF16, PRECISION_FP32
+    X, Q, idx = setup(kind, metric, d=d)
+    g = gpu_index(X, idx, metric)
+    E, m = 2048, 7
+    g.set_entry_sample(E, seed=77)
+    g._h.call("kg_set_precision", PRECISION_BF16)
+    got = g.select_entries(Q, m)
+    g._h.call("kg_set_precision", PRECISION_FP32)
+    samp = _entry_sample_ids(77, E, len(X))
+    si, sd, _ = kgo.bruteforce(X[samp], Q, m, metric)
+    same = np.mean([np.array_equal(a, b) for a, b in zip(got, samp[si])])
+    assert same >= 0.97, same
+    for q in range(len(Q)):
+        dd = [kgo.dist(metric, X[i], Q[q]) for i in got[q]]
+        assert dd == sorted(dd)
 
 
 def test_entry_selection_lifts_recall_on_separated_clusters():

@@ -118,6 +118,8 @@ def main():
     h = _lib.Handle()
     _lib.lib().kg_set_verbosity(0)
     h.call("kg_set_data", ptr(X), N, D, D, _lib.METRIC_L2SQR)
+    rows, rb, lossless = row_storage(h)
+    log("walks gather %s rows (lossless BF16 copy: %s)" % (rows, lossless))
     dQ = devmem.DeviceArray.from_host(Q)
     dI = devmem.DeviceArray((nq, K), np.uint32)
     dD = devmem.DeviceArray((nq, K), np.float32)
@@ -153,23 +155,24 @@ def main():
         # entry points are scored exactly (FP32) for the search
         h.call("kg_set_precision", _lib.PRECISION_FP32)
         h.call("kg_set_entry_sample", args.entry_sample, 1998)
-        res = []
-        for mode, P, S in sweeps:
-            sp = SearchParams(K, 0, P, S, 1, 1e30, 1998, m if mode == "entries" else 0)
-            devmem.synchronize()
-            ts = time.perf_counter()
-            ems = 0.0
-            if mode == "entries":
-                h.call("kg_select_entries_device", dQ.ptr, nq, D, m, dI.ptr, K)
-                ems = ctypes_float(h, 7)
-            h.call("kg_search_device", dQ.ptr, nq, D, sp, None, None, dI.ptr, dD.ptr, K, dC.ptr, dCo.ptr, dSc.ptr)
-            devmem.synchronize()
-            wall = time.perf_counter() - ts
-            kms = ctypes_float(h, 0)
-            res.append(dict(mode=mode, P=P, S=S, wall_s=wall, kernel_ms=kms, entry_ms=ems, ids=dI.to_host(),
-                            dists=dD.to_host(), comps=dCo.to_host(), scans=dSc.to_host(),
-                            seeds=(m if mode == "entries" else P)))
+        res = [search_once(mode, P, S) for mode, P, S in sweeps]
         return info, build_s, res, builds
+
+    def search_once(mode, P, S):
+        sp = SearchParams(K, 0, P, S, 1, 1e30, 1998, m if mode == "entries" else 0)
+        devmem.synchronize()
+        ts = time.perf_counter()
+        ems = 0.0
+        if mode == "entries":
+            h.call("kg_select_entries_device", dQ.ptr, nq, D, m, dI.ptr, K)
+            ems = ctypes_float(h, 7)
+        h.call("kg_search_device", dQ.ptr, nq, D, sp, None, None, dI.ptr, dD.ptr, K, dC.ptr, dCo.ptr, dSc.ptr)
+        devmem.synchronize()
+        wall = time.perf_counter() - ts
+        kms = ctypes_float(h, 0)
+        return dict(mode=mode, P=P, S=S, wall_s=wall, kernel_ms=kms, entry_ms=ems, ids=dI.to_host(),
+                    dists=dD.to_host(), comps=dCo.to_host(), scans=dSc.to_host(),
+                    seeds=(m if mode == "entries" else P))
 
     for w in range(args.warmup):
         log("warmup step %d" % w)
@@ -207,9 +210,10 @@ def main():
     for r in res:
         rec = recall_at_k(r["dists"], gt_d, K)
         qps = nq / r["wall_s"]
-        # algorithmic bytes per query: a row (4*D) per scored node (starting
-        # points + comps), 4 bytes per neighbour id read, 4 per starting id
-        byt = float((4 * D * (r["seeds"] + r["comps"].astype(np.float64)) + 4 * r["scans"] + 4 * r["seeds"]).sum())
+        # algorithmic bytes per query: a row (rb*D: 4 per value, 2 when walks
+        # gather the lossless BF16 copy) per scored node (starting points +
+        # comps), 4 bytes per neighbour id read, 4 per starting id
+        byt = float((rb * D * (r["seeds"] + r["comps"].astype(np.float64)) + 4 * r["scans"] + 4 * r["seeds"]).sum())
         sweep.append(dict(mode=r["mode"], P=r["P"], S=r["S"], qps=qps, recall=rec, kernel_ms=r["kernel_ms"],
                           entry_ms=r["entry_ms"], bytes=byt, mean_comps=float(r["comps"].mean()),
                           mean_scans=float(r["scans"].mean())))
@@ -218,6 +222,19 @@ def main():
     ok = [s for s in sweep if s["recall"] >= 0.95]
     pick = max(ok, key=lambda s: s["qps"]) if ok else max(sweep, key=lambda s: s["recall"])
     achieved = pick["bytes"] / (pick["kernel_ms"] * 1e-3) / 1e9
+    # the picked setting again with the walk gathering the lossless BF16 copy
+    # of the rows (half the bytes, the same results)
+    row_cmp = None
+    if lossless and rows == "fp32":
+        h.call("kg_set_row_storage", 0)
+        r16 = search_once(pick["mode"], pick["P"], pick["S"])
+        h.call("kg_set_row_storage", 1)
+        same = bool(np.array_equal(r16["ids"], res[sweep.index(pick)]["ids"]) and
+                    np.array_equal(r16["dists"], res[sweep.index(pick)]["dists"]))
+        row_cmp = {"bf16_rows_qps": round(nq / r16["wall_s"], 1), "bf16_rows_walk_ms": round(r16["kernel_ms"], 4),
+                   "fp32_rows_walk_ms": round(pick["kernel_ms"], 4), "identical_results": same}
+        log("bf16 rows: QPS=%.0f walk %.2fms (fp32 rows %.2fms), identical results: %s" % (
+            nq / r16["wall_s"], r16["kernel_ms"], pick["kernel_ms"], same))
     # join kernel: algorithmic bytes = every join-list row once (4*D) + its id (4)
     join_bytes = float(counters[1]) * (4 * D + 4)
     join_gbs = join_bytes / (join_ms * 1e-3) / 1e9 if join_ms > 0 else 0.0
@@ -237,7 +254,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "intmix-1m-d128", "N": N, "D": D, "metric": "l2sqr",
                    "build": bp, "queries": nq, "K": K, "P": pick["P"], "S": pick["S"], "start": pick["mode"],
-                   "entry_sample": args.entry_sample, "entries_per_query": m, "precision": args.precision},
+                   "entry_sample": args.entry_sample, "entries_per_query": m, "precision": args.precision,
+                   "walk_rows": rows},
         "build_s": round(build_s, 4),
         "build_s_steps": [round(b, 4) for b in build_times],
         "build_forms": {f: {"build_s": round(b["s"], 4), "iterations": b["it"],
@@ -254,7 +272,8 @@ def main():
                             "reverse": round(rev_ms, 2)},
         "roofline": {"kernel": "search_kernel", "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": measured_traffic("search_kernel", "intmix-1m-d128", pick)},
+                     "traffic": measured_traffic("search_kernel", "intmix-1m-d128", pick, rows)},
+        "walk_row_storage": row_cmp,
         "roofline_join": {"kernel": "join_kernel", "bound": "hbm", "achieved": round(join_gbs, 1),
                           "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(join_gbs / PEAK_HBM_GBS, 4),
                           "pairs": int(counters[0]), "list_entries": int(counters[1]),
@@ -292,6 +311,7 @@ def batched_workload(args):
     h = _lib.Handle()
     _lib.lib().kg_set_verbosity(0)
     h.call("kg_set_data", ptr(X), n, D, D, _lib.METRIC_ANGULAR)
+    rows, rb, _ = row_storage(h)
     h.call("kg_set_precision", _lib.PRECISION_BF16)  # BF16 MFMA screen, exact FP32 re-score
     ip = IndexParams(args.iterations, 64, 32, 10, 100, 100, 1998, 0.002, 0.99, 0, -1)
     info = IndexInfo()
@@ -327,7 +347,7 @@ def batched_workload(args):
     comps, scans, dists = dCo.to_host(), dSc.to_host(), dD.to_host()
     wall = float(np.median(walls))
     seeds = np.minimum(m, Ks.astype(np.int64) + Ps - 1).astype(np.float64)
-    byt = float((4 * D * (seeds + comps) + 4 * scans + 4 * seeds).sum())
+    byt = float((rb * D * (seeds + comps) + 4 * scans + 4 * seeds).sum())
     # recall on a query sample against the exact scan
     ns = min(1000, nq)
     gt_i = np.zeros((ns, 100), np.uint32)
@@ -454,15 +474,25 @@ def learned_workload(args):
     print(json.dumps(out), flush=True)
 
 
-def measured_traffic(kernel, workload, pick):
+def row_storage(h):
+    """('bf16', 2) when graph walks gather the lossless BF16 copy of the rows,
+    else ('fp32', 4): the row format and its bytes per value; and whether a
+    lossless BF16 copy exists."""
+    from kgraph_amd import _lib
+    v, e = _lib.ctypes.c_int(), _lib.ctypes.c_int()
+    h.call("kg_row_storage", _lib.ctypes.byref(v), _lib.ctypes.byref(e))
+    return (("bf16", 2) if v.value else ("fp32", 4)) + (bool(e.value),)
+
+
+def measured_traffic(kernel, workload, pick, rows):
     """HBM bytes per launch of `kernel` for this configuration, from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same command
     (profiles/<kernel>_<workload>_traffic.md, a line
-    `traffic P=<P> S=<S> mode=<mode> bytes=<B>`), or None."""
+    `traffic P=<P> S=<S> mode=<mode> rows=<rows> bytes=<B>`), or None."""
     path = os.path.join(ROOT, "profiles", "%s_%s_traffic.md" % (kernel, workload))
     if not os.path.exists(path):
         return None
-    want = "traffic P=%d S=%d mode=%s bytes=" % (pick["P"], pick["S"], pick["mode"])
+    want = "traffic P=%d S=%d mode=%s rows=%s bytes=" % (pick["P"], pick["S"], pick["mode"], rows)
     for line in open(path):
         line = line.strip().strip("`")
         if line.startswith(want):

@@ -103,6 +103,17 @@ int kg_build(kg_handle *h, const kg_index_params *params, kg_index_info *info);
 /* Screening precision used by kg_build and kg_search (KG_PRECISION_*). */
 int kg_set_precision(kg_handle *h, int precision);
 
+/* Row storage for graph walks (no counterpart in the reference, whose rows are
+ * the caller's Matrix, kgraph-data.h:115-175).  kg_set_data keeps a BF16 copy
+ * of the rows (the MFMA kernels' operands); when every value is representable
+ * in BF16 the copy is lossless, and mode 0 makes walks gather it: half the
+ * bytes, the same distances and ids.  Mode 1 (default) gathers the FP32 rows:
+ * the walk is latency-bound, and on a 1M x 128 index the BF16 gathers measured
+ * slower.  kg_row_storage reports *active = 1 when walks gather the BF16 copy
+ * and *lossless = 1 when the copy is lossless (either may be NULL). */
+int kg_set_row_storage(kg_handle *h, int mode);
+int kg_row_storage(kg_handle *h, int *active, int *lossless);
+
 /* Batched KGraph::search (kgraph.h:187-197, kgraph.cpp:423-564): nq queries
  * of D floats, qstride floats apart.  Ks / Ps give per-query K and P (NULL =
  * params->K / params->P).  Row q of the results starts at ids + q*ostride

@@ -11,6 +11,12 @@
 //
 // norms_kernel computes the per-row norm the metric keeps
 // (python-api.cpp:110-114): ||x||^2/2 for 'euclidean', ||x|| for 'angular'.
+//
+// bf16_rows_kernel writes the BF16 copy of the rows (round to nearest even)
+// that the MFMA kernels take their operands from, and flags the dataset when
+// any value is not representable in BF16; when none is, the copy is lossless
+// and graph walks gather it instead of the FP32 rows (half the bytes, the same
+// distances).
 #include "kg_device.hpp"
 #include "kg_internal.hpp"
 
@@ -156,6 +162,18 @@ __global__ void __launch_bounds__(256) norms_kernel(float const *X, unsigned N, 
     }
 }
 
+__global__ void __launch_bounds__(256) bf16_rows_kernel(float4 const *X, size_t n4, uint2 *Xb, uint32_t *inexact) {
+    bool lossy = false;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n4; i += size_t(gridDim.x) * blockDim.x) {
+        float4 const v = X[i];
+        uint32_t const b0 = bf16_bits(v.x), b1 = bf16_bits(v.y), b2 = bf16_bits(v.z), b3 = bf16_bits(v.w);
+        Xb[i] = make_uint2(b0 | (b1 << 16), b2 | (b3 << 16));
+        lossy |= (b0 << 16) != __float_as_uint(v.x) || (b1 << 16) != __float_as_uint(v.y) ||
+                 (b2 << 16) != __float_as_uint(v.z) || (b3 << 16) != __float_as_uint(v.w);
+    }
+    if (lossy) *inexact = 1u;  // every writer stores the same value
+}
+
 }  // namespace
 
 unsigned kg_brute_splits(unsigned nq, unsigned N, unsigned K) {
@@ -188,5 +206,15 @@ hipError_t kg_launch_norms(int metric, float const *X, unsigned N, unsigned D, u
         case METRIC_ANGULAR: hipLaunchKernelGGL(norms_kernel<METRIC_ANGULAR>, grid, block, 0, s, X, N, D, Dp, out); break;
         default: return hipSuccess;
     }
+    return hipGetLastError();
+}
+
+hipError_t kg_launch_bf16_rows(float const *X, unsigned N, unsigned Dp, uint16_t *Xb, uint32_t *inexact,
+                               hipStream_t s) {
+    size_t const n4 = size_t(N) * Dp / 4;
+    if (n4 == 0) return hipSuccess;
+    unsigned const blocks = unsigned(std::min<size_t>((n4 + 255) / 256, 8192));
+    hipLaunchKernelGGL(bf16_rows_kernel, dim3(blocks), dim3(256), 0, s, reinterpret_cast<float4 const *>(X), n4,
+                       reinterpret_cast<uint2 *>(Xb), inexact);
     return hipGetLastError();
 }

@@ -100,6 +100,7 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
 
     NndArgs a{};
     a.X = h.X.p;
+    a.Xb = h.Xb.p;
     a.norms = h.norms.p;
     a.N = N;
     a.D = h.D;

@@ -99,7 +99,15 @@ static void set_data(Handle &h, float const *X, bool dev, unsigned N, unsigned D
     h.metric = metric;
     h.norms.reserve(std::max<unsigned>(N, 1));
     KG_HIP(kg_launch_norms(metric, h.X.p, N, D, h.Dp, h.norms.p, h.stream));
+    h.Xb.reserve(std::max<size_t>(size_t(N) * h.Dp, 4));
+    DevBuf<uint32_t> inexact;
+    inexact.reserve(1);
+    KG_HIP(hipMemsetAsync(inexact.p, 0, 4, h.stream));
+    KG_HIP(kg_launch_bf16_rows(h.X.p, N, h.Dp, h.Xb.p, inexact.p, h.stream));
+    uint32_t lossy = 1;
+    KG_HIP(hipMemcpyAsync(&lossy, inexact.p, 4, hipMemcpyDeviceToHost, h.stream));
     h.sync();
+    h.rows_bf16 = lossy == 0;
     h.has_data = true;
 }
 
@@ -237,6 +245,7 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         KG_HIP(hipMemsetAsync(h.qcounter.p, 0, 4, h.stream));
         kg_search_args a{};
         a.X = h.X.p;
+        a.Xb = (h.rows_bf16 && h.row_storage == 0) ? h.Xb.p : nullptr;
         a.norms = h.norms.p;
         a.N = N;
         a.D = h.D;
@@ -633,6 +642,20 @@ int kg_set_precision(kg_handle *k, int precision) {
         if (precision != KG_PRECISION_FP32 && precision != KG_PRECISION_BF16)
             throw kga::invalid_argument("unknown precision");
         h.precision = precision;
+    });
+}
+
+int kg_set_row_storage(kg_handle *k, int mode) {
+    return guard(k, [&](Handle &h) {
+        if (mode != 0 && mode != 1) throw kga::invalid_argument("row storage mode must be 0 or 1");
+        h.row_storage = mode;
+    });
+}
+
+int kg_row_storage(kg_handle *k, int *active, int *lossless) {
+    return guard(k, [&](Handle &h) {
+        if (active) *active = (h.has_data && h.rows_bf16 && h.row_storage == 0) ? 1 : 0;
+        if (lossless) *lossless = (h.has_data && h.rows_bf16) ? 1 : 0;
     });
 }
 

@@ -73,18 +73,40 @@ __device__ __forceinline__ float quad_reduce(float4 p) {
     return (p.x + p.y) + (p.z + p.w);
 }
 
+// BF16 bits of f, rounded to nearest even (finite values).
+__device__ __forceinline__ uint16_t bf16_bits(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+
+// Chunk c (4 elements) and element i of a data row held as FP32 or as BF16
+// bits; a BF16 element widens to FP32 exactly.
+__device__ __forceinline__ float4 load_chunk(float const *r, unsigned c) {
+    return reinterpret_cast<float4 const *>(r)[c];
+}
+__device__ __forceinline__ float4 load_chunk(uint16_t const *r, unsigned c) {
+    uint2 const v = reinterpret_cast<uint2 const *>(r)[c];
+    return make_float4(__uint_as_float(v.x << 16), __uint_as_float(v.x & 0xffff0000u), __uint_as_float(v.y << 16),
+                       __uint_as_float(v.y & 0xffff0000u));
+}
+__device__ __forceinline__ float load_elem(float const *r, unsigned i) { return r[i]; }
+__device__ __forceinline__ float load_elem(uint16_t const *r, unsigned i) { return __uint_as_float(uint32_t(r[i]) << 16); }
+
 // One quad computes the canonical SSD / dot of data row `a` (global) and
 // vector `b` (LDS or global).  t = lane within the quad (0..3).  Rows are
 // 16-byte aligned; D16 = D & ~15.
 #ifndef QM_GROUP
 #define QM_GROUP 8
 #endif
-template <int METRIC>
-__device__ __forceinline__ float quad_metric(float const *__restrict__ a, float const *__restrict__ b, unsigned D,
+// The row may be BF16 bits (RowT = uint16_t, 8-byte aligned rows): its
+// values widen exactly, so the result is the FP32 row's when every value of
+// the row is representable in BF16.
+template <int METRIC, typename RowT = float>
+__device__ __forceinline__ float quad_metric(RowT const *__restrict__ a, float const *__restrict__ b, unsigned D,
                                              unsigned t) {
     unsigned const nch = (D & ~15u) >> 2;
     float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 const *a4 = reinterpret_cast<float4 const *>(a);
     float4 const *b4 = reinterpret_cast<float4 const *>(b);
     auto step = [&p](float4 x, float4 y) {
         if (METRIC == METRIC_L2SQR) {
@@ -107,18 +129,19 @@ __device__ __forceinline__ float quad_metric(float const *__restrict__ a, float 
     for (; k + 4 * (QM_GROUP - 1) < nch; k += 4 * QM_GROUP) {
         float4 x[QM_GROUP];
 #pragma unroll
-        for (int g = 0; g < QM_GROUP; ++g) x[g] = a4[k + 4 * g];
+        for (int g = 0; g < QM_GROUP; ++g) x[g] = load_chunk(a, k + 4 * g);
 #pragma unroll
         for (int g = 0; g < QM_GROUP; ++g) step(x[g], b4[k + 4 * g]);
     }
-    for (; k < nch; k += 4) step(a4[k], b4[k]);
+    for (; k < nch; k += 4) step(load_chunk(a, k), b4[k]);
     float acc = quad_reduce(p);
     for (unsigned i = D & ~15u; i < D; ++i) {
+        float const ai = load_elem(a, i);
         if (METRIC == METRIC_L2SQR) {
-            float d = a[i] - b[i];
+            float d = ai - b[i];
             acc = __builtin_fmaf(d, d, acc);
         } else {
-            acc = __builtin_fmaf(a[i], b[i], acc);
+            acc = __builtin_fmaf(ai, b[i], acc);
         }
     }
     return acc;

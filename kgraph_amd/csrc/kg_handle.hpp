@@ -72,6 +72,11 @@ struct Handle {
     unsigned N = 0, D = 0, Dp = 0;
     int metric = KG_METRIC_L2SQR;
     DevBuf<float> X, norms;
+    // BF16 copy of the rows (same stride): MFMA operands; when rows_bf16 (every
+    // value representable) graph walks gather it instead of X
+    DevBuf<uint16_t> Xb;
+    bool rows_bf16 = false;
+    int row_storage = 1;  // 0: walks gather the lossless BF16 copy when there is one, 1: FP32 rows
     bool has_data = false;
 
     // index: CSR with 32-bit edge offsets

@@ -9,6 +9,7 @@
 struct kg_search_args {
     // data rows (row stride Dp floats, 16-byte aligned) and per-row norms
     float const *X;
+    uint16_t const *Xb;  // non-NULL: lossless BF16 copy of X (same stride), gathered instead of X
     float const *norms;
     unsigned N, D, Dp;
     // graph in CSR form
@@ -119,5 +120,9 @@ hipError_t kg_launch_intcheck(float const *X, unsigned n, unsigned D, unsigned s
                               hipStream_t s);
 hipError_t kg_launch_gather_rows(float const *X, unsigned Dp, uint32_t const *ids, unsigned E, float *out,
                                  hipStream_t s);
+// BF16 copy of N x Dp rows (Dp % 4 == 0); *inexact = 1 when a value is not
+// representable in BF16 (the caller clears it first)
+hipError_t kg_launch_bf16_rows(float const *X, unsigned N, unsigned Dp, uint16_t *Xb, uint32_t *inexact,
+                               hipStream_t s);
 hipError_t kg_launch_norms(int metric, float const *X, unsigned N, unsigned D, unsigned Dp, float *out,
                            hipStream_t s);

@@ -15,6 +15,7 @@ enum { NND_STAT_PAIRS = 0, NND_STAT_DELTA = 1, NND_STAT_M = 2, NND_STAT_LISTS = 
 
 struct NndArgs {
     float const *X;      // rows, stride Dp floats
+    uint16_t const *Xb;  // BF16 copy of the rows (MFMA operands), same stride
     float const *norms;  // per-row metric norms
     unsigned N, D, Dp;
     unsigned L, K, S, R;  // IndexParams (kgraph.h:94-110)

@@ -213,23 +213,15 @@ struct MJoinLDS {
     uint16_t tgt[MB];
 };
 
-__device__ __forceinline__ short bf16_rn(float f) {  // round to nearest even (finite values)
-    uint32_t u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return short(u >> 16);
-}
-
 template <int KS>
 __device__ __forceinline__ void mj_load(NndArgs const &a, uint32_t id, bool ok, unsigned kq, mj_bf16x4 (&op)[KS]) {
-    float const *r = a.X + size_t(ok ? id : 0u) * a.Dp;
+    // operands straight from the BF16 copy of the rows (round to nearest even)
+    uint16_t const *r = a.Xb + size_t(ok ? id : 0u) * a.Dp;
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
         unsigned const k = 16 * s + kq;
-        float4 v = (ok && k < a.Dp) ? *reinterpret_cast<float4 const *>(r + k) : make_float4(0.f, 0.f, 0.f, 0.f);
-        op[s][0] = bf16_rn(v.x);
-        op[s][1] = bf16_rn(v.y);
-        op[s][2] = bf16_rn(v.z);
-        op[s][3] = bf16_rn(v.w);
+        uint2 const v = (ok && k < a.Dp) ? *reinterpret_cast<uint2 const *>(r + k) : make_uint2(0u, 0u);
+        op[s] = __builtin_bit_cast(mj_bf16x4, v);
     }
 }
 

@@ -80,16 +80,23 @@ __device__ __forceinline__ uint32_t actual_M(kg_search_args const &a, uint32_t i
 }
 
 // Distances of ids[0..cnt) (LDS, flag bit ignored) -> out[0..cnt) (LDS).
-template <int METRIC>
+// Rows are gathered from a.X (RowT = float) or from the lossless BF16 copy
+// a.Xb (RowT = uint16_t); both give the same distances.
+template <int METRIC, typename RowT>
 __device__ __forceinline__ void batch_dist(kg_search_args const &a, uint32_t const *ids, float *out, unsigned cnt,
                                            float const *q, float qn) {
+    RowT const *X;
+    if constexpr (sizeof(RowT) == 2)
+        X = a.Xb;
+    else
+        X = a.X;
     unsigned const lane = lane_id();
     unsigned const grp = lane >> 2, t = lane & 3;
     for (unsigned b0 = 0; b0 < cnt; b0 += 16) {
         unsigned c = b0 + grp;
         if (b0 + grp < cnt) {  // whole quads are active together
             uint32_t id = ids[c] & IDMASK;
-            float const *row = a.X + size_t(id) * a.Dp;
+            RowT const *row = X + size_t(id) * a.Dp;
             float acc = quad_metric<METRIC>(row, q, a.D, t);
             float na = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[id];
             float d = finish_metric<METRIC>(acc, na, qn);
@@ -238,7 +245,7 @@ __device__ __forceinline__ uint32_t load_u32_l2(uint32_t const *p) {
 #ifndef SEARCH_WAVES_PER_EU
 #define SEARCH_WAVES_PER_EU 4
 #endif
-template <int METRIC>
+template <int METRIC, typename RowT>
 __global__ void __launch_bounds__(SEARCH_MAX_WPB * KG_WAVE) __attribute__((amdgpu_waves_per_eu(SEARCH_WAVES_PER_EU)))
 search_kernel(kg_search_args a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -313,7 +320,7 @@ search_kernel(kg_search_args a) {
                 if (ntouch + i < a.touch_cap) touched[ntouch + i] = s;
             }
             ntouch += L;
-            batch_dist<METRIC>(a, w.pid, w.pdist, L, w.q, qn);
+            batch_dist<METRIC, RowT>(a, w.pid, w.pdist, L, w.q, qn);
             for (unsigned i = lane; i < L; i += KG_WAVE)
                 w.tkey[i] = (uint64_t(ord_f32(w.pdist[i])) << 32) | w.pid[i];
             wave_sync();
@@ -395,7 +402,7 @@ search_kernel(kg_search_args a) {
                     // lane j fetches candidate j's M and CSR base while the rows load
                     uint32_t cm = 0, cb = 0;
                     if (lane < nf) cm = actual_M(a, w.fid[lane], &cb);
-                    batch_dist<METRIC>(a, w.fid, w.fdist, nf, w.q, qn);
+                    batch_dist<METRIC, RowT>(a, w.fid, w.fdist, nf, w.q, qn);
                     // the nf inserts of this batch, done at once (batch_insert)
                     n_comps += nf;
                     batch_insert(w, nf, cm, cb, qcap, L, k);
@@ -526,15 +533,25 @@ hipError_t kg_launch_search(int metric, kg_search_args const &a, unsigned blocks
     if (waves_per_block == 0 || waves_per_block > SEARCH_MAX_WPB || blocks == 0) return hipErrorInvalidValue;
     size_t lds = size_t(waves_per_block) * a.lds_per_wave;
     dim3 grid(blocks), block(waves_per_block * KG_WAVE);
+    bool const bf = a.Xb != nullptr;
     switch (metric) {
         case METRIC_L2SQR:
-            hipLaunchKernelGGL(search_kernel<METRIC_L2SQR>, grid, block, lds, s, a);
+            if (bf)
+                hipLaunchKernelGGL((search_kernel<METRIC_L2SQR, uint16_t>), grid, block, lds, s, a);
+            else
+                hipLaunchKernelGGL((search_kernel<METRIC_L2SQR, float>), grid, block, lds, s, a);
             break;
         case METRIC_EUCLIDEAN:
-            hipLaunchKernelGGL(search_kernel<METRIC_EUCLIDEAN>, grid, block, lds, s, a);
+            if (bf)
+                hipLaunchKernelGGL((search_kernel<METRIC_EUCLIDEAN, uint16_t>), grid, block, lds, s, a);
+            else
+                hipLaunchKernelGGL((search_kernel<METRIC_EUCLIDEAN, float>), grid, block, lds, s, a);
             break;
         default:
-            hipLaunchKernelGGL(search_kernel<METRIC_ANGULAR>, grid, block, lds, s, a);
+            if (bf)
+                hipLaunchKernelGGL((search_kernel<METRIC_ANGULAR, uint16_t>), grid, block, lds, s, a);
+            else
+                hipLaunchKernelGGL((search_kernel<METRIC_ANGULAR, float>), grid, block, lds, s, a);
             break;
     }
     return hipGetLastError();

@@ -88,6 +88,35 @@ def test_search_parity_epsilon():
     assert (c < 10).any() and (gd[np.arange(10)[None, :] < c[:, None]] <= eps).all()
 
 
+@pytest.mark.parametrize("kind,metric,d,lossless", [("intmix", L2, 37, True), ("intmix", ANG, 48, True),
+                                                     ("intmix", EUC, 128, True), ("uniform", L2, 16, False)])
+def test_bf16_row_storage_gives_identical_walks(kind, metric, d, lossless):
+    # integer rows in [0, 255] are exact in BF16: walks may gather the BF16
+    # copy (half the bytes) and must return the oracle's ids, distances and
+    # counters; uniform floats are not, and walks keep the FP32 rows
+    import ctypes
+    X, Q, idx = setup(kind, metric, d=d)
+    g = gpu_index(X, idx, metric)
+    active, exact = ctypes.c_int(-1), ctypes.c_int(-1)
+    g._h.call("kg_row_storage", ctypes.byref(active), ctypes.byref(exact))
+    assert active.value == 0 and exact.value == int(lossless)  # FP32 rows by default
+    res = {}
+    for mode in (0, 1):
+        g._h.call("kg_set_row_storage", mode)
+        g._h.call("kg_row_storage", ctypes.byref(active), None)
+        assert active.value == int(lossless and mode == 0)
+        gi, gd = g.search(Q, K=10, P=40, withDistance=True, stats=True)
+        res[mode] = (gi, gd, g.last_counts.copy(), g.last_comps.copy(), g.last_scans.copy())
+    for a, b in zip(res[0], res[1]):
+        assert np.array_equal(a, b)
+    ids, dd, c, comps, scans = idx.search(X, Q, metric, K=10, P=40)
+    assert np.array_equal(res[0][2], c)
+    for q in range(len(Q)):
+        assert np.array_equal(res[0][0][q, :c[q]], ids[q, :c[q]]), q
+        assert np.array_equal(res[0][1][q, :c[q]], dd[q, :c[q]]), q
+    assert np.array_equal(res[0][3], comps) and np.array_equal(res[0][4], scans)
+
+
 def test_search_parity_mixed_batch():
     X, Q, idx = setup("intmix", L2, d=128)
     rs = np.random.RandomState(3)

@@ -143,14 +143,14 @@ def main():
             tb = time.perf_counter()
             h.call("kg_build", ip, info)
             devmem.synchronize()
-            cnt = np.zeros(5, dtype=np.uint64)
-            h.call("kg_build_counters", ptr(cnt), 5)
+            cnt = np.zeros(6, dtype=np.uint64)
+            h.call("kg_build_counters", ptr(cnt), 6)
             builds[form] = dict(s=time.perf_counter() - tb, it=int(info.iterations), recall=float(info.recall),
-                                join_ms=ctypes_float(h, 3), merge_ms=ctypes_float(h, 4),
+                                iter_ms=ctypes_float(h, 8), graph_replays=int(cnt[5]),
                                 join_form=["fp32", "bf16-screen", "bf16-exact"][int(cnt[4])])
-            log("build %s: %.3fs (%d it, control recall %.4f, join %s %.1f ms, merge %.1f ms)" % (
+            log("build %s: %.3fs (%d it, control recall %.4f, join %s, iterations %.1f ms, %d graph replays)" % (
                 form, builds[form]["s"], info.iterations, info.recall, builds[form]["join_form"],
-                builds[form]["join_ms"], builds[form]["merge_ms"]))
+                builds[form]["iter_ms"], builds[form]["graph_replays"]))
         build_s = builds["bf16"]["s"]
         # entry points are scored exactly (FP32) for the search
         h.call("kg_set_precision", _lib.PRECISION_FP32)
@@ -177,6 +177,24 @@ def main():
     for w in range(args.warmup):
         log("warmup step %d" % w)
         one_step()
+    # kernel breakdown (untimed): the BF16 build with its iterations launched
+    # kernel by kernel; the timed builds replay one captured graph per iteration
+    h.call("kg_set_build_graphs", 0)
+    h.call("kg_set_precision", _lib.PRECISION_BF16)
+    einfo = IndexInfo()
+    devmem.synchronize()
+    tb = time.perf_counter()
+    h.call("kg_build", ip, einfo)
+    devmem.synchronize()
+    eager_s = time.perf_counter() - tb
+    h.call("kg_set_build_graphs", 1)
+    h.call("kg_set_precision", _lib.PRECISION_FP32)  # as the search sweep left it
+    counters = np.zeros(4, dtype=np.uint64)
+    h.call("kg_build_counters", ptr(counters), 4)
+    join_ms = ctypes_float(h, 3)
+    merge_ms = ctypes_float(h, 4)
+    upd_ms = ctypes_float(h, 5)
+    eager_iter_ms = ctypes_float(h, 8)
     steps = []
     t_all = time.perf_counter()
     for s in range(args.steps):
@@ -185,17 +203,9 @@ def main():
     total_s = time.perf_counter() - t_all
     info, build_s, res, builds = steps[-1]
     build_times = [st[1] for st in steps]
-    trace = None
-    nit = ctypes_uint()
-    h.call("kg_build_trace", None, 0, _lib.ctypes.byref(nit))
-    counters = np.zeros(4, dtype=np.uint64)
-    h.call("kg_build_counters", ptr(counters), 4)
-    join_ms = ctypes_float(h, 3)
-    merge_ms = ctypes_float(h, 4)
-    upd_ms = ctypes_float(h, 5)
     rev_ms = ctypes_float(h, 6)
-    log("build: %.3fs (%d it, control recall %.4f) join %.1fms merge %.1fms update %.1fms reverse %.1fms" % (
-        build_s, info.iterations, info.recall, join_ms, merge_ms, upd_ms, rev_ms))
+    log("build: %.3fs (%d it, control recall %.4f); kernel by kernel %.3fs: join %.1fms merge %.1fms update %.1fms "
+        "reverse %.1fms" % (build_s, info.iterations, info.recall, eager_s, join_ms, merge_ms, upd_ms, rev_ms))
 
     # exact ground truth with the brute-force kernel (not timed)
     tg = time.time()
@@ -235,8 +245,10 @@ def main():
                    "fp32_rows_walk_ms": round(pick["kernel_ms"], 4), "identical_results": same}
         log("bf16 rows: QPS=%.0f walk %.2fms (fp32 rows %.2fms), identical results: %s" % (
             nq / r16["wall_s"], r16["kernel_ms"], pick["kernel_ms"], same))
-    # join kernel: algorithmic bytes = every join-list row once (4*D) + its id (4)
-    join_bytes = float(counters[1]) * (4 * D + 4)
+    # join kernel, algorithmic bytes: every join-list row once (2 bytes per
+    # value: the MFMA form's operands are the BF16 copy) + its id (4), and a
+    # radius probe at both endpoints of every pair (8)
+    join_bytes = float(counters[1]) * (2 * D + 4) + 8.0 * float(counters[0])
     join_gbs = join_bytes / (join_ms * 1e-3) / 1e9 if join_ms > 0 else 0.0
 
     out = {
@@ -260,8 +272,12 @@ def main():
         "build_s_steps": [round(b, 4) for b in build_times],
         "build_forms": {f: {"build_s": round(b["s"], 4), "iterations": b["it"],
                             "control_recall": round(b["recall"], 4), "join_form": b["join_form"],
-                            "join_ms": round(b["join_ms"], 2), "merge_ms": round(b["merge_ms"], 2)}
+                            "iterations_ms": round(b["iter_ms"], 2), "graph_replays": b["graph_replays"]}
                         for f, b in builds.items()},
+        "build_iteration_graphs": {"graph_build_s": round(build_s, 4), "graph_iterations_ms":
+                                   round(builds["bf16" if "bf16" in builds else "fp32"]["iter_ms"], 2),
+                                   "kernel_by_kernel_build_s": round(eager_s, 4),
+                                   "kernel_by_kernel_iterations_ms": round(eager_iter_ms, 2)},
         "build_speedup_bf16_vs_fp32": (round(builds["fp32"]["s"] / builds["bf16"]["s"], 2)
                                        if "fp32" in builds else None),
         "build_iterations": int(info.iterations),

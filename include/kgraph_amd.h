@@ -162,7 +162,9 @@ int kg_get_graph(kg_handle *h, uint64_t *offsets, unsigned *ids, float *dists, u
 /* Device time in milliseconds of the last call's phases, measured with HIP
  * events on the handle's stream: which = 0 search kernel, 1 brute-force
  * kernel, 2 whole build, 3 NN-descent join kernels (sum), 4 merge kernels
- * (sum), 5 update kernels (sum), 6 reverse, 7 entry selection. */
+ * (sum), 5 update kernels (sum), 6 reverse, 7 entry selection, 8 NN-descent
+ * iterations (sum; 3-5 are measured only when iterations run kernel by
+ * kernel, see kg_set_build_graphs). */
 int kg_last_time_ms(kg_handle *h, int which, float *ms);
 
 /* Per-iteration trace of the last build: 4 floats per iteration (recall,
@@ -232,9 +234,18 @@ int kg_search_latencies(kg_handle *h, float *out_us, unsigned n);
  * kgraph.cpp:805, 814), out[1] join-list entries (sum over nodes of
  * |new| + |old|), out[2] candidates pushed to merge slots, out[3]
  * iterations, out[4] local-join form (0 FP32 quads, 1 BF16 MFMA screen with
- * exact FP32 re-score, 2 BF16 MFMA exact on these rows).  n = number of
- * entries wanted (<= 5). */
+ * exact FP32 re-score, 2 BF16 MFMA exact on these rows), out[5] iterations
+ * run as replays of the captured iteration graph.  n = number of entries
+ * wanted (<= 6). */
 int kg_build_counters(kg_handle *h, uint64_t *out, unsigned n);
+
+/* NN-descent iterations as a HIP graph (no counterpart in the reference; the
+ * loop is kgraph.cpp:927-972): with on = 1 (default) the device work of an
+ * iteration after the first (update, join, merge, statistics, control pools)
+ * is captured once per build and replayed, one graph launch per iteration,
+ * with the stop rule read back between replays; on = 0 launches the same
+ * kernels one by one and times join / merge / update separately. */
+int kg_set_build_graphs(kg_handle *h, int on);
 
 #ifdef __cplusplus
 }

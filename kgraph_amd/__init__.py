@@ -80,10 +80,14 @@ class KGraph:
     def build(self, iterations=default_iterations, L=default_L, K=default_K, S=default_S,
               controls=default_controls, delta=default_delta, recall=default_recall,
               prune=default_prune, reverse=default_reverse, *, R=default_R, seed=default_seed,
-              precision=None):
+              precision=None, graphs=None):
+        """graphs: True (default) runs the iterations as replays of a captured
+        HIP graph, False launches their kernels one by one."""
         p = IndexParams(iterations, L, K, S, R, controls, seed, delta, recall, prune, reverse)
         if precision is not None:
             self._h.call("kg_set_precision", precision)
+        if graphs is not None:
+            self._h.call("kg_set_build_graphs", 1 if graphs else 0)
         info = IndexInfo()
         self._h.call("kg_build", p, info)
         self._hasIndex = True
@@ -247,12 +251,15 @@ class KGraph:
 
     def build_counters(self):
         """Last build: {'pairs', 'list_entries', 'pushes', 'iterations',
-        'join_form'} with join_form 'fp32', 'bf16-screen' or 'bf16-exact'."""
-        out = np.zeros(5, dtype=np.uint64)
-        self._h.call("kg_build_counters", ptr(out), 5)
+        'join_form', 'graph_replays'} with join_form 'fp32', 'bf16-screen' or
+        'bf16-exact' and graph_replays the iterations run as replays of the
+        captured iteration graph."""
+        out = np.zeros(6, dtype=np.uint64)
+        self._h.call("kg_build_counters", ptr(out), 6)
         forms = {0: "fp32", 1: "bf16-screen", 2: "bf16-exact"}
         return {"pairs": int(out[0]), "list_entries": int(out[1]), "pushes": int(out[2]),
-                "iterations": int(out[3]), "join_form": forms.get(int(out[4]), "fp32")}
+                "iterations": int(out[3]), "join_form": forms.get(int(out[4]), "fp32"),
+                "graph_replays": int(out[5])}
 
     def build_trace(self):
         n = _lib.ctypes.c_uint()

@@ -70,6 +70,7 @@ SIGNATURES = {
     "kg_set_data_device": (_I, [_P, _P, _U, _U, _SZ, _I]),
     "kg_set_precision": (_I, [_P, _I]),
     "kg_set_row_storage": (_I, [_P, _I]),
+    "kg_set_build_graphs": (_I, [_P, _I]),
     "kg_row_storage": (_I, [_P, _P, _P]),
     "kg_build": (_I, [_P, ctypes.POINTER(IndexParams), ctypes.POINTER(IndexInfo)]),
     "kg_search": (_I, [_P, _P, _U, _SZ, ctypes.POINTER(SearchParams), _P, _P, _P, _P, _SZ, _P, _P, _P]),

@@ -59,7 +59,45 @@ struct NndState {
     DevBuf<uint32_t> ctl, ccount;
     DevBuf<float> cdist;
     DevBuf<float2> sqn;
-    DevBuf<uint32_t> flags;
+    DevBuf<uint32_t> flags, iter;
+};
+
+// One NN-descent iteration's device work after the first: update of the
+// previous iteration's pools (M growth, new / old split, reverse pushes), then
+// the join, the merge and the control-pool gather.  The first iteration runs
+// without the update (its pools come from init).  The same launches either go
+// straight to the stream or are captured once into a HIP graph and replayed.
+void iteration_body(int metric, NndArgs const &a, bool update, uint32_t const *ctl, unsigned nc, float *cdist,
+                    uint32_t *ccount, hipStream_t s) {
+    if (update) KG_HIP(nnd_launch_update(a, s));
+    KG_HIP(nnd_launch_iter_begin(a, update ? 1 : 0, s));
+    KG_HIP(nnd_launch_join(metric, a, s));
+    KG_HIP(nnd_launch_merge(a, s));
+    if (nc) KG_HIP(nnd_launch_gather_controls(a, ctl, nc, cdist, ccount, s));
+}
+
+// Captured iteration graph, destroyed with the build.
+struct IterGraph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t e = nullptr;
+    ~IterGraph() {
+        if (e) (void)hipGraphExecDestroy(e);
+        if (g) (void)hipGraphDestroy(g);
+    }
+    void capture(int metric, NndArgs const &a, uint32_t const *ctl, unsigned nc, float *cdist, uint32_t *ccount,
+                 hipStream_t s) {
+        KG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            iteration_body(metric, a, true, ctl, nc, cdist, ccount, s);
+        } catch (...) {
+            hipGraph_t dropped = nullptr;
+            (void)hipStreamEndCapture(s, &dropped);
+            if (dropped) (void)hipGraphDestroy(dropped);
+            throw;
+        }
+        KG_HIP(hipStreamEndCapture(s, &g));
+        KG_HIP(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
+    }
 };
 
 void replace_graph(Handle &h, unsigned N, DevBuf<uint32_t> &off, DevBuf<uint32_t> &ids, DevBuf<float> &dists,
@@ -125,6 +163,7 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     a.O = 64;
     st.ovf.reserve(size_t(N) * a.O);
     st.stats.reserve(NND_NSTATS);
+    st.iter.reserve(1);
     st.part.reserve(size_t(NND_NPART) * N);
     for (auto *b : {&st.cnt, &st.Mv, &st.found, &st.nnew, &st.nold, &st.crnew, &st.crold, &st.ccnt}) b->reserve(N);
     st.samp.reserve(size_t(N) * a.Lc);
@@ -135,6 +174,7 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     a.cand = st.cand.p;
     a.ovf = st.ovf.p;
     a.stats = st.stats.p;
+    a.iter = st.iter.p;
     a.part = st.part.p;
     a.cnt = st.cnt.p;
     a.Mv = st.Mv.p;
@@ -172,6 +212,7 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     h.join_exact = a.mfma ? int(a.exact) : -1;
     KG_HIP(hipMemsetAsync(st.ovf.p, 0xff, size_t(N) * a.O * 8, s));
     KG_HIP(hipMemsetAsync(st.found.p, 0, size_t(N) * 4, s));
+    KG_HIP(hipMemsetAsync(st.iter.p, 0, 4, s));
 
     KG_HIP(hipEventRecord(h.ev[4], s));
     // controls: exact K-NN of C random nodes (GenerateControl, kgraph.cpp:254-270)
@@ -223,29 +264,57 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     std::vector<float> cpool(size_t(nc) * p.L);
     std::vector<uint32_t> ccnt(nc);
     h.trace.clear();
-    float join_ms = 0, merge_ms = 0, upd_ms = 0;
+    float join_ms = 0, merge_ms = 0, upd_ms = 0, iter_ms = 0;
+    uint64_t replays = 0;
+    IterGraph graph;
     for (unsigned it = 0; (p.iterations == 0) || (it < p.iterations); ++it) {
         ++info.iterations;
-        a.iter = it;
-        KG_HIP(hipMemsetAsync(st.stats.p, 0, NND_NSTATS * 8, s));
-        KG_HIP(hipEventRecord(h.ev[6], s));
-        KG_HIP(nnd_launch_join(h.metric, a, s));
-        KG_HIP(hipEventRecord(h.ev[7], s));
-        KG_HIP(nnd_launch_merge(a, s));
-        KG_HIP(hipEventRecord(h.ev[8], s));
+        float ms;
+        if (h.build_graphs) {
+            // iteration 0 straight to the stream, later ones as graph replays
+            if (it == 1) graph.capture(h.metric, a, st.ctl.p, nc, st.cdist.p, st.ccount.p, s);
+            KG_HIP(hipEventRecord(h.ev[16], s));
+            if (it == 0)
+                iteration_body(h.metric, a, false, st.ctl.p, nc, st.cdist.p, st.ccount.p, s);
+            else
+                KG_HIP(hipGraphLaunch(graph.e, s)), ++replays;
+            KG_HIP(hipEventRecord(h.ev[17], s));
+        } else {
+            // the same launches one by one, with per-kernel timing
+            KG_HIP(hipEventRecord(h.ev[16], s));
+            if (it > 0) {
+                KG_HIP(hipEventRecord(h.ev[9], s));
+                KG_HIP(nnd_launch_update(a, s));
+                KG_HIP(hipEventRecord(h.ev[10], s));
+            }
+            KG_HIP(nnd_launch_iter_begin(a, it > 0 ? 1 : 0, s));
+            KG_HIP(hipEventRecord(h.ev[6], s));
+            KG_HIP(nnd_launch_join(h.metric, a, s));
+            KG_HIP(hipEventRecord(h.ev[7], s));
+            KG_HIP(nnd_launch_merge(a, s));
+            KG_HIP(hipEventRecord(h.ev[8], s));
+            if (nc) KG_HIP(nnd_launch_gather_controls(a, st.ctl.p, nc, st.cdist.p, st.ccount.p, s));
+            KG_HIP(hipEventRecord(h.ev[17], s));
+        }
         uint64_t stats[NND_NSTATS];
         KG_HIP(hipMemcpyAsync(stats, st.stats.p, sizeof(stats), hipMemcpyDeviceToHost, s));
         if (nc) {
-            KG_HIP(nnd_launch_gather_controls(a, st.ctl.p, nc, st.cdist.p, st.ccount.p, s));
             KG_HIP(hipMemcpyAsync(cpool.data(), st.cdist.p, 4 * cpool.size(), hipMemcpyDeviceToHost, s));
             KG_HIP(hipMemcpyAsync(ccnt.data(), st.ccount.p, 4ull * nc, hipMemcpyDeviceToHost, s));
         }
         h.sync();
-        float ms;
-        KG_HIP(hipEventElapsedTime(&ms, h.ev[6], h.ev[7]));
-        join_ms += ms;
-        KG_HIP(hipEventElapsedTime(&ms, h.ev[7], h.ev[8]));
-        merge_ms += ms;
+        KG_HIP(hipEventElapsedTime(&ms, h.ev[16], h.ev[17]));
+        iter_ms += ms;
+        if (!h.build_graphs) {
+            KG_HIP(hipEventElapsedTime(&ms, h.ev[6], h.ev[7]));
+            join_ms += ms;
+            KG_HIP(hipEventElapsedTime(&ms, h.ev[7], h.ev[8]));
+            merge_ms += ms;
+            if (it > 0) {
+                KG_HIP(hipEventElapsedTime(&ms, h.ev[9], h.ev[10]));
+                upd_ms += ms;
+            }
+        }
         // statistics and stop rule (kgraph.cpp:927-972)
         n_comps += stats[NND_STAT_PAIRS];
         n_lists += stats[NND_STAT_LISTS];
@@ -273,12 +342,18 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
             info.stop_condition = 2;
             break;
         }
+    }
+    if (info.stop_condition == 0) {
+        // iteration limit: the reference still runs update() after the last
+        // join (kgraph.cpp:927-972), which grows M before the extraction
+        float ms;
         KG_HIP(hipEventRecord(h.ev[9], s));
         KG_HIP(nnd_launch_update(a, s));
         KG_HIP(hipEventRecord(h.ev[10], s));
         KG_HIP(hipEventSynchronize(h.ev[10]));
         KG_HIP(hipEventElapsedTime(&ms, h.ev[9], h.ev[10]));
         upd_ms += ms;
+        iter_ms += ms;
     }
     // graph extraction (kgraph.cpp:975-988): the valid pool prefix of each node
     DevBuf<uint32_t> off, ids, M, tmp;
@@ -304,9 +379,11 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     h.build_counters[1] = n_lists;
     h.build_counters[2] = n_push;
     h.build_counters[3] = info.iterations;
+    h.build_counters[4] = replays;
     h.last_ms[3] = join_ms;
     h.last_ms[4] = merge_ms;
     h.last_ms[5] = upd_ms;
+    h.last_ms[8] = iter_ms;
     if (p.reverse) reverse_index(h, p.reverse);
     if (p.prune) prune_index(h, p.prune);
     KG_HIP(hipEventRecord(h.ev[11], s));

@@ -838,7 +838,7 @@ int kg_get_graph(kg_handle *k, uint64_t *offsets, unsigned *ids, float *dists, u
 
 int kg_last_time_ms(kg_handle *k, int which, float *ms) {
     return guard(k, [&](Handle &h) {
-        if (which < 0 || which >= 8 || !ms) throw kga::invalid_argument("bad timer");
+        if (which < 0 || which >= 9 || !ms) throw kga::invalid_argument("bad timer");
         *ms = h.last_ms[which];
     });
 }
@@ -1010,7 +1010,12 @@ int kg_build_counters(kg_handle *k, uint64_t *out, unsigned n) {
         if (!out) throw kga::invalid_argument("null argument");
         for (unsigned i = 0; i < n && i < 4; ++i) out[i] = h.build_counters[i];
         if (n > 4) out[4] = uint64_t(h.join_exact + 1);
+        if (n > 5) out[5] = h.build_counters[4];
     });
+}
+
+int kg_set_build_graphs(kg_handle *k, int on) {
+    return guard(k, [&](Handle &h) { h.build_graphs = on != 0; });
 }
 
 }  // extern "C"

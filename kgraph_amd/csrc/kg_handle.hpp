@@ -124,10 +124,11 @@ struct Handle {
     DevBuf<uint32_t> eshort;
 
     // timing + build trace
-    hipEvent_t ev[16] = {};
-    float last_ms[8] = {};
+    hipEvent_t ev[18] = {};
+    float last_ms[9] = {};
     std::vector<float> trace;
-    uint64_t build_counters[4] = {};  // pairs, list entries, candidate pushes, iterations
+    uint64_t build_counters[5] = {};  // pairs, list entries, candidate pushes, iterations, graph replays
+    bool build_graphs = true;         // NN-descent iterations run as a captured HIP graph
 
     Handle();
     ~Handle();

@@ -24,7 +24,7 @@ struct NndArgs {
     unsigned Acap, Bcap;  // join list capacities (multiples of 4)
     unsigned nnew_init;   // size of the first iteration's random "new" list
     unsigned long long seed;
-    unsigned iter;
+    uint32_t *iter;  // device counter: the iteration whose update runs next (iter_begin_kernel advances it)
     // per-node state (Nhood, kgraph.cpp:709-721)
     uint64_t *pool;  // N x L keys: ord(dist) << 32 | id << 1 | new-flag
     uint32_t *cnt;   // valid pool entries (Nhood::L)
@@ -57,6 +57,10 @@ hipError_t nnd_launch_join(int metric, NndArgs const &a, hipStream_t s);
 hipError_t nnd_launch_sqn(NndArgs const &a, float2 *out, uint32_t *flags, hipStream_t s);
 hipError_t nnd_launch_merge(NndArgs const &a, hipStream_t s);
 hipError_t nnd_launch_update(NndArgs const &a, hipStream_t s);
+// Start of an iteration's join: zero the NND_NSTATS counters and, with
+// advance, step the iteration counter (a kernel, so the whole iteration can
+// be captured in a graph without memset nodes).
+hipError_t nnd_launch_iter_begin(NndArgs const &a, int advance, hipStream_t s);
 hipError_t nnd_launch_extract(NndArgs const &a, uint32_t const *off, uint32_t *ids, float *dists, uint32_t *M,
                               hipStream_t s);
 hipError_t nnd_launch_gather_controls(NndArgs const &a, uint32_t const *ctl, unsigned nc, float *dout,

@@ -607,6 +607,11 @@ __global__ void __launch_bounds__(256) sqn_kernel(float const *X, unsigned N, un
         atomicMax(flags + 1, __float_as_uint(mx));
 }
 
+__global__ void iter_begin_kernel(uint64_t *stats, uint32_t *iter, int advance) {
+    if (threadIdx.x < NND_NSTATS) stats[threadIdx.x] = 0;
+    if (threadIdx.x == 0 && advance) *iter += 1u;
+}
+
 // Per-node statistics (join and merge write one value per node and column,
 // plain stores) summed into the NND_NSTATS counters: a block per chunk of
 // nodes, one atomic per block and column.  Same-address device-scope atomics
@@ -837,6 +842,7 @@ __global__ void __launch_bounds__(256) sample_kernel(NndArgs a) {
     uint64_t *pool = a.pool + size_t(n) * a.L;
     uint32_t *samp = a.samp + size_t(n) * a.Lc;
     unsigned const M = a.Mv[n];
+    uint32_t const it = *a.iter;
     unsigned nn = 0, no = 0;
     uint64_t const lt = (1ull << lane) - 1;
     for (unsigned b = 0; b < M; b += KG_WAVE) {
@@ -854,9 +860,9 @@ __global__ void __launch_bounds__(256) sample_kernel(NndArgs a) {
         if (isold) samp[a.Lc - 1 - (no + popc64(bo & lt))] = id;
         if (act && key_dist(k) > a.radiusM[id]) {
             if (isnew)
-                push_rev(a.rnew, a.crnew, a.R, id, n, a.iter);
+                push_rev(a.rnew, a.crnew, a.R, id, n, it);
             else
-                push_rev(a.rold, a.crold, a.R, id, n, a.iter);
+                push_rev(a.rold, a.crold, a.R, id, n, it);
         }
         nn += popc64(bn);
         no += popc64(bo);
@@ -984,6 +990,11 @@ hipError_t nnd_launch_merge(NndArgs const &a, hipStream_t s) {
     hipLaunchKernelGGL(merge_kernel, dim3((a.N + 3) / 4), dim3(256), 0, s, a);
     unsigned const nb = std::min(1024u, (a.N + 255) / 256);
     hipLaunchKernelGGL(stats_reduce_kernel, dim3(nb), dim3(256), 0, s, a.part, a.N, a.stats);
+    return hipGetLastError();
+}
+
+hipError_t nnd_launch_iter_begin(NndArgs const &a, int advance, hipStream_t s) {
+    hipLaunchKernelGGL(iter_begin_kernel, dim3(1), dim3(64), 0, s, a.stats, a.iter, advance);
     return hipGetLastError();
 }
 

@@ -100,6 +100,46 @@ def test_mfma_join_recall_and_canonical_distances(kind, metric, d, form):
             assert v == kgo.dist(metric, X[i], X[j])
 
 
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_iteration_graph_replays_match_kernel_by_kernel(precision):
+    # the same build with its iterations as replays of one captured HIP graph
+    # and with their kernels launched one by one: every iteration's statistics
+    # (read back between replays) and the final graph agree.  Candidate slots
+    # that overflow keep an order-dependent subset, so two builds agree to
+    # within sampling noise, not bit for bit.
+    from kgraph_amd import PRECISION_BF16, PRECISION_FP32
+    X = _data("intmix", 20000, 64)
+    N, K = len(X), 16
+    rows = np.arange(0, N, 7)
+    _, gt_d = kgo.linear_knn(X, rows, K, L2)
+    params = dict(K=K, L=32, S=10, R=100, iterations=8, controls=100, reverse=0, recall=1.01, delta=-1.0,
+                  precision=PRECISION_BF16 if precision == "bf16" else PRECISION_FP32)
+    out = {}
+    for graphs in (True, False):
+        g = _gpu(X, L2)
+        info = g.build(graphs=graphs, **params)
+        c = g.build_counters()
+        assert info.iterations == 8 and info.stop_condition == 0
+        assert c["graph_replays"] == (7 if graphs else 0)
+        off, ids, dd, M = g.graph()
+        out[graphs] = dict(recall=_knn_recall(off[rows], dd, gt_d, K), trace=g.build_trace(), M=M.mean(),
+                           iter_ms=g.last_time_ms(8), join_ms=g.last_time_ms(3))
+        for i in range(0, N, 211):
+            lst, dl = ids[off[i]:off[i + 1]], dd[off[i]:off[i + 1]]
+            assert i not in lst and len(np.unique(lst)) == len(lst) and (np.diff(dl) >= 0).all()
+            for j, v in zip(lst[:4], dl[:4]):
+                assert v == kgo.dist(L2, X[i], X[j])
+    a, b = out[True], out[False]
+    print("graph: recall %.4f M %.2f %.1f ms; kernels: recall %.4f M %.2f %.1f ms (join %.1f ms)" % (
+        a["recall"], a["M"], a["iter_ms"], b["recall"], b["M"], b["iter_ms"], b["join_ms"]))
+    assert abs(a["recall"] - b["recall"]) <= 0.005
+    ta, tb = a["trace"], b["trace"]  # per iteration: recall, delta, mean M, cost
+    assert np.abs(ta[:, :2] - tb[:, :2]).max() <= 0.02
+    assert (np.abs(ta[:, 2:] - tb[:, 2:]) <= 0.02 * np.abs(tb[:, 2:])).all()
+    assert abs(a["M"] - b["M"]) <= 0.02 * b["M"]
+    assert a["iter_ms"] > 0 and b["iter_ms"] > 0 and b["join_ms"] > 0 and a["join_ms"] == 0
+
+
 def test_build_stop_conditions_and_errors():
     X = _data("uniform", 5000, 8)
     g = _gpu(X, EUC)

@@ -43,7 +43,8 @@ def log(*a):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["intmix-1m-d128", "batched-10m-d96", "learned-5m-d128"],
+    ap.add_argument("--workload", choices=["intmix-1m-d128", "angular-1.2m-d96", "batched-10m-d96",
+                                           "learned-5m-d128"],
                     default="intmix-1m-d128")
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
@@ -73,11 +74,13 @@ def main():
     args = parse()
     batched = args.workload == "batched-10m-d96"
     if args.n is None:
-        args.n = {"batched-10m-d96": 10_000_000, "learned-5m-d128": 5_000_000}.get(args.workload, 1_000_000)
+        args.n = {"batched-10m-d96": 10_000_000, "learned-5m-d128": 5_000_000,
+                  "angular-1.2m-d96": 1_200_000}.get(args.workload, 1_000_000)
     if args.nq is None:
         args.nq = 100_000 if batched else 10_000
     if args.entry_sample is None:
-        args.entry_sample = {"learned-5m-d128": 65536, "batched-10m-d96": 16384}.get(args.workload, 8192)
+        args.entry_sample = {"learned-5m-d128": 65536, "batched-10m-d96": 16384,
+                             "angular-1.2m-d96": 16384}.get(args.workload, 8192)
     if args.gpus != 1:
         print("bench.py runs on one GPU: the build and search shard by node / query range across GPUs by "
               "design only (no multi-GPU path is implemented or measured).", file=sys.stderr)
@@ -98,6 +101,8 @@ def main():
         return batched_workload(args)
     if args.workload == "learned-5m-d128":
         return learned_workload(args)
+    if args.workload == "angular-1.2m-d96":
+        return precision_workload(args)
     import kgraph_amd
     from kgraph_amd import _lib, datagen, devmem
     from kgraph_amd._lib import IndexInfo, IndexParams, SearchParams, ptr
@@ -393,6 +398,143 @@ def batched_workload(args):
         "roofline": {"kernel": "search_kernel", "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": None},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def precision_workload(args):
+    """1,200,000 x 96 L2-normalised 2,000-centre mixture, angular distance:
+    the plain build (exact FP32 local join) against the BF16 form (MFMA screen,
+    exact FP32 re-score of every admitted candidate), K=64 L=128, other
+    IndexParams at the reference defaults (reverse -1).  Reports both builds'
+    time and graph recall@10, search QPS / recall@10 on each index, and the
+    error against FP64 of the stored distances and of the screened (BF16 tile)
+    distances, with the screen's error bound.  sigma as in batched-10m-d96."""
+    from kgraph_amd import _lib, datagen, devmem
+    from kgraph_amd._lib import IndexInfo, IndexParams, SearchParams, ptr
+    from oracle.kgo import recall_at_k
+    n, nq, K, D = args.n, args.nq, 10, 96
+    sigma = args.sigma if args.sigma is not None else 0.3 / np.sqrt(96)
+    t0 = time.time()
+    X = datagen.angular_mixture(n, D, 2000, 6, 7, sigma=sigma)
+    Q = datagen.angular_mixture(nq, D, 2000, 6, 8, sigma=sigma)
+    log("data generated in %.1fs" % (time.time() - t0))
+    h = _lib.Handle()
+    _lib.lib().kg_set_verbosity(0)
+    h.call("kg_set_data", ptr(X), n, D, D, _lib.METRIC_ANGULAR)
+    bp = dict(K=64, L=128, S=10, R=100, iterations=30, controls=100, seed=1998, delta=0.002, recall=0.99, prune=0,
+              reverse=-1)
+    ip = IndexParams(bp["iterations"], bp["L"], bp["K"], bp["S"], bp["R"], bp["controls"], bp["seed"], bp["delta"],
+                     bp["recall"], bp["prune"], bp["reverse"])
+    # ground truth: exact top-11 of sampled rows (self first) and top-10 of the queries
+    rs = np.random.RandomState(20)
+    rows = np.sort(rs.choice(n, size=2000, replace=False))
+    Xr = np.ascontiguousarray(X[rows])
+    ri = np.zeros((len(rows), K + 1), np.uint32)
+    rd = np.zeros((len(rows), K + 1), np.float32)
+    rc = np.zeros(len(rows), np.uint32)
+    h.call("kg_bruteforce", ptr(Xr), len(rows), D, K + 1, 1e30, ptr(ri), ptr(rd), K + 1, ptr(rc))
+    row_gt = np.array([[d for i, d in zip(ii, dd) if i != r][:K] for r, ii, dd in zip(rows, ri, rd)], np.float32)
+    gt_i = np.zeros((nq, K), np.uint32)
+    gt_d = np.zeros((nq, K), np.float32)
+    gt_c = np.zeros(nq, np.uint32)
+    h.call("kg_bruteforce", ptr(Q), nq, D, K, 1e30, ptr(gt_i), ptr(gt_d), K, ptr(gt_c))
+    dQ = devmem.DeviceArray.from_host(Q)
+    dI = devmem.DeviceArray((nq, K), np.uint32)
+    dD = devmem.DeviceArray((nq, K), np.float32)
+    dC = devmem.DeviceArray((nq,), np.uint32)
+    m = K - 1
+
+    def ang64(A, B):
+        A = A.astype(np.float64)
+        B = B.astype(np.float64)
+        return -(A * B).sum(1) / (np.linalg.norm(A, axis=1) * np.linalg.norm(B, axis=1) + 1.1920928955078125e-07)
+
+    forms, errors = {}, {}
+    for form, prec in (("fp32", _lib.PRECISION_FP32), ("bf16", _lib.PRECISION_BF16)):
+        h.call("kg_set_precision", prec)
+        for w in range(args.warmup + args.steps):
+            info = IndexInfo()
+            devmem.synchronize()
+            tb = time.perf_counter()
+            h.call("kg_build", ip, info)
+            devmem.synchronize()
+            bs = time.perf_counter() - tb
+        cnt = np.zeros(6, dtype=np.uint64)
+        h.call("kg_build_counters", ptr(cnt), 6)
+        nit = _lib.ctypes.c_uint()
+        h.call("kg_build_iterations", None, 0, _lib.ctypes.byref(nit))
+        its = np.zeros(6 * nit.value, np.uint64)
+        h.call("kg_build_iterations", ptr(its), nit.value, _lib.ctypes.byref(nit))
+        its = its.reshape(-1, 6)
+        from kgraph_amd import export_graph
+        off, ids, dist, M = export_graph(h)
+        lists = [dist[off[r]:off[r] + K] for r in rows]
+        grec = recall_at_k(lists, row_gt, K)
+        # stored distances of the sampled rows' lists against FP64
+        src = np.concatenate([np.full(off[r + 1] - off[r], r, np.int64) for r in rows])
+        dst = np.concatenate([ids[off[r]:off[r + 1]] for r in rows]).astype(np.int64)
+        stored = np.concatenate([dist[off[r]:off[r + 1]] for r in rows]).astype(np.float64)
+        serr = np.abs(stored - ang64(X[src], X[dst]))
+        errors["stored_" + form] = {"pairs": int(len(serr)), "max_abs": float(serr.max()),
+                                    "mean_abs": float(serr.mean())}
+        # search on this index from selected entry points (BF16 screen of the
+        # sample on the BF16 index, exact FP32 scores on the plain one)
+        h.call("kg_set_entry_sample", args.entry_sample, 1998)
+        sweep = []
+        for P in (10, 20, 50):
+            sp = SearchParams(K, 0, P, 32, 1, 1e30, 1998, m)
+            for w in range(args.warmup + 1):
+                devmem.synchronize()
+                ts = time.perf_counter()
+                h.call("kg_select_entries_device", dQ.ptr, nq, D, m, dI.ptr, K)
+                ems = ctypes_float(h, 7)
+                h.call("kg_search_device", dQ.ptr, nq, D, sp, None, None, dI.ptr, dD.ptr, K, dC.ptr, None, None)
+                devmem.synchronize()
+                wall = time.perf_counter() - ts
+            rec = recall_at_k(dD.to_host(), gt_d, K)
+            sweep.append({"P": P, "S": 32, "qps": round(nq / wall, 1), "recall": round(rec, 4),
+                          "entry_ms": round(ems, 3), "walk_ms": round(ctypes_float(h, 0), 3)})
+        forms[form] = {"build_s": round(bs, 4), "iterations": int(info.iterations),
+                       "control_recall": round(float(info.recall), 4), "graph_recall@10": round(grec, 4),
+                       "join_form": ["fp32", "bf16-screen", "bf16-exact"][int(cnt[4])],
+                       "iteration_join_forms": "".join("TSEP"[int(f)] for f in its[:, 3]),
+                       "search": sweep}
+        log("%s: build %.3fs (%d it, control recall %.4f, graph recall@10 %.4f), stored err max %.3g; search %s" % (
+            form, bs, info.iterations, info.recall, grec, serr.max(), sweep))
+    # screened distances: the join's BF16 tile value for list pairs (near) and
+    # random pairs (far), against FP64, with the bound the screen subtracts
+    near = np.stack([src, dst], 1)
+    far = np.stack([rs.randint(0, n, size=len(near)), rs.randint(0, n, size=len(near))], 1)
+    for name, pr in (("near", near), ("random", far)):
+        pr = np.ascontiguousarray(pr.astype(np.uint32))
+        sd = np.zeros(len(pr), np.float32)
+        eb = np.zeros(len(pr), np.float32)
+        h.call("kg_screen_pairs", ptr(pr), len(pr), ptr(sd), ptr(eb))
+        err = np.abs(sd.astype(np.float64) - ang64(X[pr[:, 0]], X[pr[:, 1]]))
+        errors["screened_" + name] = {"pairs": int(len(pr)), "max_abs": float(err.max()),
+                                      "mean_abs": float(err.mean()), "bound": float(eb.max()),
+                                      "max_err_over_bound": float((err / eb).max()),
+                                      "within_bound": float((err <= eb).mean())}
+    log("errors vs FP64: %s" % errors)
+    best = [r for r in forms["bf16"]["search"] if r["recall"] >= 0.95]
+    pick = max(best, key=lambda r: r["qps"]) if best else max(forms["bf16"]["search"], key=lambda r: r["recall"])
+    out = {
+        "metric": "build s + search QPS at recall@10>=0.95: BF16-screened vs plain FP32 GPU build (angular)",
+        "value": pick["qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (join screen bf16 MFMA)",
+        "data": "synthetic",
+        "config": {"workload": "angmix-1.2m-d96-precision", "N": n, "D": D, "queries": nq, "metric": "angular",
+                   "sigma_per_coordinate": round(float(sigma), 5), "build": bp, "K": K, "P": pick["P"],
+                   "S": pick["S"], "entry_sample": args.entry_sample, "entries_per_query": m},
+        "recall@10": pick["recall"],
+        "build_forms": forms,
+        "build_speedup_bf16_vs_fp32": round(forms["fp32"]["build_s"] / forms["bf16"]["build_s"], 2),
+        "graph_recall_delta_bf16_minus_fp32": round(forms["bf16"]["graph_recall@10"] -
+                                                    forms["fp32"]["graph_recall@10"], 4),
+        "error_vs_fp64": errors,
+        "notes": "iteration_join_forms: T exact FP32 tiles, S BF16 screen + exact re-score, E exact BF16 tiles, "
+                 "P probe (nodes split between S and T, timed)",
     }
     print(json.dumps(out), flush=True)
 

@@ -239,6 +239,33 @@ int kg_search_latencies(kg_handle *h, float *out_us, unsigned n);
  * wanted (<= 6). */
 int kg_build_counters(kg_handle *h, uint64_t *out, unsigned n);
 
+/* Local-join form when the BF16 join is a screen (KG_PRECISION_BF16 on rows
+ * that are not BF16-exact; no counterpart in the reference).  Both forms admit
+ * exactly the same candidates; which is faster depends on the data.
+ * KG_JOIN_AUTO (default): every third iteration, from the first, splits the
+ * nodes between the screen and the exact FP32 tiles (halves the first time,
+ * then the current form on all but the last eighth), times both parts, and the
+ * iterations after it use the form that was faster per node.  KG_JOIN_SCREEN /
+ * KG_JOIN_TILES force one form. */
+enum { KG_JOIN_AUTO = 0, KG_JOIN_SCREEN = 1, KG_JOIN_TILES = 2 };
+int kg_set_join_form(kg_handle *h, int form);
+
+/* Per-iteration record of the last build (no counterpart in the reference):
+ * 6 values per iteration -- pairs scored, candidates pushed, join-list
+ * entries, join form (0 FP32 tiles, 1 BF16 screen + exact re-score, 2 exact
+ * BF16 tiles, 3 probe: the nodes split between screen and tiles), join
+ * kernel time in microseconds (0 when the iteration ran as a graph replay),
+ * whole-iteration device time in microseconds.  Copies
+ * min(cap, iterations) records; *n = iterations. */
+int kg_build_iterations(kg_handle *h, uint64_t *out, unsigned cap, unsigned *n);
+
+/* The BF16 MFMA local join's distance for n id pairs (pairs[2p], pairs[2p+1])
+ * and the bound on its error that the join's screen uses (no counterpart in
+ * the reference): screened[p] is bit-for-bit the value the join computes for
+ * the pair, |screened[p] - exact distance| <= bound[p].  For measuring the
+ * screen against an FP64 ground truth; D <= 256. */
+int kg_screen_pairs(kg_handle *h, const uint32_t *pairs, unsigned n, float *screened, float *bound);
+
 /* NN-descent iterations as a HIP graph (no counterpart in the reference; the
  * loop is kgraph.cpp:927-972): with on = 1 (default) the device work of an
  * iteration after the first (update, join, merge, statistics, control pools)

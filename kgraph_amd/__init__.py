@@ -80,14 +80,19 @@ class KGraph:
     def build(self, iterations=default_iterations, L=default_L, K=default_K, S=default_S,
               controls=default_controls, delta=default_delta, recall=default_recall,
               prune=default_prune, reverse=default_reverse, *, R=default_R, seed=default_seed,
-              precision=None, graphs=None):
+              precision=None, graphs=None, join=None):
         """graphs: True (default) runs the iterations as replays of a captured
-        HIP graph, False launches their kernels one by one."""
+        HIP graph, False launches their kernels one by one.  join: 'auto'
+        (default), 'screen' or 'tiles' -- the local-join form when the BF16
+        join is a screen (precision BF16 on rows that are not BF16-exact)."""
         p = IndexParams(iterations, L, K, S, R, controls, seed, delta, recall, prune, reverse)
         if precision is not None:
             self._h.call("kg_set_precision", precision)
         if graphs is not None:
             self._h.call("kg_set_build_graphs", 1 if graphs else 0)
+        if join is not None:
+            self._h.call("kg_set_join_form", {"auto": _lib.JOIN_AUTO, "screen": _lib.JOIN_SCREEN,
+                                              "tiles": _lib.JOIN_TILES}[join])
         info = IndexInfo()
         self._h.call("kg_build", p, info)
         self._hasIndex = True
@@ -260,6 +265,17 @@ class KGraph:
         return {"pairs": int(out[0]), "list_entries": int(out[1]), "pushes": int(out[2]),
                 "iterations": int(out[3]), "join_form": forms.get(int(out[4]), "fp32"),
                 "graph_replays": int(out[5])}
+
+    def build_iterations(self):
+        """Last build, one row per iteration: pairs, pushes, list entries,
+        join form (0 FP32 tiles, 1 BF16 screen, 2 exact BF16 tiles, 3 probe:
+        screen and tiles on half of the nodes each), join kernel us (0 for
+        graph replays), iteration us."""
+        n = _lib.ctypes.c_uint()
+        self._h.call("kg_build_iterations", None, 0, _lib.ctypes.byref(n))
+        out = np.zeros(6 * max(n.value, 1), dtype=np.uint64)
+        self._h.call("kg_build_iterations", ptr(out), n.value, _lib.ctypes.byref(n))
+        return out[:6 * n.value].reshape(-1, 6)
 
     def build_trace(self):
         n = _lib.ctypes.c_uint()

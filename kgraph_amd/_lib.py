@@ -29,6 +29,10 @@ FORMAT_TEXT = 128
 PRECISION_FP32 = 0
 PRECISION_BF16 = 1
 
+JOIN_AUTO = 0
+JOIN_SCREEN = 1
+JOIN_TILES = 2
+
 
 class IndexParams(ctypes.Structure):
     """KGraph::IndexParams (kgraph.h:94-110)."""
@@ -71,6 +75,9 @@ SIGNATURES = {
     "kg_set_precision": (_I, [_P, _I]),
     "kg_set_row_storage": (_I, [_P, _I]),
     "kg_set_build_graphs": (_I, [_P, _I]),
+    "kg_screen_pairs": (_I, [_P, _P, _I, _P, _P]),
+    "kg_build_iterations": (_I, [_P, _P, _I, _P]),
+    "kg_set_join_form": (_I, [_P, _I]),
     "kg_row_storage": (_I, [_P, _P, _P]),
     "kg_build": (_I, [_P, ctypes.POINTER(IndexParams), ctypes.POINTER(IndexInfo)]),
     "kg_search": (_I, [_P, _P, _U, _SZ, ctypes.POINTER(SearchParams), _P, _P, _P, _P, _SZ, _P, _P, _P]),

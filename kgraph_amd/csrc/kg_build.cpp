@@ -62,6 +62,22 @@ struct NndState {
     DevBuf<uint32_t> flags, iter;
 };
 
+// Error-bound constants of the BF16 screen.  Rounding to BF16 (8 significant
+// bits, round to nearest even) changes each operand by at most u = 2^-8
+// relative, so a product of two rounded operands (exact in FP32) is off by at
+// most (2u + u^2) |a_i b_i|; the MFMA's FP32 accumulation of D (+ zero
+// padding) products adds at most gamma = (D + 16) 2^-23 times the sum of their
+// magnitudes.  With sum |a_i b_i| <= |a| |b|:
+//   |G - a.b| <= c |a| |b|,  c = (2u + u^2 + gamma) * 1.02,
+// the 2 % margin covering (1+u)^2 on the gamma term and the rounding of the
+// bound itself.  g = 4 gamma bounds the relative error of the stored norms and
+// of the few FP32 operations that turn G into a distance.
+void screen_constants(unsigned D, NndArgs &a) {
+    double const gam = (D + 16) * std::ldexp(1.0, -23);
+    a.scr_c = float((std::ldexp(1.0, -7) + std::ldexp(1.0, -16) + gam) * 1.02);
+    a.scr_g = float(4 * gam);
+}
+
 // One NN-descent iteration's device work after the first: update of the
 // previous iteration's pools (M growth, new / old split, reverse pushes), then
 // the join, the merge and the control-pool gather.  The first iteration runs
@@ -141,6 +157,8 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     a.Xb = h.Xb.p;
     a.norms = h.norms.p;
     a.N = N;
+    a.n0 = 0;
+    a.n1 = N;
     a.D = h.D;
     a.Dp = h.Dp;
     a.L = p.L;
@@ -205,9 +223,7 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
         a.sqn = st.sqn.p;
         a.mfma = 1;
         a.exact = (fl[0] == 0 && 2.0 * h.D * double(mx) * double(mx) <= 16777216.0) ? 1u : 0u;
-        double const gam = (h.D + 16) * std::ldexp(1.0, -23);
-        a.scr_c = float((std::ldexp(1.0, -8) + std::ldexp(1.0, -17) + gam) * 1.02);
-        a.scr_g = float(4 * gam);
+        screen_constants(h.D, a);
     }
     h.join_exact = a.mfma ? int(a.exact) : -1;
     KG_HIP(hipMemsetAsync(st.ovf.p, 0xff, size_t(N) * a.O * 8, s));
@@ -264,23 +280,43 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     std::vector<float> cpool(size_t(nc) * p.L);
     std::vector<uint32_t> ccnt(nc);
     h.trace.clear();
+    h.iter_stats.clear();
     float join_ms = 0, merge_ms = 0, upd_ms = 0, iter_ms = 0;
     uint64_t replays = 0;
-    IterGraph graph;
+    // Join form per iteration.  On float rows the BF16 join is a screen with an
+    // exact FP32 re-score, and whether it beats the exact FP32 tiles depends on
+    // the data (how many pairs land within the screen's error bound of a
+    // radius) and drifts as the lists converge.  Both forms admit exactly the
+    // same candidates, so in auto mode a probe iteration (every third, from the
+    // first) splits the nodes between the two forms, times both parts, and the
+    // next iterations use the faster per node.  The first probe splits in
+    // halves; later ones give the current form all but the last eighth.
+    NndArgs a_tile = a;
+    a_tile.mfma = 0;
+    bool const screen = a.mfma && !a.exact;
+    int const form = screen ? h.join_form : KG_JOIN_AUTO;
+    bool use_mfma = a.mfma && form != KG_JOIN_TILES;
+    unsigned const half = N / 2, eighth = N / 8;
+    IterGraph graph[2];  // [0] FP32 tiles, [1] MFMA form
     for (unsigned it = 0; (p.iterations == 0) || (it < p.iterations); ++it) {
         ++info.iterations;
         float ms;
-        if (h.build_graphs) {
-            // iteration 0 straight to the stream, later ones as graph replays
-            if (it == 1) graph.capture(h.metric, a, st.ctl.p, nc, st.cdist.p, st.ccount.p, s);
+        bool const probe = screen && form == KG_JOIN_AUTO && it % 3 == 0 && eighth > 0;
+        // probe split: [0, cut) runs form `lead`, [cut, N) the other form
+        bool const lead = it == 0 ? true : use_mfma;
+        unsigned const cut = it == 0 ? half : N - eighth;
+        NndArgs const &ar = use_mfma ? a : a_tile;
+        int rec_form = probe ? 3 : (ar.mfma ? (ar.exact ? 2 : 1) : 0);
+        float jms = 0.f;
+        if (h.build_graphs && !probe && it > 0) {
+            IterGraph &gr = graph[use_mfma ? 1 : 0];
+            if (!gr.e) gr.capture(h.metric, ar, st.ctl.p, nc, st.cdist.p, st.ccount.p, s);
             KG_HIP(hipEventRecord(h.ev[16], s));
-            if (it == 0)
-                iteration_body(h.metric, a, false, st.ctl.p, nc, st.cdist.p, st.ccount.p, s);
-            else
-                KG_HIP(hipGraphLaunch(graph.e, s)), ++replays;
+            KG_HIP(hipGraphLaunch(gr.e, s));
+            ++replays;
             KG_HIP(hipEventRecord(h.ev[17], s));
         } else {
-            // the same launches one by one, with per-kernel timing
+            // the launches one by one, with per-kernel timing
             KG_HIP(hipEventRecord(h.ev[16], s));
             if (it > 0) {
                 KG_HIP(hipEventRecord(h.ev[9], s));
@@ -289,7 +325,16 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
             }
             KG_HIP(nnd_launch_iter_begin(a, it > 0 ? 1 : 0, s));
             KG_HIP(hipEventRecord(h.ev[6], s));
-            KG_HIP(nnd_launch_join(h.metric, a, s));
+            if (probe) {
+                NndArgs first = lead ? a : a_tile, second = lead ? a_tile : a;
+                first.n1 = cut;
+                second.n0 = cut;
+                KG_HIP(nnd_launch_join(h.metric, first, s));
+                KG_HIP(hipEventRecord(h.ev[18], s));
+                KG_HIP(nnd_launch_join(h.metric, second, s));
+            } else {
+                KG_HIP(nnd_launch_join(h.metric, ar, s));
+            }
             KG_HIP(hipEventRecord(h.ev[7], s));
             KG_HIP(nnd_launch_merge(a, s));
             KG_HIP(hipEventRecord(h.ev[8], s));
@@ -305,15 +350,25 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
         h.sync();
         KG_HIP(hipEventElapsedTime(&ms, h.ev[16], h.ev[17]));
         iter_ms += ms;
-        if (!h.build_graphs) {
-            KG_HIP(hipEventElapsedTime(&ms, h.ev[6], h.ev[7]));
-            join_ms += ms;
+        float const it_ms = ms;
+        bool const eager = !(h.build_graphs && !probe && it > 0);
+        if (eager) {
+            KG_HIP(hipEventElapsedTime(&jms, h.ev[6], h.ev[7]));
+            join_ms += jms;
             KG_HIP(hipEventElapsedTime(&ms, h.ev[7], h.ev[8]));
             merge_ms += ms;
             if (it > 0) {
                 KG_HIP(hipEventElapsedTime(&ms, h.ev[9], h.ev[10]));
                 upd_ms += ms;
             }
+        }
+        if (probe) {
+            float t_first, t_second;
+            KG_HIP(hipEventElapsedTime(&t_first, h.ev[6], h.ev[18]));
+            KG_HIP(hipEventElapsedTime(&t_second, h.ev[18], h.ev[7]));
+            double const c_first = double(t_first) / cut, c_second = double(t_second) / (N - cut);  // per node
+            double const c_screen = lead ? c_first : c_second, c_tile = lead ? c_second : c_first;
+            use_mfma = c_screen <= c_tile;
         }
         // statistics and stop rule (kgraph.cpp:927-972)
         n_comps += stats[NND_STAT_PAIRS];
@@ -330,6 +385,8 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
         info.recall = nc ? float(sr / nc) : 0.f;
         info.accuracy = nc ? float(sa / nc) : 0.f;
         h.trace.insert(h.trace.end(), {info.recall, info.delta, info.M, info.cost});
+        h.iter_stats.insert(h.iter_stats.end(), {stats[NND_STAT_PAIRS], stats[NND_STAT_PUSHES], stats[NND_STAT_LISTS],
+                                                 uint64_t(rec_form), uint64_t(jms * 1000.f), uint64_t(it_ms * 1000.f)});
         if (verbosity > 0) {
             fprintf(stderr, "iteration: %u recall: %g accuracy: %g cost: %g M: %g delta: %g\n", info.iterations,
                     info.recall, info.accuracy, info.cost, info.M, info.delta);
@@ -565,6 +622,40 @@ void prune_index(Handle &h, unsigned level) {
     if (!h.has_graph) throw runtime_error("no index");
     if (level & 1) prune1_index(h);
     if (level & 2) prune2_index(h);
+}
+
+void screen_pairs(Handle &h, uint32_t const *pairs, unsigned n, float *screened, float *bound) {
+    if (!h.has_data) throw runtime_error("no dataset");
+    if (h.D > 256) throw invalid_argument("BF16 tiles need D <= 256");
+    if (n && (!pairs || !screened || !bound)) throw invalid_argument("null argument");
+    for (size_t i = 0; i < 2ull * n; ++i)
+        if (pairs[i] >= h.N) throw invalid_argument("pair id out of range");
+    if (n == 0) return;
+    hipStream_t s = h.stream;
+    NndArgs a{};
+    a.X = h.X.p;
+    a.Xb = h.Xb.p;
+    a.norms = h.norms.p;
+    a.N = h.N;
+    a.D = h.D;
+    a.Dp = h.Dp;
+    DevBuf<float2> sqn;
+    DevBuf<uint32_t> flags, dp;
+    DevBuf<float> dd, de;
+    sqn.reserve(h.N);
+    flags.reserve(2);
+    dp.reserve(2ull * n);
+    dd.reserve(n);
+    de.reserve(n);
+    KG_HIP(hipMemsetAsync(flags.p, 0, 8, s));
+    KG_HIP(nnd_launch_sqn(a, sqn.p, flags.p, s));
+    a.sqn = sqn.p;
+    screen_constants(h.D, a);
+    KG_HIP(hipMemcpyAsync(dp.p, pairs, 8ull * n, hipMemcpyHostToDevice, s));
+    KG_HIP(nnd_launch_screen_pairs(h.metric, a, dp.p, n, dd.p, de.p, s));
+    KG_HIP(hipMemcpyAsync(screened, dd.p, 4ull * n, hipMemcpyDeviceToHost, s));
+    KG_HIP(hipMemcpyAsync(bound, de.p, 4ull * n, hipMemcpyDeviceToHost, s));
+    h.sync();
 }
 
 }  // namespace kga

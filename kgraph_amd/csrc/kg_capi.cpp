@@ -852,6 +852,15 @@ int kg_build_trace(kg_handle *k, float *out, unsigned cap, unsigned *n) {
     });
 }
 
+int kg_build_iterations(kg_handle *k, uint64_t *out, unsigned cap, unsigned *n) {
+    return guard(k, [&](Handle &h) {
+        unsigned it = unsigned(h.iter_stats.size() / 6);
+        if (n) *n = it;
+        if (out)
+            for (unsigned i = 0; i < std::min(it, cap) * 6; ++i) out[i] = h.iter_stats[i];
+    });
+}
+
 int kg_set_entry_sample(kg_handle *k, unsigned E, unsigned seed) {
     return guard(k, [&](Handle &h) { set_entry_sample(h, E, seed); });
 }
@@ -1011,6 +1020,18 @@ int kg_build_counters(kg_handle *k, uint64_t *out, unsigned n) {
         for (unsigned i = 0; i < n && i < 4; ++i) out[i] = h.build_counters[i];
         if (n > 4) out[4] = uint64_t(h.join_exact + 1);
         if (n > 5) out[5] = h.build_counters[4];
+    });
+}
+
+int kg_screen_pairs(kg_handle *k, const uint32_t *pairs, unsigned n, float *screened, float *bound) {
+    return guard(k, [&](Handle &h) { screen_pairs(h, pairs, n, screened, bound); });
+}
+
+int kg_set_join_form(kg_handle *k, int form) {
+    return guard(k, [&](Handle &h) {
+        if (form != KG_JOIN_AUTO && form != KG_JOIN_SCREEN && form != KG_JOIN_TILES)
+            throw kga::invalid_argument("unknown join form");
+        h.join_form = form;
     });
 }
 

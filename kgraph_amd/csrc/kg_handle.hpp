@@ -124,11 +124,13 @@ struct Handle {
     DevBuf<uint32_t> eshort;
 
     // timing + build trace
-    hipEvent_t ev[18] = {};
+    hipEvent_t ev[20] = {};
     float last_ms[9] = {};
     std::vector<float> trace;
     uint64_t build_counters[5] = {};  // pairs, list entries, candidate pushes, iterations, graph replays
+    std::vector<uint64_t> iter_stats;  // per iteration: pairs, pushes, list entries, join form, join us, iteration us
     bool build_graphs = true;         // NN-descent iterations run as a captured HIP graph
+    int join_form = KG_JOIN_AUTO;     // BF16 screen vs exact FP32 tiles on float rows (KG_JOIN_*)
 
     Handle();
     ~Handle();
@@ -152,5 +154,6 @@ void mlp_project_entries(Handle &h);
 void build_index(Handle &h, kg_index_params const &params, kg_index_info *info);
 void reverse_index(Handle &h, int rev);
 void prune_index(Handle &h, unsigned level);
+void screen_pairs(Handle &h, uint32_t const *pairs, unsigned n, float *screened, float *bound);
 
 }  // namespace kga

@@ -18,6 +18,7 @@ struct NndArgs {
     uint16_t const *Xb;  // BF16 copy of the rows (MFMA operands), same stride
     float const *norms;  // per-row metric norms
     unsigned N, D, Dp;
+    unsigned n0, n1;      // nodes the join kernels process: [n0, n1)
     unsigned L, K, S, R;  // IndexParams (kgraph.h:94-110)
     unsigned C, Cp2;      // candidate slots per node (and the next power of two)
     unsigned Lc;          // sample-list capacity per node, max(L, 2S)
@@ -57,6 +58,11 @@ hipError_t nnd_launch_join(int metric, NndArgs const &a, hipStream_t s);
 hipError_t nnd_launch_sqn(NndArgs const &a, float2 *out, uint32_t *flags, hipStream_t s);
 hipError_t nnd_launch_merge(NndArgs const &a, hipStream_t s);
 hipError_t nnd_launch_update(NndArgs const &a, hipStream_t s);
+// The join's BF16 MFMA distance and its error bound for n (a, b) id pairs
+// (pairs[2p], pairs[2p+1]); needs X, Xb, norms, sqn, D, Dp and the bound
+// constants of the args.
+hipError_t nnd_launch_screen_pairs(int metric, NndArgs const &a, uint32_t const *pairs, unsigned n, float *out_d,
+                                   float *out_e, hipStream_t s);
 // Start of an iteration's join: zero the NND_NSTATS counters and, with
 // advance, step the iteration counter (a kernel, so the whole iteration can
 // be captured in a graph without memset nodes).

@@ -225,6 +225,22 @@ __device__ __forceinline__ void mj_load(NndArgs const &a, uint32_t id, bool ok, 
     }
 }
 
+// BF16 tile distance from the dot G and the norms the metric keeps (an, on:
+// |x|^2 for SSD, else the stored norm), and the bound E on its error against
+// the exact distance (arn, orn = |x|): the join screens with d - E.
+template <int METRIC>
+__device__ __forceinline__ float mj_dist(float G, float an, float on) {
+    if (METRIC == METRIC_L2SQR) return (an + on) - 2.f * G;
+    if (METRIC == METRIC_EUCLIDEAN) return (an + on) - G;
+    return -G / __builtin_fmaf(an, on, 1.1920928955078125e-07f);
+}
+template <int METRIC>
+__device__ __forceinline__ float mj_bound(NndArgs const &a, float an, float on, float arn, float orn) {
+    if (METRIC == METRIC_L2SQR) return 2.f * a.scr_c * arn * orn + a.scr_g * (an + on);
+    if (METRIC == METRIC_EUCLIDEAN) return a.scr_c * arn * orn + a.scr_g * (an + on);
+    return a.scr_c + 2.f * a.scr_g;
+}
+
 template <class LDS>
 __device__ void mj_flush(NndArgs const &a, LDS &L, unsigned fill, unsigned nT) {
     unsigned const lane = lane_id();
@@ -318,8 +334,8 @@ template <int METRIC, int KS, bool EXACT>
 __global__ void __launch_bounds__(256) join_mfma_kernel(NndArgs a) {
     __shared__ MJoinLDS<EXACT ? 1u : MQ> lds[4];
     unsigned const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    unsigned const n = blockIdx.x * 4 + w;
-    if (n >= a.N) return;
+    unsigned const n = a.n0 + blockIdx.x * 4 + w;
+    if (n >= a.n1) return;
     auto &L = lds[w];
     unsigned const nn = a.nnew[n], no = a.nold[n];
     unsigned const rn = min(a.crnew[n], a.R), ro = min(a.crold[n], a.R);
@@ -381,14 +397,7 @@ __global__ void __launch_bounds__(256) join_mfma_kernel(NndArgs a) {
                 unsigned const ia = ag + mrow + j;
                 // new x new once per id pair (i < j by id), new x old always (kgraph.cpp:741-750)
                 bool const valid = ia < nA && ook && (io < nA ? aid[j] < oid : aid[j] != oid);
-                float const G = acc[j];
-                float d;
-                if (METRIC == METRIC_L2SQR)
-                    d = (an[j] + on) - 2.f * G;
-                else if (METRIC == METRIC_EUCLIDEAN)
-                    d = (an[j] + on) - G;
-                else
-                    d = -G / __builtin_fmaf(an[j], on, 1.1920928955078125e-07f);
+                float const d = mj_dist<METRIC>(acc[j], an[j], on);
                 pairs += valid;
                 if (EXACT) {
                     bool const pa = valid && d < arad[j];
@@ -401,14 +410,7 @@ __global__ void __launch_bounds__(256) join_mfma_kernel(NndArgs a) {
                     tgt[2 * j] = ia;
                     tgt[2 * j + 1] = io;
                 } else {
-                    float E;
-                    if (METRIC == METRIC_L2SQR)
-                        E = 2.f * a.scr_c * arn[j] * osq.y + a.scr_g * (an[j] + on);
-                    else if (METRIC == METRIC_EUCLIDEAN)
-                        E = a.scr_c * arn[j] * osq.y + a.scr_g * (an[j] + on);
-                    else
-                        E = a.scr_c + 2.f * a.scr_g;
-                    float const lo = d - E;
+                    float const lo = d - mj_bound<METRIC>(a, an[j], on, arn[j], osq.y);
                     bool const pass = valid && (lo < arad[j] || lo < orad);
                     uint64_t const b = ballot(pass);
                     if (pass) L.rq[rqn + popc64(b & ((1ull << lane) - 1))] = ia | (io << 16);
@@ -447,8 +449,8 @@ template <int METRIC, int CH>
 __global__ void __launch_bounds__(256) join_tile_kernel(NndArgs a) {
     __shared__ MJoinLDS<1> lds[4];
     unsigned const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    unsigned const n = blockIdx.x * 4 + w;
-    if (n >= a.N) return;
+    unsigned const n = a.n0 + blockIdx.x * 4 + w;
+    if (n >= a.n1) return;
     auto &L = lds[w];
     unsigned const nn = a.nnew[n], no = a.nold[n];
     unsigned const rn = min(a.crnew[n], a.R), ro = min(a.crold[n], a.R);
@@ -899,6 +901,38 @@ __global__ void gather_controls_kernel(NndArgs a, uint32_t const *ctl, unsigned 
     if (threadIdx.x == 0) cout[c] = cnt;
 }
 
+
+// The join's BF16 MFMA distance for arbitrary pairs (diagnostics of the
+// screen): one wave per 16 pairs, pair p's rows as row p of A and column p of
+// B, so C[p][p] is its dot; lane l holds C[4(l/16) + j][l % 16], the diagonal
+// sits in the lanes with l / 16 == (l % 16) / 4 at j = l % 4.
+template <int METRIC, int KS>
+__global__ void __launch_bounds__(256) screen_pairs_kernel(NndArgs a, uint32_t const *pairs, unsigned n, float *out_d,
+                                                           float *out_e) {
+    unsigned const lane = threadIdx.x & 63;
+    unsigned const base = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 16;
+    if (base >= n) return;
+    unsigned const r16 = lane & 15, kq = (lane >> 4) * 4;
+    unsigned const p = base + r16;
+    bool const ok = p < n;
+    uint32_t const ia = ok ? pairs[2 * p] : 0u, ib = ok ? pairs[2 * p + 1] : 0u;
+    mj_bf16x4 aop[KS], bop[KS];
+    mj_load<KS>(a, ia, ok, kq, aop);
+    mj_load<KS>(a, ib, ok, kq, bop);
+    mj_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < KS; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aop[s], bop[s], acc, 0, 0, 0);
+    unsigned const j = r16 & 3;
+    float const G = j == 0 ? acc[0] : j == 1 ? acc[1] : j == 2 ? acc[2] : acc[3];
+    if (ok && (lane >> 4) == (r16 >> 2)) {
+        float2 const sa = a.sqn[ia], sb = a.sqn[ib];
+        float const an = (METRIC == METRIC_L2SQR) ? sa.x : a.norms[ia];
+        float const on = (METRIC == METRIC_L2SQR) ? sb.x : a.norms[ib];
+        out_d[p] = mj_dist<METRIC>(G, an, on);
+        out_e[p] = mj_bound<METRIC>(a, an, on, sa.y, sb.y);
+    }
+}
+
 }  // namespace
 
 static int ch_for(unsigned D) {
@@ -927,7 +961,7 @@ hipError_t nnd_launch_init(int metric, NndArgs const &a, hipStream_t s) {
 
 template <int METRIC>
 static void launch_join_m(NndArgs const &a, hipStream_t s) {
-    dim3 g((a.N + 3) / 4), b(256);
+    dim3 g((a.n1 - a.n0 + 3) / 4), b(256);
     size_t lds = 0;
     switch (ch_for(a.D)) {
         case 1: hipLaunchKernelGGL((join_tile_kernel<METRIC, 1>), g, b, lds, s, a); break;
@@ -941,7 +975,7 @@ static void launch_join_m(NndArgs const &a, hipStream_t s) {
 
 template <int METRIC, bool EXACT>
 static void launch_join_mfma_e(NndArgs const &a, hipStream_t s) {
-    dim3 g((a.N + 3) / 4), b(256);
+    dim3 g((a.n1 - a.n0 + 3) / 4), b(256);
     unsigned const ks = (a.D + 15) / 16;
     if (ks <= 2)
         hipLaunchKernelGGL((join_mfma_kernel<METRIC, 2, EXACT>), g, b, 0, s, a);
@@ -967,6 +1001,7 @@ hipError_t nnd_launch_sqn(NndArgs const &a, float2 *out, uint32_t *flags, hipStr
 }
 
 hipError_t nnd_launch_join(int metric, NndArgs const &a, hipStream_t s) {
+    if (a.n1 > a.N || a.n0 >= a.n1) return a.n0 == a.n1 ? hipSuccess : hipErrorInvalidValue;
     if (a.mfma) {
         if (a.D > 256) return hipErrorInvalidValue;
         if (metric == METRIC_L2SQR)
@@ -1014,5 +1049,32 @@ hipError_t nnd_launch_gather_controls(NndArgs const &a, uint32_t const *ctl, uns
                                       hipStream_t s) {
     if (nc == 0) return hipSuccess;
     hipLaunchKernelGGL(gather_controls_kernel, dim3(nc), dim3(256), 0, s, a, ctl, nc, dout, cout);
+    return hipGetLastError();
+}
+
+hipError_t nnd_launch_screen_pairs(int metric, NndArgs const &a, uint32_t const *pairs, unsigned n, float *out_d,
+                                   float *out_e, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (a.D > 256) return hipErrorInvalidValue;
+    unsigned const waves = (n + 15) / 16, ks = (a.D + 15) / 16;
+    dim3 g((waves + 3) / 4), b(256);
+#define KG_SCREEN(M)                                                                                       \
+    do {                                                                                                   \
+        if (ks <= 2)                                                                                       \
+            hipLaunchKernelGGL((screen_pairs_kernel<M, 2>), g, b, 0, s, a, pairs, n, out_d, out_e);       \
+        else if (ks <= 4)                                                                                  \
+            hipLaunchKernelGGL((screen_pairs_kernel<M, 4>), g, b, 0, s, a, pairs, n, out_d, out_e);       \
+        else if (ks <= 8)                                                                                  \
+            hipLaunchKernelGGL((screen_pairs_kernel<M, 8>), g, b, 0, s, a, pairs, n, out_d, out_e);       \
+        else                                                                                               \
+            hipLaunchKernelGGL((screen_pairs_kernel<M, 16>), g, b, 0, s, a, pairs, n, out_d, out_e);      \
+    } while (0)
+    if (metric == METRIC_L2SQR)
+        KG_SCREEN(METRIC_L2SQR);
+    else if (metric == METRIC_EUCLIDEAN)
+        KG_SCREEN(METRIC_EUCLIDEAN);
+    else
+        KG_SCREEN(METRIC_ANGULAR);
+#undef KG_SCREEN
     return hipGetLastError();
 }

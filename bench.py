@@ -564,12 +564,6 @@ def learned_workload(args):
     build_s = time.perf_counter() - tb
     g.set_entry_sample(args.entry_sample, seed=1998)
     log("build %.2fs (%d it, control recall %.4f)" % (build_s, info.iterations, info.recall))
-    tt = time.perf_counter()
-    steps = 2_000_000 // 4096
-    trace = g.train_projector(steps=steps, batch=4096, hidden=256, out=32, lr=2e-3, margin=1.0, seed=1)
-    train_s = time.perf_counter() - tt
-    log("projector trained on %d triplets in %.2fs, loss %.4f -> %.4f" % (steps * 4096, train_s, trace[:10].mean(),
-                                                                         trace[-10:].mean()))
     gt_i = np.zeros((nq, K), np.uint32)
     gt_d = np.zeros((nq, K), np.float32)
     gt_c = np.zeros(nq, np.uint32)
@@ -580,16 +574,12 @@ def learned_workload(args):
     dC = devmem.DeviceArray((nq,), np.uint32)
     m = K - 1
     rows = []
-    for form, shortlist in (("plain", 0), ("bf16-screen", 0), ("learned", 16)):
+
+    def sweep_form(form, shortlist):
         # plain: exact FP32 scores of the whole sample; bf16-screen: fused BF16
-        # MFMA top-16 then exact re-rank; learned: the same in the projected space
+        # MFMA top-16 then exact re-rank; untrained / learned: the same in the
+        # projected space of the initial / trained projector
         g._h.call("kg_set_precision", _lib.PRECISION_BF16 if form == "bf16-screen" else _lib.PRECISION_FP32)
-        if shortlist:
-            rep = g.validate_entry_projection(probe, K=K, P=20, entries=m, shortlist=shortlist)
-            log("probe guard: %s" % rep)
-            if not rep["enabled"]:
-                rows.append({"form": form, "fallback": True, "probe": rep})
-                continue
         g.set_entry_projection(shortlist)
         for P in (10, 20, 50):
             sp = SearchParams(K, 0, P, 32, 1, 1e30, 1998, m)
@@ -606,6 +596,57 @@ def learned_workload(args):
             rows.append({"form": form, "P": P, "S": 32, "qps": round(nq / wall, 1), "recall": round(rec, 4),
                          "entry_ms": round(ems, 3), "walk_ms": round(ctypes_float(g._h, 0), 3)})
             log(rows[-1])
+
+    # held-out triplets (anchor, graph neighbour, non-neighbour: uniform for
+    # half, two hops away for the other half) for the loss before / after
+    hr = np.random.RandomState(77)
+    trip = []
+    for t in range(2048):
+        a = int(hr.randint(n))
+        ids_a, _, Ma = g.get_nn(a)
+        p = int(ids_a[hr.randint(max(1, min(Ma, len(ids_a))))])
+        q = int(hr.randint(n))
+        if t & 1:
+            ids_p, _, Mp = g.get_nn(p)
+            c = int(ids_p[hr.randint(max(1, min(Mp, len(ids_p))))])
+            if c != a and c not in set(ids_a[:Ma].tolist()):
+                q = c
+        trip += [a, p, q]
+    trip = np.array(trip, np.uint32)
+
+    def eval_loss():
+        lv = np.zeros(1, np.float32)
+        g._h.call("kg_mlp_loss_grad", ptr(trip), len(trip) // 3, 1.0, ptr(lv), None, None, None, None)
+        return float(lv[0])
+
+    def entry_agreement(ref_ids):
+        ids = g.select_entries(Q[:2000], m)
+        return float(np.mean([len(set(a) & set(b)) / m for a, b in zip(ids, ref_ids)]))
+
+    sweep_form("plain", 0)
+    exact_entries = g.select_entries(Q[:2000], m)
+    sweep_form("bf16-screen", 0)
+    g._h.call("kg_mlp_init", 256, 32, 1)
+    loss0 = eval_loss()
+    g.set_entry_projection(16)
+    agree0 = entry_agreement(exact_entries)
+    sweep_form("untrained", 16)
+    tt = time.perf_counter()
+    steps = 2_000_000 // 4096
+    trace = np.zeros(steps, np.float32)
+    g._h.call("kg_mlp_train", steps, 4096, 2e-3, 1.0, 1, ptr(trace))
+    train_s = time.perf_counter() - tt
+    loss1 = eval_loss()
+    g.set_entry_projection(16)
+    agree1 = entry_agreement(exact_entries)
+    log("projector trained on %d triplets in %.2fs; held-out loss %.4f -> %.4f; entry agreement with exact "
+        "selection %.4f -> %.4f" % (steps * 4096, train_s, loss0, loss1, agree0, agree1))
+    rep = g.validate_entry_projection(probe, K=K, P=20, entries=m, shortlist=16)
+    log("probe guard: %s" % rep)
+    if rep["enabled"]:
+        sweep_form("learned", 16)
+    else:
+        rows.append({"form": "learned", "fallback": True, "probe": rep})
     best = {}
     for r in rows:
         if "qps" in r and r["recall"] >= 0.95:
@@ -622,8 +663,13 @@ def learned_workload(args):
                    "build": {"K": 25, "L": 100, "reverse": -1, "iterations": args.iterations},
                    "projector": "128-256-32", "triplets": steps * 4096, "entry_sample": args.entry_sample},
         "build_s": round(build_s, 3), "train_s": round(train_s, 3),
-        "loss_first_last": [round(float(trace[:10].mean()), 4), round(float(trace[-10:].mean()), 4)],
+        "heldout_loss_before_after": [round(loss0, 5), round(loss1, 5)],
+        "train_loss_first_last": [round(float(trace[:10].mean()), 5), round(float(trace[-10:].mean()), 5)],
+        "entry_agreement_with_exact_before_after": [round(agree0, 4), round(agree1, 4)],
+        "probe_guard": rep,
         "sweep": rows, "best": best,
+        "speedup_learned_vs_untrained": (round(best["learned"]["qps"] / best["untrained"]["qps"], 3)
+                                         if "learned" in best and "untrained" in best else None),
         "speedup_learned_vs_plain": (round(best["learned"]["qps"] / best["plain"]["qps"], 3)
                                      if "learned" in best and "plain" in best else None),
         "speedup_learned_vs_bf16_screen": (round(best["learned"]["qps"] / best["bf16-screen"]["qps"], 3)

@@ -195,7 +195,9 @@ int kg_select_entries_device(kg_handle *h, const float *dQ, unsigned nq, size_t 
  * kg_mlp_get_normalization: mu and inv_sigma (D floats each; either may be NULL).
  * kg_mlp_train: Adam on the triplet loss max(0, |f(a)-f(p)|^2 - |f(a)-f(n)|^2
  *   + margin) over `steps` batches of `batch` triplets (a random node, one of
- *   its first M graph neighbours, a random node) drawn from the current index;
+ *   its first M graph neighbours, a node outside its list: uniform for half of
+ *   the triplets, two hops away for the other half) drawn from the current
+ *   index, learning rate decayed from lr to 0 by a cosine over the call;
  *   loss_trace (may be NULL) receives the mean loss of every step.
  * kg_mlp_loss_grad: loss and gradients for caller-given triplets
  *   (3*nt node ids), for checking the backward kernels.

@@ -128,10 +128,13 @@ void mlp_train(Handle &h, unsigned steps, unsigned batch, float lr, float margin
         KG_HIP(kg_triplet_loss(h.mlp_z.p, P, batch, margin, h.mlp_dz.p, h.mlp_loss.p + st, s));
         backward(h, h.mlp_x.p, n, h.mlp_h.p, h.mlp_dz.p);
         ++h.adam_step;
-        KG_HIP(kg_adam(h.W1.p, h.gW1.p, h.mW1.p, h.vW1.p, H * D, lr, h.adam_step, s));
-        KG_HIP(kg_adam(h.b1.p, h.gb1.p, h.mb1.p, h.vb1.p, H, lr, h.adam_step, s));
-        KG_HIP(kg_adam(h.W2.p, h.gW2.p, h.mW2.p, h.vW2.p, P * H, lr, h.adam_step, s));
-        KG_HIP(kg_adam(h.b2.p, h.gb2.p, h.mb2.p, h.vb2.p, P, lr, h.adam_step, s));
+        // cosine decay to zero over the call: Adam's normalised steps turn into
+        // a random walk once few triplets are active, and the decay ends it
+        float const lt = lr * 0.5f * (1.f + std::cos(3.14159265f * float(st) / float(steps)));
+        KG_HIP(kg_adam(h.W1.p, h.gW1.p, h.mW1.p, h.vW1.p, H * D, lt, h.adam_step, s));
+        KG_HIP(kg_adam(h.b1.p, h.gb1.p, h.mb1.p, h.vb1.p, H, lt, h.adam_step, s));
+        KG_HIP(kg_adam(h.W2.p, h.gW2.p, h.mW2.p, h.vW2.p, P * H, lt, h.adam_step, s));
+        KG_HIP(kg_adam(h.b2.p, h.gb2.p, h.mb2.p, h.vb2.p, P, lt, h.adam_step, s));
     }
     h.sync();
     if (trace && steps) KG_HIP(hipMemcpy(trace, h.mlp_loss.p, 4ull * steps, hipMemcpyDeviceToHost));

@@ -10,7 +10,8 @@
 //                    16x16 output tile; A and B are read through arbitrary
 //                    strides, so the forward, the input-gradient and the
 //                    weight-gradient products share the kernel.
-//   triplets_kernel  counter-RNG triplet sampling from the CSR graph + gather
+//   triplets_kernel  counter-RNG triplet sampling from the CSR graph (negatives
+//                    uniform or two hops away, never in the anchor's list)
 //   triplet_loss     L = mean(max(0, |za-zp|^2 - |za-zn|^2 + margin)) and dL/dZ
 //   colsum_kernel    bias gradients
 //   adam_kernel      Adam update of the FP32 master weights
@@ -86,6 +87,19 @@ __global__ void triplets_kernel(TripletArgs t) {
     uint32_t lim = min(t.gM[a], deg);
     uint32_t p = lim ? t.gids[t.goff[a] + uint32_t(mix64(s + 1) % lim)] : a;
     uint32_t n = uint32_t(mix64(s + 2) % t.N);
+    // every other triplet draws its non-neighbour from two hops away (a random
+    // neighbour of the positive that is neither the anchor nor in the anchor's
+    // list), the rest uniformly: on clustered data a uniform node is almost
+    // never close enough to leave a gradient
+    if (i & 1u) {
+        uint32_t const pdeg = t.goff[p + 1] - t.goff[p], plim = min(t.gM[p], pdeg);
+        if (plim) {
+            uint32_t const c = t.gids[t.goff[p] + uint32_t(mix64(s + 3) % plim)];
+            bool ok = c != a;
+            for (uint32_t j = 0; ok && j < lim; ++j) ok = t.gids[t.goff[a] + j] != c;
+            if (ok) n = c;
+        }
+    }
     t.idx[3 * i] = a;
     t.idx[3 * i + 1] = p;
     t.idx[3 * i + 2] = n;

@@ -295,8 +295,10 @@ def main():
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": measured_traffic("search_kernel", "intmix-1m-d128", pick, rows)},
         "walk_row_storage": row_cmp,
-        "roofline_join": {"kernel": "join_kernel", "bound": "hbm", "achieved": round(join_gbs, 1),
+        "roofline_join": {"kernel": "join_mfma_kernel", "bound": "hbm", "achieved": round(join_gbs, 1),
                           "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(join_gbs / PEAK_HBM_GBS, 4),
+                          "traffic": measured_join_traffic("intmix-1m-d128", "bf16-exact"),
+                          "launches": int(einfo.iterations),
                           "pairs": int(counters[0]), "list_entries": int(counters[1]),
                           "pushes": int(counters[2])},
     }
@@ -697,6 +699,21 @@ def measured_traffic(kernel, workload, pick, rows):
     if not os.path.exists(path):
         return None
     want = "traffic P=%d S=%d mode=%s rows=%s bytes=" % (pick["P"], pick["S"], pick["mode"], rows)
+    for line in open(path):
+        line = line.strip().strip("`")
+        if line.startswith(want):
+            return int(line[len(want):])
+    return None
+
+
+def measured_join_traffic(workload, form):
+    """Average HBM bytes per local-join launch for this workload and join form
+    from the committed FETCH_SIZE / WRITE_SIZE summary (a line
+    `traffic join form=<form> bytes=<B>`), or None."""
+    path = os.path.join(ROOT, "profiles", "search_kernel_%s_traffic.md" % workload)
+    if not os.path.exists(path):
+        return None
+    want = "traffic join form=%s bytes=" % form
     for line in open(path):
         line = line.strip().strip("`")
         if line.startswith(want):

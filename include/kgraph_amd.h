@@ -103,6 +103,18 @@ int kg_build(kg_handle *h, const kg_index_params *params, kg_index_info *info);
 /* Screening precision used by kg_build and kg_search (KG_PRECISION_*). */
 int kg_set_precision(kg_handle *h, int precision);
 
+/* Visited set of the graph walk (no counterpart in the reference, which
+ * clears an N-bit bitset per query, kgraph.cpp:436-437).  Mode 0 (default):
+ * each resident wave keeps an exact hash set of visited ids in LDS (the
+ * largest power of two up to 4096 entries that keeps 16 waves per CU) in
+ * front of its N-bit HBM bitset; a query that fills it past half moves it
+ * into the bitset and continues there.  Mode 1: the bitset only.  Results are
+ * identical either way. */
+int kg_set_search_visited(kg_handle *h, int mode);
+/* Last walk launch: LDS set entries per wave (0: bitset only) and how many
+ * queries moved their set into the bitset. */
+int kg_search_visited_stats(kg_handle *h, unsigned *vcap, unsigned *spilled);
+
 /* Row storage for graph walks (no counterpart in the reference, whose rows are
  * the caller's Matrix, kgraph-data.h:115-175).  kg_set_data keeps a BF16 copy
  * of the rows (the MFMA kernels' operands); when every value is representable

@@ -209,7 +209,14 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
     if (ngraph) {
         // seeds sorted per trial: init ids, or P random ones (largest P of the batch)
         unsigned const seed_cap = sp.init ? sp.init : std::max(poolmax, 1u);
-        size_t lpw = kg_search_lds_per_wave(poolmax, kmax + 1, h.Dp, seed_cap);
+        size_t lpw = kg_search_lds_per_wave(poolmax, kmax + 1, h.Dp, seed_cap, 0);
+        // visited hash set in LDS: the largest power of two (256..4096 entries)
+        // that keeps 16 waves per CU resident (the kernel's register occupancy)
+        unsigned vcap = 0;
+        if (h.search_visited == 0)
+            for (unsigned c = 4096; c >= 256 && !vcap; c /= 2)
+                if (kg_search_lds_per_wave(poolmax, kmax + 1, h.Dp, seed_cap, c) * 16 <= h.lds_per_cu) vcap = c;
+        lpw = kg_search_lds_per_wave(poolmax, kmax + 1, h.Dp, seed_cap, vcap);
         unsigned wpb = std::min(h.search_wpb, kg_search_max_wpb());  // the kernel's launch bound
         while (wpb > 1 && lpw * wpb > 65536) wpb /= 2;
         size_t lpb = lpw * wpb;
@@ -241,8 +248,8 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
             h.lists_unique = dup == 0;
             h.gnode_valid = true;
         }
-        h.qcounter.reserve(1);
-        KG_HIP(hipMemsetAsync(h.qcounter.p, 0, 4, h.stream));
+        h.qcounter.reserve(2);  // [0] next query, [1] queries whose visited set spilled
+        KG_HIP(hipMemsetAsync(h.qcounter.p, 0, 8, h.stream));
         kg_search_args a{};
         a.X = h.X.p;
         a.Xb = (h.rows_bf16 && h.row_storage == 0) ? h.Xb.p : nullptr;
@@ -290,6 +297,9 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         a.pool_cap = poolmax;
         a.res_cap = kmax + 1;
         a.lds_per_wave = unsigned(lpw);
+        a.vcap = vcap;
+        a.spills = h.qcounter.p + 1;
+        h.last_vcap = vcap;
         a.seed_cap = seed_cap;
         KG_HIP(hipEventRecord(h.ev[0], h.stream));
         KG_HIP(kg_launch_search(h.metric, a, blocks, wpb, h.stream));
@@ -642,6 +652,25 @@ int kg_set_precision(kg_handle *k, int precision) {
         if (precision != KG_PRECISION_FP32 && precision != KG_PRECISION_BF16)
             throw kga::invalid_argument("unknown precision");
         h.precision = precision;
+    });
+}
+
+int kg_set_search_visited(kg_handle *k, int mode) {
+    return guard(k, [&](Handle &h) {
+        if (mode != 0 && mode != 1) throw kga::invalid_argument("visited-set mode must be 0 or 1");
+        h.search_visited = mode;
+    });
+}
+
+int kg_search_visited_stats(kg_handle *k, unsigned *vcap, unsigned *spilled) {
+    return guard(k, [&](Handle &h) {
+        uint32_t sp = 0;
+        if (h.qcounter.p) {
+            h.sync();
+            KG_HIP(hipMemcpy(&sp, h.qcounter.p + 1, 4, hipMemcpyDeviceToHost));
+        }
+        if (vcap) *vcap = h.last_vcap;
+        if (spilled) *spilled = sp;
     });
 }
 

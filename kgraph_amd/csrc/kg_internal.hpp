@@ -40,6 +40,8 @@ struct kg_search_args {
     // per-wave scratch
     uint32_t *visited;
     unsigned vis_words;
+    unsigned vcap;  // entries of the per-wave LDS visited set (power of two >= 256, or 0: bitset only)
+    uint32_t *spills;  // counts queries whose LDS set moved into the bitset (may be NULL)
     uint32_t *touched;
     unsigned touch_cap;
     uint32_t *qcounter;
@@ -95,7 +97,7 @@ struct kg_rescore_args {
 };
 
 // Launchers (defined in the .hip files).
-size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp, unsigned seed_cap);
+size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp, unsigned seed_cap, unsigned vcap);
 // Largest waves per block search_kernel is compiled for (its launch bound);
 // kg_launch_search rejects larger shapes.
 unsigned kg_search_max_wpb();

@@ -14,7 +14,11 @@
 // the seed sort, the T>1 result list, the query vector and a 64-entry batch
 // buffer.  The visited set is an exact per-wave bitset in HBM, cleared on exit
 // through the list of ids the query touched (wholesale if that list
-// overflows).
+// overflows).  In front of it sits an exact hash set of visited ids in LDS
+// (open addressing, linear probing, compare-and-swap inserts): a query visits
+// ~1,000 ids, and while the set holds them (at most half full) no visited
+// test leaves the CU.  Past that the set spills into the HBM bitset and the
+// query continues there.
 //
 // Inside a batch the wave reads the S neighbour ids (one per lane), drops
 // in-batch duplicates, test-and-sets the visited bits with atomicOr, gathers
@@ -44,14 +48,16 @@ struct WaveLDS {
     float *q;
     uint32_t *fid;
     float *fdist;
+    uint32_t *vt;  // visited hash set (vcap entries, VT_EMPTY = free)
 };
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
 struct LdsLayout {
-    size_t pid, pdist, pmeta, pbase, tkey, rid, rdist, q, fid, fdist, total;
-    // tcap: starting points sorted at once (P random seeds, or init ids)
-    __host__ __device__ LdsLayout(unsigned cap, unsigned rcap, unsigned Dp, unsigned tcap) {
+    size_t pid, pdist, pmeta, pbase, tkey, rid, rdist, q, fid, fdist, vt, total;
+    // tcap: starting points sorted at once (P random seeds, or init ids);
+    // vcap: entries of the visited hash set (0: none)
+    __host__ __device__ LdsLayout(unsigned cap, unsigned rcap, unsigned Dp, unsigned tcap, unsigned vcap) {
         size_t o = 0;
         pid = o; o = align16(o + 4ull * cap);
         pdist = o; o = align16(o + 4ull * cap);
@@ -63,6 +69,7 @@ struct LdsLayout {
         q = o; o = align16(o + 4ull * Dp);
         fid = o; o = align16(o + 4ull * KG_WAVE);
         fdist = o; o = align16(o + 4ull * KG_WAVE);
+        vt = o; o = align16(o + 4ull * vcap);
         total = o;
     }
 };
@@ -242,6 +249,30 @@ __device__ __forceinline__ uint32_t load_u32_l2(uint32_t const *p) {
     return __hip_atomic_load(const_cast<uint32_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Visited hash set in LDS.  Ids are < 2^31, so VT_EMPTY never collides; the
+// set is kept at most half full plus one batch, so a probe always ends.
+constexpr uint32_t VT_EMPTY = 0xffffffffu;
+__device__ __forceinline__ unsigned vt_hash(uint32_t id, unsigned shift) { return (id * 2654435761u) >> shift; }
+// true when id was not in the set (it is now)
+__device__ __forceinline__ bool vt_insert(uint32_t *vt, unsigned mask, unsigned shift, uint32_t id) {
+    unsigned h = vt_hash(id, shift);
+    for (;;) {
+        uint32_t const old = atomicCAS(&vt[h], VT_EMPTY, id);
+        if (old == VT_EMPTY) return true;
+        if (old == id) return false;
+        h = (h + 1) & mask;
+    }
+}
+__device__ __forceinline__ bool vt_contains(uint32_t const *vt, unsigned mask, unsigned shift, uint32_t id) {
+    unsigned h = vt_hash(id, shift);
+    for (;;) {
+        uint32_t const v = vt[h];
+        if (v == id) return true;
+        if (v == VT_EMPTY) return false;
+        h = (h + 1) & mask;
+    }
+}
+
 #ifndef SEARCH_WAVES_PER_EU
 #define SEARCH_WAVES_PER_EU 4
 #endif
@@ -254,7 +285,7 @@ search_kernel(kg_search_args a) {
     unsigned const slot = blockIdx.x * (blockDim.x / KG_WAVE) + wib;
     unsigned const cap = a.pool_cap;
     unsigned const rcap = a.res_cap;
-    LdsLayout lay(cap, rcap, a.Dp, a.seed_cap);
+    LdsLayout lay(cap, rcap, a.Dp, a.seed_cap, a.vcap);
     unsigned char *base = smem + size_t(wib) * a.lds_per_wave;
     WaveLDS w;
     w.pid = reinterpret_cast<uint32_t *>(base + lay.pid);
@@ -267,6 +298,8 @@ search_kernel(kg_search_args a) {
     w.q = reinterpret_cast<float *>(base + lay.q);
     w.fid = reinterpret_cast<uint32_t *>(base + lay.fid);
     w.fdist = reinterpret_cast<float *>(base + lay.fdist);
+    w.vt = reinterpret_cast<uint32_t *>(base + lay.vt);
+    unsigned const vmask = a.vcap - 1u, vshift = a.vcap ? 32u - unsigned(__builtin_ctz(a.vcap)) : 0u;
 
     uint32_t *vis = a.visited + size_t(slot) * a.vis_words;
     uint32_t *touched = a.touched + size_t(slot) * a.touch_cap;
@@ -289,6 +322,29 @@ search_kernel(kg_search_args a) {
         wave_sync();
 
         unsigned ntouch = 0, n_comps = 0, n_scan = 0, RL = 0;
+        // visited ids: in the LDS set while it has room (vcount entries), else
+        // in the HBM bitset (ntouch ids recorded for clearing)
+        bool glob = a.vcap == 0;
+        unsigned vcount = 0;
+        for (unsigned i = lane; i < a.vcap; i += KG_WAVE) w.vt[i] = VT_EMPTY;
+        wave_sync();
+        // moves the LDS set into the bitset; later visits go there
+        auto spill = [&]() {
+            for (unsigned c = 0; c < a.vcap; c += KG_WAVE) {
+                uint32_t const e = w.vt[c + lane];
+                bool const has = e != VT_EMPTY;
+                uint64_t const b = ballot(has);
+                if (has) {
+                    atomicOr(&vis[e >> 5], 1u << (e & 31));
+                    unsigned const pos = ntouch + popc64(b & lt_mask);
+                    if (pos < a.touch_cap) touched[pos] = e;
+                }
+                ntouch += popc64(b);
+            }
+            glob = true;
+            if (lane == 0 && a.spills) atomicAdd(a.spills, 1u);
+            wave_sync();
+        };
         for (unsigned trial = 0; trial < a.T; ++trial) {
             // ---- starting points (kgraph.cpp:450-467)
             unsigned L = 0;
@@ -301,7 +357,9 @@ search_kernel(kg_search_args a) {
                     if (i < P) {
                         s = seeds[i];
                         keep = true;
-                        if (trial > 0) keep = !(load_u32_l2(&vis[s >> 5]) & (1u << (s & 31)));
+                        if (trial > 0)
+                            keep = glob ? !(load_u32_l2(&vis[s >> 5]) & (1u << (s & 31)))
+                                        : !vt_contains(w.vt, vmask, vshift, s);
                     }
                     uint64_t b = ballot(keep);
                     if (keep) w.pid[L + popc64(b & lt_mask)] = s;
@@ -314,12 +372,20 @@ search_kernel(kg_search_args a) {
             }
             wave_sync();
             // ---- mark, score and sort the starting points (kgraph.cpp:468-476)
-            for (unsigned i = lane; i < L; i += KG_WAVE) {
-                uint32_t s = w.pid[i];
-                atomicOr(&vis[s >> 5], 1u << (s & 31));
-                if (ntouch + i < a.touch_cap) touched[ntouch + i] = s;
+            if (!glob && 2 * (vcount + L + KG_WAVE) > a.vcap) spill();
+            if (glob) {
+                for (unsigned i = lane; i < L; i += KG_WAVE) {
+                    uint32_t s = w.pid[i];
+                    atomicOr(&vis[s >> 5], 1u << (s & 31));
+                    if (ntouch + i < a.touch_cap) touched[ntouch + i] = s;
+                }
+                ntouch += L;
+            } else {
+                for (unsigned c = 0; c < L; c += KG_WAVE) {
+                    bool const fresh = c + lane < L && vt_insert(w.vt, vmask, vshift, w.pid[c + lane]);
+                    vcount += popc64(ballot(fresh));
+                }
             }
-            ntouch += L;
             batch_dist<METRIC, RowT>(a, w.pid, w.pdist, L, w.q, qn);
             for (unsigned i = lane; i < L; i += KG_WAVE)
                 w.tkey[i] = (uint64_t(ord_f32(w.pdist[i])) << 32) | w.pid[i];
@@ -386,18 +452,26 @@ search_kernel(kg_search_args a) {
                     }
                     bool fresh = false;
                     if (lane < cnt && !dup) {
-                        uint32_t bit = 1u << (id & 31);
-                        fresh = !(atomicOr(&vis[id >> 5], bit) & bit);
+                        if (glob) {
+                            uint32_t bit = 1u << (id & 31);
+                            fresh = !(atomicOr(&vis[id >> 5], bit) & bit);
+                        } else {
+                            fresh = vt_insert(w.vt, vmask, vshift, id);
+                        }
                     }
                     uint64_t b = ballot(fresh);
                     unsigned const nf = popc64(b);
                     unsigned pos = popc64(b & lt_mask);
                     if (fresh) {
                         w.fid[pos] = id;
-                        if (ntouch + pos < a.touch_cap) touched[ntouch + pos] = id;
+                        if (glob && ntouch + pos < a.touch_cap) touched[ntouch + pos] = id;
                     }
-                    ntouch += nf;
+                    if (glob)
+                        ntouch += nf;
+                    else
+                        vcount += nf;
                     wave_sync();
+                    if (!glob && 2 * (vcount + KG_WAVE) > a.vcap) spill();
                     if (nf == 0) continue;
                     // lane j fetches candidate j's M and CSR base while the rows load
                     uint32_t cm = 0, cb = 0;
@@ -472,8 +546,9 @@ search_kernel(kg_search_args a) {
             if (a.scans) a.scans[qi] = n_scan;
             if (a.lat_us) a.lat_us[qi] = float(__builtin_amdgcn_s_memrealtime() - t_start) * 0.01f;
         }
-        // ---- clear the visited bits this query set
-        if (ntouch <= a.touch_cap) {
+        // ---- clear the visited bits this query set (none while the LDS set held them)
+        if (!glob) {
+        } else if (ntouch <= a.touch_cap) {
             for (unsigned i = lane; i < ntouch; i += KG_WAVE) vis[load_u32_l2(&touched[i]) >> 5] = 0u;
         } else {
             for (unsigned i = lane; i < a.vis_words; i += KG_WAVE) vis[i] = 0u;
@@ -522,8 +597,8 @@ hipError_t kg_launch_gnode(unsigned N, uint32_t const *goff, uint32_t const *gid
     return hipGetLastError();
 }
 
-size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp, unsigned seed_cap) {
-    return LdsLayout(pool_cap, res_cap, Dp, seed_cap).total;
+size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp, unsigned seed_cap, unsigned vcap) {
+    return LdsLayout(pool_cap, res_cap, Dp, seed_cap, vcap).total;
 }
 
 unsigned kg_search_max_wpb() { return SEARCH_MAX_WPB; }

@@ -117,6 +117,37 @@ def test_bf16_row_storage_gives_identical_walks(kind, metric, d, lossless):
     assert np.array_equal(res[0][3], comps) and np.array_equal(res[0][4], scans)
 
 
+@pytest.mark.parametrize("P,S,T", [(40, 10, 1), (300, 10, 1), (20, 64, 3), (500, 30, 1)])
+def test_lds_visited_set_gives_identical_walks(P, S, T):
+    # the LDS visited set (spilling into the HBM bitset once half full) and the
+    # bitset alone give the oracle's ids, distances and counters; large P
+    # makes queries spill mid-walk, T > 1 reads the set across trials
+    import ctypes
+    X, Q, idx = setup("intmix", L2, d=128)
+    g = gpu_index(X, idx, L2)
+    res = {}
+    for mode in (0, 1):
+        g._h.call("kg_set_search_visited", mode)
+        gi, gd = g.search(Q, K=10, P=P, S=S, T=T, withDistance=True, stats=True)
+        vc, sp = ctypes.c_uint(), ctypes.c_uint()
+        g._h.call("kg_search_visited_stats", ctypes.byref(vc), ctypes.byref(sp))
+        res[mode] = (gi, gd, g.last_counts.copy(), g.last_comps.copy(), g.last_scans.copy(), vc.value, sp.value)
+    print("P=%d: LDS set %d entries, %d of %d queries spilled" % (P, res[0][5], res[0][6], len(Q)))
+    assert res[1][5] == 0 and res[1][6] == 0
+    if P <= 300:
+        assert res[0][5] >= 256  # a P = 500 pool leaves no LDS room for a set: bitset only
+    if P == 300:
+        assert res[0][6] > 0
+    for a, b in zip(res[0][:5], res[1][:5]):
+        assert np.array_equal(a, b)
+    ids, dd, c, comps, scans = idx.search(X, Q, L2, K=10, P=P, S=S, T=T)
+    assert np.array_equal(res[0][2], c)
+    for q in range(len(Q)):
+        assert np.array_equal(res[0][0][q, :c[q]], ids[q, :c[q]]), q
+        assert np.array_equal(res[0][1][q, :c[q]], dd[q, :c[q]]), q
+    assert np.array_equal(res[0][3], comps) and np.array_equal(res[0][4], scans)
+
+
 def test_search_parity_mixed_batch():
     X, Q, idx = setup("intmix", L2, d=128)
     rs = np.random.RandomState(3)

@@ -237,19 +237,19 @@ def main():
     ok = [s for s in sweep if s["recall"] >= 0.95]
     pick = max(ok, key=lambda s: s["qps"]) if ok else max(sweep, key=lambda s: s["recall"])
     achieved = pick["bytes"] / (pick["kernel_ms"] * 1e-3) / 1e9
-    # the picked setting again with the walk gathering the lossless BF16 copy
-    # of the rows (half the bytes, the same results)
+    # the picked setting again with the walk gathering the FP32 rows instead of
+    # the lossless BF16 copy (same results)
     row_cmp = None
-    if lossless and rows == "fp32":
-        h.call("kg_set_row_storage", 0)
-        r16 = search_once(pick["mode"], pick["P"], pick["S"])
+    if rows == "bf16":
         h.call("kg_set_row_storage", 1)
-        same = bool(np.array_equal(r16["ids"], res[sweep.index(pick)]["ids"]) and
-                    np.array_equal(r16["dists"], res[sweep.index(pick)]["dists"]))
-        row_cmp = {"bf16_rows_qps": round(nq / r16["wall_s"], 1), "bf16_rows_walk_ms": round(r16["kernel_ms"], 4),
-                   "fp32_rows_walk_ms": round(pick["kernel_ms"], 4), "identical_results": same}
-        log("bf16 rows: QPS=%.0f walk %.2fms (fp32 rows %.2fms), identical results: %s" % (
-            nq / r16["wall_s"], r16["kernel_ms"], pick["kernel_ms"], same))
+        r32 = search_once(pick["mode"], pick["P"], pick["S"])
+        h.call("kg_set_row_storage", 0)
+        same = bool(np.array_equal(r32["ids"], res[sweep.index(pick)]["ids"]) and
+                    np.array_equal(r32["dists"], res[sweep.index(pick)]["dists"]))
+        row_cmp = {"fp32_rows_qps": round(nq / r32["wall_s"], 1), "fp32_rows_walk_ms": round(r32["kernel_ms"], 4),
+                   "bf16_rows_walk_ms": round(pick["kernel_ms"], 4), "identical_results": same}
+        log("fp32 rows: QPS=%.0f walk %.2fms (bf16 rows %.2fms), identical results: %s" % (
+            nq / r32["wall_s"], r32["kernel_ms"], pick["kernel_ms"], same))
     # join kernel, algorithmic bytes: every join-list row once (2 bytes per
     # value: the MFMA form's operands are the BF16 copy) + its id (4), and a
     # radius probe at both endpoints of every pair (8)

@@ -104,10 +104,10 @@ int kg_build(kg_handle *h, const kg_index_params *params, kg_index_info *info);
 int kg_set_precision(kg_handle *h, int precision);
 
 /* Visited set of the graph walk (no counterpart in the reference, which
- * clears an N-bit bitset per query, kgraph.cpp:436-437).  Mode 0 (default):
+ * allocates an N-bit bitset per query, kgraph.cpp:438).  Mode 0 (default):
  * each resident wave keeps an exact hash set of visited ids in LDS (the
- * largest power of two up to 4096 entries that keeps 16 waves per CU) in
- * front of its N-bit HBM bitset; a query that fills it past half moves it
+ * largest power of two up to 4096 entries that keeps 16 waves per CU, or up
+ * to 2048 entries with 12 waves) in front of its N-bit HBM bitset; a query that fills it past half moves it
  * into the bitset and continues there.  Mode 1: the bitset only.  Results are
  * identical either way. */
 int kg_set_search_visited(kg_handle *h, int mode);
@@ -118,11 +118,11 @@ int kg_search_visited_stats(kg_handle *h, unsigned *vcap, unsigned *spilled);
 /* Row storage for graph walks (no counterpart in the reference, whose rows are
  * the caller's Matrix, kgraph-data.h:115-175).  kg_set_data keeps a BF16 copy
  * of the rows (the MFMA kernels' operands); when every value is representable
- * in BF16 the copy is lossless, and mode 0 makes walks gather it: half the
- * bytes, the same distances and ids.  Mode 1 (default) gathers the FP32 rows:
- * the walk is latency-bound, and on a 1M x 128 index the BF16 gathers measured
- * slower.  kg_row_storage reports *active = 1 when walks gather the BF16 copy
- * and *lossless = 1 when the copy is lossless (either may be NULL). */
+ * in BF16 the copy is lossless, and with mode 0 (default) walks gather it: half
+ * the bytes, so a wave holds twice the rows in flight, with the same distances
+ * and ids.  Mode 1 gathers the FP32 rows.  kg_row_storage reports *active = 1
+ * when walks gather the BF16 copy and *lossless = 1 when the copy is lossless
+ * (either may be NULL). */
 int kg_set_row_storage(kg_handle *h, int mode);
 int kg_row_storage(kg_handle *h, int *active, int *lossless);
 

@@ -211,11 +211,16 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         unsigned const seed_cap = sp.init ? sp.init : std::max(poolmax, 1u);
         size_t lpw = kg_search_lds_per_wave(poolmax, kmax + 1, h.Dp, seed_cap, 0);
         // visited hash set in LDS: the largest power of two (256..4096 entries)
-        // that keeps 16 waves per CU resident (the kernel's register occupancy)
+        // that keeps 16 waves per CU resident (the kernel's register
+        // occupancy), or, up to 2048 entries (a query visits ~1,000 ids up to
+        // P = 100 and spills past half), 12 waves: an HBM visited test costs
+        // more than the lost quarter of the waves
         unsigned vcap = 0;
         if (h.search_visited == 0)
-            for (unsigned c = 4096; c >= 256 && !vcap; c /= 2)
-                if (kg_search_lds_per_wave(poolmax, kmax + 1, h.Dp, seed_cap, c) * 16 <= h.lds_per_cu) vcap = c;
+            for (unsigned c = 4096; c >= 256 && !vcap; c /= 2) {
+                size_t const waves = h.lds_per_cu / kg_search_lds_per_wave(poolmax, kmax + 1, h.Dp, seed_cap, c);
+                if (waves >= 16 || (c <= 2048 && waves >= 12)) vcap = c;
+            }
         lpw = kg_search_lds_per_wave(poolmax, kmax + 1, h.Dp, seed_cap, vcap);
         unsigned wpb = std::min(h.search_wpb, kg_search_max_wpb());  // the kernel's launch bound
         while (wpb > 1 && lpw * wpb > 65536) wpb /= 2;

@@ -76,7 +76,7 @@ struct Handle {
     // value representable) graph walks gather it instead of X
     DevBuf<uint16_t> Xb;
     bool rows_bf16 = false;
-    int row_storage = 1;  // 0: walks gather the lossless BF16 copy when there is one, 1: FP32 rows
+    int row_storage = 0;  // 0: walks gather the lossless BF16 copy when there is one, 1: FP32 rows
     int search_visited = 0;  // 0: LDS visited hash set in front of the HBM bitset, 1: bitset only
     unsigned last_vcap = 0;  // LDS visited-set entries of the last walk launch
     bool has_data = false;

@@ -99,15 +99,34 @@ __device__ __forceinline__ void batch_dist(kg_search_args const &a, uint32_t con
         X = a.X;
     unsigned const lane = lane_id();
     unsigned const grp = lane >> 2, t = lane & 3;
-    for (unsigned b0 = 0; b0 < cnt; b0 += 16) {
-        unsigned c = b0 + grp;
-        if (b0 + grp < cnt) {  // whole quads are active together
-            uint32_t id = ids[c] & IDMASK;
-            RowT const *row = X + size_t(id) * a.Dp;
-            float acc = quad_metric<METRIC>(row, q, a.D, t);
-            float na = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[id];
-            float d = finish_metric<METRIC>(acc, na, qn);
-            if (t == 0) out[c] = d;
+    if constexpr (sizeof(RowT) == 2) {
+        // BF16 rows are half the bytes: each quad takes two rows a pass, 32 per
+        // pass, in the registers 16 FP32 rows need
+        for (unsigned b0 = 0; b0 < cnt; b0 += 32) {
+            unsigned const c0 = b0 + grp, c1 = c0 + 16;
+            if (c0 < cnt) {  // whole quads are active together
+                bool const two = c1 < cnt;
+                uint32_t const id0 = ids[c0] & IDMASK, id1 = two ? (ids[c1] & IDMASK) : id0;
+                float2 const acc = quad_metric2_bf16<METRIC>(X + size_t(id0) * a.Dp, X + size_t(id1) * a.Dp, q, a.D, t);
+                float const n0 = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[id0];
+                float const n1 = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[id1];
+                if (t == 0) {
+                    out[c0] = finish_metric<METRIC>(acc.x, n0, qn);
+                    if (two) out[c1] = finish_metric<METRIC>(acc.y, n1, qn);
+                }
+            }
+        }
+    } else {
+        for (unsigned b0 = 0; b0 < cnt; b0 += 16) {
+            unsigned c = b0 + grp;
+            if (b0 + grp < cnt) {  // whole quads are active together
+                uint32_t id = ids[c] & IDMASK;
+                RowT const *row = X + size_t(id) * a.Dp;
+                float acc = quad_metric<METRIC>(row, q, a.D, t);
+                float na = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[id];
+                float d = finish_metric<METRIC>(acc, na, qn);
+                if (t == 0) out[c] = d;
+            }
         }
     }
     wave_sync();
@@ -175,7 +194,10 @@ __device__ void batch_insert(WaveLDS &w, unsigned nf, uint32_t cm, uint32_t cb, 
                              unsigned &k) {
     unsigned const lane = lane_id();
     uint64_t key = (lane < nf) ? ((uint64_t(ord_f32(w.fdist[lane])) << 32) | lane) : ~0ull;
-    for (unsigned kk = 2; kk <= KG_WAVE; kk <<= 1) {  // bitonic sort across the wave
+    // bitonic sort of lanes [0, span), span = the power of two >= nf: lanes
+    // past nf hold the largest key, and no stage crosses a span boundary
+    unsigned const span = nf <= 2 ? 2u : 1u << (32 - __builtin_clz(nf - 1));
+    for (unsigned kk = 2; kk <= span; kk <<= 1) {
         for (unsigned j = kk >> 1; j > 0; j >>= 1) {
             uint64_t const other = shfl_xor64(key, int(j));
             bool const up = (lane & kk) == 0, lower = (lane & j) == 0;

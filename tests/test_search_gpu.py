@@ -99,7 +99,7 @@ def test_bf16_row_storage_gives_identical_walks(kind, metric, d, lossless):
     g = gpu_index(X, idx, metric)
     active, exact = ctypes.c_int(-1), ctypes.c_int(-1)
     g._h.call("kg_row_storage", ctypes.byref(active), ctypes.byref(exact))
-    assert active.value == 0 and exact.value == int(lossless)  # FP32 rows by default
+    assert active.value == int(lossless) and exact.value == int(lossless)  # the BF16 copy when lossless
     res = {}
     for mode in (0, 1):
         g._h.call("kg_set_row_storage", mode)

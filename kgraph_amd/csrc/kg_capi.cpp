@@ -368,6 +368,14 @@ static void set_entry_sample(Handle &h, unsigned E, unsigned seed) {
     KG_HIP(hipMemcpyAsync(h.eids.p, ids.data(), 4ull * E, hipMemcpyHostToDevice, h.stream));
     KG_HIP(kg_launch_gather_rows(h.X.p, h.Dp, h.eids.p, E, h.erows.p, h.stream));
     KG_HIP(kg_launch_norms(h.metric, h.erows.p, E, h.D, h.Dp, h.enorms.p, h.stream));
+    {  // BF16 copy: the MFMA selection's operands
+        h.erows_b.reserve(size_t(E) * h.Dp);
+        DevBuf<uint32_t> lossy;
+        lossy.reserve(1);
+        KG_HIP(hipMemsetAsync(lossy.p, 0, 4, h.stream));
+        KG_HIP(kg_launch_bf16_rows(h.erows.p, E, h.Dp, h.erows_b.p, lossy.p, h.stream));
+        h.sync();
+    }
     // ranking scale of the BF16 screen: ||e||^2/2, or ||e|| for angular
     h.escale.reserve(E);
     KG_HIP(kg_launch_norms(h.metric == KG_METRIC_ANGULAR ? KG_METRIC_ANGULAR : KG_METRIC_EUCLIDEAN, h.erows.p, E, h.D,
@@ -437,6 +445,7 @@ static void select_entries_device(Handle &h, float const *dQ, unsigned nq, size_
     }
     kg_entry_args a{};
     a.rows = h.erows.p;
+    a.rows_b = h.erows_b.p;
     a.enorms = h.enorms.p;
     a.ids = h.eids.p;
     a.E = h.entryE;
@@ -462,6 +471,7 @@ static void select_entries_device(Handle &h, float const *dQ, unsigned nq, size_
         if (learned) {  // squared Euclidean in the projected space
             mlp_forward_rows(h, Qk, nq, qs, h.hq.p, h.zq.p);
             a.rows = h.eproj.p;
+            a.rows_b = nullptr;
             a.D = a.Dp = a.qstride = h.mlpP;
             a.Q = h.zq.p;
             KG_HIP(kg_launch_norms(KG_METRIC_EUCLIDEAN, h.zq.p, nq, h.mlpP, h.mlpP, h.qsq.p, h.stream));

@@ -237,19 +237,66 @@ __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float
 #pragma unroll
     for (int i = 0; i < MF; ++i) list[i] = ~0ull;
     unsigned const ntiles = (a.E + 15) / 16;
-    for (unsigned et = w; et < ntiles; et += 4) {
+    // operands of sample tile `et`: from the BF16 copy when there is one, else
+    // rounded from the FP32 rows (the same values)
+    auto load_tile = [&](unsigned et, bf16x4_t (&op)[KS]) {
         unsigned const e = et * 16 + r16;
         bool const eok = e < a.E;
-        float const *er = a.rows + size_t(eok ? e : 0u) * a.Dp;
-        bf16x4_t aop[KS];
+        if (a.rows_b) {
+            uint16_t const *er = a.rows_b + size_t(eok ? e : 0u) * a.Dp;
 #pragma unroll
-        for (int s = 0; s < KS; ++s) {
-            unsigned const k = 16 * s + kq;
-            float4 const v = (eok && k < a.Dp) ? *reinterpret_cast<float4 const *>(er + k) : make_float4(0, 0, 0, 0);
-            aop[s][0] = bf16_rne(v.x);
-            aop[s][1] = bf16_rne(v.y);
-            aop[s][2] = bf16_rne(v.z);
-            aop[s][3] = bf16_rne(v.w);
+            for (int s = 0; s < KS; ++s) {
+                unsigned const k = 16 * s + kq;
+                uint2 const v = (eok && k < a.Dp) ? *reinterpret_cast<uint2 const *>(er + k) : make_uint2(0u, 0u);
+                op[s] = __builtin_bit_cast(bf16x4_t, v);
+            }
+        } else {
+            float const *er = a.rows + size_t(eok ? e : 0u) * a.Dp;
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+                unsigned const k = 16 * s + kq;
+                float4 const v =
+                    (eok && k < a.Dp) ? *reinterpret_cast<float4 const *>(er + k) : make_float4(0, 0, 0, 0);
+                op[s][0] = bf16_rne(v.x);
+                op[s][1] = bf16_rne(v.y);
+                op[s][2] = bf16_rne(v.z);
+                op[s][3] = bf16_rne(v.w);
+            }
+        }
+    };
+    // the norms and ids of the 4 sampled rows whose results a lane holds
+    // (rows 16 et + kq .. + 3)
+    float const *nsrc = (METRIC == METRIC_L2SQR) ? esq : a.enorms;
+    auto load_meta = [&](unsigned et, float (&nv)[4], uint32_t (&iv)[4]) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            unsigned const ee = et * 16 + kq + j;
+            nv[j] = ee < a.E ? nsrc[ee] : 0.f;
+            iv[j] = ee < a.E ? a.ids[ee] : 0u;
+        }
+    };
+    // the next tile's operands, norms and ids load while this tile is scored
+    bf16x4_t nxt[KS];
+    float nn[4];
+    uint32_t ni[4];
+    if (w < ntiles) {
+        load_tile(w, nxt);
+        load_meta(w, nn, ni);
+    }
+    for (unsigned et = w; et < ntiles; et += 4) {
+        bf16x4_t aop[KS];
+        float en[4];
+        uint32_t eid[4];
+#pragma unroll
+        for (int s = 0; s < KS; ++s) aop[s] = nxt[s];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            en[j] = nn[j];
+            eid[j] = ni[j];
+        }
+        if (et + 4 < ntiles) {
+            load_tile(et + 4, nxt);
+            load_meta(et + 4, nn, ni);
         }
         f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -261,12 +308,12 @@ __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float
             float const G = acc[j];
             float d;
             if (METRIC == METRIC_L2SQR)
-                d = (qn + 2.f * esq[ee]) - 2.f * G;
+                d = (qn + 2.f * en[j]) - 2.f * G;
             else if (METRIC == METRIC_EUCLIDEAN)
-                d = (a.enorms[ee] + qn) - G;
+                d = (en[j] + qn) - G;
             else
-                d = -G / __builtin_fmaf(a.enorms[ee], qn, 1.1920928955078125e-07f);
-            uint64_t const key = (uint64_t(ord_f32(d)) << 32) | a.ids[ee];
+                d = -G / __builtin_fmaf(en[j], qn, 1.1920928955078125e-07f);
+            uint64_t const key = (uint64_t(ord_f32(d)) << 32) | eid[j];
             if (key < list[MF - 1]) {  // sorted insert, compile-time indices
                 bool placed = false;
 #pragma unroll

@@ -111,6 +111,7 @@ struct Handle {
     // entry-point sample
     unsigned entryE = 0;
     DevBuf<float> erows, enorms, escores, escale;
+    DevBuf<uint16_t> erows_b;  // BF16 copy of the entry sample's rows
     DevBuf<uint32_t> eids;
 
     // learned projector D -> mlpH -> mlpP (FP32 master weights, Adam state)

@@ -71,6 +71,7 @@ struct kg_brute_args {
 
 struct kg_entry_args {
     float const *rows;    // E sampled rows, stride Dp
+    uint16_t const *rows_b;  // their BF16 copy (same stride; NULL: rounded from rows)
     float const *enorms;  // their norms
     uint32_t const *ids;  // their node ids (ascending)
     unsigned E, D, Dp;

@@ -342,6 +342,11 @@ def batched_workload(args):
     h.call("kg_build", ip, info)
     build_s = time.perf_counter() - tb
     log("build %.2fs, %d iterations, control recall %.4f" % (build_s, info.iterations, info.recall))
+    nit = _lib.ctypes.c_uint()
+    h.call("kg_build_iterations", None, 0, _lib.ctypes.byref(nit))
+    its = np.zeros(6 * nit.value, np.uint64)
+    h.call("kg_build_iterations", ptr(its), nit.value, _lib.ctypes.byref(nit))
+    build_forms = "".join("TSEP"[int(f)] for f in its.reshape(-1, 6)[:, 3])
     W = 100
     dQ = devmem.DeviceArray.from_host(Q)
     dI = devmem.DeviceArray((nq, W), np.uint32)
@@ -371,6 +376,47 @@ def batched_workload(args):
     wall = float(np.median(walls))
     seeds = np.minimum(m, Ks.astype(np.int64) + Ps - 1).astype(np.float64)
     byt = float((rb * D * (seeds + comps) + 4 * scans + 4 * seeds).sum())
+    # the same work without the mixed launch: one launch per distinct (K, P)
+    # (each group's queries contiguous in HBM, entry selection per group), and
+    # one launch per query on a 1,000-query sample (the reference's per-call
+    # search interface, here through the C ABI from Python)
+    groups = sorted(set(zip(Ks.tolist(), Ps.tolist())))
+    gq = []
+    for (k_, p_) in groups:
+        sel = np.nonzero((Ks == k_) & (Ps == p_))[0]
+        gq.append((k_, p_, devmem.DeviceArray.from_host(np.ascontiguousarray(Q[sel])), len(sel)))
+    gwall = []
+    for s in range(args.warmup + 1):
+        devmem.synchronize()
+        ts = time.perf_counter()
+        for (k_, p_, dq, cnt) in gq:
+            h.call("kg_select_entries_device", dq.ptr, cnt, D, m, dI.ptr, W)
+            spg = SearchParams(k_, 0, p_, 10, 1, 1e30, 1998, min(m, k_ + p_ - 1) if k_ > 1 else 0)
+            if k_ > m:
+                h.call("kg_search_device", dq.ptr, cnt, D, spg, None, None, dI.ptr, dD.ptr, W, dC.ptr, None, None)
+            else:  # one-K calls keep the reference's rule init < K: pass per-query K/P to allow it
+                kk = np.full(cnt, k_, np.uint32)
+                pp = np.full(cnt, p_, np.uint32)
+                spg = SearchParams(k_, 0, p_, 10, 1, 1e30, 1998, m)
+                h.call("kg_search_device", dq.ptr, cnt, D, spg, ptr(kk), ptr(pp), dI.ptr, dD.ptr, W, dC.ptr, None,
+                       None)
+        devmem.synchronize()
+        gwall.append(time.perf_counter() - ts)
+    grouped_qps = nq / gwall[-1]
+    npq = min(1000, nq)
+    devmem.synchronize()
+    ts = time.perf_counter()
+    for i in range(npq):
+        kk = Ks[i:i + 1].copy()
+        pp = Ps[i:i + 1].copy()
+        qp = dQ.ptr.value + i * D * 4
+        h.call("kg_select_entries_device", _lib.ctypes.c_void_p(qp), 1, D, m, dI.ptr, W)
+        h.call("kg_search_device", _lib.ctypes.c_void_p(qp), 1, D, sp, ptr(kk), ptr(pp), dI.ptr, dD.ptr, W, dC.ptr,
+               None, None)
+    devmem.synchronize()
+    perq_qps = npq / (time.perf_counter() - ts)
+    log("mixed launch %.0f QPS; one launch per (K, P): %.0f QPS (%d launches); one launch per query: %.0f QPS" % (
+        nq / wall, grouped_qps, len(groups), perq_qps))
     # recall on a query sample against the exact scan
     ns = min(1000, nq)
     gt_i = np.zeros((ns, 100), np.uint32)
@@ -394,9 +440,14 @@ def batched_workload(args):
                    "graph": {"K": 32, "L": 64, "reverse": -1, "iterations": args.iterations}},
         "entry_ms": round(ems, 3),
         "build_s": round(build_s, 3), "build_control_recall": round(float(info.recall), 4),
+        "build_iteration_join_forms": build_forms,
         "latency_us": {"p50": round(float(np.percentile(lat, 50)), 1),
                        "p99": round(float(np.percentile(lat, 99)), 1)},
         "recall_by_K_sample": rec, "kernel_ms": round(kms, 3),
+        "plain_forms_qps": {"one_launch_per_K_P": round(grouped_qps, 1), "launches": len(groups),
+                            "one_launch_per_query": round(perq_qps, 1), "per_query_sample": npq},
+        "speedup_vs_one_launch_per_K_P": round((nq / wall) / grouped_qps, 3),
+        "speedup_vs_one_launch_per_query": round((nq / wall) / perq_qps, 2),
         "roofline": {"kernel": "search_kernel", "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": None},

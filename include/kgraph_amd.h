@@ -106,8 +106,9 @@ int kg_set_precision(kg_handle *h, int precision);
 /* Visited set of the graph walk (no counterpart in the reference, which
  * allocates an N-bit bitset per query, kgraph.cpp:438).  Mode 0 (default):
  * each resident wave keeps an exact hash set of visited ids in LDS (the
- * largest power of two up to 4096 entries that keeps 16 waves per CU, or up
- * to 2048 entries with 12 waves) in front of its N-bit HBM bitset; a query that fills it past half moves it
+ * largest power of two up to 4096 entries that keeps 16 waves per CU, or
+ * 2048 entries with 12 waves when 16 waves leave less) in front of its N-bit
+ * HBM bitset; a query that fills it past half moves it
  * into the bitset and continues there.  Mode 1: the bitset only.  Results are
  * identical either way. */
 int kg_set_search_visited(kg_handle *h, int mode);

@@ -212,15 +212,19 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         size_t lpw = kg_search_lds_per_wave(poolmax, kmax + 1, h.Dp, seed_cap, 0);
         // visited hash set in LDS: the largest power of two (256..4096 entries)
         // that keeps 16 waves per CU resident (the kernel's register
-        // occupancy), or, up to 2048 entries (a query visits ~1,000 ids up to
-        // P = 100 and spills past half), 12 waves: an HBM visited test costs
-        // more than the lost quarter of the waves
+        // occupancy); when that is under 2048 entries and 2048 fit with 12
+        // waves, 2048 at 12 waves (a query visits ~1,000 ids up to P = 100
+        // and spills past half: an HBM visited test costs more than the lost
+        // quarter of the waves, a set that spills does not)
         unsigned vcap = 0;
-        if (h.search_visited == 0)
-            for (unsigned c = 4096; c >= 256 && !vcap; c /= 2) {
-                size_t const waves = h.lds_per_cu / kg_search_lds_per_wave(poolmax, kmax + 1, h.Dp, seed_cap, c);
-                if (waves >= 16 || (c <= 2048 && waves >= 12)) vcap = c;
-            }
+        if (h.search_visited == 0) {
+            auto waves = [&](unsigned c) {
+                return h.lds_per_cu / kg_search_lds_per_wave(poolmax, kmax + 1, h.Dp, seed_cap, c);
+            };
+            for (unsigned c = 4096; c >= 256 && !vcap; c /= 2)
+                if (waves(c) >= 16) vcap = c;
+            if (vcap < 2048 && waves(2048) >= 12) vcap = 2048;
+        }
         lpw = kg_search_lds_per_wave(poolmax, kmax + 1, h.Dp, seed_cap, vcap);
         unsigned wpb = std::min(h.search_wpb, kg_search_max_wpb());  // the kernel's launch bound
         while (wpb > 1 && lpw * wpb > 65536) wpb /= 2;

@@ -209,17 +209,26 @@ __global__ void __launch_bounds__(256) intcheck_kernel(float const *X, unsigned 
 // BF16 value form the shortlist that rescore_kernel re-ranks exactly.
 // One block per 16 queries, 4 waves splitting the sample's 16-row tiles
 // (wave w takes tiles w, w + 4, ...).  A tile's operands: the sampled rows
-// (A, loaded per tile) and the block's 16 queries (B, kept in registers), so
-// lane l holds the distances of query l % 16 to rows 4 (l / 16) + j.  Each
-// lane keeps its query's MF best (dist, id) keys in registers; the four
-// lanes of a query and then the four waves are merged into the m best.
+// (A, loaded per tile, the next tile's while this one is scored) and the
+// block's 16 queries (B, kept in registers), so lane l holds the distances of
+// query l % 16 to rows 4 (l / 16) + j.  Each lane keeps the MF >= m best of
+// the rows it sees as 32-bit ordered distances with their ids: it sees rows
+// in ascending id order, so inserting after equal distances keeps (dist, id)
+// order.  The 16 lanes of a query (4 per wave, 4 waves) share a bound, the
+// smallest of their m-th best distances (any lane's m best are m keys no
+// worse than it, so the query's m-th result is no worse either): a distance
+// above it cannot enter the result and skips the insert.  The four lanes of
+// a query and then the four waves are merged into the m best.
 template <int METRIC, int KS, int MF>
 __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float const *qsq, float const *esq) {
     __shared__ uint64_t best[4][16][MF];
+    __shared__ uint32_t bound[16];  // per query: smallest m-th best ordered distance of its lanes
     unsigned const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     unsigned const r16 = lane & 15, kq = (lane >> 4) * 4;
     unsigned const q = blockIdx.x * 16 + r16;
     bool const qok = q < a.nq;
+    if (threadIdx.x < 16) bound[threadIdx.x] = 0xffffffffu;
+    __syncthreads();
     bf16x4_t bop[KS];
     {
         float const *qr = a.Q + size_t(qok ? q : 0u) * a.qstride;
@@ -233,9 +242,12 @@ __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float
         }
     }
     float const qn = qok ? ((METRIC == METRIC_L2SQR) ? 2.f * qsq[q] : a.qnorms[q]) : 0.f;
-    uint64_t list[MF];
+    uint32_t lk[MF], li[MF];  // ordered distances and ids, ascending
 #pragma unroll
-    for (int i = 0; i < MF; ++i) list[i] = ~0ull;
+    for (int i = 0; i < MF; ++i) {
+        lk[i] = 0xffffffffu;
+        li[i] = 0xffffffffu;
+    }
     unsigned const ntiles = (a.E + 15) / 16;
     // operands of sample tile `et`: from the BF16 copy when there is one, else
     // rounded from the FP32 rows (the same values)
@@ -283,7 +295,20 @@ __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float
         load_tile(w, nxt);
         load_meta(w, nn, ni);
     }
+    uint32_t tq = 0xffffffffu;  // the query's shared bound as this lane last saw it
     for (unsigned et = w; et < ntiles; et += 4) {
+        if (((et >> 2) & 3) == 3) {
+            // refresh the bound: this lane's m-th best, the four lanes of the
+            // query in this wave, then the block's other waves (LDS minimum)
+            uint32_t mine = 0xffffffffu;
+#pragma unroll
+            for (int i = 0; i < MF; ++i)
+                if (unsigned(i) + 1 == a.m) mine = lk[i];
+            mine = min(mine, uint32_t(__shfl_xor(int(mine), 16)));
+            mine = min(mine, uint32_t(__shfl_xor(int(mine), 32)));
+            atomicMin(&bound[r16], mine);
+            tq = min(tq, __hip_atomic_load(&bound[r16], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+        }
         bf16x4_t aop[KS];
         float en[4];
         uint32_t eid[4];
@@ -313,35 +338,45 @@ __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float
                 d = (en[j] + qn) - G;
             else
                 d = -G / __builtin_fmaf(en[j], qn, 1.1920928955078125e-07f);
-            uint64_t const key = (uint64_t(ord_f32(d)) << 32) | eid[j];
-            if (key < list[MF - 1]) {  // sorted insert, compile-time indices
+            uint32_t const kd = ord_f32(d);
+            if (kd <= tq && kd < lk[MF - 1]) {  // sorted insert after equal distances, compile-time indices
                 bool placed = false;
 #pragma unroll
                 for (int i = MF - 1; i > 0; --i) {
                     if (!placed) {
-                        if (list[i - 1] > key) {
-                            list[i] = list[i - 1];
+                        if (lk[i - 1] > kd) {
+                            lk[i] = lk[i - 1];
+                            li[i] = li[i - 1];
                         } else {
-                            list[i] = key;
+                            lk[i] = kd;
+                            li[i] = eid[j];
                             placed = true;
                         }
                     }
                 }
-                if (!placed) list[0] = key;
+                if (!placed) {
+                    lk[0] = kd;
+                    li[0] = eid[j];
+                }
             }
         }
     }
     // the four lanes of a query (lane % 16 equal): m rounds of the minimum head
     for (unsigned r = 0; r < a.m; ++r) {
-        uint64_t v = list[0];
+        uint64_t const head = (uint64_t(lk[0]) << 32) | li[0];
+        uint64_t v = head;
         uint64_t u = shfl_xor64(v, 16);
         v = u < v ? u : v;
         u = shfl_xor64(v, 32);
         v = u < v ? u : v;
-        if (list[0] == v && v != ~0ull) {  // keys are unique: one lane pops
+        if (head == v && v != ~0ull) {  // keys are unique: one lane pops
 #pragma unroll
-            for (int i = 0; i + 1 < MF; ++i) list[i] = list[i + 1];
-            list[MF - 1] = ~0ull;
+            for (int i = 0; i + 1 < MF; ++i) {
+                lk[i] = lk[i + 1];
+                li[i] = li[i + 1];
+            }
+            lk[MF - 1] = 0xffffffffu;
+            li[MF - 1] = 0xffffffffu;
         }
         if (lane < 16) best[w][r16][r] = v;
     }

@@ -242,13 +242,16 @@ def _entry_sample_ids(seed, E, N):
 
 
 # integer rows and queries take the exact MFMA selection (BF16 tiles equal the
-# FP32 distances there); FP32 data the canonical FP32 scan
-@pytest.mark.parametrize("kind,metric,d,m", [("intmix", L2, 128, 7), ("uniform", EUC, 16, 7), ("angular", ANG, 96, 7),
-                                             ("intmix", EUC, 64, 9), ("intmix", ANG, 32, 16), ("intmix", L2, 24, 3)])
-def test_entry_selection_parity(kind, metric, d, m):
+# FP32 distances there); FP32 data the canonical FP32 scan.  Small samples
+# (every lane sees 4 or 8 rows) make the 4-key first pass hand many queries to
+# the second pass.
+@pytest.mark.parametrize("kind,metric,d,m,E", [("intmix", L2, 128, 7, 512), ("uniform", EUC, 16, 7, 512),
+                                               ("angular", ANG, 96, 7, 512), ("intmix", EUC, 64, 9, 512),
+                                               ("intmix", ANG, 32, 16, 512), ("intmix", L2, 24, 3, 512),
+                                               ("intmix", L2, 128, 16, 64), ("intmix", EUC, 64, 12, 128)])
+def test_entry_selection_parity(kind, metric, d, m, E):
     X, Q, idx = setup(kind, metric, d=d)
     g = gpu_index(X, idx, metric)
-    E = 512
     g.set_entry_sample(E, seed=77)
     got = g.select_entries(Q, m)
     samp = _entry_sample_ids(77, E, len(X))

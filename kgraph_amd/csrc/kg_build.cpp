@@ -55,7 +55,8 @@ float eval_accuracy(float const *pool, unsigned np, float const *knn, unsigned n
 struct NndState {
     DevBuf<uint64_t> pool, cand, ovf, stats;
     DevBuf<uint32_t> part, cnt, Mv, found, samp, nnew, nold, rnew, rold, crnew, crold, ccnt;
-    DevBuf<float> radius, radiusM, kth;
+    DevBuf<float> radiusM;
+    DevBuf<float4> meta;
     DevBuf<uint32_t> ctl, ccount;
     DevBuf<float> cdist;
     DevBuf<float2> sqn;
@@ -187,7 +188,8 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     st.samp.reserve(size_t(N) * a.Lc);
     st.rnew.reserve(size_t(N) * R);
     st.rold.reserve(size_t(N) * R);
-    for (auto *b : {&st.radius, &st.radiusM, &st.kth}) b->reserve(N);
+    st.radiusM.reserve(N);
+    st.meta.reserve(N);
     a.pool = st.pool.p;
     a.cand = st.cand.p;
     a.ovf = st.ovf.p;
@@ -205,9 +207,8 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     a.crnew = st.crnew.p;
     a.crold = st.crold.p;
     a.ccnt = st.ccnt.p;
-    a.radius = st.radius.p;
+    a.meta = st.meta.p;
     a.radiusM = st.radiusM.p;
-    a.kth = st.kth.p;
     if (h.precision == KG_PRECISION_BF16 && h.D <= 256) {
         // local join as BF16 MFMA tiles; exact as it is on integer rows with
         // 2 D max|x|^2 <= 2^24, else a screen with an exact FP32 re-score
@@ -266,6 +267,7 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
         KG_HIP(hipMemcpyAsync(ctl_knn.data(), st.cdist.p, 4 * ctl_knn.size(), hipMemcpyDeviceToHost, s));
         h.sync();
     }
+    KG_HIP(nnd_launch_meta_init(h.metric, a, s));
     KG_HIP(nnd_launch_init(h.metric, a, s));
 
     kg_index_info info{};

@@ -31,9 +31,9 @@ struct NndArgs {
     uint32_t *cnt;   // valid pool entries (Nhood::L)
     uint32_t *Mv;    // Nhood::M
     uint32_t *found; // Nhood::found
-    float *radius;   // worst pool distance once full, +inf before
+    float4 *meta;    // per node: radius (worst pool distance once full, +inf before), K-th distance
+                     // (pool[K-1] once K entries exist, +inf before), join norm, |x|
     float *radiusM;  // pool[M-1].dist
-    float *kth;      // pool[K-1].dist once K entries exist, +inf before
     uint32_t *samp;  // N x Lc: sampled new ids from the front, old ids from the back
     uint32_t *nnew, *nold;
     uint32_t *rnew, *rold;    // N x R reverse lists
@@ -58,6 +58,8 @@ hipError_t nnd_launch_join(int metric, NndArgs const &a, hipStream_t s);
 hipError_t nnd_launch_sqn(NndArgs const &a, float2 *out, uint32_t *flags, hipStream_t s);
 hipError_t nnd_launch_merge(NndArgs const &a, hipStream_t s);
 hipError_t nnd_launch_update(NndArgs const &a, hipStream_t s);
+// Fills the join norms of a.meta (after the MFMA form's sqn, when used).
+hipError_t nnd_launch_meta_init(int metric, NndArgs const &a, hipStream_t s);
 // The join's BF16 MFMA distance and its error bound for n (a, b) id pairs
 // (pairs[2p], pairs[2p+1]); needs X, Xb, norms, sqn, D, Dp and the bound
 // constants of the args.

@@ -158,8 +158,9 @@ __global__ void __launch_bounds__(256) init_kernel(NndArgs a) {
         a.nold[n] = 0;
         a.crnew[n] = 0;
         a.crold[n] = 0;
-        a.radius[n] = (S >= a.L) ? key_dist(pool[a.L - 1]) : __builtin_huge_valf();
-        a.kth[n] = (S >= a.K) ? key_dist(pool[a.K - 1]) : __builtin_huge_valf();
+        *reinterpret_cast<float2 *>(&a.meta[n]) =
+            make_float2((S >= a.L) ? key_dist(pool[a.L - 1]) : __builtin_huge_valf(),
+                        (S >= a.K) ? key_dist(pool[a.K - 1]) : __builtin_huge_valf());
         a.ccnt[n] = 0;
     }
 }
@@ -208,6 +209,7 @@ template <unsigned Q>
 struct MJoinLDS {
     uint32_t ids[MT];
     uint32_t cnt[MT];
+    float an[16];  // join norms of the current 16 new rows (FP32 tile form)
     uint64_t key[MB];
     uint32_t rq[Q];  // pair (list index of the new row) | (list index of the other row) << 16
     uint16_t tgt[MB];
@@ -315,13 +317,14 @@ __device__ void mj_rescore(NndArgs const &a, LDS &L, unsigned rqn, unsigned &fil
         unsigned const ia = pr & 0xffffu, io = pr >> 16;
         uint32_t const av = L.ids[ia], o = L.ids[io];
         float const acc = quad_metric<METRIC>(a.X + size_t(av) * a.Dp, a.X + size_t(o) * a.Dp, a.D, t);
-        float const na = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[av];
-        float const no = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[o];
+        float4 const ma = a.meta[av], mo = a.meta[o];
+        float const na = (METRIC == METRIC_L2SQR) ? 0.f : ma.z;
+        float const no = (METRIC == METRIC_L2SQR) ? 0.f : mo.z;
         float const d = finish_metric<METRIC>(acc, na, no);
         bool const lead = act && t == 0;
-        bool const pa = lead && d < a.radius[av];
-        bool const po = lead && d < a.radius[o];
-        found |= lead && d < a.kth[av];
+        bool const pa = lead && d < ma.x;
+        bool const po = lead && d < mo.x;
+        found |= lead && d < ma.y;
         pushes += unsigned(pa) + unsigned(po);
         uint64_t const key[2] = {make_key(d, o, 1), make_key(d, av, 1)};
         unsigned const tgt[2] = {ia, io};
@@ -371,11 +374,11 @@ __global__ void __launch_bounds__(256) join_mfma_kernel(NndArgs a) {
             unsigned const ia = ag + mrow + j;
             bool const ok = ia < nA;
             aid[j] = ok ? L.ids[ia] : 0u;
-            arad[j] = ok ? a.radius[aid[j]] : 0.f;
-            akth[j] = ok ? a.kth[aid[j]] : 0.f;
-            float2 const sq = a.sqn[aid[j]];
-            an[j] = (METRIC == METRIC_L2SQR) ? sq.x : a.norms[aid[j]];
-            arn[j] = sq.y;
+            float4 const m = a.meta[aid[j]];  // radius, K-th distance, join norm, |x|
+            arad[j] = ok ? m.x : 0.f;
+            akth[j] = ok ? m.y : 0.f;
+            an[j] = m.z;
+            arn[j] = m.w;
         }
         for (unsigned tg = 0; tg < nT; tg += 16) {
             unsigned const io = tg + r16;
@@ -383,9 +386,8 @@ __global__ void __launch_bounds__(256) join_mfma_kernel(NndArgs a) {
             uint32_t const oid = ook ? L.ids[io] : 0u;
             mj_bf16x4 bop[KS];
             mj_load<KS>(a, oid, ook, kq, bop);
-            float const orad = a.radius[oid];
-            float2 const osq = a.sqn[oid];
-            float const on = (METRIC == METRIC_L2SQR) ? osq.x : a.norms[oid];
+            float4 const om = a.meta[oid];
+            float const orad = om.x, on = om.z;
             mj_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int s = 0; s < KS; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aop[s], bop[s], acc, 0, 0, 0);
@@ -410,7 +412,7 @@ __global__ void __launch_bounds__(256) join_mfma_kernel(NndArgs a) {
                     tgt[2 * j] = ia;
                     tgt[2 * j + 1] = io;
                 } else {
-                    float const lo = d - mj_bound<METRIC>(a, an[j], on, arn[j], osq.y);
+                    float const lo = d - mj_bound<METRIC>(a, an[j], on, arn[j], om.w);
                     bool const pass = valid && (lo < arad[j] || lo < orad);
                     uint64_t const b = ballot(pass);
                     if (pass) L.rq[rqn + popc64(b & ((1ull << lane) - 1))] = ia | (io << 16);
@@ -481,9 +483,12 @@ __global__ void __launch_bounds__(256) join_tile_kernel(NndArgs a) {
             unsigned const ia = ag + t + 4 * i;
             bool const ok = ia < nA;
             aid[i] = ok ? L.ids[ia] : 0u;
-            arad[i] = ok ? a.radius[aid[i]] : 0.f;
-            akth[i] = ok ? a.kth[aid[i]] : 0.f;
+            float4 const m = a.meta[aid[i]];  // radius, K-th distance, join norm, |x|
+            arad[i] = ok ? m.x : 0.f;
+            akth[i] = ok ? m.y : 0.f;
         }
+        if (lane < 16) L.an[lane] = (METRIC == METRIC_L2SQR || ag + lane >= nA) ? 0.f : a.meta[L.ids[ag + lane]].z;
+        wave_sync();
         for (unsigned tg = 0; tg < nT; tg += 16) {
             unsigned const io = tg + quad;
             bool const ook = io < nT;
@@ -497,8 +502,9 @@ __global__ void __launch_bounds__(256) join_tile_kernel(NndArgs a) {
                     reg[k] = (kk < nch) ? reinterpret_cast<float4 const *>(orow)[kk] : make_float4(0, 0, 0, 0);
                 }
             }
-            float const orad = a.radius[oid];
-            float const on = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[oid];
+            float4 const om = a.meta[oid];
+            float const orad = om.x;
+            float const on = (METRIC == METRIC_L2SQR) ? 0.f : om.z;
             float dres[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
@@ -540,7 +546,7 @@ __global__ void __launch_bounds__(256) join_tile_kernel(NndArgs a) {
                 } else {
                     acc = quad_metric<METRIC>(ra, orow, a.D, t);
                 }
-                float const na = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[av];
+                float const na = L.an[r];
                 float const d = finish_metric<METRIC>(acc, na, on);
                 if ((r & 3) == int(t)) dres[r >> 2] = d;
             }
@@ -607,6 +613,22 @@ __global__ void __launch_bounds__(256) sqn_kernel(float const *X, unsigned N, un
     for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
     if (lane == 0 && __float_as_uint(mx) > __hip_atomic_load(flags + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
         atomicMax(flags + 1, __float_as_uint(mx));
+}
+
+// Per-node join record: z = the norm the join's distance formula takes (|x|^2
+// for SSD on the MFMA form, else the stored metric norm, 0 for SSD on FP32
+// tiles), w = |x| (the screen bound; MFMA form only).  x, y (radius, K-th
+// distance) are written by init_kernel and merge_kernel.
+__global__ void meta_init_kernel(float4 *meta, unsigned N, int metric, float2 const *sqn, float const *norms) {
+    unsigned const n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    float z = 0.f, w = 0.f;
+    if (sqn) w = sqn[n].y;
+    if (metric != METRIC_L2SQR)
+        z = norms[n];
+    else if (sqn)
+        z = sqn[n].x;
+    meta[n] = make_float4(__builtin_huge_valf(), __builtin_huge_valf(), z, w);
 }
 
 __global__ void iter_begin_kernel(uint64_t *stats, uint32_t *iter, int advance) {
@@ -764,8 +786,8 @@ __device__ void merge_node(NndArgs const &a, unsigned n, unsigned c, unsigned m,
     if (lane == 0) {
         a.cnt[n] = newc;
         a.ccnt[n] = 0;
-        a.radius[n] = (newc >= a.L) ? rad_w : __builtin_huge_valf();
-        a.kth[n] = (newc >= a.K) ? kth_w : __builtin_huge_valf();
+        *reinterpret_cast<float2 *>(&a.meta[n]) =
+            make_float2((newc >= a.L) ? rad_w : __builtin_huge_valf(), (newc >= a.K) ? kth_w : __builtin_huge_valf());
         a.part[NND_STAT_DELTA * a.N + n] = flags;
         a.part[NND_STAT_M * a.N + n] = a.Mv[n];
     }
@@ -1076,5 +1098,11 @@ hipError_t nnd_launch_screen_pairs(int metric, NndArgs const &a, uint32_t const 
     else
         KG_SCREEN(METRIC_ANGULAR);
 #undef KG_SCREEN
+    return hipGetLastError();
+}
+
+hipError_t nnd_launch_meta_init(int metric, NndArgs const &a, hipStream_t s) {
+    hipLaunchKernelGGL(meta_init_kernel, dim3((a.N + 255) / 256), dim3(256), 0, s, a.meta, a.N, metric,
+                       a.mfma ? a.sqn : nullptr, a.norms);
     return hipGetLastError();
 }

@@ -173,45 +173,35 @@ __device__ __forceinline__ void pool_shift(WaveLDS &w, unsigned r, unsigned L) {
     }
 }
 
-__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
-    uint32_t const lo = __shfl_xor(uint32_t(v), m), hi = __shfl_xor(uint32_t(v >> 32), m);
-    return (uint64_t(hi) << 32) | lo;
-}
-
 // The reference inserts the fresh neighbours of a batch one at a time
 // (UpdateKnnList, kgraph.cpp:497-512): each goes after every pool entry with a
 // smaller or equal distance, the pool grows to K + P entries and then drops
 // its last, and the expansion index rewinds to the smallest insert position.
-// Done at once it is a stable merge: sort the nf candidates (lane order =
-// neighbour order) by (dist, order); candidate s lands at (pool entries with
-// dist <= its dist) + s, pool entry i at i + (candidates with a smaller
-// dist); positions from min(L + nf, qcap - 1) on are dropped.  The smallest
-// insert position of the sequential loop is the final position of the best
+// Done at once it is a stable merge: candidate `lane` (lane order = neighbour
+// order) has rank s = the candidates before it in (dist, order), counted
+// across the wave with no sort, and lands at (pool entries with dist <= its
+// dist) + s; pool entry i moves to i + (candidates with a smaller dist);
+// positions from min(L + nf, qcap - 1) on are dropped.  The smallest insert
+// position of the sequential loop is the final position of the best
 // candidate (no later insert lands before it, no earlier one after it), so
 // the rewind is k = min(k, that position) when it is kept.  cm / cb hold
 // candidate `lane`'s M and CSR base; w.fid / w.fdist its id and distance.
 __device__ void batch_insert(WaveLDS &w, unsigned nf, uint32_t cm, uint32_t cb, unsigned qcap, unsigned &L,
                              unsigned &k) {
     unsigned const lane = lane_id();
-    uint64_t key = (lane < nf) ? ((uint64_t(ord_f32(w.fdist[lane])) << 32) | lane) : ~0ull;
-    // bitonic sort of lanes [0, span), span = the power of two >= nf: lanes
-    // past nf hold the largest key, and no stage crosses a span boundary
-    unsigned const span = nf <= 2 ? 2u : 1u << (32 - __builtin_clz(nf - 1));
-    for (unsigned kk = 2; kk <= span; kk <<= 1) {
-        for (unsigned j = kk >> 1; j > 0; j >>= 1) {
-            uint64_t const other = shfl_xor64(key, int(j));
-            bool const up = (lane & kk) == 0, lower = (lane & j) == 0;
-            uint64_t const mn = other < key ? other : key, mx = other < key ? key : other;
-            key = (lower == up) ? mn : mx;
-        }
-    }
-    unsigned const src = uint32_t(key);  // candidate sorted to position `lane`
     bool const live = lane < nf;
-    float const sd = unord_f32(uint32_t(key >> 32));
-    uint32_t const sm = __shfl(cm, int(src & 63u)), sb = __shfl(cb, int(src & 63u));
-    uint32_t const sid = live ? w.fid[src] : 0u;
+    float const dl = live ? w.fdist[lane] : __builtin_huge_valf();
+    // rank of candidate `lane` in (dist, neighbour order): no sort needed
+    unsigned rank = 0;
+    for (unsigned jj = 0; jj < nf; ++jj) {
+        float const dj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dl), int(jj)));
+        rank += unsigned(dj < dl || (dj == dl && jj < lane));
+    }
+    uint32_t const sid = live ? w.fid[lane] : 0u;
+    uint32_t const sm = cm, sb = cb;
+    float const sd = dl;
     unsigned const L0 = L, Lf = min(L0 + nf, qcap - 1);
-    // final position of sorted candidate `lane`: pool entries with dist <= sd, plus lane
+    // final position: pool entries with dist <= sd, plus the rank
     unsigned fin = 0;
     if (live) {
         unsigned lo = 0, hi = L0;
@@ -219,11 +209,13 @@ __device__ void batch_insert(WaveLDS &w, unsigned nf, uint32_t cm, uint32_t cb, 
             unsigned const mid = (lo + hi) >> 1;
             if (w.pdist[mid] <= sd) lo = mid + 1; else hi = mid;
         }
-        fin = lo + lane;
+        fin = lo + rank;
     }
-    unsigned const fin0 = __shfl(fin, 0);
+    uint64_t const b0 = ballot(live && rank == 0);
+    unsigned const best_lane = b0 ? ctz64(b0) : 0u;
+    unsigned const fin0 = __shfl(fin, int(best_lane));
     wave_sync();
-    if (live) w.fdist[lane] = sd;  // sorted distances for the pool entries' search
+    if (live) w.fdist[rank] = sd;  // sorted distances for the pool entries' search
     wave_sync();
     // pool entries move up by the number of candidates strictly below them
     // (those before the best candidate's position stay); top chunk first, each

@@ -87,6 +87,8 @@ class KGraph:
         join is a screen (precision BF16 on rows that are not BF16-exact)."""
         p = IndexParams(iterations, L, K, S, R, controls, seed, delta, recall, prune, reverse)
         if precision is not None:
+            if isinstance(precision, str):
+                precision = {"fp32": _lib.PRECISION_FP32, "bf16": _lib.PRECISION_BF16}[precision]
             self._h.call("kg_set_precision", precision)
         if graphs is not None:
             self._h.call("kg_set_build_graphs", 1 if graphs else 0)

@@ -147,6 +147,9 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     if (N <= p.S) p.S = N - 1;
     if (p.S == 0 || p.S > NND_MAX_S) throw invalid_argument("S must be in [1, 255]");
     if (p.L == 0) throw invalid_argument("L must be >= 1");
+    // The S random starting entries fill the first S of a node's L pool slots
+    // (the reference's pool holds L + 1, kgraph.cpp:766, 786).
+    if (p.S > p.L) throw invalid_argument("S must be <= L");
     if (p.K > 1024) throw invalid_argument("K must be <= 1024");
     if (p.prune > 2) throw runtime_error("prune level not supported.");
     if (uint64_t(N) * p.L >= 0xffffffffull) throw invalid_argument("N * L too large");

@@ -28,6 +28,7 @@
 // so integer order is (dist, id) order.  Every stored distance uses the
 // canonical FP32 order of kg_device.hpp.
 #include <algorithm>
+#include <climits>
 
 #include "kg_device.hpp"
 #include "kg_nnd.hpp"
@@ -55,26 +56,49 @@ struct Rng {  // counter-based stream per node
     __device__ uint32_t operator()() { return uint32_t(mix64(s++) >> 32); }
 };
 
-// GenRandom (kgraph.cpp:44-65) on one lane; `addr` in LDS.
-__device__ void gen_random(Rng &rng, uint32_t *addr, unsigned size, unsigned N) {
+// GenRandom (kgraph.cpp:44-65) across one wave.  The node's stream is
+// counter-based (value k is mix64(s0 + k) >> 32), so lane i makes the i-th draw
+// itself: rng() % (N - size) for i < size, then the offset rng() % N.  The
+// one-lane insertion sort becomes a stable rank by counting (equal draws are
+// interchangeable), and its fix-up pass d'[i] = max(d[i], d'[i-1] + 1) equals
+// max_{j<=i}(d[j] + i - j), a running maximum of d[j] - j across the wave.
+// `raw` and `srt` are LDS scratch of `size` entries; out(i, id) receives
+// element i.  Returns the number of stream values used.
+template <class Out>
+__device__ unsigned gen_random_wave(uint64_t s0, uint32_t *raw, uint32_t *srt, unsigned size, unsigned N,
+                                    unsigned lane, Out out) {
     if (N == size) {
-        for (unsigned i = 0; i < size; ++i) addr[i] = i;
-        return;
+        for (unsigned i = lane; i < size; i += 64) out(i, i);
+        return 0;
     }
-    for (unsigned i = 0; i < size; ++i) addr[i] = rng() % (N - size);
-    for (unsigned i = 1; i < size; ++i) {  // insertion sort
-        uint32_t v = addr[i];
-        int j = int(i) - 1;
-        while (j >= 0 && addr[j] > v) {
-            addr[j + 1] = addr[j];
-            --j;
+    unsigned const m = N - size;
+    for (unsigned i = lane; i < size; i += 64) raw[i] = uint32_t(mix64(s0 + i) >> 32) % m;
+    wave_sync();
+    for (unsigned i = lane; i < size; i += 64) {
+        uint32_t const v = raw[i];
+        unsigned r = 0;
+        for (unsigned j = 0; j < size; ++j) {
+            uint32_t const u = raw[j];
+            r += (u < v) | ((u == v) & (j < i));
         }
-        addr[j + 1] = v;
+        srt[r] = v;
     }
-    for (unsigned i = 1; i < size; ++i)
-        if (addr[i] <= addr[i - 1]) addr[i] = addr[i - 1] + 1;
-    unsigned off = rng() % N;
-    for (unsigned i = 0; i < size; ++i) addr[i] = (addr[i] + off) % N;
+    wave_sync();
+    uint64_t const off = uint32_t(mix64(s0 + size) >> 32) % N;
+    long long carry = LLONG_MIN;
+    for (unsigned b = 0; b < size; b += 64) {
+        unsigned const i = b + lane;
+        long long x = (i < size) ? (long long)srt[i] - (long long)i : LLONG_MIN;
+        for (unsigned o = 1; o < 64; o <<= 1) {
+            long long const y = __shfl_up(x, o);
+            if (lane >= o) x = max(x, y);
+        }
+        x = max(x, carry);
+        carry = __shfl(x, 63);
+        if (i < size) out(i, uint32_t((uint64_t(x + i) + off) % N));
+    }
+    wave_sync();
+    return size + 1;
 }
 
 // Canonical metric of data rows i and j (both global), one quad.
@@ -89,78 +113,62 @@ __device__ __forceinline__ float pair_dist(NndArgs const &a, uint32_t i, float c
 // ------------------------------------------------------------------ init
 template <int METRIC>
 __global__ void __launch_bounds__(256) init_kernel(NndArgs a) {
+    __shared__ uint32_t raw[4][2 * NND_MAX_S];
+    __shared__ uint32_t srt[4][2 * NND_MAX_S];
     __shared__ uint32_t rnd[4][NND_MAX_S + 1];
     __shared__ uint32_t pid[4][NND_MAX_S];
-    __shared__ float pd[4][NND_MAX_S];
+    __shared__ uint64_t key[4][NND_MAX_S];
     unsigned const w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     unsigned const n = blockIdx.x * 4 + w;
     if (n >= a.N) return;
     unsigned const S = a.S;
-    if (lane == 0) {
-        Rng rng{mix64(a.seed * 0x100000001b3ull + n)};
-        unsigned nn = a.nnew_init;
-        uint32_t *dst = a.samp + size_t(n) * a.Lc;
-        // nn_new: 2S random ids (kgraph.cpp:765, 781), GenRandom written out
-        // on the node's own sample row.
-        if (a.N == nn) {
-            for (unsigned i = 0; i < nn; ++i) dst[i] = i;
-        } else {
-            for (unsigned i = 0; i < nn; ++i) dst[i] = rng() % (a.N - nn);
-            for (unsigned i = 1; i < nn; ++i) {
-                uint32_t v = dst[i];
-                int j = int(i) - 1;
-                while (j >= 0 && dst[j] > v) {
-                    dst[j + 1] = dst[j];
-                    --j;
-                }
-                dst[j + 1] = v;
-            }
-            for (unsigned i = 1; i < nn; ++i)
-                if (dst[i] <= dst[i - 1]) dst[i] = dst[i - 1] + 1;
-            unsigned off = rng() % a.N;
-            for (unsigned i = 0; i < nn; ++i) dst[i] = (dst[i] + off) % a.N;
-        }
-        // pool: S random non-self ids (kgraph.cpp:782-792)
-        gen_random(rng, rnd[w], S + 1, a.N);
-        unsigned i = 0;
-        for (unsigned l = 0; l < S; ++l) {
-            if (rnd[w][i] == n) ++i;
-            pid[w][l] = rnd[w][i++];
-        }
+    uint64_t s0 = mix64(a.seed * 0x100000001b3ull + n);
+    // nn_new: 2S random ids (kgraph.cpp:765, 781), GenRandom written out on
+    // the node's own sample row.
+    uint32_t *dst = a.samp + size_t(n) * a.Lc;
+    s0 += gen_random_wave(s0, raw[w], srt[w], a.nnew_init, a.N, lane, [&](unsigned i, uint32_t id) { dst[i] = id; });
+    // pool: S random non-self ids (kgraph.cpp:782-792); n occurs at most once
+    // among the S + 1 distinct draws and is skipped.
+    gen_random_wave(s0, raw[w], srt[w], S + 1, a.N, lane, [&](unsigned i, uint32_t id) { rnd[w][i] = id; });
+    wave_sync();
+    unsigned self = S + 1;
+    for (unsigned b = 0; b <= S; b += 64) {
+        uint64_t const hit = __ballot(b + lane <= S && rnd[w][b + lane] == n);
+        if (hit && self > S) self = b + unsigned(__ffsll((unsigned long long)hit)) - 1;
     }
+    for (unsigned l = lane; l < S; l += 64) pid[w][l] = rnd[w][l + (l >= self)];
     wave_sync();
     unsigned const grp = lane >> 2, t = lane & 3;
     float const *rn = a.X + size_t(n) * a.Dp;
     float const nn_ = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[n];
     for (unsigned b = 0; b < S; b += 16) {
         if (b + grp < S) {
-            float d = pair_dist<METRIC>(a, pid[w][b + grp], rn, nn_, t);
-            if (t == 0) pd[w][b + grp] = d;
+            uint32_t const id = pid[w][b + grp];
+            float d = pair_dist<METRIC>(a, id, rn, nn_, t);
+            if (t == 0) key[w][b + grp] = make_key(d, id, 1);
         }
     }
     wave_sync();
+    // sort the S keys by (dist, id) (kgraph.cpp:793): ids are distinct, so a
+    // key's rank is the number of smaller keys.
+    uint64_t *pool = a.pool + size_t(n) * a.L;
+    for (unsigned l = lane; l < S; l += 64) {
+        uint64_t const v = key[w][l];
+        unsigned r = 0;
+        for (unsigned j = 0; j < S; ++j) r += key[w][j] < v;
+        pool[r] = v;
+        if (r == a.L - 1) a.meta[n].x = key_dist(v);
+        if (r == a.K - 1) a.meta[n].y = key_dist(v);
+    }
     if (lane == 0) {
-        uint64_t *pool = a.pool + size_t(n) * a.L;
-        // sort S keys by (dist, id) (kgraph.cpp:793)
-        for (unsigned l = 0; l < S; ++l) pool[l] = make_key(pd[w][l], pid[w][l], 1);
-        for (unsigned l = 1; l < S; ++l) {
-            uint64_t v = pool[l];
-            int j = int(l) - 1;
-            while (j >= 0 && pool[j] > v) {
-                pool[j + 1] = pool[j];
-                --j;
-            }
-            pool[j + 1] = v;
-        }
         a.cnt[n] = S;
         a.Mv[n] = S;
         a.nnew[n] = a.nnew_init;
         a.nold[n] = 0;
         a.crnew[n] = 0;
         a.crold[n] = 0;
-        *reinterpret_cast<float2 *>(&a.meta[n]) =
-            make_float2((S >= a.L) ? key_dist(pool[a.L - 1]) : __builtin_huge_valf(),
-                        (S >= a.K) ? key_dist(pool[a.K - 1]) : __builtin_huge_valf());
+        if (S < a.L) a.meta[n].x = __builtin_huge_valf();
+        if (S < a.K) a.meta[n].y = __builtin_huge_valf();
         a.ccnt[n] = 0;
     }
 }

@@ -170,12 +170,15 @@ def main():
         ems = 0.0
         if mode == "entries":
             h.call("kg_select_entries_device", dQ.ptr, nq, D, m, dI.ptr, K)
-            ems = ctypes_float(h, 7)
         h.call("kg_search_device", dQ.ptr, nq, D, sp, None, None, dI.ptr, dD.ptr, K, dC.ptr, dCo.ptr, dSc.ptr)
         devmem.synchronize()
         wall = time.perf_counter() - ts
+        if mode == "entries":  # read after the walk: reading it earlier waits for the selection
+            ems = ctypes_float(h, 7)
         kms = ctypes_float(h, 0)
-        return dict(mode=mode, P=P, S=S, wall_s=wall, kernel_ms=kms, entry_ms=ems, ids=dI.to_host(),
+        lat = np.zeros(nq, np.float32)  # per-query walk time, stamped inside the kernel
+        h.call("kg_search_latencies", ptr(lat), nq)
+        return dict(mode=mode, P=P, S=S, wall_s=wall, kernel_ms=kms, entry_ms=ems, lat=lat, ids=dI.to_host(),
                     dists=dD.to_host(), comps=dCo.to_host(), scans=dSc.to_host(),
                     seeds=(m if mode == "entries" else P))
 
@@ -237,6 +240,17 @@ def main():
     ok = [s for s in sweep if s["recall"] >= 0.95]
     pick = max(ok, key=lambda s: s["qps"]) if ok else max(sweep, key=lambda s: s["recall"])
     achieved = pick["bytes"] / (pick["kernel_ms"] * 1e-3) / 1e9
+    lat = res[sweep.index(pick)]["lat"]
+    wall_ms = 1e3 * nq / pick["qps"]
+    # where the picked setting's wall time goes: the two kernels (HIP events),
+    # the rest (launches, the host's waits); and the walk's per-query times
+    search_time = {"wall_ms": round(wall_ms, 4), "walk_kernel_ms": round(pick["kernel_ms"], 4),
+                   "entry_kernel_ms": round(pick["entry_ms"], 4),
+                   "other_ms": round(wall_ms - pick["kernel_ms"] - pick["entry_ms"], 4),
+                   "walk_query_us": {"p50": round(float(np.percentile(lat, 50)), 1),
+                                     "p99": round(float(np.percentile(lat, 99)), 1),
+                                     "max": round(float(lat.max()), 1)}}
+    log("picked setting: %s" % search_time)
     # the picked setting again with the walk gathering the FP32 rows instead of
     # the lossless BF16 copy (same results)
     row_cmp = None
@@ -294,6 +308,7 @@ def main():
         "roofline": {"kernel": "search_kernel", "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": measured_traffic("search_kernel", "intmix-1m-d128", pick, rows)},
+        "search_time": search_time,
         "walk_row_storage": row_cmp,
         "roofline_join": {"kernel": "join_mfma_kernel", "bound": "hbm", "achieved": round(join_gbs, 1),
                           "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(join_gbs / PEAK_HBM_GBS, 4),
@@ -541,10 +556,10 @@ def precision_workload(args):
                 devmem.synchronize()
                 ts = time.perf_counter()
                 h.call("kg_select_entries_device", dQ.ptr, nq, D, m, dI.ptr, K)
-                ems = ctypes_float(h, 7)
                 h.call("kg_search_device", dQ.ptr, nq, D, sp, None, None, dI.ptr, dD.ptr, K, dC.ptr, None, None)
                 devmem.synchronize()
                 wall = time.perf_counter() - ts
+                ems = ctypes_float(h, 7)
             rec = recall_at_k(dD.to_host(), gt_d, K)
             sweep.append({"P": P, "S": 32, "qps": round(nq / wall, 1), "recall": round(rec, 4),
                           "entry_ms": round(ems, 3), "walk_ms": round(ctypes_float(h, 0), 3)})
@@ -640,11 +655,11 @@ def learned_workload(args):
                 devmem.synchronize()
                 ts = time.perf_counter()
                 g._h.call("kg_select_entries_device", dQ.ptr, nq, 128, m, dI.ptr, K)
-                ems = ctypes_float(g._h, 7)
                 g._h.call("kg_search_device", dQ.ptr, nq, 128, sp, None, None, dI.ptr, dD.ptr, K, dC.ptr, None,
                           None)
                 devmem.synchronize()
                 wall = time.perf_counter() - ts
+                ems = ctypes_float(g._h, 7)
             rec = recall_at_k(dD.to_host(), gt_d, K)
             rows.append({"form": form, "P": P, "S": 32, "qps": round(nq / wall, 1), "recall": round(rec, 4),
                          "entry_ms": round(ems, 3), "walk_ms": round(ctypes_float(g._h, 0), 3)})

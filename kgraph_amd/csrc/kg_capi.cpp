@@ -23,12 +23,14 @@ Handle::Handle() {
     lds_per_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 65536;
     KG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (auto &e : ev) KG_HIP(hipEventCreate(&e));
+    KG_HIP(hipHostMalloc(reinterpret_cast<void **>(&host_flag), 16, hipHostMallocDefault));
 }
 
 Handle::~Handle() {
     for (auto &e : ev)
         if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
+    if (host_flag) (void)hipHostFree(host_flag);
 }
 
 void Handle::pull_graph() {
@@ -127,13 +129,20 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
     if (sp.init && !Ks && !(sp.init < sp.K)) throw invalid_argument("L < params.K");
     if (sp.init > ostride) throw invalid_argument("init larger than the id row stride");
     if (sp.T == 0) throw invalid_argument("T must be >= 1");
-    if (sp.init) {  // caller's starting points must be valid node ids
-        std::vector<uint32_t> ini(size_t(nq) * ostride);
+    // caller's starting points must be valid node ids: checked on the device
+    // (qcounter word 2), before the walk, which exits at once when one is not;
+    // the call throws once the stream has drained
+    h.qcounter.reserve(3);  // [0] next query, [1] queries whose visited set spilled, [2] bad starting id
+    KG_HIP(hipMemsetAsync(h.qcounter.p, 0, 12, h.stream));
+    uint32_t *const bad_init = h.qcounter.p + 2;
+    // the check kernel also raises a pinned host word, read once the stream drains
+    auto check_init = [&]() {
         h.sync();
-        KG_HIP(hipMemcpy(ini.data(), d_ids, 4 * ini.size(), hipMemcpyDeviceToHost));
-        for (unsigned q = 0; q < nq; ++q)
-            for (unsigned i = 0; i < sp.init; ++i)
-                if (ini[size_t(q) * ostride + i] >= h.gN) throw invalid_argument("initial id out of range");
+        if (__atomic_load_n(h.host_flag, __ATOMIC_ACQUIRE)) throw invalid_argument("initial id out of range");
+    };
+    if (sp.init) {
+        __atomic_store_n(h.host_flag, 0u, __ATOMIC_RELEASE);
+        KG_HIP(kg_launch_init_check(d_ids, nq, ostride, sp.init, h.gN, bad_init, h.host_flag, h.stream));
     }
     if (sp.S == 0) throw invalid_argument("S must be >= 1");
     unsigned const N = h.gN;
@@ -179,8 +188,10 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
     if (seeds_h.empty()) seeds_h.push_back(0);
     h.seeds.reserve(seeds_h.size());
     h.seed_off.reserve(soff_h.size());
-    KG_HIP(hipMemcpyAsync(h.seeds.p, seeds_h.data(), 4 * seeds_h.size(), hipMemcpyHostToDevice, h.stream));
-    KG_HIP(hipMemcpyAsync(h.seed_off.p, soff_h.data(), 4 * soff_h.size(), hipMemcpyHostToDevice, h.stream));
+    if (sp.init == 0) {  // the walk reads no seeds when it starts from ids
+        KG_HIP(hipMemcpyAsync(h.seeds.p, seeds_h.data(), 4 * seeds_h.size(), hipMemcpyHostToDevice, h.stream));
+        KG_HIP(hipMemcpyAsync(h.seed_off.p, soff_h.data(), 4 * soff_h.size(), hipMemcpyHostToDevice, h.stream));
+    }
     if (Ks) {
         h.Ks.reserve(nq);
         KG_HIP(hipMemcpyAsync(h.Ks.p, Ks, 4ull * nq, hipMemcpyHostToDevice, h.stream));
@@ -257,8 +268,6 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
             h.lists_unique = dup == 0;
             h.gnode_valid = true;
         }
-        h.qcounter.reserve(2);  // [0] next query, [1] queries whose visited set spilled
-        KG_HIP(hipMemsetAsync(h.qcounter.p, 0, 8, h.stream));
         kg_search_args a{};
         a.X = h.X.p;
         a.Xb = (h.rows_bf16 && h.row_storage == 0) ? h.Xb.p : nullptr;
@@ -295,7 +304,8 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         a.comps = d_comps;
         a.scans = d_scans;
         h.lat.reserve(nq);
-        KG_HIP(hipMemsetAsync(h.lat.p, 0, 4ull * nq, h.stream));
+        // the walk writes every graph query's latency; brute-force ones read 0
+        if (!brute.empty()) KG_HIP(hipMemsetAsync(h.lat.p, 0, 4ull * nq, h.stream));
         a.lat_us = h.lat.p;
         h.lat_n = nq;
         a.visited = h.visited.p;
@@ -308,6 +318,7 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         a.lds_per_wave = unsigned(lpw);
         a.vcap = vcap;
         a.spills = h.qcounter.p + 1;
+        a.abort_flag = sp.init ? bad_init : nullptr;
         h.last_vcap = vcap;
         a.seed_cap = seed_cap;
         KG_HIP(hipEventRecord(h.ev[0], h.stream));
@@ -316,6 +327,7 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
     }
     // brute-force queries (P >= N, kgraph.cpp:427-433)
     if (!brute.empty()) {
+        if (sp.init) check_init();  // nothing is written after a bad starting id
         if (Ks) {
             for (uint32_t q : brute)
                 if (Ks[q] != Ks[brute[0]]) throw invalid_argument("brute-force queries need one K");
@@ -350,7 +362,10 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         KG_HIP(hipEventRecord(h.ev[3], h.stream));
         h.sync();
     }
-    h.sync();
+    if (sp.init)
+        check_init();
+    else
+        h.sync();
     if (ngraph) KG_HIP(hipEventElapsedTime(&h.last_ms[0], h.ev[0], h.ev[1]));
     if (!brute.empty()) KG_HIP(hipEventElapsedTime(&h.last_ms[1], h.ev[2], h.ev[3]));
 }
@@ -385,7 +400,7 @@ static void set_entry_sample(Handle &h, unsigned E, unsigned seed) {
     KG_HIP(kg_launch_norms(h.metric == KG_METRIC_ANGULAR ? KG_METRIC_ANGULAR : KG_METRIC_EUCLIDEAN, h.erows.p, E, h.D,
                            h.Dp, h.escale.p, h.stream));
     // integer rows (and their largest |x|) allow the exact MFMA selection
-    h.iflags.reserve(2);
+    h.iflags.reserve(3);
     KG_HIP(hipMemsetAsync(h.iflags.p, 0, 8, h.stream));
     KG_HIP(kg_launch_intcheck(h.erows.p, E, h.D, h.Dp, h.iflags.p, h.stream));
     uint32_t fl[2];
@@ -400,16 +415,20 @@ static void set_entry_sample(Handle &h, unsigned E, unsigned seed) {
 // BF16 tiles give the canonical FP32 distances exactly when every row and
 // query value is an integer of magnitude <= 256 and D (max|q|^2 + max|e|^2)
 // <= 2^24: every product, partial sum and norm is then an integer <= 2^24.
-static bool entries_exact_on_mfma(Handle &h, float const *Qk, unsigned nq, size_t qs) {
+// The test runs with the query norms ||q||^2 / 2 (into h.qsq), which the MFMA
+// selection needs either way; *qsq_done tells whether they were computed.
+static bool entries_exact_on_mfma(Handle &h, float const *Qk, unsigned nq, size_t qs, bool *qsq_done) {
+    *qsq_done = false;
     if (!h.entry_int || h.D > 256 || h.entry_max > 256.f) return false;
-    KG_HIP(hipMemsetAsync(h.iflags.p, 0, 8, h.stream));
-    KG_HIP(kg_launch_intcheck(Qk, nq, h.D, unsigned(qs), h.iflags.p, h.stream));
-    uint32_t fl[2];
-    KG_HIP(hipMemcpyAsync(fl, h.iflags.p, 8, hipMemcpyDeviceToHost, h.stream));
+    KG_HIP(hipMemsetAsync(h.iflags.p, 0, 12, h.stream));
+    uint32_t *const fl = h.host_flag + 1;  // pinned words 1..2
+    KG_HIP(kg_launch_qnorm_check(Qk, nq, h.D, unsigned(qs), h.qsq.p, h.iflags.p, fl, h.stream));
     h.sync();
+    *qsq_done = true;
     float qmax;
-    memcpy(&qmax, &fl[1], 4);
-    return fl[0] == 0 && qmax <= 256.f &&
+    uint32_t const f0 = __atomic_load_n(fl, __ATOMIC_ACQUIRE), f1 = __atomic_load_n(fl + 1, __ATOMIC_ACQUIRE);
+    memcpy(&qmax, &f1, 4);
+    return f0 == 0 && qmax <= 256.f &&
            double(h.D) * (double(qmax) * qmax + double(h.entry_max) * h.entry_max) <= 16777216.0;
 }
 
@@ -439,7 +458,10 @@ static void select_entries_device(Handle &h, float const *dQ, unsigned nq, size_
     //     then the s re-scored exactly;
     //   otherwise: canonical FP32 scores of all E, then the top-m.
     bool const learned = h.entry_shortlist > 0;
-    bool const exact_mfma = !learned && entries_exact_on_mfma(h, Qk, nq, qs);
+    h.qsq.reserve(std::max(size_t(nq), size_t(h.entryE)));
+    KG_HIP(hipEventRecord(h.ev[14], h.stream));
+    bool qsq_done = false;
+    bool const exact_mfma = !learned && entries_exact_on_mfma(h, Qk, nq, qs, &qsq_done);
     bool const screen = !learned && !exact_mfma && h.precision == KG_PRECISION_BF16 && h.D <= 256;
     unsigned const sl = std::max(m, std::min(learned ? h.entry_shortlist : 16u, std::min(16u, h.entryE)));
     if (learned) {
@@ -462,10 +484,7 @@ static void select_entries_device(Handle &h, float const *dQ, unsigned nq, size_
     a.m = m;
     a.out_ids = d_ids;
     a.ostride = unsigned(ostride);
-    KG_HIP(hipEventRecord(h.ev[14], h.stream));
-    h.qsq.reserve(std::max(size_t(nq), size_t(h.entryE)));
     if (exact_mfma) {
-        KG_HIP(kg_launch_norms(KG_METRIC_EUCLIDEAN, Qk, nq, h.D, unsigned(qs), h.qsq.p, h.stream));
         KG_HIP(kg_launch_entries_exact_mfma(h.metric, a, h.qsq.p, h.escale.p, h.stream));
     } else if (learned || screen) {
         h.eshort.reserve(size_t(nq) * sl);
@@ -481,7 +500,7 @@ static void select_entries_device(Handle &h, float const *dQ, unsigned nq, size_
             KG_HIP(kg_launch_norms(KG_METRIC_EUCLIDEAN, h.zq.p, nq, h.mlpP, h.mlpP, h.qsq.p, h.stream));
             KG_HIP(kg_launch_entries_exact_mfma(KG_METRIC_L2SQR, a, h.qsq.p, h.eprojsq.p, h.stream));
         } else {
-            KG_HIP(kg_launch_norms(KG_METRIC_EUCLIDEAN, Qk, nq, h.D, unsigned(qs), h.qsq.p, h.stream));
+            if (!qsq_done) KG_HIP(kg_launch_norms(KG_METRIC_EUCLIDEAN, Qk, nq, h.D, unsigned(qs), h.qsq.p, h.stream));
             KG_HIP(kg_launch_entries_exact_mfma(h.metric, a, h.qsq.p, h.escale.p, h.stream));
         }
         kg_rescore_args r{};
@@ -505,8 +524,7 @@ static void select_entries_device(Handle &h, float const *dQ, unsigned nq, size_
         KG_HIP(kg_launch_entries(h.metric, a, h.stream));
     }
     KG_HIP(hipEventRecord(h.ev[15], h.stream));
-    h.sync();
-    KG_HIP(hipEventElapsedTime(&h.last_ms[7], h.ev[14], h.ev[15]));
+    h.entry_ms_pending = true;  // read by kg_last_time_ms(7) once the selection is done
 }
 
 // KNNGRAPH v2 (kgraph.cpp:272-274, 291-373).
@@ -887,6 +905,11 @@ int kg_get_graph(kg_handle *k, uint64_t *offsets, unsigned *ids, float *dists, u
 int kg_last_time_ms(kg_handle *k, int which, float *ms) {
     return guard(k, [&](Handle &h) {
         if (which < 0 || which >= 9 || !ms) throw kga::invalid_argument("bad timer");
+        if (which == 7 && h.entry_ms_pending) {
+            KG_HIP(hipEventSynchronize(h.ev[15]));
+            KG_HIP(hipEventElapsedTime(&h.last_ms[7], h.ev[14], h.ev[15]));
+            h.entry_ms_pending = false;
+        }
         *ms = h.last_ms[which];
     });
 }
@@ -934,6 +957,7 @@ int kg_select_entries(kg_handle *k, float const *Q, unsigned nq, size_t qstride,
         di.reserve(size_t(nq) * ostride);
         KG_HIP(hipMemcpyAsync(di.p, ids, size_t(nq) * ostride * 4, hipMemcpyHostToDevice, h.stream));
         select_entries_device(h, dq.p, nq, h.Dp, m, di.p, ostride);
+        h.sync();
         KG_HIP(hipMemcpy(ids, di.p, size_t(nq) * ostride * 4, hipMemcpyDeviceToHost));
     });
 }

@@ -201,6 +201,45 @@ __global__ void __launch_bounds__(256) intcheck_kernel(float const *X, unsigned 
         atomicMax(flags + 1, __float_as_uint(mx));
 }
 
+// Query norms ||q||^2 / 2 (canonical order) for the MFMA selection, fused with
+// intcheck_kernel's test of the same rows: flags[0] |= 1 on a non-integer,
+// flags[1] = max |q| as float bits, flags[2] counts finished blocks (all three
+// zeroed by the caller).  The last block copies flags[0..1] to host_out
+// (pinned host memory), so the caller reads them after one stream
+// synchronisation, without a copy.
+__global__ void __launch_bounds__(256) qnorm_check_kernel(float const *Q, unsigned n, unsigned D, unsigned stride,
+                                                          float *qsq, uint32_t *flags, uint32_t *host_out) {
+    unsigned const lane = threadIdx.x & 63, t = lane & 3;
+    unsigned const row = (blockIdx.x * 256 + threadIdx.x) >> 2;
+    bool frac = false;
+    float mx = 0.f;
+    if (row < n) {  // quad-uniform: the quad reduce inside quad_metric sees all four lanes
+        float const *r = Q + size_t(row) * stride;
+        float const n2 = quad_metric<METRIC_EUCLIDEAN>(r, r, D, t);
+        if (t == 0) qsq[row] = finish_norm<METRIC_EUCLIDEAN>(n2);
+        for (unsigned c = t; c < D; c += 4) {
+            float const v = r[c];
+            frac |= !(v == rintf(v));
+            mx = fmaxf(mx, fabsf(v));
+        }
+    }
+    uint64_t const fr = ballot(frac);
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    if (lane == 0) {
+        if (fr) atomicOr(flags, 1u);
+        atomicMax(flags + 1, __float_as_uint(mx));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(flags + 2, 1u) == gridDim.x - 1) {
+            __threadfence();
+            host_out[0] = __hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            host_out[1] = __hip_atomic_load(flags + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
 // Entry selection on the matrix cores.  For rows and queries whose BF16 dot
 // products are exact (integers, |x| <= 256, D (max|q|^2 + max|e|^2) <= 2^24)
 // every product, partial sum and norm is an integer of at most 2^24 and the
@@ -402,6 +441,14 @@ __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float
 }
 
 }  // namespace
+
+hipError_t kg_launch_qnorm_check(float const *Q, unsigned n, unsigned D, unsigned stride, float *qsq,
+                                 uint32_t *flags, uint32_t *host_out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(qnorm_check_kernel, dim3((size_t(n) * 4 + 255) / 256), dim3(256), 0, s, Q, n, D, stride, qsq,
+                       flags, host_out);
+    return hipGetLastError();
+}
 
 hipError_t kg_launch_intcheck(float const *X, unsigned n, unsigned D, unsigned stride, uint32_t *flags,
                               hipStream_t s) {

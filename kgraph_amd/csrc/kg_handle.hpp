@@ -58,6 +58,7 @@ struct Handle {
     std::string err;
     hipStream_t stream = nullptr;
     int device = 0;
+    uint32_t *host_flag = nullptr;  // 4 pinned words for flags read back with the stream's work
     int num_cus = 0;
     size_t lds_per_cu = 0;
     int precision = KG_PRECISION_FP32;
@@ -129,6 +130,7 @@ struct Handle {
     // timing + build trace
     hipEvent_t ev[20] = {};
     float last_ms[9] = {};
+    bool entry_ms_pending = false;  // last_ms[7] waits on ev[14..15] of a selection still in flight
     std::vector<float> trace;
     uint64_t build_counters[5] = {};  // pairs, list entries, candidate pushes, iterations, graph replays
     std::vector<uint64_t> iter_stats;  // per iteration: pairs, pushes, list entries, join form, join us, iteration us

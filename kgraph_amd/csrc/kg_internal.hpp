@@ -45,6 +45,9 @@ struct kg_search_args {
     uint32_t *touched;
     unsigned touch_cap;
     uint32_t *qcounter;
+    // set by kg_launch_init_check when a starting id is out of range: every
+    // wave then exits at once (may be NULL)
+    uint32_t const *abort_flag;
     unsigned pool_cap, res_cap, lds_per_wave;
     unsigned seed_cap;  // starting points a query sorts: max P (random) or init
 };
@@ -107,6 +110,11 @@ hipError_t kg_launch_gnode(unsigned N, uint32_t const *goff, uint32_t const *gid
                            uint32_t *dupflag, hipStream_t s);
 hipError_t kg_launch_search(int metric, kg_search_args const &a, unsigned blocks, unsigned waves_per_block,
                             hipStream_t s);
+// *flag = *host_flag = 1 if one of the first `init` ids of a row (row q at
+// ids + q*ostride) is >= N (host_flag: pinned host memory); both are left
+// alone otherwise.
+hipError_t kg_launch_init_check(uint32_t const *ids, unsigned nq, size_t ostride, unsigned init, unsigned N,
+                                uint32_t *flag, uint32_t *host_flag, hipStream_t s);
 hipError_t kg_launch_brute(int metric, kg_brute_args const &a, hipStream_t s);
 unsigned kg_brute_splits(unsigned nq, unsigned N, unsigned K);
 hipError_t kg_launch_entries(int metric, kg_entry_args const &a, hipStream_t s);
@@ -121,6 +129,11 @@ hipError_t kg_launch_entries_exact_mfma(int metric, kg_entry_args const &a, floa
 // integer; flags[1] = max |value| as float bits.  Caller zeroes flags.
 hipError_t kg_launch_intcheck(float const *X, unsigned n, unsigned D, unsigned stride, uint32_t *flags,
                               hipStream_t s);
+// qsq[q] = ||q||^2 / 2 (canonical order) and the kg_launch_intcheck flags of
+// the same rows in one pass: flags[0..2] zeroed by the caller (flags[2]
+// counts blocks); the last block copies flags[0..1] to host_out (pinned).
+hipError_t kg_launch_qnorm_check(float const *Q, unsigned n, unsigned D, unsigned stride, float *qsq,
+                                 uint32_t *flags, uint32_t *host_out, hipStream_t s);
 hipError_t kg_launch_gather_rows(float const *X, unsigned Dp, uint32_t const *ids, unsigned E, float *out,
                                  hipStream_t s);
 // BF16 copy of N x Dp rows (Dp % 4 == 0); *inexact = 1 when a value is not

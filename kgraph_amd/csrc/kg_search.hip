@@ -318,6 +318,7 @@ search_kernel(kg_search_args a) {
     uint32_t *vis = a.visited + size_t(slot) * a.vis_words;
     uint32_t *touched = a.touched + size_t(slot) * a.touch_cap;
     uint64_t const lt_mask = (1ull << lane) - 1;
+    if (a.abort_flag && *a.abort_flag) return;  // a starting id is out of range: nothing is written
 
     for (;;) {
         unsigned qi = 0;
@@ -602,7 +603,32 @@ __global__ void __launch_bounds__(256) gnode_kernel(unsigned N, uint32_t const *
     if (ballot(dup) && lane == 0) atomicOr(dupflag, 1u);
 }
 
+__global__ void __launch_bounds__(256) init_check_kernel(uint32_t const *ids, unsigned nq, size_t ostride,
+                                                         unsigned init, unsigned N, uint32_t *flag,
+                                                         uint32_t *host_flag) {
+    size_t const total = size_t(nq) * init;
+    bool bad = false;
+    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < total; i += size_t(gridDim.x) * 256) {
+        size_t const q = i / init, j = i - q * init;
+        bad |= ids[q * ostride + j] >= N;
+    }
+    if (bad) {
+        *flag = 1u;
+        *host_flag = 1u;
+    }
+}
+
 }  // namespace
+
+hipError_t kg_launch_init_check(uint32_t const *ids, unsigned nq, size_t ostride, unsigned init, unsigned N,
+                                uint32_t *flag, uint32_t *host_flag, hipStream_t s) {
+    size_t const total = size_t(nq) * init;
+    if (total == 0) return hipSuccess;
+    unsigned const blocks = unsigned(std::min<size_t>((total + 255) / 256, 4096));
+    hipLaunchKernelGGL(init_check_kernel, dim3(blocks), dim3(256), 0, s, ids, nq, ostride, init, N, flag,
+                       host_flag);
+    return hipGetLastError();
+}
 
 hipError_t kg_launch_gnode(unsigned N, uint32_t const *goff, uint32_t const *gids, uint32_t const *gM, uint4 *gnode,
                            uint32_t *dupflag, hipStream_t s) {

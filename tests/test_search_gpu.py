@@ -207,6 +207,22 @@ def test_search_init_starting_points():
     g._h.call("kg_search", _lib.ptr(Qc), len(Q), Qc.shape[1], sp, None, None, _lib.ptr(gids), _lib.ptr(gd), 10,
               _lib.ptr(gc), None, None)
     assert np.array_equal(gc, c) and np.array_equal(gids, ids) and np.array_equal(gd, d)
+    # an out-of-range starting id (checked on the device before the walk)
+    # raises and writes nothing; the next call starts clean
+    bad = np.zeros((len(Q), 10), dtype=np.uint32)
+    bad[:, :5] = init
+    bad[len(Q) // 2, 3] = len(X)
+    bd = np.full((len(Q), 10), -1.0, dtype=np.float32)
+    bc = np.full(len(Q), 7, dtype=np.uint32)
+    with pytest.raises(ValueError, match="initial id out of range"):
+        g._h.call("kg_search", _lib.ptr(Qc), len(Q), Qc.shape[1], sp, None, None, _lib.ptr(bad), _lib.ptr(bd), 10,
+                  _lib.ptr(bc), None, None)
+    assert (bd == -1.0).all() and (bc == 7).all() and bad[len(Q) // 2, 3] == len(X)
+    gids[:] = 0
+    gids[:, :5] = init
+    g._h.call("kg_search", _lib.ptr(Qc), len(Q), Qc.shape[1], sp, None, None, _lib.ptr(gids), _lib.ptr(gd), 10,
+              _lib.ptr(gc), None, None)
+    assert np.array_equal(gc, c) and np.array_equal(gids, ids) and np.array_equal(gd, d)
 
 
 def test_bruteforce_parity():
@@ -264,6 +280,23 @@ def test_entry_selection_parity(kind, metric, d, m, E):
     assert np.array_equal(g.last_counts, oc)
     assert np.array_equal(gi, oi) and np.array_equal(gd, od)
     assert np.array_equal(g.last_comps, ocomp) and np.array_equal(g.last_scans, oscan)
+
+
+def test_entry_selection_query_forms_in_turn():
+    # integer rows: integer queries take the exact MFMA selection, others (a
+    # fraction, or values past 256) the canonical FP32 scan; one handle serves
+    # them in turn, and each selection equals the exact one
+    X, Q, idx = setup("intmix", L2, d=128)
+    g = gpu_index(X, idx, L2)
+    E, m = 512, 7
+    g.set_entry_sample(E, seed=77)
+    samp = _entry_sample_ids(77, E, len(X))
+    big = Q.copy()
+    big[:, 5] = 300.0
+    for Qv in (Q, Q + 0.25, Q, big, Q):
+        got = g.select_entries(Qv, m)
+        si, _, _ = kgo.bruteforce(X[samp], Qv, m, L2)
+        assert np.array_equal(got, samp[si])
 
 
 @pytest.mark.parametrize("kind,metric,d", [("angular", ANG, 96), ("uniform", EUC, 16)])

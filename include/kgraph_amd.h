@@ -105,16 +105,24 @@ int kg_set_precision(kg_handle *h, int precision);
 
 /* Visited set of the graph walk (no counterpart in the reference, which
  * allocates an N-bit bitset per query, kgraph.cpp:438).  Mode 0 (default):
- * each resident wave keeps an exact hash set of visited ids in LDS (the
- * largest power of two up to 4096 entries that keeps 16 waves per CU, or
- * 2048 entries with 12 waves when 16 waves leave less) in front of its N-bit
- * HBM bitset; a query that fills it past half moves it
- * into the bitset and continues there.  Mode 1: the bitset only.  Results are
- * identical either way. */
+ * each resident wave keeps an exact hash set of visited ids in LDS in front of
+ * its N-bit HBM bitset; a query that fills it past half moves it into the
+ * bitset and continues there.  Entries take 16 bits when ceil(log2 N) <=
+ * log2(entries) + 10 (the hashed id's low bits and the probe displacement,
+ * from which slot and entry give the id back), else 32.  Size: 4096 entries,
+ * else 2048, when that keeps at least 12 waves per CU resident; else the
+ * largest power of two up to 4096 that keeps 16.  Mode 1: the
+ * bitset only.  Mode 2: as mode 0 with 32-bit entries.
+ * kg_set_search_visited_size fixes the entry count (a power of two in
+ * [256, 8192]; 0 restores the choice above).  Results are identical in every
+ * form. */
 int kg_set_search_visited(kg_handle *h, int mode);
+int kg_set_search_visited_size(kg_handle *h, unsigned entries);
 /* Last walk launch: LDS set entries per wave (0: bitset only) and how many
- * queries moved their set into the bitset. */
+ * queries moved their set into the bitset; kg_search_visited_bits: the width
+ * of its entries (16 or 32). */
 int kg_search_visited_stats(kg_handle *h, unsigned *vcap, unsigned *spilled);
+int kg_search_visited_bits(kg_handle *h, unsigned *bits);
 
 /* Row storage for graph walks (no counterpart in the reference, whose rows are
  * the caller's Matrix, kgraph-data.h:115-175).  kg_set_data keeps a BF16 copy

@@ -220,28 +220,42 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
     if (ngraph) {
         // seeds sorted per trial: init ids, or P random ones (largest P of the batch)
         unsigned const seed_cap = sp.init ? sp.init : std::max(poolmax, 1u);
-        size_t lpw = kg_search_lds_per_wave(poolmax, kmax + 1, h.Dp, seed_cap, 0);
-        // visited hash set in LDS: the largest power of two (256..4096 entries)
-        // that keeps 16 waves per CU resident (the kernel's register
-        // occupancy); when that is under 2048 entries and 2048 fit with 12
-        // waves, 2048 at 12 waves (a query visits ~1,000 ids up to P = 100
-        // and spills past half: an HBM visited test costs more than the lost
-        // quarter of the waves, a set that spills does not)
+        // visited hash set in LDS, in front of the HBM bitset.  Entries are
+        // 16 bits when the ids allow (ceil(log2 N) <= log2(entries) + 10),
+        // else 32.  Size: 4096 entries, else 2048, when that keeps at least 12
+        // waves per CU resident (a query visits ~850 ids at P = 10 and ~1,000
+        // at P = 100 and spills past half: an HBM visited test costs more
+        // than a quarter of the waves); else the largest power of two
+        // (256..4096) that keeps 16 waves
+        unsigned idbits = 1;
+        while (idbits < 32 && (uint64_t(1) << idbits) < N) ++idbits;
+        auto bits_for = [&](unsigned c) {
+            unsigned const t = unsigned(__builtin_ctz(c));
+            return (h.search_visited == 0 && idbits <= t + 10) ? 16u : 32u;
+        };
+        auto lds_for = [&](unsigned c) {
+            return kg_search_lds_per_wave(poolmax, kmax + 1, h.Dp, seed_cap, c, c ? bits_for(c) : 32u);
+        };
+        auto waves = [&](unsigned c) { return std::min<size_t>(kg_search_waves_per_cu(), h.lds_per_cu / lds_for(c)); };
         unsigned vcap = 0;
-        if (h.search_visited == 0) {
-            auto waves = [&](unsigned c) {
-                return h.lds_per_cu / kg_search_lds_per_wave(poolmax, kmax + 1, h.Dp, seed_cap, c);
-            };
-            for (unsigned c = 4096; c >= 256 && !vcap; c /= 2)
-                if (waves(c) >= 16) vcap = c;
-            if (vcap < 2048 && waves(2048) >= 12) vcap = 2048;
+        if (h.search_visited != 1) {
+            if (h.search_vcap) {
+                vcap = h.search_vcap;
+            } else {
+                for (unsigned c : {4096u, 2048u})
+                    if (!vcap && waves(c) >= 12) vcap = c;
+                for (unsigned c = 4096; c >= 256 && !vcap; c /= 2)
+                    if (waves(c) >= 16) vcap = c;
+            }
         }
-        lpw = kg_search_lds_per_wave(poolmax, kmax + 1, h.Dp, seed_cap, vcap);
+        unsigned const vt_bits = vcap ? bits_for(vcap) : 32u;
+        size_t const lpw = lds_for(vcap);
         unsigned wpb = std::min(h.search_wpb, kg_search_max_wpb());  // the kernel's launch bound
         while (wpb > 1 && lpw * wpb > 65536) wpb /= 2;
         size_t lpb = lpw * wpb;
         if (lpb > 65536) throw invalid_argument("K + P too large for the search kernel's LDS budget");
-        unsigned bpc = unsigned(std::max<size_t>(1, std::min<size_t>(32 / wpb, h.lds_per_cu / lpb)));
+        unsigned bpc =
+            unsigned(std::max<size_t>(1, std::min<size_t>(kg_search_waves_per_cu() / wpb, h.lds_per_cu / lpb)));
         if (h.search_bpc) bpc = std::min(bpc, h.search_bpc);
         unsigned blocks = std::min<unsigned>((ngraph + wpb - 1) / wpb, h.num_cus * bpc);
         unsigned words = (N + 31) / 32;
@@ -318,6 +332,10 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         a.lds_per_wave = unsigned(lpw);
         a.vcap = vcap;
         a.spills = h.qcounter.p + 1;
+        a.vt_bits = vt_bits;
+        a.vt_rb = vt_bits == 16 ? std::max(idbits, unsigned(__builtin_ctz(vcap))) - unsigned(__builtin_ctz(vcap)) : 0u;
+        a.vt_idmask = vt_bits == 16 ? uint32_t((uint64_t(1) << (a.vt_rb + __builtin_ctz(vcap))) - 1) : 0u;
+        h.last_vt_bits = vt_bits;
         a.abort_flag = sp.init ? bad_init : nullptr;
         h.last_vcap = vcap;
         a.seed_cap = seed_cap;
@@ -694,8 +712,23 @@ int kg_set_precision(kg_handle *k, int precision) {
 
 int kg_set_search_visited(kg_handle *k, int mode) {
     return guard(k, [&](Handle &h) {
-        if (mode != 0 && mode != 1) throw kga::invalid_argument("visited-set mode must be 0 or 1");
+        if (mode < 0 || mode > 2) throw kga::invalid_argument("visited-set mode must be 0, 1 or 2");
         h.search_visited = mode;
+    });
+}
+
+int kg_set_search_visited_size(kg_handle *k, unsigned entries) {
+    return guard(k, [&](Handle &h) {
+        if (entries && (entries < 256 || entries > 8192 || (entries & (entries - 1))))
+            throw kga::invalid_argument("visited-set size must be 0 or a power of two in [256, 8192]");
+        h.search_vcap = entries;
+    });
+}
+
+int kg_search_visited_bits(kg_handle *k, unsigned *bits) {
+    return guard(k, [&](Handle &h) {
+        if (!bits) throw kga::invalid_argument("null argument");
+        *bits = h.last_vt_bits;
     });
 }
 

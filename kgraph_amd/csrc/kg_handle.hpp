@@ -78,7 +78,9 @@ struct Handle {
     DevBuf<uint16_t> Xb;
     bool rows_bf16 = false;
     int row_storage = 0;  // 0: walks gather the lossless BF16 copy when there is one, 1: FP32 rows
-    int search_visited = 0;  // 0: LDS visited hash set in front of the HBM bitset, 1: bitset only
+    int search_visited = 0;  // 0: LDS visited hash set in front of the HBM bitset, 1: bitset only, 2: 32-bit set entries
+    unsigned search_vcap = 0;  // LDS set entries (0: chosen per launch)
+    unsigned last_vt_bits = 0;
     unsigned last_vcap = 0;  // LDS visited-set entries of the last walk launch
     bool has_data = false;
 

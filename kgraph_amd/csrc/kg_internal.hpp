@@ -41,6 +41,10 @@ struct kg_search_args {
     uint32_t *visited;
     unsigned vis_words;
     unsigned vcap;  // entries of the per-wave LDS visited set (power of two >= 256, or 0: bitset only)
+    // entry width of that set: 32 (the id), or 16 for ids below 2^idbits
+    // (idbits <= log2(vcap) + 10): the low vt_rb = idbits - log2(vcap) bits of
+    // the hashed id and the probe displacement; vt_idmask = 2^idbits - 1
+    unsigned vt_bits, vt_rb, vt_idmask;
     uint32_t *spills;  // counts queries whose LDS set moved into the bitset (may be NULL)
     uint32_t *touched;
     unsigned touch_cap;
@@ -101,10 +105,13 @@ struct kg_rescore_args {
 };
 
 // Launchers (defined in the .hip files).
-size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp, unsigned seed_cap, unsigned vcap);
+size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp, unsigned seed_cap, unsigned vcap,
+                              unsigned vt_bits);
 // Largest waves per block search_kernel is compiled for (its launch bound);
 // kg_launch_search rejects larger shapes.
 unsigned kg_search_max_wpb();
+// Waves per CU the kernel is compiled to keep resident (its register budget).
+unsigned kg_search_waves_per_cu();
 // gnode[n] = {goff[n], deg(n), M[n], 0}; *dupflag |= 1 if some list repeats an id.
 hipError_t kg_launch_gnode(unsigned N, uint32_t const *goff, uint32_t const *gids, uint32_t const *gM, uint4 *gnode,
                            uint32_t *dupflag, hipStream_t s);

@@ -48,7 +48,7 @@ struct WaveLDS {
     float *q;
     uint32_t *fid;
     float *fdist;
-    uint32_t *vt;  // visited hash set (vcap entries, VT_EMPTY = free)
+    uint32_t *vt;  // visited hash set (vcap entries of 32 or 16 bits, all ones = free)
 };
 
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
@@ -57,7 +57,9 @@ struct LdsLayout {
     size_t pid, pdist, pmeta, pbase, tkey, rid, rdist, q, fid, fdist, vt, total;
     // tcap: starting points sorted at once (P random seeds, or init ids);
     // vcap: entries of the visited hash set (0: none)
-    __host__ __device__ LdsLayout(unsigned cap, unsigned rcap, unsigned Dp, unsigned tcap, unsigned vcap) {
+    // vbytes: bytes per visited-set entry (4, or 2 for the compact form)
+    __host__ __device__ LdsLayout(unsigned cap, unsigned rcap, unsigned Dp, unsigned tcap, unsigned vcap,
+                                  unsigned vbytes) {
         size_t o = 0;
         pid = o; o = align16(o + 4ull * cap);
         pdist = o; o = align16(o + 4ull * cap);
@@ -69,7 +71,7 @@ struct LdsLayout {
         q = o; o = align16(o + 4ull * Dp);
         fid = o; o = align16(o + 4ull * KG_WAVE);
         fdist = o; o = align16(o + 4ull * KG_WAVE);
-        vt = o; o = align16(o + 4ull * vcap);
+        vt = o; o = align16(o + size_t(vbytes) * vcap);
         total = o;
     }
 };
@@ -287,6 +289,64 @@ __device__ __forceinline__ bool vt_contains(uint32_t const *vt, unsigned mask, u
     }
 }
 
+// Compact form: 16-bit entries, two per LDS word, for ids below 2^idbits.
+// p = id * 2654435761 mod 2^idbits is a bijection; its top log2(vcap) bits
+// are the home slot and its low rb bits the remainder.  Linear probing stores
+// (displacement << rb) | remainder, so a slot and its entry give back the home
+// slot and with it the id (p * 2654435761^-1).  Displacements stop at dmax, so
+// no entry is all ones (free); a probe that would go further reports overflow
+// and the caller moves the set into the bitset.
+constexpr uint32_t VT_HASH_INV = 244002641u;  // 2654435761 * 244002641 = 1 mod 2^32
+struct VtCfg {
+    bool b16;
+    unsigned mask, shift;             // 32-bit entries
+    unsigned rb, rmask, dmax, idmask;  // 16-bit entries
+};
+enum : unsigned { VT_PRESENT = 0, VT_FRESH = 1, VT_OVERFLOW = 2 };
+__device__ __forceinline__ unsigned vt16_insert(uint32_t *vt, VtCfg const &c, uint32_t id) {
+    uint32_t const p = (id * 2654435761u) & c.idmask;
+    uint32_t const r = p & c.rmask;
+    unsigned slot = p >> c.rb;
+    for (unsigned d = 0; d <= c.dmax; ++d, slot = (slot + 1) & c.mask) {
+        uint32_t const want = (d << c.rb) | r;
+        uint32_t *const word = vt + (slot >> 1);
+        unsigned const sh = (slot & 1u) * 16u;
+        uint32_t old = *word;
+        for (;;) {
+            uint32_t const e = (old >> sh) & 0xffffu;
+            if (e == want) return VT_PRESENT;
+            if (e != 0xffffu) break;  // another id's: probe on
+            uint32_t const got = atomicCAS(word, old, (old & ~(0xffffu << sh)) | (want << sh));
+            if (got == old) return VT_FRESH;
+            old = got;  // the word changed under this lane (its other half, or this one): look again
+        }
+    }
+    return VT_OVERFLOW;
+}
+__device__ __forceinline__ bool vt16_contains(uint32_t const *vt, VtCfg const &c, uint32_t id) {
+    uint32_t const p = (id * 2654435761u) & c.idmask;
+    uint32_t const r = p & c.rmask;
+    unsigned slot = p >> c.rb;
+    for (unsigned d = 0; d <= c.dmax; ++d, slot = (slot + 1) & c.mask) {
+        uint32_t const e = (vt[slot >> 1] >> ((slot & 1u) * 16u)) & 0xffffu;
+        if (e == ((d << c.rb) | r)) return true;
+        if (e == 0xffffu) return false;
+    }
+    return false;
+}
+// id of the 16-bit entry e held in `slot`
+__device__ __forceinline__ uint32_t vt16_id(VtCfg const &c, unsigned slot, uint32_t e) {
+    unsigned const home = (slot - (e >> c.rb)) & c.mask;
+    return (((home << c.rb) | (e & c.rmask)) * VT_HASH_INV) & c.idmask;
+}
+__device__ __forceinline__ unsigned vt_put(uint32_t *vt, VtCfg const &c, uint32_t id) {
+    if (c.b16) return vt16_insert(vt, c, id);
+    return vt_insert(vt, c.mask, c.shift, id) ? VT_FRESH : VT_PRESENT;
+}
+__device__ __forceinline__ bool vt_has(uint32_t const *vt, VtCfg const &c, uint32_t id) {
+    return c.b16 ? vt16_contains(vt, c, id) : vt_contains(vt, c.mask, c.shift, id);
+}
+
 #ifndef SEARCH_WAVES_PER_EU
 #define SEARCH_WAVES_PER_EU 4
 #endif
@@ -299,7 +359,7 @@ search_kernel(kg_search_args a) {
     unsigned const slot = blockIdx.x * (blockDim.x / KG_WAVE) + wib;
     unsigned const cap = a.pool_cap;
     unsigned const rcap = a.res_cap;
-    LdsLayout lay(cap, rcap, a.Dp, a.seed_cap, a.vcap);
+    LdsLayout lay(cap, rcap, a.Dp, a.seed_cap, a.vcap, a.vt_bits / 8);
     unsigned char *base = smem + size_t(wib) * a.lds_per_wave;
     WaveLDS w;
     w.pid = reinterpret_cast<uint32_t *>(base + lay.pid);
@@ -313,7 +373,15 @@ search_kernel(kg_search_args a) {
     w.fid = reinterpret_cast<uint32_t *>(base + lay.fid);
     w.fdist = reinterpret_cast<float *>(base + lay.fdist);
     w.vt = reinterpret_cast<uint32_t *>(base + lay.vt);
-    unsigned const vmask = a.vcap - 1u, vshift = a.vcap ? 32u - unsigned(__builtin_ctz(a.vcap)) : 0u;
+    VtCfg vs;
+    vs.b16 = a.vt_bits == 16;
+    vs.mask = a.vcap - 1u;
+    vs.shift = a.vcap ? 32u - unsigned(__builtin_ctz(a.vcap)) : 0u;
+    vs.rb = a.vt_rb;
+    vs.rmask = (1u << a.vt_rb) - 1u;
+    vs.dmax = (1u << (16u - a.vt_rb)) - 2u;
+    vs.idmask = a.vt_idmask;
+    unsigned const vwords = vs.b16 ? a.vcap / 2 : a.vcap;  // LDS words of the set
 
     uint32_t *vis = a.visited + size_t(slot) * a.vis_words;
     uint32_t *touched = a.touched + size_t(slot) * a.touch_cap;
@@ -341,20 +409,28 @@ search_kernel(kg_search_args a) {
         // in the HBM bitset (ntouch ids recorded for clearing)
         bool glob = a.vcap == 0;
         unsigned vcount = 0;
-        for (unsigned i = lane; i < a.vcap; i += KG_WAVE) w.vt[i] = VT_EMPTY;
+        for (unsigned i = lane; i < vwords; i += KG_WAVE) w.vt[i] = VT_EMPTY;
         wave_sync();
         // moves the LDS set into the bitset; later visits go there
         auto spill = [&]() {
-            for (unsigned c = 0; c < a.vcap; c += KG_WAVE) {
-                uint32_t const e = w.vt[c + lane];
-                bool const has = e != VT_EMPTY;
-                uint64_t const b = ballot(has);
-                if (has) {
-                    atomicOr(&vis[e >> 5], 1u << (e & 31));
-                    unsigned const pos = ntouch + popc64(b & lt_mask);
-                    if (pos < a.touch_cap) touched[pos] = e;
+            for (unsigned c = 0; c < vwords; c += KG_WAVE) {
+                uint32_t const word = w.vt[c + lane];
+                for (unsigned half = 0; half < (vs.b16 ? 2u : 1u); ++half) {
+                    uint32_t e = word;
+                    bool has = e != VT_EMPTY;
+                    if (vs.b16) {
+                        uint32_t const e16 = (word >> (16 * half)) & 0xffffu;
+                        has = e16 != 0xffffu;
+                        e = has ? vt16_id(vs, 2 * (c + lane) + half, e16) : 0u;
+                    }
+                    uint64_t const b = ballot(has);
+                    if (has) {
+                        atomicOr(&vis[e >> 5], 1u << (e & 31));
+                        unsigned const pos = ntouch + popc64(b & lt_mask);
+                        if (pos < a.touch_cap) touched[pos] = e;
+                    }
+                    ntouch += popc64(b);
                 }
-                ntouch += popc64(b);
             }
             glob = true;
             if (lane == 0 && a.spills) atomicAdd(a.spills, 1u);
@@ -374,7 +450,7 @@ search_kernel(kg_search_args a) {
                         keep = true;
                         if (trial > 0)
                             keep = glob ? !(load_u32_l2(&vis[s >> 5]) & (1u << (s & 31)))
-                                        : !vt_contains(w.vt, vmask, vshift, s);
+                                        : !vt_has(w.vt, vs, s);
                     }
                     uint64_t b = ballot(keep);
                     if (keep) w.pid[L + popc64(b & lt_mask)] = s;
@@ -396,9 +472,20 @@ search_kernel(kg_search_args a) {
                 }
                 ntouch += L;
             } else {
+                bool ovf = false;
                 for (unsigned c = 0; c < L; c += KG_WAVE) {
-                    bool const fresh = c + lane < L && vt_insert(w.vt, vmask, vshift, w.pid[c + lane]);
-                    vcount += popc64(ballot(fresh));
+                    unsigned const st = (c + lane < L) ? vt_put(w.vt, vs, w.pid[c + lane]) : VT_PRESENT;
+                    vcount += popc64(ballot(st == VT_FRESH));
+                    ovf |= st == VT_OVERFLOW;
+                }
+                if (ballot(ovf)) {  // a 16-bit probe ran out of displacement: the bitset takes over
+                    spill();
+                    for (unsigned i = lane; i < L; i += KG_WAVE) {
+                        uint32_t s = w.pid[i];
+                        atomicOr(&vis[s >> 5], 1u << (s & 31));
+                        if (ntouch + i < a.touch_cap) touched[ntouch + i] = s;
+                    }
+                    ntouch += L;
                 }
             }
             batch_dist<METRIC, RowT>(a, w.pid, w.pdist, L, w.q, qn);
@@ -466,12 +553,24 @@ search_kernel(kg_search_args a) {
                         }
                     }
                     bool fresh = false;
+                    unsigned st = VT_PRESENT;
                     if (lane < cnt && !dup) {
                         if (glob) {
                             uint32_t bit = 1u << (id & 31);
                             fresh = !(atomicOr(&vis[id >> 5], bit) & bit);
                         } else {
-                            fresh = vt_insert(w.vt, vmask, vshift, id);
+                            st = vt_put(w.vt, vs, id);
+                            fresh = st == VT_FRESH;
+                        }
+                    }
+                    if (!glob && ballot(st == VT_OVERFLOW)) {
+                        // a 16-bit probe ran out of displacement: move the set
+                        // (this batch's inserts included) into the bitset, and
+                        // test the rest there
+                        spill();
+                        if (st == VT_OVERFLOW) {
+                            uint32_t bit = 1u << (id & 31);
+                            fresh = !(atomicOr(&vis[id >> 5], bit) & bit);
                         }
                     }
                     uint64_t b = ballot(fresh);
@@ -637,11 +736,14 @@ hipError_t kg_launch_gnode(unsigned N, uint32_t const *goff, uint32_t const *gid
     return hipGetLastError();
 }
 
-size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp, unsigned seed_cap, unsigned vcap) {
-    return LdsLayout(pool_cap, res_cap, Dp, seed_cap, vcap).total;
+size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp, unsigned seed_cap, unsigned vcap,
+                              unsigned vt_bits) {
+    return LdsLayout(pool_cap, res_cap, Dp, seed_cap, vcap, vt_bits / 8).total;
 }
 
 unsigned kg_search_max_wpb() { return SEARCH_MAX_WPB; }
+
+unsigned kg_search_waves_per_cu() { return 4 * SEARCH_WAVES_PER_EU; }
 
 hipError_t kg_launch_search(int metric, kg_search_args const &a, unsigned blocks, unsigned waves_per_block,
                             hipStream_t s) {

@@ -119,33 +119,70 @@ def test_bf16_row_storage_gives_identical_walks(kind, metric, d, lossless):
 
 @pytest.mark.parametrize("P,S,T", [(40, 10, 1), (300, 10, 1), (20, 64, 3), (500, 30, 1)])
 def test_lds_visited_set_gives_identical_walks(P, S, T):
-    # the LDS visited set (spilling into the HBM bitset once half full) and the
-    # bitset alone give the oracle's ids, distances and counters; large P
-    # makes queries spill mid-walk, T > 1 reads the set across trials
+    # the LDS visited set (16-bit entries here, N = 6000, or 32-bit; spilling
+    # into the HBM bitset once half full) and the bitset alone give the
+    # oracle's ids, distances and counters; a 256-entry set makes queries
+    # spill mid-walk, T > 1 reads the set across trials
     import ctypes
     X, Q, idx = setup("intmix", L2, d=128)
     g = gpu_index(X, idx, L2)
     res = {}
-    for mode in (0, 1):
+    for mode, size in [(0, 0), (2, 0), (1, 0), (0, 256), (2, 256)]:
         g._h.call("kg_set_search_visited", mode)
+        g._h.call("kg_set_search_visited_size", size)
         gi, gd = g.search(Q, K=10, P=P, S=S, T=T, withDistance=True, stats=True)
-        vc, sp = ctypes.c_uint(), ctypes.c_uint()
+        vc, sp, bits = ctypes.c_uint(), ctypes.c_uint(), ctypes.c_uint()
         g._h.call("kg_search_visited_stats", ctypes.byref(vc), ctypes.byref(sp))
-        res[mode] = (gi, gd, g.last_counts.copy(), g.last_comps.copy(), g.last_scans.copy(), vc.value, sp.value)
-    print("P=%d: LDS set %d entries, %d of %d queries spilled" % (P, res[0][5], res[0][6], len(Q)))
-    assert res[1][5] == 0 and res[1][6] == 0
+        g._h.call("kg_search_visited_bits", ctypes.byref(bits))
+        res[mode, size] = (gi, gd, g.last_counts.copy(), g.last_comps.copy(), g.last_scans.copy(), vc.value,
+                           sp.value, bits.value)
+        print("P=%d mode %d size %d: LDS set %d entries of %d bits, %d of %d queries spilled" % (
+            P, mode, size, vc.value, bits.value, sp.value, len(Q)))
+    g._h.call("kg_set_search_visited", 0)
+    g._h.call("kg_set_search_visited_size", 0)
+    assert res[1, 0][5] == 0 and res[1, 0][6] == 0
     if P <= 300:
-        assert res[0][5] >= 256  # a P = 500 pool leaves no LDS room for a set: bitset only
-    if P == 300:
-        assert res[0][6] > 0
-    for a, b in zip(res[0][:5], res[1][:5]):
-        assert np.array_equal(a, b)
+        assert res[0, 0][5] >= 256  # a P = 500 pool leaves no LDS room for a set: bitset only
+        assert res[0, 0][7] == 16 and res[2, 0][7] == 32
+    assert res[0, 256][5] == 256 and res[0, 256][7] == 16 and res[0, 256][6] > 0
+    assert res[2, 256][5] == 256 and res[2, 256][7] == 32 and res[2, 256][6] > 0
+    for key in res:
+        for a, b in zip(res[0, 0][:5], res[key][:5]):
+            assert np.array_equal(a, b), key
     ids, dd, c, comps, scans = idx.search(X, Q, L2, K=10, P=P, S=S, T=T)
-    assert np.array_equal(res[0][2], c)
+    r = res[0, 0]
+    assert np.array_equal(r[2], c)
     for q in range(len(Q)):
-        assert np.array_equal(res[0][0][q, :c[q]], ids[q, :c[q]]), q
-        assert np.array_equal(res[0][1][q, :c[q]], dd[q, :c[q]]), q
-    assert np.array_equal(res[0][3], comps) and np.array_equal(res[0][4], scans)
+        assert np.array_equal(r[0][q, :c[q]], ids[q, :c[q]]), q
+        assert np.array_equal(r[1][q, :c[q]], dd[q, :c[q]]), q
+    assert np.array_equal(r[3], comps) and np.array_equal(r[4], scans)
+
+
+def test_compact_visited_set_on_a_larger_index():
+    # 300,000 nodes (19-bit ids): 16-bit entries keep 9 remainder bits at
+    # 1024 entries (displacements up to 126); a 256-entry set would need 11
+    # and takes 32-bit entries; every form walks identically to the bitset
+    import ctypes
+    import kgraph_amd
+    from kgraph_amd import datagen
+    X = datagen.integer_mixture(300000, 32, 300, 21, 22)
+    Q = datagen.integer_mixture(500, 32, 300, 21, 23)
+    g = kgraph_amd.KGraph(X, "euclidean", l2sqr=True)
+    g.build(K=16, L=32, iterations=8, controls=20, reverse=-1)
+    out = {}
+    for mode, size in [(1, 0), (0, 0), (0, 1024), (0, 256), (2, 256)]:
+        g._h.call("kg_set_search_visited", mode)
+        g._h.call("kg_set_search_visited_size", size)
+        gi, gd = g.search(Q, K=10, P=60, S=20, withDistance=True, stats=True)
+        bits = ctypes.c_uint()
+        g._h.call("kg_search_visited_bits", ctypes.byref(bits))
+        out[mode, size] = (gi, gd, g.last_counts.copy(), g.last_comps.copy(), g.last_scans.copy(), bits.value)
+    g._h.call("kg_set_search_visited", 0)
+    g._h.call("kg_set_search_visited_size", 0)
+    assert out[0, 0][5] == 16 and out[0, 1024][5] == 16 and out[0, 256][5] == 32 and out[2, 256][5] == 32
+    for key in out:
+        for a, b in zip(out[1, 0][:5], out[key][:5]):
+            assert np.array_equal(a, b), key
 
 
 def test_search_parity_mixed_batch():

@@ -10,8 +10,9 @@ One step = two full builds of the index -- the plain form (exact FP32 local
 join) and the BF16 MFMA form (exact on these integer rows) -- then one search
 pass per (start, P, S) setting over the 10,000 queries (queries resident in
 HBM): random starting points with P in {50, 100, 200} (the reference's
-configuration) and starting points selected from an 8,192-node entry sample
-(exact FP32 scores) with P in {10, 20, 50, 100} and S in {10, 32, 64}.
+configuration) and starting points selected from an entry sample of 4,096 or
+8,192 nodes (exact FP32 scores; the sample's rows are index-side state, set
+outside the timed search) with P in {10, 20, 50, 100} and S in {10, 32, 64}.
 `value` is the highest search QPS whose recall@10 (distance-equality, against
 the exact K-NN from the brute-force kernel) reaches 0.95; `build_s` is the
 BF16-form build's wall time and `build_forms` holds both forms.
@@ -59,6 +60,9 @@ def parse():
     ap.add_argument("--entry-sample", type=int, default=None,
                     help="entry-point sample size (default 8192; 65536 for learned-5m-d128, whose 5,000 "
                          "separated clusters need several sampled nodes each)")
+    ap.add_argument("--entry-samples", type=str, default=None,
+                    help="intmix-1m-d128: entry-sample sizes swept with P and S (default 4096,8192; "
+                         "--entry-sample E sweeps E alone)")
     ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -78,6 +82,9 @@ def main():
                   "angular-1.2m-d96": 1_200_000}.get(args.workload, 1_000_000)
     if args.nq is None:
         args.nq = 100_000 if batched else 10_000
+    entry_given = args.entry_sample is not None
+    if args.entry_samples is None:
+        args.entry_samples = str(args.entry_sample) if entry_given else "4096,8192"
     if args.entry_sample is None:
         args.entry_sample = {"learned-5m-d128": 65536, "batched-10m-d96": 16384,
                              "angular-1.2m-d96": 16384}.get(args.workload, 8192)
@@ -133,7 +140,8 @@ def main():
     dSc = devmem.DeviceArray((nq,), np.uint32)
 
     m = K - 1  # entry points per query (SearchParams::init must stay below K)
-    sweeps = [("random", P, 10) for P in Ps] + [("entries", P, S) for P in Pe for S in Se]
+    Es = [int(x) for x in args.entry_samples.split(",") if x]
+    sweeps = [("random", P, 10, 0) for P in Ps] + [("entries", P, S, E) for E in Es for P in Pe for S in Se]
 
     def one_step():
         # the plain form (exact FP32 local join), then the BF16 MFMA form; the
@@ -159,11 +167,15 @@ def main():
         build_s = builds["bf16"]["s"]
         # entry points are scored exactly (FP32) for the search
         h.call("kg_set_precision", _lib.PRECISION_FP32)
-        h.call("kg_set_entry_sample", args.entry_sample, 1998)
-        res = [search_once(mode, P, S) for mode, P, S in sweeps]
+        res, cur = [], None
+        for mode, P, S, E in sweeps:
+            if mode == "entries" and E != cur:  # the sample's rows are index-side state, set untimed
+                h.call("kg_set_entry_sample", E, 1998)
+                cur = E
+            res.append(search_once(mode, P, S, E))
         return info, build_s, res, builds
 
-    def search_once(mode, P, S):
+    def search_once(mode, P, S, E=0):
         sp = SearchParams(K, 0, P, S, 1, 1e30, 1998, m if mode == "entries" else 0)
         devmem.synchronize()
         ts = time.perf_counter()
@@ -178,7 +190,7 @@ def main():
         kms = ctypes_float(h, 0)
         lat = np.zeros(nq, np.float32)  # per-query walk time, stamped inside the kernel
         h.call("kg_search_latencies", ptr(lat), nq)
-        return dict(mode=mode, P=P, S=S, wall_s=wall, kernel_ms=kms, entry_ms=ems, lat=lat, ids=dI.to_host(),
+        return dict(mode=mode, P=P, S=S, E=E, wall_s=wall, kernel_ms=kms, entry_ms=ems, lat=lat, ids=dI.to_host(),
                     dists=dD.to_host(), comps=dCo.to_host(), scans=dSc.to_host(),
                     seeds=(m if mode == "entries" else P))
 
@@ -232,14 +244,17 @@ def main():
         # gather the lossless BF16 copy) per scored node (starting points +
         # comps), 4 bytes per neighbour id read, 4 per starting id
         byt = float((rb * D * (r["seeds"] + r["comps"].astype(np.float64)) + 4 * r["scans"] + 4 * r["seeds"]).sum())
-        sweep.append(dict(mode=r["mode"], P=r["P"], S=r["S"], qps=qps, recall=rec, kernel_ms=r["kernel_ms"],
+        sweep.append(dict(mode=r["mode"], P=r["P"], S=r["S"], E=r["E"], qps=qps, recall=rec, kernel_ms=r["kernel_ms"],
                           entry_ms=r["entry_ms"], bytes=byt, mean_comps=float(r["comps"].mean()),
                           mean_scans=float(r["scans"].mean())))
-        log("%-7s P=%d S=%d recall@10=%.4f QPS=%.0f walk kernel %.2fms entry %.2fms (%.1f GB/s algorithmic)" % (
-            r["mode"], r["P"], r["S"], rec, qps, r["kernel_ms"], r["entry_ms"], byt / r["kernel_ms"] / 1e6))
+        log("%-7s E=%d P=%d S=%d recall@10=%.4f QPS=%.0f walk kernel %.2fms entry %.2fms (%.1f GB/s algorithmic)" % (
+            r["mode"], r["E"], r["P"], r["S"], rec, qps, r["kernel_ms"], r["entry_ms"], byt / r["kernel_ms"] / 1e6))
     ok = [s for s in sweep if s["recall"] >= 0.95]
     pick = max(ok, key=lambda s: s["qps"]) if ok else max(sweep, key=lambda s: s["recall"])
     achieved = pick["bytes"] / (pick["kernel_ms"] * 1e-3) / 1e9
+    if pick["mode"] == "entries":  # the picked sample stays set for what follows
+        args.entry_sample = pick["E"]
+        h.call("kg_set_entry_sample", pick["E"], 1998)
     lat = res[sweep.index(pick)]["lat"]
     wall_ms = 1e3 * nq / pick["qps"]
     # where the picked setting's wall time goes: the two kernels (HIP events),
@@ -256,7 +271,7 @@ def main():
     row_cmp = None
     if rows == "bf16":
         h.call("kg_set_row_storage", 1)
-        r32 = search_once(pick["mode"], pick["P"], pick["S"])
+        r32 = search_once(pick["mode"], pick["P"], pick["S"], pick["E"])
         h.call("kg_set_row_storage", 0)
         same = bool(np.array_equal(r32["ids"], res[sweep.index(pick)]["ids"]) and
                     np.array_equal(r32["dists"], res[sweep.index(pick)]["dists"]))
@@ -760,11 +775,12 @@ def measured_traffic(kernel, workload, pick, rows):
     """HBM bytes per launch of `kernel` for this configuration, from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same command
     (profiles/<kernel>_<workload>_traffic.md, a line
-    `traffic P=<P> S=<S> mode=<mode> rows=<rows> bytes=<B>`), or None."""
+    `traffic P=<P> S=<S> mode=<mode> E=<entry sample> rows=<rows> bytes=<B>`), or None."""
     path = os.path.join(ROOT, "profiles", "%s_%s_traffic.md" % (kernel, workload))
     if not os.path.exists(path):
         return None
-    want = "traffic P=%d S=%d mode=%s rows=%s bytes=" % (pick["P"], pick["S"], pick["mode"], rows)
+    want = "traffic P=%d S=%d mode=%s E=%d rows=%s bytes=" % (pick["P"], pick["S"], pick["mode"], pick.get("E", 0),
+                                                               rows)
     for line in open(path):
         line = line.strip().strip("`")
         if line.startswith(want):

@@ -70,7 +70,8 @@ def parse():
     ap.add_argument("--cpu-queries", type=int, default=1000)
     ap.add_argument("--sigma", type=float, default=None,
                     help="batched-10m-d96: per-coordinate noise (default 0.3/sqrt(96), a noise radius of 0.3)")
-    ap.add_argument("--cpu-build-n", type=int, default=100_000)
+    ap.add_argument("--cpu-build-n", type=int, default=500_000,
+                    help="rows of the CPU build baseline's sample (about 10 s of CPU work)")
     return ap.parse_args()
 
 
@@ -855,16 +856,22 @@ def cpu_baseline(args, X, Q, h, P, K, bp, mode, S=10):
         g.call("kg_set_data", _lib.ptr(Xs), n, Xs.shape[1], Xs.shape[1], 0)
         ip = _lib.IndexParams(bp["iterations"], bp["L"], bp["K"], bp["S"], bp["R"], bp["controls"], bp["seed"],
                               bp["delta"], bp["recall"], bp["prune"], bp["reverse"])
-        ginfo = _lib.IndexInfo()
-        g.call("kg_build", ip, ginfo)  # warm
-        t = time.perf_counter()
-        g.call("kg_build", ip, ginfo)
-        gpu_b = time.perf_counter() - t
-        log("cpu build of %d rows: %.2fs (%d it, recall %.3f); gpu %.3fs (%d it, recall %.3f)" % (
-            n, cpu_b, info.iterations, info.recall, gpu_b, ginfo.iterations, ginfo.recall))
+        gpu = {}
+        for form, prec in (("fp32", _lib.PRECISION_FP32), ("bf16", _lib.PRECISION_BF16)):
+            g.call("kg_set_precision", prec)
+            ginfo = _lib.IndexInfo()
+            g.call("kg_build", ip, ginfo)  # warm
+            t = time.perf_counter()
+            g.call("kg_build", ip, ginfo)
+            gpu[form] = (time.perf_counter() - t, ginfo)
+        gpu_b, ginfo = gpu["fp32"]
+        log("cpu build of %d rows: %.2fs (%d it, recall %.3f); gpu plain %.3fs (%d it, recall %.3f), bf16 %.3fs" % (
+            n, cpu_b, info.iterations, info.recall, gpu_b, ginfo.iterations, ginfo.recall, gpu["bf16"][0]))
         res.update({"build_s_sample": round(cpu_b, 3), "gpu_build_s_sample": round(gpu_b, 4),
-                    "build_sample": "first %d rows, same IndexParams" % n,
+                    "gpu_bf16_build_s_sample": round(gpu["bf16"][0], 4),
+                    "build_sample": "first %d rows, same IndexParams; GPU plain FP32 form (bf16: the BF16 form)" % n,
                     "build_speedup_sample": round(cpu_b / gpu_b, 1),
+                    "build_speedup_sample_bf16": round(cpu_b / gpu["bf16"][0], 1),
                     "cpu_build_control_recall": round(float(info.recall), 4),
                     "gpu_build_control_recall_sample": round(float(ginfo.recall), 4)})
     return res

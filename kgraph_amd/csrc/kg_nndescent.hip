@@ -729,64 +729,94 @@ __device__ __forceinline__ void wave_sort_regs(uint64_t (&v)[PER]) {
     }
 }
 
-// Sorted pool U candidates -> first L distinct ids, all in registers.
+// Sorted pool U candidates -> first L distinct keys (kgraph.cpp:176-202).
+// Equal keys are equal ids (an id's distance is the same wherever it is
+// computed, canonical FP32), so a candidate whose id is in the pool, or came
+// earlier, is dropped first: an LDS hash set of ids, pool ids inserted before
+// candidates, so an existing entry keeps its flag.  Only the new distinct keys
+// are then sorted (wave bitonic in registers: ~175 of a node's ~320
+// candidates on the 1M-node build, instead of sorting pool and candidates
+// together) and merged with the sorted pool by ranks: a new key lands at its
+// index plus the number of pool keys below it (binary search in LDS), a pool
+// key at its index plus the number of new keys below it (a histogram of the
+// new keys' ranks, prefix-summed).  The same L keys as a sort of the union.
+constexpr unsigned MG_TAB = 2048;  // hash slots per wave: load <= 1/2 for the NND_MERGE_MAX keys
+constexpr uint32_t MG_EMPTY = 0xffffffffu;
+
+__host__ __device__ inline size_t merge_lds_per_wave(unsigned L) {
+    return 4ull * MG_TAB + ((8ull * L + 15) & ~size_t(15)) + ((4ull * (L + 1) + 15) & ~size_t(15));
+}
+
+// true when id was not in the set (it is now)
+__device__ __forceinline__ bool mg_insert(uint32_t *tab, uint32_t id) {
+    unsigned h = (id * 2654435761u) >> (32 - 11);
+    for (;;) {
+        uint32_t const old = atomicCAS(&tab[h], MG_EMPTY, id);
+        if (old == MG_EMPTY) return true;
+        if (old == id) return false;
+        h = (h + 1) & (MG_TAB - 1);
+    }
+}
+
 template <int PER>
-__device__ void merge_node(NndArgs const &a, unsigned n, unsigned c, unsigned m, unsigned mall, bool over) {
+__device__ void merge_new_keys(NndArgs const &a, unsigned n, unsigned c, unsigned u, uint64_t const *stage,
+                               uint64_t const *pk, uint32_t *hist) {
     unsigned const lane = lane_id();
     uint64_t *pool = a.pool + size_t(n) * a.L;
-    uint64_t const *cand = a.cand + size_t(n) * a.C;
-    uint64_t *ovf = a.ovf + size_t(n) * a.O;
     uint64_t v[PER];
 #pragma unroll
     for (int r = 0; r < PER; ++r) {
         unsigned const i = lane * PER + r;
-        v[r] = (i < c) ? pool[i] : (i < c + m) ? ((i - c < mall) ? cand[i - c] : ovf[i - c - mall]) : ~0ull;
-    }
-    if (over) {  // the overflow area was used: empty it for the next join
-        wave_sync();
-        for (unsigned i = lane; i < a.O; i += KG_WAVE) ovf[i] = ~0ull;
+        v[r] = i < u ? stage[i] : ~0ull;
     }
     wave_sort_regs<PER>(v);
-    // keep the first of every run of equal (dist, id); an existing pool entry
-    // (flag 0) sorts before an equal candidate (flag 1) and so keeps its flag
-    uint64_t const prev_last = shfl_up64(v[PER - 1], 1);
-    unsigned keep_mask = 0, nk = 0;
-#pragma unroll
-    for (int r = 0; r < PER; ++r) {
-        uint64_t const prev = (r > 0) ? v[r - 1] : prev_last;
-        bool const first = (lane == 0 && r == 0);
-        bool const keep = v[r] != ~0ull && (first || (prev >> 1) != (v[r] >> 1));
-        keep_mask |= unsigned(keep) << r;
-        nk += keep;
-    }
-    // exclusive prefix of the per-lane counts
-    unsigned incl = nk;
-    for (int d = 1; d < 64; d <<= 1) {
-        unsigned y = __shfl_up(incl, d);
-        if (int(lane) >= d) incl += y;
-    }
-    unsigned const total = __shfl(incl, 63);
-    unsigned pos = incl - nk;
-    unsigned const newc = min(total, a.L);
     unsigned flags = 0;
     float rad = 0.f, kth = 0.f;
     bool has_rad = false, has_kth = false;
+    auto place = [&](unsigned pos, uint64_t key) {
+        if (pos < a.L) pool[pos] = key;
+        if (pos < a.K) flags += key_flag(key);
+        if (pos == a.L - 1) {
+            rad = key_dist(key);
+            has_rad = true;
+        }
+        if (pos == a.K - 1) {
+            kth = key_dist(key);
+            has_kth = true;
+        }
+    };
+    // new keys: index + pool keys below (lower bound in the sorted pool)
 #pragma unroll
     for (int r = 0; r < PER; ++r) {
-        if (keep_mask & (1u << r)) {
-            if (pos < a.L) pool[pos] = v[r];
-            if (pos < a.K) flags += key_flag(v[r]);
-            if (pos == a.L - 1) {
-                rad = key_dist(v[r]);
-                has_rad = true;
+        unsigned const i = lane * PER + r;
+        if (i < u) {
+            uint64_t const e = v[r];
+            unsigned lo = 0, hi = c;
+            while (lo < hi) {
+                unsigned const mid = (lo + hi) >> 1;
+                if (pk[mid] < e)
+                    lo = mid + 1;
+                else
+                    hi = mid;
             }
-            if (pos == a.K - 1) {
-                kth = key_dist(v[r]);
-                has_kth = true;
-            }
-            ++pos;
+            atomicAdd(&hist[lo], 1u);
+            place(i + lo, e);
         }
     }
+    wave_sync();
+    // pool keys: index + new keys below (prefix of the rank histogram)
+    unsigned carry = 0;
+    for (unsigned b = 0; b < c; b += KG_WAVE) {
+        unsigned const i = b + lane;
+        unsigned x = i < c ? hist[i] : 0u;
+        for (int d = 1; d < 64; d <<= 1) {
+            unsigned const y = __shfl_up(x, d);
+            if (int(lane) >= d) x += y;
+        }
+        if (i < c) place(i + carry + x, pk[i]);
+        carry += __shfl(x, 63);
+    }
+    unsigned const newc = min(c + u, a.L);
     for (int o = 32; o > 0; o >>= 1) flags += __shfl_xor(flags, o);
     uint64_t const br = ballot(has_rad), bk = ballot(has_kth);
     float const rad_w = br ? __shfl(rad, int(ctz64(br))) : __builtin_huge_valf();
@@ -802,24 +832,90 @@ __device__ void merge_node(NndArgs const &a, unsigned n, unsigned c, unsigned m,
 }
 
 __global__ void __launch_bounds__(256) merge_kernel(NndArgs a) {
-    unsigned const n = blockIdx.x * 4 + (threadIdx.x >> 6);
+    extern __shared__ __align__(16) unsigned char mg_smem[];
+    unsigned const w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    unsigned const n = blockIdx.x * (blockDim.x >> 6) + w;
     if (n >= a.N) return;
+    unsigned char *base = mg_smem + size_t(w) * merge_lds_per_wave(a.L);
+    uint32_t *tab = reinterpret_cast<uint32_t *>(base);           // hash set, then the staged new keys
+    uint64_t *stage = reinterpret_cast<uint64_t *>(base);
+    uint64_t *pk = reinterpret_cast<uint64_t *>(base + 4ull * MG_TAB);
+    uint32_t *hist = reinterpret_cast<uint32_t *>(base + 4ull * MG_TAB + ((8ull * a.L + 15) & ~size_t(15)));
     unsigned const c = a.cnt[n];
     unsigned const cc = a.ccnt[n];
     unsigned const mall = min(cc, a.C);                        // claimed slots
-    unsigned const mtot = mall + (cc > a.C ? a.O : 0u);        // + the overflow area
-    unsigned const m = min(mtot, NND_MERGE_MAX - c);  // the register sort holds 1024 keys
-    unsigned const tot = c + m;
-    if (tot <= 64)
-        merge_node<1>(a, n, c, m, mall, cc > a.C);
-    else if (tot <= 128)
-        merge_node<2>(a, n, c, m, mall, cc > a.C);
-    else if (tot <= 256)
-        merge_node<4>(a, n, c, m, mall, cc > a.C);
-    else if (tot <= 512)
-        merge_node<8>(a, n, c, m, mall, cc > a.C);
+    bool const over = cc > a.C;
+    unsigned const mtot = mall + (over ? a.O : 0u);            // + the overflow area
+    unsigned const m = min(mtot, NND_MERGE_MAX - c);  // at most NND_MERGE_MAX keys per merge
+    uint64_t *pool = a.pool + size_t(n) * a.L;
+    uint64_t const *cand = a.cand + size_t(n) * a.C;
+    uint64_t *ovf = a.ovf + size_t(n) * a.O;
+    // candidates, 64 a round, all loads issued first; the fresh ones stay in
+    // registers until the hash set is done with (the staging area reuses its
+    // memory)
+    constexpr int MR = NND_MERGE_MAX / KG_WAVE;
+    uint64_t held[MR];
+#pragma unroll
+    for (int r = 0; r < MR; ++r) {
+        unsigned const j = unsigned(r) * KG_WAVE + lane;
+        held[r] = ~0ull;
+        if (unsigned(r) * KG_WAVE < m && j < m) held[r] = (j < mall) ? cand[j] : ovf[j - mall];
+    }
+    constexpr int PR = 4;  // pool keys loaded up front (the first 256); a longer pool reads the rest after
+    uint64_t pr[PR];
+#pragma unroll
+    for (int r = 0; r < PR; ++r) {
+        unsigned const i = unsigned(r) * KG_WAVE + lane;
+        pr[r] = i < c ? pool[i] : ~0ull;
+    }
+    for (unsigned i = lane; i < MG_TAB; i += KG_WAVE) tab[i] = MG_EMPTY;
+    for (unsigned i = lane; i <= c; i += KG_WAVE) hist[i] = 0u;
+    wave_sync();
+#pragma unroll
+    for (int r = 0; r < PR; ++r) {
+        unsigned const i = unsigned(r) * KG_WAVE + lane;
+        if (i < c) {
+            pk[i] = pr[r];
+            mg_insert(tab, key_id(pr[r]));
+        }
+    }
+    for (unsigned i = PR * KG_WAVE + lane; i < c; i += KG_WAVE) {
+        uint64_t const k = pool[i];
+        pk[i] = k;
+        mg_insert(tab, key_id(k));
+    }
+    wave_sync();
+    uint64_t fresh_mask = 0;  // bit r: held[r] is a new key
+#pragma unroll
+    for (int r = 0; r < MR; ++r)
+        if (unsigned(r) * KG_WAVE < m && held[r] != ~0ull && mg_insert(tab, key_id(held[r]))) fresh_mask |= 1ull << r;
+    if (over) {  // the overflow area was used: empty it for the next join
+        wave_sync();
+        for (unsigned i = lane; i < a.O; i += KG_WAVE) ovf[i] = ~0ull;
+    }
+    wave_sync();
+    unsigned u = 0;
+    uint64_t const lt = (1ull << lane) - 1;
+#pragma unroll
+    for (int r = 0; r < MR; ++r) {
+        if (unsigned(r) * KG_WAVE < m) {
+            bool const f = (fresh_mask >> r) & 1u;
+            uint64_t const b = ballot(f);
+            if (f) stage[u + popc64(b & lt)] = held[r];
+            u += popc64(b);
+        }
+    }
+    wave_sync();
+    if (u <= 64)
+        merge_new_keys<1>(a, n, c, u, stage, pk, hist);
+    else if (u <= 128)
+        merge_new_keys<2>(a, n, c, u, stage, pk, hist);
+    else if (u <= 256)
+        merge_new_keys<4>(a, n, c, u, stage, pk, hist);
+    else if (u <= 512)
+        merge_new_keys<8>(a, n, c, u, stage, pk, hist);
     else
-        merge_node<16>(a, n, c, m, mall, cc > a.C);
+        merge_new_keys<16>(a, n, c, u, stage, pk, hist);
 }
 
 // ------------------------------------------------------------------ update
@@ -1052,7 +1148,11 @@ hipError_t nnd_launch_join(int metric, NndArgs const &a, hipStream_t s) {
 }
 
 hipError_t nnd_launch_merge(NndArgs const &a, hipStream_t s) {
-    hipLaunchKernelGGL(merge_kernel, dim3((a.N + 3) / 4), dim3(256), 0, s, a);
+    // waves per block: 4, fewer when their LDS would pass 64 KB (large L)
+    unsigned wpb = 4;
+    while (wpb > 1 && wpb * merge_lds_per_wave(a.L) > 65536) wpb /= 2;
+    hipLaunchKernelGGL(merge_kernel, dim3((a.N + wpb - 1) / wpb), dim3(64 * wpb), wpb * merge_lds_per_wave(a.L), s,
+                       a);
     unsigned const nb = std::min(1024u, (a.N + 255) / 256);
     hipLaunchKernelGGL(stats_reduce_kernel, dim3(nb), dim3(256), 0, s, a.part, a.N, a.stats);
     return hipGetLastError();

@@ -70,8 +70,10 @@ def parse():
     ap.add_argument("--cpu-queries", type=int, default=1000)
     ap.add_argument("--sigma", type=float, default=None,
                     help="batched-10m-d96: per-coordinate noise (default 0.3/sqrt(96), a noise radius of 0.3)")
-    ap.add_argument("--cpu-build-n", type=int, default=500_000,
-                    help="rows of the CPU build baseline's sample (about 10 s of CPU work)")
+    ap.add_argument("--cpu-build-n", type=int, default=1_000_000,
+                    help="rows of the CPU build baseline (default: the whole 1M-row C2 set, about 20 s of CPU "
+                         "work on 16 threads; smaller subsets of these separated clusters can strand a control "
+                         "node and run both builds to their iteration limit)")
     return ap.parse_args()
 
 

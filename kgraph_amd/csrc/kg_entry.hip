@@ -248,7 +248,9 @@ __global__ void __launch_bounds__(256) qnorm_check_kernel(float const *Q, unsign
 // BF16 value form the shortlist that rescore_kernel re-ranks exactly.
 // One block per 16 queries, 4 waves splitting the sample's 16-row tiles
 // (wave w takes tiles w, w + 4, ...).  A tile's operands: the sampled rows
-// (A, loaded per tile, the next tile's while this one is scored) and the
+// (A, two tiles in flight: a tile's MFMAs read its buffer, then the loads of
+// the tile two ahead refill it, branch-free so the wait for a tile leaves the
+// next one's loads outstanding) and the
 // block's 16 queries (B, kept in registers), so lane l holds the distances of
 // query l % 16 to rows 4 (l / 16) + j.  Each lane keeps the MF >= m best of
 // the rows it sees as 32-bit ordered distances with their ids: it sees rows
@@ -294,11 +296,16 @@ __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float
         unsigned const e = et * 16 + r16;
         bool const eok = e < a.E;
         if (a.rows_b) {
+            // branch-free: every lane loads from a valid address and masks the
+            // value, so the loads stay in flight past the wait for the older tile
             uint16_t const *er = a.rows_b + size_t(eok ? e : 0u) * a.Dp;
 #pragma unroll
             for (int s = 0; s < KS; ++s) {
                 unsigned const k = 16 * s + kq;
-                uint2 const v = (eok && k < a.Dp) ? *reinterpret_cast<uint2 const *>(er + k) : make_uint2(0u, 0u);
+                bool const ok = eok && k < a.Dp;
+                uint2 v = *reinterpret_cast<uint2 const *>(er + (k < a.Dp ? k : 0u));
+                v.x = ok ? v.x : 0u;
+                v.y = ok ? v.y : 0u;
                 op[s] = __builtin_bit_cast(bf16x4_t, v);
             }
         } else {
@@ -322,20 +329,25 @@ __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             unsigned const ee = et * 16 + kq + j;
-            nv[j] = ee < a.E ? nsrc[ee] : 0.f;
-            iv[j] = ee < a.E ? a.ids[ee] : 0u;
+            unsigned const ec = ee < a.E ? ee : a.E - 1;
+            nv[j] = nsrc[ec];
+            iv[j] = a.ids[ec];
         }
     };
-    // the next tile's operands, norms and ids load while this tile is scored
-    bf16x4_t nxt[KS];
-    float nn[4];
-    uint32_t ni[4];
+    // two tiles in flight: buffers A (tiles w, w + 8, ...) and B (w + 4, w + 12, ...)
+    bf16x4_t opA[KS], opB[KS];
+    float nA[4], nB[4];
+    uint32_t iA[4], iB[4];
     if (w < ntiles) {
-        load_tile(w, nxt);
-        load_meta(w, nn, ni);
+        load_tile(w, opA);
+        load_meta(w, nA, iA);
+    }
+    if (w + 4 < ntiles) {
+        load_tile(w + 4, opB);
+        load_meta(w + 4, nB, iB);
     }
     uint32_t tq = 0xffffffffu;  // the query's shared bound as this lane last saw it
-    for (unsigned et = w; et < ntiles; et += 4) {
+    auto score = [&](unsigned et, bf16x4_t (&op)[KS], float (&nv)[4], uint32_t (&iv)[4]) {
         if (((et >> 2) & 3) == 3) {
             // refresh the bound: this lane's m-th best, the four lanes of the
             // query in this wave, then the block's other waves (LDS minimum)
@@ -348,23 +360,21 @@ __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float
             atomicMin(&bound[r16], mine);
             tq = min(tq, __hip_atomic_load(&bound[r16], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
         }
-        bf16x4_t aop[KS];
+        // the MFMAs read this buffer before the loads two tiles ahead refill it
+        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < KS; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(op[s], bop[s], acc, 0, 0, 0);
         float en[4];
         uint32_t eid[4];
 #pragma unroll
-        for (int s = 0; s < KS; ++s) aop[s] = nxt[s];
-#pragma unroll
         for (int j = 0; j < 4; ++j) {
-            en[j] = nn[j];
-            eid[j] = ni[j];
+            en[j] = nv[j];
+            eid[j] = iv[j];
         }
-        if (et + 4 < ntiles) {
-            load_tile(et + 4, nxt);
-            load_meta(et + 4, nn, ni);
+        if (et + 8 < ntiles) {
+            load_tile(et + 8, op);
+            load_meta(et + 8, nv, iv);
         }
-        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int s = 0; s < KS; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aop[s], bop[s], acc, 0, 0, 0);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             unsigned const ee = et * 16 + kq + j;  // row of C: sampled row 4 (l / 16) + j
@@ -399,6 +409,10 @@ __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float
                 }
             }
         }
+    };
+    for (unsigned et = w; et < ntiles; et += 8) {
+        score(et, opA, nA, iA);
+        if (et + 4 < ntiles) score(et + 4, opB, nB, iB);
     }
     // the four lanes of a query (lane % 16 equal): m rounds of the minimum head
     for (unsigned r = 0; r < a.m; ++r) {

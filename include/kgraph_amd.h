@@ -124,6 +124,19 @@ int kg_set_search_visited_size(kg_handle *h, unsigned entries);
 int kg_search_visited_stats(kg_handle *h, unsigned *vcap, unsigned *spilled);
 int kg_search_visited_bits(kg_handle *h, unsigned *bits);
 
+/* Adjacency the graph walk reads (no counterpart in the reference, whose
+ * walk reads each node's vector of neighbours, kgraph.cpp:488-493).  Mode 0
+ * (default): besides the CSR lists the index keeps padded rows, the first W
+ * ids of every list at a fixed stride with W = max over nodes of
+ * min(M[n], deg(n)) rounded up to 16 (used when they fit in 16 GiB and the
+ * search's M <= W, which covers every id such a walk reads), so a node's ids
+ * load with its M and degree, at its first expansion, instead of a 16-byte
+ * record for every fresh candidate.  Mode 1: the CSR lists and per-node
+ * records.  Results are identical.  kg_search_adjacency reports what the last
+ * walk read (*padded = 1 for the padded rows) and W (0: none). */
+int kg_set_search_adjacency(kg_handle *h, int mode);
+int kg_search_adjacency(kg_handle *h, int *padded, unsigned *width);
+
 /* Row storage for graph walks (no counterpart in the reference, whose rows are
  * the caller's Matrix, kgraph-data.h:115-175).  kg_set_data keeps a BF16 copy
  * of the rows (the MFMA kernels' operands); when every value is representable

@@ -78,6 +78,8 @@ SIGNATURES = {
     "kg_search_visited_stats": (_I, [_P, _P, _P]),
     "kg_set_search_visited_size": (_I, [_P, _U]),
     "kg_search_visited_bits": (_I, [_P, _P]),
+    "kg_set_search_adjacency": (_I, [_P, _I]),
+    "kg_search_adjacency": (_I, [_P, _P, _P]),
     "kg_set_build_graphs": (_I, [_P, _I]),
     "kg_screen_pairs": (_I, [_P, _P, _I, _P, _P]),
     "kg_build_iterations": (_I, [_P, _P, _I, _P]),

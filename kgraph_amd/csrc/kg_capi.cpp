@@ -273,13 +273,27 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         }
         if (!h.gnode_valid) {
             h.gnode.reserve(std::max(N, 1u));
-            h.gflag.reserve(1);
-            KG_HIP(hipMemsetAsync(h.gflag.p, 0, 4, h.stream));
+            h.gflag.reserve(2);  // [0] some list repeats an id, [1] max over nodes of min(M, deg)
+            KG_HIP(hipMemsetAsync(h.gflag.p, 0, 8, h.stream));
             KG_HIP(kg_launch_gnode(N, h.goff.p, h.gids.p, h.gM.p, h.gnode.p, h.gflag.p, h.stream));
-            uint32_t dup = 0;
-            KG_HIP(hipMemcpyAsync(&dup, h.gflag.p, 4, hipMemcpyDeviceToHost, h.stream));
+            KG_HIP(kg_launch_gpad_width(N, h.goff.p, h.gM.p, h.gflag.p + 1, h.stream));
+            uint32_t fl[2] = {0, 0};
+            KG_HIP(hipMemcpyAsync(fl, h.gflag.p, 8, hipMemcpyDeviceToHost, h.stream));
             h.sync();
-            h.lists_unique = dup == 0;
+            h.lists_unique = fl[0] == 0;
+            // padded adjacency: a walk with search M <= the row width reads
+            // nothing past min(max(M[n], M), deg(n)) of a list; rows of the
+            // widest such prefix (16-id multiples) cost at most 16 GiB
+            unsigned const width = (std::max(fl[1], 1u) + 15) & ~15u;
+            h.gpad.release();
+            h.gmd.release();
+            h.gpad_stride = 0;
+            if (width < 0xffffu && size_t(N) * width * 4 <= (size_t(16) << 30)) {
+                h.gpad.reserve(size_t(N) * width);
+                h.gmd.reserve(std::max(N, 1u));
+                KG_HIP(kg_launch_gpad(N, h.goff.p, h.gids.p, h.gM.p, width, h.gpad.p, h.gmd.p, h.stream));
+                h.gpad_stride = width;
+            }
             h.gnode_valid = true;
         }
         kg_search_args a{};
@@ -293,6 +307,11 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         a.gids = h.gids.p;
         a.gM = h.gM.p;
         a.gnode = h.gnode.p;
+        bool const pad = h.search_adjacency == 0 && h.gpad_stride && sp.M <= h.gpad_stride;
+        a.gpad = pad ? h.gpad.p : nullptr;
+        a.gmd = pad ? h.gmd.p : nullptr;
+        a.gstride = pad ? h.gpad_stride : 0u;
+        h.last_adjacency = pad ? 0 : 1;
         a.lists_unique = h.lists_unique ? 1u : 0u;
         a.Q = Qk;
         a.qnorms = h.qn.p;
@@ -714,6 +733,20 @@ int kg_set_search_visited(kg_handle *k, int mode) {
     return guard(k, [&](Handle &h) {
         if (mode < 0 || mode > 2) throw kga::invalid_argument("visited-set mode must be 0, 1 or 2");
         h.search_visited = mode;
+    });
+}
+
+int kg_set_search_adjacency(kg_handle *k, int mode) {
+    return guard(k, [&](Handle &h) {
+        if (mode != 0 && mode != 1) throw kga::invalid_argument("adjacency mode must be 0 or 1");
+        h.search_adjacency = mode;
+    });
+}
+
+int kg_search_adjacency(kg_handle *k, int *padded, unsigned *width) {
+    return guard(k, [&](Handle &h) {
+        if (padded) *padded = h.last_adjacency == 0 ? 1 : 0;
+        if (width) *width = h.gpad_stride;
     });
 }
 

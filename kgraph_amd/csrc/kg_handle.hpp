@@ -96,6 +96,10 @@ struct Handle {
     // per-node walk records of the current graph (rebuilt after any change)
     DevBuf<uint4> gnode;
     DevBuf<uint32_t> gflag;
+    DevBuf<uint32_t> gpad, gmd;  // padded adjacency (kg_launch_gpad), gpad_stride ids per row
+    unsigned gpad_stride = 0;    // 0: none (too large), walks read the CSR rows
+    int search_adjacency = 0;    // 0: padded rows when they exist, 1: CSR rows
+    int last_adjacency = 1;      // what the last walk launch read (0 padded, 1 CSR)
     bool gnode_valid = false;
     bool lists_unique = false;
     std::vector<uint32_t> h_off, h_ids, h_M;

@@ -17,6 +17,12 @@ struct kg_search_args {
     uint32_t const *gids;
     uint32_t const *gM;
     uint4 const *gnode;    // per node {CSR offset, degree, M, 0} (kg_launch_gnode)
+    // padded adjacency (may be NULL): row n holds the first gstride ids of
+    // node n's list at gpad + n * gstride, gmd[n] = min(deg, 65535) << 16 | M;
+    // every id a walk reads when search M <= gstride (kg_launch_gpad)
+    uint32_t const *gpad;
+    uint32_t const *gmd;
+    unsigned gstride;
     unsigned lists_unique;  // 1: no adjacency list repeats an id
     // queries
     float const *Q;
@@ -117,6 +123,11 @@ hipError_t kg_launch_gnode(unsigned N, uint32_t const *goff, uint32_t const *gid
                            uint32_t *dupflag, hipStream_t s);
 hipError_t kg_launch_search(int metric, kg_search_args const &a, unsigned blocks, unsigned waves_per_block,
                             hipStream_t s);
+// *width = max over nodes of min(M[n], deg(n)) (atomicMax; zeroed by the caller).
+hipError_t kg_launch_gpad_width(unsigned N, uint32_t const *goff, uint32_t const *gM, uint32_t *width, hipStream_t s);
+// gpad rows (first min(deg, gstride) ids of each list) and gmd words.
+hipError_t kg_launch_gpad(unsigned N, uint32_t const *goff, uint32_t const *gids, uint32_t const *gM, unsigned gstride,
+                          uint32_t *gpad, uint32_t *gmd, hipStream_t s);
 // *flag = *host_flag = 1 if one of the first `init` ids of a row (row q at
 // ids + q*ostride) is >= N (host_flag: pinned host memory); both are left
 // alone otherwise.

@@ -76,6 +76,9 @@ struct LdsLayout {
     }
 };
 
+// Pool entry whose M is read at its first expansion (padded adjacency).
+constexpr uint32_t VT_M_UNKNOWN = 0xffffu;
+
 // actual_M (kgraph.cpp:284-286) truncated to NeighborX::M's uint16_t
 // (kgraph.cpp:79, 474, 508); also returns the node's CSR base offset.
 // One 16-byte load: gnode[id] = {CSR offset, degree, M, 0}.
@@ -507,8 +510,8 @@ search_kernel(kg_search_args a) {
             }
             wave_sync();
             for (unsigned i = lane; i < L; i += KG_WAVE) {
-                uint32_t bs;
-                uint32_t m = actual_M(a, w.pid[i] & IDMASK, &bs);
+                uint32_t bs = 0;
+                uint32_t const m = a.gpad ? VT_M_UNKNOWN : actual_M(a, w.pid[i] & IDMASK, &bs);
                 w.pmeta[i] = m << 16;
                 w.pbase[i] = bs;
             }
@@ -530,8 +533,23 @@ search_kernel(kg_search_args a) {
                 k = kk;
                 uint32_t const eid = w.pid[k];
                 uint32_t const meta = w.pmeta[k];
-                uint32_t const ebase = w.pbase[k];
-                unsigned const beginM = meta & 0xffffu, eM = meta >> 16;
+                unsigned const beginM = meta & 0xffffu;
+                unsigned eM = meta >> 16;
+                // the node's list: its CSR row, or its padded row, whose
+                // address needs nothing but the id, so its first ids load
+                // together with its M and degree
+                uint32_t const *erow =
+                    a.gpad ? a.gpad + size_t(eid & IDMASK) * a.gstride : a.gids + w.pbase[k];
+                uint32_t pre = 0xffffffffu;
+                bool const first_pad = a.gpad && eM == VT_M_UNKNOWN;
+                if (first_pad) {
+                    uint32_t const md = a.gmd[eid & IDMASK];
+                    if (lane < min(a.S, unsigned(KG_WAVE)) && beginM + lane < a.gstride) pre = erow[beginM + lane];
+                    uint32_t m = md & 0xffffu, deg = md >> 16;
+                    if (m < a.Mparam) m = a.Mparam;
+                    if (m > deg) m = deg;
+                    eM = m & 0xffffu;
+                }
                 unsigned endM = beginM + a.S;
                 bool const done = endM > eM;
                 if (done) endM = eM;
@@ -544,7 +562,8 @@ search_kernel(kg_search_args a) {
                 if (endM > beginM) n_scan += endM - beginM;
                 for (unsigned mc = beginM; mc < endM; mc += KG_WAVE) {
                     unsigned const cnt = min(endM - mc, unsigned(KG_WAVE));
-                    uint32_t id = (lane < cnt) ? a.gids[ebase + mc + lane] : 0xffffffffu;
+                    uint32_t id = 0xffffffffu;
+                    if (lane < cnt) id = (first_pad && mc == beginM) ? pre : erow[mc + lane];
                     bool dup = false;
                     if (!a.lists_unique) {  // a list may repeat an id: keep its first occurrence
                         for (unsigned j = 0; j < cnt; ++j) {
@@ -588,8 +607,8 @@ search_kernel(kg_search_args a) {
                     if (!glob && 2 * (vcount + KG_WAVE) > a.vcap) spill();
                     if (nf == 0) continue;
                     // lane j fetches candidate j's M and CSR base while the rows load
-                    uint32_t cm = 0, cb = 0;
-                    if (lane < nf) cm = actual_M(a, w.fid[lane], &cb);
+                    uint32_t cm = VT_M_UNKNOWN, cb = 0;
+                    if (lane < nf && !a.gpad) cm = actual_M(a, w.fid[lane], &cb);
                     batch_dist<METRIC, RowT>(a, w.fid, w.fdist, nf, w.q, qn);
                     // the nf inserts of this batch, done at once (batch_insert)
                     n_comps += nf;
@@ -702,6 +721,29 @@ __global__ void __launch_bounds__(256) gnode_kernel(unsigned N, uint32_t const *
     if (ballot(dup) && lane == 0) atomicOr(dupflag, 1u);
 }
 
+__global__ void __launch_bounds__(256) gpad_width_kernel(unsigned N, uint32_t const *goff, uint32_t const *gM,
+                                                         uint32_t *width) {
+    unsigned mx = 0;
+    for (unsigned n = blockIdx.x * 256 + threadIdx.x; n < N; n += gridDim.x * 256)
+        mx = max(mx, min(gM[n], goff[n + 1] - goff[n]));
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, unsigned(__shfl_xor(int(mx), o)));
+    if ((threadIdx.x & 63) == 0 && mx) atomicMax(width, mx);
+}
+
+// one wave per node: the first min(deg, gstride) ids, coalesced
+__global__ void __launch_bounds__(256) gpad_kernel(unsigned N, uint32_t const *goff, uint32_t const *gids,
+                                                   uint32_t const *gM, unsigned gstride, uint32_t *gpad,
+                                                   uint32_t *gmd) {
+    unsigned const w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    unsigned const n = blockIdx.x * 4 + w;
+    if (n >= N) return;
+    uint32_t const b = goff[n], deg = goff[n + 1] - b;
+    unsigned const cnt = min(deg, gstride);
+    uint32_t *row = gpad + size_t(n) * gstride;
+    for (unsigned j = lane; j < cnt; j += KG_WAVE) row[j] = gids[b + j];
+    if (lane == 0) gmd[n] = (min(deg, 0xffffu) << 16) | min(gM[n], 0xffffu);
+}
+
 __global__ void __launch_bounds__(256) init_check_kernel(uint32_t const *ids, unsigned nq, size_t ostride,
                                                          unsigned init, unsigned N, uint32_t *flag,
                                                          uint32_t *host_flag) {
@@ -718,6 +760,19 @@ __global__ void __launch_bounds__(256) init_check_kernel(uint32_t const *ids, un
 }
 
 }  // namespace
+
+hipError_t kg_launch_gpad_width(unsigned N, uint32_t const *goff, uint32_t const *gM, uint32_t *width, hipStream_t s) {
+    if (N == 0) return hipSuccess;
+    hipLaunchKernelGGL(gpad_width_kernel, dim3(std::min((N + 255) / 256, 1024u)), dim3(256), 0, s, N, goff, gM, width);
+    return hipGetLastError();
+}
+
+hipError_t kg_launch_gpad(unsigned N, uint32_t const *goff, uint32_t const *gids, uint32_t const *gM, unsigned gstride,
+                          uint32_t *gpad, uint32_t *gmd, hipStream_t s) {
+    if (N == 0) return hipSuccess;
+    hipLaunchKernelGGL(gpad_kernel, dim3((N + 3) / 4), dim3(256), 0, s, N, goff, gids, gM, gstride, gpad, gmd);
+    return hipGetLastError();
+}
 
 hipError_t kg_launch_init_check(uint32_t const *ids, unsigned nq, size_t ostride, unsigned init, unsigned N,
                                 uint32_t *flag, uint32_t *host_flag, hipStream_t s) {

@@ -158,6 +158,37 @@ def test_lds_visited_set_gives_identical_walks(P, S, T):
     assert np.array_equal(r[3], comps) and np.array_equal(r[4], scans)
 
 
+@pytest.mark.parametrize("P,S,M,T", [(40, 10, 0, 1), (10, 64, 0, 1), (100, 80, 0, 1), (20, 10, 30, 3),
+                                     (30, 10, 1000, 1)])
+def test_padded_adjacency_gives_identical_walks(P, S, M, T):
+    # walks reading the padded rows (a node's ids and M loaded at its first
+    # expansion) and the CSR lists with per-candidate records return the
+    # oracle's ids, distances and counters; a search M above the row width
+    # reads the CSR lists
+    import ctypes
+    X, Q, idx = setup("intmix", L2, d=128)
+    g = gpu_index(X, idx, L2)
+    res = {}
+    for mode in (0, 1):
+        g._h.call("kg_set_search_adjacency", mode)
+        gi, gd = g.search(Q, K=10, P=P, S=S, M=M, T=T, withDistance=True, stats=True)
+        padded, width = ctypes.c_int(-1), ctypes.c_uint()
+        g._h.call("kg_search_adjacency", ctypes.byref(padded), ctypes.byref(width))
+        res[mode] = (gi, gd, g.last_counts.copy(), g.last_comps.copy(), g.last_scans.copy(), padded.value,
+                     width.value)
+    g._h.call("kg_set_search_adjacency", 0)
+    assert res[0][6] >= 16 and res[0][6] % 16 == 0
+    assert res[0][5] == int(M <= res[0][6]) and res[1][5] == 0
+    for a, b in zip(res[0][:5], res[1][:5]):
+        assert np.array_equal(a, b)
+    ids, dd, c, comps, scans = idx.search(X, Q, L2, K=10, P=P, S=S, M=M, T=T)
+    assert np.array_equal(res[0][2], c)
+    for q in range(len(Q)):
+        assert np.array_equal(res[0][0][q, :c[q]], ids[q, :c[q]]), q
+        assert np.array_equal(res[0][1][q, :c[q]], dd[q, :c[q]]), q
+    assert np.array_equal(res[0][3], comps) and np.array_equal(res[0][4], scans)
+
+
 def test_compact_visited_set_on_a_larger_index():
     # 300,000 nodes (19-bit ids): 16-bit entries keep 9 remainder bits at
     # 1024 entries (displacements up to 126); a 256-entry set would need 11

@@ -294,8 +294,13 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     // radius) and drifts as the lists converge.  Both forms admit exactly the
     // same candidates, so in auto mode a probe iteration (every third, from the
     // first) splits the nodes between the two forms, times both parts, and the
-    // next iterations use the faster per node.  The first probe splits in
-    // halves; later ones give the current form all but the last eighth.
+    // next iterations use the screen when it is at least 5 % faster per node
+    // (its cost per node grows as the lists converge: on the 1.2M x 96 angular
+    // set it beats the tiles by a third in the first iterations, ties at the
+    // fourth and is 15-35 % slower after, so a tie goes to the tiles).  Every
+    // probe splits the nodes in halves: a launch over an eighth of them pays a
+    // larger share of fixed and tail costs per node, which tilted the
+    // comparison (on the angular set it kept the screen through a tie).
     NndArgs a_tile = a;
     a_tile.mfma = 0;
     bool const screen = a.mfma && !a.exact;
@@ -309,7 +314,7 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
         bool const probe = screen && form == KG_JOIN_AUTO && it % 3 == 0 && eighth > 0;
         // probe split: [0, cut) runs form `lead`, [cut, N) the other form
         bool const lead = it == 0 ? true : use_mfma;
-        unsigned const cut = it == 0 ? half : N - eighth;
+        unsigned const cut = half;
         NndArgs const &ar = use_mfma ? a : a_tile;
         int rec_form = probe ? 3 : (ar.mfma ? (ar.exact ? 2 : 1) : 0);
         float jms = 0.f;
@@ -373,7 +378,7 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
             KG_HIP(hipEventElapsedTime(&t_second, h.ev[18], h.ev[7]));
             double const c_first = double(t_first) / cut, c_second = double(t_second) / (N - cut);  // per node
             double const c_screen = lead ? c_first : c_second, c_tile = lead ? c_second : c_first;
-            use_mfma = c_screen <= c_tile;
+            use_mfma = c_screen < 0.95 * c_tile;
         }
         // statistics and stop rule (kgraph.cpp:927-972)
         n_comps += stats[NND_STAT_PAIRS];

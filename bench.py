@@ -243,8 +243,8 @@ def main():
     for r in res:
         rec = recall_at_k(r["dists"], gt_d, K)
         qps = nq / r["wall_s"]
-        # algorithmic bytes per query: a row (rb*D: 4 per value, 2 when walks
-        # gather the lossless BF16 copy) per scored node (starting points +
+        # algorithmic bytes per query: a row (rb*D: 4 per value, 2 or 1 when walks
+        # gather the lossless BF16 or 8-bit copy) per scored node (starting points +
         # comps), 4 bytes per neighbour id read, 4 per starting id
         byt = float((rb * D * (r["seeds"] + r["comps"].astype(np.float64)) + 4 * r["scans"] + 4 * r["seeds"]).sum())
         sweep.append(dict(mode=r["mode"], P=r["P"], S=r["S"], E=r["E"], qps=qps, recall=rec, kernel_ms=r["kernel_ms"],
@@ -269,19 +269,25 @@ def main():
                                      "p99": round(float(np.percentile(lat, 99)), 1),
                                      "max": round(float(lat.max()), 1)}}
     log("picked setting: %s" % search_time)
-    # the picked setting again with the walk gathering the FP32 rows instead of
-    # the lossless BF16 copy (same results)
+    # the picked setting again with the walk gathering the other row forms
+    # (the FP32 rows, and the BF16 copy when the 8-bit one was used): same results
     row_cmp = None
-    if rows == "bf16":
-        h.call("kg_set_row_storage", 1)
-        r32 = search_once(pick["mode"], pick["P"], pick["S"], pick["E"])
+    if rows != "fp32":
+        row_cmp = {"%s_rows_walk_ms" % rows: round(pick["kernel_ms"], 4)}
+        same = True
+        for mode, name in ((1, "fp32"), (2, "bf16")):
+            if name == rows or (name == "bf16" and rows != "u8"):
+                continue
+            h.call("kg_set_row_storage", mode)
+            r = search_once(pick["mode"], pick["P"], pick["S"], pick["E"])
+            same &= bool(np.array_equal(r["ids"], res[sweep.index(pick)]["ids"]) and
+                         np.array_equal(r["dists"], res[sweep.index(pick)]["dists"]))
+            row_cmp.update({"%s_rows_qps" % name: round(nq / r["wall_s"], 1),
+                            "%s_rows_walk_ms" % name: round(r["kernel_ms"], 4)})
+            log("%s rows: QPS=%.0f walk %.2fms (%s rows %.2fms)" % (name, nq / r["wall_s"], r["kernel_ms"], rows,
+                                                                    pick["kernel_ms"]))
         h.call("kg_set_row_storage", 0)
-        same = bool(np.array_equal(r32["ids"], res[sweep.index(pick)]["ids"]) and
-                    np.array_equal(r32["dists"], res[sweep.index(pick)]["dists"]))
-        row_cmp = {"fp32_rows_qps": round(nq / r32["wall_s"], 1), "fp32_rows_walk_ms": round(r32["kernel_ms"], 4),
-                   "bf16_rows_walk_ms": round(pick["kernel_ms"], 4), "identical_results": same}
-        log("fp32 rows: QPS=%.0f walk %.2fms (bf16 rows %.2fms), identical results: %s" % (
-            nq / r32["wall_s"], r32["kernel_ms"], pick["kernel_ms"], same))
+        row_cmp["identical_results"] = same
     # join kernel, algorithmic bytes: every join-list row once (2 bytes per
     # value: the MFMA form's operands are the BF16 copy) + its id (4), and a
     # radius probe at both endpoints of every pair (8)
@@ -765,13 +771,13 @@ def learned_workload(args):
 
 
 def row_storage(h):
-    """('bf16', 2) when graph walks gather the lossless BF16 copy of the rows,
-    else ('fp32', 4): the row format and its bytes per value; and whether a
-    lossless BF16 copy exists."""
+    """The rows graph walks gather and their bytes per value: ('u8', 1) for
+    the 8-bit copy, ('bf16', 2) for the lossless BF16 copy, ('fp32', 4); and
+    whether a lossless compact copy exists."""
     from kgraph_amd import _lib
     v, e = _lib.ctypes.c_int(), _lib.ctypes.c_int()
     h.call("kg_row_storage", _lib.ctypes.byref(v), _lib.ctypes.byref(e))
-    return (("bf16", 2) if v.value else ("fp32", 4)) + (bool(e.value),)
+    return {2: ("u8", 1), 1: ("bf16", 2)}.get(v.value, ("fp32", 4)) + (bool(e.value),)
 
 
 def measured_traffic(kernel, workload, pick, rows):

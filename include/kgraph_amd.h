@@ -139,12 +139,16 @@ int kg_search_adjacency(kg_handle *h, int *padded, unsigned *width);
 
 /* Row storage for graph walks (no counterpart in the reference, whose rows are
  * the caller's Matrix, kgraph-data.h:115-175).  kg_set_data keeps a BF16 copy
- * of the rows (the MFMA kernels' operands); when every value is representable
- * in BF16 the copy is lossless, and with mode 0 (default) walks gather it: half
- * the bytes, so a wave holds twice the rows in flight, with the same distances
- * and ids.  Mode 1 gathers the FP32 rows.  kg_row_storage reports *active = 1
- * when walks gather the BF16 copy and *lossless = 1 when the copy is lossless
- * (either may be NULL). */
+ * of the rows (the MFMA kernels' operands), lossless when every value is
+ * representable in BF16, and an 8-bit copy when every value is an integer in
+ * [0, 255].  Mode 0 (default): walks gather the 8-bit copy when there is one
+ * (a quarter of the bytes: one 128-byte line per 128-value row), else the
+ * lossless BF16 copy (half), else the FP32 rows; each holds the same values,
+ * so the distances and ids are the same.  Mode 1 gathers the FP32 rows, mode
+ * 2 the BF16 copy when lossless.  kg_row_storage reports *active = 0 (FP32
+ * rows), 1 (BF16 copy) or 2 (8-bit copy) for the next walk, and *lossless
+ * with bit 0 set when the BF16 copy is lossless and bit 1 when the 8-bit copy
+ * exists (either pointer may be NULL). */
 int kg_set_row_storage(kg_handle *h, int mode);
 int kg_row_storage(kg_handle *h, int *active, int *lossless);
 

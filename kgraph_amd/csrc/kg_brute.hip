@@ -162,6 +162,24 @@ __global__ void __launch_bounds__(256) norms_kernel(float const *X, unsigned N, 
     }
 }
 
+__global__ void __launch_bounds__(256) u8_rows_kernel(float4 const *X, size_t n4, uint32_t *X8, uint32_t *inexact) {
+    bool lossy = false;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n4; i += size_t(gridDim.x) * blockDim.x) {
+        float4 const v = X[i];
+        float const e[4] = {v.x, v.y, v.z, v.w};
+        uint32_t w = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            bool const in = e[j] >= 0.f && e[j] <= 255.f;
+            uint32_t const u = in ? uint32_t(e[j]) : 0u;
+            lossy |= !in || __float_as_uint(float(u)) != __float_as_uint(e[j]);
+            w |= u << (8 * j);
+        }
+        X8[i] = w;
+    }
+    if (lossy) *inexact = 1u;  // every writer stores the same value
+}
+
 __global__ void __launch_bounds__(256) bf16_rows_kernel(float4 const *X, size_t n4, uint2 *Xb, uint32_t *inexact) {
     bool lossy = false;
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n4; i += size_t(gridDim.x) * blockDim.x) {
@@ -206,6 +224,15 @@ hipError_t kg_launch_norms(int metric, float const *X, unsigned N, unsigned D, u
         case METRIC_ANGULAR: hipLaunchKernelGGL(norms_kernel<METRIC_ANGULAR>, grid, block, 0, s, X, N, D, Dp, out); break;
         default: return hipSuccess;
     }
+    return hipGetLastError();
+}
+
+hipError_t kg_launch_u8_rows(float const *X, unsigned N, unsigned Dp, uint8_t *X8, uint32_t *inexact, hipStream_t s) {
+    size_t const n4 = size_t(N) * Dp / 4;
+    if (n4 == 0) return hipSuccess;
+    unsigned const nb = unsigned(std::min<size_t>(4096, (n4 + 255) / 256));
+    hipLaunchKernelGGL(u8_rows_kernel, dim3(nb), dim3(256), 0, s, reinterpret_cast<float4 const *>(X), n4,
+                       reinterpret_cast<uint32_t *>(X8), inexact);
     return hipGetLastError();
 }
 

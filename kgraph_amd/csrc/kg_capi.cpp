@@ -108,9 +108,25 @@ static void set_data(Handle &h, float const *X, bool dev, unsigned N, unsigned D
     KG_HIP(kg_launch_bf16_rows(h.X.p, N, h.Dp, h.Xb.p, inexact.p, h.stream));
     uint32_t lossy = 1;
     KG_HIP(hipMemcpyAsync(&lossy, inexact.p, 4, hipMemcpyDeviceToHost, h.stream));
+    h.X8.reserve(std::max<size_t>(size_t(N) * h.Dp, 4));
+    DevBuf<uint32_t> inexact8;
+    inexact8.reserve(1);
+    KG_HIP(hipMemsetAsync(inexact8.p, 0, 4, h.stream));
+    KG_HIP(kg_launch_u8_rows(h.X.p, N, h.Dp, h.X8.p, inexact8.p, h.stream));
+    uint32_t lossy8 = 1;
+    KG_HIP(hipMemcpyAsync(&lossy8, inexact8.p, 4, hipMemcpyDeviceToHost, h.stream));
     h.sync();
     h.rows_bf16 = lossy == 0;
+    h.rows_u8 = lossy8 == 0;
+    if (!h.rows_u8) h.X8.release();  // kept only when it is exact
     h.has_data = true;
+}
+
+// Rows a walk gathers: 2 the 8-bit copy, 1 the BF16 copy, 0 the FP32 rows.
+static int walk_rows(Handle const &h) {
+    if (!h.has_data || h.row_storage == 1) return 0;
+    if (h.row_storage == 0 && h.rows_u8) return 2;
+    return h.rows_bf16 ? 1 : 0;
 }
 
 // Batched search over device-resident queries / outputs (kgraph.cpp:423-564).
@@ -298,7 +314,9 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         }
         kg_search_args a{};
         a.X = h.X.p;
-        a.Xb = (h.rows_bf16 && h.row_storage == 0) ? h.Xb.p : nullptr;
+        int const form = walk_rows(h);
+        a.X8 = form == 2 ? h.X8.p : nullptr;
+        a.Xb = form == 1 ? h.Xb.p : nullptr;
         a.norms = h.norms.p;
         a.N = N;
         a.D = h.D;
@@ -779,15 +797,15 @@ int kg_search_visited_stats(kg_handle *k, unsigned *vcap, unsigned *spilled) {
 
 int kg_set_row_storage(kg_handle *k, int mode) {
     return guard(k, [&](Handle &h) {
-        if (mode != 0 && mode != 1) throw kga::invalid_argument("row storage mode must be 0 or 1");
+        if (mode < 0 || mode > 2) throw kga::invalid_argument("row storage mode must be 0, 1 or 2");
         h.row_storage = mode;
     });
 }
 
 int kg_row_storage(kg_handle *k, int *active, int *lossless) {
     return guard(k, [&](Handle &h) {
-        if (active) *active = (h.has_data && h.rows_bf16 && h.row_storage == 0) ? 1 : 0;
-        if (lossless) *lossless = (h.has_data && h.rows_bf16) ? 1 : 0;
+        if (active) *active = walk_rows(h);
+        if (lossless) *lossless = h.has_data ? (int(h.rows_bf16) | (int(h.rows_u8) << 1)) : 0;
     });
 }
 

@@ -91,6 +91,7 @@ __device__ __forceinline__ float4 load_chunk(uint16_t const *r, unsigned c) {
                        __uint_as_float(v.y & 0xffff0000u));
 }
 __device__ __forceinline__ float load_elem(float const *r, unsigned i) { return r[i]; }
+__device__ __forceinline__ float load_elem(uint8_t const *r, unsigned i) { return float(r[i]); }
 __device__ __forceinline__ float load_elem(uint16_t const *r, unsigned i) { return __uint_as_float(uint32_t(r[i]) << 16); }
 
 // One quad computes the canonical SSD / dot of data row `a` (global) and
@@ -195,6 +196,69 @@ __device__ __forceinline__ float2 quad_metric2_bf16(uint16_t const *__restrict__
         float4 const y = b4[k];
         step(p0, widen_bf16x4(r0[k]), y);
         step(p1, widen_bf16x4(r1[k]), y);
+    }
+    float acc0 = quad_reduce(p0), acc1 = quad_reduce(p1);
+    for (unsigned i = D & ~15u; i < D; ++i) {
+        float const e0 = load_elem(a0, i), e1 = load_elem(a1, i);
+        if (METRIC == METRIC_L2SQR) {
+            float const d0 = e0 - b[i], d1 = e1 - b[i];
+            acc0 = __builtin_fmaf(d0, d0, acc0);
+            acc1 = __builtin_fmaf(d1, d1, acc1);
+        } else {
+            acc0 = __builtin_fmaf(e0, b[i], acc0);
+            acc1 = __builtin_fmaf(e1, b[i], acc1);
+        }
+    }
+    return make_float2(acc0, acc1);
+}
+
+__device__ __forceinline__ float4 widen_u8x4(uint32_t v) {
+    return make_float4(float(v & 0xffu), float((v >> 8) & 0xffu), float((v >> 16) & 0xffu), float(v >> 24));
+}
+
+// Two rows of the lossless 8-bit copy per quad: one 4-byte word per 4-value
+// chunk, widened at use; each result equals quad_metric's on the FP32 rows
+// (the same values in the same order).
+template <int METRIC>
+__device__ __forceinline__ float2 quad_metric2_u8(uint8_t const *__restrict__ a0, uint8_t const *__restrict__ a1,
+                                                  float const *__restrict__ b, unsigned D, unsigned t) {
+    unsigned const nch = (D & ~15u) >> 2;
+    float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f), p1 = p0;
+    float4 const *b4 = reinterpret_cast<float4 const *>(b);
+    uint32_t const *r0 = reinterpret_cast<uint32_t const *>(a0);
+    uint32_t const *r1 = reinterpret_cast<uint32_t const *>(a1);
+    auto step = [](float4 &p, float4 x, float4 y) {
+        if (METRIC == METRIC_L2SQR) {
+            float dx = x.x - y.x, dy = x.y - y.y, dz = x.z - y.z, dw = x.w - y.w;
+            p.x = __builtin_fmaf(dx, dx, p.x);
+            p.y = __builtin_fmaf(dy, dy, p.y);
+            p.z = __builtin_fmaf(dz, dz, p.z);
+            p.w = __builtin_fmaf(dw, dw, p.w);
+        } else {
+            p.x = __builtin_fmaf(x.x, y.x, p.x);
+            p.y = __builtin_fmaf(x.y, y.y, p.y);
+            p.z = __builtin_fmaf(x.z, y.z, p.z);
+            p.w = __builtin_fmaf(x.w, y.w, p.w);
+        }
+    };
+    unsigned k = t;
+    for (; k + 4 * (QM_GROUP - 1) < nch; k += 4 * QM_GROUP) {
+        uint32_t x0[QM_GROUP], x1[QM_GROUP];
+#pragma unroll
+        for (int g = 0; g < QM_GROUP; ++g) x0[g] = r0[k + 4 * g];
+#pragma unroll
+        for (int g = 0; g < QM_GROUP; ++g) x1[g] = r1[k + 4 * g];
+#pragma unroll
+        for (int g = 0; g < QM_GROUP; ++g) {
+            float4 const y = b4[k + 4 * g];
+            step(p0, widen_u8x4(x0[g]), y);
+            step(p1, widen_u8x4(x1[g]), y);
+        }
+    }
+    for (; k < nch; k += 4) {
+        float4 const y = b4[k];
+        step(p0, widen_u8x4(r0[k]), y);
+        step(p1, widen_u8x4(r1[k]), y);
     }
     float acc0 = quad_reduce(p0), acc1 = quad_reduce(p1);
     for (unsigned i = D & ~15u; i < D; ++i) {

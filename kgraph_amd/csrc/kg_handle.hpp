@@ -77,7 +77,13 @@ struct Handle {
     // value representable) graph walks gather it instead of X
     DevBuf<uint16_t> Xb;
     bool rows_bf16 = false;
-    int row_storage = 0;  // 0: walks gather the lossless BF16 copy when there is one, 1: FP32 rows
+    // 8-bit copy (row stride Dp bytes), kept when every value is an integer in
+    // [0, 255]; walks gather it first
+    DevBuf<uint8_t> X8;
+    bool rows_u8 = false;
+    // 0: walks gather the 8-bit copy, else the BF16 copy, when lossless; 1: FP32
+    // rows; 2: the BF16 copy when lossless (not the 8-bit one)
+    int row_storage = 0;
     int search_visited = 0;  // 0: LDS visited hash set in front of the HBM bitset, 1: bitset only, 2: 32-bit set entries
     unsigned search_vcap = 0;  // LDS set entries (0: chosen per launch)
     unsigned last_vt_bits = 0;

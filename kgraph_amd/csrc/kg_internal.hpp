@@ -10,6 +10,7 @@ struct kg_search_args {
     // data rows (row stride Dp floats, 16-byte aligned) and per-row norms
     float const *X;
     uint16_t const *Xb;  // non-NULL: lossless BF16 copy of X (same stride), gathered instead of X
+    uint8_t const *X8;   // non-NULL: lossless 8-bit copy of X (row stride Dp bytes), gathered first
     float const *norms;
     unsigned N, D, Dp;
     // graph in CSR form
@@ -156,6 +157,9 @@ hipError_t kg_launch_gather_rows(float const *X, unsigned Dp, uint32_t const *id
                                  hipStream_t s);
 // BF16 copy of N x Dp rows (Dp % 4 == 0); *inexact = 1 when a value is not
 // representable in BF16 (the caller clears it first)
+// X8 = the rows as bytes; *inexact = 1 if a value is not an integer in
+// [0, 255] with the bits of its byte's float (caller zeroes it).
+hipError_t kg_launch_u8_rows(float const *X, unsigned N, unsigned Dp, uint8_t *X8, uint32_t *inexact, hipStream_t s);
 hipError_t kg_launch_bf16_rows(float const *X, unsigned N, unsigned Dp, uint16_t *Xb, uint32_t *inexact,
                                hipStream_t s);
 hipError_t kg_launch_norms(int metric, float const *X, unsigned N, unsigned D, unsigned Dp, float *out,

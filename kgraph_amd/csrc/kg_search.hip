@@ -92,19 +92,38 @@ __device__ __forceinline__ uint32_t actual_M(kg_search_args const &a, uint32_t i
 }
 
 // Distances of ids[0..cnt) (LDS, flag bit ignored) -> out[0..cnt) (LDS).
-// Rows are gathered from a.X (RowT = float) or from the lossless BF16 copy
-// a.Xb (RowT = uint16_t); both give the same distances.
+// Rows are gathered from a.X (RowT = float), from the lossless BF16 copy a.Xb
+// (RowT = uint16_t) or from the lossless 8-bit copy a.X8 (RowT = uint8_t);
+// all give the same distances.
 template <int METRIC, typename RowT>
 __device__ __forceinline__ void batch_dist(kg_search_args const &a, uint32_t const *ids, float *out, unsigned cnt,
                                            float const *q, float qn) {
     RowT const *X;
-    if constexpr (sizeof(RowT) == 2)
+    if constexpr (sizeof(RowT) == 1)
+        X = a.X8;
+    else if constexpr (sizeof(RowT) == 2)
         X = a.Xb;
     else
         X = a.X;
     unsigned const lane = lane_id();
     unsigned const grp = lane >> 2, t = lane & 3;
-    if constexpr (sizeof(RowT) == 2) {
+    if constexpr (sizeof(RowT) == 1) {
+        // 8-bit rows: one 128-byte line per 128-value row, two rows per quad
+        for (unsigned b0 = 0; b0 < cnt; b0 += 32) {
+            unsigned const c0 = b0 + grp, c1 = c0 + 16;
+            if (c0 < cnt) {  // whole quads are active together
+                bool const two = c1 < cnt;
+                uint32_t const id0 = ids[c0] & IDMASK, id1 = two ? (ids[c1] & IDMASK) : id0;
+                float2 const acc = quad_metric2_u8<METRIC>(X + size_t(id0) * a.Dp, X + size_t(id1) * a.Dp, q, a.D, t);
+                float const n0 = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[id0];
+                float const n1 = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[id1];
+                if (t == 0) {
+                    out[c0] = finish_metric<METRIC>(acc.x, n0, qn);
+                    if (two) out[c1] = finish_metric<METRIC>(acc.y, n1, qn);
+                }
+            }
+        }
+    } else if constexpr (sizeof(RowT) == 2) {
         // BF16 rows are half the bytes: each quad takes two rows a pass, 32 per
         // pass, in the registers 16 FP32 rows need
         for (unsigned b0 = 0; b0 < cnt; b0 += 32) {
@@ -805,25 +824,27 @@ hipError_t kg_launch_search(int metric, kg_search_args const &a, unsigned blocks
     if (waves_per_block == 0 || waves_per_block > SEARCH_MAX_WPB || blocks == 0) return hipErrorInvalidValue;
     size_t lds = size_t(waves_per_block) * a.lds_per_wave;
     dim3 grid(blocks), block(waves_per_block * KG_WAVE);
-    bool const bf = a.Xb != nullptr;
+    // row form: the 8-bit copy, the BF16 copy, or the FP32 rows
+    auto go = [&](auto kernel8, auto kernel16, auto kernel32) {
+        if (a.X8)
+            hipLaunchKernelGGL(kernel8, grid, block, lds, s, a);
+        else if (a.Xb)
+            hipLaunchKernelGGL(kernel16, grid, block, lds, s, a);
+        else
+            hipLaunchKernelGGL(kernel32, grid, block, lds, s, a);
+    };
     switch (metric) {
         case METRIC_L2SQR:
-            if (bf)
-                hipLaunchKernelGGL((search_kernel<METRIC_L2SQR, uint16_t>), grid, block, lds, s, a);
-            else
-                hipLaunchKernelGGL((search_kernel<METRIC_L2SQR, float>), grid, block, lds, s, a);
+            go(search_kernel<METRIC_L2SQR, uint8_t>, search_kernel<METRIC_L2SQR, uint16_t>,
+               search_kernel<METRIC_L2SQR, float>);
             break;
         case METRIC_EUCLIDEAN:
-            if (bf)
-                hipLaunchKernelGGL((search_kernel<METRIC_EUCLIDEAN, uint16_t>), grid, block, lds, s, a);
-            else
-                hipLaunchKernelGGL((search_kernel<METRIC_EUCLIDEAN, float>), grid, block, lds, s, a);
+            go(search_kernel<METRIC_EUCLIDEAN, uint8_t>, search_kernel<METRIC_EUCLIDEAN, uint16_t>,
+               search_kernel<METRIC_EUCLIDEAN, float>);
             break;
         default:
-            if (bf)
-                hipLaunchKernelGGL((search_kernel<METRIC_ANGULAR, uint16_t>), grid, block, lds, s, a);
-            else
-                hipLaunchKernelGGL((search_kernel<METRIC_ANGULAR, float>), grid, block, lds, s, a);
+            go(search_kernel<METRIC_ANGULAR, uint8_t>, search_kernel<METRIC_ANGULAR, uint16_t>,
+               search_kernel<METRIC_ANGULAR, float>);
             break;
     }
     return hipGetLastError();

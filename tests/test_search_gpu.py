@@ -90,25 +90,28 @@ def test_search_parity_epsilon():
 
 @pytest.mark.parametrize("kind,metric,d,lossless", [("intmix", L2, 37, True), ("intmix", ANG, 48, True),
                                                      ("intmix", EUC, 128, True), ("uniform", L2, 16, False)])
-def test_bf16_row_storage_gives_identical_walks(kind, metric, d, lossless):
-    # integer rows in [0, 255] are exact in BF16: walks may gather the BF16
-    # copy (half the bytes) and must return the oracle's ids, distances and
-    # counters; uniform floats are not, and walks keep the FP32 rows
+def test_row_storage_forms_give_identical_walks(kind, metric, d, lossless):
+    # integer rows in [0, 255] have an 8-bit copy and a lossless BF16 copy:
+    # walks gather the 8-bit one by default (mode 2: the BF16 one) and must
+    # return the oracle's ids, distances and counters; uniform floats have
+    # neither, and every mode gathers the FP32 rows
     import ctypes
     X, Q, idx = setup(kind, metric, d=d)
     g = gpu_index(X, idx, metric)
     active, exact = ctypes.c_int(-1), ctypes.c_int(-1)
     g._h.call("kg_row_storage", ctypes.byref(active), ctypes.byref(exact))
-    assert active.value == int(lossless) and exact.value == int(lossless)  # the BF16 copy when lossless
+    assert active.value == (2 if lossless else 0) and exact.value == (3 if lossless else 0)
     res = {}
-    for mode in (0, 1):
+    for mode, want in ((0, 2), (1, 0), (2, 1)):
         g._h.call("kg_set_row_storage", mode)
         g._h.call("kg_row_storage", ctypes.byref(active), None)
-        assert active.value == int(lossless and mode == 0)
+        assert active.value == (want if lossless else 0)
         gi, gd = g.search(Q, K=10, P=40, withDistance=True, stats=True)
         res[mode] = (gi, gd, g.last_counts.copy(), g.last_comps.copy(), g.last_scans.copy())
-    for a, b in zip(res[0], res[1]):
-        assert np.array_equal(a, b)
+    g._h.call("kg_set_row_storage", 0)
+    for mode in (1, 2):
+        for a, b in zip(res[0], res[mode]):
+            assert np.array_equal(a, b)
     ids, dd, c, comps, scans = idx.search(X, Q, metric, K=10, P=40)
     assert np.array_equal(res[0][2], c)
     for q in range(len(Q)):

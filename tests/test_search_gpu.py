@@ -335,7 +335,9 @@ def _entry_sample_ids(seed, E, N):
 @pytest.mark.parametrize("kind,metric,d,m,E", [("intmix", L2, 128, 7, 512), ("uniform", EUC, 16, 7, 512),
                                                ("angular", ANG, 96, 7, 512), ("intmix", EUC, 64, 9, 512),
                                                ("intmix", ANG, 32, 16, 512), ("intmix", L2, 24, 3, 512),
-                                               ("intmix", L2, 128, 16, 64), ("intmix", EUC, 64, 12, 128)])
+                                               ("intmix", L2, 128, 16, 64), ("intmix", EUC, 64, 12, 128),
+                                               ("intmix", L2, 128, 9, 100), ("intmix", L2, 128, 9, 20),
+                                               ("uniform", L2, 16, 9, 100)])
 def test_entry_selection_parity(kind, metric, d, m, E):
     X, Q, idx = setup(kind, metric, d=d)
     g = gpu_index(X, idx, metric)

@@ -8,6 +8,7 @@
 #include <memory>
 #include <random>
 
+#include "kg_exact.hpp"
 #include "kg_handle.hpp"
 #include "kg_internal.hpp"
 #include "kg_nnd.hpp"
@@ -77,6 +78,35 @@ void screen_constants(unsigned D, NndArgs &a) {
     double const gam = (D + 16) * std::ldexp(1.0, -23);
     a.scr_c = float((std::ldexp(1.0, -7) + std::ldexp(1.0, -16) + gam) * 1.02);
     a.scr_g = float(4 * gam);
+}
+
+// Row norms for the MFMA forms (into sqn) and the value range of the rows.
+struct RowRange {
+    bool integer;
+    double maxabs, vmin, vmax;
+};
+RowRange row_range(Handle &h, NndArgs const &a, float2 *sqn, DevBuf<uint32_t> &flags, hipStream_t s) {
+    flags.reserve(4);
+    uint32_t const init[4] = {0u, 0u, 0xffffffffu, 0u};
+    KG_HIP(hipMemcpyAsync(flags.p, init, sizeof(init), hipMemcpyHostToDevice, s));
+    KG_HIP(nnd_launch_sqn(a, sqn, flags.p, s));
+    uint32_t fl[4];
+    KG_HIP(hipMemcpyAsync(fl, flags.p, sizeof(fl), hipMemcpyDeviceToHost, s));
+    h.sync();
+    auto unord = [](uint32_t k) {
+        uint32_t const u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+        float f;
+        memcpy(&f, &u, 4);
+        return double(f);
+    };
+    float mx;
+    memcpy(&mx, &fl[1], 4);
+    RowRange r;
+    r.integer = (fl[0] & 1u) == 0;
+    r.maxabs = mx;
+    r.vmin = h.N ? unord(fl[2]) : 0.0;
+    r.vmax = h.N ? unord(fl[3]) : 0.0;
+    return r;
 }
 
 // One NN-descent iteration's device work after the first: update of the
@@ -213,20 +243,14 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     a.meta = st.meta.p;
     a.radiusM = st.radiusM.p;
     if (h.precision == KG_PRECISION_BF16 && h.D <= 256) {
-        // local join as BF16 MFMA tiles; exact as it is on integer rows with
-        // 2 D max|x|^2 <= 2^24, else a screen with an exact FP32 re-score
+        // local join as BF16 MFMA tiles; exact as it is on integer rows that
+        // BF16 holds with small enough sums (bf16_tiles_exact), else a screen
+        // with an exact FP32 re-score
         st.sqn.reserve(N);
-        st.flags.reserve(2);
-        KG_HIP(hipMemsetAsync(st.flags.p, 0, 8, s));
-        KG_HIP(nnd_launch_sqn(a, st.sqn.p, st.flags.p, s));
-        uint32_t fl[2];
-        KG_HIP(hipMemcpyAsync(fl, st.flags.p, 8, hipMemcpyDeviceToHost, s));
-        h.sync();
-        float mx;
-        memcpy(&mx, &fl[1], 4);
+        RowRange rr = row_range(h, a, st.sqn.p, st.flags, s);
         a.sqn = st.sqn.p;
         a.mfma = 1;
-        a.exact = (fl[0] == 0 && 2.0 * h.D * double(mx) * double(mx) <= 16777216.0) ? 1u : 0u;
+        a.exact = bf16_tiles_exact(h.D, h.metric, rr.integer, h.rows_bf16, rr.maxabs, rr.vmin, rr.vmax) ? 1u : 0u;
         screen_constants(h.D, a);
     }
     h.join_exact = a.mfma ? int(a.exact) : -1;
@@ -653,12 +677,10 @@ void screen_pairs(Handle &h, uint32_t const *pairs, unsigned n, float *screened,
     DevBuf<uint32_t> flags, dp;
     DevBuf<float> dd, de;
     sqn.reserve(h.N);
-    flags.reserve(2);
     dp.reserve(2ull * n);
     dd.reserve(n);
     de.reserve(n);
-    KG_HIP(hipMemsetAsync(flags.p, 0, 8, s));
-    KG_HIP(nnd_launch_sqn(a, sqn.p, flags.p, s));
+    (void)row_range(h, a, sqn.p, flags, s);
     a.sqn = sqn.p;
     screen_constants(h.D, a);
     KG_HIP(hipMemcpyAsync(dp.p, pairs, 8ull * n, hipMemcpyHostToDevice, s));

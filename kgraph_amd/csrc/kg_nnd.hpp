@@ -54,7 +54,8 @@ struct NndArgs {
 hipError_t nnd_launch_init(int metric, NndArgs const &a, hipStream_t s);
 hipError_t nnd_launch_join(int metric, NndArgs const &a, hipStream_t s);
 // (|x|^2, |x|) per row into out; flags[0] |= 1 if any value is not an integer,
-// flags[1] = max |x| as float bits (both zeroed by the caller).
+// flags[1] = max |x| as float bits, flags[2] / flags[3] = ord_f32 of the
+// smallest / largest value (the caller sets flags[2] to all ones, the rest to 0).
 hipError_t nnd_launch_sqn(NndArgs const &a, float2 *out, uint32_t *flags, hipStream_t s);
 hipError_t nnd_launch_merge(NndArgs const &a, hipStream_t s);
 hipError_t nnd_launch_update(NndArgs const &a, hipStream_t s);

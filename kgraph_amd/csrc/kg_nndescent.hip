@@ -597,8 +597,10 @@ __global__ void __launch_bounds__(256) join_tile_kernel(NndArgs a) {
 }
 
 // Row norms for the MFMA join: (|x|^2 in canonical order, its square root),
-// and whether every value is an integer (flags[0]) with max |x| (flags[1],
-// float bits) -- what decides the exact form.
+// and what decides the exact form: whether every value is an integer
+// (flags[0] bit 0 set when one is not), max |x| (flags[1], float bits), the
+// smallest and largest value (flags[2] / flags[3], order-preserving bits
+// ord_f32; the caller sets flags[2] to all ones and the rest to zero).
 __global__ void __launch_bounds__(256) sqn_kernel(float const *X, unsigned N, unsigned D, unsigned Dp, float2 *out,
                                                   uint32_t *flags) {
     unsigned const lane = threadIdx.x & 63, t = lane & 3;
@@ -607,20 +609,32 @@ __global__ void __launch_bounds__(256) sqn_kernel(float const *X, unsigned N, un
     float const *r = X + size_t(ok ? row : 0u) * Dp;
     float const n2 = quad_metric<METRIC_EUCLIDEAN>(r, r, D, t);
     bool frac = false;
-    float mx = 0.f;
+    float mx = 0.f, lo = __builtin_huge_valf(), hi = -__builtin_huge_valf();
     if (ok)
         for (unsigned i = t; i < D; i += 4) {
             float const v = r[i];
             frac |= !(v == rintf(v));
             mx = fmaxf(mx, fabsf(v));
+            lo = fminf(lo, v);
+            hi = fmaxf(hi, v);
         }
     if (ok && t == 0) out[row] = make_float2(n2, sqrtf(n2));
     // atomics only while they can still change the flags
     if (ballot(frac) && lane == 0 && !__hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
         atomicOr(flags, 1u);
-    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-    if (lane == 0 && __float_as_uint(mx) > __hip_atomic_load(flags + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-        atomicMax(flags + 1, __float_as_uint(mx));
+    for (int o = 32; o > 0; o >>= 1) {
+        mx = fmaxf(mx, __shfl_xor(mx, o));
+        lo = fminf(lo, __shfl_xor(lo, o));
+        hi = fmaxf(hi, __shfl_xor(hi, o));
+    }
+    if (lane == 0) {
+        if (__float_as_uint(mx) > __hip_atomic_load(flags + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(flags + 1, __float_as_uint(mx));
+        if (ord_f32(lo) < __hip_atomic_load(flags + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMin(flags + 2, ord_f32(lo));
+        if (ord_f32(hi) > __hip_atomic_load(flags + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(flags + 3, ord_f32(hi));
+    }
 }
 
 // Per-node join record: z = the norm the join's distance formula takes (|x|^2

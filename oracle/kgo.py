@@ -67,6 +67,8 @@ _SIG = {
                              ctypes.POINTER(kgo_index_info), _P, _U]),
     "kgo_index_search": (_I, [_P, _P, _U, _U, _SZ, _I, _P, _U, _SZ, ctypes.POINTER(kgo_search_params),
                               _P, _P, _P, _P, _SZ, _P, _P, _P]),
+    "kgo_index_search_trace": (_I, [_P, _P, _U, _U, _SZ, _I, _P, ctypes.POINTER(kgo_search_params), _P, _P, _P,
+                                    _P, _P, _P, _U, _P, _P, _U, _P, _P, _U, _P]),
     "kgo_bruteforce": (_I, [_P, _U, _U, _SZ, _I, _P, _U, _SZ, _U, ctypes.c_float, _P, _P, _P]),
     "kgo_linear_knn": (_I, [_P, _U, _U, _SZ, _I, _P, _U, _U, _P, _P]),
     "kgo_index_size": (_U, [_P]),
@@ -82,18 +84,24 @@ _SIG = {
 _lib = None
 
 
+def open_lib(path):
+    """A copy of the oracle built elsewhere (tests build copies with planted
+    mistakes to check that the walk pins reject them)."""
+    l = ctypes.CDLL(path)
+    for n, (r, a) in _SIG.items():
+        f = getattr(l, n)
+        f.restype = r
+        f.argtypes = a
+    return l
+
+
 def lib():
     global _lib
     if _lib is None:
         p = lib_path()
         if not os.path.exists(p):
             build()
-        l = ctypes.CDLL(p)
-        for n, (r, a) in _SIG.items():
-            f = getattr(l, n)
-            f.restype = r
-            f.argtypes = a
-        _lib = l
+        _lib = open_lib(p)
     return _lib
 
 
@@ -164,8 +172,8 @@ def linear_knn(X, rows, K, metric):
 class Index:
     """The oracle's KGraphImpl: build / search / save / load / reverse / prune."""
 
-    def __init__(self):
-        self._l = lib()
+    def __init__(self, l=None):
+        self._l = l if l is not None else lib()
         self.h = self._l.kgo_index_new()
 
     def __del__(self):
@@ -208,6 +216,36 @@ class Index:
                                              Q.shape[1], sp, _p(Ks_a), _p(Ps_a), _p(ids), _p(d), width, _p(c),
                                              _p(comps), _p(scans)))
         return ids, d, c, comps, scans
+
+    def search_trace(self, X, q, metric, K=25, P=100, M=0, S=10, T=1, epsilon=1e30, seed=1998, init_ids=None):
+        """One query's search and everything it did: returns (ids, dists,
+        comps, scans, visited ids, their distances, expansions as rows of
+        (node, beginM, endM), start of each trial's seeds in `visited`)."""
+        X = f32(X)
+        q = f32(q).ravel()
+        init = 0 if init_ids is None else len(init_ids)
+        ids = np.zeros(max(K, init, 1), dtype=np.uint32)
+        if init:
+            ids[:init] = init_ids
+        d = np.zeros(max(K, 1), dtype=np.float32)
+        comps, scans = ctypes.c_uint(), ctypes.c_uint()
+        cap = X.shape[0] * T + 16
+        vis = np.zeros(cap, dtype=np.uint32)
+        vd = np.zeros(cap, dtype=np.float32)
+        ecap = 4 * X.shape[0] * T + 16
+        exps = np.zeros(3 * ecap, dtype=np.uint32)
+        ts = np.zeros(T + 1, dtype=np.uint32)
+        nv, ne, nt = ctypes.c_uint(), ctypes.c_uint(), ctypes.c_uint()
+        sp = kgo_search_params(K, M, P, S, T, epsilon, seed, init)
+        r = self._l.kgo_index_search_trace(self.h, _p(X), X.shape[0], X.shape[1], X.shape[1], metric, _p(q), sp,
+                                           _p(ids), _p(d), ctypes.byref(comps), ctypes.byref(scans), _p(vis), _p(vd),
+                                           cap, ctypes.byref(nv), _p(exps), ecap, ctypes.byref(ne), _p(ts), T + 1,
+                                           ctypes.byref(nt))
+        if r < 0:
+            raise RuntimeError(self._l.kgo_index_error(self.h).decode())
+        assert nv.value <= cap and ne.value <= ecap
+        return (ids[:r], d[:r], comps.value, scans.value, vis[:nv.value], vd[:nv.value],
+                exps[:3 * ne.value].reshape(-1, 3), ts[:nt.value])
 
     def export(self):
         n = self._l.kgo_index_size(self.h)

@@ -270,12 +270,46 @@ static unsigned constexpr KGRAPH_MAGIC_SIZE = 8;
 static uint32_t constexpr SIGNATURE_VERSION = 2;
 enum { FORMAT_DEFAULT = 0, FORMAT_NO_DIST = 1, FORMAT_TEXT = 128 };
 
+// Planted mistakes (KGO_PLANT, default 0 = none): tests/test_oracle_walk.py
+// compiles the oracle once per value and checks that its walk pins reject
+// every such copy.  Each value is one plausible misreading of the reference's
+// search (kgraph.cpp:423-564):
+//   1 no rewind `if (r < k) k = r`                      (:507-512)
+//   2 the whole list in one batch, S ignored             (:483-489)
+//   3 actual_M ignores SearchParams::M                   (:284-286)
+//   4 actual_M ignores the index's M[i] (whole list)     (:284-286)
+//   5 trials after the first are not merged              (:516-529)
+//   6 the visited bits are cleared between trials        (:438, 452-458)
+//   7 a later trial also starts from visited seeds       (:454-457)
+//   8 M of an insert at the old end of the pool unset    (:504-506)
+#ifndef KGO_PLANT
+#define KGO_PLANT 0
+#endif
+
+// Everything one search call did, in order (for the walk's invariant tests):
+// the ids scored (starting points first, then each fresh neighbour) with
+// their distances, and each expansion as (node, beginM, endM).
+struct SearchTrace {
+    vector<unsigned> visited;
+    vector<float> vdist;
+    vector<unsigned> expansions;  // 3 per expansion
+    vector<unsigned> trial_start;  // index into visited where each trial's seeds begin
+};
+
 struct Index {  // KGraphImpl state (kgraph.cpp:276-286)
     vector<unsigned> M;
     vector<vector<Neighbor>> graph;
     bool no_dist = false;
     unsigned actual_M(unsigned pM, unsigned i) const {  // kgraph.cpp:284-286
+#if KGO_PLANT == 3
+        (void)pM;
+        return std::min(M[i], unsigned(graph[i].size()));
+#elif KGO_PLANT == 4
+        (void)pM;
+        return unsigned(graph[i].size());
+#else
         return std::min(std::max(M[i], pM), unsigned(graph[i].size()));
+#endif
     }
 
     // KGraphImpl::load (kgraph.cpp:291-331); truncated files throw here.
@@ -360,7 +394,7 @@ struct Index {  // KGraphImpl state (kgraph.cpp:276-286)
     // evaluations after seeding (kgraph.cpp:500); n_scan counts neighbour ids
     // read (sum of endM-beginM over expansions, kgraph.cpp:485-496).
     unsigned search(QueryOracle const &oracle, SearchParams const &params, unsigned *ids, float *dists,
-                    unsigned *p_comps, unsigned *p_scan) const {
+                    unsigned *p_comps, unsigned *p_scan, SearchTrace *tr = nullptr) const {
         if (graph.size() > oracle.size()) throw std::runtime_error("dataset larger than index");
         if (params.P >= graph.size()) {
             if (p_comps) *p_comps = oracle.size();
@@ -377,11 +411,19 @@ struct Index {  // KGraphImpl state (kgraph.cpp:276-286)
         unsigned n_comps = 0, n_scan = 0;
         for (unsigned trial = 0; trial < params.T; ++trial) {
             unsigned L = params.init;
+#if KGO_PLANT == 6
+            std::fill(flags.begin(), flags.end(), 0);
+#endif
+            if (tr) tr->trial_start.push_back(unsigned(tr->visited.size()));
             if (L == 0) {
                 vector<unsigned> random(params.P);
                 GenRandom(rng, &random[0], random.size(), graph.size());
                 for (unsigned s : random)
+#if KGO_PLANT == 7
+                    knn[L++].id = s;
+#else
                     if (!flags[s]) knn[L++].id = s;
+#endif
             } else {
                 if (!ids) throw std::invalid_argument("no initial data provided via ids");
                 // kgraph.cpp:463; per-query K / P batches may start from up to K + P - 1 ids
@@ -396,6 +438,10 @@ struct Index {  // KGraphImpl state (kgraph.cpp:276-286)
                 e.dist = oracle(e.id);
                 e.m = 0;
                 e.M = actual_M(params.M, e.id);
+                if (tr) {
+                    tr->visited.push_back(e.id);
+                    tr->vdist.push_back(e.dist);
+                }
             }
             std::sort(knn.begin(), knn.begin() + L, dist_id_less<NeighborX>);
             unsigned k = 0;
@@ -406,7 +452,11 @@ struct Index {  // KGraphImpl state (kgraph.cpp:276-286)
                     continue;
                 }
                 unsigned beginM = e.m;
+#if KGO_PLANT == 2
+                unsigned endM = e.M + 1;
+#else
                 unsigned endM = beginM + params.S;
+#endif
                 if (endM > e.M) {
                     e.flag = false;
                     endM = e.M;
@@ -414,18 +464,32 @@ struct Index {  // KGraphImpl state (kgraph.cpp:276-286)
                 e.m = endM;
                 auto const &neighbors = graph[e.id];
                 if (endM > beginM) n_scan += endM - beginM;
+                if (tr) tr->expansions.insert(tr->expansions.end(), {e.id, beginM, endM});
                 for (unsigned m = beginM; m < endM; ++m) {
                     unsigned id = neighbors[m].id;
                     if (flags[id]) continue;
                     flags[id] = 1;
                     ++n_comps;
                     float dist = oracle(id);
+                    if (tr) {
+                        tr->visited.push_back(id);
+                        tr->vdist.push_back(dist);
+                    }
                     NeighborX nn(id, dist);
                     unsigned r = UpdateKnnList(&knn[0], L, nn);
+#if KGO_PLANT == 8
+                    unsigned const L_before = L;
+#endif
                     if (L + 1 < knn.size()) ++L;
+#if KGO_PLANT == 8
+                    if (r < L_before) {
+#else
                     if (r < L) {
+#endif
                         knn[r].M = actual_M(params.M, id);
+#if KGO_PLANT != 1
                         if (r < k) k = r;
+#endif
                     }
                 }
             }
@@ -434,6 +498,7 @@ struct Index {  // KGraphImpl state (kgraph.cpp:276-286)
                 results.reserve(params.K + 1);
                 results.resize(L + 1);
                 std::copy(knn.begin(), knn.begin() + L, results.begin());
+            } else if (KGO_PLANT == 5) {
             } else {
                 for (unsigned l = 0; l < L; ++l) {
                     unsigned r = UpdateKnnList(&results[0], results.size() - 1, knn[l]);
@@ -978,6 +1043,38 @@ int kgo_index_search(kgo_index *h, float const *X, unsigned N, unsigned D, size_
         }
         if (bad) throw std::runtime_error(err);
         return 0;
+    } catch (std::exception const &e) {
+        h->err = e.what();
+        return -1;
+    }
+}
+
+// One query's search with its trace (SearchTrace): visited ids / distances
+// in order (up to vcap), expansions as (node, beginM, endM) triples (up to
+// ecap), trial start positions in `visited` (up to tcap).  Totals go to
+// *nv / *ne / *nt; the return value is the result count (or -1).
+int kgo_index_search_trace(kgo_index *h, float const *X, unsigned N, unsigned D, size_t stride, int metric,
+                           float const *q, kgo_search_params const *sp, unsigned *ids, float *dists,
+                           unsigned *comps, unsigned *scans, unsigned *visited, float *vdist, unsigned vcap,
+                           unsigned *nv, unsigned *exps, unsigned ecap, unsigned *ne, unsigned *tstart,
+                           unsigned tcap, unsigned *nt) {
+    try {
+        Dense d;
+        d.init(X, N, D, stride, metric);
+        SearchParams s{sp->K, sp->M, sp->P, sp->S, sp->T, sp->epsilon, sp->seed, sp->init};
+        QueryOracle o(&d, q);
+        SearchTrace tr;
+        unsigned L = h->idx.search(o, s, ids, dists, comps, scans, &tr);
+        *nv = unsigned(tr.visited.size());
+        *ne = unsigned(tr.expansions.size() / 3);
+        *nt = unsigned(tr.trial_start.size());
+        for (unsigned i = 0; i < std::min<size_t>(vcap, tr.visited.size()); ++i) {
+            visited[i] = tr.visited[i];
+            vdist[i] = tr.vdist[i];
+        }
+        for (unsigned i = 0; i < std::min<size_t>(3ull * ecap, tr.expansions.size()); ++i) exps[i] = tr.expansions[i];
+        for (unsigned i = 0; i < std::min<size_t>(tcap, tr.trial_start.size()); ++i) tstart[i] = tr.trial_start[i];
+        return int(L);
     } catch (std::exception const &e) {
         h->err = e.what();
         return -1;

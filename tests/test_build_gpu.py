@@ -100,6 +100,32 @@ def test_mfma_join_recall_and_canonical_distances(kind, metric, d, form):
             assert v == kgo.dist(metric, X[i], X[j])
 
 
+@pytest.mark.parametrize("metric,d,lo,hi", [
+    (L2, 16, 0, 700),      # 2 D max^2 <= 2^24 but 701, 699, ... are not BF16 values
+    (EUC, 16, 0, 700),
+    (L2, 128, -200, 200),  # BF16-exact and 2 D max^2 <= 2^24, but D (max - min)^2 > 2^24
+])
+def test_bf16_join_exact_form_needs_lossless_rows_and_bounded_sums(metric, d, lo, hi):
+    # integer rows the exact form cannot hold: the build must screen and
+    # re-score, and every stored distance is still the canonical FP32 value
+    from kgraph_amd import PRECISION_BF16
+    rs = np.random.RandomState(5)
+    centres = rs.randint(lo, hi + 1, size=(30, d))
+    X = np.clip(centres[rs.randint(0, 30, 8000)] + rs.randint(-40, 41, size=(8000, d)), lo, hi).astype(np.float32)
+    g = _gpu(X, metric)
+    info = g.build(precision=PRECISION_BF16, K=16, L=32, S=10, R=100, iterations=10, controls=100, reverse=0)
+    c = g.build_counters()
+    assert c["join_form"] == "bf16-screen", c["join_form"]
+    off, ids, dd, M = g.graph()
+    rows = np.arange(0, len(X), 9)
+    _, gt_d = kgo.linear_knn(X, rows, 16, metric)
+    print("integer rows [%d, %d] D=%d: recall %.4f (%d it)" % (lo, hi, d, _knn_recall(off[rows], dd, gt_d, 16),
+                                                            info.iterations))
+    for i in range(0, len(X), 23):
+        for j, v in zip(ids[off[i]:off[i + 1]], dd[off[i]:off[i + 1]]):
+            assert v == kgo.dist(metric, X[i], X[j])
+
+
 def _fp64_dist(metric, A, B):
     A = A.astype(np.float64)
     B = B.astype(np.float64)

@@ -424,3 +424,40 @@ def test_autotuned_launch_shape_keeps_results():
     assert wpb in (1, 2, 4) and bpc in (4, 8, 16, 32)
     after = g.search(Q, K=10, P=50, withDistance=True)
     assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
+
+
+def _walk_cases():
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "walk_cases.json")
+    return json.load(open(path))["cases"]
+
+
+@pytest.mark.parametrize("case", _walk_cases(), ids=[c["name"] for c in _walk_cases()])
+def test_hand_traced_walks(case):
+    # the hand-traced walks of tests/golden/walk_cases.json (rewind, S
+    # batches, M below the degree, two trials merged) through the HIP walk
+    import kgraph_amd
+    from kgraph_amd import _lib
+    x = np.asarray(case["x"], np.float32)[:, None]
+    gl = case["graph"]
+    off = np.zeros(len(gl) + 1, np.uint64)
+    off[1:] = np.cumsum([len(v) for v in gl])
+    ids = np.array([j for v in gl for j in v], np.uint32)
+    d = np.array([(x[i, 0] - x[j, 0]) ** 2 for i, v in enumerate(gl) for j in v], np.float32)
+    g = kgraph_amd.KGraph(x, "euclidean", l2sqr=True)
+    g.set_graph(off, ids, d, np.asarray(case["M"], np.uint32))
+    p, e = case["params"], case["expect"]
+    K = p["K"]
+    init = p.get("init") or []
+    q = np.zeros((1, 1), np.float32)
+    gi = np.zeros((1, K), np.uint32)
+    gi[0, :len(init)] = init
+    gd = np.zeros((1, K), np.float32)
+    gc, gcomp, gscan = (np.zeros(1, np.uint32) for _ in range(3))
+    sp = _lib.SearchParams(K, p["M"], p["P"], p["S"], p["T"], 1e30, p.get("seed", 1998), len(init))
+    g._h.call("kg_search", _lib.ptr(q), 1, 1, sp, None, None, _lib.ptr(gi), _lib.ptr(gd), K, _lib.ptr(gc),
+              _lib.ptr(gcomp), _lib.ptr(gscan))
+    n = int(gc[0])
+    assert gi[0, :n].tolist() == e["ids"] and gd[0, :n].tolist() == [float(v) for v in e["dists"]]
+    assert (int(gcomp[0]), int(gscan[0])) == (e["comps"], e["scans"])

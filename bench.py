@@ -12,9 +12,11 @@ pass per (start, P, S) setting over the 10,000 queries (queries resident in
 HBM): random starting points with P in {50, 100, 200} (the reference's
 configuration) and starting points selected from an entry sample of 4,096 or
 8,192 nodes (exact FP32 scores; the sample's rows are index-side state, set
-outside the timed search) with P in {10, 20, 50, 100} and S in {10, 32, 64}.
-`value` is the highest search QPS whose recall@10 (distance-equality, against
-the exact K-NN from the brute-force kernel) reaches 0.95; `build_s` is the
+outside the timed search) with P in {10, 20, 50, 100, 200} and S in
+{10, 32, 64}.  `value` is the highest search QPS among the settings with P in
+the workload's {50, 100, 200} whose recall@10 (distance-equality, against the
+exact K-NN from the brute-force kernel) reaches 0.95; the P = 10 and 20 points
+are reported in `search_sweep` and `best_outside_config`.  `build_s` is the
 BF16-form build's wall time and `build_forms` holds both forms.
 
 Other workloads: --workload batched-10m-d96 (mixed K/P in one launch) and
@@ -53,7 +55,7 @@ def parse():
     ap.add_argument("--nq", type=int, default=None, help="queries (default 10k, or 100k for batched)")
     ap.add_argument("--iterations", type=int, default=15)
     ap.add_argument("--P", type=str, default="50,100,200")
-    ap.add_argument("--Pe", type=str, default="10,20,50,100",
+    ap.add_argument("--Pe", type=str, default="10,20,50,100,200",
                     help="P values for walks started from selected entry points")
     ap.add_argument("--Se", type=str, default="10,32,64",
                     help="S values (neighbours per expansion, SearchParams::S) for entry-started walks")
@@ -67,9 +69,14 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device to device (reference transfer for counter runs)")
-    ap.add_argument("--cpu-queries", type=int, default=1000)
-    ap.add_argument("--sigma", type=float, default=None,
-                    help="batched-10m-d96: per-coordinate noise (default 0.3/sqrt(96), a noise radius of 0.3)")
+    ap.add_argument("--cpu-queries", type=int, default=3000,
+                    help="queries of the all-core CPU search baseline (median of 3 runs)")
+    ap.add_argument("--cpu-queries-1t", type=int, default=300,
+                    help="queries of the one-thread CPU search baseline")
+    ap.add_argument("--sigma", type=float, default=0.3,
+                    help="angular-1.2m-d96 / batched-10m-d96: within-cluster sigma per coordinate, "
+                         "x = mu_c + sigma N(0, I) (default 0.3, as the workload specifies; 0.3/sqrt(96) "
+                         "makes sigma a noise radius and the clusters tight, reported as a variant)")
     ap.add_argument("--cpu-build-n", type=int, default=1_000_000,
                     help="rows of the CPU build baseline (default: the whole 1M-row C2 set, about 20 s of CPU "
                          "work on 16 threads; smaller subsets of these separated clusters can strand a control "
@@ -252,8 +259,13 @@ def main():
                           mean_scans=float(r["scans"].mean())))
         log("%-7s E=%d P=%d S=%d recall@10=%.4f QPS=%.0f walk kernel %.2fms entry %.2fms (%.1f GB/s algorithmic)" % (
             r["mode"], r["E"], r["P"], r["S"], rec, qps, r["kernel_ms"], r["entry_ms"], byt / r["kernel_ms"] / 1e6))
-    ok = [s for s in sweep if s["recall"] >= 0.95]
-    pick = max(ok, key=lambda s: s["qps"]) if ok else max(sweep, key=lambda s: s["recall"])
+    # the headline comes from the workload's own P values; P = 10 / 20 stay
+    # in the sweep and in best_outside_config
+    in_cfg = [s for s in sweep if s["P"] in Ps]
+    ok = [s for s in in_cfg if s["recall"] >= 0.95]
+    pick = max(ok, key=lambda s: s["qps"]) if ok else max(in_cfg, key=lambda s: s["recall"])
+    outside = [s for s in sweep if s["P"] not in Ps and s["recall"] >= 0.95]
+    best_outside = max(outside, key=lambda s: s["qps"]) if outside else None
     achieved = pick["bytes"] / (pick["kernel_ms"] * 1e-3) / 1e9
     if pick["mode"] == "entries":  # the picked sample stays set for what follows
         args.entry_sample = pick["E"]
@@ -327,6 +339,13 @@ def main():
         "build_control_recall": round(float(info.recall), 4),
         "recall@10": round(pick["recall"], 4),
         "search_sweep": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in s.items()} for s in sweep],
+        "best_by_P": {str(P): max(([{"qps": round(s["qps"], 1), "recall": round(s["recall"], 4), "mode": s["mode"],
+                                     "S": s["S"], "E": s["E"]} for s in sweep if s["P"] == P and
+                                    s["recall"] >= 0.95] or [None]), key=lambda r: r["qps"] if r else 0)
+                      for P in sorted(set(s["P"] for s in sweep))},
+        "best_outside_config": (None if best_outside is None else
+                                {"P": best_outside["P"], "S": best_outside["S"], "E": best_outside["E"],
+                                 "qps": round(best_outside["qps"], 1), "recall": round(best_outside["recall"], 4)}),
         "build_kernel_ms": {"join": round(join_ms, 2), "merge": round(merge_ms, 2), "update": round(upd_ms, 2),
                             "reverse": round(rev_ms, 2)},
         "roofline": {"kernel": "search_kernel", "bound": "hbm", "achieved": round(achieved, 1),
@@ -352,9 +371,9 @@ def batched_workload(args):
     (ratio 5:4:1) and P in {50,100,300,500} (uniform) answered by one
     persistent search launch, every walk starting from its query's 9 nearest
     nodes of a 16,384-node entry sample (a query walks from min(9, K + P - 1)
-    of them).  The within-cluster sigma 0.3 is the noise radius: 0.3 / sqrt(96)
-    per coordinate (--sigma sets the per-coordinate value; 0.3 per coordinate
-    makes the rows almost uniform on the sphere).  Reports QPS (entry
+    of them).  Rows are x = mu_c + sigma N(0, I), sigma = 0.3 per coordinate
+    as specified (--sigma 0.0306 gives the tight-cluster variant, sigma a
+    noise radius of 0.3).  Reports QPS (entry
     selection + walk), per-query p50/p99 latency of the walk (stamped inside
     the kernel), algorithmic HBM GB/s and recall@K on a query sample."""
     from kgraph_amd import _lib, datagen, devmem
@@ -362,7 +381,7 @@ def batched_workload(args):
     from oracle.kgo import recall_at_k
     n, nq = args.n, args.nq
     t0 = time.time()
-    sigma = args.sigma if args.sigma is not None else 0.3 / np.sqrt(96)
+    sigma = args.sigma
     X = datagen.angular_mixture(n, 96, 2000, 6, 9, sigma=sigma)
     Q = datagen.angular_mixture(nq, 96, 2000, 6, 10, sigma=sigma)
     rs = np.random.RandomState(11)
@@ -501,12 +520,13 @@ def precision_workload(args):
     IndexParams at the reference defaults (reverse -1).  Reports both builds'
     time and graph recall@10, search QPS / recall@10 on each index, and the
     error against FP64 of the stored distances and of the screened (BF16 tile)
-    distances, with the screen's error bound.  sigma as in batched-10m-d96."""
+    distances, with the screen's error bound.  sigma per coordinate as in
+    batched-10m-d96 (0.3 by default, as specified)."""
     from kgraph_amd import _lib, datagen, devmem
     from kgraph_amd._lib import IndexInfo, IndexParams, SearchParams, ptr
     from oracle.kgo import recall_at_k
     n, nq, K, D = args.n, args.nq, 10, 96
-    sigma = args.sigma if args.sigma is not None else 0.3 / np.sqrt(96)
+    sigma = args.sigma
     t0 = time.time()
     X = datagen.angular_mixture(n, D, 2000, 6, 7, sigma=sigma)
     Q = datagen.angular_mixture(nq, D, 2000, 6, 8, sigma=sigma)
@@ -824,10 +844,34 @@ def ctypes_uint():
     return _lib.ctypes.c_uint()
 
 
+def host_cpu():
+    """CPU model, logical CPUs this process may use, and whether SMT is on."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    smt = None
+    try:
+        smt = open("/sys/devices/system/cpu/smt/active").read().strip() == "1"
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"model": model, "logical_cpus_available": avail, "logical_cpus_total": os.cpu_count(),
+            "smt_active": smt}
+
+
 def cpu_baseline(args, X, Q, h, P, K, bp, mode, S=10):
     """The oracle (CPU restatement of the reference) on this host's cores:
-    search QPS over a query sample on the same GPU-built index, and the build
-    time on a row sample with the same IndexParams."""
+    search QPS over a query sample on the same GPU-built index (median of 3
+    runs on every thread the process may use, and one run on one thread), and
+    the build time on a row sample with the same IndexParams."""
     from kgraph_amd import export_graph
     from oracle import kgo
     threads = kgo.set_threads(0)
@@ -835,20 +879,35 @@ def cpu_baseline(args, X, Q, h, P, K, bp, mode, S=10):
     idx = kgo.Index()
     idx.import_(off, ids, d, M)
     nq = min(args.cpu_queries, len(Q))
-    t = time.perf_counter()
+    init_ids = None
     if mode == "entries":
         # same starting points: the m nearest of the same node sample, then the walk
         samp = np.sort(kgo.genrandom(1998, 1, args.entry_sample, len(X))[0])
         si, _, _ = kgo.bruteforce(X[samp], Q[:nq], K - 1, 0)
-        idx.search(X, Q[:nq], 0, K=K, P=P, S=S, init=K - 1, init_ids=samp[si])
-    else:
-        idx.search(X, Q[:nq], 0, K=K, P=P, S=S)
-    dt = time.perf_counter() - t
-    qps = nq / dt
-    log("cpu search (%s): %d queries in %.2fs on %d threads -> %.0f QPS" % (mode, nq, dt, threads, qps))
+        init_ids = samp[si]
+
+    def run(n):
+        t = time.perf_counter()
+        if init_ids is not None:
+            idx.search(X, Q[:n], 0, K=K, P=P, S=S, init=K - 1, init_ids=init_ids[:n])
+        else:
+            idx.search(X, Q[:n], 0, K=K, P=P, S=S)
+        return n / (time.perf_counter() - t)
+
+    runs = [run(nq) for _ in range(3)]
+    qps = float(np.median(runs))
+    n1 = min(args.cpu_queries_1t, nq)
+    kgo.set_threads(1)
+    qps1 = run(n1)
+    kgo.set_threads(threads)
+    cpu = host_cpu()
+    log("cpu search (%s): %d queries, %d threads: %s QPS (median %.0f); 1 thread, %d queries: %.0f QPS; %s" % (
+        mode, nq, threads, ["%.0f" % r for r in runs], qps, n1, qps1, cpu))
     res = {"value": round(qps, 1), "unit": "queries/s", "cores": threads, "kind": "port",
-           "sample": "%d of the %d queries, same GPU-built index, K=%d P=%d S=%d, %s starting points" % (
-               nq, len(Q), K, P, S, mode)}
+           "sample": "%d of the %d queries, same GPU-built index, K=%d P=%d S=%d, %s starting points; median of 3 "
+                     "runs on %d threads (OpenMP loop over queries, dynamic chunks of 8)" % (nq, len(Q), K, P, S, mode, threads),
+           "runs_qps": [round(r, 1) for r in runs], "one_thread_qps": round(qps1, 1), "one_thread_queries": n1,
+           "host": cpu}
     if args.cpu_build_n > 0:
         n = min(args.cpu_build_n, len(X))
         Xs = np.ascontiguousarray(X[:n])

@@ -124,37 +124,54 @@ public:
             for (unsigned j = 0; j < col; ++j) p[j] /= sum;
         }
     }
+    /// Rows of `dim` values each, after a `skip`-byte header, with `gap`
+    /// bytes after every row (the raw layout of kgraph-data.h:175-191): as
+    /// many whole rows as the file holds.
     void load(std::string const &path, unsigned dim, unsigned skip = 0, unsigned gap = 0) {
-        std::ifstream is(path.c_str(), std::ios::binary);
-        if (!is) throw io_error(path);
-        is.seekg(0, std::ios::end);
-        size_t size = size_t(is.tellg()) - skip;
-        unsigned line = sizeof(T) * dim + gap;
-        unsigned N = size / line;
-        reset(N, dim);
+        std::ifstream in(path.c_str(), std::ios::binary | std::ios::ate);
+        if (!in) throw io_error(path);
+        std::streamoff const bytes = in.tellg();
+        size_t const rowbytes = sizeof(T) * size_t(dim);
+        size_t const period = rowbytes + gap;
+        size_t const body = bytes > std::streamoff(skip) ? size_t(bytes) - skip : 0;
+        size_t const n = period ? body / period : 0;
+        reset(unsigned(n), dim);
         zero();
-        is.seekg(skip, std::ios::beg);
-        for (unsigned i = 0; i < N; ++i) {
-            is.read(&data[stride * i], sizeof(T) * dim);
-            is.seekg(gap, std::ios::cur);
+        in.seekg(skip);
+        // one read of the whole body when rows are packed back to back in
+        // both places, else a row at a time
+        if (gap == 0 && stride == rowbytes) {
+            in.read(data, std::streamsize(n * rowbytes));
+        } else {
+            std::vector<char> buf(period);
+            for (size_t r = 0; r < n && in; ++r) {
+                in.read(buf.data(), std::streamsize(r + 1 < n ? period : rowbytes));
+                std::memcpy(data + stride * r, buf.data(), rowbytes);
+            }
         }
-        if (!is) throw io_error(path);
+        if (!in) throw io_error(path);
     }
-    /// lshkit format: header {sizeof(T), rows, cols} then rows (kgraph-data.h:193-205).
+    /// lshkit files: a header of three 32-bit words {bytes per value, rows,
+    /// columns}, then the rows packed (kgraph-data.h:193-218).
     void load_lshkit(std::string const &path) {
-        std::ifstream is(path.c_str(), std::ios::binary);
-        unsigned header[3];
-        is.read(reinterpret_cast<char *>(header), sizeof header);
-        if (!is) throw io_error(path);
-        if (header[0] != sizeof(T)) throw io_error(path);
-        is.close();
-        load(path, header[2], 3 * sizeof(unsigned), 0);
+        uint32_t head[3] = {0, 0, 0};
+        {
+            std::ifstream in(path.c_str(), std::ios::binary);
+            if (!in.read(reinterpret_cast<char *>(head), sizeof(head)) || head[0] != sizeof(T)) throw io_error(path);
+        }
+        load(path, head[2], unsigned(sizeof(head)), 0);
     }
     void save_lshkit(std::string const &path) {
-        std::ofstream os(path.c_str(), std::ios::binary);
-        unsigned header[3] = {unsigned(sizeof(T)), row, col};
-        os.write(reinterpret_cast<char const *>(header), sizeof(header));
-        for (unsigned i = 0; i < row; ++i) os.write(&data[stride * i], sizeof(T) * col);
+        std::ofstream out(path.c_str(), std::ios::binary);
+        uint32_t const head[3] = {uint32_t(sizeof(T)), row, col};
+        out.write(reinterpret_cast<char const *>(head), sizeof(head));
+        size_t const rowbytes = sizeof(T) * size_t(col);
+        if (stride == rowbytes) {
+            out.write(data, std::streamsize(rowbytes * row));
+        } else {
+            for (unsigned r = 0; r < row; ++r) out.write(data + stride * r, std::streamsize(rowbytes));
+        }
+        if (!out) throw io_error(path);
     }
 };
 
@@ -214,31 +231,36 @@ public:
     }
 };
 
-/// Mean recall@K of result rows against ground-truth rows, both holding
-/// per-row distances sorted ascending (kgraph-data.h:333-364).
+/// Mean recall@K of result rows against ground-truth rows (kgraph-data.h:
+/// 333-364): each row holds distances sorted ascending; a result distance
+/// counts as found when it equals a ground-truth distance not yet matched, so
+/// ties never count as misses.  A result distance below every unmatched
+/// ground-truth one means the two disagree on distances: runtime_error
+/// "distance is unstable", as the reference throws.
+namespace detail {
+inline unsigned matched_prefix(float const *truth, float const *found, unsigned K) {
+    unsigned hits = 0;
+    float const *t = truth, *te = truth + K, *f = found, *fe = found + K;
+    while (t != te && f != fe) {
+        if (*f < *t) throw runtime_error("distance is unstable");
+        if (*f == *t) {
+            ++hits;
+            ++f;
+        }
+        ++t;
+    }
+    return hits;
+}
+}  // namespace detail
+
 inline float AverageRecall(Matrix<float> const &gs, Matrix<float> const &result, unsigned K = 0) {
     if (K == 0) K = result.dim();
     if (!(gs.dim() >= K)) throw invalid_argument("gs.dim() >= K");
     if (!(result.dim() >= K)) throw invalid_argument("result.dim() >= K");
     if (!(gs.size() >= result.size())) throw invalid_argument("gs.size() > result.size()");
-    float sum = 0;
-    for (unsigned i = 0; i < result.size(); ++i) {
-        float const *gs_row = gs[i];
-        float const *re_row = result[i];
-        unsigned found = 0, gs_n = 0, re_n = 0;
-        while ((gs_n < K) && (re_n < K)) {
-            if (gs_row[gs_n] < re_row[re_n])
-                ++gs_n;
-            else if (gs_row[gs_n] == re_row[re_n]) {
-                ++found;
-                ++gs_n;
-                ++re_n;
-            } else
-                throw runtime_error("distance is unstable");
-        }
-        sum += float(found) / K;
-    }
-    return sum / result.size();
+    double total = 0;
+    for (unsigned r = 0; r < result.size(); ++r) total += detail::matched_prefix(gs[r], result[r], K);
+    return float(total / K / result.size());
 }
 
 }  // namespace kgraph

@@ -144,6 +144,15 @@ public:
     virtual DenseOracleInfo const &dense_data() const = 0;
     virtual float const *dense_query() const = 0;
 };
+
+/// Extension (no counterpart in the reference, whose oracles read the caller's
+/// rows on every distance): the GPU index and SearchOracle::search keep a
+/// device copy of a dense oracle's rows, reused while the oracle's row
+/// pointer, size, dimension, stride, metric and a fingerprint of 64 sampled
+/// rows stay the same.  Rows changed in place (outside the sampled ones) are
+/// not noticed: call this after changing them, and every index and cached
+/// brute-force copy uploads the rows again at its next call.
+void forget_device_rows();
 }  // namespace kgraph
 
 #if __cplusplus > 199711L

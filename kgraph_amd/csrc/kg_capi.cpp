@@ -409,9 +409,8 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         b.counts = d_counts;
         b.comps = d_comps;
         b.scans = d_scans;
-        DevBuf<uint64_t> part;
-        part.reserve(size_t(b.nq) * kg_brute_splits(b.nq, N, b.K) * b.K);
-        b.partial = part.p;
+        h.bpart.reserve(size_t(b.nq) * kg_brute_splits(b.nq, N, b.K) * b.K);
+        b.partial = h.bpart.p;
         KG_HIP(hipEventRecord(h.ev[2], h.stream));
         KG_HIP(kg_launch_brute(h.metric, b, h.stream));
         KG_HIP(hipEventRecord(h.ev[3], h.stream));
@@ -833,8 +832,9 @@ int kg_search(kg_handle *k, float const *Q, unsigned nq, size_t qstride, kg_sear
         if (!h.has_data) throw kga::runtime_error("no dataset");
         if (qstride < h.D) throw kga::invalid_argument("bad stride");
         if (nq == 0) return;
-        DevBuf<float> dq, dd;
-        DevBuf<uint32_t> di, dc, dcomp, dscan;
+        // the handle's device buffers, grown on demand and kept
+        DevBuf<float> &dq = h.io_q, &dd = h.io_dists;
+        DevBuf<uint32_t> &di = h.io_ids, &dc = h.io_counts, &dcomp = h.io_comps, &dscan = h.io_scans;
         upload_rows(h, Q, false, nq, h.D, qstride, dq);
         di.reserve(size_t(nq) * ostride);
         dc.reserve(nq);
@@ -864,8 +864,8 @@ int kg_bruteforce(kg_handle *k, float const *Q, unsigned nq, size_t qstride, uns
         if (!Q || !ids || !counts) throw kga::invalid_argument("null argument");
         if (K == 0 || K > 1024 || K > ostride) throw kga::invalid_argument("K must be in [1, min(1024, ostride)]");
         if (nq == 0) return;
-        DevBuf<float> dq, dqn, dd;
-        DevBuf<uint32_t> di, dc;
+        DevBuf<float> &dq = h.io_q, &dqn = h.io_qn, &dd = h.io_dists;
+        DevBuf<uint32_t> &di = h.io_ids, &dc = h.io_counts;
         upload_rows(h, Q, false, nq, h.D, qstride, dq);
         dqn.reserve(nq);
         KG_HIP(kg_launch_norms(h.metric, dq.p, nq, h.D, h.Dp, dqn.p, h.stream));
@@ -890,9 +890,8 @@ int kg_bruteforce(kg_handle *k, float const *Q, unsigned nq, size_t qstride, uns
         b.out_dists = dd.p;
         b.ostride = unsigned(ostride);
         b.counts = dc.p;
-        DevBuf<uint64_t> part;
-        part.reserve(size_t(nq) * kg_brute_splits(nq, h.N, K) * K);
-        b.partial = part.p;
+        h.bpart.reserve(size_t(nq) * kg_brute_splits(nq, h.N, K) * K);
+        b.partial = h.bpart.p;
         KG_HIP(hipEventRecord(h.ev[2], h.stream));
         KG_HIP(kg_launch_brute(h.metric, b, h.stream));
         KG_HIP(hipEventRecord(h.ev[3], h.stream));

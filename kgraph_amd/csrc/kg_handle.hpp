@@ -114,8 +114,12 @@ struct Handle {
     // search scratch
     DevBuf<uint32_t> visited, touched, qcounter;
     DevBuf<uint32_t> seeds, seed_off, Ks, Ps, pidx, bsel;
-    DevBuf<float> q, qn, out_dists;
-    DevBuf<uint32_t> out_ids, counts, comps, scans;
+    DevBuf<float> q, qn;
+    DevBuf<uint64_t> bpart;  // brute-force partial top-K lists
+    // device copies of host queries and results (kg_search, kg_bruteforce):
+    // grown on demand and kept, so repeated calls allocate nothing
+    DevBuf<float> io_q, io_qn, io_dists;
+    DevBuf<uint32_t> io_ids, io_counts, io_comps, io_scans;
     unsigned vis_slots = 0, vis_words = 0;
     unsigned search_wpb = 4, search_bpc = 0;  // launch shape: waves per block (<= 4), blocks per CU (0 = as LDS allows)
     DevBuf<float> lat;      // per-query latency of the last search (microseconds)

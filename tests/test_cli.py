@@ -1,5 +1,6 @@
-"""The drop-in command-line tools (index / search, the reference's index.cpp
-and search.cpp): option handling that needs no GPU."""
+"""The drop-in command-line tools (index / search / prune / resave, the
+reference's index.cpp, search.cpp, prune.cpp and resave.cpp): option
+handling that needs no GPU."""
 import os
 import subprocess
 
@@ -40,3 +41,17 @@ def test_bad_options_fail_cleanly():
     assert r.returncode == 1 and "precision" in r.stderr
     r = _run("search", "-K")
     assert r.returncode == 1 and "missing value" in r.stderr
+
+
+def test_prune_and_resave_usage():
+    # both print their usage and exit 0 without an input index (prune.cpp:70-75,
+    # resave.cpp:69-72); unknown options fail
+    r = _run("prune", "data-only")
+    assert r.returncode == 0 and r.stdout.startswith("Usage: prune")
+    assert "--prune arg (=0)" in r.stdout
+    r = _run("resave")
+    assert r.returncode == 0 and "--no-dist" in r.stdout
+    r = _run("resave", "--bogus", "x")
+    assert r.returncode == 1 and "unknown option" in r.stderr
+    r = _run("prune", "--raw", "d", "i")
+    assert r.returncode == 1 and "--dim" in r.stderr

@@ -70,3 +70,37 @@ def test_index_then_search_with_eval(tmp_path):
     r = _run(os.path.join(BIN, "search"), data, index, query, "-K", str(K), "--linear", "--eval", gs)
     m = re.match(r"Time: (\S+) Recall: (\S+) Cost: (\S+)", r.stdout)
     assert float(m.group(2)) == 1.0 and float(m.group(3)) == 1.0
+
+
+@pytest.mark.parametrize("level", [1, 2, 3])
+def test_prune_tool_matches_oracle(tmp_path, level):
+    # prune DATA INPUT OUTPUT --prune L: the GPU prune of a saved index, whose
+    # file equals the oracle's prune of the same index, byte for byte
+    from kgraph_amd import datagen
+    X = datagen.uniform(3000, 8, 5)
+    ref = kgo.Index()
+    ref.build(X, 0, K=10, L=24, iterations=8, controls=10, reverse=-1)
+    data, src, out, want = (str(tmp_path / n) for n in ("d.lshkit", "in.kgraph", "out.kgraph", "want.kgraph"))
+    _lshkit(data, X)
+    ref.save(src)
+    r = _run(os.path.join(BIN, "prune"), data, src, out, "--prune", str(level))
+    assert re.search(r"\d+\.\d+s wall", r.stdout)
+    ref.prune(level)
+    ref.save(want)
+    assert open(out, "rb").read() == open(want, "rb").read()
+
+
+def test_resave_tool_no_dist_and_prune(tmp_path):
+    from kgraph_amd import datagen
+    X = datagen.uniform(2000, 8, 6)
+    ref = kgo.Index()
+    ref.build(X, 0, K=10, L=20, iterations=8, controls=10, reverse=-1)
+    src, out, want = (str(tmp_path / n) for n in ("in.kgraph", "out.kgraph", "want.kgraph"))
+    ref.save(src)
+    _run(os.path.join(BIN, "resave"), src, out, "--no-dist")
+    ref.save(want, 1)
+    assert open(out, "rb").read() == open(want, "rb").read()
+    _run(os.path.join(BIN, "resave"), src, out, "--prune", "1")
+    ref.prune(1)
+    ref.save(want)
+    assert open(out, "rb").read() == open(want, "rb").read()

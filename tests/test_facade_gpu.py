@@ -24,3 +24,7 @@ def test_cpp_facade_program(tmp_path):
     assert r["dist_ok"] == 1 and r["get_nn_L"] >= r["get_nn_M"] > 0
     assert r["opaque_rejected"] == 1
     assert "gfx950" in r["version"]
+    # concurrent callers, batched into shared launches, get what lone calls get
+    assert r["concurrent_same"] == r["queries"]
+    # rows changed in place are seen after kgraph::forget_device_rows()
+    assert r["forget_ok"] == 1

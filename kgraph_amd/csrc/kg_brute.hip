@@ -94,12 +94,17 @@ __global__ void __launch_bounds__(BT) brute_kernel(kg_brute_args a) {
             }
         }
         __syncthreads();
-        if (cnt + 64 > BUF || r0 + 64 >= re) {
-            unsigned n = K + cnt;
+        // every thread reads the count before any thread of the next step can
+        // add to it: a thread that read it late would see the next step's
+        // candidates, take the other branch and miss the sort's barriers
+        unsigned const c = cnt;
+        __syncthreads();
+        if (c + 64 > BUF || r0 + 64 >= re) {
+            unsigned n = K + c;
             unsigned sn = 2;
             while (sn < n) sn <<= 1;
             block_bitonic(keys, sn);
-            have = min(K, have + cnt);
+            have = min(K, have + c);
             for (unsigned i = K + threadIdx.x; i < SORTN; i += BT) keys[i] = ~0ull;
             __syncthreads();
             if (threadIdx.x == 0) {

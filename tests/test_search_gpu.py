@@ -309,6 +309,23 @@ def test_bruteforce_parity():
         assert np.array_equal(gi, bi) and np.array_equal(gd, bd), kind
 
 
+def test_bruteforce_parity_many_buffer_flushes():
+    # a block scans 40,000 rows per query and its candidate buffer fills and
+    # is folded in many times (every row is a candidate until K are found):
+    # every one of 3,000 queries must equal the oracle's exact top-K
+    import kgraph_amd
+    from kgraph_amd import datagen
+    X = datagen.uniform(40000, 16, 13)
+    Q = datagen.uniform(3000, 16, 14)
+    for metric, K in ((EUC, 25), (L2, 10)):
+        g = kgraph_amd.KGraph(X, "euclidean", l2sqr=(metric == L2))
+        for rep in range(2):
+            gi, gd = g.search(Q, K=K, withDistance=True)
+            bi, bd, bc = kgo.bruteforce(X, Q, K, metric)
+            bad = np.nonzero((gi != bi).any(1) | (gd != bd).any(1))[0]
+            assert len(bad) == 0, "queries %s differ" % bad[:10]
+
+
 def test_save_load_roundtrip_then_search(tmp_path):
     X, Q, idx = setup("uniform", EUC)
     p = str(tmp_path / "oracle.kgraph")

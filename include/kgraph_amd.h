@@ -174,6 +174,21 @@ int kg_search_device(kg_handle *h, const float *dQ, unsigned nq, size_t qstride,
 int kg_bruteforce(kg_handle *h, const float *Q, unsigned nq, size_t qstride, unsigned K, float epsilon,
                   unsigned *ids, float *dists, size_t ostride, unsigned *counts);
 
+/* Exact top-K by full scan that allocates nothing: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same results as
+ * kg_bruteforce, from device-resident queries (16-byte aligned rows, qstride
+ * a multiple of 4 floats) into device-resident ids / dists / counts (dists
+ * may be NULL).  Its scratch (query norms, partial top-K lists) is a 36 MiB
+ * device array of the library, one per GPU for the life of the process;
+ * nq <= 1,048,576 per call, larger calls are rejected with
+ * KG_ERR_INVALID_ARGUMENT.  Calls that overlap on one GPU race on that array:
+ * serialise them (one stream, or one call at a time).  Returns when the scan
+ * has finished. */
+int kg_bruteforce_device_noalloc(kg_handle *h, const float *dQ, unsigned nq, size_t qstride, unsigned K,
+                                 float epsilon, unsigned *d_ids, float *d_dists, size_t ostride,
+                                 unsigned *d_counts);
+
 /* KGraph::save / load (kgraph.h:159-165, kgraph.cpp:291-373), same
  * KNNGRAPH v2 byte format; FORMAT_TEXT is write-only. */
 int kg_save(kg_handle *h, const char *path, int format);

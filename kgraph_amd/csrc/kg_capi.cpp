@@ -10,6 +10,7 @@
 
 #include "kg_handle.hpp"
 #include "kg_internal.hpp"
+#include "kg_noalloc.hpp"
 
 namespace kga {
 
@@ -24,6 +25,7 @@ Handle::Handle() {
     KG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (auto &e : ev) KG_HIP(hipEventCreate(&e));
     KG_HIP(hipHostMalloc(reinterpret_cast<void **>(&host_flag), 16, hipHostMallocDefault));
+    noalloc_setup(device);  // the no-allocation scan's scratch address on this device (once per device)
 }
 
 Handle::~Handle() {
@@ -900,6 +902,13 @@ int kg_bruteforce(kg_handle *k, float const *Q, unsigned nq, size_t qstride, uns
         KG_HIP(hipMemcpy(ids, di.p, size_t(nq) * ostride * 4, hipMemcpyDeviceToHost));
         if (dists) KG_HIP(hipMemcpy(dists, dd.p, size_t(nq) * ostride * 4, hipMemcpyDeviceToHost));
         KG_HIP(hipMemcpy(counts, dc.p, 4ull * nq, hipMemcpyDeviceToHost));
+    });
+}
+
+int kg_bruteforce_device_noalloc(kg_handle *k, float const *dQ, unsigned nq, size_t qstride, unsigned K,
+                                 float epsilon, unsigned *d_ids, float *d_dists, size_t ostride, unsigned *d_counts) {
+    return guard(k, [&](Handle &h) {
+        kga::bruteforce_noalloc(h, dQ, nq, qstride, K, epsilon, d_ids, d_dists, ostride, d_counts);
     });
 }
 

@@ -127,7 +127,7 @@ static void set_data(Handle &h, float const *X, bool dev, unsigned N, unsigned D
 // Rows a walk gathers: 2 the 8-bit copy, 1 the BF16 copy, 0 the FP32 rows.
 static int walk_rows(Handle const &h) {
     if (!h.has_data || h.row_storage == 1) return 0;
-    if (h.row_storage == 0 && h.rows_u8) return 2;
+    if ((h.row_storage == 0 || h.row_storage == 3) && h.rows_u8) return 2;
     return h.rows_bf16 ? 1 : 0;
 }
 
@@ -236,6 +236,34 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
     // graph-walk queries
     unsigned const ngraph = nq - unsigned(brute.size());
     if (ngraph) {
+        if (!h.gnode_valid) {
+            h.gnode.reserve(std::max(N, 1u));
+            h.gflag.reserve(2);  // [0] some list repeats an id, [1] max over nodes of min(M, deg)
+            KG_HIP(hipMemsetAsync(h.gflag.p, 0, 8, h.stream));
+            KG_HIP(kg_launch_gnode(N, h.goff.p, h.gids.p, h.gM.p, h.gnode.p, h.gflag.p, h.stream));
+            KG_HIP(kg_launch_gpad_width(N, h.goff.p, h.gM.p, h.gflag.p + 1, h.stream));
+            uint32_t fl[2] = {0, 0};
+            KG_HIP(hipMemcpyAsync(fl, h.gflag.p, 8, hipMemcpyDeviceToHost, h.stream));
+            h.sync();
+            h.lists_unique = fl[0] == 0;
+            // padded adjacency: a walk with search M <= the row width reads
+            // nothing past min(max(M[n], M), deg(n)) of a list; rows of the
+            // widest such prefix (16-id multiples) cost at most 16 GiB
+            unsigned const width = (std::max(fl[1], 1u) + 15) & ~15u;
+            h.gpad.release();
+            h.gmd.release();
+            h.gpad_stride = 0;
+            if (width < 0xffffu && size_t(N) * width * 4 <= (size_t(16) << 30)) {
+                h.gpad.reserve(size_t(N) * width);
+                h.gmd.reserve(std::max(N, 1u));
+                KG_HIP(kg_launch_gpad(N, h.goff.p, h.gids.p, h.gM.p, width, h.gpad.p, h.gmd.p, h.stream));
+                h.gpad_stride = width;
+            }
+            h.gnode_valid = true;
+        }
+        bool const pad = h.search_adjacency == 0 && h.gpad_stride && sp.M <= h.gpad_stride;
+        // the T > 1 result list (a single trial's result is its pool's head)
+        unsigned const rcap = sp.T > 1 ? kmax + 1 : 0u;
         // seeds sorted per trial: init ids, or P random ones (largest P of the batch)
         unsigned const seed_cap = sp.init ? sp.init : std::max(poolmax, 1u);
         // visited hash set in LDS, in front of the HBM bitset.  Entries are
@@ -252,7 +280,7 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
             return (h.search_visited == 0 && idbits <= t + 10) ? 16u : 32u;
         };
         auto lds_for = [&](unsigned c) {
-            return kg_search_lds_per_wave(poolmax, kmax + 1, h.Dp, seed_cap, c, c ? bits_for(c) : 32u);
+            return kg_search_lds_per_wave(poolmax, rcap, h.Dp, seed_cap, c, c ? bits_for(c) : 32u, !pad);
         };
         auto waves = [&](unsigned c) { return std::min<size_t>(kg_search_waves_per_cu(), h.lds_per_cu / lds_for(c)); };
         unsigned vcap = 0;
@@ -289,31 +317,6 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
             h.vis_slots = slots;
             h.vis_words = words;
         }
-        if (!h.gnode_valid) {
-            h.gnode.reserve(std::max(N, 1u));
-            h.gflag.reserve(2);  // [0] some list repeats an id, [1] max over nodes of min(M, deg)
-            KG_HIP(hipMemsetAsync(h.gflag.p, 0, 8, h.stream));
-            KG_HIP(kg_launch_gnode(N, h.goff.p, h.gids.p, h.gM.p, h.gnode.p, h.gflag.p, h.stream));
-            KG_HIP(kg_launch_gpad_width(N, h.goff.p, h.gM.p, h.gflag.p + 1, h.stream));
-            uint32_t fl[2] = {0, 0};
-            KG_HIP(hipMemcpyAsync(fl, h.gflag.p, 8, hipMemcpyDeviceToHost, h.stream));
-            h.sync();
-            h.lists_unique = fl[0] == 0;
-            // padded adjacency: a walk with search M <= the row width reads
-            // nothing past min(max(M[n], M), deg(n)) of a list; rows of the
-            // widest such prefix (16-id multiples) cost at most 16 GiB
-            unsigned const width = (std::max(fl[1], 1u) + 15) & ~15u;
-            h.gpad.release();
-            h.gmd.release();
-            h.gpad_stride = 0;
-            if (width < 0xffffu && size_t(N) * width * 4 <= (size_t(16) << 30)) {
-                h.gpad.reserve(size_t(N) * width);
-                h.gmd.reserve(std::max(N, 1u));
-                KG_HIP(kg_launch_gpad(N, h.goff.p, h.gids.p, h.gM.p, width, h.gpad.p, h.gmd.p, h.stream));
-                h.gpad_stride = width;
-            }
-            h.gnode_valid = true;
-        }
         kg_search_args a{};
         a.X = h.X.p;
         int const form = walk_rows(h);
@@ -327,12 +330,14 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         a.gids = h.gids.p;
         a.gM = h.gM.p;
         a.gnode = h.gnode.p;
-        bool const pad = h.search_adjacency == 0 && h.gpad_stride && sp.M <= h.gpad_stride;
         a.gpad = pad ? h.gpad.p : nullptr;
         a.gmd = pad ? h.gmd.p : nullptr;
         a.gstride = pad ? h.gpad_stride : 0u;
         h.last_adjacency = pad ? 0 : 1;
         a.lists_unique = h.lists_unique ? 1u : 0u;
+        // exact integer distances over the 8-bit rows (integer queries):
+        // sums stay below 2^24 while D 255^2 < 2^24
+        a.int_dot = (form == 2 && h.row_storage == 0 && h.D <= 258) ? 1u : 0u;
         a.Q = Qk;
         a.qnorms = h.qn.p;
         a.nq = nq;
@@ -367,7 +372,7 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         a.touch_cap = tcap;
         a.qcounter = h.qcounter.p;
         a.pool_cap = poolmax;
-        a.res_cap = kmax + 1;
+        a.res_cap = rcap;
         a.lds_per_wave = unsigned(lpw);
         a.vcap = vcap;
         a.spills = h.qcounter.p + 1;
@@ -798,7 +803,7 @@ int kg_search_visited_stats(kg_handle *k, unsigned *vcap, unsigned *spilled) {
 
 int kg_set_row_storage(kg_handle *k, int mode) {
     return guard(k, [&](Handle &h) {
-        if (mode < 0 || mode > 2) throw kga::invalid_argument("row storage mode must be 0, 1 or 2");
+        if (mode < 0 || mode > 3) throw kga::invalid_argument("row storage mode must be 0, 1, 2 or 3");
         h.row_storage = mode;
     });
 }

@@ -212,6 +212,53 @@ __device__ __forceinline__ float2 quad_metric2_bf16(uint16_t const *__restrict__
     return make_float2(acc0, acc1);
 }
 
+// Two rows of the 8-bit copy per quad against a query held as 8-bit words
+// (q8, Dp / 4 words), in integer arithmetic: (a0 . q, a1 . q) and (|a0|^2,
+// |a1|^2), summed over the quad (every lane gets the totals).  v_dot4_u32_u8
+// takes four 8-bit products per instruction; the sums are exact, so when every
+// row and query value is an integer in [0, 255] and D 255^2 < 2^24 they equal
+// the canonical FP32 dot products and norms (whose partial sums are then
+// exact too) -- and |a - q|^2 = |a|^2 + |q|^2 - 2 a.q exactly.
+__device__ __forceinline__ int quad_isum(int v) {
+    v += __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);
+    v += __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);
+    return v;
+}
+__device__ __forceinline__ void quad_dot2_u8(uint8_t const *__restrict__ a0, uint8_t const *__restrict__ a1,
+                                             uint32_t const *__restrict__ q8, unsigned nw, unsigned t, uint32_t &d0,
+                                             uint32_t &d1, uint32_t &n0, uint32_t &n1) {
+    uint32_t const *r0 = reinterpret_cast<uint32_t const *>(a0);
+    uint32_t const *r1 = reinterpret_cast<uint32_t const *>(a1);
+    uint32_t s0 = 0, s1 = 0, m0 = 0, m1 = 0;
+    unsigned k = t;
+    for (; k + 4 * (QM_GROUP - 1) < nw; k += 4 * QM_GROUP) {
+        uint32_t x0[QM_GROUP], x1[QM_GROUP];
+#pragma unroll
+        for (int g = 0; g < QM_GROUP; ++g) x0[g] = r0[k + 4 * g];
+#pragma unroll
+        for (int g = 0; g < QM_GROUP; ++g) x1[g] = r1[k + 4 * g];
+#pragma unroll
+        for (int g = 0; g < QM_GROUP; ++g) {
+            uint32_t const y = q8[k + 4 * g];
+            s0 = __builtin_amdgcn_udot4(x0[g], y, s0, false);
+            s1 = __builtin_amdgcn_udot4(x1[g], y, s1, false);
+            m0 = __builtin_amdgcn_udot4(x0[g], x0[g], m0, false);
+            m1 = __builtin_amdgcn_udot4(x1[g], x1[g], m1, false);
+        }
+    }
+    for (; k < nw; k += 4) {
+        uint32_t const y = q8[k], x = r0[k], z = r1[k];
+        s0 = __builtin_amdgcn_udot4(x, y, s0, false);
+        s1 = __builtin_amdgcn_udot4(z, y, s1, false);
+        m0 = __builtin_amdgcn_udot4(x, x, m0, false);
+        m1 = __builtin_amdgcn_udot4(z, z, m1, false);
+    }
+    d0 = uint32_t(quad_isum(int(s0)));
+    d1 = uint32_t(quad_isum(int(s1)));
+    n0 = uint32_t(quad_isum(int(m0)));
+    n1 = uint32_t(quad_isum(int(m1)));
+}
+
 __device__ __forceinline__ float4 widen_u8x4(uint32_t v) {
     return make_float4(float(v & 0xffu), float((v >> 8) & 0xffu), float((v >> 16) & 0xffu), float(v >> 24));
 }

@@ -25,6 +25,7 @@ struct kg_search_args {
     uint32_t const *gmd;
     unsigned gstride;
     unsigned lists_unique;  // 1: no adjacency list repeats an id
+    unsigned int_dot;       // 1: over the 8-bit rows, integer queries take the exact v_dot4 path
     // queries
     float const *Q;
     float const *qnorms;
@@ -113,7 +114,7 @@ struct kg_rescore_args {
 
 // Launchers (defined in the .hip files).
 size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp, unsigned seed_cap, unsigned vcap,
-                              unsigned vt_bits);
+                              unsigned vt_bits, bool csr_base);
 // Largest waves per block search_kernel is compiled for (its launch bound);
 // kg_launch_search rejects larger shapes.
 unsigned kg_search_max_wpb();

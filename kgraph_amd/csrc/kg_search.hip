@@ -46,6 +46,7 @@ struct WaveLDS {
     uint32_t *rid;
     float *rdist;
     float *q;
+    uint32_t *q8;  // the query as 8-bit words (integer path over the 8-bit rows)
     uint32_t *fid;
     float *fdist;
     uint32_t *vt;  // visited hash set (vcap entries of 32 or 16 bits, all ones = free)
@@ -54,21 +55,24 @@ struct WaveLDS {
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
 struct LdsLayout {
-    size_t pid, pdist, pmeta, pbase, tkey, rid, rdist, q, fid, fdist, vt, total;
+    size_t pid, pdist, pmeta, pbase, tkey, rid, rdist, q, q8, fid, fdist, vt, total;
     // tcap: starting points sorted at once (P random seeds, or init ids);
     // vcap: entries of the visited hash set (0: none)
     // vbytes: bytes per visited-set entry (4, or 2 for the compact form)
+    // base: pool entries keep their CSR base offset (walks over the CSR
+    // lists; the padded rows need none)
     __host__ __device__ LdsLayout(unsigned cap, unsigned rcap, unsigned Dp, unsigned tcap, unsigned vcap,
-                                  unsigned vbytes) {
+                                  unsigned vbytes, bool base) {
         size_t o = 0;
         pid = o; o = align16(o + 4ull * cap);
         pdist = o; o = align16(o + 4ull * cap);
         pmeta = o; o = align16(o + 4ull * cap);
-        pbase = o; o = align16(o + 4ull * cap);
+        pbase = o; o = align16(o + (base ? 4ull * cap : 0ull));
         tkey = o; o = align16(o + 8ull * tcap);
         rid = o; o = align16(o + 4ull * rcap);
         rdist = o; o = align16(o + 4ull * rcap);
         q = o; o = align16(o + 4ull * Dp);
+        q8 = o; o = align16(o + Dp);
         fid = o; o = align16(o + 4ull * KG_WAVE);
         fdist = o; o = align16(o + 4ull * KG_WAVE);
         vt = o; o = align16(o + size_t(vbytes) * vcap);
@@ -95,9 +99,15 @@ __device__ __forceinline__ uint32_t actual_M(kg_search_args const &a, uint32_t i
 // Rows are gathered from a.X (RowT = float), from the lossless BF16 copy a.Xb
 // (RowT = uint16_t) or from the lossless 8-bit copy a.X8 (RowT = uint8_t);
 // all give the same distances.
+// The query as 8-bit words is usable (on) and its integer |q|^2 (sq).
+struct QInt {
+    bool on;
+    uint32_t sq;
+};
+
 template <int METRIC, typename RowT>
 __device__ __forceinline__ void batch_dist(kg_search_args const &a, uint32_t const *ids, float *out, unsigned cnt,
-                                           float const *q, float qn) {
+                                           float const *q, float qn, uint32_t const *q8, QInt qv) {
     RowT const *X;
     if constexpr (sizeof(RowT) == 1)
         X = a.X8;
@@ -108,6 +118,33 @@ __device__ __forceinline__ void batch_dist(kg_search_args const &a, uint32_t con
     unsigned const lane = lane_id();
     unsigned const grp = lane >> 2, t = lane & 3;
     if constexpr (sizeof(RowT) == 1) {
+        if (qv.on) {
+            // integer path: v_dot4 over 8-bit words, exact (kg_device.hpp
+            // quad_dot2_u8), then the metric's FP32 finish on exact values
+            for (unsigned b0 = 0; b0 < cnt; b0 += 32) {
+                unsigned const c0 = b0 + grp, c1 = c0 + 16;
+                if (c0 < cnt) {
+                    bool const two = c1 < cnt;
+                    uint32_t const id0 = ids[c0] & IDMASK, id1 = two ? (ids[c1] & IDMASK) : id0;
+                    uint32_t d0, d1, n0, n1;
+                    quad_dot2_u8(X + size_t(id0) * a.Dp, X + size_t(id1) * a.Dp, q8, a.Dp / 4, t, d0, d1, n0, n1);
+                    float r0, r1;
+                    if (METRIC == METRIC_L2SQR) {
+                        r0 = float(n0 + qv.sq - 2u * d0);
+                        r1 = float(n1 + qv.sq - 2u * d1);
+                    } else {
+                        r0 = finish_metric<METRIC>(float(d0), a.norms[id0], qn);
+                        r1 = finish_metric<METRIC>(float(d1), a.norms[id1], qn);
+                    }
+                    if (t == 0) {
+                        out[c0] = r0;
+                        if (two) out[c1] = r1;
+                    }
+                }
+            }
+            wave_sync();
+            return;
+        }
         // 8-bit rows: one 128-byte line per 128-value row, two rows per quad
         for (unsigned b0 = 0; b0 < cnt; b0 += 32) {
             unsigned const c0 = b0 + grp, c1 = c0 + 16;
@@ -170,33 +207,6 @@ __device__ __forceinline__ unsigned insert_pos(float const *pdist, unsigned L, f
     return r;
 }
 
-// pool[r..L) -> pool[r+1..L+1); slot L is the pool's scratch slot.
-__device__ __forceinline__ void pool_shift(WaveLDS &w, unsigned r, unsigned L) {
-    if (r >= L) return;
-    unsigned const lane = lane_id();
-    int const lo = int(r & ~(KG_WAVE - 1u));
-    for (int c = int((L - 1) & ~(KG_WAVE - 1u)); c >= lo; c -= KG_WAVE) {
-        unsigned i = unsigned(c) + lane;
-        bool mv = (i >= r) && (i < L);
-        uint32_t id = 0, meta = 0, bs = 0;
-        float d = 0.f;
-        if (mv) {
-            id = w.pid[i];
-            d = w.pdist[i];
-            meta = w.pmeta[i];
-            bs = w.pbase[i];
-        }
-        wave_sync();
-        if (mv) {
-            w.pid[i + 1] = id;
-            w.pdist[i + 1] = d;
-            w.pmeta[i + 1] = meta;
-            w.pbase[i + 1] = bs;
-        }
-        wave_sync();
-    }
-}
-
 // The reference inserts the fresh neighbours of a batch one at a time
 // (UpdateKnnList, kgraph.cpp:497-512): each goes after every pool entry with a
 // smaller or equal distance, the pool grows to K + P entries and then drops
@@ -255,7 +265,7 @@ __device__ void batch_insert(WaveLDS &w, unsigned nf, uint32_t cm, uint32_t cb, 
             id = w.pid[i];
             d = w.pdist[i];
             meta = w.pmeta[i];
-            bs = w.pbase[i];
+            if (w.pbase) bs = w.pbase[i];
             unsigned lo = 0, hi = nf;
             while (lo < hi) {
                 unsigned const mid = (lo + hi) >> 1;
@@ -268,7 +278,7 @@ __device__ void batch_insert(WaveLDS &w, unsigned nf, uint32_t cm, uint32_t cb, 
             w.pid[np] = id;
             w.pdist[np] = d;
             w.pmeta[np] = meta;
-            w.pbase[np] = bs;
+            if (w.pbase) w.pbase[np] = bs;
         }
         wave_sync();
     }
@@ -276,7 +286,7 @@ __device__ void batch_insert(WaveLDS &w, unsigned nf, uint32_t cm, uint32_t cb, 
         w.pid[fin] = sid | FLAG;
         w.pdist[fin] = sd;
         w.pmeta[fin] = sm << 16;
-        w.pbase[fin] = sb;
+        if (w.pbase) w.pbase[fin] = sb;
     }
     wave_sync();
     L = Lf;
@@ -381,17 +391,18 @@ search_kernel(kg_search_args a) {
     unsigned const slot = blockIdx.x * (blockDim.x / KG_WAVE) + wib;
     unsigned const cap = a.pool_cap;
     unsigned const rcap = a.res_cap;
-    LdsLayout lay(cap, rcap, a.Dp, a.seed_cap, a.vcap, a.vt_bits / 8);
+    LdsLayout lay(cap, rcap, a.Dp, a.seed_cap, a.vcap, a.vt_bits / 8, a.gpad == nullptr);
     unsigned char *base = smem + size_t(wib) * a.lds_per_wave;
     WaveLDS w;
     w.pid = reinterpret_cast<uint32_t *>(base + lay.pid);
     w.pdist = reinterpret_cast<float *>(base + lay.pdist);
     w.pmeta = reinterpret_cast<uint32_t *>(base + lay.pmeta);
-    w.pbase = reinterpret_cast<uint32_t *>(base + lay.pbase);
+    w.pbase = a.gpad ? nullptr : reinterpret_cast<uint32_t *>(base + lay.pbase);
     w.tkey = reinterpret_cast<uint64_t *>(base + lay.tkey);
     w.rid = reinterpret_cast<uint32_t *>(base + lay.rid);
     w.rdist = reinterpret_cast<float *>(base + lay.rdist);
     w.q = reinterpret_cast<float *>(base + lay.q);
+    w.q8 = reinterpret_cast<uint32_t *>(base + lay.q8);
     w.fid = reinterpret_cast<uint32_t *>(base + lay.fid);
     w.fdist = reinterpret_cast<float *>(base + lay.fdist);
     w.vt = reinterpret_cast<uint32_t *>(base + lay.vt);
@@ -424,6 +435,28 @@ search_kernel(kg_search_args a) {
         float const *qsrc = a.Q + size_t(qi) * a.qstride;
         for (unsigned i = lane; i < a.Dp; i += KG_WAVE) w.q[i] = (i < a.D) ? qsrc[i] : 0.f;
         float const qn = (METRIC == METRIC_L2SQR) ? 0.f : a.qnorms[qi];
+        // over the 8-bit rows, a query whose values are all integers in
+        // [0, 255] takes the integer path (exact, so the same distances)
+        QInt qv{false, 0u};
+        if constexpr (sizeof(RowT) == 1) {
+            wave_sync();
+            bool ok = true;
+            uint32_t sq = 0;
+            for (unsigned wd = lane; wd < a.Dp / 4; wd += KG_WAVE) {
+                uint32_t word = 0;
+                for (unsigned j = 0; j < 4; ++j) {
+                    float const v = w.q[4 * wd + j];
+                    ok &= v >= 0.f && v <= 255.f && v == rintf(v);
+                    uint32_t const u = ok ? uint32_t(v) : 0u;
+                    word |= u << (8 * j);
+                    sq += u * u;
+                }
+                w.q8[wd] = word;
+            }
+            for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
+            qv.on = a.int_dot && !ballot(!ok);
+            qv.sq = sq;
+        }
         wave_sync();
 
         unsigned ntouch = 0, n_comps = 0, n_scan = 0, RL = 0;
@@ -510,7 +543,7 @@ search_kernel(kg_search_args a) {
                     ntouch += L;
                 }
             }
-            batch_dist<METRIC, RowT>(a, w.pid, w.pdist, L, w.q, qn);
+            batch_dist<METRIC, RowT>(a, w.pid, w.pdist, L, w.q, qn, w.q8, qv);
             for (unsigned i = lane; i < L; i += KG_WAVE)
                 w.tkey[i] = (uint64_t(ord_f32(w.pdist[i])) << 32) | w.pid[i];
             wave_sync();
@@ -532,7 +565,7 @@ search_kernel(kg_search_args a) {
                 uint32_t bs = 0;
                 uint32_t const m = a.gpad ? VT_M_UNKNOWN : actual_M(a, w.pid[i] & IDMASK, &bs);
                 w.pmeta[i] = m << 16;
-                w.pbase[i] = bs;
+                if (w.pbase) w.pbase[i] = bs;
             }
             wave_sync();
 
@@ -628,7 +661,7 @@ search_kernel(kg_search_args a) {
                     // lane j fetches candidate j's M and CSR base while the rows load
                     uint32_t cm = VT_M_UNKNOWN, cb = 0;
                     if (lane < nf && !a.gpad) cm = actual_M(a, w.fid[lane], &cb);
-                    batch_dist<METRIC, RowT>(a, w.fid, w.fdist, nf, w.q, qn);
+                    batch_dist<METRIC, RowT>(a, w.fid, w.fdist, nf, w.q, qn, w.q8, qv);
                     // the nf inserts of this batch, done at once (batch_insert)
                     n_comps += nf;
                     batch_insert(w, nf, cm, cb, qcap, L, k);
@@ -811,8 +844,8 @@ hipError_t kg_launch_gnode(unsigned N, uint32_t const *goff, uint32_t const *gid
 }
 
 size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp, unsigned seed_cap, unsigned vcap,
-                              unsigned vt_bits) {
-    return LdsLayout(pool_cap, res_cap, Dp, seed_cap, vcap, vt_bits / 8).total;
+                              unsigned vt_bits, bool csr_base) {
+    return LdsLayout(pool_cap, res_cap, Dp, seed_cap, vcap, vt_bits / 8, csr_base).total;
 }
 
 unsigned kg_search_max_wpb() { return SEARCH_MAX_WPB; }

@@ -102,14 +102,16 @@ def test_row_storage_forms_give_identical_walks(kind, metric, d, lossless):
     g._h.call("kg_row_storage", ctypes.byref(active), ctypes.byref(exact))
     assert active.value == (2 if lossless else 0) and exact.value == (3 if lossless else 0)
     res = {}
-    for mode, want in ((0, 2), (1, 0), (2, 1)):
+    # mode 3: the 8-bit copy with FP32 arithmetic (mode 0 takes the exact
+    # integer path over it when the queries are integers in [0, 255])
+    for mode, want in ((0, 2), (1, 0), (2, 1), (3, 2)):
         g._h.call("kg_set_row_storage", mode)
         g._h.call("kg_row_storage", ctypes.byref(active), None)
         assert active.value == (want if lossless else 0)
         gi, gd = g.search(Q, K=10, P=40, withDistance=True, stats=True)
         res[mode] = (gi, gd, g.last_counts.copy(), g.last_comps.copy(), g.last_scans.copy())
     g._h.call("kg_set_row_storage", 0)
-    for mode in (1, 2):
+    for mode in (1, 2, 3):
         for a, b in zip(res[0], res[mode]):
             assert np.array_equal(a, b)
     ids, dd, c, comps, scans = idx.search(X, Q, metric, K=10, P=40)

@@ -281,6 +281,11 @@ int kg_set_entry_projection(kg_handle *h, unsigned shortlist);
 int kg_autotune_search(kg_handle *h, const float *dQ, unsigned nq, size_t qstride, const kg_search_params *params,
                        const unsigned *Ks, const unsigned *Ps, unsigned *chosen);
 
+/* Launch shape of later walks (no counterpart in the reference): waves per
+ * block (1, 2 or 4) and at most blocks_per_cu resident blocks per CU (0: as
+ * many as the LDS and the 32-wave limit allow).  Results do not depend on it. */
+int kg_set_search_launch(kg_handle *h, unsigned waves_per_block, unsigned blocks_per_cu);
+
 /* Per-query latency of the last kg_search[_device] call in microseconds,
  * stamped inside the search kernel from the query's first to its last
  * instruction (walk queries only; brute-force and skipped rows read 0).

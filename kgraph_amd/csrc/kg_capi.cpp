@@ -1132,6 +1132,16 @@ int kg_set_entry_projection(kg_handle *k, unsigned shortlist) {
     });
 }
 
+int kg_set_search_launch(kg_handle *k, unsigned waves_per_block, unsigned blocks_per_cu) {
+    return guard(k, [&](Handle &h) {
+        if (waves_per_block != 1 && waves_per_block != 2 && waves_per_block != 4)
+            throw kga::invalid_argument("waves per block must be 1, 2 or 4");
+        if (blocks_per_cu > 32) throw kga::invalid_argument("blocks per CU must be <= 32");
+        h.search_wpb = waves_per_block;
+        h.search_bpc = blocks_per_cu;
+    });
+}
+
 int kg_autotune_search(kg_handle *k, float const *dQ, unsigned nq, size_t qstride, kg_search_params const *params,
                        unsigned const *Ks, unsigned const *Ps, unsigned *chosen) {
     return guard(k, [&](Handle &h) {

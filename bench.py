@@ -170,7 +170,7 @@ def main():
             h.call("kg_build_counters", ptr(cnt), 6)
             builds[form] = dict(s=time.perf_counter() - tb, it=int(info.iterations), recall=float(info.recall),
                                 iter_ms=ctypes_float(h, 8), graph_replays=int(cnt[5]),
-                                join_form=["fp32", "bf16-screen", "bf16-exact"][int(cnt[4])])
+                                join_form=["fp32", "bf16-screen", "bf16-exact", "fp32-mfma-screen"][int(cnt[4])])
             log("build %s: %.3fs (%d it, control recall %.4f, join %s, iterations %.1f ms, %d graph replays)" % (
                 form, builds[form]["s"], info.iterations, info.recall, builds[form]["join_form"],
                 builds[form]["iter_ms"], builds[form]["graph_replays"]))
@@ -404,7 +404,7 @@ def batched_workload(args):
     h.call("kg_build_iterations", None, 0, _lib.ctypes.byref(nit))
     its = np.zeros(6 * nit.value, np.uint64)
     h.call("kg_build_iterations", ptr(its), nit.value, _lib.ctypes.byref(nit))
-    build_forms = "".join("TSEP"[int(f)] for f in its.reshape(-1, 6)[:, 3])
+    build_forms = "".join("TSEPF"[int(f)] for f in its.reshape(-1, 6)[:, 3])
     W = 100
     dQ = devmem.DeviceArray.from_host(Q)
     dI = devmem.DeviceArray((nq, W), np.uint32)
@@ -609,8 +609,8 @@ def precision_workload(args):
                           "entry_ms": round(ems, 3), "walk_ms": round(ctypes_float(h, 0), 3)})
         forms[form] = {"build_s": round(bs, 4), "iterations": int(info.iterations),
                        "control_recall": round(float(info.recall), 4), "graph_recall@10": round(grec, 4),
-                       "join_form": ["fp32", "bf16-screen", "bf16-exact"][int(cnt[4])],
-                       "iteration_join_forms": "".join("TSEP"[int(f)] for f in its[:, 3]),
+                       "join_form": ["fp32", "bf16-screen", "bf16-exact", "fp32-mfma-screen"][int(cnt[4])],
+                       "iteration_join_forms": "".join("TSEPF"[int(f)] for f in its[:, 3]),
                        "search": sweep}
         log("%s: build %.3fs (%d it, control recall %.4f, graph recall@10 %.4f), stored err max %.3g; search %s" % (
             form, bs, info.iterations, info.recall, grec, serr.max(), sweep))

@@ -46,8 +46,11 @@ enum { KG_FORMAT_DEFAULT = 0, KG_FORMAT_NO_DIST = 1, KG_FORMAT_TEXT = 128 };
 /* Screening precision of the build's local join and of search (no
  * counterpart in the reference): FP32 scores every pair exactly; BF16 scores
  * candidates with BF16 dot products on MFMA first and re-scores every
- * admission in exact FP32. */
-enum { KG_PRECISION_FP32 = 0, KG_PRECISION_BF16 = 1 };
+ * admission in exact FP32; FP32_MFMA screens the local join with FP32
+ * operands on MFMA (v_mfma_f32_16x16x4_f32, an error bound about 500 times
+ * tighter than BF16's) and re-scores the same way.  Every form admits exactly
+ * the candidates the exact FP32 comparison admits. */
+enum { KG_PRECISION_FP32 = 0, KG_PRECISION_BF16 = 1, KG_PRECISION_FP32_MFMA = 2 };
 
 /* KGraph::IndexParams (kgraph.h:94-110), same fields, order and meaning. */
 typedef struct {
@@ -297,7 +300,8 @@ int kg_search_latencies(kg_handle *h, float *out_us, unsigned n);
  * kgraph.cpp:805, 814), out[1] join-list entries (sum over nodes of
  * |new| + |old|), out[2] candidates pushed to merge slots, out[3]
  * iterations, out[4] local-join form (0 FP32 quads, 1 BF16 MFMA screen with
- * exact FP32 re-score, 2 BF16 MFMA exact on these rows), out[5] iterations
+ * exact FP32 re-score, 2 BF16 MFMA exact on these rows, 3 FP32-operand MFMA
+ * screen with exact FP32 re-score), out[5] iterations
  * run as replays of the captured iteration graph.  n = number of entries
  * wanted (<= 6). */
 int kg_build_counters(kg_handle *h, uint64_t *out, unsigned n);
@@ -316,7 +320,8 @@ int kg_set_join_form(kg_handle *h, int form);
 /* Per-iteration record of the last build (no counterpart in the reference):
  * 6 values per iteration -- pairs scored, candidates pushed, join-list
  * entries, join form (0 FP32 tiles, 1 BF16 screen + exact re-score, 2 exact
- * BF16 tiles, 3 probe: the nodes split between screen and tiles), join
+ * BF16 tiles, 3 probe: the nodes split between screen and tiles, 4 FP32-operand
+ * MFMA screen + exact re-score), join
  * kernel time in microseconds (0 when the iteration ran as a graph replay),
  * whole-iteration device time in microseconds.  Copies
  * min(cap, iterations) records; *n = iterations. */

@@ -11,8 +11,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import (FORMAT_DEFAULT, FORMAT_NO_DIST, FORMAT_TEXT, METRIC_ANGULAR, METRIC_EUCLIDEAN,
-                   METRIC_L2SQR, PRECISION_BF16, PRECISION_FP32, Handle, IndexInfo, IndexParams,
-                   SearchParams, f32c, ptr, u32c)
+                   METRIC_L2SQR, PRECISION_BF16, PRECISION_FP32, PRECISION_FP32_MFMA, Handle, IndexInfo,
+                   IndexParams, SearchParams, f32c, ptr, u32c)
 
 # kgraph.h:14-37
 default_iterations = 30
@@ -88,7 +88,8 @@ class KGraph:
         p = IndexParams(iterations, L, K, S, R, controls, seed, delta, recall, prune, reverse)
         if precision is not None:
             if isinstance(precision, str):
-                precision = {"fp32": _lib.PRECISION_FP32, "bf16": _lib.PRECISION_BF16}[precision]
+                precision = {"fp32": _lib.PRECISION_FP32, "bf16": _lib.PRECISION_BF16,
+                             "fp32-mfma": _lib.PRECISION_FP32_MFMA}[precision]
             self._h.call("kg_set_precision", precision)
         if graphs is not None:
             self._h.call("kg_set_build_graphs", 1 if graphs else 0)
@@ -263,7 +264,7 @@ class KGraph:
         captured iteration graph."""
         out = np.zeros(6, dtype=np.uint64)
         self._h.call("kg_build_counters", ptr(out), 6)
-        forms = {0: "fp32", 1: "bf16-screen", 2: "bf16-exact"}
+        forms = {0: "fp32", 1: "bf16-screen", 2: "bf16-exact", 3: "fp32-mfma-screen"}
         return {"pairs": int(out[0]), "list_entries": int(out[1]), "pushes": int(out[2]),
                 "iterations": int(out[3]), "join_form": forms.get(int(out[4]), "fp32"),
                 "graph_replays": int(out[5])}

@@ -28,6 +28,7 @@ FORMAT_TEXT = 128
 
 PRECISION_FP32 = 0
 PRECISION_BF16 = 1
+PRECISION_FP32_MFMA = 2
 
 JOIN_AUTO = 0
 JOIN_SCREEN = 1

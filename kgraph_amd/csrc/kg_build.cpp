@@ -68,15 +68,19 @@ struct NndState {
 // bits, round to nearest even) changes each operand by at most u = 2^-8
 // relative, so a product of two rounded operands (exact in FP32) is off by at
 // most (2u + u^2) |a_i b_i|; the MFMA's FP32 accumulation of D (+ zero
-// padding) products adds at most gamma = (D + 16) 2^-23 times the sum of their
-// magnitudes.  With sum |a_i b_i| <= |a| |b|:
+// padding to a multiple of 32) products adds at most gamma = (D + 32) 2^-23
+// times the sum of their magnitudes.  With sum |a_i b_i| <= |a| |b|:
 //   |G - a.b| <= c |a| |b|,  c = (2u + u^2 + gamma) * 1.02,
 // the 2 % margin covering (1+u)^2 on the gamma term and the rounding of the
 // bound itself.  g = 4 gamma bounds the relative error of the stored norms and
 // of the few FP32 operations that turn G into a distance.
-void screen_constants(unsigned D, NndArgs &a) {
-    double const gam = (D + 16) * std::ldexp(1.0, -23);
-    a.scr_c = float((std::ldexp(1.0, -7) + std::ldexp(1.0, -16) + gam) * 1.02);
+//
+// FP32 operands (f32): no operand rounding; each product is off by at most
+// 2^-24 relative and the accumulation adds gamma, so c = (2^-24 + gamma) * 1.02.
+void screen_constants(unsigned D, NndArgs &a, bool f32 = false) {
+    double const gam = (D + 32) * std::ldexp(1.0, -23);
+    double const prod = f32 ? std::ldexp(1.0, -24) : std::ldexp(1.0, -7) + std::ldexp(1.0, -16);
+    a.scr_c = float((prod + gam) * 1.02);
     a.scr_g = float(4 * gam);
 }
 
@@ -252,8 +256,17 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
         a.mfma = 1;
         a.exact = bf16_tiles_exact(h.D, h.metric, rr.integer, h.rows_bf16, rr.maxabs, rr.vmin, rr.vmax) ? 1u : 0u;
         screen_constants(h.D, a);
+    } else if (h.precision == KG_PRECISION_FP32_MFMA && h.D <= 256) {
+        // local join screened with FP32 operands on MFMA, exact FP32 re-score
+        st.sqn.reserve(N);
+        (void)row_range(h, a, st.sqn.p, st.flags, s);
+        a.sqn = st.sqn.p;
+        a.mfma = 1;
+        a.f32 = 1;
+        a.exact = 0;
+        screen_constants(h.D, a, true);
     }
-    h.join_exact = a.mfma ? int(a.exact) : -1;
+    h.join_exact = a.mfma ? (a.f32 ? 2 : int(a.exact)) : -1;
     KG_HIP(hipMemsetAsync(st.ovf.p, 0xff, size_t(N) * a.O * 8, s));
     KG_HIP(hipMemsetAsync(st.found.p, 0, size_t(N) * 4, s));
     KG_HIP(hipMemsetAsync(st.iter.p, 0, 4, s));
@@ -340,7 +353,7 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
         bool const lead = it == 0 ? true : use_mfma;
         unsigned const cut = half;
         NndArgs const &ar = use_mfma ? a : a_tile;
-        int rec_form = probe ? 3 : (ar.mfma ? (ar.exact ? 2 : 1) : 0);
+        int rec_form = probe ? 3 : (ar.mfma ? (ar.f32 ? 4 : (ar.exact ? 2 : 1)) : 0);
         float jms = 0.f;
         if (h.build_graphs && !probe && it > 0) {
             IterGraph &gr = graph[use_mfma ? 1 : 0];

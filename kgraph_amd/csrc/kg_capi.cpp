@@ -747,7 +747,7 @@ int kg_set_data_device(kg_handle *k, float const *dX, unsigned N, unsigned D, si
 
 int kg_set_precision(kg_handle *k, int precision) {
     return guard(k, [&](Handle &h) {
-        if (precision != KG_PRECISION_FP32 && precision != KG_PRECISION_BF16)
+        if (precision != KG_PRECISION_FP32 && precision != KG_PRECISION_BF16 && precision != KG_PRECISION_FP32_MFMA)
             throw kga::invalid_argument("unknown precision");
         h.precision = precision;
     });

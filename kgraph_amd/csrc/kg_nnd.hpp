@@ -47,6 +47,7 @@ struct NndArgs {
     // MFMA join (join_mfma_kernel)
     float2 const *sqn;  // per row: (|x|^2, |x|)
     unsigned mfma;      // 1: tile join on the matrix cores
+    unsigned f32;       // with mfma: FP32 operands (v_mfma_f32_16x16x4_f32) instead of BF16
     unsigned exact;     // 1: BF16 tiles are exact for these rows (no re-score)
     float scr_c, scr_g; // screen bound: relative dot error, relative norm error
 };

@@ -192,9 +192,10 @@ __device__ __forceinline__ void put_cand(NndArgs const &a, uint32_t target, unsi
 // per node; the node's list (new rows, then old rows) is in LDS.  Every 16 x 16
 // block of (new row, list row) pairs is one tile: both operands go from HBM
 // straight into MFMA operand registers as BF16 (operand row = lane % 16,
-// columns 16 s + 4 (lane / 16) .. +3 in k-step s), v_mfma_f32_16x16x16_bf16
-// accumulates the dot products in FP32, and each lane finishes the four pairs
-// it holds, C[4 (lane / 16) + j][lane % 16], from the row norms:
+// columns 32 s + 8 (lane / 16) .. +7 in k-step s), gfx950's
+// v_mfma_f32_16x16x32_bf16 accumulates the dot products in FP32 (32 columns
+// per instruction), and each lane finishes the four pairs it holds,
+// C[4 (lane / 16) + j][lane % 16], from the row norms:
 //   L2SQR  d = |a|^2 + |o|^2 - 2 a.o,   EUCLIDEAN  d = n_a + n_o - a.o,
 //   ANGULAR  d = -a.o / fmaf(n_a, n_o, FLT_EPSILON).
 // EXACT (every row value an integer and 2 D max|x|^2 <= 2^24): BF16 holds each
@@ -206,7 +207,7 @@ __device__ __forceinline__ void put_cand(NndArgs const &a, uint32_t target, unsi
 // accumulation, the norms' rounding), so no pair that passes the exact test is
 // screened out and the re-scored distance alone decides.  Candidates are staged
 // in LDS and flushed grouped by target as in join_kernel.
-typedef short mj_bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 mj_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float mj_f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr unsigned MT = 512;  // list capacity (new + old)
@@ -223,16 +224,82 @@ struct MJoinLDS {
     uint16_t tgt[MB];
 };
 
+// KS k-steps of 32 columns: lane l takes columns 32 s + 8 (l / 16) .. +7 of
+// its row (zeros past Dp; Dp is a multiple of 4, so a step's last chunk may
+// hold 4 values).  Rows start at multiples of 8 bytes: 16-byte loads when Dp
+// is a multiple of 8, else two 8-byte loads.
 template <int KS>
-__device__ __forceinline__ void mj_load(NndArgs const &a, uint32_t id, bool ok, unsigned kq, mj_bf16x4 (&op)[KS]) {
+__device__ __forceinline__ void mj_load(NndArgs const &a, uint32_t id, bool ok, unsigned k8, mj_bf16x8 (&op)[KS]) {
     // operands straight from the BF16 copy of the rows (round to nearest even)
     uint16_t const *r = a.Xb + size_t(ok ? id : 0u) * a.Dp;
+    bool const wide = (a.Dp & 7u) == 0;
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
-        unsigned const k = 16 * s + kq;
-        uint2 const v = (ok && k < a.Dp) ? *reinterpret_cast<uint2 const *>(r + k) : make_uint2(0u, 0u);
-        op[s] = __builtin_bit_cast(mj_bf16x4, v);
+        unsigned const k = 32 * s + k8;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (ok && k + 8 <= a.Dp) {
+            if (wide) {
+                v = *reinterpret_cast<uint4 const *>(r + k);
+            } else {
+                uint2 const lo = *reinterpret_cast<uint2 const *>(r + k);
+                uint2 const hi = *reinterpret_cast<uint2 const *>(r + k + 4);
+                v = make_uint4(lo.x, lo.y, hi.x, hi.y);
+            }
+        } else if (ok && k + 4 <= a.Dp) {
+            uint2 const lo = *reinterpret_cast<uint2 const *>(r + k);
+            v = make_uint4(lo.x, lo.y, 0u, 0u);
+        }
+        op[s] = __builtin_bit_cast(mj_bf16x8, v);
     }
+}
+
+// FP32 operands for v_mfma_f32_16x16x4_f32 (the screen of the FP32-operand
+// form): lane l takes the float4 at columns 4 (4 q + l / 16) .. +3 of its row
+// for chunk q (zeros past Dp), and sub-step j of chunk q feeds element j of
+// every lane group to one MFMA, so each MFMA sums the columns
+// {16 q + 4 g + j : g = 0..3} -- the same permutation of the columns for both
+// operands, hence the full dot product once all chunks are summed.
+template <int KS>
+__device__ __forceinline__ void mf_load(NndArgs const &a, uint32_t id, bool ok, unsigned g, float4 (&op)[KS]) {
+    float const *r = a.X + size_t(ok ? id : 0u) * a.Dp;
+#pragma unroll
+    for (int q = 0; q < KS; ++q) {
+        unsigned const k = 16 * q + 4 * g;
+        op[q] = (ok && k < a.Dp) ? *reinterpret_cast<float4 const *>(r + k) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+// Operand tiles of either form: BF16 (16x16x32 steps) or FP32 (16x16x4).
+template <int KS, bool F32>
+struct MjOps;
+template <int KS>
+struct MjOps<KS, false> {
+    mj_bf16x8 v[KS];
+    __device__ void load(NndArgs const &a, uint32_t id, bool ok, unsigned lane) { mj_load<KS>(a, id, ok, (lane >> 4) * 8, v); }
+};
+template <int KS>
+struct MjOps<KS, true> {
+    float4 v[KS];
+    __device__ void load(NndArgs const &a, uint32_t id, bool ok, unsigned lane) { mf_load<KS>(a, id, ok, lane >> 4, v); }
+};
+template <int KS>
+__device__ __forceinline__ mj_f32x4 mj_mma(MjOps<KS, false> const &x, MjOps<KS, false> const &y) {
+    mj_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < KS; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x.v[s], y.v[s], acc, 0, 0, 0);
+    return acc;
+}
+template <int KS>
+__device__ __forceinline__ mj_f32x4 mj_mma(MjOps<KS, true> const &x, MjOps<KS, true> const &y) {
+    mj_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int q = 0; q < KS; ++q) {
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.v[q].x, y.v[q].x, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.v[q].y, y.v[q].y, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.v[q].z, y.v[q].z, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(x.v[q].w, y.v[q].w, acc, 0, 0, 0);
+    }
+    return acc;
 }
 
 // BF16 tile distance from the dot G and the norms the metric keeps (an, on:
@@ -341,7 +408,7 @@ __device__ void mj_rescore(NndArgs const &a, LDS &L, unsigned rqn, unsigned &fil
     wave_sync();
 }
 
-template <int METRIC, int KS, bool EXACT>
+template <int METRIC, int KS, bool EXACT, bool F32>
 __global__ void __launch_bounds__(256) join_mfma_kernel(NndArgs a) {
     __shared__ MJoinLDS<EXACT ? 1u : MQ> lds[4];
     unsigned const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -365,14 +432,17 @@ __global__ void __launch_bounds__(256) join_mfma_kernel(NndArgs a) {
         L.ids[i] = v;
     }
     wave_sync();
-    unsigned const r16 = lane & 15, kq = (lane >> 4) * 4, mrow = (lane >> 4) * 4;
+    unsigned const r16 = lane & 15, mrow = (lane >> 4) * 4;
     unsigned fill = 0, rqn = 0, pairs = 0, pushes = 0;
     bool found = false;
     for (unsigned ag = 0; ag < nA; ag += 16) {
-        mj_bf16x4 aop[KS];
+        // the A tile holds list rows ag .. ag + 15 (new rows, and old ones when
+        // the group runs past the new rows: their pairs are masked below), so
+        // it is also the B tile at tg == ag and those rows load once
+        MjOps<KS, F32> aop;
         {
             unsigned const ia = ag + r16;
-            mj_load<KS>(a, ia < nA ? L.ids[ia] : 0u, ia < nA, kq, aop);
+            aop.load(a, ia < nT ? L.ids[ia] : 0u, ia < nT, lane);
         }
         // the four new rows whose pairs this lane finishes
         uint32_t aid[4];
@@ -392,13 +462,14 @@ __global__ void __launch_bounds__(256) join_mfma_kernel(NndArgs a) {
             unsigned const io = tg + r16;
             bool const ook = io < nT;
             uint32_t const oid = ook ? L.ids[io] : 0u;
-            mj_bf16x4 bop[KS];
-            mj_load<KS>(a, oid, ook, kq, bop);
+            MjOps<KS, F32> bop;
+            if (tg == ag)
+                bop = aop;
+            else
+                bop.load(a, oid, ook, lane);
             float4 const om = a.meta[oid];
             float const orad = om.x, on = om.z;
-            mj_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int s = 0; s < KS; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aop[s], bop[s], acc, 0, 0, 0);
+            mj_f32x4 const acc = mj_mma(aop, bop);
             unsigned pm = 0;
             uint64_t key[8];
             unsigned tgt[8];
@@ -452,12 +523,16 @@ __global__ void __launch_bounds__(256) join_mfma_kernel(NndArgs a) {
 
 // The same tiles in exact FP32: each quad holds one list row of the tile in
 // registers and computes its 16 pairs with the tile's new rows in canonical
-// order (kg_device.hpp), the new rows streaming through L1 for all quads at
-// once; lane t of a quad keeps the pairs of new rows t, t + 4, t + 8, t + 12,
-// and staging / flushing is shared with the MFMA form.
+// order (kg_device.hpp); lane t of a quad keeps the pairs of new rows t, t + 4,
+// t + 8, t + 12, and staging / flushing is shared with the MFMA form.  With
+// CH > 0 the tile's 16 new rows are first copied into LDS (16 Dp floats per
+// wave, dynamic shared memory), where all 16 quads read the same chunk in the
+// same instruction (one broadcast read) instead of each fetching the rows
+// through the cache.
 template <int METRIC, int CH>
 __global__ void __launch_bounds__(256) join_tile_kernel(NndArgs a) {
     __shared__ MJoinLDS<1> lds[4];
+    extern __shared__ __align__(16) float4 tile_rows[];
     unsigned const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     unsigned const n = a.n0 + blockIdx.x * 4 + w;
     if (n >= a.n1) return;
@@ -481,9 +556,20 @@ __global__ void __launch_bounds__(256) join_tile_kernel(NndArgs a) {
     wave_sync();
     unsigned const quad = lane >> 2, t = lane & 3;
     unsigned const nch = (a.D & ~15u) >> 2;
+    unsigned const dp4 = a.Dp / 4;
+    float4 *const As = tile_rows + size_t(w) * 16 * dp4;  // CH > 0: the tile's new rows
     unsigned fill = 0, pairs = 0, pushes = 0;
     bool found = false;
     for (unsigned ag = 0; ag < nA; ag += 16) {
+        if (CH > 0) {
+            wave_sync();  // the previous tile's reads are done
+            for (unsigned e = lane; e < 16 * dp4; e += KG_WAVE) {
+                unsigned const r = e / dp4, c = e - r * dp4;
+                unsigned const ia = ag + r;
+                As[e] = ia < nA ? reinterpret_cast<float4 const *>(a.X + size_t(L.ids[ia]) * a.Dp)[c]
+                                : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
         uint32_t aid[4];
         float arad[4], akth[4];
 #pragma unroll
@@ -519,6 +605,7 @@ __global__ void __launch_bounds__(256) join_tile_kernel(NndArgs a) {
                 unsigned const ia = ag + r;
                 uint32_t const av = L.ids[ia < nA ? ia : 0u];
                 float const *ra = a.X + size_t(av) * a.Dp;
+                float4 const *ra4 = As + r * dp4;
                 float acc;
                 if (CH > 0) {
                     float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -526,7 +613,7 @@ __global__ void __launch_bounds__(256) join_tile_kernel(NndArgs a) {
                     for (int k = 0; k < CH; ++k) {
                         unsigned const kk = t + 4 * k;
                         if (kk < nch) {
-                            float4 const y = reinterpret_cast<float4 const *>(ra)[kk];
+                            float4 const y = ra4[kk];
                             float4 const x = reg[k];
                             if (METRIC == METRIC_L2SQR) {
                                 float const dx = y.x - x.x, dy = y.y - x.y, dz = y.z - x.z, dw = y.w - x.w;
@@ -543,12 +630,13 @@ __global__ void __launch_bounds__(256) join_tile_kernel(NndArgs a) {
                         }
                     }
                     acc = quad_reduce(p);
+                    float const *rl = reinterpret_cast<float const *>(ra4);
                     for (unsigned i = a.D & ~15u; i < a.D; ++i) {
                         if (METRIC == METRIC_L2SQR) {
-                            float const dd = ra[i] - orow[i];
+                            float const dd = rl[i] - orow[i];
                             acc = __builtin_fmaf(dd, dd, acc);
                         } else {
-                            acc = __builtin_fmaf(ra[i], orow[i], acc);
+                            acc = __builtin_fmaf(rl[i], orow[i], acc);
                         }
                     }
                 } else {
@@ -1052,16 +1140,16 @@ __global__ void __launch_bounds__(256) screen_pairs_kernel(NndArgs a, uint32_t c
     unsigned const lane = threadIdx.x & 63;
     unsigned const base = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 16;
     if (base >= n) return;
-    unsigned const r16 = lane & 15, kq = (lane >> 4) * 4;
+    unsigned const r16 = lane & 15, k8 = (lane >> 4) * 8;
     unsigned const p = base + r16;
     bool const ok = p < n;
     uint32_t const ia = ok ? pairs[2 * p] : 0u, ib = ok ? pairs[2 * p + 1] : 0u;
-    mj_bf16x4 aop[KS], bop[KS];
-    mj_load<KS>(a, ia, ok, kq, aop);
-    mj_load<KS>(a, ib, ok, kq, bop);
+    mj_bf16x8 aop[KS], bop[KS];
+    mj_load<KS>(a, ia, ok, k8, aop);
+    mj_load<KS>(a, ib, ok, k8, bop);
     mj_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int s = 0; s < KS; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aop[s], bop[s], acc, 0, 0, 0);
+    for (int s = 0; s < KS; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aop[s], bop[s], acc, 0, 0, 0);
     unsigned const j = r16 & 3;
     float const G = j == 0 ? acc[0] : j == 1 ? acc[1] : j == 2 ? acc[2] : acc[3];
     if (ok && (lane >> 4) == (r16 >> 2)) {
@@ -1102,7 +1190,8 @@ hipError_t nnd_launch_init(int metric, NndArgs const &a, hipStream_t s) {
 template <int METRIC>
 static void launch_join_m(NndArgs const &a, hipStream_t s) {
     dim3 g((a.n1 - a.n0 + 3) / 4), b(256);
-    size_t lds = 0;
+    // CH > 0: 16 staged rows of Dp floats per wave
+    size_t const lds = ch_for(a.D) ? size_t(4) * 16 * a.Dp * 4 : 0;
     switch (ch_for(a.D)) {
         case 1: hipLaunchKernelGGL((join_tile_kernel<METRIC, 1>), g, b, lds, s, a); break;
         case 2: hipLaunchKernelGGL((join_tile_kernel<METRIC, 2>), g, b, lds, s, a); break;
@@ -1116,20 +1205,39 @@ static void launch_join_m(NndArgs const &a, hipStream_t s) {
 template <int METRIC, bool EXACT>
 static void launch_join_mfma_e(NndArgs const &a, hipStream_t s) {
     dim3 g((a.n1 - a.n0 + 3) / 4), b(256);
-    unsigned const ks = (a.D + 15) / 16;
-    if (ks <= 2)
-        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 2, EXACT>), g, b, 0, s, a);
+    unsigned const ks = (a.D + 31) / 32;  // 32-column MFMA k-steps
+    if (ks <= 1)
+        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 1, EXACT, false>), g, b, 0, s, a);
+    else if (ks <= 2)
+        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 2, EXACT, false>), g, b, 0, s, a);
     else if (ks <= 4)
-        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 4, EXACT>), g, b, 0, s, a);
-    else if (ks <= 8)
-        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 8, EXACT>), g, b, 0, s, a);
+        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 4, EXACT, false>), g, b, 0, s, a);
     else
-        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 16, EXACT>), g, b, 0, s, a);
+        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 8, EXACT, false>), g, b, 0, s, a);
+}
+
+// FP32-operand screen: KS float4 chunks of 16 columns per lane
+template <int METRIC>
+static void launch_join_mfma_f32(NndArgs const &a, hipStream_t s) {
+    dim3 g((a.n1 - a.n0 + 3) / 4), b(256);
+    unsigned const kq = (a.D + 15) / 16;
+    if (kq <= 2)
+        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 2, false, true>), g, b, 0, s, a);
+    else if (kq <= 4)
+        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 4, false, true>), g, b, 0, s, a);
+    else if (kq <= 6)
+        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 6, false, true>), g, b, 0, s, a);
+    else if (kq <= 8)
+        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 8, false, true>), g, b, 0, s, a);
+    else
+        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 16, false, true>), g, b, 0, s, a);
 }
 
 template <int METRIC>
 static void launch_join_mfma(NndArgs const &a, hipStream_t s) {
-    if (a.exact)
+    if (a.f32)
+        launch_join_mfma_f32<METRIC>(a, s);
+    else if (a.exact)
         launch_join_mfma_e<METRIC, true>(a, s);
     else
         launch_join_mfma_e<METRIC, false>(a, s);
@@ -1200,18 +1308,18 @@ hipError_t nnd_launch_screen_pairs(int metric, NndArgs const &a, uint32_t const 
                                    float *out_e, hipStream_t s) {
     if (n == 0) return hipSuccess;
     if (a.D > 256) return hipErrorInvalidValue;
-    unsigned const waves = (n + 15) / 16, ks = (a.D + 15) / 16;
+    unsigned const waves = (n + 15) / 16, ks = (a.D + 31) / 32;
     dim3 g((waves + 3) / 4), b(256);
 #define KG_SCREEN(M)                                                                                       \
     do {                                                                                                   \
-        if (ks <= 2)                                                                                       \
+        if (ks <= 1)                                                                                       \
+            hipLaunchKernelGGL((screen_pairs_kernel<M, 1>), g, b, 0, s, a, pairs, n, out_d, out_e);       \
+        else if (ks <= 2)                                                                                  \
             hipLaunchKernelGGL((screen_pairs_kernel<M, 2>), g, b, 0, s, a, pairs, n, out_d, out_e);       \
         else if (ks <= 4)                                                                                  \
             hipLaunchKernelGGL((screen_pairs_kernel<M, 4>), g, b, 0, s, a, pairs, n, out_d, out_e);       \
-        else if (ks <= 8)                                                                                  \
-            hipLaunchKernelGGL((screen_pairs_kernel<M, 8>), g, b, 0, s, a, pairs, n, out_d, out_e);       \
         else                                                                                               \
-            hipLaunchKernelGGL((screen_pairs_kernel<M, 16>), g, b, 0, s, a, pairs, n, out_d, out_e);      \
+            hipLaunchKernelGGL((screen_pairs_kernel<M, 8>), g, b, 0, s, a, pairs, n, out_d, out_e);       \
     } while (0)
     if (metric == METRIC_L2SQR)
         KG_SCREEN(METRIC_L2SQR);

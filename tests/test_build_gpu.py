@@ -72,9 +72,12 @@ def test_build_recall_matches_oracle(kind, metric, d, params):
     ("intmix", EUC, 64, "bf16-exact"),
     ("uniform", EUC, 16, "bf16-screen"),    # FP32 data: screen + exact re-score
     ("angular", ANG, 96, "bf16-screen"),
+    ("uniform", L2, 37, "fp32-mfma-screen"),  # FP32 operands on MFMA: screen + exact re-score
+    ("angular", ANG, 96, "fp32-mfma-screen"),
+    ("intmix", EUC, 128, "fp32-mfma-screen"),
 ])
 def test_mfma_join_recall_and_canonical_distances(kind, metric, d, form):
-    from kgraph_amd import PRECISION_BF16
+    from kgraph_amd import PRECISION_BF16, PRECISION_FP32_MFMA
     X = _data(kind, 20000, d)
     N, K = len(X), 16
     params = dict(K=K, L=32, S=10, R=100, iterations=12, controls=100, reverse=0)
@@ -85,7 +88,7 @@ def test_mfma_join_recall_and_canonical_distances(kind, metric, d, form):
     off, ids, dd, M = g32.graph()
     r32 = _knn_recall(off[rows], dd, gt_d, K)
     g = _gpu(X, metric)
-    info = g.build(precision=PRECISION_BF16, **params)
+    info = g.build(precision=PRECISION_FP32_MFMA if form.startswith("fp32") else PRECISION_BF16, **params)
     c = g.build_counters()
     assert c["join_form"] == form
     off, ids, dd, M = g.graph()

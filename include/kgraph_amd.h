@@ -302,8 +302,8 @@ int kg_search_latencies(kg_handle *h, float *out_us, unsigned n);
  * iterations, out[4] local-join form (0 FP32 quads, 1 BF16 MFMA screen with
  * exact FP32 re-score, 2 BF16 MFMA exact on these rows, 3 FP32-operand MFMA
  * screen with exact FP32 re-score), out[5] iterations
- * run as replays of the captured iteration graph.  n = number of entries
- * wanted (<= 6). */
+ * run as replays of the captured iteration graph, out[6] pairs the screened
+ * forms re-scored in exact FP32.  n = number of entries wanted (<= 7). */
 int kg_build_counters(kg_handle *h, uint64_t *out, unsigned n);
 
 /* Local-join form when the BF16 join is a screen (KG_PRECISION_BF16 on rows

@@ -259,15 +259,16 @@ class KGraph:
 
     def build_counters(self):
         """Last build: {'pairs', 'list_entries', 'pushes', 'iterations',
-        'join_form', 'graph_replays'} with join_form 'fp32', 'bf16-screen' or
-        'bf16-exact' and graph_replays the iterations run as replays of the
-        captured iteration graph."""
-        out = np.zeros(6, dtype=np.uint64)
-        self._h.call("kg_build_counters", ptr(out), 6)
+        'join_form', 'graph_replays', 'rescored'} with join_form 'fp32',
+        'bf16-screen', 'bf16-exact' or 'fp32-mfma-screen', graph_replays the
+        iterations run as replays of the captured iteration graph and rescored
+        the pairs the screen sent to the exact FP32 re-score."""
+        out = np.zeros(7, dtype=np.uint64)
+        self._h.call("kg_build_counters", ptr(out), 7)
         forms = {0: "fp32", 1: "bf16-screen", 2: "bf16-exact", 3: "fp32-mfma-screen"}
         return {"pairs": int(out[0]), "list_entries": int(out[1]), "pushes": int(out[2]),
                 "iterations": int(out[3]), "join_form": forms.get(int(out[4]), "fp32"),
-                "graph_replays": int(out[5])}
+                "graph_replays": int(out[5]), "rescored": int(out[6])}
 
     def build_iterations(self):
         """Last build, one row per iteration: pairs, pushes, list entries,

@@ -318,7 +318,7 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     info.iterations = 0;
     info.delta = 1.0f;
     float const total = N * float(N - 1) / 2;
-    uint64_t n_comps = 0, n_lists = 0, n_push = 0;
+    uint64_t n_comps = 0, n_lists = 0, n_push = 0, n_rescore = 0;
     std::vector<float> cpool(size_t(nc) * p.L);
     std::vector<uint32_t> ccnt(nc);
     h.trace.clear();
@@ -330,8 +330,10 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     // the data (how many pairs land within the screen's error bound of a
     // radius) and drifts as the lists converge.  Both forms admit exactly the
     // same candidates, so in auto mode a probe iteration (every third, from the
-    // first) splits the nodes between the two forms, times both parts, and the
-    // next iterations use the screen when it is at least 5 % faster per node
+    // first; the gap doubles, up to 12, after a probe the winner takes by more
+    // than 1.5x, and returns to 3 after a close one) splits the nodes between
+    // the two forms, times both parts, and the next iterations use the screen
+    // when it is at least 5 % faster per node
     // (its cost per node grows as the lists converge: on the 1.2M x 96 angular
     // set it beats the tiles by a third in the first iterations, ties at the
     // fourth and is 15-35 % slower after, so a tie goes to the tiles).  Every
@@ -345,10 +347,11 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     bool use_mfma = a.mfma && form != KG_JOIN_TILES;
     unsigned const half = N / 2, eighth = N / 8;
     IterGraph graph[2];  // [0] FP32 tiles, [1] MFMA form
+    unsigned next_probe = 0, probe_gap = 3;
     for (unsigned it = 0; (p.iterations == 0) || (it < p.iterations); ++it) {
         ++info.iterations;
         float ms;
-        bool const probe = screen && form == KG_JOIN_AUTO && it % 3 == 0 && eighth > 0;
+        bool const probe = screen && form == KG_JOIN_AUTO && it == next_probe && eighth > 0;
         // probe split: [0, cut) runs form `lead`, [cut, N) the other form
         bool const lead = it == 0 ? true : use_mfma;
         unsigned const cut = half;
@@ -416,11 +419,15 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
             double const c_first = double(t_first) / cut, c_second = double(t_second) / (N - cut);  // per node
             double const c_screen = lead ? c_first : c_second, c_tile = lead ? c_second : c_first;
             use_mfma = c_screen < 0.95 * c_tile;
+            double const margin = std::max(c_screen, c_tile) / std::max(std::min(c_screen, c_tile), 1e-30);
+            probe_gap = margin > 1.5 ? std::min(probe_gap * 2, 12u) : 3u;
+            next_probe = it + probe_gap;
         }
         // statistics and stop rule (kgraph.cpp:927-972)
         n_comps += stats[NND_STAT_PAIRS];
         n_lists += stats[NND_STAT_LISTS];
         n_push += stats[NND_STAT_PUSHES];
+        n_rescore += stats[NND_STAT_RESCORE];
         info.cost = float(n_comps) / total;
         info.delta = float(double(stats[NND_STAT_DELTA]) / (double(N) * p.K));
         info.M = float(double(stats[NND_STAT_M]) / N);
@@ -484,6 +491,7 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     h.build_counters[2] = n_push;
     h.build_counters[3] = info.iterations;
     h.build_counters[4] = replays;
+    h.build_counters[5] = n_rescore;
     h.last_ms[3] = join_ms;
     h.last_ms[4] = merge_ms;
     h.last_ms[5] = upd_ms;

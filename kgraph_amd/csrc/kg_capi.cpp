@@ -1199,6 +1199,7 @@ int kg_build_counters(kg_handle *k, uint64_t *out, unsigned n) {
         for (unsigned i = 0; i < n && i < 4; ++i) out[i] = h.build_counters[i];
         if (n > 4) out[4] = uint64_t(h.join_exact + 1);
         if (n > 5) out[5] = h.build_counters[4];
+        if (n > 6) out[6] = h.build_counters[5];
     });
 }
 

@@ -148,7 +148,7 @@ struct Handle {
     float last_ms[9] = {};
     bool entry_ms_pending = false;  // last_ms[7] waits on ev[14..15] of a selection still in flight
     std::vector<float> trace;
-    uint64_t build_counters[5] = {};  // pairs, list entries, candidate pushes, iterations, graph replays
+    uint64_t build_counters[6] = {};  // pairs, list entries, candidate pushes, iterations, graph replays, re-scored pairs
     std::vector<uint64_t> iter_stats;  // per iteration: pairs, pushes, list entries, join form, join us, iteration us
     bool build_graphs = true;         // NN-descent iterations run as a captured HIP graph
     int join_form = KG_JOIN_AUTO;     // BF16 screen vs exact FP32 tiles on float rows (KG_JOIN_*)

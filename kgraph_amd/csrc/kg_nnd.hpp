@@ -10,8 +10,8 @@
 
 // Iteration statistics: per node in `part` (column k at part[k * N + n]),
 // summed into `stats` after the merge.
-enum { NND_STAT_PAIRS = 0, NND_STAT_DELTA = 1, NND_STAT_M = 2, NND_STAT_LISTS = 3, NND_STAT_PUSHES = 4, NND_NPART = 5,
-       NND_NSTATS = 8 };
+enum { NND_STAT_PAIRS = 0, NND_STAT_DELTA = 1, NND_STAT_M = 2, NND_STAT_LISTS = 3, NND_STAT_PUSHES = 4,
+       NND_STAT_RESCORE = 5, NND_NPART = 6, NND_NSTATS = 8 };
 
 struct NndArgs {
     float const *X;      // rows, stride Dp floats

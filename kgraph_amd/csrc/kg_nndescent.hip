@@ -433,7 +433,7 @@ __global__ void __launch_bounds__(256) join_mfma_kernel(NndArgs a) {
     }
     wave_sync();
     unsigned const r16 = lane & 15, mrow = (lane >> 4) * 4;
-    unsigned fill = 0, rqn = 0, pairs = 0, pushes = 0;
+    unsigned fill = 0, rqn = 0, pairs = 0, pushes = 0, rescored = 0;
     bool found = false;
     for (unsigned ag = 0; ag < nA; ag += 16) {
         // the A tile holds list rows ag .. ag + 15 (new rows, and old ones when
@@ -501,11 +501,15 @@ __global__ void __launch_bounds__(256) join_mfma_kernel(NndArgs a) {
             if (EXACT) mj_stage<8>(a, L, fill, nT, pm, key, tgt);
             if (!EXACT && rqn > MQ - 4 * KG_WAVE) {
                 mj_rescore<METRIC>(a, L, rqn, fill, nT, found, pushes);
+                rescored += rqn;
                 rqn = 0;
             }
         }
     }
-    if (!EXACT && rqn) mj_rescore<METRIC>(a, L, rqn, fill, nT, found, pushes);
+    if (!EXACT && rqn) {
+        mj_rescore<METRIC>(a, L, rqn, fill, nT, found, pushes);
+        rescored += rqn;
+    }
     wave_sync();
     if (fill) mj_flush(a, L, fill, nT);
     uint64_t const fb = ballot(found);
@@ -518,6 +522,7 @@ __global__ void __launch_bounds__(256) join_mfma_kernel(NndArgs a) {
         a.part[NND_STAT_PAIRS * a.N + n] = pairs;
         a.part[NND_STAT_PUSHES * a.N + n] = pushes;
         a.part[NND_STAT_LISTS * a.N + n] = nA0 + nB;
+        a.part[NND_STAT_RESCORE * a.N + n] = rescored;
     }
 }
 
@@ -681,6 +686,7 @@ __global__ void __launch_bounds__(256) join_tile_kernel(NndArgs a) {
         a.part[NND_STAT_PAIRS * a.N + n] = pairs;
         a.part[NND_STAT_PUSHES * a.N + n] = pushes;
         a.part[NND_STAT_LISTS * a.N + n] = nA0 + nB;
+        a.part[NND_STAT_RESCORE * a.N + n] = 0;
     }
 }
 

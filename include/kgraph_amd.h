@@ -334,6 +334,25 @@ int kg_build_iterations(kg_handle *h, uint64_t *out, unsigned cap, unsigned *n);
  * screen against an FP64 ground truth; D <= 256. */
 int kg_screen_pairs(kg_handle *h, const uint32_t *pairs, unsigned n, float *screened, float *bound);
 
+/* Learned pre-rank of the local join (no counterpart in the reference, whose
+ * join scores every new x new and new x old pair of a node's lists,
+ * kgraph.cpp:797-815): with 0 < keep < 1 and a projector (kg_mlp_init /
+ * kg_mlp_train), every node's join lists are ranked by projected distance to
+ * the node and only the nearest fraction `keep` of them takes part in the
+ * join; full-dimension FP32 distances still judge every admission.  Guard:
+ * before the first iteration a held-out probe (each control node's exact
+ * nearest neighbours against random nodes) must be ordered correctly by the
+ * projection in at least 90 % of the cases, and the pre-rank stays on only
+ * while every iteration closes at least a quarter of the control recall's
+ * remaining gap; otherwise the build falls back to plain NN-descent.  keep = 0
+ * or 1: off (default).
+ * kg_build_prerank reports the last build: out[0] status (0 off, 1 on for the
+ * whole build, 2 turned down by the probe, 3 turned off after the control
+ * recall slowed down), out[1] the probe's accuracy, out[2] the iteration of
+ * the fallback (-1: none). */
+int kg_set_join_prerank(kg_handle *h, float keep);
+int kg_build_prerank(kg_handle *h, float *out);
+
 /* NN-descent iterations as a HIP graph (no counterpart in the reference; the
  * loop is kgraph.cpp:927-972): with on = 1 (default) the device work of an
  * iteration after the first (update, join, merge, statistics, control pools)

@@ -80,11 +80,14 @@ class KGraph:
     def build(self, iterations=default_iterations, L=default_L, K=default_K, S=default_S,
               controls=default_controls, delta=default_delta, recall=default_recall,
               prune=default_prune, reverse=default_reverse, *, R=default_R, seed=default_seed,
-              precision=None, graphs=None, join=None):
+              precision=None, graphs=None, join=None, prerank=None):
         """graphs: True (default) runs the iterations as replays of a captured
         HIP graph, False launches their kernels one by one.  join: 'auto'
         (default), 'screen' or 'tiles' -- the local-join form when the BF16
-        join is a screen (precision BF16 on rows that are not BF16-exact)."""
+        join is a screen (precision BF16 on rows that are not BF16-exact).
+        prerank: fraction of each node's join lists kept by the learned
+        pre-rank (needs a projector, train_projector; 0 or 1 = off); see
+        prerank_report()."""
         p = IndexParams(iterations, L, K, S, R, controls, seed, delta, recall, prune, reverse)
         if precision is not None:
             if isinstance(precision, str):
@@ -96,6 +99,8 @@ class KGraph:
         if join is not None:
             self._h.call("kg_set_join_form", {"auto": _lib.JOIN_AUTO, "screen": _lib.JOIN_SCREEN,
                                               "tiles": _lib.JOIN_TILES}[join])
+        if prerank is not None:
+            self._h.call("kg_set_join_prerank", float(prerank))
         info = IndexInfo()
         self._h.call("kg_build", p, info)
         self._hasIndex = True
@@ -269,6 +274,14 @@ class KGraph:
         return {"pairs": int(out[0]), "list_entries": int(out[1]), "pushes": int(out[2]),
                 "iterations": int(out[3]), "join_form": forms.get(int(out[4]), "fp32"),
                 "graph_replays": int(out[5]), "rescored": int(out[6])}
+
+    def prerank_report(self):
+        """The last build's learned pre-rank: {'status': 'off' | 'on' |
+        'probe-fallback' | 'recall-fallback', 'probe_accuracy', 'fallback_iteration'}."""
+        out = np.zeros(3, dtype=np.float32)
+        self._h.call("kg_build_prerank", ptr(out))
+        return {"status": ["off", "on", "probe-fallback", "recall-fallback"][int(out[0])],
+                "probe_accuracy": float(out[1]), "fallback_iteration": int(out[2])}
 
     def build_iterations(self):
         """Last build, one row per iteration: pairs, pushes, list entries,

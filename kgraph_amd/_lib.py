@@ -91,6 +91,8 @@ SIGNATURES = {
     "kg_search_device": (_I, [_P, _P, _U, _SZ, ctypes.POINTER(SearchParams), _P, _P, _P, _P, _SZ, _P, _P, _P]),
     "kg_bruteforce": (_I, [_P, _P, _U, _SZ, _U, ctypes.c_float, _P, _P, _SZ, _P]),
     "kg_set_search_launch": (_I, [_P, _U, _U]),
+    "kg_set_join_prerank": (_I, [_P, ctypes.c_float]),
+    "kg_build_prerank": (_I, [_P, _P]),
     "kg_bruteforce_device_noalloc": (_I, [_P, _P, _U, _SZ, _U, ctypes.c_float, _P, _P, _SZ, _P]),
     "kg_save": (_I, [_P, ctypes.c_char_p, _I]),
     "kg_load": (_I, [_P, ctypes.c_char_p]),

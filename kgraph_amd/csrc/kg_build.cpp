@@ -122,9 +122,22 @@ void iteration_body(int metric, NndArgs const &a, bool update, uint32_t const *c
                     uint32_t *ccount, hipStream_t s) {
     if (update) KG_HIP(nnd_launch_update(a, s));
     KG_HIP(nnd_launch_iter_begin(a, update ? 1 : 0, s));
+    if (a.Z) KG_HIP(nnd_launch_prerank(a, s));
     KG_HIP(nnd_launch_join(metric, a, s));
     KG_HIP(nnd_launch_merge(a, s));
     if (nc) KG_HIP(nnd_launch_gather_controls(a, ctl, nc, cdist, ccount, s));
+}
+
+// The projector's rows for every dataset row (N x mlpP), in chunks.
+void project_rows(Handle &h, float *Z) {
+    unsigned const chunk = 1u << 16;
+    DevBuf<float> hid;
+    hid.reserve(size_t(chunk) * h.mlpH);
+    for (unsigned r0 = 0; r0 < h.N; r0 += chunk) {
+        unsigned const n = std::min(chunk, h.N - r0);
+        mlp_forward_rows(h, h.X.p + size_t(r0) * h.Dp, n, h.Dp, hid.p, Z + size_t(r0) * h.mlpP);
+    }
+    h.sync();
 }
 
 // Captured iteration graph, destroyed with the build.
@@ -267,6 +280,20 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
         screen_constants(h.D, a, true);
     }
     h.join_exact = a.mfma ? (a.f32 ? 2 : int(a.exact)) : -1;
+    // learned pre-rank of the join lists: the projector's rows, guarded by a
+    // held-out probe before the first iteration and by the control recall
+    // after every iteration (plain NN-descent from the first drop on)
+    DevBuf<float> Z;
+    bool prerank = false;
+    h.prerank_status = 0;
+    h.prerank_acc = 0.f;
+    h.prerank_fallback = -1;
+    if (h.prerank_keep > 0.f && h.prerank_keep < 1.f) {
+        if (!h.mlpH) throw invalid_argument("join pre-rank needs a projector (kg_mlp_init)");
+        Z.reserve(size_t(N) * h.mlpP);
+        project_rows(h, Z.p);
+        prerank = true;
+    }
     KG_HIP(hipMemsetAsync(st.ovf.p, 0xff, size_t(N) * a.O * 8, s));
     KG_HIP(hipMemsetAsync(st.found.p, 0, size_t(N) * 4, s));
     KG_HIP(hipMemsetAsync(st.iter.p, 0, 4, s));
@@ -306,6 +333,48 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
         KG_HIP(kg_launch_brute(h.metric, b, s));
         KG_HIP(hipMemcpyAsync(ctl_knn.data(), st.cdist.p, 4 * ctl_knn.size(), hipMemcpyDeviceToHost, s));
         h.sync();
+        if (prerank) {
+            // held-out probe of the projector: for each control node, its
+            // exact nearest neighbours against random nodes -- the share the
+            // projection orders correctly
+            std::vector<uint32_t> knn_ids(size_t(nc) * p.K), pairs;
+            KG_HIP(hipMemcpy(knn_ids.data(), kid.p, 4 * knn_ids.size(), hipMemcpyDeviceToHost));
+            std::mt19937 prng(p.seed + 7);
+            unsigned const per = std::min(p.K, 10u);
+            for (unsigned c = 0; c < nc; ++c)
+                for (unsigned t = 0; t < per; ++t) {
+                    uint32_t r = prng() % N;
+                    if (r == index[c]) r = (r + 1) % N;
+                    pairs.insert(pairs.end(), {index[c], knn_ids[size_t(c) * p.K + t], index[c], r});
+                }
+            unsigned const np = unsigned(pairs.size() / 2);
+            DevBuf<uint32_t> dpairs;
+            DevBuf<float> dz;
+            dpairs.reserve(pairs.size());
+            dz.reserve(np);
+            KG_HIP(hipMemcpyAsync(dpairs.p, pairs.data(), 4 * pairs.size(), hipMemcpyHostToDevice, s));
+            KG_HIP(nnd_launch_zpair(Z.p, h.mlpP, dpairs.p, np, dz.p, s));
+            std::vector<float> hz(np);
+            KG_HIP(hipMemcpyAsync(hz.data(), dz.p, 4ull * np, hipMemcpyDeviceToHost, s));
+            h.sync();
+            unsigned good = 0;
+            for (unsigned i = 0; i + 1 < np; i += 2) good += hz[i] < hz[i + 1];
+            h.prerank_acc = np ? float(2.0 * good / np) : 0.f;
+            if (h.prerank_acc < 0.9f) {
+                prerank = false;  // the projection does not rank near neighbours first: plain NN-descent
+                h.prerank_status = 2;
+            }
+        }
+    }
+    if (prerank && nc == 0) {  // no held-out probe set to guard the pre-rank with
+        prerank = false;
+        h.prerank_status = 2;
+    }
+    if (prerank) {
+        a.Z = Z.p;
+        a.zdim = h.mlpP;
+        a.keep = h.prerank_keep;
+        h.prerank_status = 1;
     }
     KG_HIP(nnd_launch_meta_init(h.metric, a, s));
     KG_HIP(nnd_launch_init(h.metric, a, s));
@@ -346,7 +415,8 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     int const form = screen ? h.join_form : KG_JOIN_AUTO;
     bool use_mfma = a.mfma && form != KG_JOIN_TILES;
     unsigned const half = N / 2, eighth = N / 8;
-    IterGraph graph[2];  // [0] FP32 tiles, [1] MFMA form
+    IterGraph graph[2][2];  // [pre-rank][0 FP32 tiles, 1 MFMA form]
+    float prev_recall = -1.f;
     unsigned next_probe = 0, probe_gap = 3;
     for (unsigned it = 0; (p.iterations == 0) || (it < p.iterations); ++it) {
         ++info.iterations;
@@ -355,11 +425,12 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
         // probe split: [0, cut) runs form `lead`, [cut, N) the other form
         bool const lead = it == 0 ? true : use_mfma;
         unsigned const cut = half;
+        if (!a.Z) a_tile.Z = nullptr;  // after a pre-rank fallback
         NndArgs const &ar = use_mfma ? a : a_tile;
         int rec_form = probe ? 3 : (ar.mfma ? (ar.f32 ? 4 : (ar.exact ? 2 : 1)) : 0);
         float jms = 0.f;
         if (h.build_graphs && !probe && it > 0) {
-            IterGraph &gr = graph[use_mfma ? 1 : 0];
+            IterGraph &gr = graph[a.Z ? 1 : 0][use_mfma ? 1 : 0];
             if (!gr.e) gr.capture(h.metric, ar, st.ctl.p, nc, st.cdist.p, st.ccount.p, s);
             KG_HIP(hipEventRecord(h.ev[16], s));
             KG_HIP(hipGraphLaunch(gr.e, s));
@@ -374,6 +445,7 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
                 KG_HIP(hipEventRecord(h.ev[10], s));
             }
             KG_HIP(nnd_launch_iter_begin(a, it > 0 ? 1 : 0, s));
+            if (a.Z) KG_HIP(nnd_launch_prerank(a, s));
             KG_HIP(hipEventRecord(h.ev[6], s));
             if (probe) {
                 NndArgs first = lead ? a : a_tile, second = lead ? a_tile : a;
@@ -438,6 +510,16 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
         }
         info.recall = nc ? float(sr / nc) : 0.f;
         info.accuracy = nc ? float(sa / nc) : 0.f;
+        if (a.Z && it > 0 && info.recall < p.recall &&
+            info.recall - prev_recall < 0.25f * (1.f - std::max(prev_recall, 0.f))) {
+            // under the pre-rank the probe set's recall closed less than a
+            // quarter of its remaining gap in this iteration: plain
+            // NN-descent from here
+            a.Z = nullptr;
+            h.prerank_status = 3;
+            h.prerank_fallback = int(it);
+        }
+        prev_recall = info.recall;
         h.trace.insert(h.trace.end(), {info.recall, info.delta, info.M, info.cost});
         h.iter_stats.insert(h.iter_stats.end(), {stats[NND_STAT_PAIRS], stats[NND_STAT_PUSHES], stats[NND_STAT_LISTS],
                                                  uint64_t(rec_form), uint64_t(jms * 1000.f), uint64_t(it_ms * 1000.f)});

@@ -1215,6 +1215,22 @@ int kg_set_join_form(kg_handle *k, int form) {
     });
 }
 
+int kg_set_join_prerank(kg_handle *k, float keep) {
+    return guard(k, [&](Handle &h) {
+        if (!(keep >= 0.f && keep <= 1.f)) throw kga::invalid_argument("keep must be in [0, 1]");
+        h.prerank_keep = keep;
+    });
+}
+
+int kg_build_prerank(kg_handle *k, float *out) {
+    return guard(k, [&](Handle &h) {
+        if (!out) throw kga::invalid_argument("null argument");
+        out[0] = float(h.prerank_status);
+        out[1] = h.prerank_acc;
+        out[2] = float(h.prerank_fallback);
+    });
+}
+
 int kg_set_build_graphs(kg_handle *k, int on) {
     return guard(k, [&](Handle &h) { h.build_graphs = on != 0; });
 }

@@ -152,6 +152,13 @@ struct Handle {
     std::vector<uint64_t> iter_stats;  // per iteration: pairs, pushes, list entries, join form, join us, iteration us
     bool build_graphs = true;         // NN-descent iterations run as a captured HIP graph
     int join_form = KG_JOIN_AUTO;     // BF16 screen vs exact FP32 tiles on float rows (KG_JOIN_*)
+    // learned pre-rank of the join lists: fraction kept (0 or >= 1: off), and
+    // what the last build did: status 0 off, 1 on throughout, 2 turned down by
+    // the held-out probe before the first iteration, 3 turned off when the
+    // control recall slowed down (at iteration prerank_fallback); probe accuracy
+    float prerank_keep = 0.f;
+    int prerank_status = 0, prerank_fallback = -1;
+    float prerank_acc = 0.f;
 
     Handle();
     ~Handle();

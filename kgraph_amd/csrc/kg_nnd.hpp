@@ -50,6 +50,10 @@ struct NndArgs {
     unsigned f32;       // with mfma: FP32 operands (v_mfma_f32_16x16x4_f32) instead of BF16
     unsigned exact;     // 1: BF16 tiles are exact for these rows (no re-score)
     float scr_c, scr_g; // screen bound: relative dot error, relative norm error
+    // learned pre-rank of the join lists (prerank_kernel): projected rows
+    float const *Z;     // N x zdim projector outputs (NULL: no pre-rank)
+    unsigned zdim;
+    float keep;         // fraction of each node's list kept
 };
 
 hipError_t nnd_launch_init(int metric, NndArgs const &a, hipStream_t s);
@@ -59,6 +63,11 @@ hipError_t nnd_launch_join(int metric, NndArgs const &a, hipStream_t s);
 // smallest / largest value (the caller sets flags[2] to all ones, the rest to 0).
 hipError_t nnd_launch_sqn(NndArgs const &a, float2 *out, uint32_t *flags, hipStream_t s);
 hipError_t nnd_launch_merge(NndArgs const &a, hipStream_t s);
+// Prunes every node's join lists to the fraction a.keep nearest in the
+// learned projection a.Z (ties by list position), in place.
+hipError_t nnd_launch_prerank(NndArgs const &a, hipStream_t s);
+// out[i] = |Z[pairs[2i]] - Z[pairs[2i+1]]|^2 (P floats per row of Z).
+hipError_t nnd_launch_zpair(float const *Z, unsigned P, uint32_t const *pairs, unsigned n, float *out, hipStream_t s);
 hipError_t nnd_launch_update(NndArgs const &a, hipStream_t s);
 // Fills the join norms of a.meta (after the MFMA form's sqn, when used).
 hipError_t nnd_launch_meta_init(int metric, NndArgs const &a, hipStream_t s);

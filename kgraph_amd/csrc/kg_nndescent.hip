@@ -1109,6 +1109,92 @@ __global__ void __launch_bounds__(256) sample_kernel(NndArgs a) {
     }
 }
 
+// ------------------------------------------------------------------ learned pre-rank
+// The join's lists of node n pruned by a learned projection (the projector's
+// rows Z, zdim floats each): every list entry (new sample, reverse new, old
+// sample, reverse old) is ranked by its projected squared distance to n, ties
+// by list position, and the first ceil(keep * total) are kept, each sub-list
+// compacted in order in place; the join then scores only pairs of kept
+// entries.  Full-dimension FP32 distances still judge every admission.
+__global__ void __launch_bounds__(256) prerank_kernel(NndArgs a) {
+    __shared__ uint64_t key[4][MT];
+    __shared__ float zn[4][64];
+    unsigned const w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    unsigned const n = blockIdx.x * 4 + w;
+    if (n >= a.N) return;
+    unsigned const nn = a.nnew[n], no = a.nold[n];
+    unsigned const rn = min(a.crnew[n], a.R), ro = min(a.crold[n], a.R);
+    unsigned const total = min(nn + rn + no + ro, MT);
+    uint32_t *const samp = a.samp + size_t(n) * a.Lc;
+    uint32_t *const rnew = a.rnew + size_t(n) * a.R;
+    uint32_t *const rold = a.rold + size_t(n) * a.R;
+    // entry i of the list (the join's order: new sample, reverse new, old sample, reverse old)
+    auto entry = [&](unsigned i) -> uint32_t {
+        if (i < nn) return samp[i];
+        if (i < nn + rn) return rnew[i - nn];
+        if (i < nn + rn + no) return samp[a.Lc - 1 - (i - nn - rn)];
+        return rold[i - nn - rn - no];
+    };
+    unsigned const P = a.zdim;
+    for (unsigned k = lane; k < P; k += KG_WAVE) zn[w][k] = a.Z[size_t(n) * P + k];
+    wave_sync();
+    for (unsigned i = lane; i < total; i += KG_WAVE) {
+        float const *zi = a.Z + size_t(entry(i)) * P;
+        float d = 0.f;
+        for (unsigned k = 0; k < P; ++k) {
+            float const t = zi[k] - zn[w][k];
+            d = __builtin_fmaf(t, t, d);
+        }
+        key[w][i] = (uint64_t(ord_f32(d)) << 32) | i;
+    }
+    wave_sync();
+    unsigned const kc = min(total, unsigned(ceilf(a.keep * float(total))));
+    // keep flags by rank, then in-order compaction of each sub-list
+    uint32_t ids[MT / KG_WAVE];
+    bool kept[MT / KG_WAVE];
+#pragma unroll
+    for (unsigned c = 0; c < MT / KG_WAVE; ++c) {
+        unsigned const i = c * KG_WAVE + lane;
+        kept[c] = false;
+        ids[c] = 0;
+        if (i < total) {
+            uint64_t const v = key[w][i];
+            unsigned r = 0;
+            for (unsigned j = 0; j < total; ++j) r += key[w][j] < v;
+            kept[c] = r < kc;
+            ids[c] = entry(i);
+        }
+    }
+    wave_sync();  // every entry is read before the lists are rewritten
+    unsigned const bound[4] = {nn, nn + rn, nn + rn + no, total};
+    unsigned cnt[4] = {0, 0, 0, 0};
+    uint64_t const lt = (1ull << lane) - 1;
+#pragma unroll
+    for (unsigned c = 0; c < MT / KG_WAVE; ++c) {
+        unsigned const i = c * KG_WAVE + lane;
+#pragma unroll
+        for (unsigned part = 0; part < 4; ++part) {
+            bool const in = i < total && i >= (part ? bound[part - 1] : 0u) && i < bound[part];
+            bool const k = in && kept[c];
+            uint64_t const b = ballot(k);
+            if (k) {
+                unsigned const pos = cnt[part] + popc64(b & lt);
+                if (part == 0) samp[pos] = ids[c];
+                else if (part == 1) rnew[pos] = ids[c];
+                else if (part == 2) samp[a.Lc - 1 - pos] = ids[c];
+                else rold[pos] = ids[c];
+            }
+            cnt[part] += popc64(b);
+        }
+    }
+    if (lane == 0) {
+        a.nnew[n] = cnt[0];
+        a.crnew[n] = cnt[1];
+        a.nold[n] = cnt[2];
+        a.crold[n] = cnt[3];
+    }
+}
+
 // ------------------------------------------------------------------ extraction
 __global__ void extract_kernel(NndArgs a, uint32_t const *off, uint32_t *ids, float *dists, uint32_t *M) {
     unsigned const w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1288,6 +1374,34 @@ hipError_t nnd_launch_merge(NndArgs const &a, hipStream_t s) {
 
 hipError_t nnd_launch_iter_begin(NndArgs const &a, int advance, hipStream_t s) {
     hipLaunchKernelGGL(iter_begin_kernel, dim3(1), dim3(64), 0, s, a.stats, a.iter, advance);
+    return hipGetLastError();
+}
+
+namespace {
+// Squared distance of n id pairs in the projection Z (zdim floats per row).
+__global__ void __launch_bounds__(256) zpair_kernel(float const *Z, unsigned P, uint32_t const *pairs, unsigned n,
+                                                    float *out) {
+    unsigned const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float const *za = Z + size_t(pairs[2 * i]) * P, *zb = Z + size_t(pairs[2 * i + 1]) * P;
+    float d = 0.f;
+    for (unsigned k = 0; k < P; ++k) {
+        float const t = za[k] - zb[k];
+        d = __builtin_fmaf(t, t, d);
+    }
+    out[i] = d;
+}
+
+}  // namespace
+
+hipError_t nnd_launch_zpair(float const *Z, unsigned P, uint32_t const *pairs, unsigned n, float *out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(zpair_kernel, dim3((n + 255) / 256), dim3(256), 0, s, Z, P, pairs, n, out);
+    return hipGetLastError();
+}
+
+hipError_t nnd_launch_prerank(NndArgs const &a, hipStream_t s) {
+    hipLaunchKernelGGL(prerank_kernel, dim3((a.N + 3) / 4), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

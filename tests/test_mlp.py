@@ -134,3 +134,47 @@ def test_training_lowers_loss_and_learned_entries_keep_recall():
     rep = g.validate_entry_projection(Q[:100], K=10, P=20, entries=9)
     print(rep)
     assert not rep["enabled"] and rep["recall_learned"] < rep["recall_exact"]
+
+
+@pytest.mark.gpu
+def test_join_prerank_trained_untrained_zeroed():
+    """Learned pre-rank of the local-join lists, against plain NN-descent at
+    equal recall: the trained projector ranks held-out exact-neighbour pairs
+    against random pairs better than the untrained one (the same
+    architecture at its random init), every pre-ranked build ends within a
+    point of plain NN-descent's recall (the in-build guard turns the pre-rank
+    off when it slows the control recall down), and a zeroed projector
+    fails the held-out probe, so that build runs plain NN-descent throughout."""
+    import kgraph_amd
+    from kgraph_amd import _lib, datagen
+    n, D = 50000, 96
+    X = datagen.angular_mixture(n, D, 100, 6, 7, sigma=0.3)
+    g = kgraph_amd.KGraph(X, "angular")
+    bp = dict(K=25, L=100, iterations=30, reverse=0, precision="bf16", controls=200)
+    rows = np.arange(0, n, 50)
+    gi, gd = kgo.bruteforce(X, X[rows], 11, 2)[:2]
+    gt = np.array([[d for i, d in zip(ii, dd) if i != r][:10] for r, ii, dd in zip(rows, gi, gd)], np.float32)
+
+    def build(keep):
+        g.build(prerank=keep, **bp)
+        off, _, dd, _ = g.graph()
+        return kgo.recall_at_k([dd[off[r]:off[r] + 10] for r in rows], gt, 10), g.prerank_report()
+
+    r_plain, rep = build(0)
+    assert rep["status"] == "off"
+    g.train_projector(steps=300, batch=4096, seed=1)
+    r_trained, rep_t = build(0.75)
+    H, P = 256, 32
+    g._h.call("kg_mlp_init", H, P, 1)
+    r_untrained, rep_u = build(0.75)
+    z = [np.zeros((H, D), np.float32), np.zeros(H, np.float32), np.zeros((P, H), np.float32),
+         np.zeros(P, np.float32)]
+    g._h.call("kg_mlp_set_weights", *[_lib.ptr(a) for a in z])
+    r_zero, rep_z = build(0.75)
+    print("plain %.4f trained %.4f %s untrained %.4f %s zeroed %.4f %s" %
+          (r_plain, r_trained, rep_t, r_untrained, rep_u, r_zero, rep_z))
+    assert rep_t["probe_accuracy"] >= rep_u["probe_accuracy"] + 0.02
+    assert rep_t["status"] != "probe-fallback"
+    for r in (r_trained, r_untrained, r_zero):
+        assert r >= r_plain - 0.01
+    assert rep_z["status"] == "probe-fallback" and rep_z["probe_accuracy"] < 0.5

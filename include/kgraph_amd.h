@@ -140,6 +140,25 @@ int kg_search_visited_bits(kg_handle *h, unsigned *bits);
 int kg_set_search_adjacency(kg_handle *h, int mode);
 int kg_search_adjacency(kg_handle *h, int *padded, unsigned *width);
 
+/* BF16 screen of the graph walk's candidates (the reference scores every
+ * fresh neighbour in full precision before UpdateKnnList, kgraph.cpp:497-512).
+ * Mode 1: walks that gather the FP32 rows (rows with no lossless BF16 or
+ * 8-bit copy, D <= 256) score each batch of a full pool first with BF16 dot
+ * products on MFMA (v_mfma_f32_16x16x32_bf16, FP32 accumulate; the row in
+ * BF16, the query as BF16 high and low halves) and re-score in exact FP32
+ * every candidate whose screened distance minus a rigorous error bound is
+ * below the pool's last distance; the others cannot enter the pool.  Results,
+ * comps and scans are identical to mode 0 (off, default).
+ * kg_search_screen: whether the last walk launch screened and how many
+ * candidates it re-scored (comps counts every screened candidate).
+ * kg_search_screen_dists: the screen's distance and error bound for m node
+ * ids per query (ids: nq x m, out_d / out_e: nq x m), with the walk's
+ * arithmetic; |screened - exact FP32| <= bound. */
+int kg_set_search_screen(kg_handle *h, int mode);
+int kg_search_screen(kg_handle *h, int *active, unsigned long long *rescored);
+int kg_search_screen_dists(kg_handle *h, const float *Q, unsigned nq, size_t qstride, const unsigned *ids, unsigned m,
+                           float *out_d, float *out_e);
+
 /* Row storage for graph walks (no counterpart in the reference, whose rows are
  * the caller's Matrix, kgraph-data.h:115-175).  kg_set_data keeps a BF16 copy
  * of the rows (the MFMA kernels' operands), lossless when every value is

@@ -283,6 +283,30 @@ class KGraph:
         return {"status": ["off", "on", "probe-fallback", "recall-fallback"][int(out[0])],
                 "probe_accuracy": float(out[1]), "fallback_iteration": int(out[2])}
 
+    def set_search_screen(self, on):
+        """Screen walk candidates with BF16 dot products on MFMA before the
+        exact FP32 re-score (walks over the FP32 rows; same results)."""
+        self._h.call("kg_set_search_screen", 1 if on else 0)
+
+    def search_screen_report(self):
+        """The last walk launch: {'active': bool, 'rescored': candidates the
+        screen passed to the exact FP32 re-score}."""
+        act = _lib.ctypes.c_int()
+        n = _lib.ctypes.c_ulonglong()
+        self._h.call("kg_search_screen", _lib.ctypes.byref(act), _lib.ctypes.byref(n))
+        return {"active": bool(act.value), "rescored": int(n.value)}
+
+    def screen_dists(self, query, ids):
+        """The walk screen's BF16 distance and error bound of node ids[i, j]
+        against query i: (screened, bound), both nq x m."""
+        q = f32c(query)
+        ii = np.ascontiguousarray(ids, dtype=np.uint32)
+        nq, m = ii.shape
+        d = np.zeros((nq, m), np.float32)
+        e = np.zeros((nq, m), np.float32)
+        self._h.call("kg_search_screen_dists", ptr(q), nq, q.shape[1], ptr(ii), m, ptr(d), ptr(e))
+        return d, e
+
     def build_iterations(self):
         """Last build, one row per iteration: pairs, pushes, list entries,
         join form (0 FP32 tiles, 1 BF16 screen, 2 exact BF16 tiles, 3 probe:

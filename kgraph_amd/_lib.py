@@ -81,6 +81,9 @@ SIGNATURES = {
     "kg_search_visited_bits": (_I, [_P, _P]),
     "kg_set_search_adjacency": (_I, [_P, _I]),
     "kg_search_adjacency": (_I, [_P, _P, _P]),
+    "kg_set_search_screen": (_I, [_P, _I]),
+    "kg_search_screen": (_I, [_P, _P, _P]),
+    "kg_search_screen_dists": (_I, [_P, _P, _U, _SZ, _P, _U, _P, _P]),
     "kg_set_build_graphs": (_I, [_P, _I]),
     "kg_screen_pairs": (_I, [_P, _P, _I, _P, _P]),
     "kg_build_iterations": (_I, [_P, _P, _I, _P]),

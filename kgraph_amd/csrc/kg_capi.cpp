@@ -262,6 +262,9 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
             h.gnode_valid = true;
         }
         bool const pad = h.search_adjacency == 0 && h.gpad_stride && sp.M <= h.gpad_stride;
+        int const form = walk_rows(h);
+        // BF16 screen in front of the FP32 rows (lossy BF16 copy; exact copies are gathered instead)
+        bool const screen = form == 0 && h.search_screen == 1 && h.Dp <= 256;
         // the T > 1 result list (a single trial's result is its pool's head)
         unsigned const rcap = sp.T > 1 ? kmax + 1 : 0u;
         // seeds sorted per trial: init ids, or P random ones (largest P of the batch)
@@ -280,7 +283,7 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
             return (h.search_visited == 0 && idbits <= t + 10) ? 16u : 32u;
         };
         auto lds_for = [&](unsigned c) {
-            return kg_search_lds_per_wave(poolmax, rcap, h.Dp, seed_cap, c, c ? bits_for(c) : 32u, !pad);
+            return kg_search_lds_per_wave(poolmax, rcap, h.Dp, seed_cap, c, c ? bits_for(c) : 32u, !pad, screen);
         };
         auto waves = [&](unsigned c) { return std::min<size_t>(kg_search_waves_per_cu(), h.lds_per_cu / lds_for(c)); };
         unsigned vcap = 0;
@@ -319,9 +322,16 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         }
         kg_search_args a{};
         a.X = h.X.p;
-        int const form = walk_rows(h);
         a.X8 = form == 2 ? h.X8.p : nullptr;
         a.Xb = form == 1 ? h.Xb.p : nullptr;
+        a.Xs = screen ? h.Xb.p : nullptr;
+        h.last_screen = screen ? 1 : 0;
+        if (screen) {
+            kg_search_screen_constants(h.D, a);
+            h.screen_cnt.reserve(1);
+            KG_HIP(hipMemsetAsync(h.screen_cnt.p, 0, 8, h.stream));
+            a.rescored = h.screen_cnt.p;
+        }
         a.norms = h.norms.p;
         a.N = N;
         a.D = h.D;
@@ -771,6 +781,64 @@ int kg_search_adjacency(kg_handle *k, int *padded, unsigned *width) {
     return guard(k, [&](Handle &h) {
         if (padded) *padded = h.last_adjacency == 0 ? 1 : 0;
         if (width) *width = h.gpad_stride;
+    });
+}
+
+int kg_set_search_screen(kg_handle *k, int mode) {
+    return guard(k, [&](Handle &h) {
+        if (mode != 0 && mode != 1) throw kga::invalid_argument("screen mode must be 0 or 1");
+        h.search_screen = mode;
+    });
+}
+
+int kg_search_screen(kg_handle *k, int *active, unsigned long long *rescored) {
+    return guard(k, [&](Handle &h) {
+        unsigned long long n = 0;
+        if (h.last_screen && h.screen_cnt.p) {
+            KG_HIP(hipMemcpyAsync(&n, h.screen_cnt.p, 8, hipMemcpyDeviceToHost, h.stream));
+            h.sync();
+        }
+        if (active) *active = h.last_screen;
+        if (rescored) *rescored = n;
+    });
+}
+
+int kg_search_screen_dists(kg_handle *k, float const *Q, unsigned nq, size_t qstride, unsigned const *ids, unsigned m,
+                           float *out_d, float *out_e) {
+    return guard(k, [&](Handle &h) {
+        if (!h.has_data) throw kga::runtime_error("no dataset");
+        if ((!Q || !ids || !out_d || !out_e) && nq && m) throw kga::invalid_argument("null argument");
+        if (qstride < h.D) throw kga::invalid_argument("bad stride");
+        if (h.Dp > 256) throw kga::invalid_argument("the BF16 screen takes rows of at most 256 values");
+        if (nq == 0 || m == 0) return;
+        size_t const n = size_t(nq) * m;
+        for (size_t i = 0; i < n; ++i)
+            if (ids[i] >= h.N) throw kga::invalid_argument("node id out of range");
+        DevBuf<float> dq, dqn, dd, de;
+        DevBuf<uint32_t> di;
+        upload_rows(h, Q, false, nq, h.D, qstride, dq);
+        dqn.reserve(nq);
+        KG_HIP(kg_launch_norms(h.metric, dq.p, nq, h.D, h.Dp, dqn.p, h.stream));
+        di.reserve(n);
+        dd.reserve(n);
+        de.reserve(n);
+        KG_HIP(hipMemcpyAsync(di.p, ids, 4 * n, hipMemcpyHostToDevice, h.stream));
+        kg_search_args a{};
+        a.X = h.X.p;
+        a.Xs = h.Xb.p;
+        a.norms = h.norms.p;
+        a.N = h.N;
+        a.D = h.D;
+        a.Dp = h.Dp;
+        a.Q = dq.p;
+        a.qnorms = dqn.p;
+        a.nq = nq;
+        a.qstride = h.Dp;
+        kg_search_screen_constants(h.D, a);
+        KG_HIP(kg_launch_screen_dists(h.metric, a, di.p, m, dd.p, de.p, h.stream));
+        KG_HIP(hipMemcpyAsync(out_d, dd.p, 4 * n, hipMemcpyDeviceToHost, h.stream));
+        KG_HIP(hipMemcpyAsync(out_e, de.p, 4 * n, hipMemcpyDeviceToHost, h.stream));
+        h.sync();
     });
 }
 

@@ -88,6 +88,11 @@ struct Handle {
     unsigned search_vcap = 0;  // LDS set entries (0: chosen per launch)
     unsigned last_vt_bits = 0;
     unsigned last_vcap = 0;  // LDS visited-set entries of the last walk launch
+    // 1: walks over the FP32 rows screen each batch with the BF16 copy on MFMA
+    // before the exact FP32 re-score (same results); 0: off
+    int search_screen = 0;
+    int last_screen = 0;                         // whether the last walk launch screened
+    DevBuf<unsigned long long> screen_cnt;       // candidates it re-scored
     bool has_data = false;
 
     // index: CSR with 32-bit edge offsets

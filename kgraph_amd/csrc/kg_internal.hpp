@@ -10,6 +10,14 @@ struct kg_search_args {
     // data rows (row stride Dp floats, 16-byte aligned) and per-row norms
     float const *X;
     uint16_t const *Xb;  // non-NULL: lossless BF16 copy of X (same stride), gathered instead of X
+    // non-NULL (with X, Xb and X8 NULL): BF16 copy of X (same stride, rounded
+    // to nearest even) that screens each batch on MFMA before the exact FP32
+    // re-score of the candidates that can still enter the pool
+    uint16_t const *Xs;
+    // screen error bound: scr_u row rounding (2^-8 relative), scr_c the rest
+    // of the dot's error relative to |x||q|, scr_g rounding of norms and sums
+    float scr_u, scr_c, scr_g;
+    unsigned long long *rescored;  // adds the candidates the screen passed to the re-score (may be NULL)
     uint8_t const *X8;   // non-NULL: lossless 8-bit copy of X (row stride Dp bytes), gathered first
     float const *norms;
     unsigned N, D, Dp;
@@ -114,7 +122,14 @@ struct kg_rescore_args {
 
 // Launchers (defined in the .hip files).
 size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp, unsigned seed_cap, unsigned vcap,
-                              unsigned vt_bits, bool csr_base);
+                              unsigned vt_bits, bool csr_base, bool screen);
+// The walk's BF16 screen constants for rows of D values (a.scr_u / scr_c / scr_g).
+void kg_search_screen_constants(unsigned D, kg_search_args &a);
+// Screened distances and error bounds of m nodes per query (ids: a.nq x m),
+// with the walk's screen arithmetic; needs Xs, norms, D, Dp <= 256, Q,
+// qnorms, qstride, nq and the screen constants of a.
+hipError_t kg_launch_screen_dists(int metric, kg_search_args const &a, uint32_t const *ids, unsigned m, float *out_d,
+                                  float *out_e, hipStream_t s);
 // Largest waves per block search_kernel is compiled for (its launch bound);
 // kg_launch_search rejects larger shapes.
 unsigned kg_search_max_wpb();

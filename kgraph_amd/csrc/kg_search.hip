@@ -26,6 +26,8 @@
 // inserts one by one in neighbour order.  That equals the reference's
 // per-neighbour loop because neither the visited state nor the distances
 // depend on the inserts.
+#include <cmath>
+
 #include "kg_device.hpp"
 #include "kg_internal.hpp"
 
@@ -47,6 +49,8 @@ struct WaveLDS {
     float *rdist;
     float *q;
     uint32_t *q8;  // the query as 8-bit words (integer path over the 8-bit rows)
+    uint4 *qh, *ql;  // BF16 screen: the query's BF16 high and low halves, 8 values a word
+    uint8_t *sidx;   // BF16 screen: batch positions of the candidates to re-score
     uint32_t *fid;
     float *fdist;
     uint32_t *vt;  // visited hash set (vcap entries of 32 or 16 bits, all ones = free)
@@ -55,14 +59,16 @@ struct WaveLDS {
 __host__ __device__ inline size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
 struct LdsLayout {
-    size_t pid, pdist, pmeta, pbase, tkey, rid, rdist, q, q8, fid, fdist, vt, total;
+    size_t pid, pdist, pmeta, pbase, tkey, rid, rdist, q, q8, qh, ql, sidx, fid, fdist, vt, total;
     // tcap: starting points sorted at once (P random seeds, or init ids);
     // vcap: entries of the visited hash set (0: none)
     // vbytes: bytes per visited-set entry (4, or 2 for the compact form)
     // base: pool entries keep their CSR base offset (walks over the CSR
     // lists; the padded rows need none)
+    // screen: the BF16 screen's query halves and re-score list
     __host__ __device__ LdsLayout(unsigned cap, unsigned rcap, unsigned Dp, unsigned tcap, unsigned vcap,
-                                  unsigned vbytes, bool base) {
+                                  unsigned vbytes, bool base, bool screen) {
+        size_t const d32 = screen ? (Dp + 31) / 32 * 32 : 0;  // BF16 values per query half
         size_t o = 0;
         pid = o; o = align16(o + 4ull * cap);
         pdist = o; o = align16(o + 4ull * cap);
@@ -73,6 +79,9 @@ struct LdsLayout {
         rdist = o; o = align16(o + 4ull * rcap);
         q = o; o = align16(o + 4ull * Dp);
         q8 = o; o = align16(o + Dp);
+        qh = o; o = align16(o + 2 * d32);
+        ql = o; o = align16(o + 2 * d32);
+        sidx = o; o = align16(o + (screen ? KG_WAVE : 0));
         fid = o; o = align16(o + 4ull * KG_WAVE);
         fdist = o; o = align16(o + 4ull * KG_WAVE);
         vt = o; o = align16(o + size_t(vbytes) * vcap);
@@ -191,6 +200,184 @@ __device__ __forceinline__ void batch_dist(kg_search_args const &a, uint32_t con
         }
     }
     wave_sync();
+}
+
+// ---- BF16 screen (float rows, lossy BF16 copy a.Xs)
+//
+// A full pool admits a candidate only if its distance is below the pool's
+// last one (batch_insert drops every candidate at or past that distance), so
+// a candidate whose screened distance minus the screen's error bound is at
+// least that distance cannot enter it: it gets +inf, which batch_insert drops
+// in the same way, and the others are re-scored in exact FP32.  The pool, the
+// expansion order and the results are the unscreened walk's.
+//
+// The screen scores 16 candidates per v_mfma_f32_16x16x32_bf16 pass of 32
+// dimensions: A = the candidates' BF16 rows (lane l: row l & 15, values
+// 8 (l >> 4) .. + 8 of the pass), B = the query in every column.  The query
+// goes in as two BF16 halves (q = qh + ql + r, |r| <= 2^-16 |q|) in two
+// passes, so only the row's rounding (2^-8 relative) is left; products of
+// BF16 values are exact in FP32 and the MFMA accumulates in FP32.  For the
+// SSD metric a third pass with the rows as both operands gives |x~|^2 on the
+// diagonal.  C[m][n] sits in lane n + 16 (m >> 2), register m & 3, so the
+// lane with n = m (there are 16 such lanes, in candidate order) owns
+// candidate m's dot and norm.
+typedef __bf16 ws_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float ws_f32x4 __attribute__((ext_vector_type(4)));
+
+// 8 BF16 values of row r from column k (zeros at and past Dp; Dp is a
+// multiple of 4, rows are 16-byte aligned when Dp is a multiple of 8)
+__device__ __forceinline__ uint4 ws_load(uint16_t const *r, unsigned k, unsigned Dp, bool wide) {
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (k + 8 <= Dp) {
+        if (wide) {
+            v = *reinterpret_cast<uint4 const *>(r + k);
+        } else {
+            uint2 const lo = *reinterpret_cast<uint2 const *>(r + k);
+            uint2 const hi = *reinterpret_cast<uint2 const *>(r + k + 4);
+            v = make_uint4(lo.x, lo.y, hi.x, hi.y);
+        }
+    } else if (k + 4 <= Dp) {
+        uint2 const lo = *reinterpret_cast<uint2 const *>(r + k);
+        v = make_uint4(lo.x, lo.y, 0u, 0u);
+    }
+    return v;
+}
+
+__device__ __forceinline__ float ws_pick(ws_f32x4 v, unsigned r) {
+    return r == 0 ? v[0] : r == 1 ? v[1] : r == 2 ? v[2] : v[3];
+}
+
+// Screened distance of candidate rows against the query, and its error
+// bound E: |screened - exact FP32| <= E.  G = screened dot, nxs = screened
+// |x~|^2 (SSD only), na = the row's stored norm (|x| angular, |x|^2 / 2
+// Euclidean), qn = the query's, qrn = |q|, qsq = |q|^2 (SSD).
+template <int METRIC>
+__device__ __forceinline__ float2 ws_finish(kg_search_args const &a, float G, float nxs, float na, float qn, float qrn,
+                                            float qsq) {
+    float ds, E;
+    if (METRIC == METRIC_ANGULAR) {
+        ds = -G / __builtin_fmaf(na, qn, 1.1920928955078125e-07f);
+        E = (a.scr_u + a.scr_c) * 1.01f + a.scr_g * fabsf(ds);
+    } else if (METRIC == METRIC_EUCLIDEAN) {
+        ds = (na + qn) - G;
+        float const xr = sqrtf(2.f * na);
+        E = (a.scr_u + a.scr_c) * 1.01f * xr * qrn + a.scr_g * (na + qn + fabsf(G));
+    } else {
+        ds = (nxs + qsq) - 2.f * G;
+        // |x| <= |x~| (1 + 2^-7); the screen computes |x~ - q|^2 within E0,
+        // and |x - q|^2 - |x~ - q|^2 = 2 e.(x~ - q) + |e|^2 with |e| <= 2^-8 |x|
+        float const xr = sqrtf(fmaxf(nxs, 0.f)) * 1.0078125f;
+        float const E0 = 2.f * a.scr_c * xr * qrn + a.scr_g * (fmaxf(nxs, 0.f) + qsq + 2.f * fabsf(G));
+        float const dd = sqrtf(fmaxf(ds, 0.f) + E0);
+        float const E1 = 2.f * a.scr_u * xr * dd + a.scr_u * a.scr_u * xr * xr + E0;
+        E = E1 * 1.01f + a.scr_g * (fmaxf(ds, 0.f) + E1);
+    }
+    return make_float2(ds, E);
+}
+
+// MFMA screen of 16 candidates ids[b0 .. b0 + 16) (ok: lane's row exists):
+// returns, in the owner lanes, the screened dot (x) and |x~|^2 (y).
+template <int METRIC>
+__device__ __forceinline__ float2 ws_mfma(kg_search_args const &a, uint32_t id, bool ok, uint4 const *qh,
+                                          uint4 const *ql) {
+    unsigned const lane = lane_id();
+    unsigned const k8 = (lane >> 4) * 8;
+    unsigned const ks = (a.Dp + 31) / 32;
+    bool const wide = (a.Dp & 7u) == 0;
+    uint16_t const *row = a.Xs + size_t(id) * a.Dp;
+    ws_f32x4 g = {0.f, 0.f, 0.f, 0.f}, nx = {0.f, 0.f, 0.f, 0.f};
+    for (unsigned s0 = 0; s0 < ks; s0 += 4) {
+        uint4 v[4];
+#pragma unroll
+        for (unsigned j = 0; j < 4; ++j)
+            v[j] = (ok && s0 + j < ks) ? ws_load(row, 32 * (s0 + j) + k8, a.Dp, wide) : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+        for (unsigned j = 0; j < 4; ++j) {
+            if (s0 + j < ks) {
+                ws_bf16x8 const x = __builtin_bit_cast(ws_bf16x8, v[j]);
+                unsigned const qi = 4 * (s0 + j) + (lane >> 4);
+                g = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x, __builtin_bit_cast(ws_bf16x8, qh[qi]), g, 0, 0, 0);
+                g = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x, __builtin_bit_cast(ws_bf16x8, ql[qi]), g, 0, 0, 0);
+                if (METRIC == METRIC_L2SQR) nx = __builtin_amdgcn_mfma_f32_16x16x32_bf16(x, x, nx, 0, 0, 0);
+            }
+        }
+    }
+    unsigned const r = lane & 3;
+    return make_float2(ws_pick(g, r), METRIC == METRIC_L2SQR ? ws_pick(nx, r) : 0.f);
+}
+
+// Distances of ids[0..cnt) into out[0..cnt) for a full pool whose last
+// distance is thr: +inf for every candidate the screen rules out, the exact
+// FP32 distance for the rest (nres counts them).
+template <int METRIC>
+__device__ void batch_screen(kg_search_args const &a, WaveLDS const &w, uint32_t const *ids, float *out, unsigned cnt,
+                             float qn, float qrn, float qsq, float thr, unsigned &nres) {
+    unsigned const lane = lane_id();
+    uint64_t const lt_mask = (1ull << lane) - 1;
+    unsigned const n = lane & 15;
+    bool const own = (n >> 2) == (lane >> 4);  // this lane holds candidate n's dot
+    unsigned ns = 0;
+    for (unsigned b0 = 0; b0 < cnt; b0 += 16) {
+        unsigned const c = b0 + n;
+        bool const ok = c < cnt;
+        uint32_t const id = ok ? (ids[c] & IDMASK) : 0u;
+        float2 const gn = ws_mfma<METRIC>(a, id, ok, w.qh, w.ql);
+        bool const live = own && ok;
+        float const na = (METRIC != METRIC_L2SQR && live) ? a.norms[id] : 0.f;
+        float2 const de = ws_finish<METRIC>(a, gn.x, gn.y, na, qn, qrn, qsq);
+        bool const keep = live && !(de.x - de.y >= thr);  // NaN: re-score
+        uint64_t const kb = ballot(keep);
+        if (keep) w.sidx[ns + popc64(kb & lt_mask)] = uint8_t(c);
+        else if (live) out[c] = __builtin_huge_valf();
+        ns += popc64(kb);
+    }
+    wave_sync();
+    // exact FP32 distances of the candidates left, 4 lanes per row
+    unsigned const grp = lane >> 2, t = lane & 3;
+    for (unsigned b0 = 0; b0 < ns; b0 += 16) {
+        if (b0 + grp < ns) {  // whole quads are active together
+            unsigned const c = w.sidx[b0 + grp];
+            uint32_t const id = ids[c] & IDMASK;
+            float const acc = quad_metric<METRIC>(a.X + size_t(id) * a.Dp, w.q, a.D, t);
+            float const na = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[id];
+            float const d = finish_metric<METRIC>(acc, na, qn);
+            if (t == 0) out[c] = d;
+        }
+    }
+    nres += ns;
+    wave_sync();
+}
+
+// The query's BF16 halves for the screen (qh = bf16(q), ql = bf16(q - qh),
+// zero past D), and |q|^2 in FP32.
+__device__ __forceinline__ float ws_query(WaveLDS const &w, unsigned D, unsigned Dp) {
+    unsigned const lane = lane_id();
+    unsigned const words = (Dp + 31) / 32 * 4;  // uint4 words per half
+    float sq = 0.f;
+    for (unsigned i = lane; i < words; i += KG_WAVE) {
+        uint32_t h[4], l[4];
+#pragma unroll
+        for (unsigned j = 0; j < 4; ++j) {
+            uint32_t hh = 0, ll = 0;
+#pragma unroll
+            for (unsigned e = 0; e < 2; ++e) {
+                unsigned const col = 8 * i + 2 * j + e;
+                float const v = col < D ? w.q[col] : 0.f;
+                uint16_t const hb = bf16_bits(v);
+                uint16_t const lb = bf16_bits(v - __uint_as_float(uint32_t(hb) << 16));
+                hh |= uint32_t(hb) << (16 * e);
+                ll |= uint32_t(lb) << (16 * e);
+                sq = __builtin_fmaf(v, v, sq);
+            }
+            h[j] = hh;
+            l[j] = ll;
+        }
+        w.qh[i] = make_uint4(h[0], h[1], h[2], h[3]);
+        w.ql[i] = make_uint4(l[0], l[1], l[2], l[3]);
+    }
+    for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
+    wave_sync();
+    return sq;
 }
 
 // Number of entries of the sorted pool[0..L) whose dist <= nd.
@@ -382,7 +569,9 @@ __device__ __forceinline__ bool vt_has(uint32_t const *vt, VtCfg const &c, uint3
 #ifndef SEARCH_WAVES_PER_EU
 #define SEARCH_WAVES_PER_EU 4
 #endif
-template <int METRIC, typename RowT>
+// SCREEN (RowT = float): each batch of a full pool goes through the BF16
+// screen (batch_screen) before the exact FP32 re-score.
+template <int METRIC, typename RowT, bool SCREEN = false>
 __global__ void __launch_bounds__(SEARCH_MAX_WPB * KG_WAVE) __attribute__((amdgpu_waves_per_eu(SEARCH_WAVES_PER_EU)))
 search_kernel(kg_search_args a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -391,7 +580,7 @@ search_kernel(kg_search_args a) {
     unsigned const slot = blockIdx.x * (blockDim.x / KG_WAVE) + wib;
     unsigned const cap = a.pool_cap;
     unsigned const rcap = a.res_cap;
-    LdsLayout lay(cap, rcap, a.Dp, a.seed_cap, a.vcap, a.vt_bits / 8, a.gpad == nullptr);
+    LdsLayout lay(cap, rcap, a.Dp, a.seed_cap, a.vcap, a.vt_bits / 8, a.gpad == nullptr, SCREEN);
     unsigned char *base = smem + size_t(wib) * a.lds_per_wave;
     WaveLDS w;
     w.pid = reinterpret_cast<uint32_t *>(base + lay.pid);
@@ -403,6 +592,9 @@ search_kernel(kg_search_args a) {
     w.rdist = reinterpret_cast<float *>(base + lay.rdist);
     w.q = reinterpret_cast<float *>(base + lay.q);
     w.q8 = reinterpret_cast<uint32_t *>(base + lay.q8);
+    w.qh = reinterpret_cast<uint4 *>(base + lay.qh);
+    w.ql = reinterpret_cast<uint4 *>(base + lay.ql);
+    w.sidx = base + lay.sidx;
     w.fid = reinterpret_cast<uint32_t *>(base + lay.fid);
     w.fdist = reinterpret_cast<float *>(base + lay.fdist);
     w.vt = reinterpret_cast<uint32_t *>(base + lay.vt);
@@ -458,6 +650,13 @@ search_kernel(kg_search_args a) {
             qv.sq = sq;
         }
         wave_sync();
+        // BF16 screen: the query's halves, |q|^2 and |q|
+        float qsq = 0.f, qrn = 0.f;
+        unsigned n_res = 0;
+        if constexpr (SCREEN) {
+            qsq = ws_query(w, a.D, a.Dp);
+            qrn = (METRIC == METRIC_ANGULAR) ? qn : (METRIC == METRIC_EUCLIDEAN) ? sqrtf(2.f * qn) : sqrtf(qsq);
+        }
 
         unsigned ntouch = 0, n_comps = 0, n_scan = 0, RL = 0;
         // visited ids: in the LDS set while it has room (vcount entries), else
@@ -661,7 +860,10 @@ search_kernel(kg_search_args a) {
                     // lane j fetches candidate j's M and CSR base while the rows load
                     uint32_t cm = VT_M_UNKNOWN, cb = 0;
                     if (lane < nf && !a.gpad) cm = actual_M(a, w.fid[lane], &cb);
-                    batch_dist<METRIC, RowT>(a, w.fid, w.fdist, nf, w.q, qn, w.q8, qv);
+                    if (SCREEN && L == qcap - 1)
+                        batch_screen<METRIC>(a, w, w.fid, w.fdist, nf, qn, qrn, qsq, w.pdist[L - 1], n_res);
+                    else
+                        batch_dist<METRIC, RowT>(a, w.fid, w.fdist, nf, w.q, qn, w.q8, qv);
                     // the nf inserts of this batch, done at once (batch_insert)
                     n_comps += nf;
                     batch_insert(w, nf, cm, cb, qcap, L, k);
@@ -730,6 +932,7 @@ search_kernel(kg_search_args a) {
             if (a.comps) a.comps[qi] = n_comps;
             if (a.scans) a.scans[qi] = n_scan;
             if (a.lat_us) a.lat_us[qi] = float(__builtin_amdgcn_s_memrealtime() - t_start) * 0.01f;
+            if (SCREEN && a.rescored) atomicAdd(a.rescored, (unsigned long long)n_res);
         }
         // ---- clear the visited bits this query set (none while the LDS set held them)
         if (!glob) {
@@ -811,6 +1014,41 @@ __global__ void __launch_bounds__(256) init_check_kernel(uint32_t const *ids, un
     }
 }
 
+// Screened distances and bounds of m candidates per query (ids row qi), one
+// wave per query, with the walk's screen arithmetic (ws_query, ws_mfma,
+// ws_finish).
+constexpr unsigned WS_MAX_DP = 256;
+template <int METRIC>
+__global__ void __launch_bounds__(KG_WAVE) screen_dists_kernel(kg_search_args a, uint32_t const *ids, unsigned m,
+                                                                float *out_d, float *out_e) {
+    __shared__ __align__(16) float q[WS_MAX_DP];
+    __shared__ uint4 qh[WS_MAX_DP / 8], ql[WS_MAX_DP / 8];
+    WaveLDS w{};
+    w.q = q;
+    w.qh = qh;
+    w.ql = ql;
+    unsigned const qi = blockIdx.x, lane = lane_id();
+    for (unsigned i = lane; i < a.Dp; i += KG_WAVE) q[i] = i < a.D ? a.Q[size_t(qi) * a.qstride + i] : 0.f;
+    wave_sync();
+    float const qn = (METRIC == METRIC_L2SQR) ? 0.f : a.qnorms[qi];
+    float const qsq = ws_query(w, a.D, a.Dp);
+    float const qrn = (METRIC == METRIC_ANGULAR) ? qn : (METRIC == METRIC_EUCLIDEAN) ? sqrtf(2.f * qn) : sqrtf(qsq);
+    unsigned const n = lane & 15;
+    bool const own = (n >> 2) == (lane >> 4);
+    for (unsigned b0 = 0; b0 < m; b0 += 16) {
+        unsigned const c = b0 + n;
+        bool const ok = c < m;
+        uint32_t const id = ok ? ids[size_t(qi) * m + c] : 0u;
+        float2 const gn = ws_mfma<METRIC>(a, id, ok, qh, ql);
+        if (own && ok) {
+            float const na = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[id];
+            float2 const de = ws_finish<METRIC>(a, gn.x, gn.y, na, qn, qrn, qsq);
+            out_d[size_t(qi) * m + c] = de.x;
+            out_e[size_t(qi) * m + c] = de.y;
+        }
+    }
+}
+
 }  // namespace
 
 hipError_t kg_launch_gpad_width(unsigned N, uint32_t const *goff, uint32_t const *gM, uint32_t *width, hipStream_t s) {
@@ -844,8 +1082,8 @@ hipError_t kg_launch_gnode(unsigned N, uint32_t const *goff, uint32_t const *gid
 }
 
 size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp, unsigned seed_cap, unsigned vcap,
-                              unsigned vt_bits, bool csr_base) {
-    return LdsLayout(pool_cap, res_cap, Dp, seed_cap, vcap, vt_bits / 8, csr_base).total;
+                              unsigned vt_bits, bool csr_base, bool screen) {
+    return LdsLayout(pool_cap, res_cap, Dp, seed_cap, vcap, vt_bits / 8, csr_base, screen).total;
 }
 
 unsigned kg_search_max_wpb() { return SEARCH_MAX_WPB; }
@@ -857,27 +1095,62 @@ hipError_t kg_launch_search(int metric, kg_search_args const &a, unsigned blocks
     if (waves_per_block == 0 || waves_per_block > SEARCH_MAX_WPB || blocks == 0) return hipErrorInvalidValue;
     size_t lds = size_t(waves_per_block) * a.lds_per_wave;
     dim3 grid(blocks), block(waves_per_block * KG_WAVE);
-    // row form: the 8-bit copy, the BF16 copy, or the FP32 rows
-    auto go = [&](auto kernel8, auto kernel16, auto kernel32) {
+    // row form: the 8-bit copy, the BF16 copy, the FP32 rows behind the BF16
+    // screen, or the FP32 rows
+    if (a.Xs && (a.X8 || a.Xb || a.Dp > 256)) return hipErrorInvalidValue;
+    auto go = [&](auto kernel8, auto kernel16, auto kernel32s, auto kernel32) {
         if (a.X8)
             hipLaunchKernelGGL(kernel8, grid, block, lds, s, a);
         else if (a.Xb)
             hipLaunchKernelGGL(kernel16, grid, block, lds, s, a);
+        else if (a.Xs)
+            hipLaunchKernelGGL(kernel32s, grid, block, lds, s, a);
         else
             hipLaunchKernelGGL(kernel32, grid, block, lds, s, a);
     };
     switch (metric) {
         case METRIC_L2SQR:
             go(search_kernel<METRIC_L2SQR, uint8_t>, search_kernel<METRIC_L2SQR, uint16_t>,
-               search_kernel<METRIC_L2SQR, float>);
+               search_kernel<METRIC_L2SQR, float, true>, search_kernel<METRIC_L2SQR, float>);
             break;
         case METRIC_EUCLIDEAN:
             go(search_kernel<METRIC_EUCLIDEAN, uint8_t>, search_kernel<METRIC_EUCLIDEAN, uint16_t>,
-               search_kernel<METRIC_EUCLIDEAN, float>);
+               search_kernel<METRIC_EUCLIDEAN, float, true>, search_kernel<METRIC_EUCLIDEAN, float>);
             break;
         default:
             go(search_kernel<METRIC_ANGULAR, uint8_t>, search_kernel<METRIC_ANGULAR, uint16_t>,
-               search_kernel<METRIC_ANGULAR, float>);
+               search_kernel<METRIC_ANGULAR, float, true>, search_kernel<METRIC_ANGULAR, float>);
+            break;
+    }
+    return hipGetLastError();
+}
+
+void kg_search_screen_constants(unsigned D, kg_search_args &a) {
+    unsigned const Dp = (D + 3) & ~3u;
+    double const eps = std::ldexp(1.0, -24);
+    double const u = std::ldexp(1.0, -8);                        // BF16 rounding of a row value
+    double const gn = (64.0 * ((Dp + 31) / 32) + 16) * eps;      // MFMA accumulation over both query halves
+    double const gd = (D + 16) * eps;                            // the exact FP32 sums over D values
+    a.scr_u = float(u * (1 + std::ldexp(1.0, -20)));
+    a.scr_c = float((std::ldexp(1.0, -16) * (1 + u) + (gn + gd) * (1 + u) * (1 + u)) * 1.05);
+    a.scr_g = float((gn + gd + 4 * eps) * 1.05);
+}
+
+hipError_t kg_launch_screen_dists(int metric, kg_search_args const &a, uint32_t const *ids, unsigned n, float *out_d,
+                                  float *out_e, hipStream_t s) {
+    if (a.Dp > WS_MAX_DP || !a.Xs) return hipErrorInvalidValue;
+    if (a.nq == 0 || n == 0) return hipSuccess;
+    switch (metric) {
+        case METRIC_L2SQR:
+            hipLaunchKernelGGL(screen_dists_kernel<METRIC_L2SQR>, dim3(a.nq), dim3(KG_WAVE), 0, s, a, ids, n, out_d, out_e);
+            break;
+        case METRIC_EUCLIDEAN:
+            hipLaunchKernelGGL(screen_dists_kernel<METRIC_EUCLIDEAN>, dim3(a.nq), dim3(KG_WAVE), 0, s, a, ids, n, out_d,
+                               out_e);
+            break;
+        default:
+            hipLaunchKernelGGL(screen_dists_kernel<METRIC_ANGULAR>, dim3(a.nq), dim3(KG_WAVE), 0, s, a, ids, n, out_d,
+                               out_e);
             break;
     }
     return hipGetLastError();

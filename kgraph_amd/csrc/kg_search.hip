@@ -407,21 +407,33 @@ __device__ __forceinline__ unsigned insert_pos(float const *pdist, unsigned L, f
 // candidate (no later insert lands before it, no earlier one after it), so
 // the rewind is k = min(k, that position) when it is kept.  cm / cb hold
 // candidate `lane`'s M and CSR base; w.fid / w.fdist its id and distance.
+// A full pool (L = qcap - 1) keeps only candidates below its last distance:
+// any other lands at or past Lf = L whatever its rank and is below no pool
+// entry, so only the ones below take part (most candidates of a long walk
+// are not, and then nothing moves).
 __device__ void batch_insert(WaveLDS &w, unsigned nf, uint32_t cm, uint32_t cb, unsigned qcap, unsigned &L,
                              unsigned &k) {
     unsigned const lane = lane_id();
-    bool const live = lane < nf;
-    float const dl = live ? w.fdist[lane] : __builtin_huge_valf();
-    // rank of candidate `lane` in (dist, neighbour order): no sort needed
+    unsigned const L0 = L, Lf = min(L0 + nf, qcap - 1);
+    bool const full = L0 > 0 && L0 == qcap - 1;
+    float const dl = lane < nf ? w.fdist[lane] : __builtin_huge_valf();
+    bool const live = lane < nf && (!full || dl < w.pdist[L0 - 1]);
+    uint64_t const lm = ballot(live);
+    unsigned const na = popc64(lm);
+    if (na == 0) {
+        L = Lf;
+        return;
+    }
+    // rank of candidate `lane` in (dist, neighbour order) among those taking part
     unsigned rank = 0;
-    for (unsigned jj = 0; jj < nf; ++jj) {
+    for (uint64_t m = lm; m; m &= m - 1) {
+        unsigned const jj = ctz64(m);
         float const dj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dl), int(jj)));
         rank += unsigned(dj < dl || (dj == dl && jj < lane));
     }
     uint32_t const sid = live ? w.fid[lane] : 0u;
     uint32_t const sm = cm, sb = cb;
     float const sd = dl;
-    unsigned const L0 = L, Lf = min(L0 + nf, qcap - 1);
     // final position: pool entries with dist <= sd, plus the rank
     unsigned fin = 0;
     if (live) {
@@ -453,7 +465,7 @@ __device__ void batch_insert(WaveLDS &w, unsigned nf, uint32_t cm, uint32_t cb, 
             d = w.pdist[i];
             meta = w.pmeta[i];
             if (w.pbase) bs = w.pbase[i];
-            unsigned lo = 0, hi = nf;
+            unsigned lo = 0, hi = na;
             while (lo < hi) {
                 unsigned const mid = (lo + hi) >> 1;
                 if (w.fdist[mid] < d) lo = mid + 1; else hi = mid;

@@ -303,6 +303,21 @@ int kg_set_entry_projection(kg_handle *h, unsigned shortlist);
 int kg_autotune_search(kg_handle *h, const float *dQ, unsigned nq, size_t qstride, const kg_search_params *params,
                        const unsigned *Ks, const unsigned *Ps, unsigned *chosen);
 
+/* Walk autotuner (no counterpart in the reference): on the given
+ * device-resident query sample (with the caller's starting points when
+ * params->init > 0: d_init holds init ids per query, row stride istride),
+ * times the walk over the settings that leave its results unchanged and keeps
+ * the fastest for later searches on this handle: the visited set (bitset
+ * only, or an LDS set of 1024, 2048, 4096 or 8192 entries in front of it),
+ * the adjacency rows (padded or CSR), the BF16 screen (where walks gather the
+ * FP32 rows) and the launch shape (as kg_autotune_search).  One setting at a
+ * time, in that order, each kept when faster.  chosen (may be NULL) receives
+ * {waves per block, blocks per CU, visited-set entries (0: bitset only),
+ * adjacency (0 padded, 1 CSR), screen (0/1)}. */
+int kg_autotune_walk(kg_handle *h, const float *dQ, unsigned nq, size_t qstride, const kg_search_params *params,
+                     const unsigned *Ks, const unsigned *Ps, const unsigned *d_init, size_t istride,
+                     unsigned *chosen);
+
 /* Launch shape of later walks (no counterpart in the reference): waves per
  * block (1, 2 or 4) and at most blocks_per_cu resident blocks per CU (0: as
  * many as the LDS and the 32-wave limit allow).  Results do not depend on it. */

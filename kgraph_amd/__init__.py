@@ -262,6 +262,31 @@ class KGraph:
         devmem.synchronize()
         return chosen[0], chosen[1]
 
+    def autotune_walk(self, query, K=default_K, P=default_P, S=default_S, *, Ks=None, Ps=None, entries=0):
+        """Times the walk on `query` over the settings that leave its results
+        unchanged (visited set, adjacency rows, BF16 screen, launch shape),
+        starting from the entry points of `entries` as search() would, and
+        keeps the fastest for later searches.  Returns the chosen settings."""
+        from . import devmem
+        q = f32c(query)
+        nq = q.shape[0]
+        Ks_a = u32c(Ks) if Ks is not None else None
+        Ps_a = u32c(Ps) if Ps is not None else None
+        width = int(Ks_a.max()) if Ks_a is not None and nq else K
+        dq = devmem.DeviceArray.from_host(q)
+        init = None
+        if entries:
+            ids = np.zeros((nq, max(width, entries)), dtype=np.uint32)
+            self._h.call("kg_select_entries", ptr(q), nq, q.shape[1], entries, ptr(ids), ids.shape[1])
+            init = devmem.DeviceArray.from_host(ids)
+        chosen = (_lib.ctypes.c_uint * 5)()
+        sp = SearchParams(K, default_M, P, S, default_T, default_epsilon, default_seed, entries)
+        self._h.call("kg_autotune_walk", dq.ptr, nq, q.shape[1], sp, ptr(Ks_a), ptr(Ps_a),
+                     init.ptr if init is not None else None, init.shape[1] if init is not None else 0, chosen)
+        devmem.synchronize()
+        return {"waves_per_block": chosen[0], "blocks_per_cu": chosen[1], "visited_entries": chosen[2],
+                "adjacency": "csr" if chosen[3] else "padded", "screen": bool(chosen[4])}
+
     def build_counters(self):
         """Last build: {'pairs', 'list_entries', 'pushes', 'iterations',
         'join_form', 'graph_replays', 'rescored'} with join_form 'fp32',

@@ -1253,6 +1253,118 @@ int kg_autotune_search(kg_handle *k, float const *dQ, unsigned nq, size_t qstrid
     });
 }
 
+int kg_autotune_walk(kg_handle *k, float const *dQ, unsigned nq, size_t qstride, kg_search_params const *params,
+                     unsigned const *Ks, unsigned const *Ps, unsigned const *d_init, size_t istride, unsigned *chosen) {
+    return guard(k, [&](Handle &h) {
+        if (!params || !dQ || nq == 0) throw kga::invalid_argument("null argument");
+        if (params->init && (!d_init || istride < params->init))
+            throw kga::invalid_argument("starting points need a device buffer of init ids per query");
+        unsigned kmax = params->K;
+        if (Ks)
+            for (unsigned q = 0; q < nq; ++q) kmax = std::max(kmax, Ks[q]);
+        kmax = std::max(kmax, 1u);
+        size_t const ostride = std::max<size_t>(kmax, params->init ? istride : 0);
+        DevBuf<uint32_t> di, dc;
+        DevBuf<float> dd;
+        di.reserve(size_t(nq) * ostride);
+        dd.reserve(size_t(nq) * ostride);
+        dc.reserve(nq);
+        // every setting below returns the same walk; each is timed on this
+        // sample (kernel time from HIP events, second of two runs)
+        auto run = [&]() {
+            float t = 0;
+            for (int rep = 0; rep < 2; ++rep) {
+                if (params->init)
+                    KG_HIP(hipMemcpy2DAsync(di.p, ostride * 4, d_init, istride * 4, size_t(params->init) * 4, nq,
+                                            hipMemcpyDeviceToDevice, h.stream));
+                search_device(h, dQ, nq, qstride, *params, Ks, Ps, di.p, dd.p, ostride, dc.p, nullptr, nullptr);
+                t = h.last_ms[0];
+            }
+            return t;
+        };
+        int const old_visited = h.search_visited, old_adj = h.search_adjacency, old_screen = h.search_screen;
+        unsigned const old_vcap = h.search_vcap, old_wpb = h.search_wpb, old_bpc = h.search_bpc;
+        try {
+            h.search_wpb = 4;
+            h.search_bpc = 0;
+            float best = run();
+            // 1. visited set: LDS set of 1024..8192 entries in front of the
+            // bitset (16-bit entries where the ids allow), or the bitset only
+            struct V {
+                int mode;
+                unsigned vcap;
+            };
+            V bv{h.search_visited, h.search_vcap};
+            for (V v : {V{1, 0u}, V{0, 1024u}, V{0, 2048u}, V{0, 4096u}, V{0, 8192u}}) {
+                h.search_visited = v.mode;
+                h.search_vcap = v.vcap;
+                float const t = run();
+                if (t < best) {
+                    best = t;
+                    bv = v;
+                }
+            }
+            h.search_visited = bv.mode;
+            h.search_vcap = bv.vcap;
+            // 2. adjacency rows: padded or CSR
+            int badj = h.search_adjacency;
+            for (int adj : {0, 1}) {
+                if (adj == badj) continue;
+                h.search_adjacency = adj;
+                float const t = run();
+                if (t < best) {
+                    best = t;
+                    badj = adj;
+                }
+            }
+            h.search_adjacency = badj;
+            // 3. the BF16 screen, where walks gather the FP32 rows
+            int bscr = h.search_screen;
+            if (walk_rows(h) == 0 && h.Dp <= 256) {
+                h.search_screen = 1 - bscr;
+                float const t = run();
+                if (t < best) {
+                    best = t;
+                    bscr = h.search_screen;
+                }
+            }
+            h.search_screen = bscr;
+            // 4. launch shape: waves per block x resident blocks per CU
+            unsigned bw = h.search_wpb, bb = h.search_bpc;
+            for (unsigned wpb : {1u, 2u, 4u}) {
+                for (unsigned bpc : {4u, 8u, 16u, 32u}) {
+                    if (wpb * bpc > 32) continue;  // at most 32 resident waves per CU
+                    h.search_wpb = wpb;
+                    h.search_bpc = bpc;
+                    float const t = run();
+                    if (t < best) {
+                        best = t;
+                        bw = wpb;
+                        bb = bpc;
+                    }
+                }
+            }
+            h.search_wpb = bw;
+            h.search_bpc = bb;
+            if (chosen) {
+                chosen[0] = bw;
+                chosen[1] = bb;
+                chosen[2] = h.search_visited == 1 ? 0u : (h.search_vcap ? h.search_vcap : h.last_vcap);
+                chosen[3] = unsigned(h.search_adjacency);
+                chosen[4] = unsigned(h.search_screen);
+            }
+        } catch (...) {
+            h.search_visited = old_visited;
+            h.search_adjacency = old_adj;
+            h.search_screen = old_screen;
+            h.search_vcap = old_vcap;
+            h.search_wpb = old_wpb;
+            h.search_bpc = old_bpc;
+            throw;
+        }
+    });
+}
+
 int kg_mlp_get_normalization(kg_handle *k, float *mu, float *inv_sigma) {
     return guard(k, [&](Handle &h) {
         if (!h.mlpH) throw kga::runtime_error("no projector (kg_mlp_init)");

@@ -416,6 +416,45 @@ def batched_workload(args):
     E, m = args.entry_sample, 9
     h.call("kg_set_entry_sample", E, 1998)
     sp = SearchParams(10, 0, 100, 10, 1, 1e30, 1998, m)
+    groups = sorted(set(zip(Ks.tolist(), Ps.tolist())))
+    gq = []
+    for (k_, p_) in groups:
+        sel = np.nonzero((Ks == k_) & (Ps == p_))[0]
+        gq.append((k_, p_, devmem.DeviceArray.from_host(np.ascontiguousarray(Q[sel])), len(sel)))
+
+    def grouped():
+        # the same work without the mixed launch: one launch per distinct
+        # (K, P), each group's queries contiguous in HBM, entry selection per group
+        gwall = []
+        for s in range(args.warmup + 1):
+            devmem.synchronize()
+            ts = time.perf_counter()
+            for (k_, p_, dq, cnt) in gq:
+                h.call("kg_select_entries_device", dq.ptr, cnt, D, m, dI.ptr, W)
+                kk = np.full(cnt, k_, np.uint32)  # per-query K / P keep init = 9 usable for K = 1
+                pp = np.full(cnt, p_, np.uint32)
+                h.call("kg_search_device", dq.ptr, cnt, D, sp, ptr(kk), ptr(pp), dI.ptr, dD.ptr, W, dC.ptr, None,
+                       None)
+            devmem.synchronize()
+            gwall.append(time.perf_counter() - ts)
+        return nq / gwall[-1]
+
+    # the plain FP32 GPU form: the walk's default settings (FP32 rows, no
+    # screen), one launch per (K, P) group
+    plain_qps = grouped()
+    # autotuned walk settings (visited set, adjacency rows, BF16 screen,
+    # launch shape; none changes a result), tuned on the first 20k queries of
+    # the mixed batch with their entry points
+    h.call("kg_select_entries_device", dQ.ptr, nq, D, m, dI.ptr, W)
+    nt = min(20000, nq)
+    chosen = (_lib.ctypes.c_uint * 5)()
+    tt = time.perf_counter()
+    h.call("kg_autotune_walk", dQ.ptr, nt, D, sp, ptr(np.ascontiguousarray(Ks[:nt])),
+           ptr(np.ascontiguousarray(Ps[:nt])), dI.ptr, W, chosen)
+    tune_s = time.perf_counter() - tt
+    tuned = {"waves_per_block": chosen[0], "blocks_per_cu": chosen[1], "visited_entries": chosen[2],
+             "adjacency": "csr" if chosen[3] else "padded", "bf16_screen": bool(chosen[4])}
+    log("autotuned in %.1fs: %s" % (tune_s, tuned))
     walls, ems_l = [], []
     for s in range(args.warmup + args.steps):
         devmem.synchronize()
@@ -433,34 +472,16 @@ def batched_workload(args):
     comps, scans, dists = dCo.to_host(), dSc.to_host(), dD.to_host()
     wall = float(np.median(walls))
     seeds = np.minimum(m, Ks.astype(np.int64) + Ps - 1).astype(np.float64)
-    byt = float((rb * D * (seeds + comps) + 4 * scans + 4 * seeds).sum())
-    # the same work without the mixed launch: one launch per distinct (K, P)
-    # (each group's queries contiguous in HBM, entry selection per group), and
-    # one launch per query on a 1,000-query sample (the reference's per-call
-    # search interface, here through the C ABI from Python)
-    groups = sorted(set(zip(Ks.tolist(), Ps.tolist())))
-    gq = []
-    for (k_, p_) in groups:
-        sel = np.nonzero((Ks == k_) & (Ps == p_))[0]
-        gq.append((k_, p_, devmem.DeviceArray.from_host(np.ascontiguousarray(Q[sel])), len(sel)))
-    gwall = []
-    for s in range(args.warmup + 1):
-        devmem.synchronize()
-        ts = time.perf_counter()
-        for (k_, p_, dq, cnt) in gq:
-            h.call("kg_select_entries_device", dq.ptr, cnt, D, m, dI.ptr, W)
-            spg = SearchParams(k_, 0, p_, 10, 1, 1e30, 1998, min(m, k_ + p_ - 1) if k_ > 1 else 0)
-            if k_ > m:
-                h.call("kg_search_device", dq.ptr, cnt, D, spg, None, None, dI.ptr, dD.ptr, W, dC.ptr, None, None)
-            else:  # one-K calls keep the reference's rule init < K: pass per-query K/P to allow it
-                kk = np.full(cnt, k_, np.uint32)
-                pp = np.full(cnt, p_, np.uint32)
-                spg = SearchParams(k_, 0, p_, 10, 1, 1e30, 1998, m)
-                h.call("kg_search_device", dq.ptr, cnt, D, spg, ptr(kk), ptr(pp), dI.ptr, dD.ptr, W, dC.ptr, None,
-                       None)
-        devmem.synchronize()
-        gwall.append(time.perf_counter() - ts)
-    grouped_qps = nq / gwall[-1]
+    act, nres = _lib.ctypes.c_int(), _lib.ctypes.c_ulonglong()
+    h.call("kg_search_screen", _lib.ctypes.byref(act), _lib.ctypes.byref(nres))
+    if act.value:  # BF16 rows for every screened candidate, FP32 rows for the seeds and the re-scored ones
+        byt = float((2 * D * comps + 4 * scans + 4 * seeds + 4 * D * seeds).sum()) + 4.0 * D * nres.value
+    else:
+        byt = float((rb * D * (seeds + comps) + 4 * scans + 4 * seeds).sum())
+    # the tuned settings with one launch per (K, P) group, and one launch per
+    # query on a 1,000-query sample (the reference's per-call search
+    # interface, here through the C ABI from Python)
+    grouped_qps = grouped()
     npq = min(1000, nq)
     devmem.synchronize()
     ts = time.perf_counter()
@@ -473,8 +494,8 @@ def batched_workload(args):
                None, None)
     devmem.synchronize()
     perq_qps = npq / (time.perf_counter() - ts)
-    log("mixed launch %.0f QPS; one launch per (K, P): %.0f QPS (%d launches); one launch per query: %.0f QPS" % (
-        nq / wall, grouped_qps, len(groups), perq_qps))
+    log("mixed launch %.0f QPS; one launch per (K, P): %.0f QPS tuned, %.0f QPS plain (%d launches); "
+        "one launch per query: %.0f QPS" % (nq / wall, grouped_qps, plain_qps, len(groups), perq_qps))
     # recall on a query sample against the exact scan
     ns = min(1000, nq)
     gt_i = np.zeros((ns, 100), np.uint32)
@@ -502,8 +523,12 @@ def batched_workload(args):
         "latency_us": {"p50": round(float(np.percentile(lat, 50)), 1),
                        "p99": round(float(np.percentile(lat, 99)), 1)},
         "recall_by_K_sample": rec, "kernel_ms": round(kms, 3),
-        "plain_forms_qps": {"one_launch_per_K_P": round(grouped_qps, 1), "launches": len(groups),
+        "screen_rescored_frac": round(nres.value / max(float(comps.sum()), 1.0), 4) if act.value else None,
+        "autotuned": tuned, "autotune_s": round(tune_s, 2),
+        "plain_forms_qps": {"plain_fp32_one_launch_per_K_P": round(plain_qps, 1),
+                            "tuned_one_launch_per_K_P": round(grouped_qps, 1), "launches": len(groups),
                             "one_launch_per_query": round(perq_qps, 1), "per_query_sample": npq},
+        "speedup_vs_plain_fp32_form": round((nq / wall) / plain_qps, 3),
         "speedup_vs_one_launch_per_K_P": round((nq / wall) / grouped_qps, 3),
         "speedup_vs_one_launch_per_query": round((nq / wall) / perq_qps, 2),
         "roofline": {"kernel": "search_kernel", "bound": "hbm", "achieved": round(achieved, 1),

@@ -308,10 +308,15 @@ __device__ __forceinline__ float2 ws_mfma(kg_search_args const &a, uint32_t id, 
 
 // Distances of ids[0..cnt) into out[0..cnt) for a full pool whose last
 // distance is thr: +inf for every candidate the screen rules out, the exact
-// FP32 distance for the rest (nres counts them).
+// FP32 distance for the rest (nres counts them).  spec: candidate `lane`'s
+// visited test is still in flight (vold = the bitset word before its
+// atomicOr, vbit its bit, vcand whether it was tested); only the ones it
+// finds fresh are re-scored, read after the screen's rows (the others' out
+// is left as it is).
 template <int METRIC>
 __device__ void batch_screen(kg_search_args const &a, WaveLDS const &w, uint32_t const *ids, float *out, unsigned cnt,
-                             float qn, float qrn, float qsq, float thr, unsigned &nres) {
+                             float qn, float qrn, float qsq, float thr, unsigned &nres, bool spec = false,
+                             bool vcand = false, uint32_t vold = 0u, uint32_t vbit = 0u) {
     unsigned const lane = lane_id();
     uint64_t const lt_mask = (1ull << lane) - 1;
     unsigned const n = lane & 15;
@@ -322,7 +327,8 @@ __device__ void batch_screen(kg_search_args const &a, WaveLDS const &w, uint32_t
         bool const ok = c < cnt;
         uint32_t const id = ok ? (ids[c] & IDMASK) : 0u;
         float2 const gn = ws_mfma<METRIC>(a, id, ok, w.qh, w.ql);
-        bool const live = own && ok;
+        uint64_t const fm = spec ? ballot(vcand && !(vold & vbit)) : ~0ull;
+        bool const live = own && ok && ((fm >> (c & 63u)) & 1u);
         float const na = (METRIC != METRIC_L2SQR && live) ? a.norms[id] : 0.f;
         float2 const de = ws_finish<METRIC>(a, gn.x, gn.y, na, qn, qrn, qsq);
         bool const keep = live && !(de.x - de.y >= thr);  // NaN: re-score
@@ -833,6 +839,42 @@ search_kernel(kg_search_args a) {
                             uint32_t v = __shfl(id, j);
                             dup |= (j < lane) && (v == id);
                         }
+                    }
+                    if (glob && cnt <= 16) {
+                        // the visited bits and the distances at once: the
+                        // bitset atomics and the row gathers are in flight
+                        // together, and the distances of ids found visited
+                        // are dropped (16 rows take one pass either way)
+                        bool const cand = lane < cnt && !dup;
+                        uint32_t const bit = 1u << (id & 31);
+                        uint32_t old = 0u;
+                        if (cand) old = atomicOr(&vis[id >> 5], bit);
+                        if (lane < cnt) w.fid[lane] = id;
+                        wave_sync();
+                        if (SCREEN && L == qcap - 1)
+                            batch_screen<METRIC>(a, w, w.fid, w.fdist, cnt, qn, qrn, qsq, w.pdist[L - 1], n_res, true,
+                                                 cand, old, bit);
+                        else
+                            batch_dist<METRIC, RowT>(a, w.fid, w.fdist, cnt, w.q, qn, w.q8, qv);
+                        bool const fresh = cand && !(old & bit);
+                        uint64_t const b = ballot(fresh);
+                        unsigned const nf = popc64(b);
+                        unsigned const pos = popc64(b & lt_mask);
+                        float const fd = fresh ? w.fdist[lane] : 0.f;
+                        wave_sync();
+                        if (fresh) {
+                            w.fid[pos] = id;
+                            w.fdist[pos] = fd;
+                            if (ntouch + pos < a.touch_cap) touched[ntouch + pos] = id;
+                        }
+                        ntouch += nf;
+                        wave_sync();
+                        if (nf == 0) continue;
+                        uint32_t cm = VT_M_UNKNOWN, cb = 0;
+                        if (lane < nf && !a.gpad) cm = actual_M(a, w.fid[lane], &cb);
+                        n_comps += nf;
+                        batch_insert(w, nf, cm, cb, qcap, L, k);
+                        continue;
                     }
                     bool fresh = false;
                     unsigned st = VT_PRESENT;

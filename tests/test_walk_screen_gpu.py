@@ -36,13 +36,19 @@ def _gpu(X, idx, metric):
     return g
 
 
-@pytest.mark.parametrize("kind,metric,d,P,S,T", [
-    ("uniform", EUC, 16, 100, 10, 1), ("uniform", L2, 37, 50, 10, 1), ("angular", ANG, 96, 100, 10, 1),
-    ("angular", ANG, 96, 300, 64, 1), ("aniso", EUC, 100, 40, 30, 3), ("aniso", L2, 128, 200, 70, 1)])
-def test_screened_walk_matches_oracle(kind, metric, d, P, S, T):
+@pytest.mark.parametrize("kind,metric,d,P,S,T,bitset", [
+    ("uniform", EUC, 16, 100, 10, 1, False), ("uniform", L2, 37, 50, 10, 1, False),
+    ("angular", ANG, 96, 100, 10, 1, False), ("angular", ANG, 96, 300, 64, 1, False),
+    ("aniso", EUC, 100, 40, 30, 3, False), ("aniso", L2, 128, 200, 70, 1, False),
+    ("angular", ANG, 96, 100, 10, 1, True), ("aniso", L2, 128, 60, 16, 2, True), ("aniso", EUC, 100, 40, 17, 1, True)])
+def test_screened_walk_matches_oracle(kind, metric, d, P, S, T, bitset):
+    # bitset: visited ids in the HBM bitset only, where batches of up to 16
+    # ids test their bits and screen their rows at once
     X, Q, idx = _index(kind, metric, d)
     g = _gpu(X, idx, metric)
     g.set_search_screen(True)
+    if bitset:
+        g._h.call("kg_set_search_visited", 1)
     gi, gd = g.search(Q, K=10, P=P, S=S, T=T, withDistance=True, stats=True)
     rep = g.search_screen_report()
     assert rep["active"]

@@ -162,7 +162,7 @@ __global__ void __launch_bounds__(256) norms_kernel(float const *X, unsigned N, 
     unsigned const t = lane & 3;
     if ((blockIdx.x * blockDim.x + (threadIdx.x & ~3u)) / 4 < N) {
         float const *r = X + size_t(row) * Dp;
-        float n2 = quad_metric<METRIC_EUCLIDEAN>(r, r, D, t);
+        float n2 = quad_metric<METRIC_EUCLIDEAN, float, true>(r, r, D, t);
         if (t == 0) out[row] = finish_norm<METRIC>(n2);
     }
 }

@@ -103,7 +103,14 @@ __device__ __forceinline__ float load_elem(uint16_t const *r, unsigned i) { retu
 // The row may be BF16 bits (RowT = uint16_t, 8-byte aligned rows): its
 // values widen exactly, so the result is the FP32 row's when every value of
 // the row is representable in BF16.
-template <int METRIC, typename RowT = float>
+//
+// Loads: lane t takes chunks t, t + 4, ... in groups of G; every load of a
+// group is issued before its first chunk is accumulated (the last group
+// predicated: a chunk count that is no multiple of 4 G costs no extra round
+// trips), and the last D - D16 < 16 values load as whole chunks at once.  BG:
+// b is in global memory too (a row), so its chunks load with the row's.  The
+// accumulation order is the same whatever the grouping.
+template <int METRIC, typename RowT = float, bool BG = false, int G = QM_GROUP>
 __device__ __forceinline__ float quad_metric(RowT const *__restrict__ a, float const *__restrict__ b, unsigned D,
                                              unsigned t) {
     unsigned const nch = (D & ~15u) >> 2;
@@ -123,26 +130,65 @@ __device__ __forceinline__ float quad_metric(RowT const *__restrict__ a, float c
             p.w = __builtin_fmaf(x.w, y.w, p.w);
         }
     };
-    // groups of QM_GROUP chunks per lane: all of a group's row loads are in
-    // flight together (a row of 128 floats is one group), then accumulated in
-    // the same chunk order as one chunk at a time
+    float4 const z4 = make_float4(0.f, 0.f, 0.f, 0.f);
     unsigned k = t;
-    for (; k + 4 * (QM_GROUP - 1) < nch; k += 4 * QM_GROUP) {
-        float4 x[QM_GROUP];
+    for (; k + 4 * (G - 1) < nch; k += 4 * G) {  // whole groups
+        float4 x[G], y[G];
 #pragma unroll
-        for (int g = 0; g < QM_GROUP; ++g) x[g] = load_chunk(a, k + 4 * g);
+        for (int g = 0; g < G; ++g) {
+            x[g] = load_chunk(a, k + 4 * g);
+            if (BG) y[g] = b4[k + 4 * g];
+        }
 #pragma unroll
-        for (int g = 0; g < QM_GROUP; ++g) step(x[g], b4[k + 4 * g]);
+        for (int g = 0; g < G; ++g) step(x[g], BG ? y[g] : b4[k + 4 * g]);
     }
-    for (; k < nch; k += 4) step(load_chunk(a, k), b4[k]);
+    if (k < nch) {  // the last, partial group
+        float4 x[G], y[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            bool const ok = k + 4 * g < nch;
+            x[g] = ok ? load_chunk(a, k + 4 * g) : z4;
+            if (BG) y[g] = ok ? b4[k + 4 * g] : z4;
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+            if (k + 4 * g < nch) step(x[g], BG ? y[g] : b4[k + 4 * g]);
+    }
     float acc = quad_reduce(p);
-    for (unsigned i = D & ~15u; i < D; ++i) {
-        float const ai = load_elem(a, i);
-        if (METRIC == METRIC_L2SQR) {
-            float d = ai - b[i];
-            acc = __builtin_fmaf(d, d, acc);
-        } else {
-            acc = __builtin_fmaf(ai, b[i], acc);
+    unsigned const D16 = D & ~15u;
+    if (__builtin_expect(D16 < D, 0)) {
+        // whole chunks below D load as chunks, the values of a last partial
+        // one one by one (nothing at or past D is read: b may be an unpadded
+        // row); all of them before the first is used
+        float4 xt[4], yt[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            unsigned const ch = (D16 >> 2) + c;
+            if (4 * ch + 4 <= D) {
+                xt[c] = load_chunk(a, ch);
+                yt[c] = b4[ch];
+            } else {
+                unsigned const i = 4 * ch;
+                xt[c] = make_float4(i < D ? load_elem(a, i) : 0.f, i + 1 < D ? load_elem(a, i + 1) : 0.f,
+                                    i + 2 < D ? load_elem(a, i + 2) : 0.f, 0.f);
+                yt[c] = make_float4(i < D ? b[i] : 0.f, i + 1 < D ? b[i + 1] : 0.f, i + 2 < D ? b[i + 2] : 0.f, 0.f);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            float const xv[4] = {xt[c].x, xt[c].y, xt[c].z, xt[c].w};
+            float const yv[4] = {yt[c].x, yt[c].y, yt[c].z, yt[c].w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (D16 + 4 * c + e < D) {
+                    if (METRIC == METRIC_L2SQR) {
+                        float const d = xv[e] - yv[e];
+                        acc = __builtin_fmaf(d, d, acc);
+                    } else {
+                        acc = __builtin_fmaf(xv[e], yv[e], acc);
+                    }
+                }
+            }
         }
     }
     return acc;
@@ -178,8 +224,9 @@ __device__ __forceinline__ float2 quad_metric2_bf16(uint16_t const *__restrict__
             p.w = __builtin_fmaf(x.w, y.w, p.w);
         }
     };
+    uint2 const z2 = make_uint2(0u, 0u);
     unsigned k = t;
-    for (; k + 4 * (QM_GROUP - 1) < nch; k += 4 * QM_GROUP) {
+    for (; k + 4 * (QM_GROUP - 1) < nch; k += 4 * QM_GROUP) {  // whole groups
         uint2 x0[QM_GROUP], x1[QM_GROUP];
 #pragma unroll
         for (int g = 0; g < QM_GROUP; ++g) x0[g] = r0[k + 4 * g];
@@ -192,21 +239,49 @@ __device__ __forceinline__ float2 quad_metric2_bf16(uint16_t const *__restrict__
             step(p1, widen_bf16x4(x1[g]), y);
         }
     }
-    for (; k < nch; k += 4) {
-        float4 const y = b4[k];
-        step(p0, widen_bf16x4(r0[k]), y);
-        step(p1, widen_bf16x4(r1[k]), y);
+    if (__builtin_expect(k < nch, 0)) {  // the last, partial group
+        uint2 x0[QM_GROUP], x1[QM_GROUP];
+#pragma unroll
+        for (int g = 0; g < QM_GROUP; ++g) x0[g] = k + 4 * g < nch ? r0[k + 4 * g] : z2;
+#pragma unroll
+        for (int g = 0; g < QM_GROUP; ++g) x1[g] = k + 4 * g < nch ? r1[k + 4 * g] : z2;
+#pragma unroll
+        for (int g = 0; g < QM_GROUP; ++g) {
+            if (k + 4 * g < nch) {
+                float4 const y = b4[k + 4 * g];
+                step(p0, widen_bf16x4(x0[g]), y);
+                step(p1, widen_bf16x4(x1[g]), y);
+            }
+        }
     }
     float acc0 = quad_reduce(p0), acc1 = quad_reduce(p1);
-    for (unsigned i = D & ~15u; i < D; ++i) {
-        float const e0 = load_elem(a0, i), e1 = load_elem(a1, i);
-        if (METRIC == METRIC_L2SQR) {
-            float const d0 = e0 - b[i], d1 = e1 - b[i];
-            acc0 = __builtin_fmaf(d0, d0, acc0);
-            acc1 = __builtin_fmaf(d1, d1, acc1);
-        } else {
-            acc0 = __builtin_fmaf(e0, b[i], acc0);
-            acc1 = __builtin_fmaf(e1, b[i], acc1);
+    unsigned const D16 = D & ~15u;
+    if (__builtin_expect(D16 < D, 0)) {  // the last D - D16 < 16 values, their chunks loaded at once
+        uint2 t0[4], t1[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            unsigned const ch = (D16 >> 2) + c;
+            t0[c] = 4 * ch < D ? r0[ch] : z2;
+            t1[c] = 4 * ch < D ? r1[ch] : z2;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            float4 const w0 = widen_bf16x4(t0[c]), w1 = widen_bf16x4(t1[c]);
+            float const v0[4] = {w0.x, w0.y, w0.z, w0.w}, v1[4] = {w1.x, w1.y, w1.z, w1.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                unsigned const i = D16 + 4 * c + e;
+                if (i < D) {
+                    if (METRIC == METRIC_L2SQR) {
+                        float const d0 = v0[e] - b[i], d1 = v1[e] - b[i];
+                        acc0 = __builtin_fmaf(d0, d0, acc0);
+                        acc1 = __builtin_fmaf(d1, d1, acc1);
+                    } else {
+                        acc0 = __builtin_fmaf(v0[e], b[i], acc0);
+                        acc1 = __builtin_fmaf(v1[e], b[i], acc1);
+                    }
+                }
+            }
         }
     }
     return make_float2(acc0, acc1);
@@ -230,8 +305,10 @@ __device__ __forceinline__ void quad_dot2_u8(uint8_t const *__restrict__ a0, uin
     uint32_t const *r0 = reinterpret_cast<uint32_t const *>(a0);
     uint32_t const *r1 = reinterpret_cast<uint32_t const *>(a1);
     uint32_t s0 = 0, s1 = 0, m0 = 0, m1 = 0;
+    // groups of QM_GROUP words per lane, every load of a group issued first
+    // (integer sums: the grouping cannot change them)
     unsigned k = t;
-    for (; k + 4 * (QM_GROUP - 1) < nw; k += 4 * QM_GROUP) {
+    for (; k + 4 * (QM_GROUP - 1) < nw; k += 4 * QM_GROUP) {  // whole groups
         uint32_t x0[QM_GROUP], x1[QM_GROUP];
 #pragma unroll
         for (int g = 0; g < QM_GROUP; ++g) x0[g] = r0[k + 4 * g];
@@ -246,12 +323,20 @@ __device__ __forceinline__ void quad_dot2_u8(uint8_t const *__restrict__ a0, uin
             m1 = __builtin_amdgcn_udot4(x1[g], x1[g], m1, false);
         }
     }
-    for (; k < nw; k += 4) {
-        uint32_t const y = q8[k], x = r0[k], z = r1[k];
-        s0 = __builtin_amdgcn_udot4(x, y, s0, false);
-        s1 = __builtin_amdgcn_udot4(z, y, s1, false);
-        m0 = __builtin_amdgcn_udot4(x, x, m0, false);
-        m1 = __builtin_amdgcn_udot4(z, z, m1, false);
+    if (__builtin_expect(k < nw, 0)) {  // the last, partial group
+        uint32_t x0[QM_GROUP], x1[QM_GROUP];
+#pragma unroll
+        for (int g = 0; g < QM_GROUP; ++g) x0[g] = k + 4 * g < nw ? r0[k + 4 * g] : 0u;
+#pragma unroll
+        for (int g = 0; g < QM_GROUP; ++g) x1[g] = k + 4 * g < nw ? r1[k + 4 * g] : 0u;
+#pragma unroll
+        for (int g = 0; g < QM_GROUP; ++g) {
+            uint32_t const y = k + 4 * g < nw ? q8[k + 4 * g] : 0u;
+            s0 = __builtin_amdgcn_udot4(x0[g], y, s0, false);
+            s1 = __builtin_amdgcn_udot4(x1[g], y, s1, false);
+            m0 = __builtin_amdgcn_udot4(x0[g], x0[g], m0, false);
+            m1 = __builtin_amdgcn_udot4(x1[g], x1[g], m1, false);
+        }
     }
     d0 = uint32_t(quad_isum(int(s0)));
     d1 = uint32_t(quad_isum(int(s1)));
@@ -289,7 +374,7 @@ __device__ __forceinline__ float2 quad_metric2_u8(uint8_t const *__restrict__ a0
         }
     };
     unsigned k = t;
-    for (; k + 4 * (QM_GROUP - 1) < nch; k += 4 * QM_GROUP) {
+    for (; k + 4 * (QM_GROUP - 1) < nch; k += 4 * QM_GROUP) {  // whole groups
         uint32_t x0[QM_GROUP], x1[QM_GROUP];
 #pragma unroll
         for (int g = 0; g < QM_GROUP; ++g) x0[g] = r0[k + 4 * g];
@@ -302,21 +387,49 @@ __device__ __forceinline__ float2 quad_metric2_u8(uint8_t const *__restrict__ a0
             step(p1, widen_u8x4(x1[g]), y);
         }
     }
-    for (; k < nch; k += 4) {
-        float4 const y = b4[k];
-        step(p0, widen_u8x4(r0[k]), y);
-        step(p1, widen_u8x4(r1[k]), y);
+    if (__builtin_expect(k < nch, 0)) {  // the last, partial group
+        uint32_t x0[QM_GROUP], x1[QM_GROUP];
+#pragma unroll
+        for (int g = 0; g < QM_GROUP; ++g) x0[g] = k + 4 * g < nch ? r0[k + 4 * g] : 0u;
+#pragma unroll
+        for (int g = 0; g < QM_GROUP; ++g) x1[g] = k + 4 * g < nch ? r1[k + 4 * g] : 0u;
+#pragma unroll
+        for (int g = 0; g < QM_GROUP; ++g) {
+            if (k + 4 * g < nch) {
+                float4 const y = b4[k + 4 * g];
+                step(p0, widen_u8x4(x0[g]), y);
+                step(p1, widen_u8x4(x1[g]), y);
+            }
+        }
     }
     float acc0 = quad_reduce(p0), acc1 = quad_reduce(p1);
-    for (unsigned i = D & ~15u; i < D; ++i) {
-        float const e0 = load_elem(a0, i), e1 = load_elem(a1, i);
-        if (METRIC == METRIC_L2SQR) {
-            float const d0 = e0 - b[i], d1 = e1 - b[i];
-            acc0 = __builtin_fmaf(d0, d0, acc0);
-            acc1 = __builtin_fmaf(d1, d1, acc1);
-        } else {
-            acc0 = __builtin_fmaf(e0, b[i], acc0);
-            acc1 = __builtin_fmaf(e1, b[i], acc1);
+    unsigned const D16 = D & ~15u;
+    if (__builtin_expect(D16 < D, 0)) {  // the last D - D16 < 16 values, their words loaded at once
+        uint32_t t0[4], t1[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            unsigned const ch = (D16 >> 2) + c;
+            t0[c] = 4 * ch < D ? r0[ch] : 0u;
+            t1[c] = 4 * ch < D ? r1[ch] : 0u;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            float4 const w0 = widen_u8x4(t0[c]), w1 = widen_u8x4(t1[c]);
+            float const v0[4] = {w0.x, w0.y, w0.z, w0.w}, v1[4] = {w1.x, w1.y, w1.z, w1.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                unsigned const i = D16 + 4 * c + e;
+                if (i < D) {
+                    if (METRIC == METRIC_L2SQR) {
+                        float const d0 = v0[e] - b[i], d1 = v1[e] - b[i];
+                        acc0 = __builtin_fmaf(d0, d0, acc0);
+                        acc1 = __builtin_fmaf(d1, d1, acc1);
+                    } else {
+                        acc0 = __builtin_fmaf(v0[e], b[i], acc0);
+                        acc1 = __builtin_fmaf(v1[e], b[i], acc1);
+                    }
+                }
+            }
         }
     }
     return make_float2(acc0, acc1);

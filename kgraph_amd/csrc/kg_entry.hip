@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(256) rescore_kernel(kg_rescore_args a) {
     float const qn = (METRIC == METRIC_L2SQR) ? 0.f : a.qnorms[q];
     if (grp < a.s) {
         uint32_t id = a.short_ids[size_t(q) * a.s + grp];
-        float acc = quad_metric<METRIC>(a.X + size_t(id) * a.Dp, qrow, a.D, t);
+        float acc = quad_metric<METRIC, float, true>(a.X + size_t(id) * a.Dp, qrow, a.D, t);
         float na = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[id];
         float d = finish_metric<METRIC>(acc, na, qn);
         if (t == 0) keys[w][grp] = (uint64_t(ord_f32(d)) << 32) | id;
@@ -215,7 +215,7 @@ __global__ void __launch_bounds__(256) qnorm_check_kernel(float const *Q, unsign
     float mx = 0.f;
     if (row < n) {  // quad-uniform: the quad reduce inside quad_metric sees all four lanes
         float const *r = Q + size_t(row) * stride;
-        float const n2 = quad_metric<METRIC_EUCLIDEAN>(r, r, D, t);
+        float const n2 = quad_metric<METRIC_EUCLIDEAN, float, true>(r, r, D, t);
         if (t == 0) qsq[row] = finish_norm<METRIC_EUCLIDEAN>(n2);
         for (unsigned c = t; c < D; c += 4) {
             float const v = r[c];

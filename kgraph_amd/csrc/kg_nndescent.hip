@@ -105,7 +105,7 @@ __device__ unsigned gen_random_wave(uint64_t s0, uint32_t *raw, uint32_t *srt, u
 template <int METRIC>
 __device__ __forceinline__ float pair_dist(NndArgs const &a, uint32_t i, float const *rj, float nj, unsigned t) {
     float const *ri = a.X + size_t(i) * a.Dp;
-    float acc = quad_metric<METRIC>(ri, rj, a.D, t);
+    float acc = quad_metric<METRIC, float, true>(ri, rj, a.D, t);
     float ni = (METRIC == METRIC_L2SQR) ? 0.f : a.norms[i];
     return finish_metric<METRIC>(acc, ni, nj);
 }
@@ -391,7 +391,7 @@ __device__ void mj_rescore(NndArgs const &a, LDS &L, unsigned rqn, unsigned &fil
         uint32_t const pr = act ? L.rq[e] : 0u;
         unsigned const ia = pr & 0xffffu, io = pr >> 16;
         uint32_t const av = L.ids[ia], o = L.ids[io];
-        float const acc = quad_metric<METRIC>(a.X + size_t(av) * a.Dp, a.X + size_t(o) * a.Dp, a.D, t);
+        float const acc = quad_metric<METRIC, float, true, 4>(a.X + size_t(av) * a.Dp, a.X + size_t(o) * a.Dp, a.D, t);
         float4 const ma = a.meta[av], mo = a.meta[o];
         float const na = (METRIC == METRIC_L2SQR) ? 0.f : ma.z;
         float const no = (METRIC == METRIC_L2SQR) ? 0.f : mo.z;
@@ -701,7 +701,7 @@ __global__ void __launch_bounds__(256) sqn_kernel(float const *X, unsigned N, un
     unsigned const row = (blockIdx.x * blockDim.x + threadIdx.x) >> 2;
     bool const ok = row < N;
     float const *r = X + size_t(ok ? row : 0u) * Dp;
-    float const n2 = quad_metric<METRIC_EUCLIDEAN>(r, r, D, t);
+    float const n2 = quad_metric<METRIC_EUCLIDEAN, float, true>(r, r, D, t);
     bool frac = false;
     float mx = 0.f, lo = __builtin_huge_valf(), hi = -__builtin_huge_valf();
     if (ok)

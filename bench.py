@@ -617,22 +617,36 @@ def precision_workload(args):
         errors["stored_" + form] = {"pairs": int(len(serr)), "max_abs": float(serr.max()),
                                     "mean_abs": float(serr.mean())}
         # search on this index from selected entry points (BF16 screen of the
-        # sample on the BF16 index, exact FP32 scores on the plain one)
+        # sample on the BF16 index, exact FP32 scores on the plain one); on the
+        # BF16 index the walk also runs with its BF16 screen (same results)
         h.call("kg_set_entry_sample", args.entry_sample, 1998)
         sweep = []
-        for P in (10, 20, 50):
+        for P in (10, 20, 50, 100, 200):
             sp = SearchParams(K, 0, P, 32, 1, 1e30, 1998, m)
-            for w in range(args.warmup + 1):
-                devmem.synchronize()
-                ts = time.perf_counter()
-                h.call("kg_select_entries_device", dQ.ptr, nq, D, m, dI.ptr, K)
-                h.call("kg_search_device", dQ.ptr, nq, D, sp, None, None, dI.ptr, dD.ptr, K, dC.ptr, None, None)
-                devmem.synchronize()
-                wall = time.perf_counter() - ts
-                ems = ctypes_float(h, 7)
-            rec = recall_at_k(dD.to_host(), gt_d, K)
-            sweep.append({"P": P, "S": 32, "qps": round(nq / wall, 1), "recall": round(rec, 4),
-                          "entry_ms": round(ems, 3), "walk_ms": round(ctypes_float(h, 0), 3)})
+            ref = None
+            for scr in ((0, 1) if form == "bf16" else (0,)):
+                h.call("kg_set_search_screen", scr)
+                for w in range(args.warmup + 1):
+                    devmem.synchronize()
+                    ts = time.perf_counter()
+                    h.call("kg_select_entries_device", dQ.ptr, nq, D, m, dI.ptr, K)
+                    h.call("kg_search_device", dQ.ptr, nq, D, sp, None, None, dI.ptr, dD.ptr, K, dC.ptr, None, None)
+                    devmem.synchronize()
+                    wall = time.perf_counter() - ts
+                    ems = ctypes_float(h, 7)
+                got = (dI.to_host(), dD.to_host())
+                rec = recall_at_k(got[1], gt_d, K)
+                row = {"P": P, "S": 32, "walk_screen": bool(scr), "qps": round(nq / wall, 1), "recall": round(rec, 4),
+                       "entry_ms": round(ems, 3), "walk_ms": round(ctypes_float(h, 0), 3)}
+                if scr:
+                    act, nres = _lib.ctypes.c_int(), _lib.ctypes.c_ulonglong()
+                    h.call("kg_search_screen", _lib.ctypes.byref(act), _lib.ctypes.byref(nres))
+                    row["identical_to_unscreened"] = bool(np.array_equal(got[0], ref[0]) and
+                                                          np.array_equal(got[1], ref[1]))
+                    row["rescored"] = int(nres.value)
+                ref = got
+                sweep.append(row)
+        h.call("kg_set_search_screen", 0)
         forms[form] = {"build_s": round(bs, 4), "iterations": int(info.iterations),
                        "control_recall": round(float(info.recall), 4), "graph_recall@10": round(grec, 4),
                        "join_form": ["fp32", "bf16-screen", "bf16-exact", "fp32-mfma-screen"][int(cnt[4])],
@@ -654,9 +668,23 @@ def precision_workload(args):
                                       "mean_abs": float(err.mean()), "bound": float(eb.max()),
                                       "max_err_over_bound": float((err / eb).max()),
                                       "within_bound": float((err <= eb).mean())}
+    # the walk's screen: its BF16 distance (row in BF16, query as BF16 high and
+    # low halves) and bound for 200 queries x (64 random nodes + their 10
+    # nearest), against FP64
+    nqs = min(200, nq)
+    wid = np.ascontiguousarray(np.concatenate([rs.randint(0, n, (nqs, 64)), gt_i[:nqs]], 1).astype(np.uint32))
+    wd = np.zeros(wid.shape, np.float32)
+    we = np.zeros(wid.shape, np.float32)
+    h.call("kg_search_screen_dists", ptr(Q[:nqs]), nqs, D, ptr(wid), wid.shape[1], ptr(wd), ptr(we))
+    werr = np.abs(wd.astype(np.float64) - ang64(np.repeat(Q[:nqs], wid.shape[1], 0), X[wid.reshape(-1)]).reshape(
+        wid.shape))
+    errors["walk_screened"] = {"pairs": int(wid.size), "max_abs": float(werr.max()), "mean_abs": float(werr.mean()),
+                               "bound": float(we.max()), "max_err_over_bound": float((werr / we).max()),
+                               "within_bound": float((werr <= we).mean())}
     log("errors vs FP64: %s" % errors)
     best = [r for r in forms["bf16"]["search"] if r["recall"] >= 0.95]
-    pick = max(best, key=lambda r: r["qps"]) if best else max(forms["bf16"]["search"], key=lambda r: r["recall"])
+    pick = max(best, key=lambda r: r["qps"]) if best else max(forms["bf16"]["search"],
+                                                               key=lambda r: (r["recall"], r["qps"]))
     out = {
         "metric": "build s + search QPS at recall@10>=0.95: BF16-screened vs plain FP32 GPU build (angular)",
         "value": pick["qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -672,7 +700,8 @@ def precision_workload(args):
                                                     forms["fp32"]["graph_recall@10"], 4),
         "error_vs_fp64": errors,
         "notes": "iteration_join_forms: T exact FP32 tiles, S BF16 screen + exact re-score, E exact BF16 tiles, "
-                 "P probe (nodes split between S and T, timed)",
+                 "P probe (nodes split between S and T, timed); walk_screen: the walk's BF16 MFMA screen with "
+                 "exact FP32 re-score (identical results)",
     }
     print(json.dumps(out), flush=True)
 

@@ -399,8 +399,8 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     // the data (how many pairs land within the screen's error bound of a
     // radius) and drifts as the lists converge.  Both forms admit exactly the
     // same candidates, so in auto mode a probe iteration (every third, from the
-    // first; the gap doubles, up to 12, after a probe the winner takes by more
-    // than 1.5x, and returns to 3 after a close one) splits the nodes between
+    // first; the gap grows fourfold, up to 24, after a probe the winner takes
+    // by more than 1.5x, and returns to 3 after a close one) splits the nodes between
     // the two forms, times both parts, and the next iterations use the screen
     // when it is at least 5 % faster per node
     // (its cost per node grows as the lists converge: on the 1.2M x 96 angular
@@ -492,7 +492,7 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
             double const c_screen = lead ? c_first : c_second, c_tile = lead ? c_second : c_first;
             use_mfma = c_screen < 0.95 * c_tile;
             double const margin = std::max(c_screen, c_tile) / std::max(std::min(c_screen, c_tile), 1e-30);
-            probe_gap = margin > 1.5 ? std::min(probe_gap * 2, 12u) : 3u;
+            probe_gap = margin > 1.5 ? std::min(probe_gap * 4, 24u) : 3u;
             next_probe = it + probe_gap;
         }
         // statistics and stop rule (kgraph.cpp:927-972)

@@ -136,6 +136,7 @@ __global__ void __launch_bounds__(256) entry_topm_kernel(kg_entry_args a) {
 }
 
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ short bf16_rne(float f) {  // round to nearest even (finite values)
@@ -260,26 +261,32 @@ __global__ void __launch_bounds__(256) qnorm_check_kernel(float const *Q, unsign
 // worse than it, so the query's m-th result is no worse either): a distance
 // above it cannot enter the result and skips the insert.  The four lanes of
 // a query and then the four waves are merged into the m best.
+// KS: 32-value slices per row, one v_mfma_f32_16x16x32_bf16 each (lane l
+// holds values 32 s + 8 (l / 16) .. + 8 of its row or query).
 template <int METRIC, int KS, int MF>
 __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float const *qsq, float const *esq) {
     __shared__ uint64_t best[4][16][MF];
     __shared__ uint32_t bound[16];  // per query: smallest m-th best ordered distance of its lanes
     unsigned const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    unsigned const r16 = lane & 15, kq = (lane >> 4) * 4;
+    unsigned const r16 = lane & 15, kq = (lane >> 4) * 4, k8 = (lane >> 4) * 8;
     unsigned const q = blockIdx.x * 16 + r16;
     bool const qok = q < a.nq;
     if (threadIdx.x < 16) bound[threadIdx.x] = 0xffffffffu;
     __syncthreads();
-    bf16x4_t bop[KS];
+    bf16x8_t bop[KS];
     {
         float const *qr = a.Q + size_t(qok ? q : 0u) * a.qstride;
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
+            uint32_t wv[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                unsigned const k = 16 * s + kq + j;
-                bop[s][j] = bf16_rne((qok && k < a.D) ? qr[k] : 0.f);
+                unsigned const k = 32 * s + k8 + 2 * j;
+                uint32_t const lo = uint16_t(bf16_rne((qok && k < a.D) ? qr[k] : 0.f));
+                uint32_t const hi = uint16_t(bf16_rne((qok && k + 1 < a.D) ? qr[k + 1] : 0.f));
+                wv[j] = lo | (hi << 16);
             }
+            bop[s] = __builtin_bit_cast(bf16x8_t, make_uint4(wv[0], wv[1], wv[2], wv[3]));
         }
     }
     float const qn = qok ? ((METRIC == METRIC_L2SQR) ? 2.f * qsq[q] : a.qnorms[q]) : 0.f;
@@ -292,33 +299,35 @@ __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float
     unsigned const ntiles = (a.E + 15) / 16;
     // operands of sample tile `et`: from the BF16 copy when there is one, else
     // rounded from the FP32 rows (the same values)
-    auto load_tile = [&](unsigned et, bf16x4_t (&op)[KS]) {
+    auto load_tile = [&](unsigned et, bf16x8_t (&op)[KS]) {
         unsigned const e = et * 16 + r16;
         bool const eok = e < a.E;
         if (a.rows_b) {
             // branch-free: every lane loads from a valid address and masks the
-            // value, so the loads stay in flight past the wait for the older tile
+            // value, so the loads stay in flight past the wait for the older
+            // tile (two 8-byte halves: rows are 8-byte aligned, Dp a multiple of 4)
             uint16_t const *er = a.rows_b + size_t(eok ? e : 0u) * a.Dp;
 #pragma unroll
             for (int s = 0; s < KS; ++s) {
-                unsigned const k = 16 * s + kq;
-                bool const ok = eok && k < a.Dp;
-                uint2 v = *reinterpret_cast<uint2 const *>(er + (k < a.Dp ? k : 0u));
-                v.x = ok ? v.x : 0u;
-                v.y = ok ? v.y : 0u;
-                op[s] = __builtin_bit_cast(bf16x4_t, v);
+                unsigned const k0 = 32 * s + k8, k1 = k0 + 4;
+                bool const ok0 = eok && k0 < a.Dp, ok1 = eok && k1 < a.Dp;
+                uint2 v0 = *reinterpret_cast<uint2 const *>(er + (k0 < a.Dp ? k0 : 0u));
+                uint2 v1 = *reinterpret_cast<uint2 const *>(er + (k1 < a.Dp ? k1 : 0u));
+                op[s] = __builtin_bit_cast(bf16x8_t, make_uint4(ok0 ? v0.x : 0u, ok0 ? v0.y : 0u, ok1 ? v1.x : 0u,
+                                                                ok1 ? v1.y : 0u));
             }
         } else {
             float const *er = a.rows + size_t(eok ? e : 0u) * a.Dp;
 #pragma unroll
             for (int s = 0; s < KS; ++s) {
-                unsigned const k = 16 * s + kq;
-                float4 const v =
-                    (eok && k < a.Dp) ? *reinterpret_cast<float4 const *>(er + k) : make_float4(0, 0, 0, 0);
-                op[s][0] = bf16_rne(v.x);
-                op[s][1] = bf16_rne(v.y);
-                op[s][2] = bf16_rne(v.z);
-                op[s][3] = bf16_rne(v.w);
+                unsigned const k0 = 32 * s + k8, k1 = k0 + 4;
+                float4 const v0 =
+                    (eok && k0 < a.Dp) ? *reinterpret_cast<float4 const *>(er + k0) : make_float4(0, 0, 0, 0);
+                float4 const v1 =
+                    (eok && k1 < a.Dp) ? *reinterpret_cast<float4 const *>(er + k1) : make_float4(0, 0, 0, 0);
+                auto pk = [](float x, float y) { return uint32_t(uint16_t(bf16_rne(x))) | (uint32_t(uint16_t(bf16_rne(y))) << 16); };
+                op[s] = __builtin_bit_cast(bf16x8_t, make_uint4(pk(v0.x, v0.y), pk(v0.z, v0.w), pk(v1.x, v1.y),
+                                                                pk(v1.z, v1.w)));
             }
         }
     };
@@ -335,7 +344,7 @@ __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float
         }
     };
     // two tiles in flight: buffers A (tiles w, w + 8, ...) and B (w + 4, w + 12, ...)
-    bf16x4_t opA[KS], opB[KS];
+    bf16x8_t opA[KS], opB[KS];
     float nA[4], nB[4];
     uint32_t iA[4], iB[4];
     if (w < ntiles) {
@@ -347,7 +356,7 @@ __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float
         load_meta(w + 4, nB, iB);
     }
     uint32_t tq = 0xffffffffu;  // the query's shared bound as this lane last saw it
-    auto score = [&](unsigned et, bf16x4_t (&op)[KS], float (&nv)[4], uint32_t (&iv)[4]) {
+    auto score = [&](unsigned et, bf16x8_t (&op)[KS], float (&nv)[4], uint32_t (&iv)[4]) {
         if (((et >> 2) & 3) == 3) {
             // refresh the bound: this lane's m-th best, the four lanes of the
             // query in this wave, then the block's other waves (LDS minimum)
@@ -363,7 +372,7 @@ __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float
         // the MFMAs read this buffer before the loads two tiles ahead refill it
         f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int s = 0; s < KS; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(op[s], bop[s], acc, 0, 0, 0);
+        for (int s = 0; s < KS; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(op[s], bop[s], acc, 0, 0, 0);
         float en[4];
         uint32_t eid[4];
 #pragma unroll
@@ -488,13 +497,13 @@ static void launch_exact_ks(kg_entry_args const &a, float const *qsq, float cons
 
 template <int METRIC>
 static void launch_exact_m(kg_entry_args const &a, float const *qsq, float const *esq, hipStream_t s) {
-    unsigned const ks = (a.D + 15) / 16;
-    if (ks <= 4)
+    unsigned const ks = (a.D + 31) / 32;
+    if (ks <= 2)
+        launch_exact_ks<METRIC, 2>(a, qsq, esq, s);
+    else if (ks <= 4)
         launch_exact_ks<METRIC, 4>(a, qsq, esq, s);
-    else if (ks <= 8)
-        launch_exact_ks<METRIC, 8>(a, qsq, esq, s);
     else
-        launch_exact_ks<METRIC, 16>(a, qsq, esq, s);
+        launch_exact_ks<METRIC, 8>(a, qsq, esq, s);
 }
 
 hipError_t kg_launch_entries_exact_mfma(int metric, kg_entry_args const &a, float const *qsq, float const *esq,

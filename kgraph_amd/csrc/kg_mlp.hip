@@ -5,7 +5,7 @@
 // reference; the exact FP32 distance re-ranks every shortlist).
 //
 //   gemm_kernel      C = A.B (+bias, ReLU or ReLU-mask epilogue) on MFMA:
-//                    v_mfma_f32_16x16x16bf16_1k, operands rounded to BF16
+//                    v_mfma_f32_16x16x32_bf16, operands rounded to BF16
 //                    (round to nearest even), FP32 accumulation.  One wave per
 //                    16x16 output tile; A and B are read through arbitrary
 //                    strides, so the forward, the input-gradient and the
@@ -22,7 +22,7 @@ namespace {
 
 using namespace kgd;
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ short to_bf16(float f) {  // round to nearest even
@@ -31,30 +31,44 @@ __device__ __forceinline__ short to_bf16(float f) {  // round to nearest even
     return short(u >> 16);
 }
 
-// One wave per 16x16 tile of C (M x N).  MFMA 16x16x16 operand layout
-// (wave64): lane l holds A[m = l%16][k = 4*(l/16) + j] and
-// B[k = 4*(l/16) + j][n = l%16], j = 0..3; the result lane l holds
-// C[m = 4*(l/16) + j][n = l%16].
+// One wave per 16x16 tile of C (M x N).  MFMA 16x16x32 operand layout
+// (wave64): lane l holds A[m = l%16][k = 8*(l/16) + j] and
+// B[k = 8*(l/16) + j][n = l%16], j = 0..7; the result lane l holds
+// C[m = 4*(l/16) + j][n = l%16], j = 0..3.
 __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
     unsigned const lane = threadIdx.x & 63;
     unsigned const tile = blockIdx.x * 4 + (threadIdx.x >> 6);
     unsigned const tiles_n = (g.N + 15) / 16;
     if (tile >= tiles_n * ((g.M + 15) / 16)) return;
     unsigned const tm = tile / tiles_n, tn = tile - tm * tiles_n;
-    unsigned const r16 = lane & 15, kq = (lane >> 4) * 4;
+    unsigned const r16 = lane & 15, k8 = (lane >> 4) * 8;
     unsigned const am = tm * 16 + r16, bn = tn * 16 + r16;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (unsigned k0 = 0; k0 < g.K; k0 += 16) {
-        bf16x4 av, bv;
+    for (unsigned k0 = 0; k0 < g.K; k0 += 32) {
+        uint32_t aw[4], bw[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            unsigned k = k0 + kq + j;
-            float a = (am < g.M && k < g.K) ? g.A[size_t(am) * g.sam + size_t(k) * g.sak] : 0.f;
-            float b = (bn < g.N && k < g.K) ? g.B[size_t(k) * g.sbk + size_t(bn) * g.sbn] : 0.f;
-            av[j] = to_bf16(a);
-            bv[j] = to_bf16(b);
+            uint32_t alo = 0, ahi = 0, blo = 0, bhi = 0;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                unsigned const k = k0 + k8 + 2 * j + h;
+                float a = (am < g.M && k < g.K) ? g.A[size_t(am) * g.sam + size_t(k) * g.sak] : 0.f;
+                float b = (bn < g.N && k < g.K) ? g.B[size_t(k) * g.sbk + size_t(bn) * g.sbn] : 0.f;
+                uint32_t const ab = uint16_t(to_bf16(a)), bb = uint16_t(to_bf16(b));
+                if (h == 0) {
+                    alo = ab;
+                    blo = bb;
+                } else {
+                    ahi = ab;
+                    bhi = bb;
+                }
+            }
+            aw[j] = alo | (ahi << 16);
+            bw[j] = blo | (bhi << 16);
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+        bf16x8 const av = __builtin_bit_cast(bf16x8, make_uint4(aw[0], aw[1], aw[2], aw[3]));
+        bf16x8 const bv = __builtin_bit_cast(bf16x8, make_uint4(bw[0], bw[1], bw[2], bw[3]));
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {

@@ -379,8 +379,9 @@ __device__ __forceinline__ void mj_stage(NndArgs const &a, LDS &L, unsigned &fil
     }
 }
 
-// Exact re-score of the queued pairs, one quad per pair.
-template <int METRIC, class LDS>
+// Exact re-score of the queued pairs, one quad per pair; both rows load in
+// groups of G chunks per lane.
+template <int METRIC, int G, class LDS>
 __device__ void mj_rescore(NndArgs const &a, LDS &L, unsigned rqn, unsigned &fill, unsigned nT, bool &found,
                            unsigned &pushes) {
     unsigned const quad = lane_id() >> 2, t = lane_id() & 3;
@@ -391,7 +392,7 @@ __device__ void mj_rescore(NndArgs const &a, LDS &L, unsigned rqn, unsigned &fil
         uint32_t const pr = act ? L.rq[e] : 0u;
         unsigned const ia = pr & 0xffffu, io = pr >> 16;
         uint32_t const av = L.ids[ia], o = L.ids[io];
-        float const acc = quad_metric<METRIC, float, true, 4>(a.X + size_t(av) * a.Dp, a.X + size_t(o) * a.Dp, a.D, t);
+        float const acc = quad_metric<METRIC, float, true, G>(a.X + size_t(av) * a.Dp, a.X + size_t(o) * a.Dp, a.D, t);
         float4 const ma = a.meta[av], mo = a.meta[o];
         float const na = (METRIC == METRIC_L2SQR) ? 0.f : ma.z;
         float const no = (METRIC == METRIC_L2SQR) ? 0.f : mo.z;
@@ -500,14 +501,14 @@ __global__ void __launch_bounds__(256) join_mfma_kernel(NndArgs a) {
             }
             if (EXACT) mj_stage<8>(a, L, fill, nT, pm, key, tgt);
             if (!EXACT && rqn > MQ - 4 * KG_WAVE) {
-                mj_rescore<METRIC>(a, L, rqn, fill, nT, found, pushes);
+                mj_rescore<METRIC, (KS >= 8 ? 4 : 6)>(a, L, rqn, fill, nT, found, pushes);
                 rescored += rqn;
                 rqn = 0;
             }
         }
     }
     if (!EXACT && rqn) {
-        mj_rescore<METRIC>(a, L, rqn, fill, nT, found, pushes);
+        mj_rescore<METRIC, (KS >= 8 ? 4 : 6)>(a, L, rqn, fill, nT, found, pushes);
         rescored += rqn;
     }
     wave_sync();

@@ -331,8 +331,8 @@ def test_join_form_choice_keeps_the_graph(kind, metric, d):
     # float rows with the BF16 precision: the screen and the exact FP32 tiles
     # admit the same candidates, so forcing either form or letting the probe
     # iterations choose gives graphs of the same quality; in auto mode the first
-    # iteration is a probe (form 3) and the next ones follow 3, 6 or 12
-    # iterations apart
+    # iteration is a probe (form 3) and the next ones follow 3 iterations apart
+    # after a close call, 12 or 24 after a clear one
     from kgraph_amd import PRECISION_BF16
     X = _data(kind, 20000, d)
     N, K = len(X), 16
@@ -348,8 +348,10 @@ def test_join_form_choice_keeps_the_graph(kind, metric, d):
         forms = g.build_iterations()[:, 3]
         if join == "auto":
             at = np.nonzero(forms == 3)[0]
-            assert at[0] == 0 and set(np.diff(at)) <= {3, 6, 12} and set(forms) <= {0, 1, 3}
-            assert len(at) >= 2
+            assert at[0] == 0 and set(np.diff(at)) <= {3, 12, 24} and set(forms) <= {0, 1, 3}
+            # after the first probe: the next one 3 iterations on, or none
+            # within 8 iterations when the first was a clear call
+            assert len(at) >= 2 or all(f in (0, 1) for f in forms[1:])
         else:
             assert (forms == (1 if join == "screen" else 0)).all()
         off, ids, dd, M = g.graph()

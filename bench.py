@@ -448,13 +448,14 @@ def batched_workload(args):
     # the mixed batch with their entry points
     h.call("kg_select_entries_device", dQ.ptr, nq, D, m, dI.ptr, W)
     nt = min(20000, nq)
-    chosen = (_lib.ctypes.c_uint * 5)()
+    chosen = (_lib.ctypes.c_uint * 6)()
     tt = time.perf_counter()
     h.call("kg_autotune_walk", dQ.ptr, nt, D, sp, ptr(np.ascontiguousarray(Ks[:nt])),
            ptr(np.ascontiguousarray(Ps[:nt])), dI.ptr, W, chosen)
     tune_s = time.perf_counter() - tt
     tuned = {"waves_per_block": chosen[0], "blocks_per_cu": chosen[1], "visited_entries": chosen[2],
-             "adjacency": "csr" if chosen[3] else "padded", "bf16_screen": bool(chosen[4])}
+             "adjacency": "csr" if chosen[3] else "padded", "bf16_screen": bool(chosen[4]),
+             "queries_per_wave": chosen[5]}
     log("autotuned in %.1fs: %s" % (tune_s, tuned))
     walls, ems_l = [], []
     for s in range(args.warmup + args.steps):

@@ -155,6 +155,13 @@ int kg_search_adjacency(kg_handle *h, int *padded, unsigned *width);
  * ids per query (ids: nq x m, out_d / out_e: nq x m), with the walk's
  * arithmetic; |screened - exact FP32| <= bound. */
 int kg_set_search_screen(kg_handle *h, int mode);
+/* Queries per wavefront of the graph walk (no counterpart in the reference,
+ * whose walk is one query per OpenMP thread, kgraph.cpp:423-564): 1 (default)
+ * walks each query on a whole 64-lane wave, 2 walks two queries per wave on
+ * 32 lanes each, with their own pools, LDS visited sets and bitsets (twice the
+ * memory accesses in flight per wave; the BF16 screen runs with 1 only).
+ * Results are identical. */
+int kg_set_search_queries_per_wave(kg_handle *h, unsigned queries);
 int kg_search_screen(kg_handle *h, int *active, unsigned long long *rescored);
 int kg_search_screen_dists(kg_handle *h, const float *Q, unsigned nq, size_t qstride, const unsigned *ids, unsigned m,
                            float *out_d, float *out_e);
@@ -310,10 +317,11 @@ int kg_autotune_search(kg_handle *h, const float *dQ, unsigned nq, size_t qstrid
  * the fastest for later searches on this handle: the visited set (bitset
  * only, or an LDS set of 1024, 2048, 4096 or 8192 entries in front of it),
  * the adjacency rows (padded or CSR), the BF16 screen (where walks gather the
- * FP32 rows) and the launch shape (as kg_autotune_search).  One setting at a
- * time, in that order, each kept when faster.  chosen (may be NULL) receives
- * {waves per block, blocks per CU, visited-set entries (0: bitset only),
- * adjacency (0 padded, 1 CSR), screen (0/1)}. */
+ * FP32 rows), queries per wave (1, or 2 when the screen is off) and the launch
+ * shape (as kg_autotune_search).  One setting at a time, in that order, each
+ * kept when faster.  chosen (may be NULL) receives 6 values: {waves per block,
+ * blocks per CU, visited-set entries (0: bitset only), adjacency (0 padded, 1
+ * CSR), screen (0/1), queries per wave}. */
 int kg_autotune_walk(kg_handle *h, const float *dQ, unsigned nq, size_t qstride, const kg_search_params *params,
                      const unsigned *Ks, const unsigned *Ps, const unsigned *d_init, size_t istride,
                      unsigned *chosen);

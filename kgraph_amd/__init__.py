@@ -279,13 +279,14 @@ class KGraph:
             ids = np.zeros((nq, max(width, entries)), dtype=np.uint32)
             self._h.call("kg_select_entries", ptr(q), nq, q.shape[1], entries, ptr(ids), ids.shape[1])
             init = devmem.DeviceArray.from_host(ids)
-        chosen = (_lib.ctypes.c_uint * 5)()
+        chosen = (_lib.ctypes.c_uint * 6)()
         sp = SearchParams(K, default_M, P, S, default_T, default_epsilon, default_seed, entries)
         self._h.call("kg_autotune_walk", dq.ptr, nq, q.shape[1], sp, ptr(Ks_a), ptr(Ps_a),
                      init.ptr if init is not None else None, init.shape[1] if init is not None else 0, chosen)
         devmem.synchronize()
         return {"waves_per_block": chosen[0], "blocks_per_cu": chosen[1], "visited_entries": chosen[2],
-                "adjacency": "csr" if chosen[3] else "padded", "screen": bool(chosen[4])}
+                "adjacency": "csr" if chosen[3] else "padded", "screen": bool(chosen[4]),
+                "queries_per_wave": chosen[5]}
 
     def build_counters(self):
         """Last build: {'pairs', 'list_entries', 'pushes', 'iterations',
@@ -312,6 +313,11 @@ class KGraph:
         """Screen walk candidates with BF16 dot products on MFMA before the
         exact FP32 re-score (walks over the FP32 rows; same results)."""
         self._h.call("kg_set_search_screen", 1 if on else 0)
+
+    def set_search_queries_per_wave(self, q):
+        """Walk 1 query per wavefront (default) or 2, each on half a wave
+        (same results)."""
+        self._h.call("kg_set_search_queries_per_wave", int(q))
 
     def search_screen_report(self):
         """The last walk launch: {'active': bool, 'rescored': candidates the

@@ -82,6 +82,7 @@ SIGNATURES = {
     "kg_set_search_adjacency": (_I, [_P, _I]),
     "kg_search_adjacency": (_I, [_P, _P, _P]),
     "kg_set_search_screen": (_I, [_P, _I]),
+    "kg_set_search_queries_per_wave": (_I, [_P, _U]),
     "kg_autotune_walk": (_I, [_P, _P, _U, _SZ, ctypes.POINTER(SearchParams), _P, _P, _P, _SZ, _P]),
     "kg_search_screen": (_I, [_P, _P, _P]),
     "kg_search_screen_dists": (_I, [_P, _P, _U, _SZ, _P, _U, _P, _P]),

@@ -265,6 +265,8 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         int const form = walk_rows(h);
         // BF16 screen in front of the FP32 rows (lossy BF16 copy; exact copies are gathered instead)
         bool const screen = form == 0 && h.search_screen == 1 && h.Dp <= 256;
+        // queries per wave: 2 walks each query on half a wave (not with the screen)
+        unsigned const qpw = screen ? 1u : h.search_qpw;
         // the T > 1 result list (a single trial's result is its pool's head)
         unsigned const rcap = sp.T > 1 ? kmax + 1 : 0u;
         // seeds sorted per trial: init ids, or P random ones (largest P of the batch)
@@ -285,7 +287,9 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         auto lds_for = [&](unsigned c) {
             return kg_search_lds_per_wave(poolmax, rcap, h.Dp, seed_cap, c, c ? bits_for(c) : 32u, !pad, screen);
         };
-        auto waves = [&](unsigned c) { return std::min<size_t>(kg_search_waves_per_cu(), h.lds_per_cu / lds_for(c)); };
+        auto waves = [&](unsigned c) {
+            return std::min<size_t>(kg_search_waves_per_cu(), h.lds_per_cu / (qpw * lds_for(c)));
+        };
         unsigned vcap = 0;
         if (h.search_visited != 1) {
             if (h.search_vcap) {
@@ -298,7 +302,8 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
             }
         }
         unsigned const vt_bits = vcap ? bits_for(vcap) : 32u;
-        size_t const lpw = lds_for(vcap);
+        size_t const lpq = lds_for(vcap);  // one query's LDS
+        size_t const lpw = lpq * qpw;
         unsigned wpb = std::min(h.search_wpb, kg_search_max_wpb());  // the kernel's launch bound
         while (wpb > 1 && lpw * wpb > 65536) wpb /= 2;
         size_t lpb = lpw * wpb;
@@ -306,11 +311,11 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         unsigned bpc =
             unsigned(std::max<size_t>(1, std::min<size_t>(kg_search_waves_per_cu() / wpb, h.lds_per_cu / lpb)));
         if (h.search_bpc) bpc = std::min(bpc, h.search_bpc);
-        unsigned blocks = std::min<unsigned>((ngraph + wpb - 1) / wpb, h.num_cus * bpc);
+        unsigned blocks = std::min<unsigned>((ngraph + wpb * qpw - 1) / (wpb * qpw), h.num_cus * bpc);
         unsigned words = (N + 31) / 32;
         size_t budget = size_t(8) << 30;  // visited bitsets: at most 8 GiB
-        while (blocks > 1 && size_t(blocks) * wpb * words * 4 > budget) blocks /= 2;
-        unsigned slots = blocks * wpb;
+        while (blocks > 1 && size_t(blocks) * wpb * qpw * words * 4 > budget) blocks /= 2;
+        unsigned slots = blocks * wpb * qpw;  // one visited bitset per query slot
         unsigned const tcap = 4096;
         if (slots > h.vis_slots || words != h.vis_words) {
             h.visited.release();
@@ -383,7 +388,8 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         a.qcounter = h.qcounter.p;
         a.pool_cap = poolmax;
         a.res_cap = rcap;
-        a.lds_per_wave = unsigned(lpw);
+        a.lds_per_wave = unsigned(lpq);
+        h.last_qpw = qpw;
         a.vcap = vcap;
         a.spills = h.qcounter.p + 1;
         a.vt_bits = vt_bits;
@@ -394,7 +400,7 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         h.last_vcap = vcap;
         a.seed_cap = seed_cap;
         KG_HIP(hipEventRecord(h.ev[0], h.stream));
-        KG_HIP(kg_launch_search(h.metric, a, blocks, wpb, h.stream));
+        KG_HIP(kg_launch_search(h.metric, a, blocks, wpb, qpw, h.stream));
         KG_HIP(hipEventRecord(h.ev[1], h.stream));
     }
     // brute-force queries (P >= N, kgraph.cpp:427-433)
@@ -781,6 +787,13 @@ int kg_search_adjacency(kg_handle *k, int *padded, unsigned *width) {
     return guard(k, [&](Handle &h) {
         if (padded) *padded = h.last_adjacency == 0 ? 1 : 0;
         if (width) *width = h.gpad_stride;
+    });
+}
+
+int kg_set_search_queries_per_wave(kg_handle *k, unsigned q) {
+    return guard(k, [&](Handle &h) {
+        if (q != 1 && q != 2) throw kga::invalid_argument("queries per wave must be 1 or 2");
+        h.search_qpw = q;
     });
 }
 
@@ -1284,6 +1297,7 @@ int kg_autotune_walk(kg_handle *k, float const *dQ, unsigned nq, size_t qstride,
         };
         int const old_visited = h.search_visited, old_adj = h.search_adjacency, old_screen = h.search_screen;
         unsigned const old_vcap = h.search_vcap, old_wpb = h.search_wpb, old_bpc = h.search_bpc;
+        unsigned const old_qpw = h.search_qpw;
         try {
             h.search_wpb = 4;
             h.search_bpc = 0;
@@ -1329,7 +1343,19 @@ int kg_autotune_walk(kg_handle *k, float const *dQ, unsigned nq, size_t qstride,
                 }
             }
             h.search_screen = bscr;
-            // 4. launch shape: waves per block x resident blocks per CU
+            // 4. queries per wave: two per wave (half a wave each) where the
+            // screen is off
+            unsigned bq = h.search_qpw;
+            if (!(h.search_screen && walk_rows(h) == 0 && h.Dp <= 256)) {
+                h.search_qpw = bq == 1 ? 2u : 1u;
+                float const t = run();
+                if (t < best) {
+                    best = t;
+                    bq = h.search_qpw;
+                }
+            }
+            h.search_qpw = bq;
+            // 5. launch shape: waves per block x resident blocks per CU
             unsigned bw = h.search_wpb, bb = h.search_bpc;
             for (unsigned wpb : {1u, 2u, 4u}) {
                 for (unsigned bpc : {4u, 8u, 16u, 32u}) {
@@ -1352,6 +1378,7 @@ int kg_autotune_walk(kg_handle *k, float const *dQ, unsigned nq, size_t qstride,
                 chosen[2] = h.search_visited == 1 ? 0u : (h.search_vcap ? h.search_vcap : h.last_vcap);
                 chosen[3] = unsigned(h.search_adjacency);
                 chosen[4] = unsigned(h.search_screen);
+                chosen[5] = h.search_qpw;
             }
         } catch (...) {
             h.search_visited = old_visited;
@@ -1360,6 +1387,7 @@ int kg_autotune_walk(kg_handle *k, float const *dQ, unsigned nq, size_t qstride,
             h.search_vcap = old_vcap;
             h.search_wpb = old_wpb;
             h.search_bpc = old_bpc;
+            h.search_qpw = old_qpw;
             throw;
         }
     });

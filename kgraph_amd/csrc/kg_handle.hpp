@@ -92,6 +92,8 @@ struct Handle {
     // before the exact FP32 re-score (same results); 0: off
     int search_screen = 0;
     int last_screen = 0;                         // whether the last walk launch screened
+    unsigned search_qpw = 1;                     // queries per wave (1, or 2 on half a wave each)
+    unsigned last_qpw = 1;
     DevBuf<unsigned long long> screen_cnt;       // candidates it re-scored
     bool has_data = false;
 

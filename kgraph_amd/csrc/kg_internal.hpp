@@ -68,7 +68,7 @@ struct kg_search_args {
     // set by kg_launch_init_check when a starting id is out of range: every
     // wave then exits at once (may be NULL)
     uint32_t const *abort_flag;
-    unsigned pool_cap, res_cap, lds_per_wave;
+    unsigned pool_cap, res_cap, lds_per_wave;  // lds_per_wave: LDS bytes of one query's walk
     unsigned seed_cap;  // starting points a query sorts: max P (random) or init
 };
 
@@ -138,8 +138,10 @@ unsigned kg_search_waves_per_cu();
 // gnode[n] = {goff[n], deg(n), M[n], 0}; *dupflag |= 1 if some list repeats an id.
 hipError_t kg_launch_gnode(unsigned N, uint32_t const *goff, uint32_t const *gids, uint32_t const *gM, uint4 *gnode,
                            uint32_t *dupflag, hipStream_t s);
+// queries_per_wave: 1, or 2 (each query on half a wave; not with the BF16
+// screen); a.lds_per_wave is the LDS of one query.
 hipError_t kg_launch_search(int metric, kg_search_args const &a, unsigned blocks, unsigned waves_per_block,
-                            hipStream_t s);
+                            unsigned queries_per_wave, hipStream_t s);
 // *width = max over nodes of min(M[n], deg(n)) (atomicMax; zeroed by the caller).
 hipError_t kg_launch_gpad_width(unsigned N, uint32_t const *goff, uint32_t const *gM, uint32_t *width, hipStream_t s);
 // gpad rows (first min(deg, gstride) ids of each list) and gmd words.

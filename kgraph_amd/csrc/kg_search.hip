@@ -39,6 +39,42 @@ constexpr unsigned SEARCH_MAX_WPB = 4;  // waves per block the kernel is compile
 constexpr uint32_t FLAG = 0x80000000u;
 constexpr uint32_t IDMASK = 0x7fffffffu;
 
+// The G lanes of a wave that walk one query (G = 64: the whole wave; G = 32:
+// two queries per wave, one per half).  gl = lane within the group, base = the
+// group's first lane; ballots and shuffles stay inside the group.
+template <unsigned G>
+struct Grp {
+    unsigned gl, base;
+    __device__ __forceinline__ Grp() {
+        unsigned const l = lane_id();
+        gl = l % G;
+        base = l - gl;
+    }
+    __device__ __forceinline__ uint64_t ballot(bool p) const {
+        uint64_t const b = kgd::ballot(p);
+        if constexpr (G == KG_WAVE) return b;
+        else return (b >> base) & ((1ull << G) - 1);
+    }
+    __device__ __forceinline__ uint64_t lt() const { return (1ull << gl) - 1; }
+    template <typename T>
+    __device__ __forceinline__ T shfl(T v, unsigned j) const {
+        return __shfl(v, int(base + j));
+    }
+};
+// the whole wave: the lane itself, base 0
+template <>
+struct Grp<KG_WAVE> {
+    unsigned gl;
+    static constexpr unsigned base = 0;
+    __device__ __forceinline__ Grp() : gl(lane_id()) {}
+    __device__ __forceinline__ uint64_t ballot(bool p) const { return kgd::ballot(p); }
+    __device__ __forceinline__ uint64_t lt() const { return (1ull << gl) - 1; }
+    template <typename T>
+    __device__ __forceinline__ T shfl(T v, unsigned j) const {
+        return __shfl(v, int(j));
+    }
+};
+
 struct WaveLDS {
     uint32_t *pid;
     float *pdist;
@@ -114,7 +150,9 @@ struct QInt {
     uint32_t sq;
 };
 
-template <int METRIC, typename RowT>
+// G lanes per query: NQ = G / 4 quads, one row per quad a pass (FP32) or two
+// (8-bit, BF16).
+template <int METRIC, typename RowT, unsigned G = KG_WAVE>
 __device__ __forceinline__ void batch_dist(kg_search_args const &a, uint32_t const *ids, float *out, unsigned cnt,
                                            float const *q, float qn, uint32_t const *q8, QInt qv) {
     RowT const *X;
@@ -124,14 +162,15 @@ __device__ __forceinline__ void batch_dist(kg_search_args const &a, uint32_t con
         X = a.Xb;
     else
         X = a.X;
-    unsigned const lane = lane_id();
+    constexpr unsigned NQ = G / 4;
+    unsigned const lane = lane_id() % G;
     unsigned const grp = lane >> 2, t = lane & 3;
     if constexpr (sizeof(RowT) == 1) {
         if (qv.on) {
             // integer path: v_dot4 over 8-bit words, exact (kg_device.hpp
             // quad_dot2_u8), then the metric's FP32 finish on exact values
-            for (unsigned b0 = 0; b0 < cnt; b0 += 32) {
-                unsigned const c0 = b0 + grp, c1 = c0 + 16;
+            for (unsigned b0 = 0; b0 < cnt; b0 += 2 * NQ) {
+                unsigned const c0 = b0 + grp, c1 = c0 + NQ;
                 if (c0 < cnt) {
                     bool const two = c1 < cnt;
                     uint32_t const id0 = ids[c0] & IDMASK, id1 = two ? (ids[c1] & IDMASK) : id0;
@@ -155,8 +194,8 @@ __device__ __forceinline__ void batch_dist(kg_search_args const &a, uint32_t con
             return;
         }
         // 8-bit rows: one 128-byte line per 128-value row, two rows per quad
-        for (unsigned b0 = 0; b0 < cnt; b0 += 32) {
-            unsigned const c0 = b0 + grp, c1 = c0 + 16;
+        for (unsigned b0 = 0; b0 < cnt; b0 += 2 * NQ) {
+            unsigned const c0 = b0 + grp, c1 = c0 + NQ;
             if (c0 < cnt) {  // whole quads are active together
                 bool const two = c1 < cnt;
                 uint32_t const id0 = ids[c0] & IDMASK, id1 = two ? (ids[c1] & IDMASK) : id0;
@@ -172,8 +211,8 @@ __device__ __forceinline__ void batch_dist(kg_search_args const &a, uint32_t con
     } else if constexpr (sizeof(RowT) == 2) {
         // BF16 rows are half the bytes: each quad takes two rows a pass, 32 per
         // pass, in the registers 16 FP32 rows need
-        for (unsigned b0 = 0; b0 < cnt; b0 += 32) {
-            unsigned const c0 = b0 + grp, c1 = c0 + 16;
+        for (unsigned b0 = 0; b0 < cnt; b0 += 2 * NQ) {
+            unsigned const c0 = b0 + grp, c1 = c0 + NQ;
             if (c0 < cnt) {  // whole quads are active together
                 bool const two = c1 < cnt;
                 uint32_t const id0 = ids[c0] & IDMASK, id1 = two ? (ids[c1] & IDMASK) : id0;
@@ -187,7 +226,7 @@ __device__ __forceinline__ void batch_dist(kg_search_args const &a, uint32_t con
             }
         }
     } else {
-        for (unsigned b0 = 0; b0 < cnt; b0 += 16) {
+        for (unsigned b0 = 0; b0 < cnt; b0 += NQ) {
             unsigned c = b0 + grp;
             if (b0 + grp < cnt) {  // whole quads are active together
                 uint32_t id = ids[c] & IDMASK;
@@ -387,15 +426,16 @@ __device__ __forceinline__ float ws_query(WaveLDS const &w, unsigned D, unsigned
 }
 
 // Number of entries of the sorted pool[0..L) whose dist <= nd.
+template <unsigned G = KG_WAVE>
 __device__ __forceinline__ unsigned insert_pos(float const *pdist, unsigned L, float nd) {
-    unsigned const lane = lane_id();
+    Grp<G> const g;
     unsigned r = 0;
-    for (unsigned c = 0; c < L; c += KG_WAVE) {
-        unsigned i = c + lane;
+    for (unsigned c = 0; c < L; c += G) {
+        unsigned i = c + g.gl;
         bool le = (i < L) && (pdist[i] <= nd);
-        unsigned n = popc64(ballot(le));
+        unsigned n = popc64(g.ballot(le));
         r += n;
-        if (n < KG_WAVE) break;
+        if (n < G) break;
     }
     return r;
 }
@@ -417,14 +457,16 @@ __device__ __forceinline__ unsigned insert_pos(float const *pdist, unsigned L, f
 // any other lands at or past Lf = L whatever its rank and is below no pool
 // entry, so only the ones below take part (most candidates of a long walk
 // are not, and then nothing moves).
+template <unsigned G = KG_WAVE>
 __device__ void batch_insert(WaveLDS &w, unsigned nf, uint32_t cm, uint32_t cb, unsigned qcap, unsigned &L,
                              unsigned &k) {
-    unsigned const lane = lane_id();
+    Grp<G> const g;
+    unsigned const lane = g.gl;
     unsigned const L0 = L, Lf = min(L0 + nf, qcap - 1);
     bool const full = L0 > 0 && L0 == qcap - 1;
     float const dl = lane < nf ? w.fdist[lane] : __builtin_huge_valf();
     bool const live = lane < nf && (!full || dl < w.pdist[L0 - 1]);
-    uint64_t const lm = ballot(live);
+    uint64_t const lm = g.ballot(live);
     unsigned const na = popc64(lm);
     if (na == 0) {
         L = Lf;
@@ -434,7 +476,11 @@ __device__ void batch_insert(WaveLDS &w, unsigned nf, uint32_t cm, uint32_t cb, 
     unsigned rank = 0;
     for (uint64_t m = lm; m; m &= m - 1) {
         unsigned const jj = ctz64(m);
-        float const dj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dl), int(jj)));
+        float dj;
+        if constexpr (G == KG_WAVE)
+            dj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dl), int(jj)));
+        else
+            dj = g.shfl(dl, jj);
         rank += unsigned(dj < dl || (dj == dl && jj < lane));
     }
     uint32_t const sid = live ? w.fid[lane] : 0u;
@@ -450,17 +496,17 @@ __device__ void batch_insert(WaveLDS &w, unsigned nf, uint32_t cm, uint32_t cb, 
         }
         fin = lo + rank;
     }
-    uint64_t const b0 = ballot(live && rank == 0);
+    uint64_t const b0 = g.ballot(live && rank == 0);
     unsigned const best_lane = b0 ? ctz64(b0) : 0u;
-    unsigned const fin0 = __shfl(fin, int(best_lane));
+    unsigned const fin0 = g.shfl(fin, best_lane);
     wave_sync();
     if (live) w.fdist[rank] = sd;  // sorted distances for the pool entries' search
     wave_sync();
     // pool entries move up by the number of candidates strictly below them
     // (those before the best candidate's position stay); top chunk first, each
     // chunk read whole before it is written
-    int const lo = int(min(fin0, L0) & ~(KG_WAVE - 1u));
-    for (int c = int((L0 - 1) & ~(KG_WAVE - 1u)); L0 > 0 && c >= lo; c -= KG_WAVE) {
+    int const lo = int(min(fin0, L0) & ~(G - 1u));
+    for (int c = int((L0 - 1) & ~(G - 1u)); L0 > 0 && c >= lo; c -= G) {
         unsigned const i = unsigned(c) + lane;
         bool const mv = i < L0;
         uint32_t id = 0, meta = 0, bs = 0;
@@ -588,18 +634,24 @@ __device__ __forceinline__ bool vt_has(uint32_t const *vt, VtCfg const &c, uint3
 #define SEARCH_WAVES_PER_EU 4
 #endif
 // SCREEN (RowT = float): each batch of a full pool goes through the BF16
-// screen (batch_screen) before the exact FP32 re-score.
-template <int METRIC, typename RowT, bool SCREEN = false>
+// screen (batch_screen) before the exact FP32 re-score.  QPW queries per
+// wave: each walks on G = 64 / QPW lanes with its own LDS area (a.lds_per_wave
+// bytes per query) and visited bitset; every loop steps G lanes.
+template <int METRIC, typename RowT, bool SCREEN = false, int QPW = 1>
 __global__ void __launch_bounds__(SEARCH_MAX_WPB * KG_WAVE) __attribute__((amdgpu_waves_per_eu(SEARCH_WAVES_PER_EU)))
 search_kernel(kg_search_args a) {
+    constexpr unsigned G = KG_WAVE / QPW;
+    static_assert(!SCREEN || QPW == 1, "the BF16 screen runs one query per wave");
     extern __shared__ __align__(16) unsigned char smem[];
-    unsigned const lane = lane_id();
+    Grp<G> const g;
+    unsigned const lane = g.gl;  // lane within this query's group
     unsigned const wib = threadIdx.x / KG_WAVE;
-    unsigned const slot = blockIdx.x * (blockDim.x / KG_WAVE) + wib;
+    unsigned const qslot = wib * QPW + g.base / G;  // query slot within the block
+    unsigned const slot = blockIdx.x * (blockDim.x / KG_WAVE) * QPW + qslot;
     unsigned const cap = a.pool_cap;
     unsigned const rcap = a.res_cap;
     LdsLayout lay(cap, rcap, a.Dp, a.seed_cap, a.vcap, a.vt_bits / 8, a.gpad == nullptr, SCREEN);
-    unsigned char *base = smem + size_t(wib) * a.lds_per_wave;
+    unsigned char *base = smem + size_t(qslot) * a.lds_per_wave;
     WaveLDS w;
     w.pid = reinterpret_cast<uint32_t *>(base + lay.pid);
     w.pdist = reinterpret_cast<float *>(base + lay.pdist);
@@ -628,13 +680,14 @@ search_kernel(kg_search_args a) {
 
     uint32_t *vis = a.visited + size_t(slot) * a.vis_words;
     uint32_t *touched = a.touched + size_t(slot) * a.touch_cap;
-    uint64_t const lt_mask = (1ull << lane) - 1;
+    uint64_t const lt_mask = g.lt();
     if (a.abort_flag && *a.abort_flag) return;  // a starting id is out of range: nothing is written
 
     for (;;) {
         unsigned qi = 0;
         if (lane == 0) qi = atomicAdd(a.qcounter, 1u);
-        qi = uniform(__shfl(qi, 0));
+        qi = g.shfl(qi, 0u);
+        if constexpr (QPW == 1) qi = uniform(qi);
         if (qi >= a.nq) break;
         uint64_t const t_start = __builtin_amdgcn_s_memrealtime();  // 100 MHz constant clock
         unsigned const K = a.Ks ? a.Ks[qi] : a.K0;
@@ -643,7 +696,7 @@ search_kernel(kg_search_args a) {
         unsigned const qcap = K + P + 1;
 
         float const *qsrc = a.Q + size_t(qi) * a.qstride;
-        for (unsigned i = lane; i < a.Dp; i += KG_WAVE) w.q[i] = (i < a.D) ? qsrc[i] : 0.f;
+        for (unsigned i = lane; i < a.Dp; i += G) w.q[i] = (i < a.D) ? qsrc[i] : 0.f;
         float const qn = (METRIC == METRIC_L2SQR) ? 0.f : a.qnorms[qi];
         // over the 8-bit rows, a query whose values are all integers in
         // [0, 255] takes the integer path (exact, so the same distances)
@@ -652,7 +705,7 @@ search_kernel(kg_search_args a) {
             wave_sync();
             bool ok = true;
             uint32_t sq = 0;
-            for (unsigned wd = lane; wd < a.Dp / 4; wd += KG_WAVE) {
+            for (unsigned wd = lane; wd < a.Dp / 4; wd += G) {
                 uint32_t word = 0;
                 for (unsigned j = 0; j < 4; ++j) {
                     float const v = w.q[4 * wd + j];
@@ -663,8 +716,8 @@ search_kernel(kg_search_args a) {
                 }
                 w.q8[wd] = word;
             }
-            for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
-            qv.on = a.int_dot && !ballot(!ok);
+            for (int o = G / 2; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
+            qv.on = a.int_dot && !g.ballot(!ok);
             qv.sq = sq;
         }
         wave_sync();
@@ -681,11 +734,11 @@ search_kernel(kg_search_args a) {
         // in the HBM bitset (ntouch ids recorded for clearing)
         bool glob = a.vcap == 0;
         unsigned vcount = 0;
-        for (unsigned i = lane; i < vwords; i += KG_WAVE) w.vt[i] = VT_EMPTY;
+        for (unsigned i = lane; i < vwords; i += G) w.vt[i] = VT_EMPTY;
         wave_sync();
         // moves the LDS set into the bitset; later visits go there
         auto spill = [&]() {
-            for (unsigned c = 0; c < vwords; c += KG_WAVE) {
+            for (unsigned c = 0; c < vwords; c += G) {
                 uint32_t const word = w.vt[c + lane];
                 for (unsigned half = 0; half < (vs.b16 ? 2u : 1u); ++half) {
                     uint32_t e = word;
@@ -695,7 +748,7 @@ search_kernel(kg_search_args a) {
                         has = e16 != 0xffffu;
                         e = has ? vt16_id(vs, 2 * (c + lane) + half, e16) : 0u;
                     }
-                    uint64_t const b = ballot(has);
+                    uint64_t const b = g.ballot(has);
                     if (has) {
                         atomicOr(&vis[e >> 5], 1u << (e & 31));
                         unsigned const pos = ntouch + popc64(b & lt_mask);
@@ -713,7 +766,7 @@ search_kernel(kg_search_args a) {
             unsigned L = 0;
             if (a.init == 0) {
                 uint32_t const *seeds = a.seeds + a.seed_off[a.Ps ? a.pidx[qi] : 0] + size_t(trial) * P;
-                for (unsigned c = 0; c < P; c += KG_WAVE) {
+                for (unsigned c = 0; c < P; c += G) {
                     unsigned i = c + lane;
                     uint32_t s = 0;
                     bool keep = false;
@@ -724,20 +777,20 @@ search_kernel(kg_search_args a) {
                             keep = glob ? !(load_u32_l2(&vis[s >> 5]) & (1u << (s & 31)))
                                         : !vt_has(w.vt, vs, s);
                     }
-                    uint64_t b = ballot(keep);
+                    uint64_t b = g.ballot(keep);
                     if (keep) w.pid[L + popc64(b & lt_mask)] = s;
                     L += popc64(b);
                 }
             } else {
                 uint32_t const *ini = a.init_ids + size_t(qi) * a.ostride;
                 L = min(a.init, K + P - 1);  // = init whenever init < K (one-K calls)
-                for (unsigned i = lane; i < L; i += KG_WAVE) w.pid[i] = ini[i];
+                for (unsigned i = lane; i < L; i += G) w.pid[i] = ini[i];
             }
             wave_sync();
             // ---- mark, score and sort the starting points (kgraph.cpp:468-476)
-            if (!glob && 2 * (vcount + L + KG_WAVE) > a.vcap) spill();
+            if (!glob && 2 * (vcount + L + G) > a.vcap) spill();
             if (glob) {
-                for (unsigned i = lane; i < L; i += KG_WAVE) {
+                for (unsigned i = lane; i < L; i += G) {
                     uint32_t s = w.pid[i];
                     atomicOr(&vis[s >> 5], 1u << (s & 31));
                     if (ntouch + i < a.touch_cap) touched[ntouch + i] = s;
@@ -745,14 +798,14 @@ search_kernel(kg_search_args a) {
                 ntouch += L;
             } else {
                 bool ovf = false;
-                for (unsigned c = 0; c < L; c += KG_WAVE) {
+                for (unsigned c = 0; c < L; c += G) {
                     unsigned const st = (c + lane < L) ? vt_put(w.vt, vs, w.pid[c + lane]) : VT_PRESENT;
-                    vcount += popc64(ballot(st == VT_FRESH));
+                    vcount += popc64(g.ballot(st == VT_FRESH));
                     ovf |= st == VT_OVERFLOW;
                 }
-                if (ballot(ovf)) {  // a 16-bit probe ran out of displacement: the bitset takes over
+                if (g.ballot(ovf)) {  // a 16-bit probe ran out of displacement: the bitset takes over
                     spill();
-                    for (unsigned i = lane; i < L; i += KG_WAVE) {
+                    for (unsigned i = lane; i < L; i += G) {
                         uint32_t s = w.pid[i];
                         atomicOr(&vis[s >> 5], 1u << (s & 31));
                         if (ntouch + i < a.touch_cap) touched[ntouch + i] = s;
@@ -760,11 +813,11 @@ search_kernel(kg_search_args a) {
                     ntouch += L;
                 }
             }
-            batch_dist<METRIC, RowT>(a, w.pid, w.pdist, L, w.q, qn, w.q8, qv);
-            for (unsigned i = lane; i < L; i += KG_WAVE)
+            batch_dist<METRIC, RowT, G>(a, w.pid, w.pdist, L, w.q, qn, w.q8, qv);
+            for (unsigned i = lane; i < L; i += G)
                 w.tkey[i] = (uint64_t(ord_f32(w.pdist[i])) << 32) | w.pid[i];
             wave_sync();
-            for (unsigned c = 0; c < L; c += KG_WAVE) {
+            for (unsigned c = 0; c < L; c += G) {
                 unsigned i = c + lane;
                 uint64_t key = (i < L) ? w.tkey[i] : ~0ull;
                 unsigned rank = 0;
@@ -778,7 +831,7 @@ search_kernel(kg_search_args a) {
                 }
             }
             wave_sync();
-            for (unsigned i = lane; i < L; i += KG_WAVE) {
+            for (unsigned i = lane; i < L; i += G) {
                 uint32_t bs = 0;
                 uint32_t const m = a.gpad ? VT_M_UNKNOWN : actual_M(a, w.pid[i] & IDMASK, &bs);
                 w.pmeta[i] = m << 16;
@@ -790,9 +843,9 @@ search_kernel(kg_search_args a) {
             unsigned k = 0;
             for (;;) {
                 unsigned kk = L;
-                for (unsigned c = k; c < L; c += KG_WAVE) {
+                for (unsigned c = k; c < L; c += G) {
                     unsigned i = c + lane;
-                    uint64_t b = ballot(i < L && (w.pid[i] & FLAG));
+                    uint64_t b = g.ballot(i < L && (w.pid[i] & FLAG));
                     if (b) {
                         kk = c + ctz64(b);
                         break;
@@ -813,7 +866,7 @@ search_kernel(kg_search_args a) {
                 bool const first_pad = a.gpad && eM == VT_M_UNKNOWN;
                 if (first_pad) {
                     uint32_t const md = a.gmd[eid & IDMASK];
-                    if (lane < min(a.S, unsigned(KG_WAVE)) && beginM + lane < a.gstride) pre = erow[beginM + lane];
+                    if (lane < min(a.S, G) && beginM + lane < a.gstride) pre = erow[beginM + lane];
                     uint32_t m = md & 0xffffu, deg = md >> 16;
                     if (m < a.Mparam) m = a.Mparam;
                     if (m > deg) m = deg;
@@ -829,18 +882,18 @@ search_kernel(kg_search_args a) {
                 }
                 wave_sync();
                 if (endM > beginM) n_scan += endM - beginM;
-                for (unsigned mc = beginM; mc < endM; mc += KG_WAVE) {
-                    unsigned const cnt = min(endM - mc, unsigned(KG_WAVE));
+                for (unsigned mc = beginM; mc < endM; mc += G) {
+                    unsigned const cnt = min(endM - mc, G);
                     uint32_t id = 0xffffffffu;
                     if (lane < cnt) id = (first_pad && mc == beginM) ? pre : erow[mc + lane];
                     bool dup = false;
                     if (!a.lists_unique) {  // a list may repeat an id: keep its first occurrence
                         for (unsigned j = 0; j < cnt; ++j) {
-                            uint32_t v = __shfl(id, j);
+                            uint32_t v = g.shfl(id, j);
                             dup |= (j < lane) && (v == id);
                         }
                     }
-                    if (glob && cnt <= 16) {
+                    if (glob && cnt <= G / 4) {
                         // the visited bits and the distances at once: the
                         // bitset atomics and the row gathers are in flight
                         // together, and the distances of ids found visited
@@ -855,9 +908,9 @@ search_kernel(kg_search_args a) {
                             batch_screen<METRIC>(a, w, w.fid, w.fdist, cnt, qn, qrn, qsq, w.pdist[L - 1], n_res, true,
                                                  cand, old, bit);
                         else
-                            batch_dist<METRIC, RowT>(a, w.fid, w.fdist, cnt, w.q, qn, w.q8, qv);
+                            batch_dist<METRIC, RowT, G>(a, w.fid, w.fdist, cnt, w.q, qn, w.q8, qv);
                         bool const fresh = cand && !(old & bit);
-                        uint64_t const b = ballot(fresh);
+                        uint64_t const b = g.ballot(fresh);
                         unsigned const nf = popc64(b);
                         unsigned const pos = popc64(b & lt_mask);
                         float const fd = fresh ? w.fdist[lane] : 0.f;
@@ -873,7 +926,7 @@ search_kernel(kg_search_args a) {
                         uint32_t cm = VT_M_UNKNOWN, cb = 0;
                         if (lane < nf && !a.gpad) cm = actual_M(a, w.fid[lane], &cb);
                         n_comps += nf;
-                        batch_insert(w, nf, cm, cb, qcap, L, k);
+                        batch_insert<G>(w, nf, cm, cb, qcap, L, k);
                         continue;
                     }
                     bool fresh = false;
@@ -887,7 +940,7 @@ search_kernel(kg_search_args a) {
                             fresh = st == VT_FRESH;
                         }
                     }
-                    if (!glob && ballot(st == VT_OVERFLOW)) {
+                    if (!glob && g.ballot(st == VT_OVERFLOW)) {
                         // a 16-bit probe ran out of displacement: move the set
                         // (this batch's inserts included) into the bitset, and
                         // test the rest there
@@ -897,7 +950,7 @@ search_kernel(kg_search_args a) {
                             fresh = !(atomicOr(&vis[id >> 5], bit) & bit);
                         }
                     }
-                    uint64_t b = ballot(fresh);
+                    uint64_t b = g.ballot(fresh);
                     unsigned const nf = popc64(b);
                     unsigned pos = popc64(b & lt_mask);
                     if (fresh) {
@@ -909,7 +962,7 @@ search_kernel(kg_search_args a) {
                     else
                         vcount += nf;
                     wave_sync();
-                    if (!glob && 2 * (vcount + KG_WAVE) > a.vcap) spill();
+                    if (!glob && 2 * (vcount + G) > a.vcap) spill();
                     if (nf == 0) continue;
                     // lane j fetches candidate j's M and CSR base while the rows load
                     uint32_t cm = VT_M_UNKNOWN, cb = 0;
@@ -917,10 +970,10 @@ search_kernel(kg_search_args a) {
                     if (SCREEN && L == qcap - 1)
                         batch_screen<METRIC>(a, w, w.fid, w.fdist, nf, qn, qrn, qsq, w.pdist[L - 1], n_res);
                     else
-                        batch_dist<METRIC, RowT>(a, w.fid, w.fdist, nf, w.q, qn, w.q8, qv);
+                        batch_dist<METRIC, RowT, G>(a, w.fid, w.fdist, nf, w.q, qn, w.q8, qv);
                     // the nf inserts of this batch, done at once (batch_insert)
                     n_comps += nf;
-                    batch_insert(w, nf, cm, cb, qcap, L, k);
+                    batch_insert<G>(w, nf, cm, cb, qcap, L, k);
                 }
             }
             // ---- merge this trial into the results (kgraph.cpp:515-529)
@@ -928,7 +981,7 @@ search_kernel(kg_search_args a) {
             if (a.T == 1) {
                 RL = L;
             } else if (trial == 0) {
-                for (unsigned i = lane; i < L; i += KG_WAVE) {
+                for (unsigned i = lane; i < L; i += G) {
                     w.rid[i] = w.pid[i] & IDMASK;
                     w.rdist[i] = w.pdist[i];
                 }
@@ -938,10 +991,10 @@ search_kernel(kg_search_args a) {
                 for (unsigned l = 0; l < L; ++l) {
                     float const cd = w.pdist[l];
                     uint32_t const cid = w.pid[l] & IDMASK;
-                    unsigned const r = insert_pos(w.rdist, RL, cd);
+                    unsigned const r = insert_pos<G>(w.rdist, RL, cd);
                     wave_sync();
                     // shift results[r..RL) up by one (slot RL is the scratch)
-                    for (int c = int(RL) - 1; c >= int(r); c -= KG_WAVE) {
+                    for (int c = int(RL) - 1; c >= int(r); c -= G) {
                         int i = c - int(lane);
                         uint32_t id = 0;
                         float d = 0.f;
@@ -969,15 +1022,15 @@ search_kernel(kg_search_args a) {
         uint32_t const *oid = (a.T == 1) ? w.pid : w.rid;
         float const *odist = (a.T == 1) ? w.pdist : w.rdist;
         unsigned cut = RL;
-        for (unsigned c = 0; c < RL; c += KG_WAVE) {
+        for (unsigned c = 0; c < RL; c += G) {
             unsigned i = c + lane;
-            uint64_t b = ballot(i < RL && odist[i] > a.epsilon);
+            uint64_t b = g.ballot(i < RL && odist[i] > a.epsilon);
             if (b) {
                 cut = c + ctz64(b);
                 break;
             }
         }
-        for (unsigned i = lane; i < cut; i += KG_WAVE) {
+        for (unsigned i = lane; i < cut; i += G) {
             a.out_ids[size_t(qi) * a.ostride + i] = oid[i] & IDMASK;
             if (a.out_dists) a.out_dists[size_t(qi) * a.ostride + i] = odist[i];
         }
@@ -991,9 +1044,9 @@ search_kernel(kg_search_args a) {
         // ---- clear the visited bits this query set (none while the LDS set held them)
         if (!glob) {
         } else if (ntouch <= a.touch_cap) {
-            for (unsigned i = lane; i < ntouch; i += KG_WAVE) vis[load_u32_l2(&touched[i]) >> 5] = 0u;
+            for (unsigned i = lane; i < ntouch; i += G) vis[load_u32_l2(&touched[i]) >> 5] = 0u;
         } else {
-            for (unsigned i = lane; i < a.vis_words; i += KG_WAVE) vis[i] = 0u;
+            for (unsigned i = lane; i < a.vis_words; i += G) vis[i] = 0u;
         }
         __threadfence();
         wave_sync();
@@ -1145,35 +1198,44 @@ unsigned kg_search_max_wpb() { return SEARCH_MAX_WPB; }
 unsigned kg_search_waves_per_cu() { return 4 * SEARCH_WAVES_PER_EU; }
 
 hipError_t kg_launch_search(int metric, kg_search_args const &a, unsigned blocks, unsigned waves_per_block,
-                            hipStream_t s) {
+                            unsigned queries_per_wave, hipStream_t s) {
     if (waves_per_block == 0 || waves_per_block > SEARCH_MAX_WPB || blocks == 0) return hipErrorInvalidValue;
-    size_t lds = size_t(waves_per_block) * a.lds_per_wave;
+    if (queries_per_wave != 1 && queries_per_wave != 2) return hipErrorInvalidValue;
+    size_t lds = size_t(waves_per_block) * queries_per_wave * a.lds_per_wave;
     dim3 grid(blocks), block(waves_per_block * KG_WAVE);
     // row form: the 8-bit copy, the BF16 copy, the FP32 rows behind the BF16
-    // screen, or the FP32 rows
-    if (a.Xs && (a.X8 || a.Xb || a.Dp > 256)) return hipErrorInvalidValue;
-    auto go = [&](auto kernel8, auto kernel16, auto kernel32s, auto kernel32) {
+    // screen, or the FP32 rows; one or two queries per wave
+    if (a.Xs && (a.X8 || a.Xb || a.Dp > 256 || queries_per_wave != 1)) return hipErrorInvalidValue;
+    auto go = [&](auto kernel8, auto kernel16, auto kernel32s, auto kernel32, auto kernel8x2, auto kernel16x2,
+                  auto kernel32x2) {
+        bool const two = queries_per_wave == 2;
         if (a.X8)
-            hipLaunchKernelGGL(kernel8, grid, block, lds, s, a);
+            hipLaunchKernelGGL(two ? kernel8x2 : kernel8, grid, block, lds, s, a);
         else if (a.Xb)
-            hipLaunchKernelGGL(kernel16, grid, block, lds, s, a);
+            hipLaunchKernelGGL(two ? kernel16x2 : kernel16, grid, block, lds, s, a);
         else if (a.Xs)
             hipLaunchKernelGGL(kernel32s, grid, block, lds, s, a);
         else
-            hipLaunchKernelGGL(kernel32, grid, block, lds, s, a);
+            hipLaunchKernelGGL(two ? kernel32x2 : kernel32, grid, block, lds, s, a);
     };
     switch (metric) {
         case METRIC_L2SQR:
             go(search_kernel<METRIC_L2SQR, uint8_t>, search_kernel<METRIC_L2SQR, uint16_t>,
-               search_kernel<METRIC_L2SQR, float, true>, search_kernel<METRIC_L2SQR, float>);
+               search_kernel<METRIC_L2SQR, float, true>, search_kernel<METRIC_L2SQR, float>,
+               search_kernel<METRIC_L2SQR, uint8_t, false, 2>, search_kernel<METRIC_L2SQR, uint16_t, false, 2>,
+               search_kernel<METRIC_L2SQR, float, false, 2>);
             break;
         case METRIC_EUCLIDEAN:
             go(search_kernel<METRIC_EUCLIDEAN, uint8_t>, search_kernel<METRIC_EUCLIDEAN, uint16_t>,
-               search_kernel<METRIC_EUCLIDEAN, float, true>, search_kernel<METRIC_EUCLIDEAN, float>);
+               search_kernel<METRIC_EUCLIDEAN, float, true>, search_kernel<METRIC_EUCLIDEAN, float>,
+               search_kernel<METRIC_EUCLIDEAN, uint8_t, false, 2>, search_kernel<METRIC_EUCLIDEAN, uint16_t, false, 2>,
+               search_kernel<METRIC_EUCLIDEAN, float, false, 2>);
             break;
         default:
             go(search_kernel<METRIC_ANGULAR, uint8_t>, search_kernel<METRIC_ANGULAR, uint16_t>,
-               search_kernel<METRIC_ANGULAR, float, true>, search_kernel<METRIC_ANGULAR, float>);
+               search_kernel<METRIC_ANGULAR, float, true>, search_kernel<METRIC_ANGULAR, float>,
+               search_kernel<METRIC_ANGULAR, uint8_t, false, 2>, search_kernel<METRIC_ANGULAR, uint16_t, false, 2>,
+               search_kernel<METRIC_ANGULAR, float, false, 2>);
             break;
     }
     return hipGetLastError();

@@ -465,6 +465,46 @@ def test_walk_autotuner_keeps_results(kind, metric, d, entries):
     assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
 
 
+@pytest.mark.parametrize("kind,metric,d", [("intmix", L2, 128), ("uniform", EUC, 37), ("angular", ANG, 96)])
+def test_two_queries_per_wave_give_identical_walks(kind, metric, d):
+    # each query on half a wave (kg_set_search_queries_per_wave 2) against the
+    # whole-wave walk and the oracle: random starts, T > 1, M > 0, epsilon,
+    # entry points, a mixed K/P batch, the LDS set and the bitset, every row form
+    import ctypes
+    X, Q, idx = setup(kind, metric, d=d)
+    g = gpu_index(X, idx, metric)
+    g.set_entry_sample(512, seed=2)
+    rs = np.random.RandomState(4)
+    Ks = rs.choice([1, 10, 25], size=len(Q)).astype(np.uint32)
+    Ps = rs.choice([10, 40, 100], size=len(Q)).astype(np.uint32)
+    cases = [dict(K=10, P=40), dict(K=10, P=200, S=64), dict(K=10, P=30, T=3), dict(K=10, P=64, M=40),
+             dict(K=25, P=50, entries=9), dict(K=10, Ks=Ks, Ps=Ps), dict(K=10, P=100, S=16), dict(K=10, P=60, S=8)]
+    for vis in (0, 1):
+        g._h.call("kg_set_search_visited", vis)
+        for rows in ((0, 1, 2) if kind == "intmix" else (0,)):
+            g._h.call("kg_set_row_storage", rows)
+            for kw in cases:
+                out = {}
+                for q in (1, 2):
+                    g.set_search_queries_per_wave(q)
+                    ids, dd = g.search(Q, withDistance=True, stats=True, **kw)
+                    out[q] = (ids, dd, g.last_counts.copy(), g.last_comps.copy(), g.last_scans.copy())
+                g.set_search_queries_per_wave(1)
+                for a_, b_ in zip(out[1], out[2]):
+                    assert np.array_equal(a_, b_), (vis, rows, kw)
+    g._h.call("kg_set_search_visited", 0)
+    g._h.call("kg_set_row_storage", 0)
+    g.set_search_queries_per_wave(2)
+    compare_g = g
+    ids, dd, c, comps, scans = idx.search(X, Q, metric, K=10, P=100)
+    gi, gd = compare_g.search(Q, K=10, P=100, withDistance=True, stats=True)
+    assert np.array_equal(compare_g.last_counts, c)
+    for q in range(len(Q)):
+        assert np.array_equal(gi[q, :c[q]], ids[q, :c[q]]) and np.array_equal(gd[q, :c[q]], dd[q, :c[q]]), q
+    assert np.array_equal(compare_g.last_comps, comps) and np.array_equal(compare_g.last_scans, scans)
+    g.set_search_queries_per_wave(1)
+
+
 def _walk_cases():
     import json
     import os

@@ -181,7 +181,7 @@ __device__ __forceinline__ void put_cand(NndArgs const &a, uint32_t target, unsi
     if (slot < a.C) {
         a.cand[size_t(target) * a.C + slot] = key;
     } else {
-        unsigned const h = unsigned(mix64((uint64_t(target) << 32) | (uint32_t(key) >> 1)) % a.O);
+        unsigned const h = unsigned(mix64((uint64_t(target) << 32) | (uint32_t(key) >> 1))) & (a.O - 1u);  // O: a power of two
         atomicMin(reinterpret_cast<unsigned long long *>(&a.ovf[size_t(target) * a.O + h]),
                   static_cast<unsigned long long>(key));
     }
@@ -218,7 +218,8 @@ template <unsigned Q>
 struct MJoinLDS {
     uint32_t ids[MT];
     uint32_t cnt[MT];
-    float an[16];  // join norms of the current 16 new rows (FP32 tile form)
+    float4 am[32];  // per-node records of the new rows whose tiles are in registers (MFMA form)
+    float an[16];   // join norms of the current 16 new rows (FP32 tile form)
     uint64_t key[MB];
     uint32_t rq[Q];  // pair (list index of the new row) | (list index of the other row) << 16
     uint16_t tgt[MB];
@@ -380,10 +381,13 @@ __device__ __forceinline__ void mj_stage(NndArgs const &a, LDS &L, unsigned &fil
 }
 
 // Exact re-score of the queued pairs, one quad per pair; both rows load in
-// groups of G chunks per lane.
+// groups of G chunks per lane.  `found` tests the callback's first insert
+// (kgraph.cpp:805-807): into the smaller id of a new x new pair (the join
+// enumerates i < j by id, kgraph.cpp:741-745), into the new row of a new x
+// old pair; list positions below nA hold the new rows.
 template <int METRIC, int G, class LDS>
-__device__ void mj_rescore(NndArgs const &a, LDS &L, unsigned rqn, unsigned &fill, unsigned nT, bool &found,
-                           unsigned &pushes) {
+__device__ void mj_rescore(NndArgs const &a, LDS &L, unsigned rqn, unsigned nA, unsigned &fill, unsigned nT,
+                           bool &found, unsigned &pushes) {
     unsigned const quad = lane_id() >> 2, t = lane_id() & 3;
     wave_sync();
     for (unsigned b0 = 0; b0 < rqn; b0 += 16) {
@@ -400,7 +404,7 @@ __device__ void mj_rescore(NndArgs const &a, LDS &L, unsigned rqn, unsigned &fil
         bool const lead = act && t == 0;
         bool const pa = lead && d < ma.x;
         bool const po = lead && d < mo.x;
-        found |= lead && d < ma.y;
+        found |= lead && d < ((io < nA && o < av) ? mo.y : ma.y);
         pushes += unsigned(pa) + unsigned(po);
         uint64_t const key[2] = {make_key(d, o, 1), make_key(d, av, 1)};
         unsigned const tgt[2] = {ia, io};
@@ -410,7 +414,11 @@ __device__ void mj_rescore(NndArgs const &a, LDS &L, unsigned rqn, unsigned &fil
 }
 
 template <int METRIC, int KS, bool EXACT, bool F32>
-__global__ void __launch_bounds__(256) join_mfma_kernel(NndArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXACT ? 4 : 3)))
+join_mfma_kernel(NndArgs a) {
+    // 16-row new groups held in registers at once: two for BF16 operands of
+    // up to 128 values (16 VGPRs each), one otherwise
+    constexpr unsigned AG = (!F32 && KS <= 4) ? 2u : 1u;
     __shared__ MJoinLDS<EXACT ? 1u : MQ> lds[4];
     unsigned const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     unsigned const n = a.n0 + blockIdx.x * 4 + w;
@@ -436,79 +444,107 @@ __global__ void __launch_bounds__(256) join_mfma_kernel(NndArgs a) {
     unsigned const r16 = lane & 15, mrow = (lane >> 4) * 4;
     unsigned fill = 0, rqn = 0, pairs = 0, pushes = 0, rescored = 0;
     bool found = false;
-    for (unsigned ag = 0; ag < nA; ag += 16) {
-        // the A tile holds list rows ag .. ag + 15 (new rows, and old ones when
-        // the group runs past the new rows: their pairs are masked below), so
-        // it is also the B tile at tg == ag and those rows load once
-        MjOps<KS, F32> aop;
-        {
-            unsigned const ia = ag + r16;
-            aop.load(a, ia < nT ? L.ids[ia] : 0u, ia < nT, lane);
-        }
-        // the four new rows whose pairs this lane finishes
-        uint32_t aid[4];
-        float arad[4], akth[4], an[4], arn[4];
+    constexpr int RG = KS >= 8 ? 4 : 6;  // re-score load group
+    for (unsigned ag = 0; ag < nA; ag += 16 * AG) {
+        // AG groups of 16 list rows from ag (new rows, and old ones when a
+        // group runs past the new rows: their pairs are masked below) stay in
+        // registers while every later tile streams past them once; a tile
+        // that is one of the groups is taken from its registers
+        MjOps<KS, F32> aop[AG];
+        uint32_t aid[AG][4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            unsigned const ia = ag + mrow + j;
-            bool const ok = ia < nA;
-            aid[j] = ok ? L.ids[ia] : 0u;
-            float4 const m = a.meta[aid[j]];  // radius, K-th distance, join norm, |x|
-            arad[j] = ok ? m.x : 0.f;
-            akth[j] = ok ? m.y : 0.f;
-            an[j] = m.z;
-            arn[j] = m.w;
+        for (unsigned g = 0; g < AG; ++g) {
+            unsigned const ia = ag + 16 * g + r16;
+            aop[g].load(a, ia < nT ? L.ids[ia] : 0u, ia < nT, lane);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                unsigned const ib = ag + 16 * g + mrow + j;
+                aid[g][j] = ib < nA ? L.ids[ib] : 0u;
+            }
         }
-        for (unsigned tg = 0; tg < nT; tg += 16) {
+        wave_sync();  // the previous groups' records are no longer read
+        if (lane < 16 * AG) {
+            unsigned const ia = ag + lane;
+            L.am[lane] = ia < nA ? a.meta[L.ids[ia]] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        wave_sync();
+        // a tile before ag holds new rows at lower list positions only: its
+        // pairs with these rows were taken when it was the A group
+        for (unsigned tg = ag; tg < nT; tg += 16) {
             unsigned const io = tg + r16;
             bool const ook = io < nT;
             uint32_t const oid = ook ? L.ids[io] : 0u;
+            unsigned const dg = (tg - ag) >> 4;  // wave-uniform: the tile is A group dg when dg < AG
             MjOps<KS, F32> bop;
-            if (tg == ag)
-                bop = aop;
+            if (dg == 0)
+                bop = aop[0];
+            else if (AG > 1 && dg == 1)
+                bop = aop[AG - 1];
             else
                 bop.load(a, oid, ook, lane);
-            float4 const om = a.meta[oid];
-            float const orad = om.x, on = om.z;
-            mj_f32x4 const acc = mj_mma(aop, bop);
-            unsigned pm = 0;
-            uint64_t key[8];
-            unsigned tgt[8];
+            float4 const om = a.meta[oid];  // radius, K-th distance, join norm, |x|
+            // one group at a time (a real loop: the groups' epilogues do not
+            // overlap in registers)
+#pragma unroll 1
+            for (unsigned g = 0; g < AG; ++g) {
+                unsigned const gb = ag + 16 * g;
+                if (gb >= nA || gb > tg) continue;  // no new rows, or every pair below the diagonal
+                uint32_t gid[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                unsigned const ia = ag + mrow + j;
-                // new x new once per id pair (i < j by id), new x old always (kgraph.cpp:741-750)
-                bool const valid = ia < nA && ook && (io < nA ? aid[j] < oid : aid[j] != oid);
-                float const d = mj_dist<METRIC>(acc[j], an[j], on);
-                pairs += valid;
-                if (EXACT) {
-                    bool const pa = valid && d < arad[j];
-                    bool const po = valid && d < orad;
-                    found |= valid && d < akth[j];
-                    pushes += unsigned(pa) + unsigned(po);
-                    pm |= (unsigned(pa) << (2 * j)) | (unsigned(po) << (2 * j + 1));
-                    key[2 * j] = make_key(d, oid, 1);
-                    key[2 * j + 1] = make_key(d, aid[j], 1);
-                    tgt[2 * j] = ia;
-                    tgt[2 * j + 1] = io;
-                } else {
-                    float const lo = d - mj_bound<METRIC>(a, an[j], on, arn[j], om.w);
-                    bool const pass = valid && (lo < arad[j] || lo < orad);
-                    uint64_t const b = ballot(pass);
-                    if (pass) L.rq[rqn + popc64(b & ((1ull << lane) - 1))] = ia | (io << 16);
-                    rqn += popc64(b);
+                for (int j = 0; j < 4; ++j) gid[j] = g ? aid[AG - 1][j] : aid[0][j];
+                mj_f32x4 acc;
+                if (g == 0)  // wave-uniform
+                    acc = mj_mma(aop[0], bop);
+                else
+                    acc = mj_mma(aop[AG - 1], bop);
+                unsigned pm = 0;
+                uint64_t key[8];
+                unsigned tgt[8];
+                // the A rows' records are read from LDS at every tile: kept in
+                // registers across the tile loop they would cost 16 VGPRs a group
+                unsigned mb = 16 * g + mrow;
+                asm volatile("" : "+v"(mb));
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    unsigned const ia = gb + mrow + j;
+                    float4 const am = L.am[mb + j];
+                    // new x new once per pair of list positions (the reference's
+                    // i < j by id takes each id pair once, kgraph.cpp:741-745),
+                    // new x old always (:746-748); an id never pairs with itself
+                    bool const valid = ia < nA && ook && gid[j] != oid && (io >= nA || ia < io);
+                    float const d = mj_dist<METRIC>(acc[j], am.z, om.z);
+                    pairs += valid;
+                    if (EXACT) {
+                        bool const pa = valid && d < am.x;
+                        bool const po = valid && d < om.x;
+                        // the callback's first insert (kgraph.cpp:805-807): into the
+                        // smaller id of a new x new pair, into the new row otherwise
+                        found |= valid && d < ((io < nA && oid < gid[j]) ? om.y : am.y);
+                        pushes += unsigned(pa) + unsigned(po);
+                        pm |= (unsigned(pa) << (2 * j)) | (unsigned(po) << (2 * j + 1));
+                        key[2 * j] = make_key(d, oid, 1);
+                        key[2 * j + 1] = make_key(d, gid[j], 1);
+                        tgt[2 * j] = ia;
+                        tgt[2 * j + 1] = io;
+                    } else {
+                        float const lo = d - mj_bound<METRIC>(a, am.z, om.z, am.w, om.w);
+                        bool const pass = valid && (lo < am.x || lo < om.x);
+                        uint64_t const b = ballot(pass);
+                        if (pass) L.rq[rqn + popc64(b & ((1ull << lane) - 1))] = ia | (io << 16);
+                        rqn += popc64(b);
+                    }
                 }
-            }
-            if (EXACT) mj_stage<8>(a, L, fill, nT, pm, key, tgt);
-            if (!EXACT && rqn > MQ - 4 * KG_WAVE) {
-                mj_rescore<METRIC, (KS >= 8 ? 4 : 6)>(a, L, rqn, fill, nT, found, pushes);
-                rescored += rqn;
-                rqn = 0;
+                if (EXACT) mj_stage<8>(a, L, fill, nT, pm, key, tgt);
+                if (!EXACT && rqn > MQ - 4 * KG_WAVE) {
+                    mj_rescore<METRIC, RG>(a, L, rqn, nA, fill, nT, found, pushes);
+                    rescored += rqn;
+                    rqn = 0;
+                }
             }
         }
     }
     if (!EXACT && rqn) {
-        mj_rescore<METRIC, (KS >= 8 ? 4 : 6)>(a, L, rqn, fill, nT, found, pushes);
+        mj_rescore<METRIC, RG>(a, L, rqn, nA, fill, nT, found, pushes);
         rescored += rqn;
     }
     wave_sync();
@@ -531,10 +567,14 @@ __global__ void __launch_bounds__(256) join_mfma_kernel(NndArgs a) {
 // registers and computes its 16 pairs with the tile's new rows in canonical
 // order (kg_device.hpp); lane t of a quad keeps the pairs of new rows t, t + 4,
 // t + 8, t + 12, and staging / flushing is shared with the MFMA form.  With
-// CH > 0 the tile's 16 new rows are first copied into LDS (16 Dp floats per
+// CH >= 0 the tile's 16 new rows are first copied into LDS (16 Dp floats per
 // wave, dynamic shared memory), where all 16 quads read the same chunk in the
 // same instruction (one broadcast read) instead of each fetching the rows
-// through the cache.
+// through the cache; CH is then exactly the number of 4-value chunks per lane
+// in the 16-value part of a row ((D & ~15) / 16), so the chunk loop has no
+// per-chunk conditions (each would split the loop and wait on every LDS read
+// by itself), and the last D % 16 values follow one by one.  CH < 0 (rows of
+// more than 143 values) reads both rows from global memory (quad_metric).
 template <int METRIC, int CH>
 __global__ void __launch_bounds__(256) join_tile_kernel(NndArgs a) {
     __shared__ MJoinLDS<1> lds[4];
@@ -561,13 +601,13 @@ __global__ void __launch_bounds__(256) join_tile_kernel(NndArgs a) {
     }
     wave_sync();
     unsigned const quad = lane >> 2, t = lane & 3;
-    unsigned const nch = (a.D & ~15u) >> 2;
     unsigned const dp4 = a.Dp / 4;
-    float4 *const As = tile_rows + size_t(w) * 16 * dp4;  // CH > 0: the tile's new rows
+    unsigned const d16 = a.D & ~15u;  // values covered by the chunk loop
+    float4 *const As = tile_rows + size_t(w) * 16 * dp4;  // CH >= 0: the tile's new rows
     unsigned fill = 0, pairs = 0, pushes = 0;
     bool found = false;
     for (unsigned ag = 0; ag < nA; ag += 16) {
-        if (CH > 0) {
+        if (CH >= 0) {
             wave_sync();  // the previous tile's reads are done
             for (unsigned e = lane; e < 16 * dp4; e += KG_WAVE) {
                 unsigned const r = e / dp4, c = e - r * dp4;
@@ -589,55 +629,46 @@ __global__ void __launch_bounds__(256) join_tile_kernel(NndArgs a) {
         }
         if (lane < 16) L.an[lane] = (METRIC == METRIC_L2SQR || ag + lane >= nA) ? 0.f : a.meta[L.ids[ag + lane]].z;
         wave_sync();
-        for (unsigned tg = 0; tg < nT; tg += 16) {
+        // a tile before ag holds new rows at lower list positions only: its
+        // pairs with these rows were taken when it was the A group
+        for (unsigned tg = ag; tg < nT; tg += 16) {
             unsigned const io = tg + quad;
             bool const ook = io < nT;
             uint32_t const oid = ook ? L.ids[io] : 0u;
             float const *orow = a.X + size_t(oid) * a.Dp;
             float4 reg[CH > 0 ? CH : 1];
-            if (CH > 0) {
 #pragma unroll
-                for (int k = 0; k < CH; ++k) {
-                    unsigned const kk = t + 4 * k;
-                    reg[k] = (kk < nch) ? reinterpret_cast<float4 const *>(orow)[kk] : make_float4(0, 0, 0, 0);
-                }
-            }
+            for (int k = 0; k < CH; ++k) reg[k] = reinterpret_cast<float4 const *>(orow)[t + 4 * k];
             float4 const om = a.meta[oid];
             float const orad = om.x;
             float const on = (METRIC == METRIC_L2SQR) ? 0.f : om.z;
             float dres[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                unsigned const ia = ag + r;
-                uint32_t const av = L.ids[ia < nA ? ia : 0u];
-                float const *ra = a.X + size_t(av) * a.Dp;
                 float4 const *ra4 = As + r * dp4;
                 float acc;
-                if (CH > 0) {
+                if (CH >= 0) {
                     float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
                     for (int k = 0; k < CH; ++k) {
-                        unsigned const kk = t + 4 * k;
-                        if (kk < nch) {
-                            float4 const y = ra4[kk];
-                            float4 const x = reg[k];
-                            if (METRIC == METRIC_L2SQR) {
-                                float const dx = y.x - x.x, dy = y.y - x.y, dz = y.z - x.z, dw = y.w - x.w;
-                                p.x = __builtin_fmaf(dx, dx, p.x);
-                                p.y = __builtin_fmaf(dy, dy, p.y);
-                                p.z = __builtin_fmaf(dz, dz, p.z);
-                                p.w = __builtin_fmaf(dw, dw, p.w);
-                            } else {
-                                p.x = __builtin_fmaf(y.x, x.x, p.x);
-                                p.y = __builtin_fmaf(y.y, x.y, p.y);
-                                p.z = __builtin_fmaf(y.z, x.z, p.z);
-                                p.w = __builtin_fmaf(y.w, x.w, p.w);
-                            }
+                        float4 const y = ra4[t + 4 * k];
+                        float4 const x = reg[k];
+                        if (METRIC == METRIC_L2SQR) {
+                            float const dx = y.x - x.x, dy = y.y - x.y, dz = y.z - x.z, dw = y.w - x.w;
+                            p.x = __builtin_fmaf(dx, dx, p.x);
+                            p.y = __builtin_fmaf(dy, dy, p.y);
+                            p.z = __builtin_fmaf(dz, dz, p.z);
+                            p.w = __builtin_fmaf(dw, dw, p.w);
+                        } else {
+                            p.x = __builtin_fmaf(y.x, x.x, p.x);
+                            p.y = __builtin_fmaf(y.y, x.y, p.y);
+                            p.z = __builtin_fmaf(y.z, x.z, p.z);
+                            p.w = __builtin_fmaf(y.w, x.w, p.w);
                         }
                     }
                     acc = quad_reduce(p);
                     float const *rl = reinterpret_cast<float const *>(ra4);
-                    for (unsigned i = a.D & ~15u; i < a.D; ++i) {
+                    for (unsigned i = d16; i < a.D; ++i) {
                         if (METRIC == METRIC_L2SQR) {
                             float const dd = rl[i] - orow[i];
                             acc = __builtin_fmaf(dd, dd, acc);
@@ -646,7 +677,9 @@ __global__ void __launch_bounds__(256) join_tile_kernel(NndArgs a) {
                         }
                     }
                 } else {
-                    acc = quad_metric<METRIC>(ra, orow, a.D, t);
+                    unsigned const ia = ag + r;
+                    uint32_t const av = L.ids[ia < nA ? ia : 0u];
+                    acc = quad_metric<METRIC>(a.X + size_t(av) * a.Dp, orow, a.D, t);
                 }
                 float const na = L.an[r];
                 float const d = finish_metric<METRIC>(acc, na, on);
@@ -658,12 +691,16 @@ __global__ void __launch_bounds__(256) join_tile_kernel(NndArgs a) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 unsigned const ia = ag + t + 4 * i;
-                // new x new once per id pair (i < j by id), new x old always (kgraph.cpp:741-750)
-                bool const valid = ia < nA && ook && (io < nA ? aid[i] < oid : aid[i] != oid);
+                // new x new once per pair of list positions (the reference's
+                // i < j by id takes each id pair once, kgraph.cpp:741-745),
+                // new x old always (:746-748); an id never pairs with itself
+                bool const valid = ia < nA && ook && aid[i] != oid && (io >= nA || ia < io);
                 float const d = dres[i];
                 bool const pa = valid && d < arad[i];
                 bool const po = valid && d < orad;
-                found |= valid && d < akth[i];
+                // the callback's first insert (kgraph.cpp:805-807): into the
+                // smaller id of a new x new pair, into the new row otherwise
+                found |= valid && d < ((io < nA && oid < aid[i]) ? om.y : akth[i]);
                 pairs += valid;
                 pushes += unsigned(pa) + unsigned(po);
                 pm |= (unsigned(pa) << (2 * i)) | (unsigned(po) << (2 * i + 1));
@@ -1256,15 +1293,11 @@ __global__ void __launch_bounds__(256) screen_pairs_kernel(NndArgs a, uint32_t c
 
 }  // namespace
 
+// join_tile_kernel's CH: 4-value chunks per lane in the 16-value part of a
+// row, or -1 when that exceeds 8 (rows read from global memory)
 static int ch_for(unsigned D) {
-    unsigned nch = (D & ~15u) >> 2;
-    unsigned per = (nch + 3) / 4;
-    if (per <= 1) return 1;
-    if (per <= 2) return 2;
-    if (per <= 4) return 4;
-    if (per <= 6) return 6;
-    if (per <= 8) return 8;
-    return 0;
+    unsigned const per = (D & ~15u) >> 4;
+    return per <= 8 ? int(per) : -1;
 }
 
 #define KG_DISPATCH_METRIC(metric, KERNEL, ...)                                       \
@@ -1283,15 +1316,20 @@ hipError_t nnd_launch_init(int metric, NndArgs const &a, hipStream_t s) {
 template <int METRIC>
 static void launch_join_m(NndArgs const &a, hipStream_t s) {
     dim3 g((a.n1 - a.n0 + 3) / 4), b(256);
-    // CH > 0: 16 staged rows of Dp floats per wave
-    size_t const lds = ch_for(a.D) ? size_t(4) * 16 * a.Dp * 4 : 0;
-    switch (ch_for(a.D)) {
+    int const ch = ch_for(a.D);
+    // CH >= 0: 16 staged rows of Dp floats per wave
+    size_t const lds = ch >= 0 ? size_t(4) * 16 * a.Dp * 4 : 0;
+    switch (ch) {
+        case 0: hipLaunchKernelGGL((join_tile_kernel<METRIC, 0>), g, b, lds, s, a); break;
         case 1: hipLaunchKernelGGL((join_tile_kernel<METRIC, 1>), g, b, lds, s, a); break;
         case 2: hipLaunchKernelGGL((join_tile_kernel<METRIC, 2>), g, b, lds, s, a); break;
+        case 3: hipLaunchKernelGGL((join_tile_kernel<METRIC, 3>), g, b, lds, s, a); break;
         case 4: hipLaunchKernelGGL((join_tile_kernel<METRIC, 4>), g, b, lds, s, a); break;
+        case 5: hipLaunchKernelGGL((join_tile_kernel<METRIC, 5>), g, b, lds, s, a); break;
         case 6: hipLaunchKernelGGL((join_tile_kernel<METRIC, 6>), g, b, lds, s, a); break;
+        case 7: hipLaunchKernelGGL((join_tile_kernel<METRIC, 7>), g, b, lds, s, a); break;
         case 8: hipLaunchKernelGGL((join_tile_kernel<METRIC, 8>), g, b, lds, s, a); break;
-        default: hipLaunchKernelGGL((join_tile_kernel<METRIC, 0>), g, b, lds, s, a); break;
+        default: hipLaunchKernelGGL((join_tile_kernel<METRIC, -1>), g, b, lds, s, a); break;
     }
 }
 

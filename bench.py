@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--P", type=str, default="50,100,200")
     ap.add_argument("--Pe", type=str, default="10,20,50,100,200",
                     help="P values for walks started from selected entry points")
-    ap.add_argument("--Se", type=str, default="10,32,64",
+    ap.add_argument("--Se", type=str, default="10,32,64,128",
                     help="S values (neighbours per expansion, SearchParams::S) for entry-started walks")
     ap.add_argument("--entry-sample", type=int, default=None,
                     help="entry-point sample size (default 8192; 65536 for learned-5m-d128, whose 5,000 "

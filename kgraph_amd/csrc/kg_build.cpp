@@ -55,7 +55,7 @@ float eval_accuracy(float const *pool, unsigned np, float const *knn, unsigned n
 
 struct NndState {
     DevBuf<uint64_t> pool, cand, ovf, stats;
-    DevBuf<uint32_t> part, cnt, Mv, found, samp, nnew, nold, rnew, rold, crnew, crold, ccnt;
+    DevBuf<uint32_t> part, cnt, Mv, found, samp, nnew, nold, rnew, rold, crnew, crold, ccnt, csrc;
     DevBuf<float> radiusM;
     DevBuf<float4> meta;
     DevBuf<uint32_t> ctl, ccount;
@@ -229,6 +229,7 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     NndState &st = *stp;
     st.pool.reserve(size_t(N) * p.L);
     st.cand.reserve(size_t(N) * a.C);
+    st.csrc.reserve(size_t(N) * a.C);
     a.O = 64;
     st.ovf.reserve(size_t(N) * a.O);
     st.stats.reserve(NND_NSTATS);
@@ -242,6 +243,7 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
     st.meta.reserve(N);
     a.pool = st.pool.p;
     a.cand = st.cand.p;
+    a.csrc = st.csrc.p;
     a.ovf = st.ovf.p;
     a.stats = st.stats.p;
     a.iter = st.iter.p;

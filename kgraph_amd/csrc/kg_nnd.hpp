@@ -13,6 +13,8 @@
 enum { NND_STAT_PAIRS = 0, NND_STAT_DELTA = 1, NND_STAT_M = 2, NND_STAT_LISTS = 3, NND_STAT_PUSHES = 4,
        NND_STAT_RESCORE = 5, NND_NPART = 6, NND_NSTATS = 8 };
 
+constexpr uint32_t NND_NOSRC = 0xffffffffu;
+
 struct NndArgs {
     float const *X;      // rows, stride Dp floats
     uint16_t const *Xb;  // BF16 copy of the rows (MFMA operands), same stride
@@ -39,6 +41,8 @@ struct NndArgs {
     uint32_t *rnew, *rold;    // N x R reverse lists
     uint32_t *crnew, *crold;  // reverse-list push counters
     uint64_t *cand;           // N x C candidate slots, written once per claim
+    uint32_t *csrc;           // N x C: per slot whose key has its flag bit set, the node whose join
+                              // the candidate credits with `found` when it is new to the list
     uint64_t *ovf;            // N x O overflow slots (~0 = empty), hashed, atomicMin
     unsigned O;
     uint32_t *ccnt;           // candidate slot counters

@@ -177,10 +177,12 @@ __global__ void __launch_bounds__(256) init_kernel(NndArgs a) {
 // A claimed slot (< C) belongs to one candidate: a plain store.  Claims past C
 // go to a hashed slot of the target's overflow area, where a 64-bit atomicMin
 // keeps the better key (and one id always lands on the same slot).
-__device__ __forceinline__ void put_cand(NndArgs const &a, uint32_t target, unsigned slot, uint64_t key) {
+__device__ __forceinline__ void put_cand(NndArgs const &a, uint32_t target, unsigned slot, uint64_t key,
+                                         uint32_t src) {
     if (slot < a.C) {
         a.cand[size_t(target) * a.C + slot] = key;
-    } else {
+        if (src != NND_NOSRC) a.csrc[size_t(target) * a.C + slot] = src;
+    } else {  // (an overflow entry carries no credit)
         unsigned const h = unsigned(mix64((uint64_t(target) << 32) | (uint32_t(key) >> 1))) & (a.O - 1u);  // O: a power of two
         atomicMin(reinterpret_cast<unsigned long long *>(&a.ovf[size_t(target) * a.O + h]),
                   static_cast<unsigned long long>(key));
@@ -211,18 +213,32 @@ typedef __bf16 mj_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float mj_f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr unsigned MT = 512;  // list capacity (new + old)
-constexpr unsigned MB = 512;  // staged candidates per wave
-constexpr unsigned MQ = 512;  // re-score queue per wave (screened form)
+constexpr unsigned MB = 512;  // staged candidates per wave (exact tile forms)
+constexpr unsigned SB = 256;  // staged candidates per wave (screen form: a quarter as many pushes per node)
+// A staged candidate's list index, and whether it credits the joining node's
+// `found`: the insert of the callback's first endpoint (kgraph.cpp:805-807)
+// with its distance below that endpoint's K-th -- the merge keeps the credit
+// only when the id is new to the list (a duplicate returns K + 1 there,
+// kgraph.cpp:186-192).
+constexpr unsigned TGT_MASK = 0x7fffu, TGT_CREDIT = 0x8000u;
 
-template <unsigned Q>
 struct MJoinLDS {
+    static constexpr unsigned kMB = MB;
     uint32_t ids[MT];
     uint32_t cnt[MT];
     float4 am[32];  // per-node records of the new rows whose tiles are in registers (MFMA form)
     float an[16];   // join norms of the current 16 new rows (FP32 tile form)
     uint64_t key[MB];
-    uint32_t rq[Q];  // pair (list index of the new row) | (list index of the other row) << 16
     uint16_t tgt[MB];
+};
+
+struct SJoinLDS {
+    static constexpr unsigned kMB = SB;
+    uint32_t ids[MT];
+    uint32_t cnt[MT];
+    float4 am[16];  // per-node records of the tile's 16 new rows
+    uint64_t key[SB];
+    uint16_t tgt[SB];
 };
 
 // KS k-steps of 32 columns: lane l takes columns 32 s + 8 (l / 16) .. +7 of
@@ -320,15 +336,15 @@ __device__ __forceinline__ float mj_bound(NndArgs const &a, float an, float on, 
 }
 
 template <class LDS>
-__device__ void mj_flush(NndArgs const &a, LDS &L, unsigned fill, unsigned nT) {
+__device__ void mj_flush(NndArgs const &a, LDS &L, unsigned fill, unsigned nT, uint32_t n) {
     unsigned const lane = lane_id();
     for (unsigned i = lane; i < nT; i += KG_WAVE) L.cnt[i] = 0;
     wave_sync();
-    unsigned rank[MB / KG_WAVE];
+    unsigned rank[LDS::kMB / KG_WAVE];
 #pragma unroll
-    for (unsigned j = 0; j < MB / KG_WAVE; ++j) {
+    for (unsigned j = 0; j < LDS::kMB / KG_WAVE; ++j) {
         unsigned const i = lane + j * KG_WAVE;
-        rank[j] = (i < fill) ? atomicAdd(&L.cnt[L.tgt[i]], 1u) : 0u;
+        rank[j] = (i < fill) ? atomicAdd(&L.cnt[L.tgt[i] & TGT_MASK], 1u) : 0u;
     }
     wave_sync();
     for (unsigned i = lane; i < nT; i += KG_WAVE) {
@@ -337,13 +353,16 @@ __device__ void mj_flush(NndArgs const &a, LDS &L, unsigned fill, unsigned nT) {
     }
     wave_sync();
 #pragma unroll
-    for (unsigned j = 0; j < MB / KG_WAVE; ++j) {
+    for (unsigned j = 0; j < LDS::kMB / KG_WAVE; ++j) {
         unsigned const i = lane + j * KG_WAVE;
         if (i < fill) {
-            unsigned const tp = L.tgt[i];
+            // the key's flag bit says whether the slot's credit entry is
+            // written (the merge sets the flag of every entry it admits)
+            unsigned const tv = L.tgt[i], tp = tv & TGT_MASK;
+            bool const cr = (tv & TGT_CREDIT) != 0u;
             uint32_t const target = L.ids[tp];
             uint64_t const key = L.key[i];
-            put_cand(a, target, L.cnt[tp] + rank[j], key);
+            put_cand(a, target, L.cnt[tp] + rank[j], cr ? key : (key & ~1ull), cr ? n : NND_NOSRC);
         }
     }
     wave_sync();
@@ -353,8 +372,8 @@ __device__ void mj_flush(NndArgs const &a, LDS &L, unsigned fill, unsigned nT) {
 // tgt[k]); the wave first counts them and flushes the buffer if they would not
 // fit.  `fill` is wave-uniform; every lane of the wave calls this.
 template <int NP, class LDS>
-__device__ __forceinline__ void mj_stage(NndArgs const &a, LDS &L, unsigned &fill, unsigned nT, unsigned pm,
-                                         uint64_t const (&key)[NP], unsigned const (&tgt)[NP]) {
+__device__ __forceinline__ void mj_stage(NndArgs const &a, LDS &L, unsigned &fill, unsigned nT, uint32_t n,
+                                         unsigned pm, uint64_t const (&key)[NP], unsigned const (&tgt)[NP]) {
     uint64_t bal[NP];
     unsigned tot = 0;
 #pragma unroll
@@ -363,9 +382,9 @@ __device__ __forceinline__ void mj_stage(NndArgs const &a, LDS &L, unsigned &fil
         tot += popc64(bal[k]);
     }
     if (tot == 0) return;
-    if (fill + tot > MB) {
+    if (fill + tot > LDS::kMB) {
         wave_sync();
-        mj_flush(a, L, fill, nT);
+        mj_flush(a, L, fill, nT, n);
         fill = 0;
     }
     uint64_t const lt = (1ull << lane_id()) - 1;
@@ -380,53 +399,19 @@ __device__ __forceinline__ void mj_stage(NndArgs const &a, LDS &L, unsigned &fil
     }
 }
 
-// Exact re-score of the queued pairs, one quad per pair; both rows load in
-// groups of G chunks per lane.  `found` tests the callback's first insert
-// (kgraph.cpp:805-807): into the smaller id of a new x new pair (the join
-// enumerates i < j by id, kgraph.cpp:741-745), into the new row of a new x
-// old pair; list positions below nA hold the new rows.
-template <int METRIC, int G, class LDS>
-__device__ void mj_rescore(NndArgs const &a, LDS &L, unsigned rqn, unsigned nA, unsigned &fill, unsigned nT,
-                           bool &found, unsigned &pushes) {
-    unsigned const quad = lane_id() >> 2, t = lane_id() & 3;
-    wave_sync();
-    for (unsigned b0 = 0; b0 < rqn; b0 += 16) {
-        unsigned const e = b0 + quad;
-        bool const act = e < rqn;
-        uint32_t const pr = act ? L.rq[e] : 0u;
-        unsigned const ia = pr & 0xffffu, io = pr >> 16;
-        uint32_t const av = L.ids[ia], o = L.ids[io];
-        float const acc = quad_metric<METRIC, float, true, G>(a.X + size_t(av) * a.Dp, a.X + size_t(o) * a.Dp, a.D, t);
-        float4 const ma = a.meta[av], mo = a.meta[o];
-        float const na = (METRIC == METRIC_L2SQR) ? 0.f : ma.z;
-        float const no = (METRIC == METRIC_L2SQR) ? 0.f : mo.z;
-        float const d = finish_metric<METRIC>(acc, na, no);
-        bool const lead = act && t == 0;
-        bool const pa = lead && d < ma.x;
-        bool const po = lead && d < mo.x;
-        found |= lead && d < ((io < nA && o < av) ? mo.y : ma.y);
-        pushes += unsigned(pa) + unsigned(po);
-        uint64_t const key[2] = {make_key(d, o, 1), make_key(d, av, 1)};
-        unsigned const tgt[2] = {ia, io};
-        mj_stage<2>(a, L, fill, nT, unsigned(pa) | (unsigned(po) << 1), key, tgt);
-    }
-    wave_sync();
-}
-
-template <int METRIC, int KS, bool EXACT, bool F32>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXACT ? 4 : 3)))
-join_mfma_kernel(NndArgs a) {
-    // 16-row new groups held in registers at once: two for BF16 operands of
-    // up to 128 values (16 VGPRs each), one otherwise
-    constexpr unsigned AG = (!F32 && KS <= 4) ? 2u : 1u;
-    __shared__ MJoinLDS<EXACT ? 1u : MQ> lds[4];
-    unsigned const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    unsigned const n = a.n0 + blockIdx.x * 4 + w;
-    if (n >= a.n1) return;
-    auto &L = lds[w];
+// Fills the wave's list: new ids (sampled, then reverse), old ids (sampled,
+// then reverse), capped at MT (kgraph.cpp:739-751 joins nn_new with itself
+// and with nn_old, reverse lists merged in, :846-866).
+template <class LDS>
+__device__ __forceinline__ void mj_lists(NndArgs const &a, LDS &L, unsigned n, unsigned &nA, unsigned &nT,
+                                         unsigned &nlist) {
+    unsigned const lane = lane_id();
     unsigned const nn = a.nnew[n], no = a.nold[n];
     unsigned const rn = min(a.crnew[n], a.R), ro = min(a.crold[n], a.R);
-    unsigned const nA0 = nn + rn, nB = no + ro, nT = min(nA0 + nB, MT), nA = min(nA0, nT);
+    unsigned const nA0 = nn + rn, nB = no + ro;
+    nT = min(nA0 + nB, MT);
+    nA = min(nA0, nT);
+    nlist = nA0 + nB;
     uint32_t const *samp = a.samp + size_t(n) * a.Lc;
     for (unsigned i = lane; i < nT; i += KG_WAVE) {
         uint32_t v;
@@ -441,16 +426,51 @@ join_mfma_kernel(NndArgs a) {
         L.ids[i] = v;
     }
     wave_sync();
+}
+
+// Per-node statistics of a join wave.  The node's `found` (kgraph.cpp:801,
+// 813) starts false here; the merge that follows sets it when one of the
+// node's credited candidates turns out to be new to its list.
+__device__ __forceinline__ void mj_finish(NndArgs const &a, unsigned n, unsigned pairs, unsigned pushes,
+                                          unsigned nlist, unsigned rescored) {
+    unsigned const lane = lane_id();
+    for (int o = 32; o > 0; o >>= 1) {
+        pairs += __shfl_xor(pairs, o);
+        pushes += __shfl_xor(pushes, o);
+        rescored += __shfl_xor(rescored, o);
+    }
+    if (lane == 0) {
+        a.found[n] = 0u;
+        a.part[NND_STAT_PAIRS * a.N + n] = pairs;
+        a.part[NND_STAT_PUSHES * a.N + n] = pushes;
+        a.part[NND_STAT_LISTS * a.N + n] = nlist;
+        a.part[NND_STAT_RESCORE * a.N + n] = rescored;
+    }
+}
+
+// The exact BF16 form (integer rows, see above): the tile value is the
+// canonical distance and decides alone.
+template <int METRIC, int KS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+join_mfma_kernel(NndArgs a) {
+    // 16-row new groups held in registers at once: two for operands of up to
+    // 128 values (16 VGPRs each), one otherwise
+    constexpr unsigned AG = KS <= 4 ? 2u : 1u;
+    __shared__ MJoinLDS lds[4];
+    unsigned const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned const n = a.n0 + blockIdx.x * 4 + w;
+    if (n >= a.n1) return;
+    auto &L = lds[w];
+    unsigned nA, nT, nlist;
+    mj_lists(a, L, n, nA, nT, nlist);
     unsigned const r16 = lane & 15, mrow = (lane >> 4) * 4;
-    unsigned fill = 0, rqn = 0, pairs = 0, pushes = 0, rescored = 0;
-    bool found = false;
-    constexpr int RG = KS >= 8 ? 4 : 6;  // re-score load group
+    unsigned fill = 0, pairs = 0, pushes = 0;
     for (unsigned ag = 0; ag < nA; ag += 16 * AG) {
         // AG groups of 16 list rows from ag (new rows, and old ones when a
         // group runs past the new rows: their pairs are masked below) stay in
         // registers while every later tile streams past them once; a tile
         // that is one of the groups is taken from its registers
-        MjOps<KS, F32> aop[AG];
+        MjOps<KS, false> aop[AG];
         uint32_t aid[AG][4];
 #pragma unroll
         for (unsigned g = 0; g < AG; ++g) {
@@ -475,7 +495,7 @@ join_mfma_kernel(NndArgs a) {
             bool const ook = io < nT;
             uint32_t const oid = ook ? L.ids[io] : 0u;
             unsigned const dg = (tg - ag) >> 4;  // wave-uniform: the tile is A group dg when dg < AG
-            MjOps<KS, F32> bop;
+            MjOps<KS, false> bop;
             if (dg == 0)
                 bop = aop[0];
             else if (AG > 1 && dg == 1)
@@ -514,53 +534,184 @@ join_mfma_kernel(NndArgs a) {
                     bool const valid = ia < nA && ook && gid[j] != oid && (io >= nA || ia < io);
                     float const d = mj_dist<METRIC>(acc[j], am.z, om.z);
                     pairs += valid;
-                    if (EXACT) {
-                        bool const pa = valid && d < am.x;
-                        bool const po = valid && d < om.x;
-                        // the callback's first insert (kgraph.cpp:805-807): into the
-                        // smaller id of a new x new pair, into the new row otherwise
-                        found |= valid && d < ((io < nA && oid < gid[j]) ? om.y : am.y);
-                        pushes += unsigned(pa) + unsigned(po);
-                        pm |= (unsigned(pa) << (2 * j)) | (unsigned(po) << (2 * j + 1));
-                        key[2 * j] = make_key(d, oid, 1);
-                        key[2 * j + 1] = make_key(d, gid[j], 1);
-                        tgt[2 * j] = ia;
-                        tgt[2 * j + 1] = io;
-                    } else {
-                        float const lo = d - mj_bound<METRIC>(a, am.z, om.z, am.w, om.w);
-                        bool const pass = valid && (lo < am.x || lo < om.x);
-                        uint64_t const b = ballot(pass);
-                        if (pass) L.rq[rqn + popc64(b & ((1ull << lane) - 1))] = ia | (io << 16);
-                        rqn += popc64(b);
-                    }
+                    bool const pa = valid && d < am.x;
+                    bool const po = valid && d < om.x;
+                    // the callback's first endpoint (kgraph.cpp:741-750, 805-807):
+                    // the smaller id of a new x new pair, the new row otherwise
+                    bool const bfirst = io < nA && oid < gid[j];
+                    pushes += unsigned(pa) + unsigned(po);
+                    pm |= (unsigned(pa) << (2 * j)) | (unsigned(po) << (2 * j + 1));
+                    key[2 * j] = make_key(d, oid, 1);
+                    key[2 * j + 1] = make_key(d, gid[j], 1);
+                    tgt[2 * j] = ia | ((!bfirst && d < am.y) ? TGT_CREDIT : 0u);
+                    tgt[2 * j + 1] = io | ((bfirst && d < om.y) ? TGT_CREDIT : 0u);
                 }
-                if (EXACT) mj_stage<8>(a, L, fill, nT, pm, key, tgt);
-                if (!EXACT && rqn > MQ - 4 * KG_WAVE) {
-                    mj_rescore<METRIC, RG>(a, L, rqn, nA, fill, nT, found, pushes);
-                    rescored += rqn;
-                    rqn = 0;
-                }
+                mj_stage<8>(a, L, fill, nT, n, pm, key, tgt);
             }
         }
     }
-    if (!EXACT && rqn) {
-        mj_rescore<METRIC, RG>(a, L, rqn, nA, fill, nT, found, pushes);
-        rescored += rqn;
+    wave_sync();
+    if (fill) mj_flush(a, L, fill, nT, n);
+    mj_finish(a, n, pairs, pushes, nlist, 0u);
+}
+
+// The screen forms (BF16 operands, or FP32 operands on v_mfma_f32_16x16x4f32):
+// the MFMA tile value minus its error bound E (mj_bound) decides which pairs
+// could pass the exact test; only those are scored exactly, and the exact
+// value alone is compared and stored.  The tile's 16 new rows are staged in
+// LDS as FP32 (16 Dp floats per wave, dynamic shared memory); after a tile's
+// screen, quad q takes list row tg + q when any of its pairs passed, loads
+// that row's FP32 values once (lane t: 4-value chunks t, t + 4, ... of the
+// canonical order, kg_device.hpp), and scores it against each passing new row
+// read from LDS (the same chunks, one broadcast read per chunk), then the
+// last D % 16 values one by one.  Chunk slots past the row's (D & ~15) / 4
+// chunks hold zeros in both operands (they add fmaf(0, 0, p) = p), so the
+// loads carry no per-chunk conditions.  CHM = chunk slots per lane.
+template <int METRIC, int KS, bool F32>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((F32 ? KS : 2 * KS) > 8 ? 2 : 3)))
+join_screen_kernel(NndArgs a) {
+    constexpr int CHM = F32 ? KS : 2 * KS;
+    __shared__ SJoinLDS lds[4];
+    extern __shared__ __align__(16) float4 arows[];
+    unsigned const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned const n = a.n0 + blockIdx.x * 4 + w;
+    if (n >= a.n1) return;
+    auto &L = lds[w];
+    unsigned nA, nT, nlist;
+    mj_lists(a, L, n, nA, nT, nlist);
+    unsigned const r16 = lane & 15, mrow = (lane >> 4) * 4;
+    unsigned const quad = lane >> 2, t = lane & 3;
+    unsigned const dp4 = a.Dp / 4;
+    unsigned const nch = (a.D & ~15u) >> 2, d16 = a.D & ~15u;
+    float4 *const As = arows + size_t(w) * 16 * dp4;
+    unsigned fill = 0, pairs = 0, pushes = 0, rescored = 0;
+    for (unsigned ag = 0; ag < nA; ag += 16) {
+        MjOps<KS, F32> aop;
+        {
+            unsigned const ia = ag + r16;
+            aop.load(a, ia < nT ? L.ids[ia] : 0u, ia < nT, lane);
+        }
+        uint32_t aid[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            unsigned const ib = ag + mrow + j;
+            aid[j] = ib < nA ? L.ids[ib] : 0u;
+        }
+        wave_sync();  // the previous group's rows and records are no longer read
+        if (lane < 16) {
+            unsigned const ia = ag + lane;
+            L.am[lane] = ia < nA ? a.meta[L.ids[ia]] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        for (unsigned e = lane; e < 16 * dp4; e += KG_WAVE) {
+            unsigned const r = e / dp4, c = e - r * dp4;
+            unsigned const ia = ag + r;
+            As[e] = ia < nA ? reinterpret_cast<float4 const *>(a.X + size_t(L.ids[ia]) * a.Dp)[c]
+                            : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        wave_sync();
+        for (unsigned tg = ag; tg < nT; tg += 16) {
+            unsigned const io = tg + r16;
+            bool const ook = io < nT;
+            uint32_t const oid = ook ? L.ids[io] : 0u;
+            MjOps<KS, F32> bop;
+            if (tg == ag)  // wave-uniform: the tile is the A group
+                bop = aop;
+            else
+                bop.load(a, oid, ook, lane);
+            float4 const om = a.meta[oid];
+            mj_f32x4 const acc = mj_mma(aop, bop);
+            // the screen: bit j of lane l passes pair (new row mrow + j, column r16)
+            uint64_t bj[4];
+            unsigned mb = mrow;
+            asm volatile("" : "+v"(mb));
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                unsigned const ia = ag + mrow + j;
+                float4 const am = L.am[mb + j];
+                // new x new once per pair of list positions, new x old always
+                // (kgraph.cpp:741-748); an id never pairs with itself
+                bool const valid = ia < nA && ook && aid[j] != oid && (io >= nA || ia < io);
+                pairs += valid;
+                float const d = mj_dist<METRIC>(acc[j], am.z, om.z);
+                float const lo = d - mj_bound<METRIC>(a, am.z, om.z, am.w, om.w);
+                bj[j] = ballot(valid && (lo < am.x || lo < om.x));
+            }
+            if (!(bj[0] | bj[1] | bj[2] | bj[3])) continue;  // wave-uniform
+            // quad q's column: its passing new rows 4 g + j sit in bit q + 16 g of bj[j]
+            unsigned mq = 0;
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) mq |= unsigned((bj[j] >> (quad + 16 * g)) & 1ull) << (4 * g + j);
+            unsigned const cio = tg + quad;  // the quad's list row (valid whenever mq != 0)
+            uint32_t const cid = mq ? L.ids[cio] : 0u;
+            float const *crow = a.X + size_t(cid) * a.Dp;
+            float4 reg[CHM];
+#pragma unroll
+            for (int k = 0; k < CHM; ++k) {
+                unsigned const kk = t + 4 * k;
+                bool const ok = kk < nch;
+                float4 const v = reinterpret_cast<float4 const *>(crow)[ok ? kk : 0u];
+                reg[k] = ok ? v : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            float4 const cm = a.meta[cid];
+            float const cn = (METRIC == METRIC_L2SQR) ? 0.f : cm.z;
+            if (t == 0) rescored += __builtin_popcount(mq);
+            while (ballot(mq != 0)) {
+                bool const act = mq != 0;
+                unsigned const r = act ? unsigned(__builtin_ctz(mq)) : 0u;
+                mq &= mq - 1u;
+                float4 const *ra4 = As + r * dp4;
+                float4 pp = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                for (int k = 0; k < CHM; ++k) {
+                    unsigned const kk = t + 4 * k;
+                    bool const ok = kk < nch;
+                    float4 y = ra4[ok ? kk : 0u];
+                    if (!ok) y = make_float4(0.f, 0.f, 0.f, 0.f);
+                    float4 const x = reg[k];
+                    if (METRIC == METRIC_L2SQR) {
+                        float const dx = y.x - x.x, dy = y.y - x.y, dz = y.z - x.z, dw = y.w - x.w;
+                        pp.x = __builtin_fmaf(dx, dx, pp.x);
+                        pp.y = __builtin_fmaf(dy, dy, pp.y);
+                        pp.z = __builtin_fmaf(dz, dz, pp.z);
+                        pp.w = __builtin_fmaf(dw, dw, pp.w);
+                    } else {
+                        pp.x = __builtin_fmaf(y.x, x.x, pp.x);
+                        pp.y = __builtin_fmaf(y.y, x.y, pp.y);
+                        pp.z = __builtin_fmaf(y.z, x.z, pp.z);
+                        pp.w = __builtin_fmaf(y.w, x.w, pp.w);
+                    }
+                }
+                float acc2 = quad_reduce(pp);
+                float const *rl = reinterpret_cast<float const *>(ra4);
+                for (unsigned i = d16; i < a.D; ++i) {
+                    if (METRIC == METRIC_L2SQR) {
+                        float const dd = rl[i] - crow[i];
+                        acc2 = __builtin_fmaf(dd, dd, acc2);
+                    } else {
+                        acc2 = __builtin_fmaf(rl[i], crow[i], acc2);
+                    }
+                }
+                float4 const am = L.am[r];
+                uint32_t const rid = L.ids[ag + r];
+                float const na = (METRIC == METRIC_L2SQR) ? 0.f : am.z;
+                float const d = finish_metric<METRIC>(acc2, na, cn);
+                bool const lead = act && t == 0;
+                bool const pa = lead && d < am.x;
+                bool const po = lead && d < cm.x;
+                bool const bfirst = cio < nA && cid < rid;  // the callback's first endpoint is the list row
+                pushes += unsigned(pa) + unsigned(po);
+                uint64_t const key[2] = {make_key(d, cid, 1), make_key(d, rid, 1)};
+                unsigned const tgt[2] = {(ag + r) | ((!bfirst && d < am.y) ? TGT_CREDIT : 0u),
+                                         cio | ((bfirst && d < cm.y) ? TGT_CREDIT : 0u)};
+                mj_stage<2>(a, L, fill, nT, n, unsigned(pa) | (unsigned(po) << 1), key, tgt);
+            }
+        }
     }
     wave_sync();
-    if (fill) mj_flush(a, L, fill, nT);
-    uint64_t const fb = ballot(found);
-    for (int o = 32; o > 0; o >>= 1) {
-        pairs += __shfl_xor(pairs, o);
-        pushes += __shfl_xor(pushes, o);
-    }
-    if (lane == 0) {
-        a.found[n] = fb ? 1u : 0u;
-        a.part[NND_STAT_PAIRS * a.N + n] = pairs;
-        a.part[NND_STAT_PUSHES * a.N + n] = pushes;
-        a.part[NND_STAT_LISTS * a.N + n] = nA0 + nB;
-        a.part[NND_STAT_RESCORE * a.N + n] = rescored;
-    }
+    if (fill) mj_flush(a, L, fill, nT, n);
+    mj_finish(a, n, pairs, pushes, nlist, rescored);
 }
 
 // The same tiles in exact FP32: each quad holds one list row of the tile in
@@ -577,7 +728,7 @@ join_mfma_kernel(NndArgs a) {
 // more than 143 values) reads both rows from global memory (quad_metric).
 template <int METRIC, int CH>
 __global__ void __launch_bounds__(256) join_tile_kernel(NndArgs a) {
-    __shared__ MJoinLDS<1> lds[4];
+    __shared__ MJoinLDS lds[4];
     extern __shared__ __align__(16) float4 tile_rows[];
     unsigned const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     unsigned const n = a.n0 + blockIdx.x * 4 + w;
@@ -605,7 +756,6 @@ __global__ void __launch_bounds__(256) join_tile_kernel(NndArgs a) {
     unsigned const d16 = a.D & ~15u;  // values covered by the chunk loop
     float4 *const As = tile_rows + size_t(w) * 16 * dp4;  // CH >= 0: the tile's new rows
     unsigned fill = 0, pairs = 0, pushes = 0;
-    bool found = false;
     for (unsigned ag = 0; ag < nA; ag += 16) {
         if (CH >= 0) {
             wave_sync();  // the previous tile's reads are done
@@ -698,34 +848,23 @@ __global__ void __launch_bounds__(256) join_tile_kernel(NndArgs a) {
                 float const d = dres[i];
                 bool const pa = valid && d < arad[i];
                 bool const po = valid && d < orad;
-                // the callback's first insert (kgraph.cpp:805-807): into the
-                // smaller id of a new x new pair, into the new row otherwise
-                found |= valid && d < ((io < nA && oid < aid[i]) ? om.y : akth[i]);
+                // the callback's first endpoint (kgraph.cpp:741-750, 805-807):
+                // the smaller id of a new x new pair, the new row otherwise
+                bool const bfirst = io < nA && oid < aid[i];
                 pairs += valid;
                 pushes += unsigned(pa) + unsigned(po);
                 pm |= (unsigned(pa) << (2 * i)) | (unsigned(po) << (2 * i + 1));
                 key[2 * i] = make_key(d, oid, 1);
                 key[2 * i + 1] = make_key(d, aid[i], 1);
-                tgt[2 * i] = ia;
-                tgt[2 * i + 1] = io;
+                tgt[2 * i] = ia | ((!bfirst && d < akth[i]) ? TGT_CREDIT : 0u);
+                tgt[2 * i + 1] = io | ((bfirst && d < om.y) ? TGT_CREDIT : 0u);
             }
-            mj_stage<8>(a, L, fill, nT, pm, key, tgt);
+            mj_stage<8>(a, L, fill, nT, n, pm, key, tgt);
         }
     }
     wave_sync();
-    if (fill) mj_flush(a, L, fill, nT);
-    uint64_t const fb = ballot(found);
-    for (int o = 32; o > 0; o >>= 1) {
-        pairs += __shfl_xor(pairs, o);
-        pushes += __shfl_xor(pushes, o);
-    }
-    if (lane == 0) {
-        a.found[n] = fb ? 1u : 0u;
-        a.part[NND_STAT_PAIRS * a.N + n] = pairs;
-        a.part[NND_STAT_PUSHES * a.N + n] = pushes;
-        a.part[NND_STAT_LISTS * a.N + n] = nA0 + nB;
-        a.part[NND_STAT_RESCORE * a.N + n] = 0;
-    }
+    if (fill) mj_flush(a, L, fill, nT, n);
+    mj_finish(a, n, pairs, pushes, nA0 + nB, 0u);
 }
 
 // Row norms for the MFMA join: (|x|^2 in canonical order, its square root),
@@ -1001,11 +1140,17 @@ __global__ void __launch_bounds__(256) merge_kernel(NndArgs a) {
     // memory)
     constexpr int MR = NND_MERGE_MAX / KG_WAVE;
     uint64_t held[MR];
+    uint32_t hsrc[MR];  // the credit entries of the claimed slots, loaded with them (read only for flagged keys)
+    uint32_t const *csrc = a.csrc + size_t(n) * a.C;
 #pragma unroll
     for (int r = 0; r < MR; ++r) {
         unsigned const j = unsigned(r) * KG_WAVE + lane;
         held[r] = ~0ull;
-        if (unsigned(r) * KG_WAVE < m && j < m) held[r] = (j < mall) ? cand[j] : ovf[j - mall];
+        hsrc[r] = NND_NOSRC;
+        if (unsigned(r) * KG_WAVE < m && j < m) {
+            held[r] = (j < mall) ? cand[j] : ovf[j - mall];
+            if (j < mall) hsrc[r] = csrc[j];
+        }
     }
     constexpr int PR = 4;  // pool keys loaded up front (the first 256); a longer pool reads the rest after
     uint64_t pr[PR];
@@ -1035,6 +1180,15 @@ __global__ void __launch_bounds__(256) merge_kernel(NndArgs a) {
 #pragma unroll
     for (int r = 0; r < MR; ++r)
         if (unsigned(r) * KG_WAVE < m && held[r] != ~0ull && mg_insert(tab, key_id(held[r]))) fresh_mask |= 1ull << r;
+    // a fresh candidate from a claimed slot credits the node whose join sent
+    // it, when that join marked it (flag bit set, source in csrc): the
+    // reference's `found` (kgraph.cpp:805-813), an insert into the first K
+    // that is no duplicate
+#pragma unroll
+    for (int r = 0; r < MR; ++r) {
+        unsigned const j = unsigned(r) * KG_WAVE + lane;
+        if (((fresh_mask >> r) & 1u) && j < mall && key_flag(held[r])) a.found[hsrc[r]] = 1u;
+    }
     if (over) {  // the overflow area was used: empty it for the next join
         wave_sync();
         for (unsigned i = lane; i < a.O; i += KG_WAVE) ovf[i] = ~0ull;
@@ -1047,7 +1201,7 @@ __global__ void __launch_bounds__(256) merge_kernel(NndArgs a) {
         if (unsigned(r) * KG_WAVE < m) {
             bool const f = (fresh_mask >> r) & 1u;
             uint64_t const b = ballot(f);
-            if (f) stage[u + popc64(b & lt)] = held[r];
+            if (f) stage[u + popc64(b & lt)] = held[r] | 1ull;  // admitted entries are new (flag set)
             u += popc64(b);
         }
     }
@@ -1333,45 +1487,59 @@ static void launch_join_m(NndArgs const &a, hipStream_t s) {
     }
 }
 
-template <int METRIC, bool EXACT>
-static void launch_join_mfma_e(NndArgs const &a, hipStream_t s) {
+template <int METRIC>
+static void launch_join_exact(NndArgs const &a, hipStream_t s) {
     dim3 g((a.n1 - a.n0 + 3) / 4), b(256);
     unsigned const ks = (a.D + 31) / 32;  // 32-column MFMA k-steps
     if (ks <= 1)
-        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 1, EXACT, false>), g, b, 0, s, a);
+        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 1>), g, b, 0, s, a);
     else if (ks <= 2)
-        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 2, EXACT, false>), g, b, 0, s, a);
+        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 2>), g, b, 0, s, a);
     else if (ks <= 4)
-        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 4, EXACT, false>), g, b, 0, s, a);
+        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 4>), g, b, 0, s, a);
     else
-        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 8, EXACT, false>), g, b, 0, s, a);
+        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 8>), g, b, 0, s, a);
 }
 
-// FP32-operand screen: KS float4 chunks of 16 columns per lane
+// The screen forms: 16 staged FP32 rows of Dp floats per wave.  BF16: KS
+// 32-column k-steps; FP32 operands: KS float4 chunks of 16 columns per lane.
 template <int METRIC>
-static void launch_join_mfma_f32(NndArgs const &a, hipStream_t s) {
+static void launch_join_screen(NndArgs const &a, hipStream_t s) {
     dim3 g((a.n1 - a.n0 + 3) / 4), b(256);
-    unsigned const kq = (a.D + 15) / 16;
-    if (kq <= 2)
-        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 2, false, true>), g, b, 0, s, a);
-    else if (kq <= 4)
-        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 4, false, true>), g, b, 0, s, a);
-    else if (kq <= 6)
-        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 6, false, true>), g, b, 0, s, a);
-    else if (kq <= 8)
-        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 8, false, true>), g, b, 0, s, a);
+    size_t const lds = size_t(4) * 16 * a.Dp * 4;
+    if (a.f32) {
+        unsigned const kq = (a.D + 15) / 16;
+        if (kq <= 2)
+            hipLaunchKernelGGL((join_screen_kernel<METRIC, 2, true>), g, b, lds, s, a);
+        else if (kq <= 4)
+            hipLaunchKernelGGL((join_screen_kernel<METRIC, 4, true>), g, b, lds, s, a);
+        else if (kq <= 6)
+            hipLaunchKernelGGL((join_screen_kernel<METRIC, 6, true>), g, b, lds, s, a);
+        else if (kq <= 8)
+            hipLaunchKernelGGL((join_screen_kernel<METRIC, 8, true>), g, b, lds, s, a);
+        else
+            hipLaunchKernelGGL((join_screen_kernel<METRIC, 16, true>), g, b, lds, s, a);
+        return;
+    }
+    unsigned const ks = (a.D + 31) / 32;
+    if (ks <= 1)
+        hipLaunchKernelGGL((join_screen_kernel<METRIC, 1, false>), g, b, lds, s, a);
+    else if (ks <= 2)
+        hipLaunchKernelGGL((join_screen_kernel<METRIC, 2, false>), g, b, lds, s, a);
+    else if (ks <= 3)
+        hipLaunchKernelGGL((join_screen_kernel<METRIC, 3, false>), g, b, lds, s, a);
+    else if (ks <= 4)
+        hipLaunchKernelGGL((join_screen_kernel<METRIC, 4, false>), g, b, lds, s, a);
     else
-        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 16, false, true>), g, b, 0, s, a);
+        hipLaunchKernelGGL((join_screen_kernel<METRIC, 8, false>), g, b, lds, s, a);
 }
 
 template <int METRIC>
 static void launch_join_mfma(NndArgs const &a, hipStream_t s) {
-    if (a.f32)
-        launch_join_mfma_f32<METRIC>(a, s);
-    else if (a.exact)
-        launch_join_mfma_e<METRIC, true>(a, s);
+    if (a.exact && !a.f32)
+        launch_join_exact<METRIC>(a, s);
     else
-        launch_join_mfma_e<METRIC, false>(a, s);
+        launch_join_screen<METRIC>(a, s);
 }
 
 hipError_t nnd_launch_sqn(NndArgs const &a, float2 *out, uint32_t *flags, hipStream_t s) {

@@ -30,10 +30,14 @@ def _gpu(X, metric):
     return kgraph_amd.KGraph(X, "angular" if metric == ANG else "euclidean", l2sqr=(metric == L2))
 
 
+# Each build runs the same number of iterations (recall=1 and delta=0 keep
+# the stop rule from firing): where the rule stops on its 100 control nodes is
+# noisy, and a build stopped one iteration earlier holds a lower recall.
 @pytest.mark.parametrize("kind,metric,d,params", [
-    ("uniform", EUC, 16, dict(K=25, L=100, S=10, R=100, iterations=30)),
-    ("intmix", L2, 128, dict(K=20, L=40, S=10, R=100, iterations=15)),
-    ("angular", ANG, 96, dict(K=16, L=32, S=10, R=100, iterations=15)),
+    ("uniform", EUC, 16, dict(K=25, L=100, S=10, R=100, iterations=5, recall=1.0, delta=0.0)),
+    ("intmix", L2, 128, dict(K=20, L=40, S=10, R=100, iterations=4, recall=1.0, delta=0.0)),
+    ("angular", ANG, 96, dict(K=16, L=32, S=10, R=100, iterations=9, recall=1.0, delta=0.0)),
+    ("uniform", EUC, 16, dict(K=25, L=100, S=10, R=100, iterations=30)),  # the stop rule as it is
 ])
 def test_build_recall_matches_oracle(kind, metric, d, params):
     X = _data(kind, 20000, d)
@@ -52,7 +56,16 @@ def test_build_recall_matches_oracle(kind, metric, d, params):
     off, ids, dd, M = g.graph()
     g_rec = _knn_recall(off[rows], dd, gt_d, K)
     print(f"{kind}: oracle recall {r_rec:.4f} ({rinfo.iterations} it), gpu recall {g_rec:.4f} ({info.iterations} it)")
-    assert g_rec >= r_rec - 0.005
+    if "recall" in params:
+        # at most 0.5 pt below the oracle; above it by at most 2 pt: the GPU's
+        # `found` reads the K-th distance at the start of the iteration, where
+        # the reference's insert position reads the live list (kgraph.cpp:
+        # 805-807), so its M can grow a little sooner -- but not the way a
+        # `found` that counted duplicates grew it (29 pt at K = 100)
+        assert info.iterations == rinfo.iterations == params["iterations"]
+        assert r_rec - 0.005 <= g_rec <= r_rec + 0.02
+    else:
+        assert g_rec >= r_rec - 0.0075 and info.iterations >= rinfo.iterations - 1
     # invariants: sorted, no self, no duplicate ids, M in [1, L]; canonical distances
     L = params["L"]
     for i in range(0, N, 97):
@@ -65,6 +78,40 @@ def test_build_recall_matches_oracle(kind, metric, d, params):
             assert v == kgo.dist(metric, X[i], X[j])
     tr = g.build_trace()
     assert len(tr) == info.iterations and tr[-1, 0] >= tr[0, 0]
+
+
+@pytest.mark.parametrize("form", ["fp32", "bf16-exact"])
+def test_build_recall_at_the_c2_parameters(form):
+    """The 1M x 128 integer-mixture workload's generator and IndexParams (K=100,
+    L=150, S=10, R=100, 15 iterations) at 60,000 rows: recall@100 of the GPU
+    build (plain FP32 tiles, and the BF16 MFMA form that is exact on these
+    rows) against exact K-NN is within 0.5 pt of the oracle's build.  Reverse
+    edges are left out (reverse=0) so that every list holds its L nearest
+    found, as recall@K reads them; reverse-edge completion is tested on its
+    own against the oracle."""
+    from kgraph_amd import PRECISION_BF16, PRECISION_FP32, datagen
+    X = datagen.integer_mixture(60000, 128, 1000, 3, 4)
+    N, K = len(X), 100
+    params = dict(K=K, L=150, S=10, R=100, iterations=15, controls=100, reverse=0)
+    rows = np.arange(0, N, 60)
+    _, gt_d = kgo.linear_knn(X, rows, K, L2)
+    ref = kgo.Index()
+    rinfo = ref.build(X, L2, **params)
+    roff, rids, rd, rM = ref.export()
+    r_rec = _knn_recall(roff[rows], rd, gt_d, K)
+    g = _gpu(X, L2)
+    info = g.build(precision=PRECISION_BF16 if form.startswith("bf16") else PRECISION_FP32, **params)
+    assert g.build_counters()["join_form"] == form
+    off, ids, dd, M = g.graph()
+    g_rec = _knn_recall(off[rows], dd, gt_d, K)
+    print(f"C2 parameters, {form}: oracle recall@100 {r_rec:.4f} ({rinfo.iterations} it), "
+          f"gpu {g_rec:.4f} ({info.iterations} it)")
+    assert r_rec - 0.005 <= g_rec <= r_rec + 0.02
+    for i in range(0, N, 997):
+        lst, dl = ids[off[i]:off[i + 1]], dd[off[i]:off[i + 1]]
+        assert i not in lst and len(np.unique(lst)) == len(lst) and (np.diff(dl) >= 0).all()
+        for j, v in zip(lst, dl):
+            assert v == kgo.dist(L2, X[i], X[j])
 
 
 @pytest.mark.parametrize("kind,metric,d,form", [

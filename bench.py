@@ -726,12 +726,34 @@ def learned_workload(args):
     log("data generated in %.1fs" % (time.time() - t0))
     g = kgraph_amd.KGraph(X, "euclidean", l2sqr=True)
     _lib.lib().kg_set_verbosity(0)
-    tb = time.perf_counter()
-    info = g.build(K=25, L=100, iterations=args.iterations, reverse=-1, precision=_lib.PRECISION_BF16)
+    # the plain FP32 GPU form (exact FP32 tiles) first, on the same handle,
+    # then the BF16 form whose graph the rest of the workload uses
+    # two builds of each form: where the stop rule fires varies from build to
+    # build on these separated clusters (9 iterations, or all 15 when a control
+    # node stays outside its cluster, as the reference's NN-descent does), so
+    # the forms are compared per iteration
+    builds = {}
+    for form, prec in (("fp32", _lib.PRECISION_FP32), ("bf16", _lib.PRECISION_BF16)):
+        runs = []
+        for rep_b in range(2):
+            devmem.synchronize()
+            tb = time.perf_counter()
+            info = g.build(K=25, L=100, iterations=args.iterations, reverse=-1, precision=prec)
+            devmem.synchronize()
+            bs = time.perf_counter() - tb
+            c = g.build_counters()
+            its = g.build_iterations()
+            runs.append({"build_s": round(bs, 4), "iterations": int(info.iterations),
+                         "ms_per_iteration": round(1e3 * bs / max(int(info.iterations), 1), 2),
+                         "control_recall": round(float(info.recall), 4), "join_form": c["join_form"],
+                         "iteration_join_forms": "".join("TSEPF"[int(f)] for f in its[:, 3]),
+                         "pairs": c["pairs"], "rescored": c["rescored"]})
+            log("build %s: %.2fs (%d it, control recall %.4f, join %s)" % (form, bs, info.iterations, info.recall,
+                                                                           runs[-1]["iteration_join_forms"]))
+        builds[form] = runs
     g._h.call("kg_set_precision", _lib.PRECISION_FP32)
-    build_s = time.perf_counter() - tb
+    build_s = builds["bf16"][-1]["build_s"]
     g.set_entry_sample(args.entry_sample, seed=1998)
-    log("build %.2fs (%d it, control recall %.4f)" % (build_s, info.iterations, info.recall))
     gt_i = np.zeros((nq, K), np.uint32)
     gt_d = np.zeros((nq, K), np.float32)
     gt_c = np.zeros(nq, np.uint32)
@@ -830,6 +852,10 @@ def learned_workload(args):
         "config": {"workload": "aniso-5m-d128-learned", "N": n, "D": 128, "queries": nq,
                    "build": {"K": 25, "L": 100, "reverse": -1, "iterations": args.iterations},
                    "projector": "128-256-32", "triplets": steps * 4096, "entry_sample": args.entry_sample},
+        "build_forms": builds,
+        "build_speedup_bf16_vs_fp32_per_iteration": round(
+            float(np.median([r["ms_per_iteration"] for r in builds["fp32"]])) /
+            float(np.median([r["ms_per_iteration"] for r in builds["bf16"]])), 3),
         "build_s": round(build_s, 3), "train_s": round(train_s, 3),
         "heldout_loss_before_after": [round(loss0, 5), round(loss1, 5)],
         "train_loss_first_last": [round(float(trace[:10].mean()), 5), round(float(trace[-10:].mean()), 5)],

@@ -114,6 +114,41 @@ def test_build_recall_at_the_c2_parameters(form):
             assert v == kgo.dist(L2, X[i], X[j])
 
 
+@pytest.mark.parametrize("form", ["fp32", "bf16-screen"])
+def test_build_recall_at_the_c3_parameters(form):
+    """The 1.2M x 96 angular workload's generator (2,000 unit centres,
+    sigma = 0.3 per coordinate, L2-normalised rows) and IndexParams (K=64,
+    L=128) at 60,000 rows, 8 iterations each (recall=1 and delta=0 keep the
+    stop rule from firing): recall@64 of the GPU build -- plain FP32 tiles, and
+    the BF16 screen with its exact FP32 re-score -- is within 0.5 pt of the
+    oracle's build (at most 2 pt above it), every stored distance canonical."""
+    from kgraph_amd import PRECISION_BF16, PRECISION_FP32, datagen
+    X = datagen.angular_mixture(60000, 96, 2000, 6, 7, sigma=0.3)
+    N, K = len(X), 64
+    params = dict(K=K, L=128, S=10, R=100, iterations=8, controls=100, reverse=0, recall=1.0, delta=0.0)
+    rows = np.arange(0, N, 60)
+    _, gt_d = kgo.linear_knn(X, rows, K, ANG)
+    ref = kgo.Index()
+    rinfo = ref.build(X, ANG, **params)
+    roff, rids, rd, rM = ref.export()
+    r_rec = _knn_recall(roff[rows], rd, gt_d, K)
+    g = _gpu(X, ANG)
+    info = g.build(precision=PRECISION_BF16 if form.startswith("bf16") else PRECISION_FP32,
+                   join="screen" if form.startswith("bf16") else None, **params)
+    assert g.build_counters()["join_form"] == form
+    off, ids, dd, M = g.graph()
+    g_rec = _knn_recall(off[rows], dd, gt_d, K)
+    print(f"C3 parameters, {form}: oracle recall@64 {r_rec:.4f} ({rinfo.iterations} it), "
+          f"gpu {g_rec:.4f} ({info.iterations} it)")
+    assert info.iterations == rinfo.iterations == 8
+    assert r_rec - 0.005 <= g_rec <= r_rec + 0.02
+    for i in range(0, N, 997):
+        lst, dl = ids[off[i]:off[i + 1]], dd[off[i]:off[i + 1]]
+        assert i not in lst and len(np.unique(lst)) == len(lst) and (np.diff(dl) >= 0).all()
+        for j, v in zip(lst, dl):
+            assert v == kgo.dist(ANG, X[i], X[j])
+
+
 @pytest.mark.parametrize("kind,metric,d,form", [
     ("intmix", L2, 128, "bf16-exact"),      # integers <= 255: BF16 tiles are exact
     ("intmix", EUC, 64, "bf16-exact"),

@@ -540,3 +540,39 @@ def test_hand_traced_walks(case):
     n = int(gc[0])
     assert gi[0, :n].tolist() == e["ids"] and gd[0, :n].tolist() == [float(v) for v in e["dists"]]
     assert (int(gcomp[0]), int(gscan[0])) == (e["comps"], e["scans"])
+
+
+def test_search_parity_at_the_headline_scale():
+    """The 1M x 128 integer-mixture workload at its full size: the index the
+    GPU builds with the workload's IndexParams (K=100, L=150, S=10, R=100, 15
+    iterations, reverse -1; the exact BF16 join), searched at the benchmark's
+    picked setting (K=10, P=50, S=128, starting from each query's 9 nearest
+    nodes of a 4,096-node entry sample).  On 300 of the workload's queries the
+    GPU walk returns the oracle's ids, distances, result counts, comps and
+    scans bit-exactly over the same graph and the same starting points, and
+    the GPU's entry selection equals the exact nearest sampled nodes."""
+    import kgraph_amd
+    from kgraph_amd import PRECISION_BF16, datagen
+    N, E, m = 1_000_000, 4096, 9
+    X = datagen.integer_mixture(N, 128, 1000, 3, 4)
+    Q = np.ascontiguousarray(datagen.integer_mixture(10_000, 128, 1000, 3, 5)[:300])
+    g = kgraph_amd.KGraph(X, "euclidean", l2sqr=True)
+    info = g.build(K=100, L=150, S=10, R=100, iterations=15, controls=100, reverse=-1, precision=PRECISION_BF16)
+    assert g.build_counters()["join_form"] == "bf16-exact" and info.recall > 0.95
+    g.set_entry_sample(E, seed=1998)
+    sel = g.select_entries(Q, m)
+    samp = _entry_sample_ids(1998, E, N)
+    si, _, _ = kgo.bruteforce(X[samp], Q, m, L2)
+    assert np.array_equal(sel, samp[si])
+    gi, gd = g.search(Q, K=10, P=50, S=128, entries=m, withDistance=True, stats=True)
+    idx = kgo.Index()
+    idx.import_(*g.graph())
+    init = np.zeros((len(Q), 10), np.uint32)
+    init[:, :m] = sel
+    oi, od, oc, ocomp, oscan = idx.search(X, Q, L2, K=10, P=50, S=128, init=m, init_ids=init)
+    assert np.array_equal(g.last_counts, oc)
+    for q in range(len(Q)):
+        n = oc[q]
+        assert np.array_equal(gi[q, :n], oi[q, :n]) and np.array_equal(gd[q, :n], od[q, :n]), q
+    assert np.array_equal(g.last_comps, ocomp) and np.array_equal(g.last_scans, oscan)
+    print("1M-node index: %d queries bit-exact, mean comps %.0f, scans %.0f" % (len(Q), ocomp.mean(), oscan.mean()))

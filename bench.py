@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--n", type=int, default=None, help="rows (default 1M, or 10M for batched-10m-d96)")
     ap.add_argument("--nq", type=int, default=None, help="queries (default 10k, or 100k for batched)")
     ap.add_argument("--iterations", type=int, default=15)
+    ap.add_argument("--L", type=int, default=None, help="batched-10m-d96: IndexParams::L of the K=32 graph "
+                    "(default 64)")
     ap.add_argument("--P", type=str, default="50,100,200")
     ap.add_argument("--Pe", type=str, default="10,20,50,100,200",
                     help="P values for walks started from selected entry points")
@@ -395,7 +397,8 @@ def batched_workload(args):
     h.call("kg_set_data", ptr(X), n, D, D, _lib.METRIC_ANGULAR)
     rows, rb, _ = row_storage(h)
     h.call("kg_set_precision", _lib.PRECISION_BF16)  # BF16 MFMA screen, exact FP32 re-score
-    ip = IndexParams(args.iterations, 64, 32, 10, 100, 100, 1998, 0.002, 0.99, 0, -1)
+    Lb = args.L or 64
+    ip = IndexParams(args.iterations, Lb, 32, 10, 100, 100, 1998, 0.002, 0.99, 0, -1)
     info = IndexInfo()
     tb = time.perf_counter()
     h.call("kg_build", ip, info)
@@ -518,7 +521,7 @@ def batched_workload(args):
         "config": {"workload": "angmix-10m-d96-batched", "N": n, "D": D, "queries": nq,
                    "K_mix": "1/10/100 at 5:4:1", "P_mix": "50/100/300/500 uniform",
                    "sigma_per_coordinate": round(float(sigma), 5), "entry_sample": E, "entries_per_query": m,
-                   "graph": {"K": 32, "L": 64, "reverse": -1, "iterations": args.iterations}},
+                   "graph": {"K": 32, "L": Lb, "reverse": -1, "iterations": args.iterations}},
         "entry_ms": round(ems, 3),
         "build_s": round(build_s, 3), "build_control_recall": round(float(info.recall), 4),
         "build_iteration_join_forms": build_forms,

@@ -179,6 +179,12 @@ def main():
         build_s = builds["bf16"]["s"]
         # entry points are scored exactly (FP32) for the search
         h.call("kg_set_precision", _lib.PRECISION_FP32)
+        # the walk's index-side state (padded adjacency, walk layout and its
+        # row copies), made once per index and untimed, like the entry sample
+        devmem.synchronize()
+        tp = time.perf_counter()
+        h.call("kg_prepare_search")
+        builds["bf16"]["prepare_s"] = time.perf_counter() - tp
         res, cur = [], None
         for mode, P, S, E in sweeps:
             if mode == "entries" and E != cur:  # the sample's rows are index-side state, set untimed
@@ -294,6 +300,7 @@ def main():
             if name == rows or (name == "bf16" and rows != "u8"):
                 continue
             h.call("kg_set_row_storage", mode)
+            h.call("kg_prepare_search")
             r = search_once(pick["mode"], pick["P"], pick["S"], pick["E"])
             same &= bool(np.array_equal(r["ids"], res[sweep.index(pick)]["ids"]) and
                          np.array_equal(r["dists"], res[sweep.index(pick)]["dists"]))
@@ -303,6 +310,23 @@ def main():
                                                                     pick["kernel_ms"]))
         h.call("kg_set_row_storage", 0)
         row_cmp["identical_results"] = same
+    # the walk layout: the picked setting again with the walk reading the rows
+    # and adjacency in dataset order (same results)
+    lact, llev, lms = _lib.ctypes.c_int(), _lib.ctypes.c_uint(), _lib.ctypes.c_float()
+    h.call("kg_search_order", _lib.ctypes.byref(lact), _lib.ctypes.byref(llev), _lib.ctypes.byref(lms), None)
+    layout = {"order": "breadth-first" if lact.value else "dataset", "levels": llev.value,
+              "order_ms": round(lms.value, 2), "prepare_s": round(builds["bf16"]["prepare_s"], 4),
+              "walk_ms": round(pick["kernel_ms"], 4)}
+    h.call("kg_set_search_order", 0)
+    h.call("kg_prepare_search")
+    r = search_once(pick["mode"], pick["P"], pick["S"], pick["E"])
+    layout.update({"dataset_order_walk_ms": round(r["kernel_ms"], 4),
+                   "dataset_order_qps": round(nq / r["wall_s"], 1),
+                   "identical_results": bool(np.array_equal(r["ids"], res[sweep.index(pick)]["ids"]) and
+                                             np.array_equal(r["dists"], res[sweep.index(pick)]["dists"]))})
+    h.call("kg_set_search_order", 1)
+    h.call("kg_prepare_search")
+    log("walk layout: %s" % layout)
     # join kernel, algorithmic bytes: every join-list row once (2 bytes per
     # value: the MFMA form's operands are the BF16 copy) + its id (4), and a
     # radius probe at both endpoints of every pair (8)
@@ -356,6 +380,7 @@ def main():
                      "traffic": measured_traffic("search_kernel", "intmix-1m-d128", pick, rows)},
         "search_time": search_time,
         "walk_row_storage": row_cmp,
+        "walk_layout": layout,
         "roofline_join": {"kernel": "join_mfma_kernel", "bound": "hbm", "achieved": round(join_gbs, 1),
                           "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(join_gbs / PEAK_HBM_GBS, 4),
                           "traffic": measured_join_traffic("intmix-1m-d128", "bf16-exact"),

@@ -140,6 +140,27 @@ int kg_search_visited_bits(kg_handle *h, unsigned *bits);
 int kg_set_search_adjacency(kg_handle *h, int mode);
 int kg_search_adjacency(kg_handle *h, int *padded, unsigned *width);
 
+/* Walk layout (no counterpart in the reference; its search reads the rows in
+ * dataset order).  1 (default): graph walks read copies of the rows they
+ * gather, of the norms and of the padded adjacency laid out in a
+ * breadth-first order of the graph (the nodes a walk reads together sit
+ * together in memory); 0: dataset order.  Results are identical in both: the
+ * walk maps its starting ids in and its results out.  The order is computed
+ * on the device once per index, at the first search that needs it.  Walks
+ * over the CSR rows (kg_set_search_adjacency 1, search M beyond the padded
+ * width) use dataset order. */
+int kg_set_search_order(kg_handle *h, int mode);
+/* Whether the last walk launch read the reordered copies, the order's
+ * breadth-first levels and the milliseconds it took (with the reordered
+ * adjacency; 0 when none exists); perm (may be NULL): N entries, perm[i] =
+ * dataset row of node i of the order. */
+int kg_search_order(kg_handle *h, int *active, unsigned *levels, float *ms, unsigned *perm);
+/* Makes the walk's index-side state now rather than at the first search
+ * after a build, load or graph change (no counterpart in the reference, whose
+ * search reads the index as built): per-node records, the padded adjacency,
+ * the walk layout and its copies of the rows the current settings gather. */
+int kg_prepare_search(kg_handle *h);
+
 /* BF16 screen of the graph walk's candidates (the reference scores every
  * fresh neighbour in full precision before UpdateKnnList, kgraph.cpp:497-512).
  * Mode 1: walks that gather the FP32 rows (rows with no lossless BF16 or

@@ -3,6 +3,8 @@
 // upload, search-seed generation, scratch sizing, the KNNGRAPH file format
 // and index exchange.
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <ctime>
 #include <fstream>
@@ -122,6 +124,7 @@ static void set_data(Handle &h, float const *X, bool dev, unsigned N, unsigned D
     h.rows_u8 = lossy8 == 0;
     if (!h.rows_u8) h.X8.release();  // kept only when it is exact
     h.has_data = true;
+    h.wforms = 0;  // the walk's reordered row copies are stale
 }
 
 // Rows a walk gathers: 2 the 8-bit copy, 1 the BF16 copy, 0 the FP32 rows.
@@ -129,6 +132,120 @@ static int walk_rows(Handle const &h) {
     if (!h.has_data || h.row_storage == 1) return 0;
     if ((h.row_storage == 0 || h.row_storage == 3) && h.rows_u8) return 2;
     return h.rows_bf16 ? 1 : 0;
+}
+
+// The walk layout (kg_layout.hip): the breadth-first node order of the
+// padded adjacency, which is then replaced by its copy in that order.
+static void walk_order(Handle &h, unsigned N) {
+    auto const t0 = std::chrono::steady_clock::now();
+    size_t const n1 = size_t(N) + 1, nt = N / 1024 + 2;
+    DevBuf<uint32_t> words;
+    DevBuf<unsigned long long> claim;
+    words.reserve(8 * size_t(N) + 4 * n1 + nt + 1);
+    claim.reserve(std::max(N, 1u));
+    uint32_t *p = words.p;
+    auto take = [&](size_t n) {
+        uint32_t *r = p;
+        p += n;
+        return r;
+    };
+    uint32_t host_word = 0;
+    kg_order_scratch w{};
+    w.lab = take(N);
+    w.flag = take(N);
+    w.vis = take(N);
+    w.tree = take(N);
+    w.ord = take(N);
+    w.kids = take(N);
+    w.first = take(N);
+    w.done = take(N);
+    w.pos = take(n1);
+    w.koff = take(n1);
+    w.tsize = take(n1);
+    w.tbase = take(n1);
+    w.tmp = take(nt);
+    w.word = take(1);
+    w.host = &host_word;
+    w.claim = claim.p;
+    h.wperm.reserve(std::max(N, 1u));
+    h.winv.reserve(std::max(N, 1u));
+    unsigned levels = 0, rounds = 0;
+    KG_HIP(kg_walk_order(N, h.gpad.p, h.gmd.p, h.gpad_stride, w, h.wperm.p, h.winv.p, &levels, &rounds, h.stream));
+    DevBuf<uint32_t> gpad2, gmd2;
+    gpad2.reserve(std::max<size_t>(size_t(N) * h.gpad_stride, 1));
+    gmd2.reserve(std::max(N, 1u));
+    KG_HIP(kg_permute_adjacency(N, h.gpad_stride, h.wperm.p, h.winv.p, h.gpad.p, h.gmd.p, gpad2.p, gmd2.p,
+                                h.stream));
+    h.sync();
+    std::swap(h.gpad.p, gpad2.p);
+    std::swap(h.gpad.n, gpad2.n);
+    std::swap(h.gmd.p, gmd2.p);
+    std::swap(h.gmd.n, gmd2.n);
+    h.worder_valid = true;
+    h.worder_levels = levels;
+    h.wforms = 0;
+    h.worder_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (verbosity > 1)
+        std::fprintf(stderr, "walk layout: %u nodes, %u levels, %u labelling rounds, %.2f ms\n", N, levels, rounds,
+                     double(h.worder_ms));
+}
+
+// The walk's copies of the rows it gathers and of the norms, in the layout's
+// node order (made on first use; N graph nodes).
+static void walk_copies(Handle &h, unsigned N, int form, bool screen) {
+    if (!(h.wforms & 8u)) {
+        h.wnorms.reserve(std::max(N, 1u));
+        KG_HIP(kg_permute_rows(N, 1, h.wperm.p, h.norms.p, h.wnorms.p, h.stream));
+        h.wforms |= 8u;
+    }
+    if (form == 2 && !(h.wforms & 4u)) {
+        h.wX8.reserve(std::max<size_t>(size_t(N) * h.Dp, 4));
+        KG_HIP(kg_permute_rows(N, h.Dp / 4, h.wperm.p, h.X8.p, h.wX8.p, h.stream));
+        h.wforms |= 4u;
+    }
+    if ((form == 1 || screen) && !(h.wforms & 2u)) {
+        h.wXb.reserve(std::max<size_t>(size_t(N) * h.Dp, 4));
+        KG_HIP(kg_permute_rows(N, h.Dp / 2, h.wperm.p, h.Xb.p, h.wXb.p, h.stream));
+        h.wforms |= 2u;
+    }
+    if (form == 0 && !(h.wforms & 1u)) {
+        h.wX.reserve(std::max<size_t>(size_t(N) * h.Dp, 4));
+        KG_HIP(kg_permute_rows(N, h.Dp, h.wperm.p, h.X.p, h.wX.p, h.stream));
+        h.wforms |= 1u;
+    }
+}
+
+// The walk's index-side state, made once per index: per-node CSR records,
+// the padded adjacency and, in graph order, the walk layout.
+static void prepare_walk(Handle &h) {
+    if (h.gnode_valid) return;
+    unsigned const N = h.gN;
+    h.gnode.reserve(std::max(N, 1u));
+    h.gflag.reserve(2);  // [0] some list repeats an id, [1] max over nodes of min(M, deg)
+    KG_HIP(hipMemsetAsync(h.gflag.p, 0, 8, h.stream));
+    KG_HIP(kg_launch_gnode(N, h.goff.p, h.gids.p, h.gM.p, h.gnode.p, h.gflag.p, h.stream));
+    KG_HIP(kg_launch_gpad_width(N, h.goff.p, h.gM.p, h.gflag.p + 1, h.stream));
+    uint32_t fl[2] = {0, 0};
+    KG_HIP(hipMemcpyAsync(fl, h.gflag.p, 8, hipMemcpyDeviceToHost, h.stream));
+    h.sync();
+    h.lists_unique = fl[0] == 0;
+    // padded adjacency: a walk with search M <= the row width reads
+    // nothing past min(max(M[n], M), deg(n)) of a list; rows of the
+    // widest such prefix (16-id multiples) cost at most 16 GiB
+    unsigned const width = (std::max(fl[1], 1u) + 15) & ~15u;
+    h.gpad.release();
+    h.gmd.release();
+    h.gpad_stride = 0;
+    if (width < 0xffffu && size_t(N) * width * 4 <= (size_t(16) << 30)) {
+        h.gpad.reserve(size_t(N) * width);
+        h.gmd.reserve(std::max(N, 1u));
+        KG_HIP(kg_launch_gpad(N, h.goff.p, h.gids.p, h.gM.p, width, h.gpad.p, h.gmd.p, h.stream));
+        h.gpad_stride = width;
+    }
+    h.worder_valid = false;
+    h.wforms = 0;
+    if (h.search_order == 1 && h.gpad_stride) walk_order(h, N);
+    h.gnode_valid = true;
 }
 
 // Batched search over device-resident queries / outputs (kgraph.cpp:423-564).
@@ -236,35 +353,15 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
     // graph-walk queries
     unsigned const ngraph = nq - unsigned(brute.size());
     if (ngraph) {
-        if (!h.gnode_valid) {
-            h.gnode.reserve(std::max(N, 1u));
-            h.gflag.reserve(2);  // [0] some list repeats an id, [1] max over nodes of min(M, deg)
-            KG_HIP(hipMemsetAsync(h.gflag.p, 0, 8, h.stream));
-            KG_HIP(kg_launch_gnode(N, h.goff.p, h.gids.p, h.gM.p, h.gnode.p, h.gflag.p, h.stream));
-            KG_HIP(kg_launch_gpad_width(N, h.goff.p, h.gM.p, h.gflag.p + 1, h.stream));
-            uint32_t fl[2] = {0, 0};
-            KG_HIP(hipMemcpyAsync(fl, h.gflag.p, 8, hipMemcpyDeviceToHost, h.stream));
-            h.sync();
-            h.lists_unique = fl[0] == 0;
-            // padded adjacency: a walk with search M <= the row width reads
-            // nothing past min(max(M[n], M), deg(n)) of a list; rows of the
-            // widest such prefix (16-id multiples) cost at most 16 GiB
-            unsigned const width = (std::max(fl[1], 1u) + 15) & ~15u;
-            h.gpad.release();
-            h.gmd.release();
-            h.gpad_stride = 0;
-            if (width < 0xffffu && size_t(N) * width * 4 <= (size_t(16) << 30)) {
-                h.gpad.reserve(size_t(N) * width);
-                h.gmd.reserve(std::max(N, 1u));
-                KG_HIP(kg_launch_gpad(N, h.goff.p, h.gids.p, h.gM.p, width, h.gpad.p, h.gmd.p, h.stream));
-                h.gpad_stride = width;
-            }
-            h.gnode_valid = true;
-        }
+        prepare_walk(h);
         bool const pad = h.search_adjacency == 0 && h.gpad_stride && sp.M <= h.gpad_stride;
         int const form = walk_rows(h);
         // BF16 screen in front of the FP32 rows (lossy BF16 copy; exact copies are gathered instead)
         bool const screen = form == 0 && h.search_screen == 1 && h.Dp <= 256;
+        // the padded rows are in the layout's order: the walk reads its row copies too
+        bool const lay = pad && h.worder_valid;
+        if (lay) walk_copies(h, N, form, screen);
+        h.last_order = lay ? 1 : 0;
         // queries per wave: 2 walks each query on half a wave (not with the screen)
         unsigned const qpw = screen ? 1u : h.search_qpw;
         // the T > 1 result list (a single trial's result is its pool's head)
@@ -326,10 +423,12 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
             h.vis_words = words;
         }
         kg_search_args a{};
-        a.X = h.X.p;
-        a.X8 = form == 2 ? h.X8.p : nullptr;
-        a.Xb = form == 1 ? h.Xb.p : nullptr;
-        a.Xs = screen ? h.Xb.p : nullptr;
+        a.X = lay && form == 0 ? h.wX.p : h.X.p;  // (read by the FP32 walk and the screen's re-score only)
+        a.X8 = form == 2 ? (lay ? h.wX8.p : h.X8.p) : nullptr;
+        a.Xb = form == 1 ? (lay ? h.wXb.p : h.Xb.p) : nullptr;
+        a.Xs = screen ? (lay ? h.wXb.p : h.Xb.p) : nullptr;
+        a.perm = lay ? h.wperm.p : nullptr;
+        a.inv = lay ? h.winv.p : nullptr;
         h.last_screen = screen ? 1 : 0;
         if (screen) {
             kg_search_screen_constants(h.D, a);
@@ -337,7 +436,7 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
             KG_HIP(hipMemsetAsync(h.screen_cnt.p, 0, 8, h.stream));
             a.rescored = h.screen_cnt.p;
         }
-        a.norms = h.norms.p;
+        a.norms = lay ? h.wnorms.p : h.norms.p;
         a.N = N;
         a.D = h.D;
         a.Dp = h.Dp;
@@ -787,6 +886,39 @@ int kg_search_adjacency(kg_handle *k, int *padded, unsigned *width) {
     return guard(k, [&](Handle &h) {
         if (padded) *padded = h.last_adjacency == 0 ? 1 : 0;
         if (width) *width = h.gpad_stride;
+    });
+}
+
+int kg_prepare_search(kg_handle *k) {
+    return guard(k, [&](Handle &h) {
+        if (!h.has_data) throw kga::runtime_error("no dataset");
+        if (!h.has_graph) throw kga::runtime_error("no index");
+        if (h.gN > h.N) throw kga::runtime_error("dataset larger than index");
+        prepare_walk(h);
+        int const form = walk_rows(h);
+        bool const screen = form == 0 && h.search_screen == 1 && h.Dp <= 256;
+        if (h.worder_valid) walk_copies(h, h.gN, form, screen);
+        h.sync();
+    });
+}
+
+int kg_set_search_order(kg_handle *k, int mode) {
+    return guard(k, [&](Handle &h) {
+        if (mode != 0 && mode != 1) throw kga::invalid_argument("order mode must be 0 or 1");
+        if (mode != h.search_order) h.gnode_valid = false;  // the padded rows are rebuilt in the other order
+        h.search_order = mode;
+    });
+}
+
+int kg_search_order(kg_handle *k, int *active, unsigned *levels, float *ms, unsigned *perm) {
+    return guard(k, [&](Handle &h) {
+        if (active) *active = h.last_order;
+        if (levels) *levels = h.worder_valid ? h.worder_levels : 0u;
+        if (ms) *ms = h.worder_valid ? h.worder_ms : 0.f;
+        if (perm) {
+            if (!h.worder_valid) throw kga::runtime_error("no walk layout (it is made by the first search)");
+            KG_HIP(hipMemcpy(perm, h.wperm.p, 4ull * h.gN, hipMemcpyDeviceToHost));
+        }
     });
 }
 

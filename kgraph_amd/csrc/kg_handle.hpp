@@ -115,6 +115,19 @@ struct Handle {
     int last_adjacency = 1;      // what the last walk launch read (0 padded, 1 CSR)
     bool gnode_valid = false;
     bool lists_unique = false;
+    // walk layout: 1 = walks read copies of the rows, norms and padded
+    // adjacency in the breadth-first node order of kg_walk_order (wperm[i] =
+    // dataset row of node i, winv its inverse), 0 = dataset order
+    int search_order = 1;
+    bool worder_valid = false;   // wperm / winv match the graph; gpad / gmd are in that order
+    unsigned worder_levels = 0;  // breadth-first levels of the order
+    float worder_ms = 0.f;       // time to compute it and reorder the padded adjacency
+    unsigned wforms = 0;         // row copies in that order: 1 FP32, 2 BF16, 4 8-bit, 8 norms
+    int last_order = 0;          // whether the last walk launch read them
+    DevBuf<uint32_t> wperm, winv;
+    DevBuf<float> wX, wnorms;
+    DevBuf<uint16_t> wXb;
+    DevBuf<uint8_t> wX8;
     std::vector<uint32_t> h_off, h_ids, h_M;
     std::vector<float> h_dist;
 

@@ -32,6 +32,12 @@ struct kg_search_args {
     uint32_t const *gpad;
     uint32_t const *gmd;
     unsigned gstride;
+    // walk layout (both NULL: dataset order).  The rows, norms and padded
+    // adjacency above are the walk's copies in another node order: node i is
+    // dataset row perm[i], row r is node inv[r].  Starting ids and results are
+    // dataset rows; ties between starting points go by dataset row.
+    uint32_t const *perm;
+    uint32_t const *inv;
     unsigned lists_unique;  // 1: no adjacency list repeats an id
     unsigned int_dot;       // 1: over the 8-bit rows, integer queries take the exact v_dot4 path
     // queries
@@ -147,6 +153,24 @@ hipError_t kg_launch_gpad_width(unsigned N, uint32_t const *goff, uint32_t const
 // gpad rows (first min(deg, gstride) ids of each list) and gmd words.
 hipError_t kg_launch_gpad(unsigned N, uint32_t const *goff, uint32_t const *gids, uint32_t const *gM, unsigned gstride,
                           uint32_t *gpad, uint32_t *gmd, hipStream_t s);
+// Walk layout (kg_layout.hip).  Scratch of kg_walk_order: lab, flag, vis,
+// tree, ord, kids, first, done: N words; pos, koff, tsize, tbase: N + 1; tmp:
+// N / 1024 + 2; claim: N; word: 1; host: one word of pinned host memory.
+struct kg_order_scratch {
+    uint32_t *lab, *flag, *vis, *tree, *ord, *kids, *first, *done;
+    uint32_t *pos, *koff, *tsize, *tbase, *tmp, *word, *host;
+    unsigned long long *claim;
+};
+// perm / inv of the breadth-first node order of the padded adjacency (gpad
+// rows of width W, gmd words); waits on the stream once per level.
+hipError_t kg_walk_order(unsigned N, uint32_t const *gpad, uint32_t const *gmd, unsigned W, kg_order_scratch const &w,
+                         uint32_t *perm, uint32_t *inv, unsigned *levels_out, unsigned *rounds_out, hipStream_t s);
+// out row i = in row perm[i] (rows of `words` 32-bit words)
+hipError_t kg_permute_rows(unsigned N, unsigned words, uint32_t const *perm, void const *in, void *out, hipStream_t s);
+// gpad2 row i = gpad row perm[i] with ids renamed by inv; gmd2[i] = gmd[perm[i]]
+hipError_t kg_permute_adjacency(unsigned N, unsigned W, uint32_t const *perm, uint32_t const *inv,
+                                uint32_t const *gpad, uint32_t const *gmd, uint32_t *gpad2, uint32_t *gmd2,
+                                hipStream_t s);
 // *flag = *host_flag = 1 if one of the first `init` ids of a row (row q at
 // ids + q*ostride) is >= N (host_flag: pinned host memory); both are left
 // alone otherwise.

@@ -772,6 +772,7 @@ search_kernel(kg_search_args a) {
                     bool keep = false;
                     if (i < P) {
                         s = seeds[i];
+                        if (a.inv) s = a.inv[s];
                         keep = true;
                         if (trial > 0)
                             keep = glob ? !(load_u32_l2(&vis[s >> 5]) & (1u << (s & 31)))
@@ -784,7 +785,7 @@ search_kernel(kg_search_args a) {
             } else {
                 uint32_t const *ini = a.init_ids + size_t(qi) * a.ostride;
                 L = min(a.init, K + P - 1);  // = init whenever init < K (one-K calls)
-                for (unsigned i = lane; i < L; i += G) w.pid[i] = ini[i];
+                for (unsigned i = lane; i < L; i += G) w.pid[i] = a.inv ? a.inv[ini[i]] : ini[i];
             }
             wave_sync();
             // ---- mark, score and sort the starting points (kgraph.cpp:468-476)
@@ -814,8 +815,9 @@ search_kernel(kg_search_args a) {
                 }
             }
             batch_dist<METRIC, RowT, G>(a, w.pid, w.pdist, L, w.q, qn, w.q8, qv);
+            // (dist, dataset row) order
             for (unsigned i = lane; i < L; i += G)
-                w.tkey[i] = (uint64_t(ord_f32(w.pdist[i])) << 32) | w.pid[i];
+                w.tkey[i] = (uint64_t(ord_f32(w.pdist[i])) << 32) | (a.perm ? a.perm[w.pid[i]] : w.pid[i]);
             wave_sync();
             for (unsigned c = 0; c < L; c += G) {
                 unsigned i = c + lane;
@@ -826,7 +828,7 @@ search_kernel(kg_search_args a) {
                     rank += (kj < key) || (kj == key && j < i);
                 }
                 if (i < L) {
-                    w.pid[rank] = uint32_t(key) | FLAG;
+                    w.pid[rank] = (a.inv ? a.inv[uint32_t(key)] : uint32_t(key)) | FLAG;
                     w.pdist[rank] = unord_f32(uint32_t(key >> 32));
                 }
             }
@@ -1031,7 +1033,8 @@ search_kernel(kg_search_args a) {
             }
         }
         for (unsigned i = lane; i < cut; i += G) {
-            a.out_ids[size_t(qi) * a.ostride + i] = oid[i] & IDMASK;
+            uint32_t const id = oid[i] & IDMASK;
+            a.out_ids[size_t(qi) * a.ostride + i] = a.perm ? a.perm[id] : id;
             if (a.out_dists) a.out_dists[size_t(qi) * a.ostride + i] = odist[i];
         }
         if (lane == 0) {

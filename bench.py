@@ -172,7 +172,7 @@ def main():
             h.call("kg_build_counters", ptr(cnt), 6)
             builds[form] = dict(s=time.perf_counter() - tb, it=int(info.iterations), recall=float(info.recall),
                                 iter_ms=ctypes_float(h, 8), graph_replays=int(cnt[5]),
-                                join_form=["fp32", "bf16-screen", "bf16-exact", "fp32-mfma-screen"][int(cnt[4])])
+                                join_form=["fp32", "bf16-screen", "bf16-exact", "fp32-mfma-screen", "int8-exact"][int(cnt[4])])
             log("build %s: %.3fs (%d it, control recall %.4f, join %s, iterations %.1f ms, %d graph replays)" % (
                 form, builds[form]["s"], info.iterations, info.recall, builds[form]["join_form"],
                 builds[form]["iter_ms"], builds[form]["graph_replays"]))
@@ -227,8 +227,9 @@ def main():
     eager_s = time.perf_counter() - tb
     h.call("kg_set_build_graphs", 1)
     h.call("kg_set_precision", _lib.PRECISION_FP32)  # as the search sweep left it
-    counters = np.zeros(4, dtype=np.uint64)
-    h.call("kg_build_counters", ptr(counters), 4)
+    counters = np.zeros(5, dtype=np.uint64)
+    h.call("kg_build_counters", ptr(counters), 5)
+    eager_form = ["fp32", "bf16-screen", "bf16-exact", "fp32-mfma-screen", "int8-exact"][int(counters[4])]
     join_ms = ctypes_float(h, 3)
     merge_ms = ctypes_float(h, 4)
     upd_ms = ctypes_float(h, 5)
@@ -327,10 +328,11 @@ def main():
     h.call("kg_set_search_order", 1)
     h.call("kg_prepare_search")
     log("walk layout: %s" % layout)
-    # join kernel, algorithmic bytes: every join-list row once (2 bytes per
-    # value: the MFMA form's operands are the BF16 copy) + its id (4), and a
-    # radius probe at both endpoints of every pair (8)
-    join_bytes = float(counters[1]) * (2 * D + 4) + 8.0 * float(counters[0])
+    # join kernel, algorithmic bytes: every join-list row once (its operand
+    # bytes: 1 per value for the 8-bit form, 2 for the BF16 copy) + its id (4),
+    # and a radius probe at both endpoints of every pair (8)
+    op_bytes = 1 if eager_form == "int8-exact" else 2
+    join_bytes = float(counters[1]) * (op_bytes * D + 4) + 8.0 * float(counters[0])
     join_gbs = join_bytes / (join_ms * 1e-3) / 1e9 if join_ms > 0 else 0.0
 
     out = {
@@ -383,7 +385,7 @@ def main():
         "walk_layout": layout,
         "roofline_join": {"kernel": "join_mfma_kernel", "bound": "hbm", "achieved": round(join_gbs, 1),
                           "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(join_gbs / PEAK_HBM_GBS, 4),
-                          "traffic": measured_join_traffic("intmix-1m-d128", "bf16-exact"),
+                          "traffic": measured_join_traffic("intmix-1m-d128", eager_form), "form": eager_form,
                           "launches": int(einfo.iterations),
                           "pairs": int(counters[0]), "list_entries": int(counters[1]),
                           "pushes": int(counters[2])},
@@ -678,7 +680,7 @@ def precision_workload(args):
         h.call("kg_set_search_screen", 0)
         forms[form] = {"build_s": round(bs, 4), "iterations": int(info.iterations),
                        "control_recall": round(float(info.recall), 4), "graph_recall@10": round(grec, 4),
-                       "join_form": ["fp32", "bf16-screen", "bf16-exact", "fp32-mfma-screen"][int(cnt[4])],
+                       "join_form": ["fp32", "bf16-screen", "bf16-exact", "fp32-mfma-screen", "int8-exact"][int(cnt[4])],
                        "iteration_join_forms": "".join("TSEPF"[int(f)] for f in its[:, 3]),
                        "search": sweep}
         log("%s: build %.3fs (%d it, control recall %.4f, graph recall@10 %.4f), stored err max %.3g; search %s" % (

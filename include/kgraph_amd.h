@@ -364,10 +364,17 @@ int kg_search_latencies(kg_handle *h, float *out_us, unsigned n);
  * |new| + |old|), out[2] candidates pushed to merge slots, out[3]
  * iterations, out[4] local-join form (0 FP32 quads, 1 BF16 MFMA screen with
  * exact FP32 re-score, 2 BF16 MFMA exact on these rows, 3 FP32-operand MFMA
- * screen with exact FP32 re-score), out[5] iterations
+ * screen with exact FP32 re-score, 4 8-bit MFMA exact on these rows), out[5] iterations
  * run as replays of the captured iteration graph, out[6] pairs the screened
  * forms re-scored in exact FP32.  n = number of entries wanted (<= 7). */
 int kg_build_counters(kg_handle *h, uint64_t *out, unsigned n);
+
+/* 8-bit local join (no counterpart in the reference): 1 (default) = with
+ * KG_PRECISION_BF16, rows whose values are all integers in [0, 255] and a
+ * Euclidean metric (L2SQR or half-SSD) with 255^2 D <= 2^24 are joined on
+ * v_mfma_i32_16x16x64_i8 (shifted to [-128, 127]; every distance is the exact
+ * canonical one); 0 = the BF16 form. */
+int kg_set_join_int8(kg_handle *h, int on);
 
 /* Local-join form when the BF16 join is a screen (KG_PRECISION_BF16 on rows
  * that are not BF16-exact; no counterpart in the reference).  Both forms admit

@@ -80,14 +80,16 @@ class KGraph:
     def build(self, iterations=default_iterations, L=default_L, K=default_K, S=default_S,
               controls=default_controls, delta=default_delta, recall=default_recall,
               prune=default_prune, reverse=default_reverse, *, R=default_R, seed=default_seed,
-              precision=None, graphs=None, join=None, prerank=None):
+              precision=None, graphs=None, join=None, prerank=None, int8=None):
         """graphs: True (default) runs the iterations as replays of a captured
         HIP graph, False launches their kernels one by one.  join: 'auto'
         (default), 'screen' or 'tiles' -- the local-join form when the BF16
         join is a screen (precision BF16 on rows that are not BF16-exact).
         prerank: fraction of each node's join lists kept by the learned
         pre-rank (needs a projector, train_projector; 0 or 1 = off); see
-        prerank_report()."""
+        prerank_report().  int8: True (default) lets precision BF16 join rows
+        of integers in [0, 255] on 8-bit MFMA operands (exact), False keeps
+        the BF16 form."""
         p = IndexParams(iterations, L, K, S, R, controls, seed, delta, recall, prune, reverse)
         if precision is not None:
             if isinstance(precision, str):
@@ -101,6 +103,8 @@ class KGraph:
                                               "tiles": _lib.JOIN_TILES}[join])
         if prerank is not None:
             self._h.call("kg_set_join_prerank", float(prerank))
+        if int8 is not None:
+            self._h.call("kg_set_join_int8", 1 if int8 else 0)
         info = IndexInfo()
         self._h.call("kg_build", p, info)
         self._hasIndex = True
@@ -291,12 +295,12 @@ class KGraph:
     def build_counters(self):
         """Last build: {'pairs', 'list_entries', 'pushes', 'iterations',
         'join_form', 'graph_replays', 'rescored'} with join_form 'fp32',
-        'bf16-screen', 'bf16-exact' or 'fp32-mfma-screen', graph_replays the
+        'bf16-screen', 'bf16-exact', 'fp32-mfma-screen' or 'int8-exact', graph_replays the
         iterations run as replays of the captured iteration graph and rescored
         the pairs the screen sent to the exact FP32 re-score."""
         out = np.zeros(7, dtype=np.uint64)
         self._h.call("kg_build_counters", ptr(out), 7)
-        forms = {0: "fp32", 1: "bf16-screen", 2: "bf16-exact", 3: "fp32-mfma-screen"}
+        forms = {0: "fp32", 1: "bf16-screen", 2: "bf16-exact", 3: "fp32-mfma-screen", 4: "int8-exact"}
         return {"pairs": int(out[0]), "list_entries": int(out[1]), "pushes": int(out[2]),
                 "iterations": int(out[3]), "join_form": forms.get(int(out[4]), "fp32"),
                 "graph_replays": int(out[5]), "rescored": int(out[6])}

@@ -270,6 +270,14 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
         a.sqn = st.sqn.p;
         a.mfma = 1;
         a.exact = bf16_tiles_exact(h.D, h.metric, rr.integer, h.rows_bf16, rr.maxabs, rr.vmin, rr.vmax) ? 1u : 0u;
+        // 8-bit operands when the rows' 8-bit copy is exact and the distance
+        // allows it (i8_tiles_exact): exact tiles with half the operand bytes
+        // and MFMA issues of the BF16 form
+        if (h.join_i8 && i8_tiles_exact(h.D, h.metric, h.rows_u8)) {
+            a.exact = 1;
+            a.i8 = 1;
+            a.X8 = h.X8.p;
+        }
         screen_constants(h.D, a);
     } else if (h.precision == KG_PRECISION_FP32_MFMA && h.D <= 256) {
         // local join screened with FP32 operands on MFMA, exact FP32 re-score
@@ -281,7 +289,7 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
         a.exact = 0;
         screen_constants(h.D, a, true);
     }
-    h.join_exact = a.mfma ? (a.f32 ? 2 : int(a.exact)) : -1;
+    h.join_exact = a.mfma ? (a.f32 ? 2 : a.i8 ? 3 : int(a.exact)) : -1;
     // learned pre-rank of the join lists: the projector's rows, guarded by a
     // held-out probe before the first iteration and by the control recall
     // after every iteration (plain NN-descent from the first drop on)
@@ -379,6 +387,7 @@ void build_index(Handle &h, kg_index_params const &params, kg_index_info *pinfo)
         h.prerank_status = 1;
     }
     KG_HIP(nnd_launch_meta_init(h.metric, a, s));
+    if (a.i8) KG_HIP(nnd_launch_n8(a, h.metric == KG_METRIC_EUCLIDEAN ? 1 : 0, s));  // the 8-bit form's join norms
     KG_HIP(nnd_launch_init(h.metric, a, s));
 
     kg_index_info info{};

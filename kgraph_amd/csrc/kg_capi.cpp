@@ -1533,6 +1533,13 @@ int kg_mlp_get_normalization(kg_handle *k, float *mu, float *inv_sigma) {
     });
 }
 
+int kg_set_join_int8(kg_handle *k, int on) {
+    return guard(k, [&](Handle &h) {
+        if (on != 0 && on != 1) throw kga::invalid_argument("int8 join switch must be 0 or 1");
+        h.join_i8 = on == 1;
+    });
+}
+
 int kg_build_counters(kg_handle *k, uint64_t *out, unsigned n) {
     return guard(k, [&](Handle &h) {
         if (!out) throw kga::invalid_argument("null argument");

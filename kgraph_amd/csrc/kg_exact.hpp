@@ -33,4 +33,17 @@ inline bool bf16_tiles_exact(unsigned D, int metric, bool integer, bool lossless
     return true;
 }
 
+// The 8-bit form: rows whose values are all integers in [0, 255] (their
+// 8-bit copy is exact) enter v_mfma_i32_16x16x64_i8 shifted to [-128, 127]
+// (x - 128: the top bit flipped).  The L2SQR and half-SSD distances do not
+// change under the shift; dots (|a' . o'| <= 2^14 D) and norms
+// (|x'|^2 <= 2^14 D) are exact int32 sums, and the tile value
+// (n_a + n_o) - 2 a'.o' (halved norms and a'.o' for EUCLIDEAN) is the exact
+// integer |a - o|^2 <= 255^2 D (its half), which FP32 holds, as it holds every
+// partial sum of the canonical order, when 255^2 D <= 2^24.  ANGULAR divides by
+// the unshifted norms and takes the BF16 form.
+inline bool i8_tiles_exact(unsigned D, int metric, bool u8rows) {
+    return u8rows && metric != 2 && 65025.0 * double(D) <= 16777216.0;
+}
+
 }  // namespace kga

@@ -62,7 +62,8 @@ struct Handle {
     int num_cus = 0;
     size_t lds_per_cu = 0;
     int precision = KG_PRECISION_FP32;
-    int join_exact = -1;
+    int join_exact = -1;  // last build's join: -1 FP32 tiles, 0 BF16 screen, 1 BF16 exact, 2 FP32-operand screen, 3 8-bit exact
+    bool join_i8 = true;  // the exact join takes 8-bit operands when the rows allow (kg_set_join_int8)
     bool entry_int = false;  // entry sample rows are all integers
     float entry_max = 0.f;   // their largest |value|
     DevBuf<float> qsq;       // ||q||^2 / 2 of the last query batch

@@ -53,6 +53,11 @@ struct NndArgs {
     unsigned mfma;      // 1: tile join on the matrix cores
     unsigned f32;       // with mfma: FP32 operands (v_mfma_f32_16x16x4_f32) instead of BF16
     unsigned exact;     // 1: BF16 tiles are exact for these rows (no re-score)
+    // with exact: 8-bit operands (v_mfma_i32_16x16x64_i8) from X8, the rows'
+    // 8-bit copy (row stride Dp bytes); meta's join norms are then those of
+    // the shifted rows (nnd_launch_n8)
+    unsigned i8;
+    uint8_t const *X8;
     float scr_c, scr_g; // screen bound: relative dot error, relative norm error
     // learned pre-rank of the join lists (prerank_kernel): projected rows
     float const *Z;     // N x zdim projector outputs (NULL: no pre-rank)
@@ -66,6 +71,10 @@ hipError_t nnd_launch_join(int metric, NndArgs const &a, hipStream_t s);
 // flags[1] = max |x| as float bits, flags[2] / flags[3] = ord_f32 of the
 // smallest / largest value (the caller sets flags[2] to all ones, the rest to 0).
 hipError_t nnd_launch_sqn(NndArgs const &a, float2 *out, uint32_t *flags, hipStream_t s);
+// Join norms of the 8-bit form: |x - 128|^2 over the D columns of each row of
+// a.X8 (half of it when half != 0), exact integers, into a.meta[n].z (after
+// nnd_launch_meta_init; only the 8-bit join reads them).
+hipError_t nnd_launch_n8(NndArgs const &a, int half, hipStream_t s);
 hipError_t nnd_launch_merge(NndArgs const &a, hipStream_t s);
 // Prunes every node's join lists to the fraction a.keep nearest in the
 // learned projection a.Z (ties by list position), in place.

@@ -29,6 +29,7 @@
 // canonical FP32 order of kg_device.hpp.
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 
 #include "kg_device.hpp"
 #include "kg_nnd.hpp"
@@ -211,6 +212,7 @@ __device__ __forceinline__ void put_cand(NndArgs const &a, uint32_t target, unsi
 // in LDS and flushed grouped by target as in join_kernel.
 typedef __bf16 mj_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float mj_f32x4 __attribute__((ext_vector_type(4)));
+typedef int mj_i32x4 __attribute__((ext_vector_type(4)));
 
 constexpr unsigned MT = 512;  // list capacity (new + old)
 constexpr unsigned MB = 512;  // staged candidates per wave (exact tile forms)
@@ -299,6 +301,52 @@ struct MjOps<KS, true> {
     float4 v[KS];
     __device__ void load(NndArgs const &a, uint32_t id, bool ok, unsigned lane) { mf_load<KS>(a, id, ok, lane >> 4, v); }
 };
+// 8-bit operands for v_mfma_i32_16x16x64_i8 (rows of the exact 8-bit copy,
+// shifted to [-128, 127] by flipping each byte's top bit): KS steps of 64
+// columns, lane l takes columns 64 s + 16 (l / 16) .. +15 of its row; columns
+// at D and beyond are zero after the shift.  16-byte loads when rows start at
+// multiples of 16 bytes, else 4-byte ones.
+template <int KS>
+struct MjOps8 {
+    mj_i32x4 v[KS];
+    __device__ void load(NndArgs const &a, uint32_t id, bool ok, unsigned lane) {
+        uint8_t const *r = a.X8 + size_t(ok ? id : 0u) * a.Dp;
+        bool const wide = (a.Dp & 15u) == 0;
+        unsigned const g16 = (lane >> 4) * 16;
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            unsigned const k = 64 * s + g16;
+            uint32_t w[4] = {0u, 0u, 0u, 0u};
+            if (ok && k < a.D) {
+                if (wide) {
+                    uint4 const x = *reinterpret_cast<uint4 const *>(r + k);
+                    w[0] = x.x;
+                    w[1] = x.y;
+                    w[2] = x.z;
+                    w[3] = x.w;
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        if (k + 4 * q + 4 <= a.Dp) w[q] = *reinterpret_cast<uint32_t const *>(r + k + 4 * q);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    int const live = int(a.D) - int(k + 4 * q);  // columns of this word below D
+                    uint32_t const keep = live >= 4 ? 0xffffffffu : live <= 0 ? 0u : (1u << (8 * live)) - 1u;
+                    w[q] = (w[q] ^ 0x80808080u) & keep;
+                }
+            }
+            v[s] = mj_i32x4{int(w[0]), int(w[1]), int(w[2]), int(w[3])};
+        }
+    }
+};
+template <int KS>
+__device__ __forceinline__ mj_f32x4 mj_mma(MjOps8<KS> const &x, MjOps8<KS> const &y) {
+    mj_i32x4 acc = {0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < KS; ++s) acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(x.v[s], y.v[s], acc, 0, 0, 0);
+    return mj_f32x4{float(acc[0]), float(acc[1]), float(acc[2]), float(acc[3])};
+}
 template <int KS>
 __device__ __forceinline__ mj_f32x4 mj_mma(MjOps<KS, false> const &x, MjOps<KS, false> const &y) {
     mj_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
@@ -448,14 +496,17 @@ __device__ __forceinline__ void mj_finish(NndArgs const &a, unsigned n, unsigned
     }
 }
 
-// The exact BF16 form (integer rows, see above): the tile value is the
-// canonical distance and decides alone.
-template <int METRIC, int KS>
+// The exact forms (integer rows, see above): BF16 operands, or 8-bit ones
+// (I8: rows of the exact 8-bit copy; the records' join norms are then those
+// of the shifted rows, nnd_launch_n8); the tile value is the canonical
+// distance and decides alone.
+template <int METRIC, int KS, bool I8>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 join_mfma_kernel(NndArgs a) {
+    using Ops = typename std::conditional<I8, MjOps8<KS>, MjOps<KS, false>>::type;
     // 16-row new groups held in registers at once: two for operands of up to
-    // 128 values (16 VGPRs each), one otherwise
-    constexpr unsigned AG = KS <= 4 ? 2u : 1u;
+    // 128 values (16 VGPRs each as BF16, 8 as bytes), one otherwise
+    constexpr unsigned AG = (I8 ? KS <= 2 : KS <= 4) ? 2u : 1u;
     __shared__ MJoinLDS lds[4];
     unsigned const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     unsigned const n = a.n0 + blockIdx.x * 4 + w;
@@ -470,7 +521,7 @@ join_mfma_kernel(NndArgs a) {
         // group runs past the new rows: their pairs are masked below) stay in
         // registers while every later tile streams past them once; a tile
         // that is one of the groups is taken from its registers
-        MjOps<KS, false> aop[AG];
+        Ops aop[AG];
         uint32_t aid[AG][4];
 #pragma unroll
         for (unsigned g = 0; g < AG; ++g) {
@@ -495,7 +546,7 @@ join_mfma_kernel(NndArgs a) {
             bool const ook = io < nT;
             uint32_t const oid = ook ? L.ids[io] : 0u;
             unsigned const dg = (tg - ag) >> 4;  // wave-uniform: the tile is A group dg when dg < AG
-            MjOps<KS, false> bop;
+            Ops bop;
             if (dg == 0)
                 bop = aop[0];
             else if (AG > 1 && dg == 1)
@@ -912,6 +963,24 @@ __global__ void __launch_bounds__(256) sqn_kernel(float const *X, unsigned N, un
 // for SSD on the MFMA form, else the stored metric norm, 0 for SSD on FP32
 // tiles), w = |x| (the screen bound; MFMA form only).  x, y (radius, K-th
 // distance) are written by init_kernel and merge_kernel.
+// |x - 128|^2 over the D columns of an 8-bit row (half of it for half-SSD)
+// into the join norm of its record
+__global__ void __launch_bounds__(256) n8_kernel(uint8_t const *X8, unsigned N, unsigned D, unsigned Dp, int half,
+                                                 float4 *meta) {
+    unsigned const n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= N) return;
+    uint32_t const *r = reinterpret_cast<uint32_t const *>(X8 + size_t(n) * Dp);  // Dp % 4 == 0
+    int sum = 0;
+    for (unsigned w = 0; 4 * w < D; ++w) {
+        uint32_t const word = r[w];
+        for (unsigned j = 0; j < 4 && 4 * w + j < D; ++j) {
+            int const v = int((word >> (8 * j)) & 0xffu) - 128;
+            sum += v * v;
+        }
+    }
+    meta[n].z = half ? 0.5f * float(sum) : float(sum);
+}
+
 __global__ void meta_init_kernel(float4 *meta, unsigned N, int metric, float2 const *sqn, float const *norms) {
     unsigned const n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
@@ -1490,15 +1559,27 @@ static void launch_join_m(NndArgs const &a, hipStream_t s) {
 template <int METRIC>
 static void launch_join_exact(NndArgs const &a, hipStream_t s) {
     dim3 g((a.n1 - a.n0 + 3) / 4), b(256);
+    if constexpr (METRIC != METRIC_ANGULAR) {
+        if (a.i8) {
+            unsigned const ks8 = (a.D + 63) / 64;  // 64-column 8-bit k-steps
+            if (ks8 <= 1)
+                hipLaunchKernelGGL((join_mfma_kernel<METRIC, 1, true>), g, b, 0, s, a);
+            else if (ks8 <= 2)
+                hipLaunchKernelGGL((join_mfma_kernel<METRIC, 2, true>), g, b, 0, s, a);
+            else
+                hipLaunchKernelGGL((join_mfma_kernel<METRIC, 4, true>), g, b, 0, s, a);
+            return;
+        }
+    }
     unsigned const ks = (a.D + 31) / 32;  // 32-column MFMA k-steps
     if (ks <= 1)
-        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 1>), g, b, 0, s, a);
+        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 1, false>), g, b, 0, s, a);
     else if (ks <= 2)
-        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 2>), g, b, 0, s, a);
+        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 2, false>), g, b, 0, s, a);
     else if (ks <= 4)
-        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 4>), g, b, 0, s, a);
+        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 4, false>), g, b, 0, s, a);
     else
-        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 8>), g, b, 0, s, a);
+        hipLaunchKernelGGL((join_mfma_kernel<METRIC, 8, false>), g, b, 0, s, a);
 }
 
 // The screen forms: 16 staged FP32 rows of Dp floats per wave.  BF16: KS
@@ -1540,6 +1621,12 @@ static void launch_join_mfma(NndArgs const &a, hipStream_t s) {
         launch_join_exact<METRIC>(a, s);
     else
         launch_join_screen<METRIC>(a, s);
+}
+
+hipError_t nnd_launch_n8(NndArgs const &a, int half, hipStream_t s) {
+    if (a.N == 0) return hipSuccess;
+    hipLaunchKernelGGL(n8_kernel, dim3((a.N + 255) / 256), dim3(256), 0, s, a.X8, a.N, a.D, a.Dp, half, a.meta);
+    return hipGetLastError();
 }
 
 hipError_t nnd_launch_sqn(NndArgs const &a, float2 *out, uint32_t *flags, hipStream_t s) {

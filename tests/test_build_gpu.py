@@ -80,12 +80,12 @@ def test_build_recall_matches_oracle(kind, metric, d, params):
     assert len(tr) == info.iterations and tr[-1, 0] >= tr[0, 0]
 
 
-@pytest.mark.parametrize("form", ["fp32", "bf16-exact"])
+@pytest.mark.parametrize("form", ["fp32", "bf16-exact", "int8-exact"])
 def test_build_recall_at_the_c2_parameters(form):
     """The 1M x 128 integer-mixture workload's generator and IndexParams (K=100,
     L=150, S=10, R=100, 15 iterations) at 60,000 rows: recall@100 of the GPU
-    build (plain FP32 tiles, and the BF16 MFMA form that is exact on these
-    rows) against exact K-NN is within 0.5 pt of the oracle's build.  Reverse
+    build (plain FP32 tiles, and the BF16 and 8-bit MFMA forms that are exact
+    on these rows) against exact K-NN is within 0.5 pt of the oracle's build.  Reverse
     edges are left out (reverse=0) so that every list holds its L nearest
     found, as recall@K reads them; reverse-edge completion is tested on its
     own against the oracle."""
@@ -100,7 +100,8 @@ def test_build_recall_at_the_c2_parameters(form):
     roff, rids, rd, rM = ref.export()
     r_rec = _knn_recall(roff[rows], rd, gt_d, K)
     g = _gpu(X, L2)
-    info = g.build(precision=PRECISION_BF16 if form.startswith("bf16") else PRECISION_FP32, **params)
+    info = g.build(precision=PRECISION_FP32 if form == "fp32" else PRECISION_BF16, int8=form == "int8-exact",
+                   **params)
     assert g.build_counters()["join_form"] == form
     off, ids, dd, M = g.graph()
     g_rec = _knn_recall(off[rows], dd, gt_d, K)
@@ -152,6 +153,12 @@ def test_build_recall_at_the_c3_parameters(form):
 @pytest.mark.parametrize("kind,metric,d,form", [
     ("intmix", L2, 128, "bf16-exact"),      # integers <= 255: BF16 tiles are exact
     ("intmix", EUC, 64, "bf16-exact"),
+    ("intmix", L2, 128, "int8-exact"),      # ... and so are 8-bit ones
+    ("intmix", EUC, 64, "int8-exact"),
+    ("intmix", L2, 37, "int8-exact"),       # rows of 40 bytes: 4-byte loads, a partial word
+    ("intmix", EUC, 100, "int8-exact"),     # a partial 64-column step
+    ("intmix", L2, 200, "int8-exact"),      # 2 D 255^2 > 2^24: no BF16-exact form, 8-bit still exact
+    ("intmix", ANG, 64, "bf16-exact"),      # angular keeps the BF16 form
     ("uniform", EUC, 16, "bf16-screen"),    # FP32 data: screen + exact re-score
     ("angular", ANG, 96, "bf16-screen"),
     ("uniform", L2, 37, "fp32-mfma-screen"),  # FP32 operands on MFMA: screen + exact re-score
@@ -170,7 +177,8 @@ def test_mfma_join_recall_and_canonical_distances(kind, metric, d, form):
     off, ids, dd, M = g32.graph()
     r32 = _knn_recall(off[rows], dd, gt_d, K)
     g = _gpu(X, metric)
-    info = g.build(precision=PRECISION_FP32_MFMA if form.startswith("fp32") else PRECISION_BF16, **params)
+    info = g.build(precision=PRECISION_FP32_MFMA if form.startswith("fp32") else PRECISION_BF16,
+                   int8=None if metric == ANG else form != "bf16-exact", **params)
     c = g.build_counters()
     assert c["join_form"] == form
     off, ids, dd, M = g.graph()

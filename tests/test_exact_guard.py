@@ -111,3 +111,80 @@ def test_predicate_is_sufficient_on_extreme_rows(predicate):
             exact = float(((a.astype(np.int64) - o) ** 2).sum()) * (1.0 if metric == L2 else 0.5)
             tile, canon = _tile_and_canonical_f32(a, o, metric)
             assert float(tile) == exact and float(canon) == exact, (D, metric, mx, lo)
+
+
+PROGRAM_I8 = r'''
+#include <cstdio>
+#include "kg_exact.hpp"
+int main() {
+    unsigned D; int metric, u8;
+    while (std::scanf("%u %d %d", &D, &metric, &u8) == 3)
+        std::printf("%d\n", kga::i8_tiles_exact(D, metric, u8 != 0) ? 1 : 0);
+    return 0;
+}
+'''
+
+
+@pytest.fixture(scope="module")
+def predicate_i8(tmp_path_factory):
+    d = tmp_path_factory.mktemp("exact_i8")
+    src, exe = d / "p.cpp", d / "p"
+    src.write_text(PROGRAM_I8)
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I" + CSRC, str(src), "-o", str(exe)], check=True)
+
+    def run(cases):
+        inp = "".join("%d %d %d\n" % c for c in cases)
+        out = subprocess.run([str(exe)], input=inp, capture_output=True, text=True, check=True).stdout
+        return [int(v) for v in out.split()]
+    return run
+
+
+def test_i8_exact_cases(predicate_i8):
+    # 8-bit rows (integers in [0, 255]): exact while 255^2 D <= 2^24
+    cases = [
+        ((128, L2, 1), 1), ((128, EUC, 1), 1),
+        ((200, L2, 1), 1),    # 2 D 255^2 > 2^24: the BF16 form is not exact here, the 8-bit one is
+        ((256, L2, 1), 1),    # 255^2 256 = 16,646,400
+        ((258, EUC, 1), 1),   # 255^2 258 = 16,776,450
+        ((259, L2, 1), 0),    # 255^2 259 = 16,841,475 > 2^24
+        ((128, ANG, 1), 0),   # the angular distance is not shift-invariant
+        ((128, L2, 0), 0),    # no exact 8-bit copy
+    ]
+    assert predicate_i8([c for c, _ in cases]) == [e for _, e in cases]
+
+
+def _i8_tile_and_canonical_f32(a, o, metric):
+    """The 8-bit form's FP32 tile value (exact int32 norms and dot of the rows
+    shifted by -128, then (n_a + n_o) - 2 G, or the halved form) and a
+    sequential FP32 sum of squared differences (every partial sum is an
+    integer FP32 holds here, so any summation order gives this value)."""
+    f = np.float32
+    sa, so = a.astype(np.int64) - 128, o.astype(np.int64) - 128
+    na, no, g = int((sa * sa).sum()), int((so * so).sum()), int((sa * so).sum())
+    ssd = f(0)
+    for x, y in zip(a.astype(np.float32), o.astype(np.float32)):
+        d = f(x - y)
+        ssd = f(ssd + d * d)
+    if metric == L2:
+        return f(f(f(na) + f(no)) - f(2) * f(g)), ssd
+    return f(f(f(na) * f(0.5) + f(no) * f(0.5)) - f(g)), f(ssd * f(0.5))
+
+
+def test_i8_predicate_is_sufficient(predicate_i8):
+    # where the 8-bit predicate says "exact", extreme rows (all 255 against all
+    # 0, alternating) and random 8-bit rows give the exact integer through the
+    # tile formula and the canonical order
+    rs = np.random.RandomState(1)
+    Ds = [1, 3, 37, 64, 100, 128, 200, 255, 256, 258]
+    got = predicate_i8([(D, m, 1) for D in Ds for m in (L2, EUC)])
+    assert all(got)
+    for D in Ds:
+        for metric in (L2, EUC):
+            alt = np.where(np.arange(D) % 2 == 0, 255, 0)
+            pairs = [(np.full(D, 255), np.zeros(D, np.int64)), (alt, 255 - alt), (np.zeros(D, np.int64),
+                                                                                  np.zeros(D, np.int64))]
+            pairs += [(rs.randint(0, 256, D), rs.randint(0, 256, D)) for _ in range(5)]
+            for a, o in pairs:
+                exact = float(((a.astype(np.int64) - o) ** 2).sum()) * (1.0 if metric == L2 else 0.5)
+                tile, canon = _i8_tile_and_canonical_f32(a, o, metric)
+                assert float(tile) == exact and float(canon) == exact, (D, metric)

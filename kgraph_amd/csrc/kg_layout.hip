@@ -44,11 +44,13 @@ __global__ void __launch_bounds__(LB) lay_iota(unsigned N, uint32_t *lab) {
 // For every edge u -> v whose endpoints' labels differ, the entry of the
 // larger label takes the smaller one.  Labels only decrease, each is the id
 // of a node of the same weak component, and lab[x] <= x throughout.
+// `limit`: the first edges of each list taken (a first round on a few edges
+// merges most of a component at a fraction of the atomics).
 __global__ void __launch_bounds__(LB) lay_hook(unsigned N, uint32_t const *gpad, uint32_t const *gmd, unsigned W,
-                                               uint32_t *lab, uint32_t *changed) {
+                                               unsigned limit, uint32_t *lab, uint32_t *changed) {
     unsigned const u = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (u >= N) return;
-    unsigned const cnt = out_deg(gmd[u], W);
+    unsigned const cnt = min(out_deg(gmd[u], W), limit);
     uint32_t const *row = gpad + size_t(u) * W;
     bool ch = false;
     for (unsigned j = lane; j < cnt; j += KG_WAVE) {
@@ -155,11 +157,6 @@ __global__ void __launch_bounds__(LB) lay_emit(unsigned lo, unsigned hi, uint32_
     }
 }
 
-__global__ void __launch_bounds__(LB) lay_tree_sizes(unsigned R, uint32_t const *ord, uint32_t const *tree,
-                                                     uint32_t *tsize) {
-    for (unsigned i = blockIdx.x * LB + threadIdx.x; i < R; i += gridDim.x * LB) atomicAdd(&tsize[tree[ord[i]]], 1u);
-}
-
 // A level lists each tree's nodes in one run (level 0 is sorted by tree, and
 // children follow their parents).  first[t]: where tree t's run starts.
 __global__ void __launch_bounds__(LB) lay_first(unsigned lo, unsigned hi, uint32_t const *ord, uint32_t const *tree,
@@ -253,10 +250,14 @@ hipError_t kg_walk_order(unsigned N, uint32_t const *gpad, uint32_t const *gmd, 
     hipLaunchKernelGGL(lay_iota, dim3(gN), dim3(LB), 0, s, N, w.lab);
     unsigned rounds = 0;
     for (; rounds < 64; ++rounds) {
+        // round 0 on each list's first two edges; the rest on all of them
+        // until one changes nothing
         KG_TRY(hipMemsetAsync(w.word, 0, 4, s));
-        hipLaunchKernelGGL(lay_hook, dim3(waves_grid(N)), dim3(LB), 0, s, N, gpad, gmd, W, w.lab, w.word);
+        hipLaunchKernelGGL(lay_hook, dim3(waves_grid(N)), dim3(LB), 0, s, N, gpad, gmd, W, rounds ? W : 2u, w.lab,
+                           w.word);
         hipLaunchKernelGGL(lay_compress, dim3(gN), dim3(LB), 0, s, N, w.lab);
         KG_TRY(hipGetLastError());
+        if (rounds == 0) continue;
         uint32_t changed = 0;
         KG_TRY(read_word(w.word, changed));
         if (!changed) break;
@@ -293,10 +294,18 @@ hipError_t kg_walk_order(unsigned N, uint32_t const *gpad, uint32_t const *gmd, 
         lev.push_back(hi);
     }
     unsigned const R = hi;
-    // positions: trees in root order, a tree's levels in order
+    // positions: trees in root order, a tree's levels in order.  A first pass
+    // over the levels sums each tree's runs into its size (no atomics: a
+    // level holds one run per tree), the second places the nodes.
     KG_TRY(hipMemsetAsync(w.tsize, 0, size_t(N) * 4, s));
     KG_TRY(hipMemsetAsync(w.done, 0, size_t(N) * 4, s));
-    hipLaunchKernelGGL(lay_tree_sizes, dim3(grid_for(R)), dim3(LB), 0, s, R, w.ord, w.tree, w.tsize);
+    for (size_t l = 0; l + 1 < lev.size(); ++l) {
+        unsigned const a = lev[l], b = lev[l + 1];
+        if (b <= a) continue;
+        dim3 const g((b - a + LB - 1) / LB);
+        hipLaunchKernelGGL(lay_first, g, dim3(LB), 0, s, a, b, w.ord, w.tree, w.first);
+        hipLaunchKernelGGL(lay_done, g, dim3(LB), 0, s, a, b, w.ord, w.tree, w.first, w.tsize);
+    }
     KG_TRY(kg_scan_u32(w.tsize, w.tbase, N, w.tmp, s));
     for (size_t l = 0; l + 1 < lev.size(); ++l) {
         unsigned const a = lev[l], b = lev[l + 1];

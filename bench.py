@@ -69,6 +69,8 @@ def parse():
                          "--entry-sample E sweeps E alone)")
     ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-join-int8", action="store_true",
+                    help="keep the BF16 MFMA join on rows of integers in [0, 255] (default: the exact 8-bit form)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 1 GiB device to device (reference transfer for counter runs)")
     ap.add_argument("--cpu-queries", type=int, default=3000,
@@ -142,6 +144,8 @@ def main():
     h = _lib.Handle()
     _lib.lib().kg_set_verbosity(0)
     h.call("kg_set_data", ptr(X), N, D, D, _lib.METRIC_L2SQR)
+    if args.no_join_int8:
+        h.call("kg_set_join_int8", 0)
     rows, rb, lossless = row_storage(h)
     log("walks gather %s rows (lossless BF16 copy: %s)" % (rows, lossless))
     dQ = devmem.DeviceArray.from_host(Q)

@@ -337,6 +337,27 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         KG_HIP(hipMemcpyAsync(h.Ps.p, Ps, 4ull * nq, hipMemcpyHostToDevice, h.stream));
         KG_HIP(hipMemcpyAsync(h.pidx.p, pidx_h.data(), 4ull * nq, hipMemcpyHostToDevice, h.stream));
     }
+    // mixed P: walks start in order of decreasing P (a counting sort over the
+    // distinct values, stable), so the launch does not end on a long walk
+    // that started last; results do not depend on the order
+    std::vector<uint32_t> qorder_h;
+    if (Ps && pvals.size() > 1) {
+        std::vector<unsigned> rank(pvals.size());
+        for (unsigned i = 0; i < pvals.size(); ++i) rank[i] = i;
+        std::sort(rank.begin(), rank.end(), [&](unsigned x, unsigned y) { return pvals[x] > pvals[y]; });
+        std::vector<size_t> start(pvals.size() + 1, 0);
+        for (uint32_t p : pidx_h) ++start[p + 1];
+        std::vector<size_t> at(pvals.size());
+        size_t o = 0;
+        for (unsigned r : rank) {
+            at[r] = o;
+            o += start[r + 1];
+        }
+        qorder_h.resize(nq);
+        for (unsigned q = 0; q < nq; ++q) qorder_h[at[pidx_h[q]]++] = q;
+        h.qorder.reserve(nq);
+        KG_HIP(hipMemcpyAsync(h.qorder.p, qorder_h.data(), 4ull * nq, hipMemcpyHostToDevice, h.stream));
+    }
     // queries: aligned copy when the caller's layout does not allow float4 loads
     float const *Qk = dQ;
     size_t qs = qstride;
@@ -459,6 +480,7 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         a.Ks = Ks ? h.Ks.p : nullptr;
         a.Ps = Ps ? h.Ps.p : nullptr;
         a.pidx = Ps ? h.pidx.p : nullptr;
+        a.qorder = qorder_h.empty() ? nullptr : h.qorder.p;
         a.K0 = sp.K;
         a.P0 = sp.P;
         a.S = sp.S;

@@ -134,7 +134,7 @@ struct Handle {
 
     // search scratch
     DevBuf<uint32_t> visited, touched, qcounter;
-    DevBuf<uint32_t> seeds, seed_off, Ks, Ps, pidx, bsel;
+    DevBuf<uint32_t> seeds, seed_off, Ks, Ps, pidx, bsel, qorder;
     DevBuf<float> q, qn;
     DevBuf<uint64_t> bpart;  // brute-force partial top-K lists
     // device copies of host queries and results (kg_search, kg_bruteforce):

@@ -47,6 +47,10 @@ struct kg_search_args {
     uint32_t const *Ks;    // per-query K (NULL = K0)
     uint32_t const *Ps;    // per-query P (NULL = P0)
     uint32_t const *pidx;  // per-query index into seed_off (when Ps != NULL)
+    // order in which the walks start (NULL: query order): the i-th query a
+    // wave takes is qorder[i]; larger P first, so that the last walks to
+    // start are short ones
+    uint32_t const *qorder;
     unsigned K0, P0;
     unsigned S, T, Mparam, init;
     float epsilon;

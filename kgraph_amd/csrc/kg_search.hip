@@ -689,6 +689,10 @@ search_kernel(kg_search_args a) {
         qi = g.shfl(qi, 0u);
         if constexpr (QPW == 1) qi = uniform(qi);
         if (qi >= a.nq) break;
+        if (a.qorder) {  // longest walks first
+            qi = a.qorder[qi];
+            if constexpr (QPW == 1) qi = uniform(qi);
+        }
         uint64_t const t_start = __builtin_amdgcn_s_memrealtime();  // 100 MHz constant clock
         unsigned const K = a.Ks ? a.Ks[qi] : a.K0;
         unsigned const P = a.Ps ? a.Ps[qi] : a.P0;

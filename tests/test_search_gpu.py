@@ -545,7 +545,7 @@ def test_hand_traced_walks(case):
 def test_search_parity_at_the_headline_scale():
     """The 1M x 128 integer-mixture workload at its full size: the index the
     GPU builds with the workload's IndexParams (K=100, L=150, S=10, R=100, 15
-    iterations, reverse -1; the exact BF16 join), searched at the benchmark's
+    iterations, reverse -1; the exact 8-bit join), searched at the benchmark's
     picked setting (K=10, P=50, S=128, starting from each query's 9 nearest
     nodes of a 4,096-node entry sample).  On 300 of the workload's queries the
     GPU walk returns the oracle's ids, distances, result counts, comps and
@@ -558,7 +558,7 @@ def test_search_parity_at_the_headline_scale():
     Q = np.ascontiguousarray(datagen.integer_mixture(10_000, 128, 1000, 3, 5)[:300])
     g = kgraph_amd.KGraph(X, "euclidean", l2sqr=True)
     info = g.build(K=100, L=150, S=10, R=100, iterations=15, controls=100, reverse=-1, precision=PRECISION_BF16)
-    assert g.build_counters()["join_form"] == "bf16-exact" and info.recall > 0.95
+    assert g.build_counters()["join_form"] == "int8-exact" and info.recall > 0.95
     g.set_entry_sample(E, seed=1998)
     sel = g.select_entries(Q, m)
     samp = _entry_sample_ids(1998, E, N)

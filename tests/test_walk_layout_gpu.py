@@ -219,3 +219,32 @@ def test_layout_with_components_and_unreached_nodes():
     idx = kgo.Index()
     idx.import_(off, ids, dists, M)
     check_oracle(out[1], X, Q, idx, L2, K=10, P=30)
+
+
+def test_layout_follows_graph_and_data_changes():
+    # the layout and its row copies are remade after the graph or the rows
+    # change: walks after set_graph (another graph), prune and new rows of the
+    # same shape return the oracle's results for the new state
+    from kgraph_amd import _lib
+    X, Q, idx = setup("uniform", EUC, d=37)
+    g = gpu_index(X, idx.export(), EUC)
+    walk(g, Q, K=10, P=40)
+    assert order_state(g)[0] == 1
+    # another graph
+    idx2 = kgo.Index()
+    idx2.build(X, EUC, K=12, L=30, S=10, R=100, iterations=6, controls=20, reverse=0, seed=7)
+    g.set_graph(*idx2.export())
+    r = walk(g, Q, K=10, P=40)
+    assert order_state(g)[0] == 1
+    check_oracle(r, X, Q, idx2, EUC, K=10, P=40)
+    # pruned on the device (level 1 keeps the first min(M, K) edges of each list)
+    g._h.call("kg_prune", 1)
+    r = walk(g, Q, K=10, P=40)
+    idx3 = kgo.Index()
+    idx3.import_(*g.graph())
+    check_oracle(r, X, Q, idx3, EUC, K=10, P=40)
+    # new rows of the same shape under the same graph
+    X2 = np.ascontiguousarray(X[::-1])
+    g._h.call("kg_set_data", X2.ctypes.data, X2.shape[0], X2.shape[1], X2.shape[1], EUC)
+    r = walk(g, Q, K=10, P=40)
+    check_oracle(r, X2, Q, idx3, EUC, K=10, P=40)

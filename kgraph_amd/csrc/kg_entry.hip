@@ -489,6 +489,8 @@ static void launch_exact_ks(kg_entry_args const &a, float const *qsq, float cons
         hipLaunchKernelGGL((entry_exact_kernel<METRIC, KS, 4>), g, b, 0, s, a, qsq, esq);
     else if (a.m <= 8)
         hipLaunchKernelGGL((entry_exact_kernel<METRIC, KS, 8>), g, b, 0, s, a, qsq, esq);
+    else if (a.m <= 9)
+        hipLaunchKernelGGL((entry_exact_kernel<METRIC, KS, 9>), g, b, 0, s, a, qsq, esq);
     else if (a.m <= 12)
         hipLaunchKernelGGL((entry_exact_kernel<METRIC, KS, 12>), g, b, 0, s, a, qsq, esq);
     else

@@ -248,3 +248,41 @@ def test_layout_follows_graph_and_data_changes():
     g._h.call("kg_set_data", X2.ctypes.data, X2.shape[0], X2.shape[1], X2.shape[1], EUC)
     r = walk(g, Q, K=10, P=40)
     check_oracle(r, X2, Q, idx3, EUC, K=10, P=40)
+
+
+@pytest.mark.parametrize("shape", ["no-edges", "two-nodes", "path", "star"])
+def test_layout_on_small_and_degenerate_graphs(shape):
+    # graphs with no edges (every node its own component), two nodes, a long
+    # path (one level per node) and a star: the order is the documented one
+    # and walks match dataset order and the oracle
+    rs = np.random.RandomState(3)
+    N = {"no-edges": 50, "two-nodes": 2, "path": 600, "star": 300}[shape]
+    X = rs.rand(N, 8).astype(np.float32)
+    Q = rs.rand(20, 8).astype(np.float32)
+    if shape == "no-edges":
+        lists = [np.zeros(0, np.int64) for _ in range(N)]
+    elif shape == "two-nodes":
+        lists = [np.array([1]), np.array([0])]
+    elif shape == "path":  # i -> i + 1 (and the last node -> 0): N levels from node 0
+        lists = [np.array([(i + 1) % N]) for i in range(N)]
+    else:  # every node -> the hub 0, the hub -> everyone
+        lists = [np.arange(1, N)] + [np.array([0]) for _ in range(1, N)]
+    off = np.zeros(N + 1, np.uint64)
+    off[1:] = np.cumsum([len(l) for l in lists])
+    ids = np.concatenate(lists).astype(np.uint32) if off[-1] else np.zeros(0, np.uint32)
+    dists = np.concatenate([((X[l] - X[i]) ** 2).sum(1) for i, l in enumerate(lists)]).astype(np.float32) \
+        if off[-1] else np.zeros(0, np.float32)
+    M = np.array([max(1, len(l)) for l in lists], np.uint32)
+    g = gpu_index(X, (off, ids, dists, M), L2)
+    P = 1 if N == 2 else 5
+    out = {}
+    for order in (0, 1):
+        g._h.call("kg_set_search_order", order)
+        out[order] = walk(g, Q, K=1, P=P, S=10)
+    for a_, b_ in zip(out[0], out[1]):
+        assert np.array_equal(a_, b_)
+    active, levels, ms, perm = order_state(g, N)
+    assert np.array_equal(perm, bfs_layout(off, ids, M)), shape
+    idx = kgo.Index()
+    idx.import_(off, ids, dists, M)
+    check_oracle(out[1], X, Q, idx, L2, K=1, P=P, S=10)

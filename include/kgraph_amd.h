@@ -337,10 +337,11 @@ int kg_autotune_search(kg_handle *h, const float *dQ, unsigned nq, size_t qstrid
  * times the walk over the settings that leave its results unchanged and keeps
  * the fastest for later searches on this handle: the visited set (bitset
  * only, or an LDS set of 1024, 2048, 4096 or 8192 entries in front of it),
- * the adjacency rows (padded or CSR), the BF16 screen (where walks gather the
- * FP32 rows), queries per wave (1, or 2 when the screen is off) and the launch
- * shape (as kg_autotune_search).  One setting at a time, in that order, each
- * kept when faster.  chosen (may be NULL) receives 6 values: {waves per block,
+ * the adjacency rows (padded or CSR), with padded rows the walk layout
+ * (kg_set_search_order; kg_search_order reports the choice), the BF16 screen
+ * (where walks gather the FP32 rows), queries per wave (1, or 2 when the
+ * screen is off) and the launch shape (as kg_autotune_search).  One setting at
+ * a time, in that order, each kept when faster.  chosen (may be NULL) receives 6 values: {waves per block,
  * blocks per CU, visited-set entries (0: bitset only), adjacency (0 padded, 1
  * CSR), screen (0/1), queries per wave}. */
 int kg_autotune_walk(kg_handle *h, const float *dQ, unsigned nq, size_t qstride, const kg_search_params *params,

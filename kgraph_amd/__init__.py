@@ -268,9 +268,10 @@ class KGraph:
 
     def autotune_walk(self, query, K=default_K, P=default_P, S=default_S, *, Ks=None, Ps=None, entries=0):
         """Times the walk on `query` over the settings that leave its results
-        unchanged (visited set, adjacency rows, BF16 screen, launch shape),
-        starting from the entry points of `entries` as search() would, and
-        keeps the fastest for later searches.  Returns the chosen settings."""
+        unchanged (visited set, adjacency rows, walk layout, BF16 screen,
+        queries per wave, launch shape), starting from the entry points of
+        `entries` as search() would, and keeps the fastest for later searches.
+        Returns the chosen settings."""
         from . import devmem
         q = f32c(query)
         nq = q.shape[0]
@@ -288,9 +289,11 @@ class KGraph:
         self._h.call("kg_autotune_walk", dq.ptr, nq, q.shape[1], sp, ptr(Ks_a), ptr(Ps_a),
                      init.ptr if init is not None else None, init.shape[1] if init is not None else 0, chosen)
         devmem.synchronize()
+        order = _lib.ctypes.c_int()
+        self._h.call("kg_search_order", _lib.ctypes.byref(order), None, None, None)
         return {"waves_per_block": chosen[0], "blocks_per_cu": chosen[1], "visited_entries": chosen[2],
                 "adjacency": "csr" if chosen[3] else "padded", "screen": bool(chosen[4]),
-                "queries_per_wave": chosen[5]}
+                "queries_per_wave": chosen[5], "layout": "breadth-first" if order.value else "dataset"}
 
     def build_counters(self):
         """Last build: {'pairs', 'list_entries', 'pushes', 'iterations',

@@ -1450,6 +1450,7 @@ int kg_autotune_walk(kg_handle *k, float const *dQ, unsigned nq, size_t qstride,
             return t;
         };
         int const old_visited = h.search_visited, old_adj = h.search_adjacency, old_screen = h.search_screen;
+        int const old_order = h.search_order;
         unsigned const old_vcap = h.search_vcap, old_wpb = h.search_wpb, old_bpc = h.search_bpc;
         unsigned const old_qpw = h.search_qpw;
         try {
@@ -1486,6 +1487,20 @@ int kg_autotune_walk(kg_handle *k, float const *dQ, unsigned nq, size_t qstride,
                 }
             }
             h.search_adjacency = badj;
+            // 2b. with the padded rows: the walk layout, or dataset order (a
+            // switch remakes the padded rows in the other order)
+            if (h.search_adjacency == 0 && h.gpad_stride) {
+                int const bord = h.search_order;
+                h.search_order = 1 - bord;
+                h.gnode_valid = false;
+                float const t = run();
+                if (t < best) {
+                    best = t;
+                } else {
+                    h.search_order = bord;
+                    h.gnode_valid = false;
+                }
+            }
             // 3. the BF16 screen, where walks gather the FP32 rows
             int bscr = h.search_screen;
             if (walk_rows(h) == 0 && h.Dp <= 256) {
@@ -1542,6 +1557,10 @@ int kg_autotune_walk(kg_handle *k, float const *dQ, unsigned nq, size_t qstride,
             h.search_wpb = old_wpb;
             h.search_bpc = old_bpc;
             h.search_qpw = old_qpw;
+            if (h.search_order != old_order) {
+                h.search_order = old_order;
+                h.gnode_valid = false;
+            }
             throw;
         }
     });

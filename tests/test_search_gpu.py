@@ -447,8 +447,8 @@ def test_autotuned_launch_shape_keeps_results():
 
 @pytest.mark.parametrize("kind,metric,d,entries", [("uniform", EUC, 16, 0), ("angular", ANG, 96, 7)])
 def test_walk_autotuner_keeps_results(kind, metric, d, entries):
-    # every setting the walk autotuner tries (visited set, adjacency rows,
-    # BF16 screen, launch shape) returns the same walk, with random starts
+    # every setting the walk autotuner tries (visited set, adjacency rows, walk
+    # layout, BF16 screen, launch shape) returns the same walk, with random starts
     # and with selected entry points
     X, Q, idx = setup(kind, metric, d=d)
     g = gpu_index(X, idx, metric)
@@ -461,6 +461,7 @@ def test_walk_autotuner_keeps_results(kind, metric, d, entries):
     assert got["visited_entries"] in (0, 256, 512, 1024, 2048, 4096, 8192)
     assert got["adjacency"] in ("padded", "csr")
     assert got["screen"] in (False, True)
+    assert got["layout"] in ("breadth-first", "dataset")
     after = g.search(Q, K=10, P=50, withDistance=True, entries=entries)
     assert np.array_equal(before[0], after[0]) and np.array_equal(before[1], after[1])
 

@@ -116,7 +116,7 @@ int kg_set_precision(kg_handle *h, int precision);
  * else 2048, when that keeps at least 12 waves per CU resident; else the
  * largest power of two up to 4096 that keeps 16.  Mode 1: the
  * bitset only.  Mode 2: as mode 0 with 32-bit entries.
- * kg_set_search_visited_size fixes the entry count (a power of two in
+ * kg_set_search_visited_size fixes the entry count (2^k or 3 * 2^k in
  * [256, 8192]; 0 restores the choice above).  Results are identical in every
  * form. */
 int kg_set_search_visited(kg_handle *h, int mode);

@@ -398,10 +398,23 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         // (256..4096) that keeps 16 waves
         unsigned idbits = 1;
         while (idbits < 32 && (uint64_t(1) << idbits) < N) ++idbits;
-        auto bits_for = [&](unsigned c) {
-            unsigned const t = unsigned(__builtin_ctz(c));
-            return (h.search_visited == 0 && idbits <= t + 10) ? 16u : 32u;
+        // a set of c 2^k entries (c = 1 or 3): 16-bit entries hash ids of
+        // B = max(idbits, k + c / 3) bits; S = B - k; the remainder takes
+        // S (c = 1) or S - 1 (c = 3) bits, at most 10
+        struct VtGeom {
+            bool c3;
+            unsigned k, B, S, rb;
         };
+        auto geom = [&](unsigned c) {
+            VtGeom g{};
+            g.c3 = (c & (c - 1)) != 0;
+            g.k = unsigned(__builtin_ctz(g.c3 ? c / 3 : c));
+            g.B = std::max(idbits, g.k + (g.c3 ? 1u : 0u));
+            g.S = g.B - g.k;
+            g.rb = g.c3 ? g.S - 1 : g.S;
+            return g;
+        };
+        auto bits_for = [&](unsigned c) { return (h.search_visited == 0 && geom(c).rb <= 10) ? 16u : 32u; };
         auto lds_for = [&](unsigned c) {
             return kg_search_lds_per_wave(poolmax, rcap, h.Dp, seed_cap, c, c ? bits_for(c) : 32u, !pad, screen);
         };
@@ -514,8 +527,12 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         a.vcap = vcap;
         a.spills = h.qcounter.p + 1;
         a.vt_bits = vt_bits;
-        a.vt_rb = vt_bits == 16 ? std::max(idbits, unsigned(__builtin_ctz(vcap))) - unsigned(__builtin_ctz(vcap)) : 0u;
-        a.vt_idmask = vt_bits == 16 ? uint32_t((uint64_t(1) << (a.vt_rb + __builtin_ctz(vcap))) - 1) : 0u;
+        if (vt_bits == 16) {
+            VtGeom const g = geom(vcap);
+            a.vt_rb = g.rb;
+            a.vt_shift = g.S;
+            a.vt_idmask = uint32_t((uint64_t(1) << g.B) - 1);
+        }
         h.last_vt_bits = vt_bits;
         a.abort_flag = sp.init ? bad_init : nullptr;
         h.last_vcap = vcap;
@@ -1011,8 +1028,9 @@ int kg_search_screen_dists(kg_handle *k, float const *Q, unsigned nq, size_t qst
 
 int kg_set_search_visited_size(kg_handle *k, unsigned entries) {
     return guard(k, [&](Handle &h) {
-        if (entries && (entries < 256 || entries > 8192 || (entries & (entries - 1))))
-            throw kga::invalid_argument("visited-set size must be 0 or a power of two in [256, 8192]");
+        unsigned const k = entries % 3u == 0u ? entries / 3u : entries;  // 3 * 2^k or 2^k entries
+        if (entries && (entries < 256 || entries > 8192 || (k & (k - 1))))
+            throw kga::invalid_argument("visited-set size must be 0, 2^k or 3 * 2^k in [256, 8192]");
         h.search_vcap = entries;
     });
 }

@@ -66,11 +66,12 @@ struct kg_search_args {
     // per-wave scratch
     uint32_t *visited;
     unsigned vis_words;
-    unsigned vcap;  // entries of the per-wave LDS visited set (power of two >= 256, or 0: bitset only)
-    // entry width of that set: 32 (the id), or 16 for ids below 2^idbits
-    // (idbits <= log2(vcap) + 10): the low vt_rb = idbits - log2(vcap) bits of
-    // the hashed id and the probe displacement; vt_idmask = 2^idbits - 1
-    unsigned vt_bits, vt_rb, vt_idmask;
+    unsigned vcap;  // entries of the per-wave LDS visited set (2^k or 3 * 2^k >= 256, or 0: bitset only)
+    // entry width of that set: 32 (the id), or 16 for hashed ids below 2^B
+    // (vt_idmask = 2^B - 1) with vcap = c 2^k: home slot (c p) >> vt_shift
+    // (vt_shift = B - k), remainder of vt_rb bits (B - k for c = 1, B - k - 1
+    // for c = 3, at most 10) beside the probe displacement
+    unsigned vt_bits, vt_rb, vt_idmask, vt_shift;
     uint32_t *spills;  // counts queries whose LDS set moved into the bitset (may be NULL)
     uint32_t *touched;
     unsigned touch_cap;

@@ -549,48 +549,59 @@ __device__ __forceinline__ uint32_t load_u32_l2(uint32_t const *p) {
 }
 
 // Visited hash set in LDS.  Ids are < 2^31, so VT_EMPTY never collides; the
-// set is kept at most half full plus one batch, so a probe always ends.
+// set is kept at most half full plus one batch, so a probe always ends.  Its
+// size is 2^k or 3 * 2^k entries: a hashed value's home slot is its product
+// with the size, shifted (multiply-shift range reduction), and probes wrap.
 constexpr uint32_t VT_EMPTY = 0xffffffffu;
-__device__ __forceinline__ unsigned vt_hash(uint32_t id, unsigned shift) { return (id * 2654435761u) >> shift; }
+__device__ __forceinline__ unsigned vt_next(unsigned h, unsigned vcap) { return h + 1u == vcap ? 0u : h + 1u; }
+__device__ __forceinline__ unsigned vt_home32(uint32_t id, unsigned vcap) {
+    return unsigned((uint64_t(id * 2654435761u) * vcap) >> 32);
+}
 // true when id was not in the set (it is now)
-__device__ __forceinline__ bool vt_insert(uint32_t *vt, unsigned mask, unsigned shift, uint32_t id) {
-    unsigned h = vt_hash(id, shift);
+__device__ __forceinline__ bool vt_insert(uint32_t *vt, unsigned vcap, uint32_t id) {
+    unsigned h = vt_home32(id, vcap);
     for (;;) {
         uint32_t const old = atomicCAS(&vt[h], VT_EMPTY, id);
         if (old == VT_EMPTY) return true;
         if (old == id) return false;
-        h = (h + 1) & mask;
+        h = vt_next(h, vcap);
     }
 }
-__device__ __forceinline__ bool vt_contains(uint32_t const *vt, unsigned mask, unsigned shift, uint32_t id) {
-    unsigned h = vt_hash(id, shift);
+__device__ __forceinline__ bool vt_contains(uint32_t const *vt, unsigned vcap, uint32_t id) {
+    unsigned h = vt_home32(id, vcap);
     for (;;) {
         uint32_t const v = vt[h];
         if (v == id) return true;
         if (v == VT_EMPTY) return false;
-        h = (h + 1) & mask;
+        h = vt_next(h, vcap);
     }
 }
 
-// Compact form: 16-bit entries, two per LDS word, for ids below 2^idbits.
-// p = id * 2654435761 mod 2^idbits is a bijection; its top log2(vcap) bits
-// are the home slot and its low rb bits the remainder.  Linear probing stores
-// (displacement << rb) | remainder, so a slot and its entry give back the home
-// slot and with it the id (p * 2654435761^-1).  Displacements stop at dmax, so
-// no entry is all ones (free); a probe that would go further reports overflow
-// and the caller moves the set into the bitset.
+// Compact form: 16-bit entries, two per LDS word, for ids below 2^B
+// (B = idbits).  p = id * 2654435761 mod 2^B is a bijection.  With a size
+// of c 2^k entries (c = 1 or 3) and S = B - k, p's home slot is
+// h = (c p) >> S and its remainder p - start(h), start(h) = ceil(h 2^S / c),
+// which takes rb = S bits (c = 1) or S - 1 (c = 3).  Linear probing stores
+// (displacement << rb) | remainder, so a slot and its entry give back the
+// home slot, p and the id (p * 2654435761^-1).  Displacements stop at dmax,
+// so no entry is all ones (free); a probe that would go further reports
+// overflow and the caller moves the set into the bitset.
 constexpr uint32_t VT_HASH_INV = 244002641u;  // 2654435761 * 244002641 = 1 mod 2^32
 struct VtCfg {
     bool b16;
-    unsigned mask, shift;             // 32-bit entries
-    unsigned rb, rmask, dmax, idmask;  // 16-bit entries
+    unsigned vcap;                           // entries
+    bool c3;                                 // vcap = 3 * 2^k (else 2^k)
+    unsigned S, rb, rmask, dmax, idmask;     // 16-bit entries
 };
+__device__ __forceinline__ uint32_t vt16_start(VtCfg const &c, unsigned h) {
+    return c.c3 ? ((h << c.S) + 2u) / 3u : (h << c.S);
+}
 enum : unsigned { VT_PRESENT = 0, VT_FRESH = 1, VT_OVERFLOW = 2 };
 __device__ __forceinline__ unsigned vt16_insert(uint32_t *vt, VtCfg const &c, uint32_t id) {
     uint32_t const p = (id * 2654435761u) & c.idmask;
-    uint32_t const r = p & c.rmask;
-    unsigned slot = p >> c.rb;
-    for (unsigned d = 0; d <= c.dmax; ++d, slot = (slot + 1) & c.mask) {
+    unsigned slot = ((c.c3 ? 3u : 1u) * p) >> c.S;
+    uint32_t const r = p - vt16_start(c, slot);
+    for (unsigned d = 0; d <= c.dmax; ++d, slot = vt_next(slot, c.vcap)) {
         uint32_t const want = (d << c.rb) | r;
         uint32_t *const word = vt + (slot >> 1);
         unsigned const sh = (slot & 1u) * 16u;
@@ -608,9 +619,9 @@ __device__ __forceinline__ unsigned vt16_insert(uint32_t *vt, VtCfg const &c, ui
 }
 __device__ __forceinline__ bool vt16_contains(uint32_t const *vt, VtCfg const &c, uint32_t id) {
     uint32_t const p = (id * 2654435761u) & c.idmask;
-    uint32_t const r = p & c.rmask;
-    unsigned slot = p >> c.rb;
-    for (unsigned d = 0; d <= c.dmax; ++d, slot = (slot + 1) & c.mask) {
+    unsigned slot = ((c.c3 ? 3u : 1u) * p) >> c.S;
+    uint32_t const r = p - vt16_start(c, slot);
+    for (unsigned d = 0; d <= c.dmax; ++d, slot = vt_next(slot, c.vcap)) {
         uint32_t const e = (vt[slot >> 1] >> ((slot & 1u) * 16u)) & 0xffffu;
         if (e == ((d << c.rb) | r)) return true;
         if (e == 0xffffu) return false;
@@ -619,15 +630,16 @@ __device__ __forceinline__ bool vt16_contains(uint32_t const *vt, VtCfg const &c
 }
 // id of the 16-bit entry e held in `slot`
 __device__ __forceinline__ uint32_t vt16_id(VtCfg const &c, unsigned slot, uint32_t e) {
-    unsigned const home = (slot - (e >> c.rb)) & c.mask;
-    return (((home << c.rb) | (e & c.rmask)) * VT_HASH_INV) & c.idmask;
+    unsigned const d = e >> c.rb;
+    unsigned const home = slot >= d ? slot - d : slot + c.vcap - d;
+    return ((vt16_start(c, home) + (e & c.rmask)) * VT_HASH_INV) & c.idmask;
 }
 __device__ __forceinline__ unsigned vt_put(uint32_t *vt, VtCfg const &c, uint32_t id) {
     if (c.b16) return vt16_insert(vt, c, id);
-    return vt_insert(vt, c.mask, c.shift, id) ? VT_FRESH : VT_PRESENT;
+    return vt_insert(vt, c.vcap, id) ? VT_FRESH : VT_PRESENT;
 }
 __device__ __forceinline__ bool vt_has(uint32_t const *vt, VtCfg const &c, uint32_t id) {
-    return c.b16 ? vt16_contains(vt, c, id) : vt_contains(vt, c.mask, c.shift, id);
+    return c.b16 ? vt16_contains(vt, c, id) : vt_contains(vt, c.vcap, id);
 }
 
 #ifndef SEARCH_WAVES_PER_EU
@@ -670,8 +682,9 @@ search_kernel(kg_search_args a) {
     w.vt = reinterpret_cast<uint32_t *>(base + lay.vt);
     VtCfg vs;
     vs.b16 = a.vt_bits == 16;
-    vs.mask = a.vcap - 1u;
-    vs.shift = a.vcap ? 32u - unsigned(__builtin_ctz(a.vcap)) : 0u;
+    vs.vcap = a.vcap;
+    vs.c3 = a.vcap && (a.vcap & (a.vcap - 1u)) != 0u;
+    vs.S = a.vt_shift;
     vs.rb = a.vt_rb;
     vs.rmask = (1u << a.vt_rb) - 1u;
     vs.dmax = (1u << (16u - a.vt_rb)) - 2u;

@@ -163,6 +163,46 @@ def test_lds_visited_set_gives_identical_walks(P, S, T):
     assert np.array_equal(r[3], comps) and np.array_equal(r[4], scans)
 
 
+@pytest.mark.parametrize("P,S,T", [(40, 10, 1), (300, 10, 1), (20, 64, 3)])
+def test_visited_sets_of_three_times_a_power_of_two(P, S, T):
+    # sets of 3 * 2^k entries (home slot by multiply-shift, probes wrap at the
+    # size; 16-bit entries keep a remainder of S - 1 bits): 384 entries make
+    # queries spill mid-walk, 3072 hold them; 16- and 32-bit entries; the
+    # same walks as the bitset alone and the oracle
+    import ctypes
+    X, Q, idx = setup("intmix", L2, d=128)
+    g = gpu_index(X, idx, L2)
+    res = {}
+    for mode, size in [(1, 0), (0, 384), (0, 3072), (2, 384), (2, 3072)]:
+        g._h.call("kg_set_search_visited", mode)
+        g._h.call("kg_set_search_visited_size", size)
+        gi, gd = g.search(Q, K=10, P=P, S=S, T=T, withDistance=True, stats=True)
+        vc, sp, bits = ctypes.c_uint(), ctypes.c_uint(), ctypes.c_uint()
+        g._h.call("kg_search_visited_stats", ctypes.byref(vc), ctypes.byref(sp))
+        g._h.call("kg_search_visited_bits", ctypes.byref(bits))
+        res[mode, size] = (gi, gd, g.last_counts.copy(), g.last_comps.copy(), g.last_scans.copy(), vc.value,
+                           sp.value, bits.value)
+        print("P=%d mode %d size %d: LDS set %d entries of %d bits, %d of %d queries spilled" % (
+            P, mode, size, vc.value, bits.value, sp.value, len(Q)))
+    g._h.call("kg_set_search_visited", 0)
+    g._h.call("kg_set_search_visited_size", 0)
+    for size in (384, 3072):
+        if P <= 40:  # (a P = 300 pool leaves no LDS room for 3072 32-bit entries at 16 waves)
+            assert res[0, size][5] == size and res[0, size][7] == 16
+            assert res[2, size][5] == size and res[2, size][7] == 32
+    assert res[0, 384][6] > 0
+    for key in res:
+        for a_, b_ in zip(res[1, 0][:5], res[key][:5]):
+            assert np.array_equal(a_, b_), key
+    ids, dd, c, comps, scans = idx.search(X, Q, L2, K=10, P=P, S=S, T=T)
+    r = res[0, 3072]
+    assert np.array_equal(r[2], c)
+    for q in range(len(Q)):
+        assert np.array_equal(r[0][q, :c[q]], ids[q, :c[q]]), q
+        assert np.array_equal(r[1][q, :c[q]], dd[q, :c[q]]), q
+    assert np.array_equal(r[3], comps) and np.array_equal(r[4], scans)
+
+
 @pytest.mark.parametrize("P,S,M,T", [(40, 10, 0, 1), (10, 64, 0, 1), (100, 80, 0, 1), (20, 10, 30, 3),
                                      (30, 10, 1000, 1)])
 def test_padded_adjacency_gives_identical_walks(P, S, M, T):

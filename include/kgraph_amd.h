@@ -160,6 +160,14 @@ int kg_search_order(kg_handle *h, int *active, unsigned *levels, float *ms, unsi
  * search reads the index as built): per-node records, the padded adjacency,
  * the walk layout and its copies of the rows the current settings gather. */
 int kg_prepare_search(kg_handle *h);
+/* Lean walk (no counterpart in the reference): 1 (default) = a batch of
+ * queries over the 8-bit rows that starts from caller-given points (init > 0,
+ * T = 1, no BF16 screen, one query per wave) is walked by a form of the walk
+ * without random seeds, trial merge or FP32 query copy, which fits 5 waves per
+ * SIMD, when every query is an integer vector in [0, 255] (checked on the
+ * device; otherwise the general walk runs); 0 = the general walk only.
+ * Results are identical. */
+int kg_set_search_lean(kg_handle *h, int on);
 
 /* BF16 screen of the graph walk's candidates (the reference scores every
  * fresh neighbour in full precision before UpdateKnnList, kgraph.cpp:497-512).

@@ -109,6 +109,7 @@ SIGNATURES = {
     "kg_set_search_order": (_I, [_P, _I]),
     "kg_search_order": (_I, [_P, _P, _P, _P, _P]),
     "kg_prepare_search": (_I, [_P]),
+    "kg_set_search_lean": (_I, [_P, _I]),
     "kg_set_join_int8": (_I, [_P, _I]),
     "kg_get_graph": (_I, [_P, _P, _P, _P, _P]),
     "kg_last_time_ms": (_I, [_P, _I, _P]),

@@ -435,6 +435,47 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         unsigned const vt_bits = vcap ? bits_for(vcap) : 32u;
         size_t const lpq = lds_for(vcap);  // one query's LDS
         size_t const lpw = lpq * qpw;
+        // exact integer distances over the 8-bit rows (integer queries):
+        // sums stay below 2^24 while D 255^2 < 2^24
+        bool const int_dot = form == 2 && h.row_storage == 0 && h.D <= 258;
+        // The lean walk (integer queries over the 8-bit rows from the caller's
+        // starting points, one trial) holds 5 waves per SIMD: it runs when a
+        // visited set of 4096 or 3072 entries (or the fixed size, or the
+        // bitset only) keeps its LDS within 20 waves per CU (a 2048-entry set
+        // spills most queries at P = 100 and loses more than the waves gain),
+        // beside the general walk, and the batch's query check decides on the
+        // device which of the two launches walks (the other returns at once).
+        bool lean = h.search_lean && int_dot && sp.init > 0 && sp.T == 1 && !screen && qpw == 1;
+        unsigned lean_vcap = 0, lean_bits = 32, lean_wpb = 0, lean_blocks = 0;
+        size_t lean_lpq = 0;
+        if (lean) {
+            unsigned const lw = kg_search_lean_waves_per_cu();
+            auto lean_lds = [&](unsigned c) {
+                return kg_search_lds_per_wave(poolmax, 0, h.Dp, seed_cap, c, c ? bits_for(c) : 32u, !pad, false, true);
+            };
+            lean_vcap = ~0u;
+            for (unsigned c : {4096u, 3072u}) {
+                if (h.search_visited == 1) c = 0u;
+                else if (h.search_vcap) c = h.search_vcap;
+                if (lean_vcap == ~0u && h.lds_per_cu / lean_lds(c) >= lw) lean_vcap = c;
+            }
+            if (lean_vcap == ~0u) {
+                lean = false;  // no set size leaves the lean walk more resident waves
+            } else {
+                lean_bits = lean_vcap ? bits_for(lean_vcap) : 32u;
+                lean_lpq = lean_lds(lean_vcap);
+                lean_wpb = std::min(h.search_wpb, kg_search_max_wpb());
+                while (lean_wpb > 1 && lean_lpq * lean_wpb > 65536) lean_wpb /= 2;
+                size_t const llpb = lean_lpq * lean_wpb;
+                if (llpb > 65536) {
+                    lean = false;
+                } else {
+                    unsigned lbpc = unsigned(std::max<size_t>(1, std::min<size_t>(lw / lean_wpb, h.lds_per_cu / llpb)));
+                    if (h.search_bpc) lbpc = std::min(lbpc, h.search_bpc);
+                    lean_blocks = std::min<unsigned>((ngraph + lean_wpb - 1) / lean_wpb, h.num_cus * lbpc);
+                }
+            }
+        }
         unsigned wpb = std::min(h.search_wpb, kg_search_max_wpb());  // the kernel's launch bound
         while (wpb > 1 && lpw * wpb > 65536) wpb /= 2;
         size_t lpb = lpw * wpb;
@@ -446,7 +487,9 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         unsigned words = (N + 31) / 32;
         size_t budget = size_t(8) << 30;  // visited bitsets: at most 8 GiB
         while (blocks > 1 && size_t(blocks) * wpb * qpw * words * 4 > budget) blocks /= 2;
-        unsigned slots = blocks * wpb * qpw;  // one visited bitset per query slot
+        while (lean && lean_blocks > 1 && size_t(lean_blocks) * lean_wpb * words * 4 > budget) lean_blocks /= 2;
+        // one visited bitset per query slot (of the larger launch of a pair)
+        unsigned slots = std::max(blocks * wpb * qpw, lean ? lean_blocks * lean_wpb : 0u);
         unsigned const tcap = 4096;
         if (slots > h.vis_slots || words != h.vis_words) {
             h.visited.release();
@@ -483,9 +526,7 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         a.gstride = pad ? h.gpad_stride : 0u;
         h.last_adjacency = pad ? 0 : 1;
         a.lists_unique = h.lists_unique ? 1u : 0u;
-        // exact integer distances over the 8-bit rows (integer queries):
-        // sums stay below 2^24 while D 255^2 < 2^24
-        a.int_dot = (form == 2 && h.row_storage == 0 && h.D <= 258) ? 1u : 0u;
+        a.int_dot = int_dot ? 1u : 0u;
         a.Q = Qk;
         a.qnorms = h.qn.p;
         a.nq = nq;
@@ -537,9 +578,37 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         a.abort_flag = sp.init ? bad_init : nullptr;
         h.last_vcap = vcap;
         a.seed_cap = seed_cap;
-        KG_HIP(hipEventRecord(h.ev[0], h.stream));
-        KG_HIP(kg_launch_search(h.metric, a, blocks, wpb, qpw, h.stream));
-        KG_HIP(hipEventRecord(h.ev[1], h.stream));
+        h.last_lean = lean;
+        if (lean) {
+            // the batch's query check, then the lean / general pair
+            h.leanflag.reserve(1);
+            KG_HIP(hipMemsetAsync(h.leanflag.p, 0, 4, h.stream));
+            KG_HIP(kg_launch_q8check(Qk, nq, h.D, qs, h.leanflag.p, h.stream));
+            a.lean_flag = h.leanflag.p;
+            kg_search_args al = a;
+            al.lean = 1;
+            al.res_cap = 0;
+            al.vcap = lean_vcap;
+            al.vt_bits = lean_bits;
+            al.vt_rb = al.vt_shift = al.vt_idmask = 0;
+            if (lean_bits == 16) {
+                VtGeom const g = geom(lean_vcap);
+                al.vt_rb = g.rb;
+                al.vt_shift = g.S;
+                al.vt_idmask = uint32_t((uint64_t(1) << g.B) - 1);
+            }
+            al.lds_per_wave = unsigned(lean_lpq);
+            h.last_lean_vcap = lean_vcap;
+            h.last_lean_bits = lean_bits;
+            KG_HIP(hipEventRecord(h.ev[0], h.stream));
+            KG_HIP(kg_launch_search(h.metric, al, lean_blocks, lean_wpb, 1, h.stream));
+            KG_HIP(kg_launch_search(h.metric, a, blocks, wpb, qpw, h.stream));
+            KG_HIP(hipEventRecord(h.ev[1], h.stream));
+        } else {
+            KG_HIP(hipEventRecord(h.ev[0], h.stream));
+            KG_HIP(kg_launch_search(h.metric, a, blocks, wpb, qpw, h.stream));
+            KG_HIP(hipEventRecord(h.ev[1], h.stream));
+        }
     }
     // brute-force queries (P >= N, kgraph.cpp:427-433)
     if (!brute.empty()) {
@@ -941,6 +1010,13 @@ int kg_prepare_search(kg_handle *k) {
     });
 }
 
+int kg_set_search_lean(kg_handle *k, int on) {
+    return guard(k, [&](Handle &h) {
+        if (on != 0 && on != 1) throw kga::invalid_argument("lean walk switch must be 0 or 1");
+        h.search_lean = on;
+    });
+}
+
 int kg_set_search_order(kg_handle *k, int mode) {
     return guard(k, [&](Handle &h) {
         if (mode != 0 && mode != 1) throw kga::invalid_argument("order mode must be 0 or 1");
@@ -1035,10 +1111,19 @@ int kg_set_search_visited_size(kg_handle *k, unsigned entries) {
     });
 }
 
+// Whether the last walk was the lean one of a pair (reads the batch's check).
+static bool lean_walked(Handle &h) {
+    if (!h.last_lean) return false;
+    uint32_t f = 1;
+    KG_HIP(hipMemcpyAsync(&f, h.leanflag.p, 4, hipMemcpyDeviceToHost, h.stream));
+    h.sync();
+    return f == 0;
+}
+
 int kg_search_visited_bits(kg_handle *k, unsigned *bits) {
     return guard(k, [&](Handle &h) {
         if (!bits) throw kga::invalid_argument("null argument");
-        *bits = h.last_vt_bits;
+        *bits = lean_walked(h) ? h.last_lean_bits : h.last_vt_bits;
     });
 }
 
@@ -1049,7 +1134,7 @@ int kg_search_visited_stats(kg_handle *k, unsigned *vcap, unsigned *spilled) {
             h.sync();
             KG_HIP(hipMemcpy(&sp, h.qcounter.p + 1, 4, hipMemcpyDeviceToHost));
         }
-        if (vcap) *vcap = h.last_vcap;
+        if (vcap) *vcap = lean_walked(h) ? h.last_lean_vcap : h.last_vcap;
         if (spilled) *spilled = sp;
     });
 }

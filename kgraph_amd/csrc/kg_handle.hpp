@@ -125,6 +125,12 @@ struct Handle {
     float worder_ms = 0.f;       // time to compute it and reorder the padded adjacency
     unsigned wforms = 0;         // row copies in that order: 1 FP32, 2 BF16, 4 8-bit, 8 norms
     int last_order = 0;          // whether the last walk launch read them
+    // the lean walk (kg_set_search_lean): integer queries over the 8-bit rows
+    // from caller-given starting points, launched beside the general walk
+    int search_lean = 1;
+    bool last_lean = false;      // the last walk was a lean / general pair
+    unsigned last_lean_vcap = 0, last_lean_bits = 0;
+    DevBuf<uint32_t> leanflag;   // the pair's query check (0: every query an integer vector in [0, 255])
     DevBuf<uint32_t> wperm, winv;
     DevBuf<float> wX, wnorms;
     DevBuf<uint16_t> wXb;

@@ -51,6 +51,13 @@ struct kg_search_args {
     // wave takes is qorder[i]; larger P first, so that the last walks to
     // start are short ones
     uint32_t const *qorder;
+    // lean = 1: this launch is the lean walk (8-bit rows, integer queries,
+    // caller's starting points, T = 1); lean_flag (may be NULL): a word that
+    // is 0 when every query of the batch is an integer vector in [0, 255] --
+    // of a lean / general pair of launches for one batch, only the lean one
+    // runs then, only the general one otherwise
+    unsigned lean;
+    uint32_t const *lean_flag;
     unsigned K0, P0;
     unsigned S, T, Mparam, init;
     float epsilon;
@@ -133,7 +140,7 @@ struct kg_rescore_args {
 
 // Launchers (defined in the .hip files).
 size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp, unsigned seed_cap, unsigned vcap,
-                              unsigned vt_bits, bool csr_base, bool screen);
+                              unsigned vt_bits, bool csr_base, bool screen, bool lean = false);
 // The walk's BF16 screen constants for rows of D values (a.scr_u / scr_c / scr_g).
 void kg_search_screen_constants(unsigned D, kg_search_args &a);
 // Screened distances and error bounds of m nodes per query (ids: a.nq x m),
@@ -146,6 +153,10 @@ hipError_t kg_launch_screen_dists(int metric, kg_search_args const &a, uint32_t 
 unsigned kg_search_max_wpb();
 // Waves per CU the kernel is compiled to keep resident (its register budget).
 unsigned kg_search_waves_per_cu();
+// The same for the lean walk (kg_search_args::lean).
+unsigned kg_search_lean_waves_per_cu();
+// *flag = 1 when a value of the queries is not an integer in [0, 255] (caller zeroes it).
+hipError_t kg_launch_q8check(float const *Q, unsigned nq, unsigned D, size_t stride, uint32_t *flag, hipStream_t s);
 // gnode[n] = {goff[n], deg(n), M[n], 0}; *dupflag |= 1 if some list repeats an id.
 hipError_t kg_launch_gnode(unsigned N, uint32_t const *goff, uint32_t const *gids, uint32_t const *gM, uint4 *gnode,
                            uint32_t *dupflag, hipStream_t s);

@@ -102,8 +102,9 @@ struct LdsLayout {
     // base: pool entries keep their CSR base offset (walks over the CSR
     // lists; the padded rows need none)
     // screen: the BF16 screen's query halves and re-score list
+    // lean: the lean walk keeps no FP32 copy of the query
     __host__ __device__ LdsLayout(unsigned cap, unsigned rcap, unsigned Dp, unsigned tcap, unsigned vcap,
-                                  unsigned vbytes, bool base, bool screen) {
+                                  unsigned vbytes, bool base, bool screen, bool lean = false) {
         size_t const d32 = screen ? (Dp + 31) / 32 * 32 : 0;  // BF16 values per query half
         size_t o = 0;
         pid = o; o = align16(o + 4ull * cap);
@@ -113,7 +114,7 @@ struct LdsLayout {
         tkey = o; o = align16(o + 8ull * tcap);
         rid = o; o = align16(o + 4ull * rcap);
         rdist = o; o = align16(o + 4ull * rcap);
-        q = o; o = align16(o + 4ull * Dp);
+        q = o; o = align16(o + (lean ? 0ull : 4ull * Dp));
         q8 = o; o = align16(o + Dp);
         qh = o; o = align16(o + 2 * d32);
         ql = o; o = align16(o + 2 * d32);
@@ -152,7 +153,8 @@ struct QInt {
 
 // G lanes per query: NQ = G / 4 quads, one row per quad a pass (FP32) or two
 // (8-bit, BF16).
-template <int METRIC, typename RowT, unsigned G = KG_WAVE>
+// INTONLY: every query takes the integer path (8-bit rows, integer queries).
+template <int METRIC, typename RowT, unsigned G = KG_WAVE, bool INTONLY = false>
 __device__ __forceinline__ void batch_dist(kg_search_args const &a, uint32_t const *ids, float *out, unsigned cnt,
                                            float const *q, float qn, uint32_t const *q8, QInt qv) {
     RowT const *X;
@@ -166,7 +168,7 @@ __device__ __forceinline__ void batch_dist(kg_search_args const &a, uint32_t con
     unsigned const lane = lane_id() % G;
     unsigned const grp = lane >> 2, t = lane & 3;
     if constexpr (sizeof(RowT) == 1) {
-        if (qv.on) {
+        if (INTONLY || qv.on) {
             // integer path: v_dot4 over 8-bit words, exact (kg_device.hpp
             // quad_dot2_u8), then the metric's FP32 finish on exact values
             for (unsigned b0 = 0; b0 < cnt; b0 += 2 * NQ) {
@@ -193,6 +195,7 @@ __device__ __forceinline__ void batch_dist(kg_search_args const &a, uint32_t con
             wave_sync();
             return;
         }
+        if constexpr (INTONLY) return;
         // 8-bit rows: one 128-byte line per 128-value row, two rows per quad
         for (unsigned b0 = 0; b0 < cnt; b0 += 2 * NQ) {
             unsigned const c0 = b0 + grp, c1 = c0 + NQ;
@@ -645,13 +648,21 @@ __device__ __forceinline__ bool vt_has(uint32_t const *vt, VtCfg const &c, uint3
 #ifndef SEARCH_WAVES_PER_EU
 #define SEARCH_WAVES_PER_EU 4
 #endif
+#ifndef SEARCH_LEAN_WAVES_PER_EU
+#define SEARCH_LEAN_WAVES_PER_EU 5
+#endif
 // SCREEN (RowT = float): each batch of a full pool goes through the BF16
 // screen (batch_screen) before the exact FP32 re-score.  QPW queries per
 // wave: each walks on G = 64 / QPW lanes with its own LDS area (a.lds_per_wave
 // bytes per query) and visited bitset; every loop steps G lanes.
-template <int METRIC, typename RowT, bool SCREEN = false, int QPW = 1>
-__global__ void __launch_bounds__(SEARCH_MAX_WPB * KG_WAVE) __attribute__((amdgpu_waves_per_eu(SEARCH_WAVES_PER_EU)))
+// LEAN: the walk of a batch of integer queries over the 8-bit rows from
+// caller-given starting points, one trial (no random seeds, no result merge,
+// no FP32 query copy): fewer registers and less LDS, so 5 waves per SIMD.
+template <int METRIC, typename RowT, bool SCREEN = false, int QPW = 1, bool LEAN = false>
+__global__ void __launch_bounds__(SEARCH_MAX_WPB * KG_WAVE)
+    __attribute__((amdgpu_waves_per_eu(LEAN ? SEARCH_LEAN_WAVES_PER_EU : SEARCH_WAVES_PER_EU)))
 search_kernel(kg_search_args a) {
+    static_assert(!LEAN || (sizeof(RowT) == 1 && !SCREEN && QPW == 1), "the lean walk: 8-bit rows, one query a wave");
     constexpr unsigned G = KG_WAVE / QPW;
     static_assert(!SCREEN || QPW == 1, "the BF16 screen runs one query per wave");
     extern __shared__ __align__(16) unsigned char smem[];
@@ -662,7 +673,7 @@ search_kernel(kg_search_args a) {
     unsigned const slot = blockIdx.x * (blockDim.x / KG_WAVE) * QPW + qslot;
     unsigned const cap = a.pool_cap;
     unsigned const rcap = a.res_cap;
-    LdsLayout lay(cap, rcap, a.Dp, a.seed_cap, a.vcap, a.vt_bits / 8, a.gpad == nullptr, SCREEN);
+    LdsLayout lay(cap, rcap, a.Dp, a.seed_cap, a.vcap, a.vt_bits / 8, a.gpad == nullptr, SCREEN, LEAN);
     unsigned char *base = smem + size_t(qslot) * a.lds_per_wave;
     WaveLDS w;
     w.pid = reinterpret_cast<uint32_t *>(base + lay.pid);
@@ -695,6 +706,9 @@ search_kernel(kg_search_args a) {
     uint32_t *touched = a.touched + size_t(slot) * a.touch_cap;
     uint64_t const lt_mask = g.lt();
     if (a.abort_flag && *a.abort_flag) return;  // a starting id is out of range: nothing is written
+    // a lean / general pair of launches for one batch: the lean walk runs when
+    // every query is an integer vector in [0, 255], the general one otherwise
+    if (a.lean_flag && LEAN != (*a.lean_flag == 0u)) return;
 
     for (;;) {
         unsigned qi = 0;
@@ -713,12 +727,29 @@ search_kernel(kg_search_args a) {
         unsigned const qcap = K + P + 1;
 
         float const *qsrc = a.Q + size_t(qi) * a.qstride;
-        for (unsigned i = lane; i < a.Dp; i += G) w.q[i] = (i < a.D) ? qsrc[i] : 0.f;
+        if constexpr (!LEAN)
+            for (unsigned i = lane; i < a.Dp; i += G) w.q[i] = (i < a.D) ? qsrc[i] : 0.f;
         float const qn = (METRIC == METRIC_L2SQR) ? 0.f : a.qnorms[qi];
         // over the 8-bit rows, a query whose values are all integers in
         // [0, 255] takes the integer path (exact, so the same distances)
         QInt qv{false, 0u};
-        if constexpr (sizeof(RowT) == 1) {
+        if constexpr (LEAN) {
+            // (every query of the batch is one: checked before the launch)
+            uint32_t sq = 0;
+            for (unsigned wd = lane; wd < a.Dp / 4; wd += G) {
+                uint32_t word = 0;
+                for (unsigned j = 0; j < 4; ++j) {
+                    unsigned const c = 4 * wd + j;
+                    uint32_t const u = c < a.D ? uint32_t(qsrc[c]) : 0u;
+                    word |= u << (8 * j);
+                    sq += u * u;
+                }
+                w.q8[wd] = word;
+            }
+            for (int o = G / 2; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
+            qv.on = true;
+            qv.sq = sq;
+        } else if constexpr (sizeof(RowT) == 1) {
             wave_sync();
             bool ok = true;
             uint32_t sq = 0;
@@ -778,10 +809,10 @@ search_kernel(kg_search_args a) {
             if (lane == 0 && a.spills) atomicAdd(a.spills, 1u);
             wave_sync();
         };
-        for (unsigned trial = 0; trial < a.T; ++trial) {
+        for (unsigned trial = 0; trial < (LEAN ? 1u : a.T); ++trial) {
             // ---- starting points (kgraph.cpp:450-467)
             unsigned L = 0;
-            if (a.init == 0) {
+            if (!LEAN && a.init == 0) {
                 uint32_t const *seeds = a.seeds + a.seed_off[a.Ps ? a.pidx[qi] : 0] + size_t(trial) * P;
                 for (unsigned c = 0; c < P; c += G) {
                     unsigned i = c + lane;
@@ -831,7 +862,7 @@ search_kernel(kg_search_args a) {
                     ntouch += L;
                 }
             }
-            batch_dist<METRIC, RowT, G>(a, w.pid, w.pdist, L, w.q, qn, w.q8, qv);
+            batch_dist<METRIC, RowT, G, LEAN>(a, w.pid, w.pdist, L, w.q, qn, w.q8, qv);
             // (dist, dataset row) order
             for (unsigned i = lane; i < L; i += G)
                 w.tkey[i] = (uint64_t(ord_f32(w.pdist[i])) << 32) | (a.perm ? a.perm[w.pid[i]] : w.pid[i]);
@@ -927,7 +958,7 @@ search_kernel(kg_search_args a) {
                             batch_screen<METRIC>(a, w, w.fid, w.fdist, cnt, qn, qrn, qsq, w.pdist[L - 1], n_res, true,
                                                  cand, old, bit);
                         else
-                            batch_dist<METRIC, RowT, G>(a, w.fid, w.fdist, cnt, w.q, qn, w.q8, qv);
+                            batch_dist<METRIC, RowT, G, LEAN>(a, w.fid, w.fdist, cnt, w.q, qn, w.q8, qv);
                         bool const fresh = cand && !(old & bit);
                         uint64_t const b = g.ballot(fresh);
                         unsigned const nf = popc64(b);
@@ -989,7 +1020,7 @@ search_kernel(kg_search_args a) {
                     if (SCREEN && L == qcap - 1)
                         batch_screen<METRIC>(a, w, w.fid, w.fdist, nf, qn, qrn, qsq, w.pdist[L - 1], n_res);
                     else
-                        batch_dist<METRIC, RowT, G>(a, w.fid, w.fdist, nf, w.q, qn, w.q8, qv);
+                        batch_dist<METRIC, RowT, G, LEAN>(a, w.fid, w.fdist, nf, w.q, qn, w.q8, qv);
                     // the nf inserts of this batch, done at once (batch_insert)
                     n_comps += nf;
                     batch_insert<G>(w, nf, cm, cb, qcap, L, k);
@@ -997,7 +1028,7 @@ search_kernel(kg_search_args a) {
             }
             // ---- merge this trial into the results (kgraph.cpp:515-529)
             if (L > K) L = K;
-            if (a.T == 1) {
+            if (LEAN || a.T == 1) {
                 RL = L;
             } else if (trial == 0) {
                 for (unsigned i = lane; i < L; i += G) {
@@ -1038,8 +1069,8 @@ search_kernel(kg_search_args a) {
             }
         }
         // ---- epsilon cut and output (kgraph.cpp:531-563)
-        uint32_t const *oid = (a.T == 1) ? w.pid : w.rid;
-        float const *odist = (a.T == 1) ? w.pdist : w.rdist;
+        uint32_t const *oid = (LEAN || a.T == 1) ? w.pid : w.rid;
+        float const *odist = (LEAN || a.T == 1) ? w.pdist : w.rdist;
         unsigned cut = RL;
         for (unsigned c = 0; c < RL; c += G) {
             unsigned i = c + lane;
@@ -1126,6 +1157,20 @@ __global__ void __launch_bounds__(256) gpad_kernel(unsigned N, uint32_t const *g
     if (lane == 0) gmd[n] = (min(deg, 0xffffu) << 16) | min(gM[n], 0xffffu);
 }
 
+// *flag = 1 when a value of the nq x D queries is not an integer in [0, 255]
+// (the caller zeroes it)
+__global__ void __launch_bounds__(256) q8check_kernel(float const *Q, unsigned nq, unsigned D, size_t stride,
+                                                      uint32_t *flag) {
+    size_t const total = size_t(nq) * D;
+    bool bad = false;
+    for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < total; i += size_t(gridDim.x) * 256) {
+        size_t const q = i / D, j = i - q * D;
+        float const v = Q[q * stride + j];
+        bad |= !(v >= 0.f && v <= 255.f && v == rintf(v));
+    }
+    if (ballot(bad) && (threadIdx.x & 63) == 0) *flag = 1u;
+}
+
 __global__ void __launch_bounds__(256) init_check_kernel(uint32_t const *ids, unsigned nq, size_t ostride,
                                                          unsigned init, unsigned N, uint32_t *flag,
                                                          uint32_t *host_flag) {
@@ -1191,6 +1236,14 @@ hipError_t kg_launch_gpad(unsigned N, uint32_t const *goff, uint32_t const *gids
     return hipGetLastError();
 }
 
+hipError_t kg_launch_q8check(float const *Q, unsigned nq, unsigned D, size_t stride, uint32_t *flag, hipStream_t s) {
+    size_t const total = size_t(nq) * D;
+    if (total == 0) return hipSuccess;
+    unsigned const blocks = unsigned(std::min<size_t>((total + 255) / 256, 2048));
+    hipLaunchKernelGGL(q8check_kernel, dim3(blocks), dim3(256), 0, s, Q, nq, D, stride, flag);
+    return hipGetLastError();
+}
+
 hipError_t kg_launch_init_check(uint32_t const *ids, unsigned nq, size_t ostride, unsigned init, unsigned N,
                                 uint32_t *flag, uint32_t *host_flag, hipStream_t s) {
     size_t const total = size_t(nq) * init;
@@ -1209,13 +1262,15 @@ hipError_t kg_launch_gnode(unsigned N, uint32_t const *goff, uint32_t const *gid
 }
 
 size_t kg_search_lds_per_wave(unsigned pool_cap, unsigned res_cap, unsigned Dp, unsigned seed_cap, unsigned vcap,
-                              unsigned vt_bits, bool csr_base, bool screen) {
-    return LdsLayout(pool_cap, res_cap, Dp, seed_cap, vcap, vt_bits / 8, csr_base, screen).total;
+                              unsigned vt_bits, bool csr_base, bool screen, bool lean) {
+    return LdsLayout(pool_cap, res_cap, Dp, seed_cap, vcap, vt_bits / 8, csr_base, screen, lean).total;
 }
 
 unsigned kg_search_max_wpb() { return SEARCH_MAX_WPB; }
 
 unsigned kg_search_waves_per_cu() { return 4 * SEARCH_WAVES_PER_EU; }
+
+unsigned kg_search_lean_waves_per_cu() { return 4 * SEARCH_LEAN_WAVES_PER_EU; }
 
 hipError_t kg_launch_search(int metric, kg_search_args const &a, unsigned blocks, unsigned waves_per_block,
                             unsigned queries_per_wave, hipStream_t s) {
@@ -1226,10 +1281,14 @@ hipError_t kg_launch_search(int metric, kg_search_args const &a, unsigned blocks
     // row form: the 8-bit copy, the BF16 copy, the FP32 rows behind the BF16
     // screen, or the FP32 rows; one or two queries per wave
     if (a.Xs && (a.X8 || a.Xb || a.Dp > 256 || queries_per_wave != 1)) return hipErrorInvalidValue;
+    if (a.lean && (!a.X8 || queries_per_wave != 1 || a.T != 1 || a.init == 0 || !a.lean_flag))
+        return hipErrorInvalidValue;
     auto go = [&](auto kernel8, auto kernel16, auto kernel32s, auto kernel32, auto kernel8x2, auto kernel16x2,
-                  auto kernel32x2) {
+                  auto kernel32x2, auto kernel8lean) {
         bool const two = queries_per_wave == 2;
-        if (a.X8)
+        if (a.lean)
+            hipLaunchKernelGGL(kernel8lean, grid, block, lds, s, a);
+        else if (a.X8)
             hipLaunchKernelGGL(two ? kernel8x2 : kernel8, grid, block, lds, s, a);
         else if (a.Xb)
             hipLaunchKernelGGL(two ? kernel16x2 : kernel16, grid, block, lds, s, a);
@@ -1243,19 +1302,19 @@ hipError_t kg_launch_search(int metric, kg_search_args const &a, unsigned blocks
             go(search_kernel<METRIC_L2SQR, uint8_t>, search_kernel<METRIC_L2SQR, uint16_t>,
                search_kernel<METRIC_L2SQR, float, true>, search_kernel<METRIC_L2SQR, float>,
                search_kernel<METRIC_L2SQR, uint8_t, false, 2>, search_kernel<METRIC_L2SQR, uint16_t, false, 2>,
-               search_kernel<METRIC_L2SQR, float, false, 2>);
+               search_kernel<METRIC_L2SQR, float, false, 2>, search_kernel<METRIC_L2SQR, uint8_t, false, 1, true>);
             break;
         case METRIC_EUCLIDEAN:
             go(search_kernel<METRIC_EUCLIDEAN, uint8_t>, search_kernel<METRIC_EUCLIDEAN, uint16_t>,
                search_kernel<METRIC_EUCLIDEAN, float, true>, search_kernel<METRIC_EUCLIDEAN, float>,
                search_kernel<METRIC_EUCLIDEAN, uint8_t, false, 2>, search_kernel<METRIC_EUCLIDEAN, uint16_t, false, 2>,
-               search_kernel<METRIC_EUCLIDEAN, float, false, 2>);
+               search_kernel<METRIC_EUCLIDEAN, float, false, 2>, search_kernel<METRIC_EUCLIDEAN, uint8_t, false, 1, true>);
             break;
         default:
             go(search_kernel<METRIC_ANGULAR, uint8_t>, search_kernel<METRIC_ANGULAR, uint16_t>,
                search_kernel<METRIC_ANGULAR, float, true>, search_kernel<METRIC_ANGULAR, float>,
                search_kernel<METRIC_ANGULAR, uint8_t, false, 2>, search_kernel<METRIC_ANGULAR, uint16_t, false, 2>,
-               search_kernel<METRIC_ANGULAR, float, false, 2>);
+               search_kernel<METRIC_ANGULAR, float, false, 2>, search_kernel<METRIC_ANGULAR, uint8_t, false, 1, true>);
             break;
     }
     return hipGetLastError();

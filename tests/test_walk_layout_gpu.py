@@ -286,3 +286,38 @@ def test_layout_on_small_and_degenerate_graphs(shape):
     idx = kgo.Index()
     idx.import_(off, ids, dists, M)
     check_oracle(out[1], X, Q, idx, L2, K=1, P=P, S=10)
+
+
+@pytest.mark.parametrize("metric", [L2, EUC, ANG])
+def test_lean_walk_gives_identical_walks(metric):
+    # the lean walk (integer queries over the 8-bit rows from selected entry
+    # points, one trial; 5 waves per SIMD with its own visited-set size) and
+    # the general walk return the same; a batch with one non-integer query
+    # is walked by the general launch of the pair
+    X, Q, idx = setup("intmix", metric, d=128)
+    g = gpu_index(X, idx.export(), metric)
+    g.set_entry_sample(512, seed=2)
+    Qf = Q.copy()
+    Qf[7, 3] += 0.5  # not an integer
+    rs = np.random.RandomState(4)
+    Ks = rs.choice([1, 10, 25], size=len(Q)).astype(np.uint32)
+    Ps = rs.choice([10, 40, 100], size=len(Q)).astype(np.uint32)
+    cases = [dict(K=10, P=50, S=128, entries=9), dict(K=25, P=20, S=10, entries=9),
+             dict(K=10, P=200, S=64, entries=9), dict(K=10, Ks=Ks, Ps=Ps, entries=9)]
+    for qs in (Q, Qf):
+        for kw in cases:
+            out = {}
+            for lean in (0, 1):
+                g._h.call("kg_set_search_lean", lean)
+                out[lean] = walk(g, qs, **kw)
+                vc, sp, bits = ctypes.c_uint(), ctypes.c_uint(), ctypes.c_uint()
+                g._h.call("kg_search_visited_stats", ctypes.byref(vc), ctypes.byref(sp))
+                g._h.call("kg_search_visited_bits", ctypes.byref(bits))
+                print(metric, kw.get("P"), "lean", lean, "non-integer" if qs is Qf else "integer",
+                      "set", vc.value, bits.value, "spilled", sp.value)
+            for a_, b_ in zip(out[0], out[1]):
+                assert np.array_equal(a_, b_), (kw, qs is Qf)
+    g._h.call("kg_set_search_lean", 1)
+    ids = g.select_entries(Q, 9)
+    r = walk(g, Q, K=10, P=50, S=128, entries=9)
+    check_oracle(r, X, Q, idx, metric, K=10, P=50, S=128, init=9, init_ids=ids)

@@ -346,7 +346,8 @@ int kg_autotune_search(kg_handle *h, const float *dQ, unsigned nq, size_t qstrid
  * the fastest for later searches on this handle: the visited set (bitset
  * only, or an LDS set of 1024, 2048, 4096 or 8192 entries in front of it),
  * the adjacency rows (padded or CSR), with padded rows the walk layout
- * (kg_set_search_order; kg_search_order reports the choice), the BF16 screen
+ * (kg_set_search_order; kg_search_order reports the choice), the lean walk
+ * (kg_set_search_lean, where it applies), the BF16 screen
  * (where walks gather the FP32 rows), queries per wave (1, or 2 when the
  * screen is off) and the launch shape (as kg_autotune_search).  One setting at
  * a time, in that order, each kept when faster.  chosen (may be NULL) receives 6 values: {waves per block,

@@ -1553,7 +1553,7 @@ int kg_autotune_walk(kg_handle *k, float const *dQ, unsigned nq, size_t qstride,
             return t;
         };
         int const old_visited = h.search_visited, old_adj = h.search_adjacency, old_screen = h.search_screen;
-        int const old_order = h.search_order;
+        int const old_order = h.search_order, old_lean = h.search_lean;
         unsigned const old_vcap = h.search_vcap, old_wpb = h.search_wpb, old_bpc = h.search_bpc;
         unsigned const old_qpw = h.search_qpw;
         try {
@@ -1603,6 +1603,16 @@ int kg_autotune_walk(kg_handle *k, float const *dQ, unsigned nq, size_t qstride,
                     h.search_order = bord;
                     h.gnode_valid = false;
                 }
+            }
+            // 2c. the lean walk, where it applies (the launch pair decides per batch)
+            {
+                int const blean = h.search_lean;
+                h.search_lean = 1 - blean;
+                float const t = run();
+                if (t < best)
+                    best = t;
+                else
+                    h.search_lean = blean;
             }
             // 3. the BF16 screen, where walks gather the FP32 rows
             int bscr = h.search_screen;
@@ -1664,6 +1674,7 @@ int kg_autotune_walk(kg_handle *k, float const *dQ, unsigned nq, size_t qstride,
                 h.search_order = old_order;
                 h.gnode_valid = false;
             }
+            h.search_lean = old_lean;
             throw;
         }
     });

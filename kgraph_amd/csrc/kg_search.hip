@@ -111,7 +111,10 @@ struct LdsLayout {
         pdist = o; o = align16(o + 4ull * cap);
         pmeta = o; o = align16(o + 4ull * cap);
         pbase = o; o = align16(o + (base ? 4ull * cap : 0ull));
-        tkey = o; o = align16(o + 8ull * tcap);
+        // the lean walk sorts its starting points in the batch buffers (fid,
+        // fdist), which no expansion uses yet, when they hold them
+        bool const tk_alias = lean && 8ull * tcap <= 8ull * KG_WAVE;
+        tkey = o; o = align16(o + (tk_alias ? 0ull : 8ull * tcap));
         rid = o; o = align16(o + 4ull * rcap);
         rdist = o; o = align16(o + 4ull * rcap);
         q = o; o = align16(o + (lean ? 0ull : 4ull * Dp));
@@ -120,7 +123,8 @@ struct LdsLayout {
         ql = o; o = align16(o + 2 * d32);
         sidx = o; o = align16(o + (screen ? KG_WAVE : 0));
         fid = o; o = align16(o + 4ull * KG_WAVE);
-        fdist = o; o = align16(o + 4ull * KG_WAVE);
+        fdist = o; o = align16(o + 4ull * KG_WAVE);  // (right after fid: 8 KG_WAVE bytes together)
+        if (tk_alias) tkey = fid;
         vt = o; o = align16(o + size_t(vbytes) * vcap);
         total = o;
     }

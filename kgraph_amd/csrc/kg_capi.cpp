@@ -667,20 +667,15 @@ static void set_entry_sample(Handle &h, unsigned E, unsigned seed) {
     std::sort(ids.begin(), ids.end());
     h.eids.reserve(E);
     h.erows.reserve(size_t(E) * h.Dp);
-    h.enorms.reserve(E);
+    h.enorms.reserve((E + 15) / 16 * 16);  // whole 16-row tiles for the MFMA selection
     KG_HIP(hipMemcpyAsync(h.eids.p, ids.data(), 4ull * E, hipMemcpyHostToDevice, h.stream));
     KG_HIP(kg_launch_gather_rows(h.X.p, h.Dp, h.eids.p, E, h.erows.p, h.stream));
     KG_HIP(kg_launch_norms(h.metric, h.erows.p, E, h.D, h.Dp, h.enorms.p, h.stream));
-    {  // BF16 copy: the MFMA selection's operands
-        h.erows_b.reserve(size_t(E) * h.Dp);
-        DevBuf<uint32_t> lossy;
-        lossy.reserve(1);
-        KG_HIP(hipMemsetAsync(lossy.p, 0, 4, h.stream));
-        KG_HIP(kg_launch_bf16_rows(h.erows.p, E, h.Dp, h.erows_b.p, lossy.p, h.stream));
-        h.sync();
-    }
+    // BF16 operand tiles of the MFMA selection
+    h.etiles.reserve(kg_entry_tiles_size(E, h.D));
+    KG_HIP(kg_launch_entry_tiles(h.erows.p, E, h.D, h.Dp, h.etiles.p, h.stream));
     // ranking scale of the BF16 screen: ||e||^2/2, or ||e|| for angular
-    h.escale.reserve(E);
+    h.escale.reserve((E + 15) / 16 * 16);
     KG_HIP(kg_launch_norms(h.metric == KG_METRIC_ANGULAR ? KG_METRIC_ANGULAR : KG_METRIC_EUCLIDEAN, h.erows.p, E, h.D,
                            h.Dp, h.escale.p, h.stream));
     // integer rows (and their largest |x|) allow the exact MFMA selection
@@ -755,7 +750,7 @@ static void select_entries_device(Handle &h, float const *dQ, unsigned nq, size_
     }
     kg_entry_args a{};
     a.rows = h.erows.p;
-    a.rows_b = h.erows_b.p;
+    a.tiles = h.etiles.p;
     a.enorms = h.enorms.p;
     a.ids = h.eids.p;
     a.E = h.entryE;
@@ -778,7 +773,7 @@ static void select_entries_device(Handle &h, float const *dQ, unsigned nq, size_
         if (learned) {  // squared Euclidean in the projected space
             mlp_forward_rows(h, Qk, nq, qs, h.hq.p, h.zq.p);
             a.rows = h.eproj.p;
-            a.rows_b = nullptr;
+            a.tiles = h.eproj_tiles.p;
             a.D = a.Dp = a.qstride = h.mlpP;
             a.Q = h.zq.p;
             KG_HIP(kg_launch_norms(KG_METRIC_EUCLIDEAN, h.zq.p, nq, h.mlpP, h.mlpP, h.qsq.p, h.stream));

@@ -241,6 +241,25 @@ __global__ void __launch_bounds__(256) qnorm_check_kernel(float const *Q, unsign
     }
 }
 
+// One thread per 16-byte chunk of the tiled sample (layout in kg_internal.hpp).
+__global__ void __launch_bounds__(256) entry_tiles_kernel(float const *rows, unsigned E, unsigned D, unsigned Dp,
+                                                          unsigned KS, size_t nchunks, uint4 *tiles) {
+    size_t const i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= nchunks) return;
+    unsigned const l = unsigned(i % KG_WAVE), s = unsigned((i / KG_WAVE) % KS);
+    unsigned const t = unsigned(i / (size_t(KS) * KG_WAVE));
+    unsigned const row = 16 * t + (l & 15), k0 = 32 * s + 8 * (l >> 4);
+    uint32_t w[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        unsigned const k = k0 + 2 * j;
+        float const lo = (row < E && k < D) ? rows[size_t(row) * Dp + k] : 0.f;
+        float const hi = (row < E && k + 1 < D) ? rows[size_t(row) * Dp + k + 1] : 0.f;
+        w[j] = uint32_t(bf16_bits(lo)) | (uint32_t(bf16_bits(hi)) << 16);
+    }
+    tiles[i] = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
 // Entry selection on the matrix cores.  For rows and queries whose BF16 dot
 // products are exact (integers, |x| <= 256, D (max|q|^2 + max|e|^2) <= 2^24)
 // every product, partial sum and norm is an integer of at most 2^24 and the
@@ -250,13 +269,13 @@ __global__ void __launch_bounds__(256) qnorm_check_kernel(float const *Q, unsign
 // One block per 16 queries, 4 waves splitting the sample's 16-row tiles
 // (wave w takes tiles w, w + 4, ...).  A tile's operands: the sampled rows
 // (A, two tiles in flight: a tile's MFMAs read its buffer, then the loads of
-// the tile two ahead refill it, branch-free so the wait for a tile leaves the
-// next one's loads outstanding) and the
-// block's 16 queries (B, kept in registers), so lane l holds the distances of
-// query l % 16 to rows 4 (l / 16) + j.  Each lane keeps the MF >= m best of
-// the rows it sees as 32-bit ordered distances with their ids: it sees rows
-// in ascending id order, so inserting after equal distances keeps (dist, id)
-// order.  The 16 lanes of a query (4 per wave, 4 waves) share a bound, the
+// the tile two ahead refill it; no load is conditional and no loaded value is
+// used before its tile is scored, so the wait for a tile leaves the next
+// one's loads outstanding) and the block's 16 queries (B, kept in registers),
+// so lane l holds the distances of query l % 16 to rows 4 (l / 16) + j.  Each
+// lane keeps the MF >= m best of the rows it sees as 32-bit ordered distances
+// with their sample positions: it sees rows in ascending position (and id)
+// order, so inserting after equal distances keeps (dist, id) order.  The 16 lanes of a query (4 per wave, 4 waves) share a bound, the
 // smallest of their m-th best distances (any lane's m best are m keys no
 // worse than it, so the query's m-th result is no worse either): a distance
 // above it cannot enter the result and skips the insert.  The four lanes of
@@ -297,66 +316,35 @@ __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float
         li[i] = 0xffffffffu;
     }
     unsigned const ntiles = (a.E + 15) / 16;
-    // operands of sample tile `et`: from the BF16 copy when there is one, else
-    // rounded from the FP32 rows (the same values)
+    // operands of sample tile `et`: KS contiguous 1 KiB loads of the tiled
+    // BF16 copy, nothing computed from them here, so they stay in flight
+    // until the tile is scored
     auto load_tile = [&](unsigned et, bf16x8_t (&op)[KS]) {
-        unsigned const e = et * 16 + r16;
-        bool const eok = e < a.E;
-        if (a.rows_b) {
-            // branch-free: every lane loads from a valid address and masks the
-            // value, so the loads stay in flight past the wait for the older
-            // tile (two 8-byte halves: rows are 8-byte aligned, Dp a multiple of 4)
-            uint16_t const *er = a.rows_b + size_t(eok ? e : 0u) * a.Dp;
+        uint4 const *tp = a.tiles + size_t(et) * (KS * KG_WAVE) + lane;
 #pragma unroll
-            for (int s = 0; s < KS; ++s) {
-                unsigned const k0 = 32 * s + k8, k1 = k0 + 4;
-                bool const ok0 = eok && k0 < a.Dp, ok1 = eok && k1 < a.Dp;
-                uint2 v0 = *reinterpret_cast<uint2 const *>(er + (k0 < a.Dp ? k0 : 0u));
-                uint2 v1 = *reinterpret_cast<uint2 const *>(er + (k1 < a.Dp ? k1 : 0u));
-                op[s] = __builtin_bit_cast(bf16x8_t, make_uint4(ok0 ? v0.x : 0u, ok0 ? v0.y : 0u, ok1 ? v1.x : 0u,
-                                                                ok1 ? v1.y : 0u));
-            }
-        } else {
-            float const *er = a.rows + size_t(eok ? e : 0u) * a.Dp;
-#pragma unroll
-            for (int s = 0; s < KS; ++s) {
-                unsigned const k0 = 32 * s + k8, k1 = k0 + 4;
-                float4 const v0 =
-                    (eok && k0 < a.Dp) ? *reinterpret_cast<float4 const *>(er + k0) : make_float4(0, 0, 0, 0);
-                float4 const v1 =
-                    (eok && k1 < a.Dp) ? *reinterpret_cast<float4 const *>(er + k1) : make_float4(0, 0, 0, 0);
-                auto pk = [](float x, float y) { return uint32_t(uint16_t(bf16_rne(x))) | (uint32_t(uint16_t(bf16_rne(y))) << 16); };
-                op[s] = __builtin_bit_cast(bf16x8_t, make_uint4(pk(v0.x, v0.y), pk(v0.z, v0.w), pk(v1.x, v1.y),
-                                                                pk(v1.z, v1.w)));
-            }
-        }
+        for (int s = 0; s < KS; ++s) op[s] = __builtin_bit_cast(bf16x8_t, tp[s * KG_WAVE]);
     };
-    // the norms and ids of the 4 sampled rows whose results a lane holds
-    // (rows 16 et + kq .. + 3)
+    // the norms of the 4 sampled rows whose results a lane holds (rows
+    // 16 et + kq .. + 3): one 16-byte load, the norm arrays hold 16 ntiles
+    // entries (the ones past E are never used)
     float const *nsrc = (METRIC == METRIC_L2SQR) ? esq : a.enorms;
-    auto load_meta = [&](unsigned et, float (&nv)[4], uint32_t (&iv)[4]) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            unsigned const ee = et * 16 + kq + j;
-            unsigned const ec = ee < a.E ? ee : a.E - 1;
-            nv[j] = nsrc[ec];
-            iv[j] = a.ids[ec];
-        }
+    auto load_norms = [&](unsigned et, float4 &nv) {
+        nv = *reinterpret_cast<float4 const *>(nsrc + et * 16 + kq);
     };
-    // two tiles in flight: buffers A (tiles w, w + 8, ...) and B (w + 4, w + 12, ...)
+    // two tiles in flight: buffers A (tiles w, w + 8, ...) and B (w + 4,
+    // w + 12, ...).  Every call loads and scores, also past the last tile
+    // (a clamped tile index; its rows are skipped as >= E), so no load is
+    // conditional and no loaded value is copied on the loop's back edge,
+    // which would make the wave wait for it there.
+    auto clamp_tile = [&](unsigned et) { return et < ntiles ? et : ntiles - 1; };
     bf16x8_t opA[KS], opB[KS];
-    float nA[4], nB[4];
-    uint32_t iA[4], iB[4];
-    if (w < ntiles) {
-        load_tile(w, opA);
-        load_meta(w, nA, iA);
-    }
-    if (w + 4 < ntiles) {
-        load_tile(w + 4, opB);
-        load_meta(w + 4, nB, iB);
-    }
+    float4 nA, nB;
+    load_tile(clamp_tile(w), opA);
+    load_norms(clamp_tile(w), nA);
+    load_tile(clamp_tile(w + 4), opB);
+    load_norms(clamp_tile(w + 4), nB);
     uint32_t tq = 0xffffffffu;  // the query's shared bound as this lane last saw it
-    auto score = [&](unsigned et, bf16x8_t (&op)[KS], float (&nv)[4], uint32_t (&iv)[4]) {
+    auto score = [&](unsigned et, bf16x8_t (&op)[KS], float4 &nv) {
         if (((et >> 2) & 3) == 3) {
             // refresh the bound: this lane's m-th best, the four lanes of the
             // query in this wave, then the block's other waves (LDS minimum)
@@ -369,25 +357,15 @@ __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float
             atomicMin(&bound[r16], mine);
             tq = min(tq, __hip_atomic_load(&bound[r16], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
         }
-        // the MFMAs read this buffer before the loads two tiles ahead refill it
+        // the MFMAs read this buffer, then the loads two tiles ahead refill it
         f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int s = 0; s < KS; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(op[s], bop[s], acc, 0, 0, 0);
-        float en[4];
-        uint32_t eid[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            en[j] = nv[j];
-            eid[j] = iv[j];
-        }
-        if (et + 8 < ntiles) {
-            load_tile(et + 8, op);
-            load_meta(et + 8, nv, iv);
-        }
+        load_tile(clamp_tile(et + 8), op);
+        float const en[4] = {nv.x, nv.y, nv.z, nv.w};
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             unsigned const ee = et * 16 + kq + j;  // row of C: sampled row 4 (l / 16) + j
-            if (!qok || ee >= a.E) continue;
             float const G = acc[j];
             float d;
             if (METRIC == METRIC_L2SQR)
@@ -397,31 +375,30 @@ __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float
             else
                 d = -G / __builtin_fmaf(en[j], qn, 1.1920928955078125e-07f);
             uint32_t const kd = ord_f32(d);
-            if (kd <= tq && kd < lk[MF - 1]) {  // sorted insert after equal distances, compile-time indices
-                bool placed = false;
+            // li holds sample positions: the sample's ids ascend with their
+            // positions, so (distance, position) orders as (distance, id)
+            bool const pass = qok && ee < a.E && kd <= tq && kd < lk[MF - 1];
+            if (ballot(pass)) {
+                // branch-free sorted insert after equal distances (c_i = lk[i] > k
+                // is monotone in i); a lane that does not pass inserts nothing
+                uint32_t const k = pass ? kd : 0xffffffffu;
+                bool c[MF];
+#pragma unroll
+                for (int i = 0; i < MF; ++i) c[i] = lk[i] > k;
 #pragma unroll
                 for (int i = MF - 1; i > 0; --i) {
-                    if (!placed) {
-                        if (lk[i - 1] > kd) {
-                            lk[i] = lk[i - 1];
-                            li[i] = li[i - 1];
-                        } else {
-                            lk[i] = kd;
-                            li[i] = eid[j];
-                            placed = true;
-                        }
-                    }
+                    lk[i] = c[i - 1] ? lk[i - 1] : (c[i] ? k : lk[i]);
+                    li[i] = c[i - 1] ? li[i - 1] : (c[i] ? ee : li[i]);
                 }
-                if (!placed) {
-                    lk[0] = kd;
-                    li[0] = eid[j];
-                }
+                lk[0] = c[0] ? k : lk[0];
+                li[0] = c[0] ? ee : li[0];
             }
         }
+        load_norms(clamp_tile(et + 8), nv);
     };
     for (unsigned et = w; et < ntiles; et += 8) {
-        score(et, opA, nA, iA);
-        if (et + 4 < ntiles) score(et + 4, opB, nB, iB);
+        score(et, opA, nA);
+        score(et + 4, opB, nB);
     }
     // the four lanes of a query (lane % 16 equal): m rounds of the minimum head
     for (unsigned r = 0; r < a.m; ++r) {
@@ -458,7 +435,7 @@ __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float
                 }
             }
             ++pos[src];
-            a.out_ids[size_t(q) * a.ostride + r] = uint32_t(v);
+            a.out_ids[size_t(q) * a.ostride + r] = uint32_t(v) < a.E ? a.ids[uint32_t(v)] : 0xffffffffu;
         }
     }
 }
@@ -499,24 +476,46 @@ static void launch_exact_ks(kg_entry_args const &a, float const *qsq, float cons
 
 template <int METRIC>
 static void launch_exact_m(kg_entry_args const &a, float const *qsq, float const *esq, hipStream_t s) {
-    unsigned const ks = (a.D + 31) / 32;
-    if (ks <= 2)
+    unsigned const ks = kg_entry_slices(a.D);
+    if (ks == 2)
         launch_exact_ks<METRIC, 2>(a, qsq, esq, s);
-    else if (ks <= 4)
+    else if (ks == 4)
         launch_exact_ks<METRIC, 4>(a, qsq, esq, s);
     else
         launch_exact_ks<METRIC, 8>(a, qsq, esq, s);
 }
 
+// The sample's operands are its tiles (a.tiles, kg_launch_entry_tiles).  The
+// norm array the kernel reads (esq for squared Euclidean, a.enorms otherwise)
+// must hold 16 ceil(E / 16) floats: it is read a 16-row tile at a time.  a.ids
+// must ascend (a sorted sample of distinct ids).
 hipError_t kg_launch_entries_exact_mfma(int metric, kg_entry_args const &a, float const *qsq, float const *esq,
                                         hipStream_t s) {
-    if (a.m == 0 || a.m > MAXM || a.E == 0 || a.nq == 0 || a.D > 256) return hipErrorInvalidValue;
+    if (a.m == 0 || a.m > MAXM || a.E == 0 || a.nq == 0 || a.D > 256 || !a.tiles) return hipErrorInvalidValue;
     if (metric == METRIC_L2SQR)
         launch_exact_m<METRIC_L2SQR>(a, qsq, esq, s);
     else if (metric == METRIC_EUCLIDEAN)
         launch_exact_m<METRIC_EUCLIDEAN>(a, qsq, esq, s);
     else
         launch_exact_m<METRIC_ANGULAR>(a, qsq, esq, s);
+    return hipGetLastError();
+}
+
+unsigned kg_entry_slices(unsigned D) {
+    unsigned const ks = (D + 31) / 32;
+    return ks <= 2 ? 2u : ks <= 4 ? 4u : ks <= 8 ? 8u : ks;
+}
+
+size_t kg_entry_tiles_size(unsigned E, unsigned D) {
+    return size_t((E + 15) / 16) * kg_entry_slices(D) * KG_WAVE;
+}
+
+hipError_t kg_launch_entry_tiles(float const *rows, unsigned E, unsigned D, unsigned Dp, uint4 *tiles,
+                                 hipStream_t s) {
+    size_t const n = kg_entry_tiles_size(E, D);
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(entry_tiles_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, rows, E, D, Dp,
+                       kg_entry_slices(D), n, tiles);
     return hipGetLastError();
 }
 

@@ -155,7 +155,7 @@ struct Handle {
     // entry-point sample
     unsigned entryE = 0;
     DevBuf<float> erows, enorms, escores, escale;
-    DevBuf<uint16_t> erows_b;  // BF16 copy of the entry sample's rows
+    DevBuf<uint4> etiles;  // the entry sample as MFMA operand tiles (kg_launch_entry_tiles)
     DevBuf<uint32_t> eids;
 
     // learned projector D -> mlpH -> mlpP (FP32 master weights, Adam state)
@@ -168,6 +168,7 @@ struct Handle {
     // entry selection through the projector: shortlist size (0 = exact scoring)
     unsigned entry_shortlist = 0, eproj_version = ~0u;
     DevBuf<float> eproj, zq, hq;
+    DevBuf<uint4> eproj_tiles;  // eproj as MFMA operand tiles
     DevBuf<uint32_t> eshort;
 
     // timing + build trace

@@ -112,7 +112,7 @@ struct kg_brute_args {
 
 struct kg_entry_args {
     float const *rows;    // E sampled rows, stride Dp
-    uint16_t const *rows_b;  // their BF16 copy (same stride; NULL: rounded from rows)
+    uint4 const *tiles;   // their BF16 copy in MFMA operand order (kg_launch_entry_tiles)
     float const *enorms;  // their norms
     uint32_t const *ids;  // their node ids (ascending)
     unsigned E, D, Dp;
@@ -200,6 +200,15 @@ hipError_t kg_launch_rescore(int metric, kg_rescore_args const &a, hipStream_t s
 // Fused MFMA top-m entry selection (see kg_entry.hip): exact for rows /
 // queries whose BF16 tiles are exact, a screen otherwise; qsq / esq:
 // ||q||^2/2 and ||e||^2/2 (used for L2SQR).
+// The entry sample in the MFMA selection's operand order: 16-row tiles of
+// kg_entry_slices(D) 32-value slices; 16 bytes per lane of a wave, chunk
+// (t KS + s) 64 + l holding row 16 t + l % 16, values 32 s + 8 (l / 16) .. + 8
+// as BF16 (round to nearest even; zero past D and past E), so a wave reads a
+// slice of a tile as one contiguous 1 KiB.
+unsigned kg_entry_slices(unsigned D);
+size_t kg_entry_tiles_size(unsigned E, unsigned D);  // uint4 chunks
+hipError_t kg_launch_entry_tiles(float const *rows, unsigned E, unsigned D, unsigned Dp, uint4 *tiles,
+                                 hipStream_t s);
 hipError_t kg_launch_entries_exact_mfma(int metric, kg_entry_args const &a, float const *qsq, float const *esq,
                                         hipStream_t s);
 // flags[0] |= 1 if a value of the n x D rows (row stride `stride`) is not an

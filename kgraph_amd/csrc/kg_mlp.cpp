@@ -176,8 +176,10 @@ void mlp_project_entries(Handle &h) {
     h.eproj.reserve(size_t(h.entryE) * h.mlpP);
     h.hq.reserve(size_t(h.entryE) * h.mlpH);
     mlp_forward_rows(h, h.erows.p, h.entryE, h.Dp, h.hq.p, h.eproj.p);
-    h.eprojsq.reserve(h.entryE);  // ||z||^2 / 2 of the projected sample
+    h.eprojsq.reserve((h.entryE + 15) / 16 * 16);  // ||z||^2 / 2 of the projected sample
     KG_HIP(kg_launch_norms(KG_METRIC_EUCLIDEAN, h.eproj.p, h.entryE, h.mlpP, h.mlpP, h.eprojsq.p, h.stream));
+    h.eproj_tiles.reserve(kg_entry_tiles_size(h.entryE, h.mlpP));  // BF16 operand tiles (a screen)
+    KG_HIP(kg_launch_entry_tiles(h.eproj.p, h.entryE, h.mlpP, h.mlpP, h.eproj_tiles.p, h.stream));
     h.sync();
     h.eproj_version = h.mlp_version;
 }

@@ -679,7 +679,7 @@ static void set_entry_sample(Handle &h, unsigned E, unsigned seed) {
     KG_HIP(kg_launch_norms(h.metric == KG_METRIC_ANGULAR ? KG_METRIC_ANGULAR : KG_METRIC_EUCLIDEAN, h.erows.p, E, h.D,
                            h.Dp, h.escale.p, h.stream));
     // integer rows (and their largest |x|) allow the exact MFMA selection
-    h.iflags.reserve(3);
+    h.iflags.reserve(4);
     KG_HIP(hipMemsetAsync(h.iflags.p, 0, 8, h.stream));
     KG_HIP(kg_launch_intcheck(h.erows.p, E, h.D, h.Dp, h.iflags.p, h.stream));
     uint32_t fl[2];
@@ -691,24 +691,30 @@ static void set_entry_sample(Handle &h, unsigned E, unsigned seed) {
     h.eproj_version = ~0u;
 }
 
-// BF16 tiles give the canonical FP32 distances exactly when every row and
-// query value is an integer of magnitude <= 256 and D (max|q|^2 + max|e|^2)
-// <= 2^24: every product, partial sum and norm is then an integer <= 2^24.
-// The test runs with the query norms ||q||^2 / 2 (into h.qsq), which the MFMA
-// selection needs either way; *qsq_done tells whether they were computed.
-static bool entries_exact_on_mfma(Handle &h, float const *Qk, unsigned nq, size_t qs, bool *qsq_done) {
+// Exact MFMA selection (kg_entry_exact_bf16) on integer sample rows: the
+// query test runs with the query norms ||q||^2 / 2 (into h.qsq), which the
+// MFMA selection needs either way, and sets a device flag that gates the
+// MFMA selection `a`, queued right behind it; the host waits for the test
+// alone (its event) while the selection runs, and reads the same verdict
+// from pinned memory.  Returns whether the selection applies (false: it
+// returned at once and another form must run); *qsq_done tells whether the
+// norms were computed.
+static bool entries_exact_on_mfma(Handle &h, float const *Qk, unsigned nq, size_t qs, kg_entry_args a,
+                                  bool *qsq_done) {
     *qsq_done = false;
     if (!h.entry_int || h.D > 256 || h.entry_max > 256.f) return false;
-    KG_HIP(hipMemsetAsync(h.iflags.p, 0, 12, h.stream));
+    KG_HIP(hipMemsetAsync(h.iflags.p, 0, 16, h.stream));
     uint32_t *const fl = h.host_flag + 1;  // pinned words 1..2
-    KG_HIP(kg_launch_qnorm_check(Qk, nq, h.D, unsigned(qs), h.qsq.p, h.iflags.p, fl, h.stream));
-    h.sync();
+    KG_HIP(kg_launch_qnorm_check(Qk, nq, h.D, unsigned(qs), h.qsq.p, h.iflags.p, fl, h.entry_max, h.stream));
+    KG_HIP(hipEventRecord(h.ev[19], h.stream));
+    a.gate = h.iflags.p + 3;
+    KG_HIP(kg_launch_entries_exact_mfma(h.metric, a, h.qsq.p, h.escale.p, h.stream));
+    KG_HIP(hipEventSynchronize(h.ev[19]));
     *qsq_done = true;
     float qmax;
     uint32_t const f0 = __atomic_load_n(fl, __ATOMIC_ACQUIRE), f1 = __atomic_load_n(fl + 1, __ATOMIC_ACQUIRE);
     memcpy(&qmax, &f1, 4);
-    return f0 == 0 && qmax <= 256.f &&
-           double(h.D) * (double(qmax) * qmax + double(h.entry_max) * h.entry_max) <= 16777216.0;
+    return kg_entry_exact_bf16(f0, qmax, h.D, h.entry_max);
 }
 
 // m nearest sampled nodes per query -> ids rows [0, m) (device buffers).
@@ -739,15 +745,6 @@ static void select_entries_device(Handle &h, float const *dQ, unsigned nq, size_
     bool const learned = h.entry_shortlist > 0;
     h.qsq.reserve(std::max(size_t(nq), size_t(h.entryE)));
     KG_HIP(hipEventRecord(h.ev[14], h.stream));
-    bool qsq_done = false;
-    bool const exact_mfma = !learned && entries_exact_on_mfma(h, Qk, nq, qs, &qsq_done);
-    bool const screen = !learned && !exact_mfma && h.precision == KG_PRECISION_BF16 && h.D <= 256;
-    unsigned const sl = std::max(m, std::min(learned ? h.entry_shortlist : 16u, std::min(16u, h.entryE)));
-    if (learned) {
-        mlp_project_entries(h);
-        h.zq.reserve(size_t(nq) * h.mlpP);
-        h.hq.reserve(std::max(size_t(nq), size_t(h.entryE)) * h.mlpH);
-    }
     kg_entry_args a{};
     a.rows = h.erows.p;
     a.tiles = h.etiles.p;
@@ -763,8 +760,17 @@ static void select_entries_device(Handle &h, float const *dQ, unsigned nq, size_
     a.m = m;
     a.out_ids = d_ids;
     a.ostride = unsigned(ostride);
+    bool qsq_done = false;
+    bool const exact_mfma = !learned && entries_exact_on_mfma(h, Qk, nq, qs, a, &qsq_done);
+    bool const screen = !learned && !exact_mfma && h.precision == KG_PRECISION_BF16 && h.D <= 256;
+    unsigned const sl = std::max(m, std::min(learned ? h.entry_shortlist : 16u, std::min(16u, h.entryE)));
+    if (learned) {
+        mlp_project_entries(h);
+        h.zq.reserve(size_t(nq) * h.mlpP);
+        h.hq.reserve(std::max(size_t(nq), size_t(h.entryE)) * h.mlpH);
+    }
     if (exact_mfma) {
-        KG_HIP(kg_launch_entries_exact_mfma(h.metric, a, h.qsq.p, h.escale.p, h.stream));
+        // queued by entries_exact_on_mfma
     } else if (learned || screen) {
         h.eshort.reserve(size_t(nq) * sl);
         a.m = sl;

@@ -206,10 +206,13 @@ __global__ void __launch_bounds__(256) intcheck_kernel(float const *X, unsigned 
 // intcheck_kernel's test of the same rows: flags[0] |= 1 on a non-integer,
 // flags[1] = max |q| as float bits, flags[2] counts finished blocks (all three
 // zeroed by the caller).  The last block copies flags[0..1] to host_out
-// (pinned host memory), so the caller reads them after one stream
-// synchronisation, without a copy.
+// (pinned host memory), so the caller reads them once the kernel is done,
+// without a copy, and sets flags[3] to whether the BF16 tiles give exact
+// distances for these queries against sample rows of largest |x| emax (read
+// by the MFMA selection's gate).
 __global__ void __launch_bounds__(256) qnorm_check_kernel(float const *Q, unsigned n, unsigned D, unsigned stride,
-                                                          float *qsq, uint32_t *flags, uint32_t *host_out) {
+                                                          float *qsq, uint32_t *flags, uint32_t *host_out,
+                                                          float emax) {
     unsigned const lane = threadIdx.x & 63, t = lane & 3;
     unsigned const row = (blockIdx.x * 256 + threadIdx.x) >> 2;
     bool frac = false;
@@ -235,8 +238,11 @@ __global__ void __launch_bounds__(256) qnorm_check_kernel(float const *Q, unsign
         __threadfence();
         if (atomicAdd(flags + 2, 1u) == gridDim.x - 1) {
             __threadfence();
-            host_out[0] = __hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            host_out[1] = __hip_atomic_load(flags + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t const f0 = __hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t const f1 = __hip_atomic_load(flags + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            host_out[0] = f0;
+            host_out[1] = f1;
+            flags[3] = kg_entry_exact_bf16(f0, __uint_as_float(f1), D, emax) ? 1u : 0u;
         }
     }
 }
@@ -288,6 +294,7 @@ __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float
     __shared__ uint32_t bound[16];  // per query: smallest m-th best ordered distance of its lanes
     unsigned const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     unsigned const r16 = lane & 15, kq = (lane >> 4) * 4, k8 = (lane >> 4) * 8;
+    if (a.gate && *a.gate == 0u) return;  // another form selects for these queries
     unsigned const q = blockIdx.x * 16 + r16;
     bool const qok = q < a.nq;
     if (threadIdx.x < 16) bound[threadIdx.x] = 0xffffffffu;
@@ -443,10 +450,10 @@ __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float
 }  // namespace
 
 hipError_t kg_launch_qnorm_check(float const *Q, unsigned n, unsigned D, unsigned stride, float *qsq,
-                                 uint32_t *flags, uint32_t *host_out, hipStream_t s) {
+                                 uint32_t *flags, uint32_t *host_out, float emax, hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(qnorm_check_kernel, dim3((size_t(n) * 4 + 255) / 256), dim3(256), 0, s, Q, n, D, stride, qsq,
-                       flags, host_out);
+                       flags, host_out, emax);
     return hipGetLastError();
 }
 

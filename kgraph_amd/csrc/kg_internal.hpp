@@ -113,6 +113,7 @@ struct kg_brute_args {
 struct kg_entry_args {
     float const *rows;    // E sampled rows, stride Dp
     uint4 const *tiles;   // their BF16 copy in MFMA operand order (kg_launch_entry_tiles)
+    uint32_t const *gate; // MFMA selection: non-NULL, the kernel returns at once while *gate == 0
     float const *enorms;  // their norms
     uint32_t const *ids;  // their node ids (ascending)
     unsigned E, D, Dp;
@@ -215,11 +216,20 @@ hipError_t kg_launch_entries_exact_mfma(int metric, kg_entry_args const &a, floa
 // integer; flags[1] = max |value| as float bits.  Caller zeroes flags.
 hipError_t kg_launch_intcheck(float const *X, unsigned n, unsigned D, unsigned stride, uint32_t *flags,
                               hipStream_t s);
+// BF16 tiles give the canonical FP32 distances exactly when every row and
+// query value is an integer of magnitude <= 256 and D (max|q|^2 + max|e|^2)
+// <= 2^24: every product, partial sum and norm is then an integer <= 2^24.
+// qflag0: the queries' non-integer flag; qmax, emax: largest |q|, |e|.
+__host__ __device__ inline bool kg_entry_exact_bf16(uint32_t qflag0, float qmax, unsigned D, float emax) {
+    return qflag0 == 0 && qmax <= 256.f && emax <= 256.f &&
+           double(D) * (double(qmax) * qmax + double(emax) * emax) <= 16777216.0;
+}
 // qsq[q] = ||q||^2 / 2 (canonical order) and the kg_launch_intcheck flags of
 // the same rows in one pass: flags[0..2] zeroed by the caller (flags[2]
-// counts blocks); the last block copies flags[0..1] to host_out (pinned).
+// counts blocks); the last block copies flags[0..1] to host_out (pinned) and
+// sets flags[3] = kg_entry_exact_bf16(...) against sample rows of largest |x| emax.
 hipError_t kg_launch_qnorm_check(float const *Q, unsigned n, unsigned D, unsigned stride, float *qsq,
-                                 uint32_t *flags, uint32_t *host_out, hipStream_t s);
+                                 uint32_t *flags, uint32_t *host_out, float emax, hipStream_t s);
 hipError_t kg_launch_gather_rows(float const *X, unsigned Dp, uint32_t const *ids, unsigned E, float *out,
                                  hipStream_t s);
 // BF16 copy of N x Dp rows (Dp % 4 == 0); *inexact = 1 when a value is not

@@ -267,17 +267,22 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
     // caller's starting points must be valid node ids: checked on the device
     // (qcounter word 2), before the walk, which exits at once when one is not;
     // the call throws once the stream has drained
-    h.qcounter.reserve(3);  // [0] next query, [1] queries whose visited set spilled, [2] bad starting id
-    KG_HIP(hipMemsetAsync(h.qcounter.p, 0, 12, h.stream));
+    // [0] next query, [1] queries whose visited set spilled, [2] bad starting id,
+    // [3] the queries are not all integers in [0, 255] (the lean walk's check)
+    h.qcounter.reserve(4);
+    KG_HIP(hipMemsetAsync(h.qcounter.p, 0, 16, h.stream));
     uint32_t *const bad_init = h.qcounter.p + 2;
     // the check kernel also raises a pinned host word, read once the stream drains
     auto check_init = [&]() {
         h.sync();
         if (__atomic_load_n(h.host_flag, __ATOMIC_ACQUIRE)) throw invalid_argument("initial id out of range");
     };
+    // a walk that may be the lean one has its query check done in the same launch
+    bool const q8_checked = sp.init && h.search_lean && sp.T == 1;
     if (sp.init) {
         __atomic_store_n(h.host_flag, 0u, __ATOMIC_RELEASE);
-        KG_HIP(kg_launch_init_check(d_ids, nq, ostride, sp.init, h.gN, bad_init, h.host_flag, h.stream));
+        KG_HIP(kg_launch_init_check(d_ids, nq, ostride, sp.init, h.gN, bad_init, h.host_flag,
+                                    q8_checked ? dQ : nullptr, h.D, qstride, h.qcounter.p + 3, h.stream));
     }
     if (sp.S == 0) throw invalid_argument("S must be >= 1");
     unsigned const N = h.gN;
@@ -581,10 +586,8 @@ static void search_device(Handle &h, float const *dQ, unsigned nq, size_t qstrid
         h.last_lean = lean;
         if (lean) {
             // the batch's query check, then the lean / general pair
-            h.leanflag.reserve(1);
-            KG_HIP(hipMemsetAsync(h.leanflag.p, 0, 4, h.stream));
-            KG_HIP(kg_launch_q8check(Qk, nq, h.D, qs, h.leanflag.p, h.stream));
-            a.lean_flag = h.leanflag.p;
+            if (!q8_checked) KG_HIP(kg_launch_q8check(Qk, nq, h.D, qs, h.qcounter.p + 3, h.stream));
+            a.lean_flag = h.qcounter.p + 3;
             kg_search_args al = a;
             al.lean = 1;
             al.res_cap = 0;
@@ -1116,7 +1119,7 @@ int kg_set_search_visited_size(kg_handle *k, unsigned entries) {
 static bool lean_walked(Handle &h) {
     if (!h.last_lean) return false;
     uint32_t f = 1;
-    KG_HIP(hipMemcpyAsync(&f, h.leanflag.p, 4, hipMemcpyDeviceToHost, h.stream));
+    KG_HIP(hipMemcpyAsync(&f, h.qcounter.p + 3, 4, hipMemcpyDeviceToHost, h.stream));
     h.sync();
     return f == 0;
 }

@@ -130,7 +130,6 @@ struct Handle {
     int search_lean = 1;
     bool last_lean = false;      // the last walk was a lean / general pair
     unsigned last_lean_vcap = 0, last_lean_bits = 0;
-    DevBuf<uint32_t> leanflag;   // the pair's query check (0: every query an integer vector in [0, 255])
     DevBuf<uint32_t> wperm, winv;
     DevBuf<float> wX, wnorms;
     DevBuf<uint16_t> wXb;

@@ -191,8 +191,11 @@ hipError_t kg_permute_adjacency(unsigned N, unsigned W, uint32_t const *perm, ui
 // *flag = *host_flag = 1 if one of the first `init` ids of a row (row q at
 // ids + q*ostride) is >= N (host_flag: pinned host memory); both are left
 // alone otherwise.
+// With Q non-NULL it also runs kg_launch_q8check's test of the nq x D queries
+// (row stride qstride) into qflag, in the same launch.
 hipError_t kg_launch_init_check(uint32_t const *ids, unsigned nq, size_t ostride, unsigned init, unsigned N,
-                                uint32_t *flag, uint32_t *host_flag, hipStream_t s);
+                                uint32_t *flag, uint32_t *host_flag, float const *Q, unsigned D, size_t qstride,
+                                uint32_t *qflag, hipStream_t s);
 hipError_t kg_launch_brute(int metric, kg_brute_args const &a, hipStream_t s);
 unsigned kg_brute_splits(unsigned nq, unsigned N, unsigned K);
 hipError_t kg_launch_entries(int metric, kg_entry_args const &a, hipStream_t s);

@@ -1175,9 +1175,12 @@ __global__ void __launch_bounds__(256) q8check_kernel(float const *Q, unsigned n
     if (ballot(bad) && (threadIdx.x & 63) == 0) *flag = 1u;
 }
 
+// The starting-id check; with Q, also q8check_kernel's test of the queries
+// into qflag (one launch for both).
 __global__ void __launch_bounds__(256) init_check_kernel(uint32_t const *ids, unsigned nq, size_t ostride,
                                                          unsigned init, unsigned N, uint32_t *flag,
-                                                         uint32_t *host_flag) {
+                                                         uint32_t *host_flag, float const *Q, unsigned D,
+                                                         size_t qstride, uint32_t *qflag) {
     size_t const total = size_t(nq) * init;
     bool bad = false;
     for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < total; i += size_t(gridDim.x) * 256) {
@@ -1187,6 +1190,16 @@ __global__ void __launch_bounds__(256) init_check_kernel(uint32_t const *ids, un
     if (bad) {
         *flag = 1u;
         *host_flag = 1u;
+    }
+    if (Q) {
+        size_t const qtotal = size_t(nq) * D;
+        bool q8bad = false;
+        for (size_t i = size_t(blockIdx.x) * 256 + threadIdx.x; i < qtotal; i += size_t(gridDim.x) * 256) {
+            size_t const q = i / D, j = i - q * D;
+            float const v = Q[q * qstride + j];
+            q8bad |= !(v >= 0.f && v <= 255.f && v == rintf(v));
+        }
+        if (ballot(q8bad) && (threadIdx.x & 63) == 0) *qflag = 1u;
     }
 }
 
@@ -1249,12 +1262,13 @@ hipError_t kg_launch_q8check(float const *Q, unsigned nq, unsigned D, size_t str
 }
 
 hipError_t kg_launch_init_check(uint32_t const *ids, unsigned nq, size_t ostride, unsigned init, unsigned N,
-                                uint32_t *flag, uint32_t *host_flag, hipStream_t s) {
-    size_t const total = size_t(nq) * init;
+                                uint32_t *flag, uint32_t *host_flag, float const *Q, unsigned D, size_t qstride,
+                                uint32_t *qflag, hipStream_t s) {
+    size_t const total = std::max(size_t(nq) * init, Q ? size_t(nq) * D : size_t(0));
     if (total == 0) return hipSuccess;
     unsigned const blocks = unsigned(std::min<size_t>((total + 255) / 256, 4096));
     hipLaunchKernelGGL(init_check_kernel, dim3(blocks), dim3(256), 0, s, ids, nq, ostride, init, N, flag,
-                       host_flag);
+                       host_flag, Q, D, qstride, qflag);
     return hipGetLastError();
 }
 

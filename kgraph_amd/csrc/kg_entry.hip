@@ -221,11 +221,21 @@ __global__ void __launch_bounds__(256) qnorm_check_kernel(float const *Q, unsign
         float const *r = Q + size_t(row) * stride;
         float const n2 = quad_metric<METRIC_EUCLIDEAN, float, true>(r, r, D, t);
         if (t == 0) qsq[row] = finish_norm<METRIC_EUCLIDEAN>(n2);
-        for (unsigned c = t; c < D; c += 4) {
-            float const v = r[c];
+        // the same row again (cache hits), 16 bytes a load (rows are 16-byte
+        // aligned, stride a multiple of 4), the last D % 4 values one by one
+        auto chk = [&](float v) {
             frac |= !(v == rintf(v));
             mx = fmaxf(mx, fabsf(v));
+        };
+        unsigned const d4 = D >> 2;
+        for (unsigned c = t; c < d4; c += 4) {
+            float4 const v = reinterpret_cast<float4 const *>(r)[c];
+            chk(v.x);
+            chk(v.y);
+            chk(v.z);
+            chk(v.w);
         }
+        for (unsigned c = 4 * d4 + t; c < D; c += 4) chk(r[c]);
     }
     uint64_t const fr = ballot(frac);
     for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));

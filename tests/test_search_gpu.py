@@ -396,7 +396,8 @@ def _entry_sample_ids(seed, E, N):
                                                ("intmix", ANG, 32, 16, 512), ("intmix", L2, 24, 3, 512),
                                                ("intmix", L2, 128, 16, 64), ("intmix", EUC, 64, 12, 128),
                                                ("intmix", L2, 128, 9, 100), ("intmix", L2, 128, 9, 20),
-                                               ("uniform", L2, 16, 9, 100)])
+                                               ("uniform", L2, 16, 9, 100), ("intmix", L2, 200, 9, 100),
+                                               ("intmix", EUC, 20, 5, 40), ("intmix", ANG, 72, 9, 4100)])
 def test_entry_selection_parity(kind, metric, d, m, E):
     X, Q, idx = setup(kind, metric, d=d)
     g = gpu_index(X, idx, metric)

@@ -27,6 +27,7 @@ using namespace kgd;
 constexpr unsigned QB = 16;    // queries per block
 constexpr unsigned EB = 64;    // sampled rows per block (one per quad)
 constexpr unsigned MAXM = 16;  // largest m
+constexpr unsigned ECB = 8;    // MFMA selection: buffered candidates per lane
 
 template <int METRIC>
 __global__ void __launch_bounds__(256) entry_dist_kernel(kg_entry_args a) {
@@ -294,7 +295,10 @@ __global__ void __launch_bounds__(256) entry_tiles_kernel(float const *rows, uns
 // order, so inserting after equal distances keeps (dist, id) order.  The 16 lanes of a query (4 per wave, 4 waves) share a bound, the
 // smallest of their m-th best distances (any lane's m best are m keys no
 // worse than it, so the query's m-th result is no worse either): a distance
-// above it cannot enter the result and skips the insert.  The four lanes of
+// above it cannot enter the result and skips the insert.  Passing
+// candidates wait in a per-lane LDS buffer (8 entries) and are inserted
+// together, so a wave takes an insert step per buffered candidate of its
+// fullest lane rather than one per row that any of its 64 lanes passes.  The four lanes of
 // a query and then the four waves are merged into the m best.
 // KS: 32-value slices per row, one v_mfma_f32_16x16x32_bf16 each (lane l
 // holds values 32 s + 8 (l / 16) .. + 8 of its row or query).
@@ -302,6 +306,7 @@ template <int METRIC, int KS, int MF>
 __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float const *qsq, float const *esq) {
     __shared__ uint64_t best[4][16][MF];
     __shared__ uint32_t bound[16];  // per query: smallest m-th best ordered distance of its lanes
+    __shared__ uint64_t cbuf[ECB * 256];  // per lane: candidates that passed, (distance, position)
     unsigned const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     unsigned const r16 = lane & 15, kq = (lane >> 4) * 4, k8 = (lane >> 4) * 8;
     if (a.gate && *a.gate == 0u) return;  // another form selects for these queries
@@ -360,9 +365,39 @@ __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float
     load_norms(clamp_tile(w), nA);
     load_tile(clamp_tile(w + 4), opB);
     load_norms(clamp_tile(w + 4), nB);
+    // candidates that pass the bound wait in the lane's LDS buffer (in the
+    // order the lane sees them: ascending position) and are inserted
+    // together, so a wave runs an insert step per buffered candidate of its
+    // fullest lane instead of one per row that any of its 64 lanes passes
+    unsigned cnt = 0;
+    auto insert = [&](uint32_t k, uint32_t e) __attribute__((always_inline)) {
+        // branch-free sorted insert after equal distances (c_i = lk[i] > k is
+        // monotone in i); k = 0xffffffff inserts nothing
+        bool c[MF];
+#pragma unroll
+        for (int i = 0; i < MF; ++i) c[i] = lk[i] > k;
+#pragma unroll
+        for (int i = MF - 1; i > 0; --i) {
+            lk[i] = c[i - 1] ? lk[i - 1] : (c[i] ? k : lk[i]);
+            li[i] = c[i - 1] ? li[i - 1] : (c[i] ? e : li[i]);
+        }
+        lk[0] = c[0] ? k : lk[0];
+        li[0] = c[0] ? e : li[0];
+    };
+    auto flush = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (unsigned i = 0; i < ECB; ++i) {
+            if (ballot(i < cnt)) {
+                uint64_t const key = i < cnt ? cbuf[i * 256 + threadIdx.x] : ~0ull;
+                insert(uint32_t(key >> 32), uint32_t(key));
+            }
+        }
+        cnt = 0;
+    };
     uint32_t tq = 0xffffffffu;  // the query's shared bound as this lane last saw it
     auto score = [&](unsigned et, bf16x8_t (&op)[KS], float4 &nv) {
         if (((et >> 2) & 3) == 3) {
+            flush();
             // refresh the bound: this lane's m-th best, the four lanes of the
             // query in this wave, then the block's other waves (LDS minimum)
             uint32_t mine = 0xffffffffu;
@@ -395,28 +430,16 @@ __global__ void __launch_bounds__(256) entry_exact_kernel(kg_entry_args a, float
             // li holds sample positions: the sample's ids ascend with their
             // positions, so (distance, position) orders as (distance, id)
             bool const pass = qok && ee < a.E && kd <= tq && kd < lk[MF - 1];
-            if (ballot(pass)) {
-                // branch-free sorted insert after equal distances (c_i = lk[i] > k
-                // is monotone in i); a lane that does not pass inserts nothing
-                uint32_t const k = pass ? kd : 0xffffffffu;
-                bool c[MF];
-#pragma unroll
-                for (int i = 0; i < MF; ++i) c[i] = lk[i] > k;
-#pragma unroll
-                for (int i = MF - 1; i > 0; --i) {
-                    lk[i] = c[i - 1] ? lk[i - 1] : (c[i] ? k : lk[i]);
-                    li[i] = c[i - 1] ? li[i - 1] : (c[i] ? ee : li[i]);
-                }
-                lk[0] = c[0] ? k : lk[0];
-                li[0] = c[0] ? ee : li[0];
-            }
+            if (pass) cbuf[(cnt++) * 256 + threadIdx.x] = (uint64_t(kd) << 32) | ee;
         }
+        if (ballot(cnt + 4 > ECB)) flush();  // room for the next tile's 4 rows
         load_norms(clamp_tile(et + 8), nv);
     };
     for (unsigned et = w; et < ntiles; et += 8) {
         score(et, opA, nA);
         score(et + 4, opB, nB);
     }
+    flush();
     // the four lanes of a query (lane % 16 equal): m rounds of the minimum head
     for (unsigned r = 0; r < a.m; ++r) {
         uint64_t const head = (uint64_t(lk[0]) << 32) | li[0];

@@ -290,9 +290,10 @@ int kg_build_trace(kg_handle *h, float *out, unsigned cap, unsigned *n);
  * the m nearest sampled nodes (exact FP32 distance, ties by id, m <= 16) to
  * ids[q*ostride + 0..m); a following kg_search with params->init = m (m < K)
  * walks from them.  kg_select_entries_device returns once the selection is
- * queued on the handle's stream: a following call on the same handle (such as
- * kg_search_device) sees its ids, any other reader synchronises the device
- * first (hipDeviceSynchronize). */
+ * queued on the handle's stream (it waits only for its short query test, which
+ * picks the form, while the selection itself runs): a following call on the
+ * same handle (such as kg_search_device) sees its ids, any other reader
+ * synchronises the device first (hipDeviceSynchronize). */
 int kg_set_entry_sample(kg_handle *h, unsigned E, unsigned seed);
 int kg_select_entries(kg_handle *h, const float *Q, unsigned nq, size_t qstride, unsigned m, unsigned *ids,
                       size_t ostride);
